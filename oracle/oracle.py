@@ -1,0 +1,75 @@
+"""ctypes wrapper for the C restatement ``oracle/liboracle.so``.
+
+TEST INFRASTRUCTURE ONLY: the checker that the HIP library is compared with.
+The functions take the same arrays as the reference routines they restate
+(see spr_oracle.h for the file:line of each).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+I8 = np.int64
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} missing: run `make -C oracle`")
+        _lib = C.CDLL(LIB_PATH)
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _i(v):
+    return C.byref(C.c_int64(int(v)))
+
+
+def sprsol(iop: int, mspar, mtrees, msifa, sm, b=None, tol=None):
+    """Restated SPRSOL; sm and b are modified in place. Returns (ierr, tol)."""
+    neq = int(mspar[7])
+    tolv = np.zeros(3) if tol is None else np.asarray(tol, dtype=np.float64).copy()
+    if b is None:
+        b = np.zeros((neq, 1), order="F")
+    assert b.flags.f_contiguous and b.dtype == np.float64
+    nrhs = b.shape[1] if b.ndim == 2 else 1
+    ierr = C.c_int64(0)
+    lib().oracle_sprsol(_i(iop), _p(mspar), _p(mtrees), _p(msifa), _p(sm), _p(b),
+                        _i(neq), _i(nrhs), _p(tolv), None, None, _i(-1), C.byref(ierr))
+    return int(ierr.value), tolv
+
+
+def sprfc3(mspar, mtrees, msifa, sm, tol):
+    """Restated SPRFC3 (singular-tolerant factor). Returns (ierr, tol, dcor(2,neq))."""
+    neq = int(mspar[7])
+    tolv = np.asarray(tol, dtype=np.float64).copy()
+    assert tolv.size >= 4
+    dcor = np.zeros(2 * neq)
+    ierr = C.c_int64(0)
+    lib().oracle_sprfc3(_p(mspar), _p(mtrees), _p(msifa), _p(sm), _p(tolv), None, None,
+                        _p(dcor), _i(-1), C.byref(ierr))
+    return int(ierr.value), tolv, dcor.reshape(neq, 2).T
+
+
+def sprprm(a, b, mspar, mtrees, msifa, iflag=1, c=None):
+    """Restated SPRPRM: C = +-A*B (|iflag|=1) or C +-= A*B (|iflag|=2)."""
+    b = np.asfortranarray(b, dtype=np.float64)
+    if b.ndim == 1:
+        b = b[:, None]
+    m, n = b.shape
+    c = np.zeros((m, n), order="F") if c is None else c
+    wa = np.zeros(m)
+    ierr = C.c_int64(0)
+    lib().oracle_sprprm(_p(a), _p(b), _p(c), _p(wa), _p(mspar), _p(mtrees), _p(msifa),
+                        _i(m), _i(n), _i(1), _i(iflag), _i(-1), C.byref(ierr))
+    return int(ierr.value), c
