@@ -1,0 +1,163 @@
+/*
+ * sprgpu -- MI355X (gfx950) numeric factorization and solves for the SPR
+ * sparse LDL' solver of sam-lib.
+ *
+ * Two layers of entry points, all plain C, all integers INTEGER*8:
+ *
+ * 1. Drop-in replacements for the reference's Fortran-77 numeric routines.
+ *    Same argument lists, every argument by reference, same array layouts
+ *    (MSPAR/MTREES/MSIFA from the reference symbolic phase, SM from SPRADM),
+ *    same IERR codes and MSPAR/TOL outputs.  The Fortran module in
+ *    sam-lib_amd/fortran/sprgpu_dropin.f90 re-exports them under the
+ *    reference names SPRFC1, SPRFC3, SPRFS1, SPRBS1, SPRSOL.  One extra
+ *    IERR value is added: -5 = a HIP runtime error.
+ *
+ * 2. A device-resident handle API: the factor lives in HBM between calls,
+ *    inputs and outputs are device pointers, and the FP32-factor + FP64
+ *    iterative-refinement mode, multiple right-hand sides, and the adjoint
+ *    are available.  The drop-in layer is built on it.
+ */
+#ifndef SPRGPU_H
+#define SPRGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int64_t sprgpu_int;
+
+/* ------------------------------------------------------------------------ */
+/* 1. Drop-in entry points                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces SPRFC1 (src/SPR/sprfc1.f:5-7): SM(1:NEQ+NZEROL) in, L and D out in
+ * the same packed supernodal layout; TOL(1) in, TOL(2:3) out; MSPAR(1),
+ * MSPAR(26) (NUMNEG), MSPAR(27) (SUPEQT) out.  The factor is also kept on the
+ * GPU, keyed by the SM address, for the SPRFS1/SPRBS1 replacements.  MVMULT
+ * and MMMULT are accepted for signature compatibility and ignored: the dense
+ * front kernels replace LMVY18/LMMY18 (lmvy18.f:5, lmmy18.f:5). */
+void sprgpu_sprfc1(sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa, double *sm,
+                   double *tol, sprgpu_int *iwork, double *rwork, const sprgpu_int *lpu,
+                   sprgpu_int *ierr, void (*mvmult)(void), void (*mmmult)(void));
+
+/* Replaces SPRFC3 (src/SPR/sprfc3.f:5-7): as SPRFC1, but a zero diagonal is
+ * replaced by one and a pivot with |d| <= TOL(1)*|a_jj| by
+ * sign(d)*TOL(1)*max(1,|TOL(4)*a_jj|), recorded in DCOR(2,NEQ). */
+void sprgpu_sprfc3(sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa, double *sm,
+                   double *tol, sprgpu_int *iwork, double *rwork, double *dcor,
+                   const sprgpu_int *lpu, sprgpu_int *ierr, void (*mvmult)(void),
+                   void (*mmmult)(void));
+
+/* Replaces SPRFS1 (src/SPR/sprfs1.f:5-7): B(LDB,NRHS) <- D^-1 L^-1 B. */
+void sprgpu_sprfs1(sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa, double *sm,
+                   double *b, const sprgpu_int *ldb, const sprgpu_int *nrhs, sprgpu_int *iwork,
+                   double *rwork, const sprgpu_int *lpu, sprgpu_int *ierr);
+
+/* Replaces SPRBS1 (src/SPR/sprbs1.f:5-6): B(LDB,NRHS) <- L^-T B. */
+void sprgpu_sprbs1(sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa, double *sm,
+                   double *b, const sprgpu_int *ldb, const sprgpu_int *nrhs, double *rwork,
+                   const sprgpu_int *lpu, sprgpu_int *ierr);
+
+/* Replaces SPRSOL (src/SPR/sprsol.f:5-6): IOP<=3 factor, 2<=IOP<=5 forward,
+ * IOP>=3 and IOP!=5 backward. */
+void sprgpu_sprsol(const sprgpu_int *iop, sprgpu_int *mspar, sprgpu_int *mtrees,
+                   sprgpu_int *msifa, double *sm, double *b, const sprgpu_int *ldb,
+                   const sprgpu_int *nrhs, double *tol, sprgpu_int *iwork, double *rwork,
+                   const sprgpu_int *lpu, sprgpu_int *ierr);
+
+/* Releases the device factor that the drop-in layer keeps for SM. */
+void sprgpu_release(const double *sm);
+
+/* ------------------------------------------------------------------------ */
+/* 2. Device-resident handle API                                            */
+/* ------------------------------------------------------------------------ */
+
+typedef struct sprgpu_handle sprgpu_handle;
+
+enum sprgpu_precision { SPRGPU_FP64 = 0, SPRGPU_FP32 = 1 };
+
+/* Factor statistics, the reference's outputs (sprfc1.f:60-80). */
+typedef struct {
+    sprgpu_int ierr;     /* 0, 3 (NUMNEG>0), -3 singular, -5 HIP error */
+    sprgpu_int numneg;   /* MSPAR(26) */
+    sprgpu_int supeqt;   /* MSPAR(27): failing equation, or -#replacements (SPRFC3) */
+    double tol2;         /* TOL(2) */
+    double tol3;         /* TOL(3) */
+    sprgpu_int nzero_diag; /* zero diagonals found before factorization */
+} sprgpu_factor_info;
+
+typedef struct {
+    sprgpu_int iterations; /* refinement steps taken */
+    double relres;         /* final max over columns of ||b - A x||_2 / ||b||_2 */
+    sprgpu_int converged;  /* 1 if relres <= requested tolerance */
+} sprgpu_ir_info;
+
+/* Builds the device plan from the symbolic arrays.  precision selects the
+ * factor storage/arithmetic.  Returns 0 or a negative HIP/argument error. */
+int sprgpu_create(const sprgpu_int *mspar, const sprgpu_int *mtrees, const sprgpu_int *msifa,
+                  int precision, sprgpu_handle **handle);
+int sprgpu_destroy(sprgpu_handle *handle);
+
+/* hipStream_t to run on (NULL = default stream). */
+int sprgpu_set_stream(sprgpu_handle *handle, void *stream);
+
+/* Device-memory pattern of A used by the FP64 residual.  Must be called once
+ * with a host copy of SM (assembled, before factorization) before
+ * sprgpu_solve_ir or sprgpu_spmv; later factorizations check that no entry
+ * outside this pattern became non-zero and rebuild it if one did. */
+int sprgpu_set_pattern(sprgpu_handle *handle, const double *sm_host);
+
+/* Numeric factorization of SM given in device memory (NEQ+NZEROL doubles, the
+ * SPRADM layout; not modified).  tol points to host TOL(1..4) (TOL(4) is read
+ * only when mode==3).  mode 1 = SPRFC1 semantics, 3 = SPRFC3 semantics.
+ * info may be NULL (then the call returns without synchronizing). */
+int sprgpu_factor(sprgpu_handle *handle, const double *sm_dev, const double *tol, int mode,
+                  sprgpu_factor_info *info);
+
+/* Reads the factor statistics of the last factorization (synchronizes). */
+int sprgpu_factor_info_get(sprgpu_handle *handle, sprgpu_factor_info *info);
+
+/* Forward (D^-1 L^-1) and/or backward (L^-T) substitution in place on device
+ * B(LDB,NRHS) in FP64 (the factor may be FP32).  which: 1 forward, 2
+ * backward, 3 both. */
+int sprgpu_solve(sprgpu_handle *handle, double *b_dev, sprgpu_int ldb, sprgpu_int nrhs,
+                 int which);
+
+/* Y = A X with the assembled matrix of the last factorization, FP64. */
+int sprgpu_spmv(sprgpu_handle *handle, const double *x_dev, double *y_dev, sprgpu_int ld,
+                sprgpu_int nrhs);
+
+/* Solves A X = B with the current factor plus FP64 iterative refinement
+ * until max_col ||B - A X|| / ||B|| <= rtol or maxit steps. */
+int sprgpu_solve_ir(sprgpu_handle *handle, const double *b_dev, double *x_dev, sprgpu_int ld,
+                    sprgpu_int nrhs, double rtol, int maxit, sprgpu_ir_info *info);
+
+/* Copies the factor into SM layout (NEQ+NZEROL doubles): d in SM(1:NEQ) and
+ * the packed trapezoids with d on the diagonal and l_ij below.  sm_out may be
+ * host or device memory. */
+int sprgpu_get_factor(sprgpu_handle *handle, double *sm_out);
+
+/* Size queries. */
+sprgpu_int sprgpu_neq(const sprgpu_handle *handle);
+sprgpu_int sprgpu_nzerol(const sprgpu_handle *handle);
+/* Bytes of device memory held by the handle. */
+sprgpu_int sprgpu_device_bytes(const sprgpu_handle *handle);
+
+/* Host-only: builds the device schedule for a symbolic structure without
+ * touching a GPU and reports it, for checking and sizing.  out[0..9] =
+ * NEQ, NSUPER, tree heights, kernel launches per factorization, front-pool
+ * elements, solve-workspace elements per RHS, fronts factored in LDS,
+ * panel steps, MFMA update tiles, factor flops (RINFO(3) formula,
+ * sprtrs.f:1961-1969).  Returns 0 or -2 (structure not supported). */
+int sprgpu_plan_summary(const sprgpu_int *mspar, const sprgpu_int *mtrees,
+                        const sprgpu_int *msifa, double *out);
+
+/* Last error message (thread-local, empty if none). */
+const char *sprgpu_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

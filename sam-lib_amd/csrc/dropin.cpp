@@ -1,0 +1,346 @@
+// Drop-in replacements of the reference numeric entry points.
+//
+// Each follows the reference driver's control flow: the MSPAR(1) stage
+// protocol, the MSPAR-derived array-length checks, and the IERR / MSPAR(26)
+// / MSPAR(27) / TOL(2:3) / DCOR outputs:
+//   sprgpu_sprfc1  <- SPRFC1  sprfc1.f:5-283
+//   sprgpu_sprfc3  <- SPRFC3  sprfc3.f:5-280
+//   sprgpu_sprfs1  <- SPRFS1  sprfs1.f:5-263 (+ SPRFS4 permutation, :396-528)
+//   sprgpu_sprbs1  <- SPRBS1  sprbs1.f:5-248 (+ SPRBS5 permutation, :371-495)
+//   sprgpu_sprsol  <- SPRSOL  sprsol.f:5-52
+// The factor stays on the GPU between calls, keyed by the SM address; SM on
+// the host always receives the factor too, so reference routines that read
+// it afterwards (SPREXD, SPRDAD, ...) still work.
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+
+#include "sprgpu_internal.h"
+
+using namespace sprgpu;
+
+namespace {
+
+struct Entry {
+    sprgpu_handle *h = nullptr;
+    double *d_sm = nullptr;   // device copy of SM (NEQ+NZEROL)
+    double *d_b = nullptr;    // device right-hand sides
+    i64 b_cap = 0;
+    std::vector<i64> key;     // fingerprint of the symbolic structure
+};
+
+std::mutex g_mu;
+std::map<const double *, Entry> g_cache;
+
+std::vector<i64> fingerprint(const i64 *mspar, const i64 *mtrees, const i64 *msifa)
+{
+    auto M = [&](int k) { return mspar[k - 1]; };
+    std::vector<i64> key = {M(8), M(11), M(14), M(15), M(16), M(45), M(46), M(47), M(48)};
+    // hash of XSUPER, XLINDX and LINDX (FNV-1a over 64-bit words)
+    unsigned long long hsh = 1469598103934665603ull;
+    auto mix = [&](i64 v) {
+        hsh ^= (unsigned long long)v;
+        hsh *= 1099511628211ull;
+    };
+    const i64 ns = M(11);
+    for (i64 j = 0; j <= ns; ++j) mix(mtrees[M(45) - 1 + j]);
+    for (i64 j = 0; j <= ns; ++j) mix(mtrees[M(46) - 1 + j]);
+    for (i64 k = 0; k < M(15); ++k) mix(msifa[M(47) - 1 + k]);
+    key.push_back(i64(hsh));
+    return key;
+}
+
+void release_entry(Entry &e)
+{
+    if (e.h) sprgpu_destroy(e.h);
+    if (e.d_sm) (void)hipFree(e.d_sm);
+    if (e.d_b) (void)hipFree(e.d_b);
+    e = Entry{};
+}
+
+// Finds or creates the device state for SM.  Returns 0, -2 (structure not
+// supported) or -5 (HIP error).
+int get_entry(const double *sm, const i64 *mspar, const i64 *mtrees, const i64 *msifa,
+              Entry *&out, bool *fresh)
+{
+    auto key = fingerprint(mspar, mtrees, msifa);
+    auto it = g_cache.find(sm);
+    *fresh = false;
+    if (it != g_cache.end() && it->second.key == key) {
+        out = &it->second;
+        return 0;
+    }
+    if (it != g_cache.end()) release_entry(it->second);
+    Entry e;
+    e.key = key;
+    int rc = sprgpu_create(mspar, mtrees, msifa, SPRGPU_FP64, &e.h);
+    if (rc) return rc == -5 ? -5 : -2;
+    const i64 n = e.h->plan.neq + e.h->plan.nzerol;
+    if (hipMalloc(&e.d_sm, sizeof(double) * n) != hipSuccess) {
+        release_entry(e);
+        set_error("device allocation of SM failed");
+        return -5;
+    }
+    g_cache[sm] = e;
+    out = &g_cache[sm];
+    *fresh = true;
+    return 0;
+}
+
+// MTREES / MSIFA length checks as carved by the reference drivers
+// (sprfc1.f:175-236, sprfs1.f:178-230).
+bool lengths_ok_factor(const i64 *mspar)
+{
+    auto M = [&](int k) { return mspar[k - 1]; };
+    const i64 nsuper = M(11), nb = M(12), nelact = M(19), nblock = M(22);
+    const i64 xlindx = M(46);
+    const i64 supsup = xlindx + nsuper + 1, elseq = supsup + nsuper;
+    const i64 xelseq = elseq + nelact, xblock = xelseq + nb + 1, mtused = xblock + nb;
+    const i64 xlnz = M(48), xsplit = xlnz + nsuper + 1, split = xsplit + nsuper;
+    const i64 mlused = split + nblock;
+    return !(mtused > M(4) || mlused > M(3));
+}
+
+bool lengths_ok_solve(const i64 *mspar)
+{
+    auto M = [&](int k) { return mspar[k - 1]; };
+    const i64 nsuper = M(11), nb = M(12), nelact = M(19);
+    const i64 xlindx = M(46);
+    const i64 supsup = xlindx + nsuper + 1, elseq = supsup + nsuper;
+    const i64 xelseq = elseq + nelact, xblock = xelseq + nb + 1, mtused = xblock + nb;
+    const i64 mlused = M(47) + M(15) - 1;
+    return !(mtused > M(4) || mlused > M(3));
+}
+
+// Messages go to Fortran unit LPU from the Fortran layer; the C layer only
+// records the reason (sprgpu_last_error).
+void report(const sprgpu_int *, const char *, i64) {}
+
+void factor_dropin(int mode, sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa,
+                   double *sm, double *tol, double *dcor, const sprgpu_int *lpu,
+                   sprgpu_int *ierr, const char *name)
+{
+    // stage protocol (sprfc1.f:140-148)
+    if (mspar[0] < 0) {
+        mspar[0] = -5;
+        *ierr = -1;
+        set_error("error from a previous SPR stage is not cleared");
+        report(lpu, name, *ierr);
+        return;
+    }
+    mspar[0] = 5;
+    *ierr = 0;
+    if (!lengths_ok_factor(mspar)) {
+        mspar[0] = -2;
+        *ierr = -2;
+        set_error("MTREES and/or MSIFA are not great enough");
+        report(lpu, name, *ierr);
+        return;
+    }
+    std::lock_guard<std::mutex> lock(g_mu);
+    Entry *e = nullptr;
+    bool fresh = false;
+    int rc = get_entry(sm, mspar, mtrees, msifa, e, &fresh);
+    if (rc) {
+        *ierr = rc == -5 ? -5 : -2;
+        if (*ierr == -2) mspar[0] = -2;
+        report(lpu, name, *ierr);
+        return;
+    }
+    sprgpu_handle *h = e->h;
+    const i64 neq = h->plan.neq, n = neq + h->plan.nzerol;
+    sprgpu_factor_info info{};
+    double t4[4] = {tol[0], 0.0, 0.0, mode == 3 ? tol[3] : 0.0};
+    hipError_t he = hipMemcpy(e->d_sm, sm, sizeof(double) * n, hipMemcpyHostToDevice);
+    if (he != hipSuccess) rc = hip_fail(he, "upload SM");
+    if (!rc) rc = sprgpu_factor(h, e->d_sm, t4, mode, &info);
+    if (!rc) rc = copy_factor_to_sm(h, e->d_sm);
+    if (!rc) {
+        he = hipMemcpy(sm, e->d_sm, sizeof(double) * n, hipMemcpyDeviceToHost);
+        if (he != hipSuccess) rc = hip_fail(he, "download SM");
+    }
+    if (!rc && mode == 3 && dcor) {
+        he = hipMemcpy(dcor, h->d.dcor, sizeof(double) * 2 * neq, hipMemcpyDeviceToHost);
+        if (he != hipSuccess) rc = hip_fail(he, "download DCOR");
+    }
+    if (rc) {
+        *ierr = -5;
+        report(lpu, name, *ierr);
+        return;
+    }
+    // outputs (sprfc1.f:245-267, sprfc3.f:245-266)
+    tol[1] = info.tol2;
+    tol[2] = info.tol3;
+    if (info.ierr == -3) {
+        *ierr = -3;
+        mspar[26] = info.supeqt;
+    } else if (info.ierr == 3) {
+        *ierr = 3;
+    }
+    mspar[25] = info.numneg;
+    report(lpu, name, *ierr);
+}
+
+// Solve driver shared by SPRFS1 and SPRBS1.  which = 1 forward, 2 backward.
+void solve_dropin(int which, sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa,
+                  double *sm, double *b, const sprgpu_int *ldb, const sprgpu_int *nrhs,
+                  const sprgpu_int *lpu, sprgpu_int *ierr, const char *name)
+{
+    if (mspar[0] < 0) {
+        mspar[0] = -7;
+        *ierr = -1;
+        set_error("error from a previous SPR stage is not cleared");
+        report(lpu, name, *ierr);
+        return;
+    }
+    mspar[0] = 7;
+    *ierr = 0;
+    const i64 neq = mspar[7];
+    if (mspar[56] == 1 && *ldb < neq) {
+        mspar[0] = -2;
+        *ierr = -2;
+        set_error("partial solve (MSPAR(57)=1, LDB<NEQ) is not supported");
+        report(lpu, name, *ierr);
+        return;
+    }
+    if (!lengths_ok_solve(mspar) || *ldb < neq) {
+        mspar[0] = -2;
+        *ierr = -2;
+        set_error("MTREES, MSIFA or LDB are not great enough");
+        report(lpu, name, *ierr);
+        return;
+    }
+    const i64 k = *nrhs;
+    if (k <= 0) return;
+    std::lock_guard<std::mutex> lock(g_mu);
+    Entry *e = nullptr;
+    bool fresh = false;
+    int rc = get_entry(sm, mspar, mtrees, msifa, e, &fresh);
+    if (rc) {
+        *ierr = rc == -5 ? -5 : -2;
+        if (*ierr == -2) mspar[0] = -2;
+        report(lpu, name, *ierr);
+        return;
+    }
+    sprgpu_handle *h = e->h;
+    hipError_t he = hipSuccess;
+    if (fresh || !h->factored) {
+        // the factor was computed elsewhere (e.g. by the reference SPRFC1):
+        // take it from SM
+        const i64 n = neq + h->plan.nzerol;
+        he = hipMemcpy(e->d_sm, sm, sizeof(double) * n, hipMemcpyHostToDevice);
+        if (he == hipSuccess) rc = load_factor_from_sm(h, e->d_sm);
+        else rc = hip_fail(he, "upload SM");
+    }
+    if (!rc && e->b_cap < neq * k) {
+        if (e->d_b) (void)hipFree(e->d_b);
+        e->d_b = nullptr;
+        he = hipMalloc(&e->d_b, sizeof(double) * neq * k);
+        if (he != hipSuccess) rc = hip_fail(he, "allocate B");
+        else e->b_cap = neq * k;
+    }
+    // PERI2E = MSIFA(1:NEQ) when MSPAR(58) >= 1 (sprfs1.f:192, SPRFB3/SPRFB4)
+    const bool perm = mspar[57] >= 1;
+    std::vector<double> tmp;
+    if (!rc) {
+        if (perm) {
+            tmp.resize(neq * k);
+            for (i64 c = 0; c < k; ++c)
+                for (i64 i = 0; i < neq; ++i) tmp[c * neq + i] = b[c * (*ldb) + msifa[i] - 1];
+            he = hipMemcpy(e->d_b, tmp.data(), sizeof(double) * neq * k, hipMemcpyHostToDevice);
+        } else {
+            he = hipMemcpy2D(e->d_b, sizeof(double) * neq, b, sizeof(double) * (*ldb),
+                             sizeof(double) * neq, k, hipMemcpyHostToDevice);
+        }
+        if (he != hipSuccess) rc = hip_fail(he, "upload B");
+    }
+    if (!rc) rc = run_solve(h, e->d_b, neq, k, which);
+    if (!rc) {
+        if (perm) {
+            he = hipMemcpy(tmp.data(), e->d_b, sizeof(double) * neq * k, hipMemcpyDeviceToHost);
+            if (he == hipSuccess)
+                for (i64 c = 0; c < k; ++c)
+                    for (i64 i = 0; i < neq; ++i) b[c * (*ldb) + msifa[i] - 1] = tmp[c * neq + i];
+        } else {
+            he = hipMemcpy2D(b, sizeof(double) * (*ldb), e->d_b, sizeof(double) * neq,
+                             sizeof(double) * neq, k, hipMemcpyDeviceToHost);
+        }
+        if (he != hipSuccess) rc = hip_fail(he, "download B");
+    }
+    if (rc) {
+        *ierr = -5;
+        report(lpu, name, *ierr);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void sprgpu_sprfc1(sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa, double *sm,
+                   double *tol, sprgpu_int *iwork, double *rwork, const sprgpu_int *lpu,
+                   sprgpu_int *ierr, void (*mvmult)(void), void (*mmmult)(void))
+{
+    (void)iwork;
+    (void)rwork;
+    (void)mvmult;
+    (void)mmmult;
+    factor_dropin(1, mspar, mtrees, msifa, sm, tol, nullptr, lpu, ierr, "SPRFC1");
+}
+
+void sprgpu_sprfc3(sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa, double *sm,
+                   double *tol, sprgpu_int *iwork, double *rwork, double *dcor,
+                   const sprgpu_int *lpu, sprgpu_int *ierr, void (*mvmult)(void),
+                   void (*mmmult)(void))
+{
+    (void)iwork;
+    (void)rwork;
+    (void)mvmult;
+    (void)mmmult;
+    factor_dropin(3, mspar, mtrees, msifa, sm, tol, dcor, lpu, ierr, "SPRFC3");
+}
+
+void sprgpu_sprfs1(sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa, double *sm,
+                   double *b, const sprgpu_int *ldb, const sprgpu_int *nrhs, sprgpu_int *iwork,
+                   double *rwork, const sprgpu_int *lpu, sprgpu_int *ierr)
+{
+    (void)iwork;
+    (void)rwork;
+    solve_dropin(1, mspar, mtrees, msifa, sm, b, ldb, nrhs, lpu, ierr, "SPRFS1");
+}
+
+void sprgpu_sprbs1(sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa, double *sm,
+                   double *b, const sprgpu_int *ldb, const sprgpu_int *nrhs, double *rwork,
+                   const sprgpu_int *lpu, sprgpu_int *ierr)
+{
+    (void)rwork;
+    solve_dropin(2, mspar, mtrees, msifa, sm, b, ldb, nrhs, lpu, ierr, "SPRBS1");
+}
+
+void sprgpu_sprsol(const sprgpu_int *iop, sprgpu_int *mspar, sprgpu_int *mtrees,
+                   sprgpu_int *msifa, double *sm, double *b, const sprgpu_int *ldb,
+                   const sprgpu_int *nrhs, double *tol, sprgpu_int *iwork, double *rwork,
+                   const sprgpu_int *lpu, sprgpu_int *ierr)
+{
+    // sprsol.f:29-50
+    *ierr = 0;
+    if (*iop <= 3)
+        sprgpu_sprfc1(mspar, mtrees, msifa, sm, tol, iwork, rwork, lpu, ierr, nullptr, nullptr);
+    if (*iop >= 2 && *iop <= 5 && *ierr >= 0)
+        sprgpu_sprfs1(mspar, mtrees, msifa, sm, b, ldb, nrhs, iwork, rwork, lpu, ierr);
+    if (*iop >= 3 && *iop != 5 && *ierr >= 0)
+        sprgpu_sprbs1(mspar, mtrees, msifa, sm, b, ldb, nrhs, rwork, lpu, ierr);
+}
+
+void sprgpu_release(const double *sm)
+{
+    std::lock_guard<std::mutex> lock(g_mu);
+    auto it = g_cache.find(sm);
+    if (it != g_cache.end()) {
+        release_entry(it->second);
+        g_cache.erase(it);
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,688 @@
+// Multifrontal supernodal LDL' factorization on gfx950.
+//
+// Replaces the numeric work of SPRF11/SPRF13/SPRF15/SPRF17 and the
+// LMVY18/LMMY18 callbacks (sprfc1.f:286-1299, lmvy18.f, lmmy18.f).  The
+// reference walks the supernodes in postorder with a LIFO stack of packed
+// update matrices; here every front lives in one HBM pool as a dense
+// column-major block, and all supernodes of one tree height are processed by
+// the same launches:
+//   k_front_init   copy the assembled trapezoid (packed LNZ) into the front,
+//                  zero the update block                      (sprfc1.f:518-523)
+//   k_extend_add   add each child's update block into its parent through
+//                  the child->parent relative map            (SPRF12/SPRF13)
+//   k_small        fronts with F<=64: whole LDL' + Schur update in LDS
+//   k_pdiag        blocked fronts: LDL' of the kPanelNB-wide diagonal block
+//   k_prows        ... and the panel TRSM of the rows below it (SPRF17)
+//   k_gemm         MFMA update C -= L D L^T on lower tiles: the trailing
+//                  update inside the panel and the Schur complement of the
+//                  update block                              (LMVY18/LMMY18)
+// Pivots follow the reference's rules: d is tested against TOL(1)*|a_jj|
+// with a_jj the assembled diagonal (sprfc1.f:1272-1279), columns are scaled
+// by the reciprocal 1/d (sprfc1.f:1288-1294), and in SPRFC3 mode small pivots
+// are replaced as in sprfc3.f:1083-1094.
+#include <cfloat>
+#include <climits>
+
+#include "sprgpu_internal.h"
+
+namespace sprgpu {
+
+typedef double dv4 __attribute__((ext_vector_type(4)));
+typedef float fv4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+    typedef dv4 acc_t;
+    static __device__ __forceinline__ acc_t op(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Mfma<float> {
+    typedef fv4 acc_t;
+    static __device__ __forceinline__ acc_t op(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// pivot rule
+// ---------------------------------------------------------------------------
+struct PivotArgs {
+    double tol1, tol4;
+    int mode;  // 1: SPRFC1 (record failure), 3: SPRFC3 (replace)
+};
+
+// Value of the pivot after the SPRFC3 replacement rule (identity in mode 1).
+__device__ __forceinline__ double pivot_value(double d, double a, const PivotArgs &pa)
+{
+    if (pa.mode == 3 && fabs(d) <= pa.tol1 * fabs(a)) {
+        const double mag = pa.tol1 * fmax(1.0, fabs(pa.tol4 * a));
+        return copysign(mag, d);
+    }
+    return d;
+}
+
+// Book-keeping for pivot eq (one thread, once per pivot).
+__device__ __forceinline__ void pivot_record(double d_in, double d_out, double a, int eq,
+                                             const PivotArgs &pa, double *dvec, double *dcor,
+                                             FactorStatus *st)
+{
+    dvec[eq] = d_out;
+    if (fabs(d_in) <= pa.tol1 * fabs(a)) {
+        if (pa.mode == 3) {
+            atomicAdd(&st->nrepl, 1ull);
+            dcor[2 * i64(eq)] = d_in;
+            dcor[2 * i64(eq) + 1] = d_out;
+        } else {
+            atomicMin(&st->first_fail, (unsigned long long)eq);
+        }
+    }
+    if (pa.mode == 3 && d_out < 0.0 && dcor[2 * i64(eq)] == 0.0) dcor[2 * i64(eq)] = d_out;
+}
+
+// ---------------------------------------------------------------------------
+// status reset (a kernel rather than a memset so that the sequence can be
+// captured in a graph)
+// ---------------------------------------------------------------------------
+__global__ void k_status_reset(FactorStatus *st)
+{
+    if (threadIdx.x == 0) {
+        st->first_fail = ULLONG_MAX;
+        st->first_zero = ULLONG_MAX;
+        st->nzero = 0;
+        st->nrepl = 0;
+        st->nnz_seen = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// front initialisation
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_front_init(const ColTask *__restrict__ tasks,
+                                                    const NodeInfo *__restrict__ nodes,
+                                                    const double *__restrict__ lnz,
+                                                    T *__restrict__ pool, double *adiag,
+                                                    double *dcor, FactorStatus *st, int mode)
+{
+    const ColTask t = tasks[blockIdx.x];
+    const NodeInfo n = nodes[t.node];
+    unsigned long long nnz = 0;
+    for (int c = t.c0; c < t.c1; ++c) {
+        T *col = pool + n.foff + i64(c) * n.ld;
+        if (c < n.S) {
+            // packed column c of the trapezoid starts at row c:
+            // offset c*F - c*(c-1)/2 (OFFSET(F,c) in sprfc1.f)
+            const double *src = lnz + n.xlnz + i64(c) * n.F - (i64(c) * (c - 1)) / 2;
+            for (int r = c + threadIdx.x; r < n.F; r += blockDim.x) {
+                double v = src[r - c];
+                nnz += (v != 0.0);
+                if (r == c) {
+                    const int eq = n.first + c;
+                    if (mode == 3) {
+                        dcor[2 * i64(eq)] = 0.0;
+                        dcor[2 * i64(eq) + 1] = 0.0;
+                    }
+                    if (v == 0.0) {
+                        if (mode == 3) {  // sprfc3.f:560-575
+                            v = 1.0;
+                            dcor[2 * i64(eq) + 1] = 1.0;
+                        } else {          // sprfc1.f:473-480
+                            atomicAdd(&st->nzero, 1ull);
+                            atomicMin(&st->first_zero, (unsigned long long)eq);
+                        }
+                    }
+                    adiag[eq] = v;
+                }
+                col[r] = T(v);
+            }
+        } else {
+            for (int r = c + threadIdx.x; r < n.F; r += blockDim.x) col[r] = T(0);
+        }
+    }
+    // block reduction of the non-zero count
+    __shared__ unsigned long long red[4];
+    for (int o = 32; o > 0; o >>= 1) nnz += __shfl_down(nnz, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = nnz;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long s = red[0] + red[1] + red[2] + red[3];
+        if (s) atomicAdd(&st->nnz_seen, s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// extend-add: parent(map[i], map[j]) += U_child(i, j), i >= j
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_extend_add(const EaTask *__restrict__ tasks,
+                                                    const NodeInfo *__restrict__ nodes,
+                                                    const i32 *__restrict__ relmap,
+                                                    T *__restrict__ pool)
+{
+    const EaTask t = tasks[blockIdx.x];
+    const NodeInfo c = nodes[t.child];
+    const NodeInfo p = nodes[c.parent];
+    const i32 *map = relmap + c.mapoff;
+    const int U = c.F - c.S;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int jc = t.c0 + wave; jc < t.c1; jc += 4) {
+        const int pj = map[jc];
+        const T *src = pool + c.foff + i64(c.S + jc) * c.ld + c.S;
+        T *dst = pool + p.foff + i64(pj) * p.ld;
+        for (int i = jc + lane; i < U; i += 64) dst[map[i]] += src[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// small fronts: everything in LDS, one workgroup per front
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_small(const i32 *__restrict__ list,
+                                               const NodeInfo *__restrict__ nodes,
+                                               T *__restrict__ pool, double *dvec,
+                                               const double *__restrict__ adiag, double *dcor,
+                                               FactorStatus *st, PivotArgs pa)
+{
+    constexpr int LDM = kSmallF + 1;
+    __shared__ T M[kSmallF * LDM];
+    __shared__ T lcol[kSmallF];
+    __shared__ T dsh;
+    const NodeInfo n = nodes[list[blockIdx.x]];
+    const int F = n.F, S = n.S, tid = threadIdx.x;
+    T *fr = pool + n.foff;
+    for (int idx = tid; idx < F * F; idx += 256) {
+        const int r = idx % F, c = idx / F;
+        if (r >= c) M[r + c * LDM] = fr[r + i64(c) * n.ld];
+    }
+    __syncthreads();
+    for (int j = 0; j < S; ++j) {
+        if (tid == 0) {
+            const int eq = n.first + j;
+            const double d_in = double(M[j + j * LDM]);
+            const double a = adiag[eq];
+            const double d = pivot_value(d_in, a, pa);
+            pivot_record(d_in, d, a, eq, pa, dvec, dcor, st);
+            M[j + j * LDM] = T(d);
+            dsh = T(d);
+        }
+        __syncthreads();
+        const T inv = T(1) / dsh;
+        for (int r = j + 1 + tid; r < F; r += 256) lcol[r] = M[r + j * LDM] * inv;
+        __syncthreads();
+        const int m = F - j - 1;
+        for (int idx = tid; idx < m * m; idx += 256) {
+            const int rr = idx % m, cc = idx / m;
+            if (rr >= cc) {
+                const int r = j + 1 + rr, c = j + 1 + cc;
+                M[r + c * LDM] -= M[r + j * LDM] * lcol[c];
+            }
+        }
+        __syncthreads();
+        for (int r = j + 1 + tid; r < F; r += 256) M[r + j * LDM] = lcol[r];
+        __syncthreads();
+    }
+    for (int idx = tid; idx < F * F; idx += 256) {
+        const int r = idx % F, c = idx / F;
+        if (r >= c) fr[r + i64(c) * n.ld] = M[r + c * LDM];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// panel step: factor the nb x nb diagonal block at column k0 and solve the
+// rows below it (L21 = A21 L11^-T D^-1), one row per thread
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int find_desc(const i64 *prefix, int n, i64 b)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Diagonal block: one workgroup per front of the launch factors the nb x nb
+// block at column k0 in LDS and writes d (diagonal) and l (below) back.
+template <typename T>
+__global__ __launch_bounds__(256) void k_pdiag(const PanelDesc *__restrict__ descs,
+                                               const NodeInfo *__restrict__ nodes,
+                                               T *__restrict__ pool, double *dvec,
+                                               const double *__restrict__ adiag, double *dcor,
+                                               FactorStatus *st, PivotArgs pa)
+{
+    constexpr int NB = kPanelNB;
+    __shared__ T L[NB][NB + 1];
+    __shared__ T dsh[NB];
+    const PanelDesc pd = descs[blockIdx.x];
+    const NodeInfo n = nodes[pd.node];
+    T *fr = pool + n.foff;
+    const i64 ld = n.ld;
+    const int k0 = pd.k0, nb = pd.nb, tid = threadIdx.x;
+    for (int idx = tid; idx < nb * nb; idx += 256) {
+        const int r = idx % nb, c = idx / nb;
+        if (r >= c) L[r][c] = fr[(k0 + r) + (k0 + c) * ld];
+    }
+    __syncthreads();
+    for (int j = 0; j < nb; ++j) {
+        if (tid == 0) {
+            const int eq = n.first + k0 + j;
+            const double d_in = double(L[j][j]);
+            const double a = adiag[eq];
+            const double d = pivot_value(d_in, a, pa);
+            pivot_record(d_in, d, a, eq, pa, dvec, dcor, st);
+            L[j][j] = T(d);
+            dsh[j] = T(d);
+        }
+        __syncthreads();
+        const T inv = T(1) / dsh[j];
+        const int m = nb - j - 1;
+        for (int idx = tid; idx < m * m; idx += 256) {
+            const int rr = idx % m, cc = idx / m;
+            if (rr >= cc) {
+                const int r = j + 1 + rr, c = j + 1 + cc;
+                L[r][c] -= L[r][j] * (L[c][j] * inv);
+            }
+        }
+        __syncthreads();
+        for (int r = j + 1 + tid; r < nb; r += 256) L[r][j] *= inv;
+        __syncthreads();
+    }
+    for (int idx = tid; idx < nb * nb; idx += 256) {
+        const int r = idx % nb, c = idx / nb;
+        if (r >= c) fr[(k0 + r) + (k0 + c) * ld] = L[r][c];
+    }
+}
+
+// Panel rows below the diagonal block: L21 = A21 L11^-T D^-1, one row per
+// thread, reading the factored diagonal block written by k_pdiag.
+template <typename T>
+__global__ __launch_bounds__(256) void k_prows(const PanelDesc *__restrict__ descs, int ndesc,
+                                               const i64 *__restrict__ prefix,
+                                               const NodeInfo *__restrict__ nodes,
+                                               T *__restrict__ pool)
+{
+    constexpr int NB = kPanelNB;
+    __shared__ T L[NB][NB + 1];
+    __shared__ T dinv[NB];
+    const int q = find_desc(prefix, ndesc, blockIdx.x);
+    const PanelDesc pd = descs[q];
+    const int rb = int(blockIdx.x - prefix[q]);
+    const NodeInfo n = nodes[pd.node];
+    T *fr = pool + n.foff;
+    const i64 ld = n.ld;
+    const int k0 = pd.k0, nb = pd.nb, tid = threadIdx.x;
+    for (int idx = tid; idx < nb * nb; idx += 256) {
+        const int r = idx % nb, c = idx / nb;
+        if (r >= c) L[r][c] = fr[(k0 + r) + (k0 + c) * ld];
+    }
+    __syncthreads();
+    if (tid < nb) dinv[tid] = T(1) / L[tid][tid];
+    __syncthreads();
+    const int r = k0 + nb + rb * kPanelRows + tid;
+    if (r < n.F) {
+        T x[NB];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) x[c] = (c < nb) ? fr[r + (k0 + c) * ld] : T(0);
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            T s = x[c];
+#pragma unroll
+            for (int i = 0; i < c; ++i) s -= x[i] * L[c][i];
+            x[c] = s;  // w_rc = l_rc * d_c
+        }
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+            if (c < nb) fr[r + (k0 + c) * ld] = x[c] * dinv[c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// MFMA update: C(i,j) -= sum_k L(i,k) d_k L(j,k) on lower tiles
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm(const GemmDesc *__restrict__ descs, int ndesc,
+                                              const i64 *__restrict__ prefix,
+                                              T *__restrict__ pool)
+{
+    constexpr int KB = 16;
+    constexpr int LDS = kTile + 2;
+    __shared__ T As[KB][LDS];
+    __shared__ T Bs[KB][LDS];
+    const int q = find_desc(prefix, ndesc, blockIdx.x);
+    const GemmDesc g = descs[q];
+    const i64 t = i64(blockIdx.x) - prefix[q];
+    // tile (ti,tj), column-major over the lower trapezoid: column tj holds
+    // nti - tj tiles and starts at tj*nti - tj*(tj-1)/2
+    int tj = 0;
+    {
+        const double a = double(g.nti) + 0.5;
+        double est = a - sqrt(a * a - 2.0 * double(t));
+        tj = int(est);
+        if (tj < 0) tj = 0;
+        if (tj > g.ntj - 1) tj = g.ntj - 1;
+        auto start = [&](i64 x) { return x * g.nti - (x * (x - 1)) / 2; };
+        while (tj > 0 && start(tj) > t) --tj;
+        while (tj + 1 < g.ntj && start(tj + 1) <= t) ++tj;
+        (void)est;
+    }
+    const int ti = tj + int(t - (i64(tj) * g.nti - (i64(tj) * (tj - 1)) / 2));
+    const int i0 = g.r0 + ti * kTile, j0 = g.r0 + tj * kTile;
+    T *fr = pool + g.foff;
+    const i64 ld = g.ld;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+
+    typename Mfma<T>::acc_t acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = typename Mfma<T>::acc_t{0, 0, 0, 0};
+
+    const int kend = g.k0 + g.kn;
+    const int lr = tid & 63;
+    for (int kk = g.k0; kk < kend; kk += KB) {
+        for (int kc = tid >> 6; kc < KB; kc += 4) {
+            const int col = kk + kc;
+            const bool cv = col < kend;
+            const int ra = i0 + lr, rbw = j0 + lr;
+            const T dcol = cv ? fr[col + col * ld] : T(0);
+            As[kc][lr] = (cv && ra < g.rend) ? fr[ra + col * ld] : T(0);
+            Bs[kc][lr] = (cv && rbw < g.cend) ? fr[rbw + col * ld] * dcol : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k4 = 0; k4 < KB; k4 += 4) {
+            const int kidx = k4 + (lane >> 4);
+            const T a0 = As[kidx][wr * 32 + (lane & 15)];
+            const T a1 = As[kidx][wr * 32 + 16 + (lane & 15)];
+            const T b0 = Bs[kidx][wc * 32 + (lane & 15)];
+            const T b1 = Bs[kidx][wc * 32 + 16 + (lane & 15)];
+            acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
+            acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
+            acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
+            acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int ii = 0; ii < 4; ++ii) {
+                const int row = i0 + wr * 32 + bi * 16 + 4 * (lane >> 4) + ii;
+                const int col = j0 + wc * 32 + bj * 16 + (lane & 15);
+                if (row < g.rend && col < g.cend && row >= col) fr[row + col * ld] -= acc[bi][bj][ii];
+            }
+}
+
+// ---------------------------------------------------------------------------
+// statistics: NUMNEG, TOL(2), TOL(3) (sprfc1.f:686-708, :1281-1285)
+// ---------------------------------------------------------------------------
+// stats layout: [0] numneg, [1] max a/d, [2] sum a, [3] min d>0
+constexpr int kStatBlocks = 256;
+
+__global__ __launch_bounds__(256) void k_stats_partial(const double *dvec, const double *adiag,
+                                                       i64 neq, const FactorStatus *st,
+                                                       double *part)
+{
+    const unsigned long long lim = st->first_fail;
+    double nneg = 0, mx = 0, sa = 0, mn = DBL_MAX;
+    for (i64 i = blockIdx.x * 256 + threadIdx.x; i < neq; i += i64(gridDim.x) * 256) {
+        const double d = dvec[i], a = adiag[i];
+        sa += a;
+        if (i64(i) < i64(lim < (unsigned long long)neq ? lim : neq)) {
+            if (d < 0) nneg += 1;
+            const double r = a / d;
+            if (r > mx) mx = r;
+        }
+        if (d > 0 && d < mn) mn = d;
+    }
+    __shared__ double s[4][256];
+    s[0][threadIdx.x] = nneg;
+    s[1][threadIdx.x] = mx;
+    s[2][threadIdx.x] = sa;
+    s[3][threadIdx.x] = mn;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            s[0][threadIdx.x] += s[0][threadIdx.x + o];
+            s[1][threadIdx.x] = fmax(s[1][threadIdx.x], s[1][threadIdx.x + o]);
+            s[2][threadIdx.x] += s[2][threadIdx.x + o];
+            s[3][threadIdx.x] = fmin(s[3][threadIdx.x], s[3][threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; ++k) part[4 * blockIdx.x + k] = s[k][0];
+}
+
+__global__ void k_stats_final(double *part, int nparts)
+{
+    if (threadIdx.x == 0) {
+        double nneg = 0, mx = 0, sa = 0, mn = DBL_MAX;
+        for (int b = 0; b < nparts; ++b) {
+            nneg += part[4 * b];
+            mx = fmax(mx, part[4 * b + 1]);
+            sa += part[4 * b + 2];
+            mn = fmin(mn, part[4 * b + 3]);
+        }
+        part[4 * nparts + 0] = nneg;
+        part[4 * nparts + 1] = mx;
+        part[4 * nparts + 2] = sa;
+        part[4 * nparts + 3] = mn;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// copy the factor into the packed SM layout
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy_out(const ColTask *__restrict__ tasks,
+                                                  const NodeInfo *__restrict__ nodes,
+                                                  const T *__restrict__ pool, double *sm,
+                                                  i64 neq, i32 node_limit)
+{
+    const ColTask t = tasks[blockIdx.x];
+    if (t.node >= node_limit) return;
+    const NodeInfo n = nodes[t.node];
+    double *lnz = sm + neq;
+    for (int c = t.c0; c < t.c1 && c < n.S; ++c) {
+        double *dst = lnz + n.xlnz + i64(c) * n.F - (i64(c) * (c - 1)) / 2;
+        const T *col = pool + n.foff + i64(c) * n.ld;
+        for (int r = c + threadIdx.x; r < n.F; r += blockDim.x) dst[r - c] = double(col[r]);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_load_factor(const ColTask *__restrict__ tasks,
+                                                     const NodeInfo *__restrict__ nodes,
+                                                     T *__restrict__ pool,
+                                                     const double *__restrict__ sm, i64 neq)
+{
+    const ColTask t = tasks[blockIdx.x];
+    const NodeInfo n = nodes[t.node];
+    const double *lnz = sm + neq;
+    for (int c = t.c0; c < t.c1 && c < n.S; ++c) {
+        const double *src = lnz + n.xlnz + i64(c) * n.F - (i64(c) * (c - 1)) / 2;
+        T *col = pool + n.foff + i64(c) * n.ld;
+        for (int r = c + threadIdx.x; r < n.F; r += blockDim.x) col[r] = T(src[r - c]);
+    }
+}
+
+__global__ void k_copy_diag(const double *dvec, const double *adiag, double *sm, i64 neq,
+                            i64 limit)
+{
+    for (i64 i = blockIdx.x * i64(blockDim.x) + threadIdx.x; i < neq;
+         i += i64(gridDim.x) * blockDim.x)
+        sm[i] = (i < limit) ? dvec[i] : adiag[i];
+}
+
+// ---------------------------------------------------------------------------
+// host orchestration
+// ---------------------------------------------------------------------------
+template <typename T>
+static int factor_impl(sprgpu_handle *h, const double *sm_dev, int mode, const double *tol)
+{
+    const Plan &p = h->plan;
+    DeviceBuffers &d = h->d;
+    hipStream_t s = h->stream;
+    T *pool = static_cast<T *>(d.pool);
+    const double *lnz = sm_dev + p.neq;
+    PivotArgs pa{tol[0], tol[3], mode};
+    k_status_reset<<<1, 64, 0, s>>>(d.status);
+    for (const Launch &L : p.launches) {
+        const unsigned blocks = unsigned(L.blocks);
+        switch (L.kind) {
+        case L_INIT:
+            k_front_init<T><<<blocks, 256, 0, s>>>(d.init_tasks + L.off, d.nodes, lnz, pool,
+                                                   d.adiag, d.dcor, d.status, mode);
+            break;
+        case L_EA:
+            k_extend_add<T><<<blocks, 256, 0, s>>>(d.ea_tasks + L.off, d.nodes, d.relmap, pool);
+            break;
+        case L_SMALL:
+            k_small<T><<<blocks, 256, 0, s>>>(d.small_nodes + L.off, d.nodes, pool, d.dvec,
+                                              d.adiag, d.dcor, d.status, pa);
+            break;
+        case L_PANEL:
+            k_pdiag<T><<<unsigned(L.count), 256, 0, s>>>(d.panel_desc + L.off, d.nodes, pool,
+                                                         d.dvec, d.adiag, d.dcor, d.status, pa);
+            k_prows<T><<<blocks, 256, 0, s>>>(d.panel_desc + L.off, int(L.count),
+                                              d.panel_prefix + L.off, d.nodes, pool);
+            break;
+        case L_GEMM:
+            k_gemm<T><<<blocks, 256, 0, s>>>(d.gemm_desc + L.off, int(L.count),
+                                             d.gemm_prefix + L.off, pool);
+            break;
+        }
+    }
+    k_stats_partial<<<kStatBlocks, 256, 0, s>>>(d.dvec, d.adiag, p.neq, d.status, d.stats);
+    k_stats_final<<<1, 64, 0, s>>>(d.stats, kStatBlocks);
+    SPRGPU_HIP(hipGetLastError());
+    return 0;
+}
+
+int run_factor(sprgpu_handle *h, const double *sm_dev, int mode, const double *tol)
+{
+    h->mode = mode;
+    for (int k = 0; k < 4; ++k) h->tol[k] = tol[k];
+    h->last_sm_dev = sm_dev;
+    int rc = (h->precision == SPRGPU_FP32) ? factor_impl<float>(h, sm_dev, mode, tol)
+                                           : factor_impl<double>(h, sm_dev, mode, tol);
+    if (rc == 0) h->factored = true;
+    return rc;
+}
+
+int read_factor_info(sprgpu_handle *h, sprgpu_factor_info *info)
+{
+    FactorStatus st;
+    double agg[4];
+    SPRGPU_HIP(hipMemcpyAsync(&st, h->d.status, sizeof(st), hipMemcpyDeviceToHost, h->stream));
+    SPRGPU_HIP(hipMemcpyAsync(agg, h->d.stats + 4 * kStatBlocks, sizeof(agg),
+                              hipMemcpyDeviceToHost, h->stream));
+    SPRGPU_HIP(hipStreamSynchronize(h->stream));
+    const i64 neq = h->plan.neq;
+    info->nzero_diag = i64(st.nzero);
+    info->ierr = 0;
+    info->numneg = 0;
+    info->supeqt = 0;
+    info->tol2 = 0;
+    info->tol3 = 0;
+    if (h->mode != 3 && st.nzero > 0) {
+        // zero diagonals before the factorization: sprfc1.f:463-488
+        info->ierr = -3;
+        info->numneg = i64(st.nzero);
+        info->supeqt = i64(st.first_zero) + 1;
+        info->tol2 = 0;
+        info->tol3 = agg[2];
+        return 0;
+    }
+    if (h->mode != 3 && st.first_fail != ULLONG_MAX) {
+        // pivot reduced to zero: sprfc1.f:1274-1279, :604-611
+        const i64 eq = i64(st.first_fail);
+        double dfail = 0, afail = 0;
+        SPRGPU_HIP(hipMemcpy(&dfail, h->d.dvec + eq, sizeof(double), hipMemcpyDeviceToHost));
+        SPRGPU_HIP(hipMemcpy(&afail, h->d.adiag + eq, sizeof(double), hipMemcpyDeviceToHost));
+        info->ierr = -3;
+        info->numneg = i64(agg[0]);
+        info->supeqt = eq + 1;
+        info->tol2 = dfail;
+        info->tol3 = afail;
+        return 0;
+    }
+    info->numneg = i64(agg[0]);
+    info->tol2 = (agg[1] != 0.0) ? 1.0 / agg[1] : 0.0;
+    const double mdiag = (agg[3] == DBL_MAX) ? 0.0 : agg[3];
+    info->tol3 = (mdiag != 0.0) ? agg[2] / mdiag : 0.0;
+    if (h->mode == 3 && st.nrepl > 0) {
+        info->ierr = -3;
+        info->supeqt = -i64(st.nrepl);
+    } else if (info->numneg > 0) {
+        info->ierr = 3;
+    }
+    (void)neq;
+    return 0;
+}
+
+int copy_factor_to_sm(sprgpu_handle *h, double *sm_out)
+{
+    // supernodes before the failing one only (the reference stops there)
+    FactorStatus st;
+    SPRGPU_HIP(hipMemcpyAsync(&st, h->d.status, sizeof(st), hipMemcpyDeviceToHost, h->stream));
+    SPRGPU_HIP(hipStreamSynchronize(h->stream));
+    const Plan &p = h->plan;
+    i32 node_limit = i32(p.nsuper);
+    i64 diag_limit = p.neq;
+    if (h->mode != 3 && st.nzero > 0) {
+        node_limit = 0;
+        diag_limit = 0;
+    } else if (h->mode != 3 && st.first_fail != ULLONG_MAX) {
+        diag_limit = i64(st.first_fail);
+        node_limit = 0;
+        while (node_limit < i32(p.nsuper) &&
+               p.nodes[node_limit].first + p.nodes[node_limit].S <= diag_limit)
+            ++node_limit;
+    }
+    const unsigned nt = unsigned(p.init_tasks.size());
+    if (nt) {
+        if (h->precision == SPRGPU_FP32)
+            k_copy_out<float><<<nt, 256, 0, h->stream>>>(
+                h->d.init_tasks, h->d.nodes, static_cast<const float *>(h->d.pool), sm_out,
+                p.neq, node_limit);
+        else
+            k_copy_out<double><<<nt, 256, 0, h->stream>>>(
+                h->d.init_tasks, h->d.nodes, static_cast<const double *>(h->d.pool), sm_out,
+                p.neq, node_limit);
+    }
+    k_copy_diag<<<256, 256, 0, h->stream>>>(h->d.dvec, h->d.adiag, sm_out, p.neq, diag_limit);
+    SPRGPU_HIP(hipGetLastError());
+    return 0;
+}
+
+int load_factor_from_sm(sprgpu_handle *h, const double *sm_dev)
+{
+    const Plan &p = h->plan;
+    const unsigned nt = unsigned(p.init_tasks.size());
+    if (nt) {
+        if (h->precision == SPRGPU_FP32)
+            k_load_factor<float><<<nt, 256, 0, h->stream>>>(
+                h->d.init_tasks, h->d.nodes, static_cast<float *>(h->d.pool), sm_dev, p.neq);
+        else
+            k_load_factor<double><<<nt, 256, 0, h->stream>>>(
+                h->d.init_tasks, h->d.nodes, static_cast<double *>(h->d.pool), sm_dev, p.neq);
+    }
+    SPRGPU_HIP(hipMemcpyAsync(h->d.dvec, sm_dev, sizeof(double) * p.neq,
+                              hipMemcpyDeviceToDevice, h->stream));
+    SPRGPU_HIP(hipGetLastError());
+    h->factored = true;
+    return 0;
+}
+
+}  // namespace sprgpu

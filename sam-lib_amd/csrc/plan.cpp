@@ -1,0 +1,361 @@
+// Host planner: turns the reference symbolic output into the device schedule.
+//
+// Inputs follow the reference layout exactly (sprfc1.f:161-203):
+//   MTREES(MSPAR(45)...) = XSUPER(NSUPER+1), XSIBL(NSUPER+1), SIBL(NSUPER+1)
+//   MTREES(MSPAR(46)...) = XLINDX(NSUPER+1), SUPSUP(NSUPER), ...
+//   MSIFA(MSPAR(47)...)  = LINDX(NOFSUB);  MSIFA(MSPAR(48)...) = XLNZ(NSUPER+1)
+// Supernodes are in postorder (children before parents), LINDX lists are
+// sorted ascending and start with the supernode's own columns
+// (sprsmb.f:512-575).
+//
+// The schedule groups supernodes by height in the elimination tree (leaves
+// at height 0): every supernode of one height is independent of the others,
+// so each kernel launch covers a whole height.  Within a height, children of
+// one parent are extend-added in rounds by sibling index so that no two
+// work items of one launch write the same front entry.
+#include <algorithm>
+#include <numeric>
+#include <thread>
+
+#include "sprgpu_internal.h"
+
+namespace sprgpu {
+
+namespace {
+
+inline i64 tri_tiles(i64 nti, i64 ntj)
+{
+    // tiles (ti,tj) with 0<=tj<ntj, tj<=ti<nti
+    return ntj * nti - ntj * (ntj - 1) / 2;
+}
+
+inline i64 round_up(i64 x, i64 m) { return (x + m - 1) / m * m; }
+
+}  // namespace
+
+std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, Plan &p)
+{
+    auto M = [&](int k) { return mspar[k - 1]; };
+    if (M(1) < 0) return "MSPAR(1) < 0: the symbolic phase did not complete";
+    p.neq = M(8);
+    p.nsuper = M(11);
+    p.maxsup = M(13);
+    p.mfront = M(14);
+    p.nofsub = M(15);
+    p.nzerol = M(16);
+    if (M(57) > 0) return "partial factorization (MSPAR(57)>0, retained supernodes) is not supported";
+    if (p.neq <= 0 || p.nsuper <= 0) return "empty system";
+    if (p.neq >= (i64(1) << 31)) return "NEQ too large for 32-bit row indices";
+    const i64 ns = p.nsuper;
+    const i64 *xsuper = mtrees + (M(45) - 1);
+    const i64 *xsibl = xsuper + (ns + 1);
+    const i64 *sibl = xsibl + (ns + 1);
+    const i64 *xlindx = mtrees + (M(46) - 1);
+    const i64 *supsup = xlindx + (ns + 1);
+    const i64 *lindx = msifa + (M(47) - 1);
+    const i64 *xlnz = msifa + (M(48) - 1);
+    for (i64 j = 0; j < ns; ++j)
+        if (supsup[j] > 0) return "retained supernodes (SUPSUP>0) are not supported";
+
+    p.lindx.resize(p.nofsub);
+    for (i64 k = 0; k < p.nofsub; ++k) p.lindx[k] = i32(lindx[k] - 1);
+
+    p.nodes.resize(ns);
+    p.children.clear();
+    p.children.reserve(ns);
+    std::vector<i32> parent(ns, -1);
+    for (i64 j = 0; j < ns; ++j) {
+        NodeInfo &n = p.nodes[j];
+        n.first = i32(xsuper[j] - 1);
+        n.S = i32(xsuper[j + 1] - xsuper[j]);
+        n.F = i32(xlindx[j + 1] - xlindx[j]);
+        n.lxoff = xlindx[j] - 1;
+        n.xlnz = xlnz[j] - 1;
+        n.child0 = i32(p.children.size());
+        for (i64 c = xsibl[j]; c < xsibl[j + 1]; ++c) {
+            const i32 ch = i32(sibl[c - 1] - 1);
+            if (ch < 0 || ch >= j) return "SIBL does not describe a postordered tree";
+            p.children.push_back(ch);
+            parent[ch] = i32(j);
+        }
+        n.nchild = i32(p.children.size()) - n.child0;
+        if (n.S <= 0 || n.F < n.S) return "inconsistent supernode partition";
+    }
+    for (i64 j = 0; j < ns; ++j) p.nodes[j].parent = parent[j];
+
+    // heights
+    std::vector<i32> height(ns, 0);
+    for (i64 j = 0; j < ns; ++j)
+        if (parent[j] >= 0) height[parent[j]] = std::max(height[parent[j]], height[j] + 1);
+    p.nlevels = ns ? 1 + *std::max_element(height.begin(), height.end()) : 0;
+    p.level_ptr.assign(p.nlevels + 1, 0);
+    for (i64 j = 0; j < ns; ++j) p.level_ptr[height[j] + 1]++;
+    for (int l = 0; l < p.nlevels; ++l) p.level_ptr[l + 1] += p.level_ptr[l];
+    p.level_nodes.resize(ns);
+    {
+        std::vector<i32> fill(p.level_ptr.begin(), p.level_ptr.end() - 1);
+        for (i64 j = 0; j < ns; ++j) p.level_nodes[fill[height[j]]++] = i32(j);
+    }
+
+    // front pool, vector workspace and child relative maps (SPRF12 logic,
+    // sprfc1.f:716-790: monotone merge of the sorted row lists)
+    i64 pool = 0, vw = 0, rm = 0;
+    for (i64 j = 0; j < ns; ++j) {
+        NodeInfo &n = p.nodes[j];
+        n.ld = i32(round_up(n.F, 2));
+        n.foff = pool;
+        pool += round_up(i64(n.ld) * n.F, 64);
+        n.voff = vw;
+        vw += n.F;
+        n.mapoff = rm;
+        rm += n.F - n.S;
+    }
+    p.pool_elems = pool;
+    p.vwork_elems = vw;
+    p.relmap.assign(rm, -1);
+    for (i64 c = 0; c < ns; ++c) {
+        const NodeInfo &n = p.nodes[c];
+        const i32 U = n.F - n.S;
+        if (U == 0) continue;
+        if (n.parent < 0) return "a root supernode has off-diagonal rows";
+        const NodeInfo &par = p.nodes[n.parent];
+        const i32 *sub = &p.lindx[n.lxoff + n.S];
+        const i32 *sup = &p.lindx[par.lxoff];
+        i32 q = 0;
+        for (i32 i = 0; i < U; ++i) {
+            while (q < par.F && sup[q] < sub[i]) ++q;
+            if (q >= par.F || sup[q] != sub[i]) return "child row list is not contained in the parent's";
+            p.relmap[n.mapoff + i] = q;
+        }
+    }
+
+    // flop count (RINFO(3), sprtrs.f:1961-1969)
+    p.factor_flops = 0;
+    for (i64 j = 0; j < ns; ++j) {
+        const double F = p.nodes[j].F, S = p.nodes[j].S;
+        p.factor_flops += (F * F + 2 * F) * S - 2 * (1 + F) * S * (S + 1) / 2 +
+                          S * (S + 1) * (2 * S + 1) / 6;
+    }
+
+    // ----------------------------------------------------------------------
+    // schedule
+    // ----------------------------------------------------------------------
+    p.init_tasks.clear();
+    p.ea_tasks.clear();
+    p.small_nodes.clear();
+    p.panel_desc.clear();
+    p.panel_prefix.clear();
+    p.gemm_desc.clear();
+    p.gemm_prefix.clear();
+    p.launches.clear();
+
+    auto add_launch = [&](LaunchKind kind, int level, i64 off, i64 count, i64 blocks) {
+        if (count > 0 && blocks > 0) p.launches.push_back(Launch{kind, level, off, count, blocks});
+    };
+
+    for (int l = 0; l < p.nlevels; ++l) {
+        const i32 *lv = &p.level_nodes[p.level_ptr[l]];
+        const i64 nl = p.level_ptr[l + 1] - p.level_ptr[l];
+        // 1. front initialisation
+        i64 off = i64(p.init_tasks.size());
+        for (i64 q = 0; q < nl; ++q) {
+            const NodeInfo &n = p.nodes[lv[q]];
+            for (i32 c = 0; c < n.F; c += kInitCols)
+                p.init_tasks.push_back(ColTask{lv[q], c, std::min(n.F, c + kInitCols), 0});
+        }
+        add_launch(L_INIT, l, off, i64(p.init_tasks.size()) - off, i64(p.init_tasks.size()) - off);
+        // 2. extend-add, one round per sibling index
+        i32 maxch = 0;
+        for (i64 q = 0; q < nl; ++q) maxch = std::max(maxch, p.nodes[lv[q]].nchild);
+        for (i32 r = 0; r < maxch; ++r) {
+            off = i64(p.ea_tasks.size());
+            for (i64 q = 0; q < nl; ++q) {
+                const NodeInfo &n = p.nodes[lv[q]];
+                if (r >= n.nchild) continue;
+                const i32 ch = p.children[n.child0 + r];
+                const i32 U = p.nodes[ch].F - p.nodes[ch].S;
+                for (i32 c = 0; c < U; c += kEaCols)
+                    p.ea_tasks.push_back(EaTask{ch, c, std::min(U, c + kEaCols), 0});
+            }
+            add_launch(L_EA, l, off, i64(p.ea_tasks.size()) - off, i64(p.ea_tasks.size()) - off);
+        }
+        // 3. small fronts in LDS
+        off = i64(p.small_nodes.size());
+        std::vector<i32> big;
+        for (i64 q = 0; q < nl; ++q) {
+            if (p.nodes[lv[q]].F <= kSmallF)
+                p.small_nodes.push_back(lv[q]);
+            else
+                big.push_back(lv[q]);
+        }
+        add_launch(L_SMALL, l, off, i64(p.small_nodes.size()) - off,
+                   i64(p.small_nodes.size()) - off);
+        // 4. blocked fronts: panel steps with trailing updates inside the panel,
+        //    then one Schur-complement update of the U block with K = S.
+        i32 maxsteps = 0;
+        for (i32 j : big) maxsteps = std::max(maxsteps, (p.nodes[j].S + kPanelNB - 1) / kPanelNB);
+        for (i32 s = 0; s < maxsteps; ++s) {
+            off = i64(p.panel_desc.size());
+            i64 nblk = 0;
+            for (i32 j : big) {
+                const NodeInfo &n = p.nodes[j];
+                const i32 k0 = s * kPanelNB;
+                if (k0 >= n.S) continue;
+                const i32 nb = std::min(kPanelNB, n.S - k0);
+                const i32 rows = n.F - (k0 + nb);
+                const i32 nrb = std::max(1, (rows + kPanelRows - 1) / kPanelRows);
+                p.panel_desc.push_back(PanelDesc{j, k0, nb, nrb});
+                p.panel_prefix.push_back(nblk);
+                nblk += nrb;
+            }
+            add_launch(L_PANEL, l, off, i64(p.panel_desc.size()) - off, nblk);
+            // trailing update restricted to the remaining panel columns
+            off = i64(p.gemm_desc.size());
+            i64 ntiles = 0;
+            for (i32 j : big) {
+                const NodeInfo &n = p.nodes[j];
+                const i32 k0 = s * kPanelNB;
+                if (k0 >= n.S) continue;
+                const i32 nb = std::min(kPanelNB, n.S - k0);
+                const i32 r0 = k0 + nb;
+                if (r0 >= n.S) continue;
+                GemmDesc g;
+                g.foff = n.foff;
+                g.ld = n.ld;
+                g.r0 = r0;
+                g.rend = n.F;
+                g.cend = n.S;
+                g.k0 = k0;
+                g.kn = nb;
+                g.nti = (n.F - r0 + kTile - 1) / kTile;
+                g.ntj = (n.S - r0 + kTile - 1) / kTile;
+                p.gemm_desc.push_back(g);
+                p.gemm_prefix.push_back(ntiles);
+                ntiles += tri_tiles(g.nti, g.ntj);
+            }
+            add_launch(L_GEMM, l, off, i64(p.gemm_desc.size()) - off, ntiles);
+        }
+        // Schur complement of every blocked front of this level
+        off = i64(p.gemm_desc.size());
+        i64 ntiles = 0;
+        for (i32 j : big) {
+            const NodeInfo &n = p.nodes[j];
+            if (n.F == n.S) continue;
+            GemmDesc g;
+            g.foff = n.foff;
+            g.ld = n.ld;
+            g.r0 = n.S;
+            g.rend = n.F;
+            g.cend = n.F;
+            g.k0 = 0;
+            g.kn = n.S;
+            g.nti = (n.F - n.S + kTile - 1) / kTile;
+            g.ntj = g.nti;
+            p.gemm_desc.push_back(g);
+            p.gemm_prefix.push_back(ntiles);
+            ntiles += tri_tiles(g.nti, g.ntj);
+        }
+        add_launch(L_GEMM, l, off, i64(p.gemm_desc.size()) - off, ntiles);
+    }
+    return "";
+}
+
+// CSR pattern of the assembled matrix from the non-zeros of the packed LNZ.
+// Rows are sorted by column: columns are visited in ascending order, each
+// contributing its sub-diagonal entries to later rows (their lower parts)
+// and, in its own row, the diagonal and the upper part.  Threads own
+// contiguous column ranges, so per-thread counts keep the order.
+void build_csr_pattern(const Plan &p, const double *lnz, CsrPattern &pat)
+{
+    const i64 neq = p.neq, ns = p.nsuper;
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    // split supernodes into nt ranges of similar LNZ size
+    std::vector<i64> cut(nt + 1, ns);
+    cut[0] = 0;
+    {
+        const i64 total = p.nzerol;
+        i64 acc = 0;
+        unsigned t = 1;
+        for (i64 j = 0; j < ns && t < nt; ++j) {
+            const i64 F = p.nodes[j].F, S = p.nodes[j].S;
+            acc += F * S - S * (S - 1) / 2;
+            if (acc >= total * i64(t) / i64(nt)) cut[t++] = j + 1;
+        }
+        for (; t < nt; ++t) cut[t] = ns;
+    }
+    std::vector<std::vector<i32>> cnt(nt, std::vector<i32>(neq, 0));
+    std::vector<i64> nzl(nt, 0);
+    auto count = [&](unsigned t) {
+        auto &c = cnt[t];
+        for (i64 j = cut[t]; j < cut[t + 1]; ++j) {
+            const NodeInfo &n = p.nodes[j];
+            const i32 *rows = &p.lindx[n.lxoff];
+            i64 pos = n.xlnz;
+            for (i32 col = 0; col < n.S; ++col) {
+                const i32 ceq = n.first + col;
+                for (i32 r = col; r < n.F; ++r, ++pos) {
+                    if (lnz[pos] == 0.0) continue;
+                    nzl[t]++;
+                    c[ceq]++;
+                    if (r != col) c[rows[r]]++;
+                }
+            }
+        }
+    };
+    {
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nt; ++t) th.emplace_back(count, t);
+        for (auto &x : th) x.join();
+    }
+    pat.rowptr.assign(neq + 1, 0);
+    for (i64 i = 0; i < neq; ++i) {
+        i64 s = 0;
+        for (unsigned t = 0; t < nt; ++t) s += cnt[t][i];
+        pat.rowptr[i + 1] = pat.rowptr[i] + s;
+    }
+    pat.nnz_lower = 0;
+    for (unsigned t = 0; t < nt; ++t) pat.nnz_lower += nzl[t];
+    const i64 nnz = pat.rowptr[neq];
+    pat.col.assign(nnz, 0);
+    pat.src.assign(nnz, 0);
+    // per-thread start cursor in each row = rowptr + counts of earlier threads
+    std::vector<std::vector<i64>> cur(nt);
+    {
+        std::vector<i64> acc(pat.rowptr.begin(), pat.rowptr.end() - 1);
+        for (unsigned t = 0; t < nt; ++t) {
+            cur[t].assign(acc.begin(), acc.end());
+            for (i64 i = 0; i < neq; ++i) acc[i] += cnt[t][i];
+        }
+    }
+    auto fill = [&](unsigned t) {
+        auto &cu = cur[t];
+        for (i64 j = cut[t]; j < cut[t + 1]; ++j) {
+            const NodeInfo &n = p.nodes[j];
+            const i32 *rows = &p.lindx[n.lxoff];
+            i64 pos = n.xlnz;
+            for (i32 col = 0; col < n.S; ++col) {
+                const i32 ceq = n.first + col;
+                for (i32 r = col; r < n.F; ++r, ++pos) {
+                    if (lnz[pos] == 0.0) continue;
+                    // row ceq, column rows[r] (upper part incl. diagonal)
+                    i64 k = cu[ceq]++;
+                    pat.col[k] = rows[r];
+                    pat.src[k] = pos;
+                    if (r != col) {
+                        // row rows[r], column ceq (lower part)
+                        k = cu[rows[r]]++;
+                        pat.col[k] = ceq;
+                        pat.src[k] = pos;
+                    }
+                }
+            }
+        }
+    };
+    {
+        std::vector<std::thread> th;
+        for (unsigned t = 0; t < nt; ++t) th.emplace_back(fill, t);
+        for (auto &x : th) x.join();
+    }
+}
+
+}  // namespace sprgpu

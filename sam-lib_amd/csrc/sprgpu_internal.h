@@ -1,0 +1,179 @@
+// Internal declarations of the sprgpu library (host plan, device handle,
+// kernel launchers).  Not part of the public C ABI (include/sprgpu.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "sprgpu.h"
+
+namespace sprgpu {
+
+using i64 = int64_t;
+using i32 = int32_t;
+
+// Tile and block sizes shared by the planner and the kernels.
+constexpr int kTile = 64;        // output tile edge of the MFMA update kernel
+constexpr int kPanelNB = 32;     // columns per panel step of the blocked front factor
+constexpr int kPanelRows = 256;  // panel rows per workgroup (one row per thread)
+constexpr int kSmallF = 64;      // fronts with F <= kSmallF are factored in LDS
+constexpr int kInitCols = 8;     // front columns per workgroup in the init kernel
+constexpr int kEaCols = 8;       // child columns per workgroup in extend-add
+
+// One supernode as the device sees it.  Rows of the front are the supernode's
+// LINDX list (sorted ascending, the S own columns first); the front is a dense
+// column-major F x F block (lower triangle used) with leading dimension ld.
+struct NodeInfo {
+    i64 foff;      // offset of the front in the pool (elements)
+    i64 lxoff;     // offset of the row list in lindx
+    i64 xlnz;      // offset of the packed trapezoid in LNZ (0-based)
+    i64 voff;      // offset of the node's vector slot in the solve workspace
+    i64 mapoff;    // offset of this node's (child) relative map in relmap
+    i32 first;     // first equation (0-based)
+    i32 S, F, ld;
+    i32 parent;    // -1 for a root
+    i32 child0, nchild;  // range in the children array (SIBL order)
+};
+
+// Work descriptors consumed by the kernels (built once per plan).
+struct ColTask { i32 node, c0, c1, pad; };        // front columns [c0,c1)
+struct EaTask { i32 child, c0, c1, pad; };        // child update columns [c0,c1)
+struct PanelDesc { i32 node, k0, nb, nrb; };       // panel step at column k0, nrb row blocks
+// Lower-trapezoidal tile set: rows [r0,rend) x cols [r0,cend) (tiles of
+// kTile anchored at r0, only tiles with ti>=tj), updated with columns
+// [k0,k0+kn) of the same front.
+struct GemmDesc {
+    i64 foff;
+    i32 ld, r0, rend, cend, k0, kn;
+    i32 nti, ntj;  // tile rows / tile cols
+};
+
+enum LaunchKind : i32 {
+    L_INIT = 0,
+    L_EA = 1,
+    L_SMALL = 2,
+    L_PANEL = 3,
+    L_GEMM = 4,
+};
+
+// A launch of one kernel over a contiguous range of a task array.
+struct Launch {
+    i32 kind;
+    i32 level;
+    i64 off, count;  // range in the task array of this kind
+    i64 blocks;      // grid size
+};
+
+struct Plan {
+    i64 neq = 0, nsuper = 0, nofsub = 0, nzerol = 0, mfront = 0, maxsup = 0;
+    i64 pool_elems = 0, vwork_elems = 0;
+    i32 nlevels = 0;
+    std::vector<NodeInfo> nodes;
+    std::vector<i32> children;
+    std::vector<i32> lindx;         // 0-based equations, NOFSUB
+    std::vector<i32> relmap;        // per child: its U rows -> parent local row
+    std::vector<i32> level_ptr, level_nodes;  // nodes grouped by height
+    // factor schedule
+    std::vector<ColTask> init_tasks;
+    std::vector<EaTask> ea_tasks;
+    std::vector<i32> small_nodes;
+    std::vector<PanelDesc> panel_desc;
+    std::vector<i64> panel_prefix;  // per launch-local prefix of row blocks (exclusive)
+    std::vector<GemmDesc> gemm_desc;
+    std::vector<i64> gemm_prefix;   // per launch-local prefix of tiles (exclusive)
+    std::vector<Launch> launches;
+    double factor_flops = 0;        // RINFO(3) formula, sprtrs.f:1961-1969
+};
+
+// Builds the plan from the reference symbolic arrays (1-based, INTEGER*8).
+// Returns an empty string or an error message.
+std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, Plan &plan);
+
+// Pattern of A used by the FP64 residual: CSR over all rows, with the
+// position of each value in the packed LNZ array.
+struct CsrPattern {
+    std::vector<i64> rowptr;  // neq+1
+    std::vector<i32> col;     // nnz
+    std::vector<i64> src;     // nnz, index into LNZ (0-based)
+    i64 nnz_lower = 0;        // non-zeros in the packed lower triangle
+};
+void build_csr_pattern(const Plan &plan, const double *lnz, CsrPattern &pat);
+
+struct FactorStatus {
+    unsigned long long first_fail;   // smallest failing equation (UINT64_MAX = none)
+    unsigned long long first_zero;   // smallest zero diagonal before the factor
+    unsigned long long nzero;        // zero diagonals before the factor
+    unsigned long long nrepl;        // pivots replaced (SPRFC3 mode)
+    unsigned long long nnz_seen;     // non-zeros in the packed input
+    unsigned long long pad[3];
+};
+
+struct DeviceBuffers {
+    NodeInfo *nodes = nullptr;
+    i32 *children = nullptr, *lindx = nullptr, *relmap = nullptr;
+    ColTask *init_tasks = nullptr;
+    EaTask *ea_tasks = nullptr;
+    i32 *small_nodes = nullptr;
+    i32 *level_nodes = nullptr;
+    PanelDesc *panel_desc = nullptr;
+    i64 *panel_prefix = nullptr;
+    GemmDesc *gemm_desc = nullptr;
+    i64 *gemm_prefix = nullptr;
+    void *pool = nullptr;       // T[pool_elems]
+    double *dvec = nullptr;     // pivots d (NEQ), FP64
+    double *adiag = nullptr;    // a_jj of the assembled matrix (NEQ)
+    double *dcor = nullptr;     // DCOR(2,NEQ) for SPRFC3 mode
+    FactorStatus *status = nullptr;
+    double *stats = nullptr;    // reduction scratch for TOL(2)/TOL(3)/NUMNEG
+    double *vwork = nullptr;    // solve workspace (vwork_elems * nrhs_cap)
+    i64 vwork_cap = 0;
+    // residual pattern
+    i64 *csr_rowptr = nullptr;
+    i32 *csr_col = nullptr;
+    i64 *csr_src = nullptr;
+    double *csr_val = nullptr;
+    i64 csr_nnz = 0, nnz_lower = 0;
+    double *irwork = nullptr;   // IR vectors
+    i64 irwork_cap = 0;
+};
+
+}  // namespace sprgpu
+
+struct sprgpu_handle {
+    sprgpu::Plan plan;
+    sprgpu::DeviceBuffers d;
+    int precision = SPRGPU_FP64;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    bool factored = false;
+    bool has_pattern = false;
+    int mode = 1;
+    double tol[4] = {0, 0, 0, 0};
+    const double *last_sm_dev = nullptr;  // input of the last factorization
+    sprgpu_int device_bytes = 0;
+};
+
+namespace sprgpu {
+void set_error(const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+#define SPRGPU_HIP(call)                                              \
+    do {                                                              \
+        hipError_t e_ = (call);                                       \
+        if (e_ != hipSuccess) return ::sprgpu::hip_fail(e_, #call);   \
+    } while (0)
+
+// factor.hip
+int run_factor(sprgpu_handle *h, const double *sm_dev, int mode, const double *tol);
+int read_factor_info(sprgpu_handle *h, sprgpu_factor_info *info);
+int copy_factor_to_sm(sprgpu_handle *h, double *sm_dev_out);
+int load_factor_from_sm(sprgpu_handle *h, const double *sm_dev);
+// solve.hip
+int run_solve(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which);
+int run_spmv(sprgpu_handle *h, const double *x, double *y, i64 ld, i64 nrhs);
+int run_solve_ir(sprgpu_handle *h, const double *b, double *x, i64 ld, i64 nrhs, double rtol,
+                 int maxit, sprgpu_ir_info *info);
+int upload_pattern(sprgpu_handle *h, const CsrPattern &pat);
+}  // namespace sprgpu

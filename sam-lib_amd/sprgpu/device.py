@@ -1,0 +1,119 @@
+"""Device-resident factor handle (include/sprgpu.h, section 2).
+
+Inputs and outputs are torch tensors on the GPU; torch only provides the
+device memory and the stream.  Right-hand sides are (nrhs, neq) contiguous
+float64 tensors, i.e. column-major (neq, nrhs) blocks with LDB = neq.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import FactorInfo, IrInfo, check, lib
+
+FP64, FP32 = 0, 1
+
+
+def _ptr(t) -> C.c_void_p:
+    return C.c_void_p(t.data_ptr())
+
+
+def _np(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class SprFactor:
+    """A symbolic structure plus the GPU state to factor and solve with it."""
+
+    def __init__(self, mspar, mtrees, msifa, precision: str = "fp64", stream=None):
+        import torch
+
+        if not torch.cuda.is_available():
+            raise RuntimeError("SprFactor needs a GPU")
+        self._torch = torch
+        self.mspar = np.ascontiguousarray(mspar, dtype=np.int64)
+        self.mtrees = np.ascontiguousarray(mtrees, dtype=np.int64)
+        self.msifa = np.ascontiguousarray(msifa, dtype=np.int64)
+        prec = {"fp64": FP64, "fp32": FP32}[precision]
+        self.precision = precision
+        h = C.c_void_p()
+        check(lib().sprgpu_create(_np(self.mspar), _np(self.mtrees), _np(self.msifa), prec,
+                                  C.byref(h)), "sprgpu_create")
+        self._h = h
+        self.neq = int(lib().sprgpu_neq(h))
+        self.nzerol = int(lib().sprgpu_nzerol(h))
+        self.set_stream(stream)
+
+    def set_stream(self, stream=None):
+        torch = self._torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        self._stream = s
+        check(lib().sprgpu_set_stream(self._h, C.c_void_p(s.cuda_stream)), "sprgpu_set_stream")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sprgpu_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def device_bytes(self) -> int:
+        return int(lib().sprgpu_device_bytes(self._h))
+
+    def set_pattern(self, sm_host: np.ndarray):
+        sm_host = np.ascontiguousarray(sm_host, dtype=np.float64)
+        check(lib().sprgpu_set_pattern(self._h, _np(sm_host)), "sprgpu_set_pattern")
+
+    def factor(self, sm_dev, tol=(0.0, 0.0, 0.0, 0.0), mode: int = 1, info: bool = True):
+        tolv = np.zeros(4)
+        tolv[:len(tol)] = tol
+        if info:
+            fi = FactorInfo()
+            check(lib().sprgpu_factor(self._h, _ptr(sm_dev), _np(tolv), mode, C.byref(fi)),
+                  "sprgpu_factor")
+            return fi.as_dict()
+        check(lib().sprgpu_factor(self._h, _ptr(sm_dev), _np(tolv), mode, None), "sprgpu_factor")
+        return None
+
+    def factor_info(self) -> dict:
+        fi = FactorInfo()
+        check(lib().sprgpu_factor_info_get(self._h, C.byref(fi)), "sprgpu_factor_info_get")
+        return fi.as_dict()
+
+    def solve(self, b_dev, which: int = 3):
+        """In-place forward (1), backward (2) or both (3) on b_dev (nrhs, neq)."""
+        b2 = b_dev if b_dev.dim() == 2 else b_dev.view(1, -1)
+        check(lib().sprgpu_solve(self._h, _ptr(b2), self.neq, b2.shape[0], which), "sprgpu_solve")
+        return b_dev
+
+    def spmv(self, x_dev, y_dev):
+        x2 = x_dev if x_dev.dim() == 2 else x_dev.view(1, -1)
+        y2 = y_dev if y_dev.dim() == 2 else y_dev.view(1, -1)
+        check(lib().sprgpu_spmv(self._h, _ptr(x2), _ptr(y2), self.neq, x2.shape[0]),
+              "sprgpu_spmv")
+        return y_dev
+
+    def solve_ir(self, b_dev, x_dev, rtol: float = 1e-10, maxit: int = 20) -> dict:
+        b2 = b_dev if b_dev.dim() == 2 else b_dev.view(1, -1)
+        x2 = x_dev if x_dev.dim() == 2 else x_dev.view(1, -1)
+        info = IrInfo()
+        check(lib().sprgpu_solve_ir(self._h, _ptr(b2), _ptr(x2), self.neq, b2.shape[0],
+                                    float(rtol), int(maxit), C.byref(info)), "sprgpu_solve_ir")
+        return {"iterations": info.iterations, "relres": info.relres,
+                "converged": bool(info.converged)}
+
+    def get_factor(self, out=None):
+        """The factor in SM layout (numpy on the host, or a device tensor)."""
+        if out is None:
+            out = np.zeros(self.neq + self.nzerol)
+        if isinstance(out, np.ndarray):
+            check(lib().sprgpu_get_factor(self._h, _np(out)), "sprgpu_get_factor")
+        else:
+            check(lib().sprgpu_get_factor(self._h, _ptr(out)), "sprgpu_get_factor")
+        return out

@@ -1,0 +1,222 @@
+"""Pins the oracle (oracle/spr_oracle.c, the C restatement of the SPR numeric
+path) before it is trusted as the checker of the GPU library.
+
+1. Golden fixtures written by the reference itself (tests/golden/*.npz): the
+   oracle must reproduce the reference's factor, solution, TOL, MSPAR, IERR
+   and DCOR bit for bit, including the error paths (-35 zero diagonal, -30
+   pivot below TOL(1), SPRFC3 replacement) and cache panelling (MSPAR(9)).
+2. The live reference build (oracle/_ref) on further models, bit for bit.
+3. Mathematics that does not depend on the reference: a single dense front
+   equals the textbook LDL' obtained from numpy's Cholesky; NUMNEG equals the
+   number of negative eigenvalues (Sylvester's law of inertia); the solution
+   of A x = A x* recovers x*; SPRPRM equals the dense product.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import make_system
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+
+
+def _load(path):
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+@pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p)[:-4] for p in GOLDEN])
+def test_oracle_reproduces_reference_fixture(path):
+    g = _load(path)
+    ms = g["mspar_in"].copy()
+    sm = g["sm_in"].copy()
+    x = np.asfortranarray(g["b"].copy())
+    if int(g["mode"]) == 3:
+        ierr, tol, dcor = oracle.sprfc3(ms, g["mtrees"], g["msifa"], sm, g["tol_in"])
+        assert np.array_equal(dcor.T.ravel(), g["dcor"])
+        if "ierr_solve" in g:
+            ierr2, _ = oracle.sprsol(4, ms, g["mtrees"], g["msifa"], sm, x, tol=tol[:3])
+            assert ierr2 == int(g["ierr_solve"])
+    else:
+        ierr, tol = oracle.sprsol(3, ms, g["mtrees"], g["msifa"], sm, x, tol=g["tol_in"])
+    assert ierr == int(g["ierr"])
+    assert np.array_equal(ms, g["mspar_out"])
+    assert np.array_equal(tol, g["tol_out"])
+    assert np.array_equal(sm, g["sm_out"])
+    assert np.array_equal(x, g["x"])
+
+
+def test_fixtures_cover_the_error_paths():
+    by = {os.path.basename(p)[:-4]: _load(p) for p in GOLDEN}
+    # SPRSOL(IOP=3) returns the IERR of its last step: SPRFS1 resets the
+    # factor's warning 3 to 0 (sprsol.f:38-47); NUMNEG stays in MSPAR(26)
+    assert int(by["hex8_3_indefinite"]["ierr"]) == 0 and by["hex8_3_indefinite"]["mspar_out"][25] > 0
+    assert int(by["hex8_2_zero_diag"]["ierr"]) == -3
+    assert int(by["hex8_2_singular"]["ierr"]) == -3
+    assert int(by["hex8_2_sprfc3"]["mspar_out"][26]) < 0   # -#replaced pivots
+    assert int(by["hex8_4_panels"]["mspar_in"][21]) > 0      # NBLOCK > 0: panelled path
+
+
+def test_live_reference_symbolic_matches_fixture():
+    """The reference build used by the tests is the one that wrote the fixtures."""
+    import refspr
+    from sprgpu import fem
+
+    g = _load(os.path.join(os.path.dirname(__file__), "golden", "hex8_3_lognormal.npz"))
+    model = fem.hex8_grid(3, e_sigma=0.5, e_seed=1)
+    sysm = refspr.symbolic(model)
+    assert np.array_equal(sysm.mspar, g["mspar_in"])
+    assert np.array_equal(sysm.mtrees, g["mtrees"])
+    assert np.array_equal(sysm.msifa, g["msifa"])
+    assert np.array_equal(refspr.assemble(model, sysm), g["sm_in"])
+
+
+LIVE = [("hex8", 5, {"e_sigma": 0.5, "e_seed": 1}), ("quad4", 12, {}),
+        ("frame", 120, {"seed": 5}), ("hex8", 7, {})]
+
+
+@pytest.mark.parametrize("kind,n,kw", LIVE, ids=[f"{c[0]}{c[1]}" for c in LIVE])
+def test_oracle_bitexact_vs_live_reference(kind, n, kw):
+    import refspr
+
+    model, sysm, sm = make_system(kind, n, **kw)
+    b = np.asfortranarray(np.random.default_rng(7).uniform(-1, 1, (sysm.neq, 2)))
+    sm_r, x_r, ms_r = sm.copy(), b.copy(order="F"), sysm.mspar.copy()
+    ierr_r, _, tol_r = refspr.sol(3, sysm, sm_r, x_r, mspar=ms_r)
+    sm_o, x_o, ms_o = sm.copy(), b.copy(order="F"), sysm.mspar.copy()
+    ierr_o, tol_o = oracle.sprsol(3, ms_o, sysm.mtrees, sysm.msifa, sm_o, x_o)
+    assert ierr_o == ierr_r == 0
+    assert np.array_equal(sm_o, sm_r)
+    assert np.array_equal(x_o, x_r)
+    assert np.array_equal(tol_o, tol_r)
+    assert np.array_equal(ms_o, ms_r)
+
+
+# ---------------------------------------------------------------------------
+# mathematics, independent of the reference
+# ---------------------------------------------------------------------------
+
+def dense_equation_matrix(model, sysm):
+    """Assembled A in SPR equation order, built from the element matrices."""
+    neq = sysm.neq
+    A = np.zeros((neq, neq))
+    for e in range(model.nel):
+        dofs = model.element_dofs(e)
+        eq = sysm.meqn[dofs]
+        k = model.element_matrix(e)
+        keep = eq > 0
+        idx = eq[keep] - 1
+        A[np.ix_(idx, idx)] += k[np.ix_(keep, keep)]
+    return A
+
+
+def _single_front_model(nn, rng):
+    """One element coupling nn one-dof nodes: the factor is a single dense front."""
+    from sprgpu.fem import SamModel
+
+    Q = rng.normal(size=(nn, nn))
+    K = Q @ Q.T + nn * np.eye(nn)
+    return SamModel("dense", nn, 1, nn, np.arange(1, nn + 2, dtype=np.int64),
+                    np.ones(nn, dtype=np.int64), np.array([1, nn + 1], dtype=np.int64),
+                    np.arange(1, nn + 1, dtype=np.int64), np.array([nn], dtype=np.int64),
+                    K.ravel(order="F").copy(), np.zeros(1, dtype=np.int64), None)
+
+
+def test_single_front_equals_textbook_ldlt():
+    import refspr
+
+    model = _single_front_model(40, np.random.default_rng(11))
+    sysm = refspr.symbolic(model)
+    assert sysm.nsuper == 1
+    sm = refspr.assemble(model, sysm)
+    A = dense_equation_matrix(model, sysm)
+    ms = sysm.mspar.copy()
+    ierr, _ = oracle.sprsol(1, ms, sysm.mtrees, sysm.msifa, sm)
+    assert ierr == 0
+    C = np.linalg.cholesky(A)          # A = C C^T, C = L sqrt(D)
+    d_ref = np.diag(C) ** 2
+    L_ref = C / np.diag(C)
+    n = sysm.neq
+    np.testing.assert_allclose(sm[:n], d_ref, rtol=1e-12)
+    # packed columns: d on the diagonal, l below
+    pos = n
+    for j in range(n):
+        col = sm[pos:pos + n - j]
+        np.testing.assert_allclose(col[0], d_ref[j], rtol=1e-12)
+        np.testing.assert_allclose(col[1:], L_ref[j + 1:, j], rtol=1e-10, atol=1e-13)
+        pos += n - j
+
+
+def test_numneg_is_inertia():
+    """Sylvester: NUMNEG of LDL' = number of negative eigenvalues of A."""
+    import refspr
+    from sprgpu import fem
+
+    sys_path = os.path.join(os.path.dirname(__file__), "golden")
+    g = _load(os.path.join(sys_path, "hex8_3_indefinite.npz"))
+    model = fem.hex8_grid(3)
+    sysm = refspr.symbolic(model)
+    # the fixture's matrix: element matrices K_e - 0.08 I
+    A = dense_equation_matrix(model, sysm) - 0.08 * np.diag(_multiplicity(model, sysm))
+    nneg = int(np.sum(np.linalg.eigvalsh(A) < 0))
+    assert nneg == int(g["mspar_out"][25]) > 0
+    ms = g["mspar_in"].copy()
+    sm = g["sm_in"].copy()
+    ierr, _ = oracle.sprsol(1, ms, g["mtrees"], g["msifa"], sm)
+    assert ierr == 3 and ms[25] == nneg
+
+
+def _multiplicity(model, sysm):
+    m = np.zeros(sysm.neq)
+    for e in range(model.nel):
+        eq = sysm.meqn[model.element_dofs(e)]
+        eq = eq[eq > 0] - 1
+        m[eq] += 1
+    return m
+
+
+@pytest.mark.parametrize("kind,n", [("hex8", 4), ("quad4", 10), ("frame", 50)])
+def test_manufactured_solution(kind, n):
+    kw = {"seed": 2} if kind == "frame" else {}
+    model, sysm, sm = make_system(kind, n, **kw)
+    A = dense_equation_matrix(model, sysm)
+    xs = np.random.default_rng(3).normal(size=(sysm.neq, 2))
+    b = np.asfortranarray(A @ xs)
+    ms = sysm.mspar.copy()
+    ierr, _ = oracle.sprsol(3, ms, sysm.mtrees, sysm.msifa, sm, b)
+    assert ierr == 0
+    cond = np.linalg.cond(A)
+    assert np.linalg.norm(b - xs) / np.linalg.norm(xs) < 50 * cond * np.finfo(float).eps
+
+
+def test_sprprm_equals_dense_product():
+    model, sysm, sm = make_system("hex8", 4, e_sigma=0.5, e_seed=1)
+    A = dense_equation_matrix(model, sysm)
+    x = np.random.default_rng(5).normal(size=(sysm.neq, 3))
+    ierr, c = oracle.sprprm(sm[sysm.neq:], x, sysm.mspar.copy(), sysm.mtrees, sysm.msifa)
+    assert ierr == 0
+    np.testing.assert_allclose(c, A @ x, rtol=1e-12, atol=1e-12 * np.abs(A).max())
+    ierr, c2 = oracle.sprprm(sm[sysm.neq:], x, sysm.mspar.copy(), sysm.mtrees, sysm.msifa,
+                             iflag=-1)
+    np.testing.assert_array_equal(c2, -c)
+
+
+def test_planted_mistake_is_caught():
+    """A dropped Schur term in the factor must break the dense-LDL' check."""
+    import refspr
+
+    model = _single_front_model(12, np.random.default_rng(2))
+    sysm = refspr.symbolic(model)
+    sm = refspr.assemble(model, sysm)
+    A = dense_equation_matrix(model, sysm)
+    ms = sysm.mspar.copy()
+    oracle.sprsol(1, ms, sysm.mtrees, sysm.msifa, sm)
+    d = sm[:sysm.neq].copy()
+    d_wrong = d.copy()
+    d_wrong[-1] += 1e-6 * abs(d_wrong[-1])  # what a dropped rank-1 term would leave
+    C = np.linalg.cholesky(A)
+    assert np.allclose(d, np.diag(C) ** 2, rtol=1e-12)
+    assert not np.allclose(d_wrong, np.diag(C) ** 2, rtol=1e-12)
