@@ -1,0 +1,240 @@
+#!/usr/bin/env python
+"""Factor + solve benchmark: 40^3 hex8 elasticity, FP64 vs FP32 factor + FP64 refinement.
+
+One step = numeric factorization of the assembled matrix (resident in HBM)
+plus the solve of one right-hand side:
+  * chosen form: FP32 factor on MFMA, then solve + FP64 iterative refinement
+    (residual b - A x as an FP64 SpMV) until ||b - A x|| / ||b|| <= 1e-10;
+  * plain form: FP64 factor + FP64 forward/backward solve.
+value = factor flops (the reference's RINFO(3) count) / seconds of one
+chosen-form step, in TFLOP/s.
+
+Set-up outside the timed region, as the host application does it: the mesh,
+the reference SPR symbolic phase and SPRADM assembly (oracle/_ref, the
+reference's own code), the upload of SM and b to HBM, and the residual
+pattern.  The CPU baseline is the reference's SPRFC1 + SPRFS1 + SPRBS1 on the
+20^3 case, on one host core.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, "sam-lib_amd"), os.path.join(ROOT, "oracle")]
+
+METRIC = "factor+solve s & TFLOP/s, 40^3 hex8: FP64 vs FP32+IR at 1e-10 residual"
+PEAK = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense MFMA TFLOP/s
+HBM_PEAK = 8000.0  # GB/s
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_problem(n):
+    import refspr
+    from sprgpu import fem
+
+    model = fem.hex8_grid(n, e_sigma=0.5, e_seed=1)
+    rng = np.random.default_rng(1)
+    rng.normal(0.0, 0.5, model.nel)           # the moduli drawn by hex8_grid
+    f = rng.uniform(-1.0, 1.0, model.ndof)    # then the load vector
+    t = time.perf_counter()
+    sysm = refspr.symbolic(model)
+    sm = refspr.assemble(model, sysm)
+    b = refspr.loads_to_equations(sysm, f)[:, 0].copy()
+    log(f"set-up {n}^3: neq={sysm.neq} nsuper={sysm.nsuper} nzerol={sysm.nzerol} "
+        f"mfront={sysm.mfront} flops={refspr.factor_flops(sysm):.4e} ({time.perf_counter() - t:.1f}s)")
+    return model, sysm, sm, b
+
+
+def cpu_baseline():
+    """Reference SPRFC1/SPRFS1/SPRBS1 on the 20^3 case (E=1, nu=0.3, loads seed 0), 1 core."""
+    import refspr
+    from sprgpu import fem
+
+    model = fem.hex8_grid(20)
+    sysm = refspr.symbolic(model)
+    sm = refspr.assemble(model, sysm)
+    b = refspr.loads_to_equations(sysm, fem.uniform_loads(model.ndof, np.random.default_rng(0)))
+    b = np.asfortranarray(b)
+    ierr, tf, _ = refspr.sol(1, sysm, sm, None)
+    ierr2, ts, _ = refspr.sol(4, sysm, sm, b)
+    assert ierr == 0 and ierr2 == 0
+    flops = refspr.factor_flops(sysm)
+    return {"value": flops / (tf + ts) / 1e12, "unit": "TFLOP/s", "cores": 1,
+            "kind": "reference",
+            "sample": (f"20^3 hex8 (NEQ {sysm.neq}, RINFO(3) {flops:.3e}): full reference "
+                       f"SPRFC1 {tf:.2f}s + SPRFS1/SPRBS1 {ts:.3f}s, flang -O2, one thread"),
+            "factor_s": tf, "solve_s": ts}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=40, help="elements per edge of the hex8 cube")
+    ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--maxit", type=int, default=30)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel timing pass")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path's multi-GPU "
+              "sharding is a design note, not a measured configuration.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import refspr
+    from sprgpu.device import SprFactor
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    model, sysm, sm, b = build_problem(args.n)
+    flops = refspr.factor_flops(sysm)
+    t = time.perf_counter()
+    sm_dev = torch.from_numpy(sm).to(dev)
+    b_dev = torch.from_numpy(b).to(dev).view(1, -1)
+    h64 = SprFactor(sysm.mspar, sysm.mtrees, sysm.msifa, "fp64")
+    h32 = SprFactor(sysm.mspar, sysm.mtrees, sysm.msifa, "fp32")
+    h64.set_pattern(sm)
+    h32.set_pattern(sm)
+    torch.cuda.synchronize()
+    log(f"device set-up {time.perf_counter() - t:.1f}s; device bytes fp64 {h64.device_bytes/1e9:.1f} GB "
+        f"fp32 {h32.device_bytes/1e9:.1f} GB")
+
+    x32 = torch.empty_like(b_dev)
+    x64 = torch.empty_like(b_dev)
+    tol = (0.0, 0.0, 0.0, 0.0)
+
+    def step32():
+        h32.factor(sm_dev, tol, info=False)
+        return h32.solve_ir(b_dev, x32, rtol=args.rtol, maxit=args.maxit)
+
+    def step64():
+        h64.factor(sm_dev, tol, info=False)
+        x64.copy_(b_dev)
+        h64.solve(x64)
+
+    def timed(fn, k):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = None
+        for _ in range(k):
+            out = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / k, out
+
+    for _ in range(args.warmup):
+        step64()
+        step32()
+    torch.cuda.synchronize()
+    log("warm-up done")
+    t32, ir = timed(step32, args.steps)
+    t64, _ = timed(step64, args.steps)
+    log(f"timed: fp32+IR {t32*1e3:.1f} ms/step ({ir}), fp64 {t64*1e3:.1f} ms/step")
+    # split of each form into factor and solve (separately timed, outside the headline)
+    tf32, _ = timed(lambda: h32.factor(sm_dev, tol, info=False), args.steps)
+    tf64, _ = timed(lambda: h64.factor(sm_dev, tol, info=False), args.steps)
+    info64 = h64.factor_info()
+    info32 = h32.factor_info()
+    assert info64["ierr"] == 0 and info32["ierr"] == 0, (info64, info32)
+
+    # accuracy
+    def relres(x):
+        r = torch.empty_like(b_dev)
+        h64.spmv(x, r)
+        return float(torch.linalg.norm(b_dev - r) / torch.linalg.norm(b_dev))
+
+    x64.copy_(b_dev)
+    h64.solve(x64)
+    rr64 = relres(x64)
+    ir = h32.solve_ir(b_dev, x32, rtol=args.rtol, maxit=args.maxit)
+    rr32 = relres(x32)
+    err = {}
+    ref_path = os.path.join(ROOT, "tests", "golden", f"hex8_{args.n}_reference_solution.npy")
+    if os.path.exists(ref_path):
+        xr = np.load(ref_path, allow_pickle=False)
+        for name, x in (("fp64", x64), ("fp32_ir", x32)):
+            xx = x.view(-1).cpu().numpy()
+            err[name] = float(np.linalg.norm(xx - xr) / np.linalg.norm(xr))
+    xd = (x32 - x64).view(-1)
+    err_32_vs_64 = float(torch.linalg.norm(xd) / torch.linalg.norm(x64))
+
+    result = {
+        "metric": METRIC,
+        "value": flops / t32 / 1e12,
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t32 * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 factor, f64 residual/update",
+        "data": "synthetic",
+        "config": {"workload": (f"hex8 {args.n}^3 linear elasticity, E log-normal(0,0.5) seed 1, "
+                                f"nu 0.3, z=0 clamped, 1 RHS U[-1,1]"),
+                   "neq": sysm.neq, "nsuper": sysm.nsuper, "nzerol": sysm.nzerol,
+                   "factor_flops": flops, "rtol": args.rtol},
+        "fp32_ir": {"step_s": t32, "factor_s": tf32, "solve_ir_s": t32 - tf32,
+                    "iterations": int(ir["iterations"]), "relres": rr32,
+                    "tflops_factor": flops / tf32 / 1e12,
+                    "rel_err_vs_reference": err.get("fp32_ir"),
+                    "rel_diff_vs_fp64_gpu": err_32_vs_64},
+        "fp64": {"step_s": t64, "factor_s": tf64, "solve_s": t64 - tf64, "relres": rr64,
+                 "tflops": flops / t64 / 1e12, "tflops_factor": flops / tf64 / 1e12,
+                 "rel_err_vs_reference": err.get("fp64")},
+        "speedup_fp32ir_vs_fp64": t64 / t32,
+        "factor_info": {"numneg": info64["numneg"], "tol2": info64["tol2"],
+                        "tol3": info64["tol3"]},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+    if not args.no_profile:
+        import ctypes as C
+        from sprgpu import _lib
+
+        out = np.zeros(21)
+        tolv = np.zeros(4)
+        classes = ["init", "extend_add", "small_lds", "panel_diag", "panel_rows", "mfma_update",
+                   "other"]
+        per = {}
+        for name, hh in (("fp32", h32), ("fp64", h64)):
+            _lib.check(_lib.lib().sprgpu_profile_factor(
+                hh._h, C.c_void_p(sm_dev.data_ptr()), tolv.ctypes.data_as(C.c_void_p), 1,
+                out.ctypes.data_as(C.c_void_p)), "profile")
+            per[name] = {c: {"ms": out[3 * i], "launches": int(out[3 * i + 1]),
+                             "flops": out[3 * i + 2]} for i, c in enumerate(classes)}
+        g = per["fp32"]["mfma_update"]
+        ach = g["flops"] / (g["ms"] * 1e-3) / 1e12
+        result["roofline"] = {"bound": "mfma", "achieved": ach, "peak": PEAK["fp32"],
+                              "unit": "TFLOP/s", "frac": ach / PEAK["fp32"], "traffic": None,
+                              "kernel": "k_gemm<float> (all launches of one factorization)",
+                              "avg_launch_ms": g["ms"] / max(g["launches"], 1)}
+        g64 = per["fp64"]["mfma_update"]
+        result["roofline_fp64"] = {"bound": "mfma",
+                                   "achieved": g64["flops"] / (g64["ms"] * 1e-3) / 1e12,
+                                   "peak": PEAK["fp64"], "unit": "TFLOP/s"}
+        result["roofline_fp64"]["frac"] = result["roofline_fp64"]["achieved"] / PEAK["fp64"]
+        result["kernel_classes_ms"] = {k: {c: round(v[c]["ms"], 3) for c in classes}
+                                       for k, v in per.items()}
+        log(f"profile: {result['kernel_classes_ms']}")
+
+    if not args.no_cpu:
+        log("cpu baseline (reference, 20^3, one core) ...")
+        result["cpu_baseline"] = cpu_baseline()
+        log(f"cpu baseline {result['cpu_baseline']['value']*1e3:.2f} GFLOP/s")
+
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

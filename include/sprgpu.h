@@ -145,6 +145,13 @@ sprgpu_int sprgpu_nzerol(const sprgpu_handle *handle);
 /* Bytes of device memory held by the handle. */
 sprgpu_int sprgpu_device_bytes(const sprgpu_handle *handle);
 
+/* sprgpu_factor with a HIP event after every kernel (slower; for
+ * measurement).  out[3*c+0..2] = milliseconds, kernel launches, algorithmic
+ * flops for kernel class c: 0 front init, 1 extend-add, 2 LDS small fronts,
+ * 3 diagonal blocks, 4 panel rows, 5 MFMA updates, 6 other. */
+int sprgpu_profile_factor(sprgpu_handle *handle, const double *sm_dev, const double *tol,
+                          int mode, double *out);
+
 /* Host-only: builds the device schedule for a symbolic structure without
  * touching a GPU and reports it, for checking and sizing.  out[0..9] =
  * NEQ, NSUPER, tree heights, kernel launches per factorization, front-pool

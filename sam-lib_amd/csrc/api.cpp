@@ -39,7 +39,9 @@ static void free_all(sprgpu_handle *h)
                     d.ea_tasks, d.small_nodes, d.panel_desc, d.panel_prefix, d.gemm_desc,
                     d.gemm_prefix, d.pool,     d.dvec,       d.adiag,      d.dcor,
                     d.status,   d.stats,      d.vwork,       d.csr_rowptr, d.csr_col,
-                    d.csr_src,  d.csr_val,    d.irwork,      d.level_nodes};
+                    d.csr_src,  d.csr_val,    d.irwork,      d.level_nodes,
+                    d.solve_small, d.solve_big, d.fwd_desc,  d.bwd_desc,   d.fwd_prefix,
+                    d.bwd_prefix, d.partial};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     d = DeviceBuffers{};
@@ -84,7 +86,12 @@ int sprgpu_create(const sprgpu_int *mspar, const sprgpu_int *mtrees, const sprgp
         (rc = upload(d.panel_prefix, p.panel_prefix, B)) ||
         (rc = upload(d.gemm_desc, p.gemm_desc, B)) ||
         (rc = upload(d.gemm_prefix, p.gemm_prefix, B)) ||
-        (rc = upload(d.level_nodes, p.level_nodes, B))) {
+        (rc = upload(d.level_nodes, p.level_nodes, B)) ||
+        (rc = upload(d.solve_small, p.solve_small, B)) ||
+        (rc = upload(d.solve_big, p.solve_big, B)) || (rc = upload(d.fwd_desc, p.fwd_desc, B)) ||
+        (rc = upload(d.bwd_desc, p.bwd_desc, B)) ||
+        (rc = upload(d.fwd_prefix, p.fwd_prefix, B)) ||
+        (rc = upload(d.bwd_prefix, p.bwd_prefix, B))) {
         free_all(h.get());
         return rc;
     }
@@ -146,6 +153,14 @@ int sprgpu_factor(sprgpu_handle *h, const double *sm_dev, const double *tol, int
     if (rc) return rc;
     if (info) return read_factor_info(h, info);
     return 0;
+}
+
+int sprgpu_profile_factor(sprgpu_handle *h, const double *sm_dev, const double *tol, int mode,
+                          double *out)
+{
+    if (!h || !sm_dev || !tol || !out || (mode != 1 && mode != 3)) return -2;
+    double t4[4] = {tol[0], 0.0, 0.0, mode == 3 ? tol[3] : 0.0};
+    return profile_factor(h, sm_dev, mode, t4, out);
 }
 
 int sprgpu_factor_info_get(sprgpu_handle *h, sprgpu_factor_info *info)

@@ -30,6 +30,11 @@ namespace sprgpu {
 typedef double dv4 __attribute__((ext_vector_type(4)));
 typedef float fv4 __attribute__((ext_vector_type(4)));
 
+// 16x16x4 MFMA wrappers.  Both take A[i][k] from lane i + 16k and B[k][j]
+// from lane j + 16k.  The 4 accumulator elements of a lane hold column
+// j = lane & 15 and rows that differ between the two instructions (measured
+// on gfx950): f32 element e is row 4*(lane>>4) + e, f64 element e is row
+// (lane>>4) + 4*e.
 template <typename T> struct Mfma;
 template <> struct Mfma<double> {
     typedef dv4 acc_t;
@@ -37,6 +42,7 @@ template <> struct Mfma<double> {
     {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lane, int e) { return (lane >> 4) + 4 * e; }
 };
 template <> struct Mfma<float> {
     typedef fv4 acc_t;
@@ -44,6 +50,7 @@ template <> struct Mfma<float> {
     {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lane, int e) { return 4 * (lane >> 4) + e; }
 };
 
 // ---------------------------------------------------------------------------
@@ -416,7 +423,7 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc *__restrict__ descs
         for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
             for (int ii = 0; ii < 4; ++ii) {
-                const int row = i0 + wr * 32 + bi * 16 + 4 * (lane >> 4) + ii;
+                const int row = i0 + wr * 32 + bi * 16 + Mfma<T>::row(lane, ii);
                 const int col = j0 + wc * 32 + bj * 16 + (lane & 15);
                 if (row < g.rend && col < g.cend && row >= col) fr[row + col * ld] -= acc[bi][bj][ii];
             }
@@ -527,8 +534,16 @@ __global__ void k_copy_diag(const double *dvec, const double *adiag, double *sm,
 // ---------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------
+// Kernel classes for the per-class timing of sprgpu_profile_factor.
+enum ProfClass { P_INIT = 0, P_EA, P_SMALL, P_PDIAG, P_PROWS, P_GEMM, P_OTHER, P_NCLASS };
+
+// Launches the whole factorization.  With ev != nullptr an event is recorded
+// after every kernel and the kernel's class appended to cls, so consecutive
+// events bracket one kernel each (the stream serialises them).
 template <typename T>
-static int factor_impl(sprgpu_handle *h, const double *sm_dev, int mode, const double *tol)
+static int factor_impl(sprgpu_handle *h, const double *sm_dev, int mode, const double *tol,
+                       std::vector<hipEvent_t> *ev = nullptr, std::vector<int> *cls = nullptr,
+                       std::vector<double> *fl = nullptr)
 {
     const Plan &p = h->plan;
     DeviceBuffers &d = h->d;
@@ -536,30 +551,47 @@ static int factor_impl(sprgpu_handle *h, const double *sm_dev, int mode, const d
     T *pool = static_cast<T *>(d.pool);
     const double *lnz = sm_dev + p.neq;
     PivotArgs pa{tol[0], tol[3], mode};
+    auto mark = [&](int c, double flops) {
+        if (!ev) return;
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        (void)hipEventRecord(e, s);
+        ev->push_back(e);
+        cls->push_back(c);
+        fl->push_back(flops);
+    };
+    mark(P_OTHER, 0.0);
     k_status_reset<<<1, 64, 0, s>>>(d.status);
+    mark(P_OTHER, 0.0);
     for (const Launch &L : p.launches) {
         const unsigned blocks = unsigned(L.blocks);
         switch (L.kind) {
         case L_INIT:
             k_front_init<T><<<blocks, 256, 0, s>>>(d.init_tasks + L.off, d.nodes, lnz, pool,
                                                    d.adiag, d.dcor, d.status, mode);
+            mark(P_INIT, 0.0);
             break;
         case L_EA:
             k_extend_add<T><<<blocks, 256, 0, s>>>(d.ea_tasks + L.off, d.nodes, d.relmap, pool);
+            mark(P_EA, 0.0);
             break;
         case L_SMALL:
             k_small<T><<<blocks, 256, 0, s>>>(d.small_nodes + L.off, d.nodes, pool, d.dvec,
                                               d.adiag, d.dcor, d.status, pa);
+            mark(P_SMALL, 0.0);
             break;
         case L_PANEL:
             k_pdiag<T><<<unsigned(L.count), 256, 0, s>>>(d.panel_desc + L.off, d.nodes, pool,
                                                          d.dvec, d.adiag, d.dcor, d.status, pa);
+            mark(P_PDIAG, 0.0);
             k_prows<T><<<blocks, 256, 0, s>>>(d.panel_desc + L.off, int(L.count),
                                               d.panel_prefix + L.off, d.nodes, pool);
+            mark(P_PROWS, 0.0);
             break;
         case L_GEMM:
             k_gemm<T><<<blocks, 256, 0, s>>>(d.gemm_desc + L.off, int(L.count),
                                              d.gemm_prefix + L.off, pool);
+            mark(P_GEMM, L.flops);
             break;
         }
     }
@@ -577,6 +609,33 @@ int run_factor(sprgpu_handle *h, const double *sm_dev, int mode, const double *t
     int rc = (h->precision == SPRGPU_FP32) ? factor_impl<float>(h, sm_dev, mode, tol)
                                            : factor_impl<double>(h, sm_dev, mode, tol);
     if (rc == 0) h->factored = true;
+    return rc;
+}
+
+int profile_factor(sprgpu_handle *h, const double *sm_dev, int mode, const double *tol,
+                   double *out)
+{
+    std::vector<hipEvent_t> ev;
+    std::vector<int> cls;
+    std::vector<double> fl;
+    h->mode = mode;
+    h->last_sm_dev = sm_dev;
+    int rc = (h->precision == SPRGPU_FP32) ? factor_impl<float>(h, sm_dev, mode, tol, &ev, &cls, &fl)
+                                           : factor_impl<double>(h, sm_dev, mode, tol, &ev, &cls, &fl);
+    if (rc == 0) h->factored = true;
+    for (int k = 0; k < 3 * P_NCLASS; ++k) out[k] = 0.0;
+    if (!ev.empty()) {
+        SPRGPU_HIP(hipEventSynchronize(ev.back()));
+        for (size_t i = 1; i < ev.size(); ++i) {
+            float ms = 0.f;
+            SPRGPU_HIP(hipEventElapsedTime(&ms, ev[i - 1], ev[i]));
+            const int c = cls[i];
+            out[3 * c + 0] += ms;
+            out[3 * c + 1] += 1.0;
+            out[3 * c + 2] += fl[i];
+        }
+    }
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     return rc;
 }
 

@@ -149,8 +149,17 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, Pl
     p.gemm_prefix.clear();
     p.launches.clear();
 
-    auto add_launch = [&](LaunchKind kind, int level, i64 off, i64 count, i64 blocks) {
-        if (count > 0 && blocks > 0) p.launches.push_back(Launch{kind, level, off, count, blocks});
+    auto add_launch = [&](LaunchKind kind, int level, i64 off, i64 count, i64 blocks,
+                          double flops = 0.0) {
+        if (count > 0 && blocks > 0)
+            p.launches.push_back(Launch{kind, level, off, count, blocks, flops});
+    };
+    // useful flops of a GEMM descriptor: 2*kn per entry (i,j) of the lower
+    // trapezoid rows [r0,rend) x cols [r0,cend), i >= j
+    auto gemm_flops = [](const GemmDesc &g) {
+        const double nc = double(g.cend - g.r0);
+        const double pairs = nc * g.rend - (double(g.r0) + g.cend - 1) * nc / 2.0;
+        return 2.0 * g.kn * pairs;
     };
 
     for (int l = 0; l < p.nlevels; ++l) {
@@ -209,35 +218,67 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, Pl
                 nblk += nrb;
             }
             add_launch(L_PANEL, l, off, i64(p.panel_desc.size()) - off, nblk);
-            // trailing update restricted to the remaining panel columns
+            // Two-level blocking of the panel: the rank-nb update of a step
+            // stops at the end of its kPanelOuter-wide outer block; the
+            // columns beyond get one rank-kPanelOuter update per outer block.
+            const i32 k0 = s * kPanelNB;
+            const i32 oend = (k0 / kPanelOuter + 1) * kPanelOuter;
             off = i64(p.gemm_desc.size());
             i64 ntiles = 0;
+            double fl = 0.0;
             for (i32 j : big) {
                 const NodeInfo &n = p.nodes[j];
-                const i32 k0 = s * kPanelNB;
                 if (k0 >= n.S) continue;
                 const i32 nb = std::min(kPanelNB, n.S - k0);
                 const i32 r0 = k0 + nb;
-                if (r0 >= n.S) continue;
+                const i32 cend = std::min(n.S, oend);
+                if (r0 >= cend) continue;
                 GemmDesc g;
                 g.foff = n.foff;
                 g.ld = n.ld;
                 g.r0 = r0;
                 g.rend = n.F;
-                g.cend = n.S;
+                g.cend = cend;
                 g.k0 = k0;
                 g.kn = nb;
                 g.nti = (n.F - r0 + kTile - 1) / kTile;
-                g.ntj = (n.S - r0 + kTile - 1) / kTile;
+                g.ntj = (cend - r0 + kTile - 1) / kTile;
                 p.gemm_desc.push_back(g);
                 p.gemm_prefix.push_back(ntiles);
                 ntiles += tri_tiles(g.nti, g.ntj);
+                fl += gemm_flops(g);
             }
-            add_launch(L_GEMM, l, off, i64(p.gemm_desc.size()) - off, ntiles);
+            add_launch(L_GEMM, l, off, i64(p.gemm_desc.size()) - off, ntiles, fl);
+            if (k0 + kPanelNB == oend) {
+                const i32 ob = oend - kPanelOuter;
+                off = i64(p.gemm_desc.size());
+                ntiles = 0;
+                fl = 0.0;
+                for (i32 j : big) {
+                    const NodeInfo &n = p.nodes[j];
+                    if (oend >= n.S) continue;
+                    GemmDesc g;
+                    g.foff = n.foff;
+                    g.ld = n.ld;
+                    g.r0 = oend;
+                    g.rend = n.F;
+                    g.cend = n.S;
+                    g.k0 = ob;
+                    g.kn = kPanelOuter;
+                    g.nti = (n.F - oend + kTile - 1) / kTile;
+                    g.ntj = (n.S - oend + kTile - 1) / kTile;
+                    p.gemm_desc.push_back(g);
+                    p.gemm_prefix.push_back(ntiles);
+                    ntiles += tri_tiles(g.nti, g.ntj);
+                    fl += gemm_flops(g);
+                }
+                add_launch(L_GEMM, l, off, i64(p.gemm_desc.size()) - off, ntiles, fl);
+            }
         }
         // Schur complement of every blocked front of this level
         off = i64(p.gemm_desc.size());
         i64 ntiles = 0;
+        double fl = 0.0;
         for (i32 j : big) {
             const NodeInfo &n = p.nodes[j];
             if (n.F == n.S) continue;
@@ -254,8 +295,99 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, Pl
             p.gemm_desc.push_back(g);
             p.gemm_prefix.push_back(ntiles);
             ntiles += tri_tiles(g.nti, g.ntj);
+            fl += gemm_flops(g);
         }
-        add_launch(L_GEMM, l, off, i64(p.gemm_desc.size()) - off, ntiles);
+        add_launch(L_GEMM, l, off, i64(p.gemm_desc.size()) - off, ntiles, fl);
+    }
+
+    // ----------------------------------------------------------------------
+    // solve schedule
+    // ----------------------------------------------------------------------
+    p.solve_small.clear();
+    p.solve_big.clear();
+    p.solve_small_ptr.assign(p.nlevels + 1, 0);
+    p.solve_big_ptr.assign(p.nlevels + 1, 0);
+    p.fwd_desc.clear();
+    p.bwd_desc.clear();
+    p.fwd_prefix.clear();
+    p.bwd_prefix.clear();
+    p.fwd_steps.clear();
+    p.bwd_steps.clear();
+    p.max_partial_chunks = 0;
+    for (int l = 0; l < p.nlevels; ++l) {
+        for (i32 q = p.level_ptr[l]; q < p.level_ptr[l + 1]; ++q) {
+            const i32 j = p.level_nodes[q];
+            if (p.nodes[j].S > kSolveBigS)
+                p.solve_big.push_back(j);
+            else
+                p.solve_small.push_back(j);
+        }
+        p.solve_small_ptr[l + 1] = i32(p.solve_small.size());
+        p.solve_big_ptr[l + 1] = i32(p.solve_big.size());
+    }
+    // forward: for block b of each wide supernode, solve the diagonal block,
+    // then update every row below it
+    for (int l = 0; l < p.nlevels; ++l) {
+        const i32 b0 = p.solve_big_ptr[l], b1 = p.solve_big_ptr[l + 1];
+        if (b0 == b1) continue;
+        p.fwd_steps.push_back(SolveStep{0, l, b0, b1 - b0, b1 - b0});
+        i32 maxblk = 0;
+        for (i32 q = b0; q < b1; ++q)
+            maxblk = std::max(maxblk, (p.nodes[p.solve_big[q]].S + kSolveNB - 1) / kSolveNB);
+        for (i32 s = 0; s < maxblk; ++s) {
+            const i64 off = i64(p.fwd_desc.size());
+            i64 chunks = 0;
+            for (i32 q = b0; q < b1; ++q) {
+                const NodeInfo &n = p.nodes[p.solve_big[q]];
+                const i32 c0 = s * kSolveNB;
+                if (c0 >= n.S) continue;
+                const i32 nb = std::min(kSolveNB, n.S - c0);
+                const i32 rows = n.F - (c0 + nb);
+                const i32 nch = (rows + kSolveRows - 1) / kSolveRows;
+                p.fwd_desc.push_back(SolveDesc{p.solve_big[q], c0, nb, nch});
+                p.fwd_prefix.push_back(chunks);
+                chunks += nch;
+            }
+            const i64 cnt = i64(p.fwd_desc.size()) - off;
+            p.fwd_steps.push_back(SolveStep{1, l, off, cnt, cnt});
+            if (chunks > 0) p.fwd_steps.push_back(SolveStep{2, l, off, cnt, chunks});
+        }
+        p.fwd_steps.push_back(SolveStep{3, l, b0, b1 - b0, b1 - b0});
+    }
+    // backward: blocks from the last to the first; partial sums of L^T x over
+    // the rows below the block, then the reduction and the diagonal block
+    for (int l = p.nlevels - 1; l >= 0; --l) {
+        const i32 b0 = p.solve_big_ptr[l], b1 = p.solve_big_ptr[l + 1];
+        if (b0 == b1) continue;
+        p.bwd_steps.push_back(SolveStep{0, l, b0, b1 - b0, b1 - b0});
+        i32 maxblk = 0;
+        for (i32 q = b0; q < b1; ++q)
+            maxblk = std::max(maxblk, (p.nodes[p.solve_big[q]].S + kSolveNB - 1) / kSolveNB);
+        for (i32 s = maxblk - 1; s >= 0; --s) {
+            const i64 off = i64(p.bwd_desc.size());
+            i64 chunks = 0;
+            for (i32 q = b0; q < b1; ++q) {
+                const NodeInfo &n = p.nodes[p.solve_big[q]];
+                const i32 nblk = (n.S + kSolveNB - 1) / kSolveNB;
+                // align the last blocks of all nodes so each node runs its
+                // blocks from last to first
+                const i32 bi = s - (maxblk - nblk);
+                if (bi < 0) continue;
+                const i32 c0 = bi * kSolveNB;
+                const i32 nb = std::min(kSolveNB, n.S - c0);
+                const i32 rows = n.F - (c0 + nb);
+                const i32 nch = (rows + kSolveRows - 1) / kSolveRows;
+                p.bwd_desc.push_back(SolveDesc{p.solve_big[q], c0, nb, nch});
+                p.bwd_prefix.push_back(chunks);
+                chunks += nch;
+            }
+            const i64 cnt = i64(p.bwd_desc.size()) - off;
+            if (cnt == 0) continue;
+            if (chunks > 0) p.bwd_steps.push_back(SolveStep{2, l, off, cnt, chunks});
+            p.bwd_steps.push_back(SolveStep{1, l, off, cnt, cnt});
+            p.max_partial_chunks = std::max(p.max_partial_chunks, chunks);
+        }
+        p.bwd_steps.push_back(SolveStep{3, l, b0, b1 - b0, b1 - b0});
     }
     return "";
 }

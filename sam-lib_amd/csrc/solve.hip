@@ -105,6 +105,209 @@ __global__ __launch_bounds__(256) void k_bwd(const i32 *__restrict__ lnodes,
 }
 
 // ---------------------------------------------------------------------------
+// wide supernodes (S > kSolveBigS): blocked forward and backward sweeps
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int find_sdesc(const i64 *prefix, int n, i64 b)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// v = [B(rows 0..S-1); 0] + sum_children extend(u_child)
+__global__ __launch_bounds__(256) void k_fwd_assemble(const i32 *__restrict__ big,
+                                                      const NodeInfo *__restrict__ nodes,
+                                                      const i32 *__restrict__ children,
+                                                      const i32 *__restrict__ lindx,
+                                                      const i32 *__restrict__ relmap,
+                                                      const double *b, i64 ldb, double *vwork,
+                                                      i64 vstride)
+{
+    const NodeInfo n = nodes[big[blockIdx.x]];
+    const int rhs = blockIdx.y, tid = threadIdx.x;
+    double *v = vwork + rhs * vstride + n.voff;
+    const double *bb = b + rhs * ldb;
+    const i32 *rows = lindx + n.lxoff;
+    for (int i = tid; i < n.F; i += 256) v[i] = (i < n.S) ? bb[rows[i]] : 0.0;
+    __syncthreads();
+    for (int ci = 0; ci < n.nchild; ++ci) {
+        const NodeInfo c = nodes[children[n.child0 + ci]];
+        const double *vc = vwork + rhs * vstride + c.voff + c.S;
+        const i32 *map = relmap + c.mapoff;
+        for (int i = tid; i < c.F - c.S; i += 256) v[map[i]] += vc[i];
+        __syncthreads();
+    }
+}
+
+// x_block = L(block,block)^-1 v_block (unit lower), in place in v
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_diag(const SolveDesc *__restrict__ desc,
+                                                  const NodeInfo *__restrict__ nodes,
+                                                  const T *__restrict__ pool, double *vwork,
+                                                  i64 vstride)
+{
+    __shared__ double x[kSolveNB];
+    __shared__ T Lb[kSolveNB][kSolveNB + 1];
+    const SolveDesc sd = desc[blockIdx.x];
+    const NodeInfo n = nodes[sd.node];
+    const int rhs = blockIdx.y, tid = threadIdx.x, nb = sd.nb, c0 = sd.c0;
+    double *v = vwork + rhs * vstride + n.voff;
+    const T *L = pool + n.foff;
+    const i64 ld = n.ld;
+    if (tid < nb) x[tid] = v[c0 + tid];
+    for (int idx = tid; idx < nb * nb; idx += 256) {
+        const int r = idx % nb, c = idx / nb;
+        if (r > c) Lb[r][c] = L[(c0 + r) + (c0 + c) * ld];
+    }
+    __syncthreads();
+    for (int c = 0; c < nb - 1; ++c) {
+        const double xc = x[c];
+        if (tid > c && tid < nb) x[tid] -= double(Lb[tid][c]) * xc;
+        __syncthreads();
+    }
+    if (tid < nb) v[c0 + tid] = x[tid];
+}
+
+// v(rows below the block) -= L(rows, block) x_block
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_upd(const SolveDesc *__restrict__ desc, int ndesc,
+                                                 const i64 *__restrict__ prefix,
+                                                 const NodeInfo *__restrict__ nodes,
+                                                 const T *__restrict__ pool, double *vwork,
+                                                 i64 vstride)
+{
+    __shared__ double x[kSolveNB];
+    const int q = find_sdesc(prefix, ndesc, blockIdx.x);
+    const SolveDesc sd = desc[q];
+    const int chunk = int(blockIdx.x - prefix[q]);
+    const NodeInfo n = nodes[sd.node];
+    const int rhs = blockIdx.y, tid = threadIdx.x, nb = sd.nb, c0 = sd.c0;
+    double *v = vwork + rhs * vstride + n.voff;
+    const T *L = pool + n.foff;
+    const i64 ld = n.ld;
+    if (tid < nb) x[tid] = v[c0 + tid];
+    __syncthreads();
+    const int r = c0 + nb + chunk * kSolveRows + tid;
+    if (r < n.F) {
+        double s = v[r];
+        for (int c = 0; c < nb; ++c) s -= double(L[r + (c0 + c) * ld]) * x[c];
+        v[r] = s;
+    }
+}
+
+// B(rows 0..S-1) = x / d
+__global__ __launch_bounds__(256) void k_fwd_finish(const i32 *__restrict__ big,
+                                                    const NodeInfo *__restrict__ nodes,
+                                                    const i32 *__restrict__ lindx,
+                                                    const double *__restrict__ dvec, double *b,
+                                                    i64 ldb, const double *vwork, i64 vstride)
+{
+    const NodeInfo n = nodes[big[blockIdx.x]];
+    const int rhs = blockIdx.y;
+    const double *v = vwork + rhs * vstride + n.voff;
+    double *bb = b + rhs * ldb;
+    const i32 *rows = lindx + n.lxoff;
+    for (int c = threadIdx.x; c < n.S; c += 256) bb[rows[c]] = v[c] / dvec[n.first + c];
+}
+
+__global__ __launch_bounds__(256) void k_bwd_assemble(const i32 *__restrict__ big,
+                                                      const NodeInfo *__restrict__ nodes,
+                                                      const i32 *__restrict__ lindx,
+                                                      const double *b, i64 ldb, double *vwork,
+                                                      i64 vstride)
+{
+    const NodeInfo n = nodes[big[blockIdx.x]];
+    const int rhs = blockIdx.y;
+    double *v = vwork + rhs * vstride + n.voff;
+    const double *bb = b + rhs * ldb;
+    const i32 *rows = lindx + n.lxoff;
+    for (int i = threadIdx.x; i < n.F; i += 256) v[i] = bb[rows[i]];
+}
+
+// partial[c] = sum over one chunk of rows below the block of L(r, c) v(r)
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_partial(const SolveDesc *__restrict__ desc,
+                                                     int ndesc, const i64 *__restrict__ prefix,
+                                                     const NodeInfo *__restrict__ nodes,
+                                                     const T *__restrict__ pool,
+                                                     const double *vwork, i64 vstride,
+                                                     double *partial, i64 pstride)
+{
+    const int q = find_sdesc(prefix, ndesc, blockIdx.x);
+    const SolveDesc sd = desc[q];
+    const int chunk = int(blockIdx.x - prefix[q]);
+    const NodeInfo n = nodes[sd.node];
+    const int rhs = blockIdx.y, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const double *v = vwork + rhs * vstride + n.voff;
+    const T *L = pool + n.foff;
+    const i64 ld = n.ld;
+    const int r0 = sd.c0 + sd.nb + chunk * kSolveRows;
+    const int r1 = min(n.F, r0 + kSolveRows);
+    double *out = partial + rhs * pstride + (prefix[q] + chunk) * kSolveNB;
+    for (int c = wave; c < sd.nb; c += 4) {
+        double s = 0.0;
+        for (int r = r0 + lane; r < r1; r += 64) s += double(L[r + (sd.c0 + c) * ld]) * v[r];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o);
+        if (lane == 0) out[c] = s;
+    }
+}
+
+// x_block = L(block,block)^-T (v_block - sum of partials), unit upper
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_diag(const SolveDesc *__restrict__ desc,
+                                                  const i64 *__restrict__ prefix,
+                                                  const NodeInfo *__restrict__ nodes,
+                                                  const T *__restrict__ pool, double *vwork,
+                                                  i64 vstride, const double *partial,
+                                                  i64 pstride)
+{
+    __shared__ double x[kSolveNB];
+    __shared__ T Lb[kSolveNB][kSolveNB + 1];
+    const SolveDesc sd = desc[blockIdx.x];
+    const NodeInfo n = nodes[sd.node];
+    const int rhs = blockIdx.y, tid = threadIdx.x, nb = sd.nb, c0 = sd.c0;
+    double *v = vwork + rhs * vstride + n.voff;
+    const T *L = pool + n.foff;
+    const i64 ld = n.ld;
+    if (tid < nb) {
+        double s = v[c0 + tid];
+        const double *pp = partial + rhs * pstride + prefix[blockIdx.x] * kSolveNB + tid;
+        for (int k = 0; k < sd.nchunk; ++k) s -= pp[k * kSolveNB];
+        x[tid] = s;
+    }
+    for (int idx = tid; idx < nb * nb; idx += 256) {
+        const int r = idx % nb, c = idx / nb;
+        if (r > c) Lb[r][c] = L[(c0 + r) + (c0 + c) * ld];
+    }
+    __syncthreads();
+    for (int r = nb - 1; r > 0; --r) {
+        const double xr = x[r];
+        if (tid < r) x[tid] -= double(Lb[r][tid]) * xr;
+        __syncthreads();
+    }
+    if (tid < nb) v[c0 + tid] = x[tid];
+}
+
+__global__ __launch_bounds__(256) void k_bwd_finish(const i32 *__restrict__ big,
+                                                    const NodeInfo *__restrict__ nodes,
+                                                    const i32 *__restrict__ lindx, double *b,
+                                                    i64 ldb, const double *vwork, i64 vstride)
+{
+    const NodeInfo n = nodes[big[blockIdx.x]];
+    const int rhs = blockIdx.y;
+    const double *v = vwork + rhs * vstride + n.voff;
+    double *bb = b + rhs * ldb;
+    const i32 *rows = lindx + n.lxoff;
+    for (int c = threadIdx.x; c < n.S; c += 256) bb[rows[c]] = v[c];
+}
+
+// ---------------------------------------------------------------------------
 // FP64 residual: y = alpha*b + beta*A x (CSR, 16 lanes per row)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_spmv(const i64 *__restrict__ rowptr,
@@ -218,21 +421,79 @@ static int solve_impl(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which)
     if (int rc = ensure_vwork(h, nrhs)) return rc;
     const T *pool = static_cast<const T *>(d.pool);
     const i64 vstride = p.vwork_elems;
-    const i32 *lnodes = d.level_nodes;
+    const unsigned ur = unsigned(nrhs);
+    const i64 pstride = std::max<i64>(p.max_partial_chunks, 1) * kSolveNB;
+    if ((which & 2) && p.max_partial_chunks > 0 && d.partial_cap < pstride * nrhs) {
+        if (d.partial) SPRGPU_HIP(hipFree(d.partial));
+        d.partial = nullptr;
+        SPRGPU_HIP(hipMalloc(&d.partial, sizeof(double) * pstride * nrhs));
+        d.partial_cap = pstride * nrhs;
+    }
     if (which & 1) {
+        size_t si = 0;
         for (int l = 0; l < p.nlevels; ++l) {
-            const unsigned nb = unsigned(p.level_ptr[l + 1] - p.level_ptr[l]);
-            dim3 grid(nb, unsigned(nrhs));
-            k_fwd<T><<<grid, 256, 0, s>>>(lnodes + p.level_ptr[l], d.nodes, d.children, d.lindx,
-                                           d.relmap, pool, d.dvec, b, ldb, d.vwork, vstride);
+            const unsigned ns = unsigned(p.solve_small_ptr[l + 1] - p.solve_small_ptr[l]);
+            if (ns)
+                k_fwd<T><<<dim3(ns, ur), 256, 0, s>>>(d.solve_small + p.solve_small_ptr[l],
+                                                      d.nodes, d.children, d.lindx, d.relmap,
+                                                      pool, d.dvec, b, ldb, d.vwork, vstride);
+            for (; si < p.fwd_steps.size() && p.fwd_steps[si].level == l; ++si) {
+                const SolveStep &st = p.fwd_steps[si];
+                const dim3 g(unsigned(st.blocks), ur);
+                switch (st.kind) {
+                case 0:
+                    k_fwd_assemble<<<g, 256, 0, s>>>(d.solve_big + st.off, d.nodes, d.children,
+                                                     d.lindx, d.relmap, b, ldb, d.vwork, vstride);
+                    break;
+                case 1:
+                    k_fwd_diag<T><<<g, 256, 0, s>>>(d.fwd_desc + st.off, d.nodes, pool, d.vwork,
+                                                    vstride);
+                    break;
+                case 2:
+                    k_fwd_upd<T><<<g, 256, 0, s>>>(d.fwd_desc + st.off, int(st.count),
+                                                   d.fwd_prefix + st.off, d.nodes, pool, d.vwork,
+                                                   vstride);
+                    break;
+                case 3:
+                    k_fwd_finish<<<g, 256, 0, s>>>(d.solve_big + st.off, d.nodes, d.lindx,
+                                                   d.dvec, b, ldb, d.vwork, vstride);
+                    break;
+                }
+            }
         }
     }
     if (which & 2) {
+        size_t si = 0;
         for (int l = p.nlevels - 1; l >= 0; --l) {
-            const unsigned nb = unsigned(p.level_ptr[l + 1] - p.level_ptr[l]);
-            dim3 grid(nb, unsigned(nrhs));
-            k_bwd<T><<<grid, 256, 0, s>>>(lnodes + p.level_ptr[l], d.nodes, d.lindx, pool, b,
-                                           ldb, d.vwork, vstride);
+            const unsigned ns = unsigned(p.solve_small_ptr[l + 1] - p.solve_small_ptr[l]);
+            if (ns)
+                k_bwd<T><<<dim3(ns, ur), 256, 0, s>>>(d.solve_small + p.solve_small_ptr[l],
+                                                      d.nodes, d.lindx, pool, b, ldb, d.vwork,
+                                                      vstride);
+            for (; si < p.bwd_steps.size() && p.bwd_steps[si].level == l; ++si) {
+                const SolveStep &st = p.bwd_steps[si];
+                const dim3 g(unsigned(st.blocks), ur);
+                switch (st.kind) {
+                case 0:
+                    k_bwd_assemble<<<g, 256, 0, s>>>(d.solve_big + st.off, d.nodes, d.lindx, b,
+                                                     ldb, d.vwork, vstride);
+                    break;
+                case 1:
+                    k_bwd_diag<T><<<g, 256, 0, s>>>(d.bwd_desc + st.off, d.bwd_prefix + st.off,
+                                                    d.nodes, pool, d.vwork, vstride, d.partial,
+                                                    pstride);
+                    break;
+                case 2:
+                    k_bwd_partial<T><<<g, 256, 0, s>>>(d.bwd_desc + st.off, int(st.count),
+                                                       d.bwd_prefix + st.off, d.nodes, pool,
+                                                       d.vwork, vstride, d.partial, pstride);
+                    break;
+                case 3:
+                    k_bwd_finish<<<g, 256, 0, s>>>(d.solve_big + st.off, d.nodes, d.lindx, b,
+                                                   ldb, d.vwork, vstride);
+                    break;
+                }
+            }
         }
     }
     SPRGPU_HIP(hipGetLastError());

@@ -19,9 +19,25 @@ using i32 = int32_t;
 constexpr int kTile = 64;        // output tile edge of the MFMA update kernel
 constexpr int kPanelNB = 32;     // columns per panel step of the blocked front factor
 constexpr int kPanelRows = 256;  // panel rows per workgroup (one row per thread)
+constexpr int kPanelOuter = 256; // outer block of the two-level panel blocking
 constexpr int kSmallF = 64;      // fronts with F <= kSmallF are factored in LDS
 constexpr int kInitCols = 8;     // front columns per workgroup in the init kernel
 constexpr int kEaCols = 8;       // child columns per workgroup in extend-add
+constexpr int kSolveBigS = 128;  // supernodes wider than this use the blocked solve
+constexpr int kSolveNB = 64;     // columns per block of the blocked solve
+constexpr int kSolveRows = 256;  // rows per workgroup in the blocked solve updates
+
+// One column block of a wide supernode in the blocked solves: columns
+// [c0,c0+nb), and nchunk row chunks of kSolveRows below the block.
+struct SolveDesc { i32 node, c0, nb, nchunk; };
+
+// Launch list of the blocked solves (one list for each direction).
+struct SolveStep {
+    i32 kind;       // 0 assemble, 1 diag, 2 update/partial, 3 finish (per level)
+    i32 level;
+    i64 off, count; // range in solve_desc (kinds 1,2) or in solve_big (kinds 0,3)
+    i64 blocks;
+};
 
 // One supernode as the device sees it.  Rows of the front are the supernode's
 // LINDX list (sorted ascending, the S own columns first); the front is a dense
@@ -65,6 +81,7 @@ struct Launch {
     i32 level;
     i64 off, count;  // range in the task array of this kind
     i64 blocks;      // grid size
+    double flops;    // algorithmic flops of the launch (multiply-add = 2)
 };
 
 struct Plan {
@@ -86,6 +103,13 @@ struct Plan {
     std::vector<i64> gemm_prefix;   // per launch-local prefix of tiles (exclusive)
     std::vector<Launch> launches;
     double factor_flops = 0;        // RINFO(3) formula, sprtrs.f:1961-1969
+    // solve schedule: narrow supernodes one workgroup each, wide ones blocked
+    std::vector<i32> solve_small, solve_big;          // grouped by level
+    std::vector<i32> solve_small_ptr, solve_big_ptr;  // nlevels+1
+    std::vector<SolveDesc> fwd_desc, bwd_desc;
+    std::vector<i64> fwd_prefix, bwd_prefix;          // launch-local chunk prefixes
+    std::vector<SolveStep> fwd_steps, bwd_steps;      // in execution order
+    i64 max_partial_chunks = 0;                       // backward partial-sum scratch
 };
 
 // Builds the plan from the reference symbolic arrays (1-based, INTEGER*8).
@@ -137,6 +161,11 @@ struct DeviceBuffers {
     double *csr_val = nullptr;
     i64 csr_nnz = 0, nnz_lower = 0;
     double *irwork = nullptr;   // IR vectors
+    i32 *solve_small = nullptr, *solve_big = nullptr;
+    SolveDesc *fwd_desc = nullptr, *bwd_desc = nullptr;
+    i64 *fwd_prefix = nullptr, *bwd_prefix = nullptr;
+    double *partial = nullptr;  // backward partial sums (max_partial_chunks*kSolveNB*nrhs)
+    i64 partial_cap = 0;
     i64 irwork_cap = 0;
 };
 
@@ -170,6 +199,8 @@ int run_factor(sprgpu_handle *h, const double *sm_dev, int mode, const double *t
 int read_factor_info(sprgpu_handle *h, sprgpu_factor_info *info);
 int copy_factor_to_sm(sprgpu_handle *h, double *sm_dev_out);
 int load_factor_from_sm(sprgpu_handle *h, const double *sm_dev);
+int profile_factor(sprgpu_handle *h, const double *sm_dev, int mode, const double *tol,
+                   double *out);
 // solve.hip
 int run_solve(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which);
 int run_spmv(sprgpu_handle *h, const double *x, double *y, i64 ld, i64 nrhs);

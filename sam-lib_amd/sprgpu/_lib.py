@@ -52,6 +52,7 @@ SIGNATURES = {
     "sprgpu_nzerol": (I64, [P]),
     "sprgpu_device_bytes": (I64, [P]),
     "sprgpu_plan_summary": (C.c_int, [P, P, P, P]),
+    "sprgpu_profile_factor": (C.c_int, [P, P, P, C.c_int, P]),
     "sprgpu_last_error": (C.c_char_p, []),
 }
 
