@@ -202,10 +202,10 @@ def main():
         import ctypes as C
         from sprgpu import _lib
 
-        out = np.zeros(21)
+        out = np.zeros(24)
         tolv = np.zeros(4)
-        classes = ["init", "extend_add", "small_lds", "panel_diag", "panel_rows", "mfma_update",
-                   "other"]
+        classes = ["init", "extend_add", "small_lds", "diag_writeback", "panel", "mfma_update",
+                   "other", "mfma_inner"]
         per = {}
         for name, hh in (("fp32", h32), ("fp64", h64)):
             _lib.check(_lib.lib().sprgpu_profile_factor(
@@ -213,17 +213,21 @@ def main():
                 out.ctypes.data_as(C.c_void_p)), "profile")
             per[name] = {c: {"ms": out[3 * i], "launches": int(out[3 * i + 1]),
                              "flops": out[3 * i + 2]} for i, c in enumerate(classes)}
-        g = per["fp32"]["mfma_update"]
-        ach = g["flops"] / (g["ms"] * 1e-3) / 1e12
-        result["roofline"] = {"bound": "mfma", "achieved": ach, "peak": PEAK["fp32"],
-                              "unit": "TFLOP/s", "frac": ach / PEAK["fp32"], "traffic": None,
-                              "kernel": "k_gemm<float> (all launches of one factorization)",
-                              "avg_launch_ms": g["ms"] / max(g["launches"], 1)}
-        g64 = per["fp64"]["mfma_update"]
-        result["roofline_fp64"] = {"bound": "mfma",
-                                   "achieved": g64["flops"] / (g64["ms"] * 1e-3) / 1e12,
-                                   "peak": PEAK["fp64"], "unit": "TFLOP/s"}
-        result["roofline_fp64"]["frac"] = result["roofline_fp64"]["achieved"] / PEAK["fp64"]
+        def mfma_roofline(prec, cls, kernel):
+            g = per[prec][cls]
+            ach = g["flops"] / (g["ms"] * 1e-3) / 1e12 if g["ms"] > 0 else 0.0
+            return {"bound": "mfma", "achieved": ach, "peak": PEAK[prec], "unit": "TFLOP/s",
+                    "frac": ach / PEAK[prec], "traffic": None, "kernel": kernel,
+                    "launches": g["launches"], "avg_launch_ms": g["ms"] / max(g["launches"], 1)}
+
+        result["roofline"] = mfma_roofline(
+            "fp32", "mfma_update", "k_gemm<float,128> (rank-256 and Schur updates, one factorization)")
+        result["roofline_inner_fp32"] = mfma_roofline(
+            "fp32", "mfma_inner", "k_gemm<float,64> (rank-32 updates, one factorization)")
+        result["roofline_fp64"] = mfma_roofline(
+            "fp64", "mfma_update", "k_gemm<double,128> (rank-256 and Schur updates)")
+        result["roofline_inner_fp64"] = mfma_roofline(
+            "fp64", "mfma_inner", "k_gemm<double,64> (rank-32 updates)")
         result["kernel_classes_ms"] = {k: {c: round(v[c]["ms"], 3) for c in classes}
                                        for k, v in per.items()}
         log(f"profile: {result['kernel_classes_ms']}")

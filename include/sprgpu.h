@@ -146,9 +146,11 @@ sprgpu_int sprgpu_nzerol(const sprgpu_handle *handle);
 sprgpu_int sprgpu_device_bytes(const sprgpu_handle *handle);
 
 /* sprgpu_factor with a HIP event after every kernel (slower; for
- * measurement).  out[3*c+0..2] = milliseconds, kernel launches, algorithmic
- * flops for kernel class c: 0 front init, 1 extend-add, 2 LDS small fronts,
- * 3 diagonal blocks, 4 panel rows, 5 MFMA updates, 6 other. */
+ * measurement).  out[3*c+0..2] (c < 8, so out holds 24 values) =
+ * milliseconds, kernel launches, algorithmic flops for kernel class c:
+ * 0 front init, 1 extend-add, 2 LDS small fronts, 3 diagonal-block
+ * write-back, 4 panel steps, 5 MFMA rank-256 and Schur updates, 6 other,
+ * 7 MFMA rank-32 updates inside a 256-column block. */
 int sprgpu_profile_factor(sprgpu_handle *handle, const double *sm_dev, const double *tol,
                           int mode, double *out);
 

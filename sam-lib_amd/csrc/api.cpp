@@ -39,9 +39,10 @@ static void free_all(sprgpu_handle *h)
                     d.ea_tasks, d.small_nodes, d.panel_desc, d.panel_prefix, d.gemm_desc,
                     d.gemm_prefix, d.pool,     d.dvec,       d.adiag,      d.dcor,
                     d.status,   d.stats,      d.vwork,       d.csr_rowptr, d.csr_col,
+                    d.dscr,
                     d.csr_src,  d.csr_val,    d.irwork,      d.level_nodes,
-                    d.solve_small, d.solve_big, d.fwd_desc,  d.bwd_desc,   d.fwd_prefix,
-                    d.bwd_prefix, d.partial};
+                    d.solve_small, d.wide_nodes, d.wide,     d.wide_fwd_prefix,
+                    d.wide_bwd_prefix, d.flags, d.spin_error};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     d = DeviceBuffers{};
@@ -68,7 +69,7 @@ int sprgpu_create(const sprgpu_int *mspar, const sprgpu_int *mtrees, const sprgp
     }
     std::unique_ptr<sprgpu_handle> h(new sprgpu_handle);
     h->precision = precision;
-    std::string err = build_plan(mspar, mtrees, msifa, h->plan);
+    std::string err = build_plan(mspar, mtrees, msifa, precision, h->plan);
     if (!err.empty()) {
         set_error(err);
         return -2;
@@ -88,10 +89,9 @@ int sprgpu_create(const sprgpu_int *mspar, const sprgpu_int *mtrees, const sprgp
         (rc = upload(d.gemm_prefix, p.gemm_prefix, B)) ||
         (rc = upload(d.level_nodes, p.level_nodes, B)) ||
         (rc = upload(d.solve_small, p.solve_small, B)) ||
-        (rc = upload(d.solve_big, p.solve_big, B)) || (rc = upload(d.fwd_desc, p.fwd_desc, B)) ||
-        (rc = upload(d.bwd_desc, p.bwd_desc, B)) ||
-        (rc = upload(d.fwd_prefix, p.fwd_prefix, B)) ||
-        (rc = upload(d.bwd_prefix, p.bwd_prefix, B))) {
+        (rc = upload(d.wide_nodes, p.wide_nodes, B)) || (rc = upload(d.wide, p.wide, B)) ||
+        (rc = upload(d.wide_fwd_prefix, p.wide_fwd_prefix, B)) ||
+        (rc = upload(d.wide_bwd_prefix, p.wide_bwd_prefix, B))) {
         free_all(h.get());
         return rc;
     }
@@ -102,6 +102,7 @@ int sprgpu_create(const sprgpu_int *mspar, const sprgpu_int *mtrees, const sprgp
     if (e == hipSuccess) e = hipMalloc(&d.dcor, sizeof(double) * 2 * p.neq);
     if (e == hipSuccess) e = hipMalloc(&d.status, sizeof(FactorStatus));
     if (e == hipSuccess) e = hipMalloc(&d.stats, sizeof(double) * 4 * 257);
+    if (e == hipSuccess) e = hipMalloc(&d.dscr, esz * size_t(p.neq) * kPanelNB);
     if (e != hipSuccess) {
         free_all(h.get());
         return hip_fail(e, "device allocation");
@@ -227,7 +228,7 @@ int sprgpu_plan_summary(const sprgpu_int *mspar, const sprgpu_int *mtrees,
 {
     if (!mspar || !mtrees || !msifa || !out) return -2;
     Plan p;
-    std::string err = build_plan(mspar, mtrees, msifa, p);
+    std::string err = build_plan(mspar, mtrees, msifa, SPRGPU_FP64, p);
     if (!err.empty()) {
         set_error(err);
         return -2;

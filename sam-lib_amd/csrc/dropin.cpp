@@ -256,6 +256,7 @@ void solve_dropin(int which, sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *
         if (he != hipSuccess) rc = hip_fail(he, "upload B");
     }
     if (!rc) rc = run_solve(h, e->d_b, neq, k, which);
+    if (!rc) rc = check_solve_error(h);
     if (!rc) {
         if (perm) {
             he = hipMemcpy(tmp.data(), e->d_b, sizeof(double) * neq * k, hipMemcpyDeviceToHost);

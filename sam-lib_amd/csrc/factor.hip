@@ -11,8 +11,8 @@
 //   k_extend_add   add each child's update block into its parent through
 //                  the child->parent relative map            (SPRF12/SPRF13)
 //   k_small        fronts with F<=64: whole LDL' + Schur update in LDS
-//   k_pdiag        blocked fronts: LDL' of the kPanelNB-wide diagonal block
-//   k_prows        ... and the panel TRSM of the rows below it (SPRF17)
+//   k_panel        blocked fronts: LDL' of a kPanelNB-wide diagonal block and
+//                  the panel TRSM of the rows below it         (SPRF17)
 //   k_gemm         MFMA update C -= L D L^T on lower tiles: the trailing
 //                  update inside the panel and the Schur complement of the
 //                  update block                              (LMVY18/LMMY18)
@@ -254,179 +254,329 @@ __device__ __forceinline__ int find_desc(const i64 *prefix, int n, i64 b)
     return lo;
 }
 
-// Diagonal block: one workgroup per front of the launch factors the nb x nb
-// block at column k0 in LDS and writes d (diagonal) and l (below) back.
+// Value of lane l's v in every lane (l uniform).
+__device__ __forceinline__ float lane_bcast(float v, int l)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ __forceinline__ double lane_bcast(double v, int l)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(unsigned(u)), l));
+    const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(unsigned(u >> 32)), l));
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
+// Panel step: columns [k0, k0+nb) of a blocked front, nb <= kPanelNB.
+// Every workgroup factors the nb x nb diagonal block itself: one wavefront,
+// lane r holding row r in registers, right-looking over the columns with the
+// column entries broadcast by readlane (SPRF17's recurrence
+// l_rj = (a_rj - sum_k l_rk d_k l_jk) / d_j, applied one column at a time).
+// The same wavefront inverts the unit lower factor L11 (lane c builds column
+// c of L11^-1), and the workgroup computes its kPanelRows rows below the
+// block as the product L21 = A21 (L11^-T D^-1) on MFMA, one 64-row slice per
+// wavefront.  The A21 loads are issued before the diagonal factor so their
+// latency overlaps it.  Workgroup 0 records the pivots and stores the factored
+// block in a scratch array indexed by equation; the diagonal blocks are
+// written into the fronts after the whole factorization (k_diag_writeback),
+// so no workgroup ever reads a block another one writes.
 template <typename T>
-__global__ __launch_bounds__(256) void k_pdiag(const PanelDesc *__restrict__ descs,
+__global__ __launch_bounds__(256) void k_panel(const PanelDesc *__restrict__ descs, int ndesc,
+                                               const i64 *__restrict__ prefix,
                                                const NodeInfo *__restrict__ nodes,
                                                T *__restrict__ pool, double *dvec,
                                                const double *__restrict__ adiag, double *dcor,
-                                               FactorStatus *st, PivotArgs pa)
+                                               FactorStatus *st, PivotArgs pa, T *dscr)
 {
     constexpr int NB = kPanelNB;
-    __shared__ T L[NB][NB + 1];
-    __shared__ T dsh[NB];
-    const PanelDesc pd = descs[blockIdx.x];
-    const NodeInfo n = nodes[pd.node];
-    T *fr = pool + n.foff;
-    const i64 ld = n.ld;
-    const int k0 = pd.k0, nb = pd.nb, tid = threadIdx.x;
-    for (int idx = tid; idx < nb * nb; idx += 256) {
-        const int r = idx % nb, c = idx / nb;
-        if (r >= c) L[r][c] = fr[(k0 + r) + (k0 + c) * ld];
-    }
-    __syncthreads();
-    for (int j = 0; j < nb; ++j) {
-        if (tid == 0) {
-            const int eq = n.first + k0 + j;
-            const double d_in = double(L[j][j]);
-            const double a = adiag[eq];
-            const double d = pivot_value(d_in, a, pa);
-            pivot_record(d_in, d, a, eq, pa, dvec, dcor, st);
-            L[j][j] = T(d);
-            dsh[j] = T(d);
-        }
-        __syncthreads();
-        const T inv = T(1) / dsh[j];
-        const int m = nb - j - 1;
-        for (int idx = tid; idx < m * m; idx += 256) {
-            const int rr = idx % m, cc = idx / m;
-            if (rr >= cc) {
-                const int r = j + 1 + rr, c = j + 1 + cc;
-                L[r][c] -= L[r][j] * (L[c][j] * inv);
-            }
-        }
-        __syncthreads();
-        for (int r = j + 1 + tid; r < nb; r += 256) L[r][j] *= inv;
-        __syncthreads();
-    }
-    for (int idx = tid; idx < nb * nb; idx += 256) {
-        const int r = idx % nb, c = idx / nb;
-        if (r >= c) fr[(k0 + r) + (k0 + c) * ld] = L[r][c];
-    }
-}
-
-// Panel rows below the diagonal block: L21 = A21 L11^-T D^-1, one row per
-// thread, reading the factored diagonal block written by k_pdiag.
-template <typename T>
-__global__ __launch_bounds__(256) void k_prows(const PanelDesc *__restrict__ descs, int ndesc,
-                                               const i64 *__restrict__ prefix,
-                                               const NodeInfo *__restrict__ nodes,
-                                               T *__restrict__ pool)
-{
-    constexpr int NB = kPanelNB;
-    __shared__ T L[NB][NB + 1];
-    __shared__ T dinv[NB];
+    static_assert(NB == 32 && kPanelRows == 256,
+                  "one lane per diagonal-block row; four 64-row MFMA slices per workgroup");
+    __shared__ T Bm[NB][NB + 1];  // L11^-T D^-1 (upper triangular), zero outside nb x nb
     const int q = find_desc(prefix, ndesc, blockIdx.x);
     const PanelDesc pd = descs[q];
     const int rb = int(blockIdx.x - prefix[q]);
     const NodeInfo n = nodes[pd.node];
     T *fr = pool + n.foff;
     const i64 ld = n.ld;
-    const int k0 = pd.k0, nb = pd.nb, tid = threadIdx.x;
-    for (int idx = tid; idx < nb * nb; idx += 256) {
-        const int r = idx % nb, c = idx / nb;
-        if (r >= c) L[r][c] = fr[(k0 + r) + (k0 + c) * ld];
-    }
-    __syncthreads();
-    if (tid < nb) dinv[tid] = T(1) / L[tid][tid];
-    __syncthreads();
-    const int r = k0 + nb + rb * kPanelRows + tid;
-    if (r < n.F) {
-        T x[NB];
+    const int k0 = pd.k0, nb = pd.nb, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+    // this wavefront's 64 rows of A21 in the MFMA A layout: a[ti][ks] holds
+    // A21(rw + 16 ti + (lane & 15), 4 ks + (lane >> 4))
+    const int rw = k0 + nb + rb * kPanelRows + wave * 64;
+    const bool active = rw < n.F;
+    T a[4][NB / 4];
 #pragma unroll
-        for (int c = 0; c < NB; ++c) x[c] = (c < nb) ? fr[r + (k0 + c) * ld] : T(0);
+    for (int ti = 0; ti < 4; ++ti)
 #pragma unroll
-        for (int c = 0; c < NB; ++c) {
-            T s = x[c];
-#pragma unroll
-            for (int i = 0; i < c; ++i) s -= x[i] * L[c][i];
-            x[c] = s;  // w_rc = l_rc * d_c
+        for (int ks = 0; ks < NB / 4; ++ks) {
+            const int row = rw + ti * 16 + (lane & 15), col = ks * 4 + (lane >> 4);
+            a[ti][ks] = (row < n.F && col < nb) ? fr[row + (k0 + col) * ld] : T(0);
         }
+    if (wave == 0) {
+        const int r = lane;  // lanes >= nb hold zero rows
+        T row[NB];
 #pragma unroll
         for (int c = 0; c < NB; ++c)
-            if (c < nb) fr[r + (k0 + c) * ld] = x[c] * dinv[c];
+            row[c] = (r < nb && c <= r) ? fr[(k0 + r) + (k0 + c) * ld] : T(0);
+        const double avr = (r < nb) ? adiag[n.first + k0 + r] : 0.0;
+        T dinvr = T(0);  // lane j: 1/d_j
+        // After step j, lane r > j holds l_rj in row[j] and lane j holds d_j;
+        // entries right of a lane's diagonal are scratch and never read.
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            if (j < nb) {
+                const T dj = lane_bcast(row[j], j);
+                const double av = lane_bcast(avr, j);
+                const int eq = n.first + k0 + j;
+                const double dp = pivot_value(double(dj), av, pa);
+                if (rb == 0 && r == 0) pivot_record(double(dj), dp, av, eq, pa, dvec, dcor, st);
+                const T d = T(dp);
+                const T di = T(1) / d;
+                const T w = row[j];  // lane r > j: a_rj - sum_{k<j} l_rk d_k l_jk = l_rj d_j
+                const T l = w * di;
+#pragma unroll
+                for (int k = j + 1; k < NB; ++k) {
+                    const T wk = lane_bcast(w, k);
+                    if (r > j) row[k] -= l * wk;
+                }
+                row[j] = (r > j) ? l : ((r == j) ? d : row[j]);
+                if (r == j) dinvr = di;
+            }
+        }
+        if (rb == 0 && r < nb) {
+            T *out = dscr + i64(n.first + k0 + r) * NB;
+#pragma unroll
+            for (int c = 0; c < NB; ++c) out[c] = (c <= r) ? row[c] : T(0);
+        }
+        // lane c: column c of M = L11^-1, m_i = delta_ic - sum_{k<i} l_ik m_k
+        T m[NB];
+#pragma unroll
+        for (int i = 0; i < NB; ++i) m[i] = (i == r) ? T(1) : T(0);
+#pragma unroll
+        for (int i = 1; i < NB; ++i) {
+            if (i < nb) {
+#pragma unroll
+                for (int k = 0; k < i; ++k) m[i] -= lane_bcast(row[k], i) * m[k];
+            }
+        }
+        // Bm[i][c] = M[c][i] / d_c: lane i writes row i
+        if (r < NB) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) Bm[r][c] = m[c] * lane_bcast(dinvr, c);
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    typename Mfma<T>::acc_t acc[4][2];
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = typename Mfma<T>::acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < NB / 4; ++ks) {
+        T b[2];
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj) b[tj] = Bm[ks * 4 + (lane >> 4)][tj * 16 + (lane & 15)];
+#pragma unroll
+        for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < 2; ++tj) acc[ti][tj] = Mfma<T>::op(a[ti][ks], b[tj], acc[ti][tj]);
+    }
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int row = rw + ti * 16 + Mfma<T>::row(lane, e);
+                const int col = tj * 16 + (lane & 15);
+                if (row < n.F && col < nb) fr[row + (k0 + col) * ld] = acc[ti][tj][e];
+            }
+}
+
+// Writes the factored diagonal blocks of every panel step into the fronts.
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag_writeback(const PanelDesc *__restrict__ descs,
+                                                        const NodeInfo *__restrict__ nodes,
+                                                        T *__restrict__ pool,
+                                                        const T *__restrict__ dscr)
+{
+    const PanelDesc pd = descs[blockIdx.x];
+    const NodeInfo n = nodes[pd.node];
+    T *fr = pool + n.foff;
+    const i64 ld = n.ld;
+    for (int idx = threadIdx.x; idx < pd.nb * pd.nb; idx += 256) {
+        const int r = idx % pd.nb, c = idx / pd.nb;
+        if (r >= c)
+            fr[(pd.k0 + r) + (pd.k0 + c) * ld] = dscr[i64(n.first + pd.k0 + r) * kPanelNB + c];
     }
 }
 
 // ---------------------------------------------------------------------------
 // MFMA update: C(i,j) -= sum_k L(i,k) d_k L(j,k) on lower tiles
 // ---------------------------------------------------------------------------
-template <typename T>
+// One workgroup computes a BM x BM tile of C; its four wavefronts own 2x2
+// quadrants of WM = BM/2 rows and columns, each a grid of TM x TM MFMAs of
+// 16x16x4.  Inside a quadrant, MFMA sub-tile ti covers the rows TM*m + ti
+// (m = 0..15), so the TM operand values a lane needs at one k are adjacent
+// in LDS (one 16-byte read for TM*sizeof(T) = 16) and the TM accumulator
+// values it holds for one column are adjacent rows of C (one vector load and
+// store).  The update is accumulated apart from C (for accuracy in FP32) and
+// subtracted in the epilogue; B is scaled by the pivots as it is staged.
+// The K loop moves KB columns of L per stage with 16-byte global
+// loads: the next stage's loads are issued into registers before the current
+// stage's MFMAs and stored into the other LDS buffer after them, so each
+// stage needs a single barrier.  Consecutive workgroups are dispatched
+// round-robin over the 8 XCDs; the block -> tile map gives each XCD a
+// contiguous range of tiles, so tiles that share L columns share an L2.
+// Tiles are anchored at r0 rounded down to kGemmAlign and front leading
+// dimensions are multiples of kFrontLdAlign, so all vectors are aligned.
+constexpr int kXcds = 8;
+
+template <typename T> struct GemmCfg;
+template <> struct GemmCfg<float> { static constexpr int KB = 32; };
+template <> struct GemmCfg<double> { static constexpr int KB = 16; };
+
+template <typename T, int BM>
 __global__ __launch_bounds__(256) void k_gemm(const GemmDesc *__restrict__ descs, int ndesc,
                                               const i64 *__restrict__ prefix,
-                                              T *__restrict__ pool)
+                                              T *__restrict__ pool,
+                                              const double *__restrict__ dvec)
 {
-    constexpr int KB = 16;
-    constexpr int LDS = kTile + 2;
-    __shared__ T As[KB][LDS];
-    __shared__ T Bs[KB][LDS];
-    const int q = find_desc(prefix, ndesc, blockIdx.x);
+    constexpr int KB = GemmCfg<T>::KB;
+    constexpr int VE = 16 / sizeof(T);     // elements per 16-byte vector
+    constexpr int WM = BM / 2;             // quadrant edge
+    constexpr int TM = WM / 16;            // MFMA sub-tiles per quadrant edge
+    constexpr int LDR = BM + (VE > TM ? VE : TM);  // padded LDS row (one k), TM-vector aligned
+    constexpr int NVC = BM / VE;           // vectors per column of a stage
+    constexpr int NV = NVC * KB / 256;     // vectors per thread per operand per stage
+    static_assert(NV >= 1 && NVC * KB % 256 == 0, "stage must split evenly over 256 threads");
+    static_assert(TM % VE == 0 || VE % TM == 0, "operand vectors");
+    static_assert(kGemmAlign % VE == 0 && kFrontLdAlign % VE == 0, "vector alignment");
+    typedef T vec_t __attribute__((ext_vector_type(VE)));
+    typedef T tm_t __attribute__((ext_vector_type(TM)));
+    __shared__ __attribute__((aligned(16))) T As[2][KB][LDR];
+    __shared__ __attribute__((aligned(16))) T Bs[2][KB][LDR];
+
+    // XCD-aware bijection of block ids onto tiles
+    const i64 nb = gridDim.x, bid = blockIdx.x;
+    const i64 per = nb / kXcds, rem = nb % kXcds;
+    const i64 xcd = bid % kXcds, idx = bid / kXcds;
+    const i64 lin = (xcd < rem) ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+
+    const int q = find_desc(prefix, ndesc, lin);
     const GemmDesc g = descs[q];
-    const i64 t = i64(blockIdx.x) - prefix[q];
+    const i64 t = lin - prefix[q];
     // tile (ti,tj), column-major over the lower trapezoid: column tj holds
     // nti - tj tiles and starts at tj*nti - tj*(tj-1)/2
-    int tj = 0;
+    int tj;
     {
         const double a = double(g.nti) + 0.5;
-        double est = a - sqrt(a * a - 2.0 * double(t));
-        tj = int(est);
-        if (tj < 0) tj = 0;
-        if (tj > g.ntj - 1) tj = g.ntj - 1;
+        tj = int(a - sqrt(a * a - 2.0 * double(t)));
+        tj = max(0, min(tj, g.ntj - 1));
         auto start = [&](i64 x) { return x * g.nti - (x * (x - 1)) / 2; };
         while (tj > 0 && start(tj) > t) --tj;
         while (tj + 1 < g.ntj && start(tj + 1) <= t) ++tj;
-        (void)est;
     }
     const int ti = tj + int(t - (i64(tj) * g.nti - (i64(tj) * (tj - 1)) / 2));
-    const int i0 = g.r0 + ti * kTile, j0 = g.r0 + tj * kTile;
+    const int a0 = g.r0 / kGemmAlign * kGemmAlign;
+    const int i0 = a0 + ti * BM, j0 = a0 + tj * BM;
     T *fr = pool + g.foff;
     const i64 ld = g.ld;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = wave >> 1, wc = wave & 1;
-
-    typename Mfma<T>::acc_t acc[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = typename Mfma<T>::acc_t{0, 0, 0, 0};
-
     const int kend = g.k0 + g.kn;
-    const int lr = tid & 63;
-    for (int kk = g.k0; kk < kend; kk += KB) {
-        for (int kc = tid >> 6; kc < KB; kc += 4) {
+    // first row / column of this lane's accumulator vectors
+    const int qi0 = i0 + wr * WM, qj0 = j0 + wc * WM;
+
+    // acc[ti][tj][e] accumulates the update of
+    // C(qi0 + TM*Mfma::row(lane,e) + ti, qj0 + TM*(lane&15) + tj)
+    typename Mfma<T>::acc_t acc[TM][TM];
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TM; ++b) acc[a][b] = typename Mfma<T>::acc_t{0, 0, 0, 0};
+
+    // Stage loads are unconditional (an out-of-range vector reads the front's
+    // first element instead) so that they all stay in flight across the
+    // MFMAs; masking and the pivot scaling happen when the stage is stored.
+    vec_t ra[NV], rb[NV];
+    double dk[NV];
+    unsigned vmask = 0;  // bit 2m: A vector m valid, bit 2m+1: B vector m valid
+    auto load_stage = [&](int kk) {
+        vmask = 0;
+#pragma unroll
+        for (int m = 0; m < NV; ++m) {
+            const int v = tid + 256 * m;
+            const int rv = (v % NVC) * VE, kc = v / NVC;
             const int col = kk + kc;
             const bool cv = col < kend;
-            const int ra = i0 + lr, rbw = j0 + lr;
-            const T dcol = cv ? fr[col + col * ld] : T(0);
-            As[kc][lr] = (cv && ra < g.rend) ? fr[ra + col * ld] : T(0);
-            Bs[kc][lr] = (cv && rbw < g.cend) ? fr[rbw + col * ld] * dcol : T(0);
+            const int arow = i0 + rv, brow = j0 + rv;
+            const bool va = cv && arow < g.rend, vb = cv && brow < g.cend;
+            vmask |= (va ? 1u : 0u) << (2 * m);
+            vmask |= (vb ? 1u : 0u) << (2 * m + 1);
+            const i64 cofs = i64(cv ? col : g.k0) * ld;
+            ra[m] = *reinterpret_cast<const vec_t *>(fr + (va ? arow + cofs : 0));
+            rb[m] = *reinterpret_cast<const vec_t *>(fr + (vb ? brow + cofs : 0));
+            dk[m] = dvec[g.first + (cv ? col : g.k0)];
         }
-        __syncthreads();
+    };
+    auto store_stage = [&](int buf) {
+#pragma unroll
+        for (int m = 0; m < NV; ++m) {
+            const int v = tid + 256 * m;
+            const int rv = (v % NVC) * VE, kc = v / NVC;
+            const vec_t za = (vmask >> (2 * m)) & 1u ? ra[m] : vec_t{};
+            const vec_t zb = (vmask >> (2 * m + 1)) & 1u ? rb[m] * T(dk[m]) : vec_t{};
+            *reinterpret_cast<vec_t *>(&As[buf][kc][rv]) = za;
+            *reinterpret_cast<vec_t *>(&Bs[buf][kc][rv]) = zb;
+        }
+    };
+
+    load_stage(g.k0);
+    store_stage(0);
+    __syncthreads();
+    int cur = 0;
+    const int ar = wr * WM + TM * (lane & 15), bc = wc * WM + TM * (lane & 15);
+    for (int kk = g.k0; kk < kend; kk += KB) {
+        const bool more = kk + KB < kend;
+        if (more) load_stage(kk + KB);
 #pragma unroll
         for (int k4 = 0; k4 < KB; k4 += 4) {
             const int kidx = k4 + (lane >> 4);
-            const T a0 = As[kidx][wr * 32 + (lane & 15)];
-            const T a1 = As[kidx][wr * 32 + 16 + (lane & 15)];
-            const T b0 = Bs[kidx][wc * 32 + (lane & 15)];
-            const T b1 = Bs[kidx][wc * 32 + 16 + (lane & 15)];
-            acc[0][0] = Mfma<T>::op(a0, b0, acc[0][0]);
-            acc[0][1] = Mfma<T>::op(a0, b1, acc[0][1]);
-            acc[1][0] = Mfma<T>::op(a1, b0, acc[1][0]);
-            acc[1][1] = Mfma<T>::op(a1, b1, acc[1][1]);
+            const tm_t av = *reinterpret_cast<const tm_t *>(&As[cur][kidx][ar]);
+            const tm_t bv = *reinterpret_cast<const tm_t *>(&Bs[cur][kidx][bc]);
+#pragma unroll
+            for (int a = 0; a < TM; ++a)
+#pragma unroll
+                for (int b = 0; b < TM; ++b) acc[a][b] = Mfma<T>::op(av[a], bv[b], acc[a][b]);
         }
+        if (more) store_stage(cur ^ 1);
         __syncthreads();
+        cur ^= 1;
     }
+    // write back the lower-trapezoid entries of rows [r0,rend) x cols [r0,cend)
 #pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
+    for (int b = 0; b < TM; ++b)
 #pragma unroll
-        for (int bj = 0; bj < 2; ++bj)
+        for (int e = 0; e < 4; ++e) {
+            const int row = qi0 + TM * Mfma<T>::row(lane, e);
+            const int col = qj0 + TM * (lane & 15) + b;
+            if (col < g.r0 || col >= g.cend || row >= g.rend || row + TM - 1 < col) continue;
+            if (row >= g.r0 && row >= col && row + TM <= g.rend) {
+                tm_t *pc = reinterpret_cast<tm_t *>(fr + row + col * ld);
+                tm_t v = *pc;
 #pragma unroll
-            for (int ii = 0; ii < 4; ++ii) {
-                const int row = i0 + wr * 32 + bi * 16 + Mfma<T>::row(lane, ii);
-                const int col = j0 + wc * 32 + bj * 16 + (lane & 15);
-                if (row < g.rend && col < g.cend && row >= col) fr[row + col * ld] -= acc[bi][bj][ii];
+                for (int a = 0; a < TM; ++a) v[a] -= acc[a][b][e];
+                *pc = v;
+            } else {
+#pragma unroll
+                for (int a = 0; a < TM; ++a) {
+                    const int r = row + a;
+                    if (r >= g.r0 && r >= col && r < g.rend) fr[r + col * ld] -= acc[a][b][e];
+                }
             }
+        }
 }
 
 // ---------------------------------------------------------------------------
@@ -535,7 +685,7 @@ __global__ void k_copy_diag(const double *dvec, const double *adiag, double *sm,
 // host orchestration
 // ---------------------------------------------------------------------------
 // Kernel classes for the per-class timing of sprgpu_profile_factor.
-enum ProfClass { P_INIT = 0, P_EA, P_SMALL, P_PDIAG, P_PROWS, P_GEMM, P_OTHER, P_NCLASS };
+enum ProfClass { P_INIT = 0, P_EA, P_SMALL, P_PDIAG, P_PROWS, P_GEMM, P_OTHER, P_GEMM_INNER, P_NCLASS };
 
 // Launches the whole factorization.  With ev != nullptr an event is recorded
 // after every kernel and the kernel's class appended to cls, so consecutive
@@ -581,19 +731,27 @@ static int factor_impl(sprgpu_handle *h, const double *sm_dev, int mode, const d
             mark(P_SMALL, 0.0);
             break;
         case L_PANEL:
-            k_pdiag<T><<<unsigned(L.count), 256, 0, s>>>(d.panel_desc + L.off, d.nodes, pool,
-                                                         d.dvec, d.adiag, d.dcor, d.status, pa);
-            mark(P_PDIAG, 0.0);
-            k_prows<T><<<blocks, 256, 0, s>>>(d.panel_desc + L.off, int(L.count),
-                                              d.panel_prefix + L.off, d.nodes, pool);
+            k_panel<T><<<blocks, 256, 0, s>>>(d.panel_desc + L.off, int(L.count),
+                                              d.panel_prefix + L.off, d.nodes, pool, d.dvec,
+                                              d.adiag, d.dcor, d.status, pa,
+                                              static_cast<T *>(d.dscr));
             mark(P_PROWS, 0.0);
             break;
         case L_GEMM:
-            k_gemm<T><<<blocks, 256, 0, s>>>(d.gemm_desc + L.off, int(L.count),
-                                             d.gemm_prefix + L.off, pool);
-            mark(P_GEMM, L.flops);
+            if (L.tile == kTileBig)
+                k_gemm<T, kTileBig><<<blocks, 256, 0, s>>>(d.gemm_desc + L.off, int(L.count),
+                                                           d.gemm_prefix + L.off, pool, d.dvec);
+            else
+                k_gemm<T, kTileSmall><<<blocks, 256, 0, s>>>(d.gemm_desc + L.off, int(L.count),
+                                                             d.gemm_prefix + L.off, pool, d.dvec);
+            mark(L.outer ? P_GEMM : P_GEMM_INNER, L.flops);
             break;
         }
+    }
+    if (!p.panel_desc.empty()) {
+        k_diag_writeback<T><<<unsigned(p.panel_desc.size()), 256, 0, s>>>(
+            d.panel_desc, d.nodes, pool, static_cast<const T *>(d.dscr));
+        mark(P_PDIAG, 0.0);
     }
     k_stats_partial<<<kStatBlocks, 256, 0, s>>>(d.dvec, d.adiag, p.neq, d.status, d.stats);
     k_stats_final<<<1, 64, 0, s>>>(d.stats, kStatBlocks);

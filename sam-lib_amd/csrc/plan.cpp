@@ -33,8 +33,10 @@ inline i64 round_up(i64 x, i64 m) { return (x + m - 1) / m * m; }
 
 }  // namespace
 
-std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, Plan &p)
+std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, int precision,
+                       Plan &p)
 {
+    p.big_tile = (precision == SPRGPU_FP32) ? kTileBig : kTileSmall;
     auto M = [&](int k) { return mspar[k - 1]; };
     if (M(1) < 0) return "MSPAR(1) < 0: the symbolic phase did not complete";
     p.neq = M(8);
@@ -102,7 +104,7 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, Pl
     i64 pool = 0, vw = 0, rm = 0;
     for (i64 j = 0; j < ns; ++j) {
         NodeInfo &n = p.nodes[j];
-        n.ld = i32(round_up(n.F, 2));
+        n.ld = i32(round_up(n.F, kFrontLdAlign));
         n.foff = pool;
         pool += round_up(i64(n.ld) * n.F, 64);
         n.voff = vw;
@@ -150,9 +152,9 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, Pl
     p.launches.clear();
 
     auto add_launch = [&](LaunchKind kind, int level, i64 off, i64 count, i64 blocks,
-                          double flops = 0.0) {
+                          double flops = 0.0, i32 tile = 0, i32 outer = 0) {
         if (count > 0 && blocks > 0)
-            p.launches.push_back(Launch{kind, level, off, count, blocks, flops});
+            p.launches.push_back(Launch{kind, level, off, count, blocks, flops, tile, outer});
     };
     // useful flops of a GEMM descriptor: 2*kn per entry (i,j) of the lower
     // trapezoid rows [r0,rend) x cols [r0,cend), i >= j
@@ -160,6 +162,26 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, Pl
         const double nc = double(g.cend - g.r0);
         const double pairs = nc * g.rend - (double(g.r0) + g.cend - 1) * nc / 2.0;
         return 2.0 * g.kn * pairs;
+    };
+    // Update of rows [r0,rend) x cols [r0,cend) of front n with its columns
+    // [k0,k0+kn).  The tile grid is anchored at r0 rounded down to a multiple
+    // of kGemmAlign so that every tile starts on a 16-byte boundary.
+    auto make_gemm = [&](const NodeInfo &n, i32 r0, i32 rend, i32 cend, i32 k0, i32 kn,
+                         i32 tile) {
+        GemmDesc g;
+        g.foff = n.foff;
+        g.first = n.first;
+        g.pad = 0;
+        g.ld = n.ld;
+        g.r0 = r0;
+        g.rend = rend;
+        g.cend = cend;
+        g.k0 = k0;
+        g.kn = kn;
+        const i32 a0 = r0 / kGemmAlign * kGemmAlign;
+        g.nti = (rend - a0 + tile - 1) / tile;
+        g.ntj = (cend - a0 + tile - 1) / tile;
+        return g;
     };
 
     for (int l = 0; l < p.nlevels; ++l) {
@@ -233,22 +255,13 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, Pl
                 const i32 r0 = k0 + nb;
                 const i32 cend = std::min(n.S, oend);
                 if (r0 >= cend) continue;
-                GemmDesc g;
-                g.foff = n.foff;
-                g.ld = n.ld;
-                g.r0 = r0;
-                g.rend = n.F;
-                g.cend = cend;
-                g.k0 = k0;
-                g.kn = nb;
-                g.nti = (n.F - r0 + kTile - 1) / kTile;
-                g.ntj = (cend - r0 + kTile - 1) / kTile;
+                const GemmDesc g = make_gemm(n, r0, n.F, cend, k0, nb, kTileSmall);
                 p.gemm_desc.push_back(g);
                 p.gemm_prefix.push_back(ntiles);
                 ntiles += tri_tiles(g.nti, g.ntj);
                 fl += gemm_flops(g);
             }
-            add_launch(L_GEMM, l, off, i64(p.gemm_desc.size()) - off, ntiles, fl);
+            add_launch(L_GEMM, l, off, i64(p.gemm_desc.size()) - off, ntiles, fl, kTileSmall);
             if (k0 + kPanelNB == oend) {
                 const i32 ob = oend - kPanelOuter;
                 off = i64(p.gemm_desc.size());
@@ -257,22 +270,13 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, Pl
                 for (i32 j : big) {
                     const NodeInfo &n = p.nodes[j];
                     if (oend >= n.S) continue;
-                    GemmDesc g;
-                    g.foff = n.foff;
-                    g.ld = n.ld;
-                    g.r0 = oend;
-                    g.rend = n.F;
-                    g.cend = n.S;
-                    g.k0 = ob;
-                    g.kn = kPanelOuter;
-                    g.nti = (n.F - oend + kTile - 1) / kTile;
-                    g.ntj = (n.S - oend + kTile - 1) / kTile;
+                    const GemmDesc g = make_gemm(n, oend, n.F, n.S, ob, kPanelOuter, p.big_tile);
                     p.gemm_desc.push_back(g);
                     p.gemm_prefix.push_back(ntiles);
                     ntiles += tri_tiles(g.nti, g.ntj);
                     fl += gemm_flops(g);
                 }
-                add_launch(L_GEMM, l, off, i64(p.gemm_desc.size()) - off, ntiles, fl);
+                add_launch(L_GEMM, l, off, i64(p.gemm_desc.size()) - off, ntiles, fl, p.big_tile, 1);
             }
         }
         // Schur complement of every blocked front of this level
@@ -282,112 +286,57 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, Pl
         for (i32 j : big) {
             const NodeInfo &n = p.nodes[j];
             if (n.F == n.S) continue;
-            GemmDesc g;
-            g.foff = n.foff;
-            g.ld = n.ld;
-            g.r0 = n.S;
-            g.rend = n.F;
-            g.cend = n.F;
-            g.k0 = 0;
-            g.kn = n.S;
-            g.nti = (n.F - n.S + kTile - 1) / kTile;
-            g.ntj = g.nti;
+            const GemmDesc g = make_gemm(n, n.S, n.F, n.F, 0, n.S, p.big_tile);
             p.gemm_desc.push_back(g);
             p.gemm_prefix.push_back(ntiles);
             ntiles += tri_tiles(g.nti, g.ntj);
             fl += gemm_flops(g);
         }
-        add_launch(L_GEMM, l, off, i64(p.gemm_desc.size()) - off, ntiles, fl);
+        add_launch(L_GEMM, l, off, i64(p.gemm_desc.size()) - off, ntiles, fl, p.big_tile, 1);
     }
 
     // ----------------------------------------------------------------------
     // solve schedule
     // ----------------------------------------------------------------------
     p.solve_small.clear();
-    p.solve_big.clear();
+    p.wide_nodes.clear();
+    p.wide.clear();
+    p.wide_fwd_prefix.clear();
+    p.wide_bwd_prefix.clear();
     p.solve_small_ptr.assign(p.nlevels + 1, 0);
-    p.solve_big_ptr.assign(p.nlevels + 1, 0);
-    p.fwd_desc.clear();
-    p.bwd_desc.clear();
-    p.fwd_prefix.clear();
-    p.bwd_prefix.clear();
-    p.fwd_steps.clear();
-    p.bwd_steps.clear();
-    p.max_partial_chunks = 0;
+    p.wide_ptr.assign(p.nlevels + 1, 0);
+    p.wide_fwd_wgs.assign(p.nlevels, 0);
+    p.wide_bwd_wgs.assign(p.nlevels, 0);
+    p.nflags = 0;
     for (int l = 0; l < p.nlevels; ++l) {
+        i64 wf = 0, wb = 0;
         for (i32 q = p.level_ptr[l]; q < p.level_ptr[l + 1]; ++q) {
             const i32 j = p.level_nodes[q];
-            if (p.nodes[j].S > kSolveBigS)
-                p.solve_big.push_back(j);
-            else
+            const NodeInfo &n = p.nodes[j];
+            if (n.S <= kSolveSmallS && n.F <= kSolveSmallF) {
                 p.solve_small.push_back(j);
+                continue;
+            }
+            WideDesc w;
+            w.node = j;
+            w.nblk = (n.S + kSolveNB - 1) / kSolveNB;
+            w.nrect = (n.F - n.S + kSolveRows - 1) / kSolveRows;
+            w.pad = 0;
+            w.wg0_fwd = wf;
+            w.wg0_bwd = wb;
+            w.flag0 = p.nflags;
+            p.wide.push_back(w);
+            p.wide_nodes.push_back(j);
+            p.wide_fwd_prefix.push_back(wf);
+            p.wide_bwd_prefix.push_back(wb);
+            wf += w.nblk + w.nrect;
+            wb += w.nblk;
+            p.nflags += w.nblk;
         }
         p.solve_small_ptr[l + 1] = i32(p.solve_small.size());
-        p.solve_big_ptr[l + 1] = i32(p.solve_big.size());
-    }
-    // forward: for block b of each wide supernode, solve the diagonal block,
-    // then update every row below it
-    for (int l = 0; l < p.nlevels; ++l) {
-        const i32 b0 = p.solve_big_ptr[l], b1 = p.solve_big_ptr[l + 1];
-        if (b0 == b1) continue;
-        p.fwd_steps.push_back(SolveStep{0, l, b0, b1 - b0, b1 - b0});
-        i32 maxblk = 0;
-        for (i32 q = b0; q < b1; ++q)
-            maxblk = std::max(maxblk, (p.nodes[p.solve_big[q]].S + kSolveNB - 1) / kSolveNB);
-        for (i32 s = 0; s < maxblk; ++s) {
-            const i64 off = i64(p.fwd_desc.size());
-            i64 chunks = 0;
-            for (i32 q = b0; q < b1; ++q) {
-                const NodeInfo &n = p.nodes[p.solve_big[q]];
-                const i32 c0 = s * kSolveNB;
-                if (c0 >= n.S) continue;
-                const i32 nb = std::min(kSolveNB, n.S - c0);
-                const i32 rows = n.F - (c0 + nb);
-                const i32 nch = (rows + kSolveRows - 1) / kSolveRows;
-                p.fwd_desc.push_back(SolveDesc{p.solve_big[q], c0, nb, nch});
-                p.fwd_prefix.push_back(chunks);
-                chunks += nch;
-            }
-            const i64 cnt = i64(p.fwd_desc.size()) - off;
-            p.fwd_steps.push_back(SolveStep{1, l, off, cnt, cnt});
-            if (chunks > 0) p.fwd_steps.push_back(SolveStep{2, l, off, cnt, chunks});
-        }
-        p.fwd_steps.push_back(SolveStep{3, l, b0, b1 - b0, b1 - b0});
-    }
-    // backward: blocks from the last to the first; partial sums of L^T x over
-    // the rows below the block, then the reduction and the diagonal block
-    for (int l = p.nlevels - 1; l >= 0; --l) {
-        const i32 b0 = p.solve_big_ptr[l], b1 = p.solve_big_ptr[l + 1];
-        if (b0 == b1) continue;
-        p.bwd_steps.push_back(SolveStep{0, l, b0, b1 - b0, b1 - b0});
-        i32 maxblk = 0;
-        for (i32 q = b0; q < b1; ++q)
-            maxblk = std::max(maxblk, (p.nodes[p.solve_big[q]].S + kSolveNB - 1) / kSolveNB);
-        for (i32 s = maxblk - 1; s >= 0; --s) {
-            const i64 off = i64(p.bwd_desc.size());
-            i64 chunks = 0;
-            for (i32 q = b0; q < b1; ++q) {
-                const NodeInfo &n = p.nodes[p.solve_big[q]];
-                const i32 nblk = (n.S + kSolveNB - 1) / kSolveNB;
-                // align the last blocks of all nodes so each node runs its
-                // blocks from last to first
-                const i32 bi = s - (maxblk - nblk);
-                if (bi < 0) continue;
-                const i32 c0 = bi * kSolveNB;
-                const i32 nb = std::min(kSolveNB, n.S - c0);
-                const i32 rows = n.F - (c0 + nb);
-                const i32 nch = (rows + kSolveRows - 1) / kSolveRows;
-                p.bwd_desc.push_back(SolveDesc{p.solve_big[q], c0, nb, nch});
-                p.bwd_prefix.push_back(chunks);
-                chunks += nch;
-            }
-            const i64 cnt = i64(p.bwd_desc.size()) - off;
-            if (cnt == 0) continue;
-            if (chunks > 0) p.bwd_steps.push_back(SolveStep{2, l, off, cnt, chunks});
-            p.bwd_steps.push_back(SolveStep{1, l, off, cnt, cnt});
-            p.max_partial_chunks = std::max(p.max_partial_chunks, chunks);
-        }
-        p.bwd_steps.push_back(SolveStep{3, l, b0, b1 - b0, b1 - b0});
+        p.wide_ptr[l + 1] = i32(p.wide.size());
+        p.wide_fwd_wgs[l] = wf;
+        p.wide_bwd_wgs[l] = wb;
     }
     return "";
 }

@@ -1,0 +1,251 @@
+// FP64 residual SpMV and iterative refinement.
+//
+// The FP64 SpMV plays the role of the reference's SPRPRM product
+// (sprprm.f:243-324: C = A B on the unfactored matrix) for the refinement
+// residual r = b - A x; it runs on a CSR copy of the assembled matrix's
+// non-zeros (the pattern is built once per handle, the values are gathered
+// from SM at each factorization).  Refinement: x = A^-1 b with the current
+// factor (FP32 or FP64), then r = b - A x in FP64 and x += A^-1 r until the
+// relative residual of every column is at most the requested tolerance.
+#include <cfloat>
+#include <cmath>
+
+#include "sprgpu_internal.h"
+
+namespace sprgpu {
+
+// ---------------------------------------------------------------------------
+// FP64 residual: y = alpha*b + beta*A x (CSR, 16 lanes per row)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_spmv(const i64 *__restrict__ rowptr,
+                                              const i32 *__restrict__ col,
+                                              const double *__restrict__ val,
+                                              const double *__restrict__ x, double *y,
+                                              const double *__restrict__ bvec, i64 n, i64 ld,
+                                              double beta)
+{
+    const i64 row = (i64(blockIdx.x) * 256 + threadIdx.x) >> 4;
+    const int sub = threadIdx.x & 15;
+    const int rhs = blockIdx.y;
+    const double *xx = x + rhs * ld;
+    double s = 0.0;
+    if (row < n) {
+        for (i64 k = rowptr[row] + sub; k < rowptr[row + 1]; k += 16) s += val[k] * xx[col[k]];
+    }
+    for (int o = 8; o > 0; o >>= 1) s += __shfl_down(s, o, 16);
+    if (row < n && sub == 0) {
+        const double b0 = bvec ? bvec[rhs * ld + row] : 0.0;
+        y[rhs * ld + row] = b0 + beta * s;
+    }
+}
+
+// gathers the CSR values from the packed assembled matrix and counts the
+// non-zeros found on or below the diagonal (each packed entry once)
+__global__ __launch_bounds__(256) void k_gather_vals(const i64 *__restrict__ rowptr,
+                                                     const i32 *__restrict__ col,
+                                                     const i64 *__restrict__ src,
+                                                     const double *__restrict__ lnz,
+                                                     double *val, i64 n,
+                                                     unsigned long long *count)
+{
+    unsigned long long c = 0;
+    for (i64 row = i64(blockIdx.x) * 256 + threadIdx.x; row < n; row += i64(gridDim.x) * 256) {
+        for (i64 k = rowptr[row]; k < rowptr[row + 1]; ++k) {
+            const double v = lnz[src[k]];
+            val[k] = v;
+            c += (v != 0.0 && col[k] <= row);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+}
+
+// per-column sums of squares, deterministic two-pass reduction
+constexpr int kNormBlocks = 128;
+__global__ __launch_bounds__(256) void k_sumsq_partial(const double *x, i64 n, i64 ld,
+                                                       double *part)
+{
+    const int rhs = blockIdx.y;
+    const double *xx = x + rhs * ld;
+    double s = 0.0;
+    for (i64 i = i64(blockIdx.x) * 256 + threadIdx.x; i < n; i += i64(gridDim.x) * 256)
+        s += xx[i] * xx[i];
+    __shared__ double sh[256];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[rhs * kNormBlocks + blockIdx.x] = sh[0];
+}
+
+__global__ void k_sumsq_final(const double *part, double *out, int nrhs)
+{
+    const int rhs = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rhs < nrhs) {
+        double s = 0.0;
+        for (int b = 0; b < kNormBlocks; ++b) s += part[rhs * kNormBlocks + b];
+        out[rhs] = s;
+    }
+}
+
+__global__ void k_axpy_cols(double *x, const double *dx, i64 n, i64 ld)
+{
+    const int rhs = blockIdx.y;
+    for (i64 i = i64(blockIdx.x) * 256 + threadIdx.x; i < n; i += i64(gridDim.x) * 256)
+        x[rhs * ld + i] += dx[rhs * ld + i];
+}
+
+__global__ void k_copy_cols(double *dst, i64 ldd, const double *src, i64 lds, i64 n)
+{
+    const int rhs = blockIdx.y;
+    for (i64 i = i64(blockIdx.x) * 256 + threadIdx.x; i < n; i += i64(gridDim.x) * 256)
+        dst[rhs * ldd + i] = src[rhs * lds + i];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+int upload_pattern(sprgpu_handle *h, const CsrPattern &pat)
+{
+    DeviceBuffers &d = h->d;
+    const i64 n = h->plan.neq, nnz = pat.rowptr[n];
+    if (d.csr_rowptr) SPRGPU_HIP(hipFree(d.csr_rowptr));
+    if (d.csr_col) SPRGPU_HIP(hipFree(d.csr_col));
+    if (d.csr_src) SPRGPU_HIP(hipFree(d.csr_src));
+    if (d.csr_val) SPRGPU_HIP(hipFree(d.csr_val));
+    d.csr_rowptr = nullptr;
+    d.csr_col = nullptr;
+    d.csr_src = nullptr;
+    d.csr_val = nullptr;
+    SPRGPU_HIP(hipMalloc(&d.csr_rowptr, sizeof(i64) * (n + 1)));
+    SPRGPU_HIP(hipMalloc(&d.csr_col, sizeof(i32) * std::max<i64>(nnz, 1)));
+    SPRGPU_HIP(hipMalloc(&d.csr_src, sizeof(i64) * std::max<i64>(nnz, 1)));
+    SPRGPU_HIP(hipMalloc(&d.csr_val, sizeof(double) * std::max<i64>(nnz, 1)));
+    SPRGPU_HIP(hipMemcpy(d.csr_rowptr, pat.rowptr.data(), sizeof(i64) * (n + 1),
+                         hipMemcpyHostToDevice));
+    if (nnz) {
+        SPRGPU_HIP(hipMemcpy(d.csr_col, pat.col.data(), sizeof(i32) * nnz, hipMemcpyHostToDevice));
+        SPRGPU_HIP(hipMemcpy(d.csr_src, pat.src.data(), sizeof(i64) * nnz, hipMemcpyHostToDevice));
+    }
+    h->device_bytes += (sizeof(i64) * 2 + sizeof(i32) + sizeof(double)) * nnz;
+    d.csr_nnz = nnz;
+    d.nnz_lower = pat.nnz_lower;
+    h->has_pattern = true;
+    return 0;
+}
+
+// gathers A's values for the last factorization's input and verifies that the
+// pattern still covers every non-zero
+static int refresh_values(sprgpu_handle *h)
+{
+    DeviceBuffers &d = h->d;
+    if (!h->has_pattern) {
+        set_error("no residual pattern: call sprgpu_set_pattern with the assembled SM first");
+        return -1;
+    }
+    if (!h->last_sm_dev) {
+        set_error("no factorization input recorded");
+        return -1;
+    }
+    unsigned long long *cnt = reinterpret_cast<unsigned long long *>(&d.status->pad[0]);
+    SPRGPU_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), h->stream));
+    k_gather_vals<<<1024, 256, 0, h->stream>>>(d.csr_rowptr, d.csr_col, d.csr_src,
+                                               h->last_sm_dev + h->plan.neq, d.csr_val,
+                                               h->plan.neq, cnt);
+    SPRGPU_HIP(hipGetLastError());
+    FactorStatus st;
+    SPRGPU_HIP(hipMemcpyAsync(&st, d.status, sizeof(st), hipMemcpyDeviceToHost, h->stream));
+    SPRGPU_HIP(hipStreamSynchronize(h->stream));
+    if (st.pad[0] != st.nnz_seen) {
+        set_error("the assembled matrix has non-zeros outside the residual pattern; "
+                  "call sprgpu_set_pattern again");
+        h->has_pattern = false;
+        return -1;
+    }
+    return 0;
+}
+
+int run_spmv(sprgpu_handle *h, const double *x, double *y, i64 ld, i64 nrhs)
+{
+    if (int rc = refresh_values(h)) return rc;
+    const i64 n = h->plan.neq;
+    dim3 grid(unsigned((n * 16 + 255) / 256), unsigned(nrhs));
+    k_spmv<<<grid, 256, 0, h->stream>>>(h->d.csr_rowptr, h->d.csr_col, h->d.csr_val, x, y,
+                                        nullptr, n, ld, 1.0);
+    SPRGPU_HIP(hipGetLastError());
+    return 0;
+}
+
+static int col_norms(sprgpu_handle *h, const double *x, i64 n, i64 ld, i64 nrhs, double *part,
+                     double *out_dev, std::vector<double> &out)
+{
+    dim3 g1(kNormBlocks, unsigned(nrhs));
+    k_sumsq_partial<<<g1, 256, 0, h->stream>>>(x, n, ld, part);
+    k_sumsq_final<<<unsigned((nrhs + 63) / 64), 64, 0, h->stream>>>(part, out_dev, int(nrhs));
+    out.resize(nrhs);
+    SPRGPU_HIP(hipMemcpyAsync(out.data(), out_dev, sizeof(double) * nrhs, hipMemcpyDeviceToHost,
+                              h->stream));
+    SPRGPU_HIP(hipStreamSynchronize(h->stream));
+    for (auto &v : out) v = std::sqrt(v);
+    return 0;
+}
+
+int run_solve_ir(sprgpu_handle *h, const double *b, double *x, i64 ld, i64 nrhs, double rtol,
+                 int maxit, sprgpu_ir_info *info)
+{
+    if (int rc = refresh_values(h)) return rc;
+    DeviceBuffers &d = h->d;
+    const i64 n = h->plan.neq;
+    hipStream_t s = h->stream;
+    // workspace: r (n*nrhs), partials, norms
+    const i64 need = n * nrhs + i64(kNormBlocks) * nrhs + nrhs;
+    if (d.irwork_cap < need) {
+        if (d.irwork) SPRGPU_HIP(hipFree(d.irwork));
+        d.irwork = nullptr;
+        SPRGPU_HIP(hipMalloc(&d.irwork, sizeof(double) * need));
+        d.irwork_cap = need;
+    }
+    double *r = d.irwork;
+    double *part = r + n * nrhs;
+    double *nrm = part + i64(kNormBlocks) * nrhs;
+    const unsigned gx = unsigned(std::min<i64>((n + 255) / 256, 1024));
+    dim3 gcol(gx, unsigned(nrhs));
+    std::vector<double> bnorm, rnorm;
+    if (int rc = col_norms(h, b, n, ld, nrhs, part, nrm, bnorm)) return rc;
+    // x = A^-1 b
+    k_copy_cols<<<gcol, 256, 0, s>>>(x, ld, b, ld, n);
+    if (int rc = run_solve(h, x, ld, nrhs, 3)) return rc;
+    dim3 gsp(unsigned((n * 16 + 255) / 256), unsigned(nrhs));
+    int it = 0;
+    double worst = 0.0;
+    for (;;) {
+        // r = b - A x
+        k_spmv<<<gsp, 256, 0, s>>>(d.csr_rowptr, d.csr_col, d.csr_val, x, r, b, n, n, -1.0);
+        if (ld != n) {
+            set_error("solve_ir requires ld == neq");
+            return -1;
+        }
+        if (int rc = col_norms(h, r, n, n, nrhs, part, nrm, rnorm)) return rc;
+        if (int rc = check_solve_error(h)) return rc;
+        worst = 0.0;
+        for (i64 k = 0; k < nrhs; ++k) {
+            const double rel = bnorm[k] > 0 ? rnorm[k] / bnorm[k] : rnorm[k];
+            worst = std::max(worst, rel);
+        }
+        if (!(worst > rtol) || it >= maxit) break;
+        if (int rc = run_solve(h, r, n, nrhs, 3)) return rc;
+        k_axpy_cols<<<gcol, 256, 0, s>>>(x, r, n, ld);
+        ++it;
+    }
+    SPRGPU_HIP(hipGetLastError());
+    if (info) {
+        info->iterations = it;
+        info->relres = worst;
+        info->converged = (worst <= rtol) ? 1 : 0;
+    }
+    return 0;
+}
+
+}  // namespace sprgpu

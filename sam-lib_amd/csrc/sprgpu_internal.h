@@ -16,27 +16,31 @@ using i64 = int64_t;
 using i32 = int32_t;
 
 // Tile and block sizes shared by the planner and the kernels.
-constexpr int kTile = 64;        // output tile edge of the MFMA update kernel
+constexpr int kTileSmall = 64;   // MFMA update tile for the rank-kPanelNB panel updates
+constexpr int kTileBig = 128;    // ... and for the rank-kPanelOuter and Schur updates in FP32
+                                 // (FP64 uses kTileSmall throughout: a 128 tile needs too
+                                 // many registers for two waves per SIMD)
+constexpr int kGemmAlign = 4;    // MFMA update tiles start on rows that are multiples of this
+constexpr int kFrontLdAlign = 16; // front leading dimensions are multiples of this (64 B in FP32)
 constexpr int kPanelNB = 32;     // columns per panel step of the blocked front factor
 constexpr int kPanelRows = 256;  // panel rows per workgroup (one row per thread)
 constexpr int kPanelOuter = 256; // outer block of the two-level panel blocking
 constexpr int kSmallF = 64;      // fronts with F <= kSmallF are factored in LDS
 constexpr int kInitCols = 8;     // front columns per workgroup in the init kernel
 constexpr int kEaCols = 8;       // child columns per workgroup in extend-add
-constexpr int kSolveBigS = 128;  // supernodes wider than this use the blocked solve
-constexpr int kSolveNB = 64;     // columns per block of the blocked solve
-constexpr int kSolveRows = 256;  // rows per workgroup in the blocked solve updates
+constexpr int kSolveSmallS = 64;   // narrow supernodes: S <= this ...
+constexpr int kSolveSmallF = 4096; // ... and F <= this: one workgroup, vector in LDS
+constexpr int kSolveNB = 64;       // column block of the wide-supernode dataflow solve
+constexpr int kSolveRows = 256;    // rectangle rows per workgroup in the wide solve
 
-// One column block of a wide supernode in the blocked solves: columns
-// [c0,c0+nb), and nchunk row chunks of kSolveRows below the block.
-struct SolveDesc { i32 node, c0, nb, nchunk; };
-
-// Launch list of the blocked solves (one list for each direction).
-struct SolveStep {
-    i32 kind;       // 0 assemble, 1 diag, 2 update/partial, 3 finish (per level)
-    i32 level;
-    i64 off, count; // range in solve_desc (kinds 1,2) or in solve_big (kinds 0,3)
-    i64 blocks;
+// A wide supernode in the dataflow solves: its diagonal part is split into
+// nblk blocks of kSolveNB columns (one workgroup each, ordered so that every
+// workgroup only waits for lower-indexed ones), and in the forward sweep its
+// F-S rectangle rows into nrect chunks of kSolveRows.  wg0 is the index of
+// its first workgroup in the level's launch; flag0 its first ready flag.
+struct WideDesc {
+    i32 node, nblk, nrect, pad;
+    i64 wg0_fwd, wg0_bwd, flag0;
 };
 
 // One supernode as the device sees it.  Rows of the front are the supernode's
@@ -58,13 +62,16 @@ struct NodeInfo {
 struct ColTask { i32 node, c0, c1, pad; };        // front columns [c0,c1)
 struct EaTask { i32 child, c0, c1, pad; };        // child update columns [c0,c1)
 struct PanelDesc { i32 node, k0, nb, nrb; };       // panel step at column k0, nrb row blocks
-// Lower-trapezoidal tile set: rows [r0,rend) x cols [r0,cend) (tiles of
-// kTile anchored at r0, only tiles with ti>=tj), updated with columns
+// Lower-trapezoidal tile set: rows [r0,rend) x cols [r0,cend) (square tiles
+// of the launch's tile edge anchored at r0, only tiles with ti>=tj), updated
+// with columns
 // [k0,k0+kn) of the same front.
 struct GemmDesc {
     i64 foff;
     i32 ld, r0, rend, cend, k0, kn;
     i32 nti, ntj;  // tile rows / tile cols
+    i32 first;     // first equation of the front (pivots d come from the D array)
+    i32 pad;
 };
 
 enum LaunchKind : i32 {
@@ -82,12 +89,15 @@ struct Launch {
     i64 off, count;  // range in the task array of this kind
     i64 blocks;      // grid size
     double flops;    // algorithmic flops of the launch (multiply-add = 2)
+    i32 tile;        // MFMA update tile edge (L_GEMM)
+    i32 outer;       // L_GEMM: 1 for rank-kPanelOuter and Schur updates, 0 for rank-kPanelNB
 };
 
 struct Plan {
     i64 neq = 0, nsuper = 0, nofsub = 0, nzerol = 0, mfront = 0, maxsup = 0;
     i64 pool_elems = 0, vwork_elems = 0;
     i32 nlevels = 0;
+    i32 big_tile = 0;               // MFMA tile of the rank-kPanelOuter and Schur updates
     std::vector<NodeInfo> nodes;
     std::vector<i32> children;
     std::vector<i32> lindx;         // 0-based equations, NOFSUB
@@ -103,18 +113,22 @@ struct Plan {
     std::vector<i64> gemm_prefix;   // per launch-local prefix of tiles (exclusive)
     std::vector<Launch> launches;
     double factor_flops = 0;        // RINFO(3) formula, sprtrs.f:1961-1969
-    // solve schedule: narrow supernodes one workgroup each, wide ones blocked
-    std::vector<i32> solve_small, solve_big;          // grouped by level
-    std::vector<i32> solve_small_ptr, solve_big_ptr;  // nlevels+1
-    std::vector<SolveDesc> fwd_desc, bwd_desc;
-    std::vector<i64> fwd_prefix, bwd_prefix;          // launch-local chunk prefixes
-    std::vector<SolveStep> fwd_steps, bwd_steps;      // in execution order
-    i64 max_partial_chunks = 0;                       // backward partial-sum scratch
+    // solve schedule: narrow supernodes one workgroup each, wide ones by a
+    // dataflow kernel per level
+    std::vector<i32> solve_small, solve_small_ptr;    // grouped by level
+    std::vector<i32> wide_nodes;                      // grouped by level
+    std::vector<WideDesc> wide;                       // grouped by level
+    std::vector<i64> wide_fwd_prefix, wide_bwd_prefix;// launch-local workgroup prefixes
+    std::vector<i32> wide_ptr;                        // nlevels+1
+    std::vector<i64> wide_fwd_wgs, wide_bwd_wgs;      // workgroups per level
+    i64 nflags = 0;                                   // ready flags per right-hand side
 };
 
-// Builds the plan from the reference symbolic arrays (1-based, INTEGER*8).
-// Returns an empty string or an error message.
-std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, Plan &plan);
+// Builds the plan from the reference symbolic arrays (1-based, INTEGER*8)
+// for a factor in the given precision (SPRGPU_FP64 / SPRGPU_FP32: the MFMA
+// update tiles differ).  Returns an empty string or an error message.
+std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, int precision,
+                       Plan &plan);
 
 // Pattern of A used by the FP64 residual: CSR over all rows, with the
 // position of each value in the packed LNZ array.
@@ -152,6 +166,7 @@ struct DeviceBuffers {
     double *dcor = nullptr;     // DCOR(2,NEQ) for SPRFC3 mode
     FactorStatus *status = nullptr;
     double *stats = nullptr;    // reduction scratch for TOL(2)/TOL(3)/NUMNEG
+    void *dscr = nullptr;       // factored diagonal blocks of the panel steps, T[NEQ*kPanelNB]
     double *vwork = nullptr;    // solve workspace (vwork_elems * nrhs_cap)
     i64 vwork_cap = 0;
     // residual pattern
@@ -161,11 +176,13 @@ struct DeviceBuffers {
     double *csr_val = nullptr;
     i64 csr_nnz = 0, nnz_lower = 0;
     double *irwork = nullptr;   // IR vectors
-    i32 *solve_small = nullptr, *solve_big = nullptr;
-    SolveDesc *fwd_desc = nullptr, *bwd_desc = nullptr;
-    i64 *fwd_prefix = nullptr, *bwd_prefix = nullptr;
-    double *partial = nullptr;  // backward partial sums (max_partial_chunks*kSolveNB*nrhs)
-    i64 partial_cap = 0;
+    i32 *solve_small = nullptr, *wide_nodes = nullptr;
+    WideDesc *wide = nullptr;
+    i64 *wide_fwd_prefix = nullptr, *wide_bwd_prefix = nullptr;
+    unsigned *flags = nullptr;  // ready flags (nflags * nrhs), compared with an epoch
+    i64 flags_cap = 0;
+    unsigned epoch = 0;
+    int *spin_error = nullptr;  // set when a dataflow wait gave up
     i64 irwork_cap = 0;
 };
 
@@ -203,6 +220,7 @@ int profile_factor(sprgpu_handle *h, const double *sm_dev, int mode, const doubl
                    double *out);
 // solve.hip
 int run_solve(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which);
+int check_solve_error(sprgpu_handle *h);
 int run_spmv(sprgpu_handle *h, const double *x, double *y, i64 ld, i64 nrhs);
 int run_solve_ir(sprgpu_handle *h, const double *b, double *x, i64 ld, i64 nrhs, double rtol,
                  int maxit, sprgpu_ir_info *info);
