@@ -41,8 +41,9 @@ static void free_all(sprgpu_handle *h)
                     d.status,   d.stats,      d.vwork,       d.csr_rowptr, d.csr_col,
                     d.dscr,
                     d.csr_src,  d.csr_val,    d.irwork,      d.level_nodes,
-                    d.solve_small, d.wide_nodes, d.wide,     d.wide_fwd_prefix,
-                    d.wide_bwd_prefix, d.flags, d.spin_error};
+                    d.solve_small, d.wide,     d.wide_fwd_prefix, d.wide_bwd_prefix,
+                    d.wide_rect_prefix, d.upd_ptr, d.upd_src, d.flags, d.partials,
+                    d.spin_error};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     d = DeviceBuffers{};
@@ -89,9 +90,11 @@ int sprgpu_create(const sprgpu_int *mspar, const sprgpu_int *mtrees, const sprgp
         (rc = upload(d.gemm_prefix, p.gemm_prefix, B)) ||
         (rc = upload(d.level_nodes, p.level_nodes, B)) ||
         (rc = upload(d.solve_small, p.solve_small, B)) ||
-        (rc = upload(d.wide_nodes, p.wide_nodes, B)) || (rc = upload(d.wide, p.wide, B)) ||
+        (rc = upload(d.wide, p.wide, B)) ||
         (rc = upload(d.wide_fwd_prefix, p.wide_fwd_prefix, B)) ||
-        (rc = upload(d.wide_bwd_prefix, p.wide_bwd_prefix, B))) {
+        (rc = upload(d.wide_bwd_prefix, p.wide_bwd_prefix, B)) ||
+        (rc = upload(d.wide_rect_prefix, p.wide_rect_prefix, B)) ||
+        (rc = upload(d.upd_ptr, p.upd_ptr, B)) || (rc = upload(d.upd_src, p.upd_src, B))) {
         free_all(h.get());
         return rc;
     }

@@ -23,6 +23,7 @@
 #include <cfloat>
 #include <climits>
 
+#include "lanes.h"
 #include "sprgpu_internal.h"
 
 namespace sprgpu {
@@ -239,34 +240,8 @@ __global__ __launch_bounds__(256) void k_small(const i32 *__restrict__ list,
 
 // ---------------------------------------------------------------------------
 // panel step: factor the nb x nb diagonal block at column k0 and solve the
-// rows below it (L21 = A21 L11^-T D^-1), one row per thread
+// rows below it (L21 = A21 L11^-T D^-1)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int find_desc(const i64 *prefix, int n, i64 b)
-{
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (prefix[mid] <= b)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    return lo;
-}
-
-// Value of lane l's v in every lane (l uniform).
-__device__ __forceinline__ float lane_bcast(float v, int l)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
-}
-__device__ __forceinline__ double lane_bcast(double v, int l)
-{
-    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-    const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(unsigned(u)), l));
-    const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(unsigned(u >> 32)), l));
-    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
-}
-
 // Panel step: columns [k0, k0+nb) of a blocked front, nb <= kPanelNB.
 // Every workgroup factors the nb x nb diagonal block itself: one wavefront,
 // lane r holding row r in registers, right-looking over the columns with the

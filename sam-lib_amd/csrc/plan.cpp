@@ -299,44 +299,74 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
     // solve schedule
     // ----------------------------------------------------------------------
     p.solve_small.clear();
-    p.wide_nodes.clear();
     p.wide.clear();
     p.wide_fwd_prefix.clear();
     p.wide_bwd_prefix.clear();
+    p.wide_rect_prefix.clear();
     p.solve_small_ptr.assign(p.nlevels + 1, 0);
+    p.small_maxF.assign(p.nlevels, 0);
+    p.small_maxS.assign(p.nlevels, 0);
     p.wide_ptr.assign(p.nlevels + 1, 0);
     p.wide_fwd_wgs.assign(p.nlevels, 0);
     p.wide_bwd_wgs.assign(p.nlevels, 0);
+    p.wide_rect_wgs.assign(p.nlevels, 0);
     p.nflags = 0;
+    p.npart = 0;
     for (int l = 0; l < p.nlevels; ++l) {
-        i64 wf = 0, wb = 0;
+        i64 wf = 0, wb = 0, wr = 0;
         for (i32 q = p.level_ptr[l]; q < p.level_ptr[l + 1]; ++q) {
             const i32 j = p.level_nodes[q];
             const NodeInfo &n = p.nodes[j];
             if (n.S <= kSolveSmallS && n.F <= kSolveSmallF) {
                 p.solve_small.push_back(j);
+                p.small_maxF[l] = std::max(p.small_maxF[l], n.F);
+                p.small_maxS[l] = std::max(p.small_maxS[l], n.S);
                 continue;
             }
             WideDesc w;
             w.node = j;
             w.nblk = (n.S + kSolveNB - 1) / kSolveNB;
-            w.nrect = (n.F - n.S + kSolveRows - 1) / kSolveRows;
+            w.nrect = (n.F - n.S + kSolveNB - 1) / kSolveNB;
             w.pad = 0;
-            w.wg0_fwd = wf;
-            w.wg0_bwd = wb;
             w.flag0 = p.nflags;
+            w.pbase = p.npart;
             p.wide.push_back(w);
-            p.wide_nodes.push_back(j);
             p.wide_fwd_prefix.push_back(wf);
             p.wide_bwd_prefix.push_back(wb);
+            p.wide_rect_prefix.push_back(wr);
             wf += w.nblk + w.nrect;
             wb += w.nblk;
+            wr += i64(w.nblk) * w.nrect;
             p.nflags += w.nblk;
+            p.npart += i64(w.nblk) * w.nrect * kSolveNB;
         }
         p.solve_small_ptr[l + 1] = i32(p.solve_small.size());
         p.wide_ptr[l + 1] = i32(p.wide.size());
         p.wide_fwd_wgs[l] = wf;
         p.wide_bwd_wgs[l] = wb;
+        p.wide_rect_wgs[l] = wr;
+    }
+    // gather form of the forward assembly (inverse of the child maps)
+    if (p.vwork_elems >= (i64(1) << 31)) return "solve workspace too large for 32-bit indices";
+    p.upd_ptr.assign(p.vwork_elems + 1, 0);
+    for (i64 c = 0; c < ns; ++c) {
+        const NodeInfo &n = p.nodes[c];
+        if (n.parent < 0) continue;
+        const i64 pv = p.nodes[n.parent].voff;
+        for (i32 i = 0; i < n.F - n.S; ++i) p.upd_ptr[pv + p.relmap[n.mapoff + i] + 1]++;
+    }
+    for (i64 i = 0; i < p.vwork_elems; ++i) p.upd_ptr[i + 1] += p.upd_ptr[i];
+    p.upd_src.assign(p.upd_ptr[p.vwork_elems], 0);
+    {
+        std::vector<i64> fill(p.upd_ptr.begin(), p.upd_ptr.end() - 1);
+        for (i64 j = 0; j < ns; ++j) {  // parents in order, children in SIBL order
+            const NodeInfo &par = p.nodes[j];
+            for (i32 k = 0; k < par.nchild; ++k) {
+                const NodeInfo &n = p.nodes[p.children[par.child0 + k]];
+                for (i32 i = 0; i < n.F - n.S; ++i)
+                    p.upd_src[fill[par.voff + p.relmap[n.mapoff + i]]++] = i32(n.voff + n.S + i);
+            }
+        }
     }
     return "";
 }

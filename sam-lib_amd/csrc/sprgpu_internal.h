@@ -29,18 +29,17 @@ constexpr int kSmallF = 64;      // fronts with F <= kSmallF are factored in LDS
 constexpr int kInitCols = 8;     // front columns per workgroup in the init kernel
 constexpr int kEaCols = 8;       // child columns per workgroup in extend-add
 constexpr int kSolveSmallS = 64;   // narrow supernodes: S <= this ...
-constexpr int kSolveSmallF = 4096; // ... and F <= this: one workgroup, vector in LDS
-constexpr int kSolveNB = 64;       // column block of the wide-supernode dataflow solve
-constexpr int kSolveRows = 256;    // rectangle rows per workgroup in the wide solve
+constexpr int kSolveSmallF = 3072; // ... and F <= this: one workgroup, vector in LDS
+constexpr int kSolveNB = 128;      // column block (and rectangle row chunk) of the wide solve
 
 // A wide supernode in the dataflow solves: its diagonal part is split into
 // nblk blocks of kSolveNB columns (one workgroup each, ordered so that every
-// workgroup only waits for lower-indexed ones), and in the forward sweep its
-// F-S rectangle rows into nrect chunks of kSolveRows.  wg0 is the index of
-// its first workgroup in the level's launch; flag0 its first ready flag.
+// workgroup only waits for lower-indexed ones) and its F-S rectangle rows
+// into nrect chunks of kSolveNB rows.  flag0 is its first ready flag, pbase
+// the offset of its nblk*nrect*kSolveNB backward partial sums.
 struct WideDesc {
     i32 node, nblk, nrect, pad;
-    i64 wg0_fwd, wg0_bwd, flag0;
+    i64 flag0, pbase;
 };
 
 // One supernode as the device sees it.  Rows of the front are the supernode's
@@ -116,12 +115,18 @@ struct Plan {
     // solve schedule: narrow supernodes one workgroup each, wide ones by a
     // dataflow kernel per level
     std::vector<i32> solve_small, solve_small_ptr;    // grouped by level
-    std::vector<i32> wide_nodes;                      // grouped by level
+    std::vector<i32> small_maxF, small_maxS;          // per level, for the LDS size
     std::vector<WideDesc> wide;                       // grouped by level
-    std::vector<i64> wide_fwd_prefix, wide_bwd_prefix;// launch-local workgroup prefixes
+    std::vector<i64> wide_fwd_prefix, wide_bwd_prefix, wide_rect_prefix;  // launch-local
     std::vector<i32> wide_ptr;                        // nlevels+1
-    std::vector<i64> wide_fwd_wgs, wide_bwd_wgs;      // workgroups per level
+    std::vector<i64> wide_fwd_wgs, wide_bwd_wgs, wide_rect_wgs;  // workgroups per level
     i64 nflags = 0;                                   // ready flags per right-hand side
+    i64 npart = 0;                                    // backward partial sums per RHS
+    // forward assembly as a gather: entry i of the solve workspace (node
+    // n's local row i - n.voff) receives vwork[upd_src[k]] for k in
+    // [upd_ptr[i], upd_ptr[i+1]), the children's update entries in SIBL order
+    std::vector<i64> upd_ptr;
+    std::vector<i32> upd_src;
 };
 
 // Builds the plan from the reference symbolic arrays (1-based, INTEGER*8)
@@ -176,11 +181,15 @@ struct DeviceBuffers {
     double *csr_val = nullptr;
     i64 csr_nnz = 0, nnz_lower = 0;
     double *irwork = nullptr;   // IR vectors
-    i32 *solve_small = nullptr, *wide_nodes = nullptr;
+    i32 *solve_small = nullptr;
     WideDesc *wide = nullptr;
-    i64 *wide_fwd_prefix = nullptr, *wide_bwd_prefix = nullptr;
+    i64 *wide_fwd_prefix = nullptr, *wide_bwd_prefix = nullptr, *wide_rect_prefix = nullptr;
+    i64 *upd_ptr = nullptr;
+    i32 *upd_src = nullptr;
     unsigned *flags = nullptr;  // ready flags (nflags * nrhs), compared with an epoch
     i64 flags_cap = 0;
+    double *partials = nullptr; // backward partial sums (npart * nrhs)
+    i64 partials_cap = 0;
     unsigned epoch = 0;
     int *spin_error = nullptr;  // set when a dataflow wait gave up
     i64 irwork_cap = 0;

@@ -4,38 +4,51 @@
 // SPRBS1/SPRBS4/SPRBS6 (sprbs1.f:5-574).  The reference walks the supernodes
 // one at a time, one right-hand side at a time, gathering B(LINDX) into a
 // local vector and scattering the updated entries back by assignment.  Here
-// the supernodes of one tree height run together:
+// the supernodes of one tree height run together, one launch per kind:
 //
 // * Narrow supernodes (S <= kSolveSmallS, F <= kSolveSmallF) get one
-//   workgroup each with the local vector in LDS and the unit-triangular part
-//   solved by a single wavefront with lane shuffles.
+//   workgroup each with the local vector in LDS (sized for the level) and
+//   the unit-triangular part solved by one wavefront with readlane
+//   broadcasts.
 // * Wide supernodes are split into kSolveNB-column blocks, one workgroup per
-//   block, plus workgroups for the rows below the diagonal part (forward).  A
-//   block's workgroup applies the finished blocks it depends on as their
-//   ready flags appear, then solves its diagonal block and publishes it.
+//   block, plus (forward) one workgroup per kSolveNB rows below the diagonal
+//   part.  A block's workgroup loads the L values of each block it depends
+//   on before waiting for that block's ready flag, applies it as a GEMV with
+//   two threads per row, and finally solves its own 128 x 128 unit-triangular
+//   block as four 32 x 32 sub-blocks: a readlane chain over registers on one
+//   wavefront per sub-block, the updates between sub-blocks in parallel.
 //   Every workgroup only waits for lower-indexed workgroups of the same
 //   launch, which the hardware dispatches first, and each wait is bounded.
+//   In the backward sweep the products with the rows below the diagonal part
+//   (known on entry) are taken by a separate, fully parallel launch into
+//   partial sums that the block workgroups add in a fixed order.
 //
 // The forward sweep hands each supernode's update vector (its F-S rows below
-// the diagonal block) to the parent through the child->parent relative map,
-// the same map the factor's extend-add uses, so no two workgroups write the
+// the diagonal block) to the parent as a gather: entry i of the parent's
+// vector sums the children's entries that map to it, in SIBL order (the same
+// relative map the factor's extend-add uses), so no two workgroups write the
 // same entry and every sum is taken in a fixed order: results do not depend
 // on scheduling.  Vectors are FP64 whatever the factor precision; the
 // division by d is a true division (sprfs1.f:622-625).
+#include "lanes.h"
 #include "sprgpu_internal.h"
 
 namespace sprgpu {
 
-__device__ __forceinline__ double wave_sum(double t)
-{
-    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
-    return t;
-}
-
 constexpr unsigned long long kSpinLimit = 1ull << 24;
+constexpr int NB = kSolveNB;
+constexpr int NH = NB / 2;  // columns (forward) or rows (backward) per thread of a block GEMV
+static_assert(NB == 128, "two threads per row of a block; one 32-row sub-block per wavefront");
 
-// One thread waits until *flag reaches epoch (bounded); then the workgroup
-// acquires.  Returns false if the wait gave up.
+// Block results cross workgroups (and XCDs, whose L2 caches are not kept
+// coherent with each other) as agent-scope atomic stores and loads, which
+// are coherent on their own; the publishing wavefront waits for its stores
+// to complete (s_waitcnt) before it stores the ready flag, and readers load
+// the results only after seeing the flag.  This avoids the agent-scope
+// fences, which write back or invalidate a whole L2.
+
+// One thread waits until *flag reaches epoch (bounded), then the workgroup
+// synchronises.  A wait that gives up sets *err and continues.
 __device__ __forceinline__ void wait_ready(const unsigned *flag, unsigned epoch, int *err)
 {
     if (threadIdx.x == 0) {
@@ -49,358 +62,529 @@ __device__ __forceinline__ void wait_ready(const unsigned *flag, unsigned epoch,
         }
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// Called by every lane of the wavefront that wrote the block.
+__device__ __forceinline__ void store_shared(double *p, double x)
+{
+    __hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double load_shared(const double *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Called by every lane of the wavefront that stored the block (store_shared).
 __device__ __forceinline__ void publish_ready(unsigned *flag, unsigned epoch)
 {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);  // vmcnt = expcnt = lgkmcnt = 0: the stores have completed
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
     if ((threadIdx.x & 63) == 0)
         __hip_atomic_store(flag, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ int find_wide(const i64 *prefix, int n, i64 b)
+// Offset of column c in a packed lower-triangular NB x NB block stored by
+// columns (column c holds rows c..NB-1).
+__device__ __forceinline__ int packed_col(int c) { return c * NB - c * (c - 1) / 2; }
+// The packed blocks in LDS carry one extra zero entry: a read of an entry
+// outside the block selects this address instead of masking the loaded value
+// (a select on a loaded value makes the compiler wait for each load).
+constexpr int kPacked = NB * (NB + 1) / 2;
+constexpr int kZero = kPacked;
+
+// sum of p[k*stride] for k = k0, k0+step, ... < n in that order; the loads
+// are issued eight at a time (a loop with a run-time trip count would wait
+// for each load in turn)
+__device__ __forceinline__ double strided_sum(const double *p, i64 stride, int k0, int step, int n)
 {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (prefix[mid] <= b)
-            lo = mid;
-        else
-            hi = mid - 1;
+    double s = 0.0;
+    int k = k0;
+    for (; k + 7 * step < n; k += 8 * step) {
+        double t[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t[u] = p[i64(k + u * step) * stride];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += t[u];
     }
-    return lo;
+    for (; k < n; k += step) s += p[i64(k) * stride];
+    return s;
 }
 
 // ---------------------------------------------------------------------------
-// narrow supernodes: S <= kSolveSmallS (<= 64), F <= kSolveSmallF
+// narrow supernodes: S <= kSolveSmallS, F <= kSolveSmallF
 // ---------------------------------------------------------------------------
+// Dynamic LDS: v[maxF] (FP64), then L11 (S x S, T).
 template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_small(const i32 *__restrict__ list,
-                                                   const NodeInfo *__restrict__ nodes,
-                                                   const i32 *__restrict__ children,
-                                                   const i32 *__restrict__ lindx,
-                                                   const i32 *__restrict__ relmap,
-                                                   const T *__restrict__ pool,
-                                                   const double *__restrict__ dvec, double *b,
-                                                   i64 ldb, double *vwork, i64 vstride)
+__global__ __launch_bounds__(256) void k_fwd_narrow(const i32 *__restrict__ list, int maxF,
+                                                    const NodeInfo *__restrict__ nodes,
+                                                    const i32 *__restrict__ lindx,
+                                                    const i64 *__restrict__ upd_ptr,
+                                                    const i32 *__restrict__ upd_src,
+                                                    const T *__restrict__ pool,
+                                                    const double *__restrict__ dvec, double *b,
+                                                    i64 ldb, double *vwork, i64 vstride)
 {
-    __shared__ double v[kSolveSmallF];
-    __shared__ T Lb[kSolveNB][kSolveNB + 1];
+    extern __shared__ double smem[];
+    __shared__ double vx[64];  // x of the S columns, zero past S
+    double *v = smem;
+    T *Lb = reinterpret_cast<T *>(smem + maxF);  // Lb[c*S + r] = L(r,c), r > c
     const NodeInfo n = nodes[list[blockIdx.x]];
-    const int rhs = blockIdx.y, tid = threadIdx.x, lane = tid & 63;
+    const int rhs = blockIdx.y, tid = threadIdx.x, nt = blockDim.x, lane = tid & 63;
     const int S = n.S, F = n.F;
-    double *vg = vwork + rhs * vstride + n.voff;
+    double *vw = vwork + rhs * vstride;
     double *bb = b + rhs * ldb;
     const i32 *rows = lindx + n.lxoff;
     const T *L = pool + n.foff;
     const i64 ld = n.ld;
-    for (int i = tid; i < F; i += 256) v[i] = (i < S) ? bb[rows[i]] : 0.0;
-    for (int idx = tid; idx < S * S; idx += 256) {
+    const i64 *up = upd_ptr + n.voff;
+    for (int i = tid; i < F; i += nt) {
+        double s = (i < S) ? bb[rows[i]] : 0.0;
+        for (i64 k = up[i]; k < up[i + 1]; ++k) s += vw[upd_src[k]];
+        v[i] = s;
+    }
+    for (int idx = tid; idx < S * S; idx += nt) {
         const int r = idx % S, c = idx / S;
-        if (r > c) Lb[r][c] = L[r + c * ld];
+        if (r > c) Lb[idx] = L[r + c * ld];
     }
     __syncthreads();
-    for (int ci = 0; ci < n.nchild; ++ci) {
-        const NodeInfo c = nodes[children[n.child0 + ci]];
-        const double *vc = vwork + rhs * vstride + c.voff + c.S;
-        const i32 *map = relmap + c.mapoff;
-        for (int i = tid; i < c.F - c.S; i += 256) v[map[i]] += vc[i];
-        __syncthreads();
-    }
     if (tid < 64) {
-        // unit lower L11, row lane in a register
+        // unit lower L11: lane r holds x_r; column c is final when reached
+        // columns are read from LDS eight at a time (in bounds for every
+        // lane) ahead of the dependent readlane/FMA chain
         double x = (lane < S) ? v[lane] : 0.0;
-        for (int c = 0; c < S - 1; ++c) {
-            const double xc = __shfl(x, c);
-            if (lane > c && lane < S) x -= double(Lb[lane][c]) * xc;
+        const int lc = min(lane, S - 1);
+        for (int cb = 0; cb + 1 < S; cb += 8) {
+            T la[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) la[u] = Lb[min(cb + u, S - 1) * S + lc];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int c = cb + u;
+                if (c + 1 < S) {
+                    const double xc = lane_bcast(x, c);
+                    x -= ((lane > c && lane < S) ? double(la[u]) : 0.0) * xc;
+                }
+            }
         }
-        if (lane < S) v[lane] = x;
+        if (lane < S) {
+            v[lane] = x;
+            bb[rows[lane]] = x / dvec[n.first + lane];
+        }
     }
+    if (tid < 64) vx[tid] = (tid < S) ? v[tid] : 0.0;
     __syncthreads();
-    for (int i = S + tid; i < F; i += 256) {
+    // update vector for the parent: s_i = v_i - sum_{c<S} L(i,c) x_c, the
+    // columns in batches of 16 (clamped loads, vx is zero past S)
+    for (int i = S + tid; i < F; i += nt) {
         double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-        int c = 0;
-        for (; c + 3 < S; c += 4) {
-            s0 += double(L[i + c * ld]) * v[c];
-            s1 += double(L[i + (c + 1) * ld]) * v[c + 1];
-            s2 += double(L[i + (c + 2) * ld]) * v[c + 2];
-            s3 += double(L[i + (c + 3) * ld]) * v[c + 3];
+        const T *Li = L + i;
+        for (int cb = 0; cb < S; cb += 16) {
+            T lv[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) lv[u] = Li[i64(min(cb + u, S - 1)) * ld];
+#pragma unroll
+            for (int u = 0; u < 16; u += 4) {
+                s0 += double(lv[u]) * vx[cb + u];
+                s1 += double(lv[u + 1]) * vx[cb + u + 1];
+                s2 += double(lv[u + 2]) * vx[cb + u + 2];
+                s3 += double(lv[u + 3]) * vx[cb + u + 3];
+            }
         }
-        for (; c < S; ++c) s0 += double(L[i + c * ld]) * v[c];
-        vg[i] = v[i] - ((s0 + s1) + (s2 + s3));  // update vector handed to the parent
+        vw[n.voff + i] = v[i] - ((s0 + s1) + (s2 + s3));
     }
-    for (int c = tid; c < S; c += 256) bb[rows[c]] = v[c] / dvec[n.first + c];
 }
 
+// Dynamic LDS: v[maxF] (FP64), part[4*64] (FP64), then L11 (S x S, T).
 template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_small(const i32 *__restrict__ list,
-                                                   const NodeInfo *__restrict__ nodes,
-                                                   const i32 *__restrict__ lindx,
-                                                   const T *__restrict__ pool, double *b,
-                                                   i64 ldb)
+__global__ __launch_bounds__(256) void k_bwd_narrow(const i32 *__restrict__ list, int maxF,
+                                                    const NodeInfo *__restrict__ nodes,
+                                                    const i32 *__restrict__ lindx,
+                                                    const T *__restrict__ pool, double *b,
+                                                    i64 ldb)
 {
-    __shared__ double v[kSolveSmallF];
-    __shared__ T Lb[kSolveNB][kSolveNB + 1];
-    __shared__ double part[4][64];
+    extern __shared__ double smem[];
+    double *v = smem;                 // F entries and 16 zeros
+    double *part = smem + maxF + 16;
+    T *Lb = reinterpret_cast<T *>(smem + maxF + 16 + 256);  // Lb[r*S + c] = L(r,c), r > c
     const NodeInfo n = nodes[list[blockIdx.x]];
-    const int rhs = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int rhs = blockIdx.y, tid = threadIdx.x, nt = blockDim.x, lane = tid & 63;
+    const int wave = tid >> 6, nw = nt >> 6;
     const int S = n.S, F = n.F;
     double *bb = b + rhs * ldb;
     const i32 *rows = lindx + n.lxoff;
     const T *L = pool + n.foff;
     const i64 ld = n.ld;
-    for (int i = tid; i < F; i += 256) v[i] = bb[rows[i]];
-    for (int idx = tid; idx < S * S; idx += 256) {
+    for (int i = tid; i < F + 16; i += nt) v[i] = (i < F) ? bb[rows[i]] : 0.0;
+    for (int idx = tid; idx < S * S; idx += nt) {
         const int r = idx % S, c = idx / S;
-        if (r > c) Lb[r][c] = L[r + c * ld];
+        if (r > c) Lb[r * S + c] = L[r + c * ld];
     }
     __syncthreads();
-    // x1 -= L21^T x2: lane = column, the four waves interleave the rows
+    // x1 -= L21^T x2: lane = column, the waves split the rows in multiples
+    // of 16 that are read 16 at a time (clamped loads, v is zero past F)
     double acc = 0.0;
-    if (lane < S) {
-        const T *Lcol = L + i64(lane) * ld;
-        const int nrw = F - S, per = (nrw + 3) / 4;
+    {
+        const T *Lcol = L + i64(min(lane, S - 1)) * ld;
+        const int per = ((F - S + nw - 1) / nw + 15) / 16 * 16;
         const int ra = S + wave * per, rz = min(F, ra + per);
         double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-        int r = ra;
-        for (; r + 3 < rz; r += 4) {
-            a0 += double(Lcol[r]) * v[r];
-            a1 += double(Lcol[r + 1]) * v[r + 1];
-            a2 += double(Lcol[r + 2]) * v[r + 2];
-            a3 += double(Lcol[r + 3]) * v[r + 3];
+        for (int r = ra; r < rz; r += 16) {
+            T lv[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) lv[u] = Lcol[min(r + u, F - 1)];
+#pragma unroll
+            for (int u = 0; u < 16; u += 4) {
+                a0 += double(lv[u]) * v[r + u];
+                a1 += double(lv[u + 1]) * v[r + u + 1];
+                a2 += double(lv[u + 2]) * v[r + u + 2];
+                a3 += double(lv[u + 3]) * v[r + u + 3];
+            }
         }
-        for (; r < rz; ++r) a0 += double(Lcol[r]) * v[r];
         acc = (a0 + a1) + (a2 + a3);
     }
-    part[wave][lane] = acc;
+    part[wave * 64 + lane] = acc;
     __syncthreads();
     if (tid < 64) {
-        // L11^T x1 = x1 by columns from the last: x_c -= l_rc x_r for c < r
-        double x = (lane < S)
-                       ? v[lane] - (((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane])
-                       : 0.0;
-        for (int r = S - 1; r > 0; --r) {
-            const double xr = __shfl(x, r);
-            if (lane < r) x -= double(Lb[r][lane]) * xr;
+        // L11^T x1 = y1 by columns from the last: x_c -= l_rc x_r for c < r
+        double x = 0.0;
+        if (lane < S) {
+            double sp = part[lane];
+            for (int k = 1; k < nw; ++k) sp += part[k * 64 + lane];
+            x = v[lane] - sp;
+        }
+        const int lc = min(lane, S - 1);
+        for (int rb = S - 1; rb > 0; rb -= 8) {
+            T la[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) la[u] = Lb[max(rb - u, 1) * S + lc];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int r = rb - u;
+                if (r > 0) {
+                    const double xr = lane_bcast(x, r);
+                    x -= ((lane < r) ? double(la[u]) : 0.0) * xr;
+                }
+            }
         }
         if (lane < S) bb[rows[lane]] = x;
     }
 }
 
 // ---------------------------------------------------------------------------
-// wide supernodes
+// wide supernodes, forward
 // ---------------------------------------------------------------------------
-// v = [B(rows 0..S-1); 0] + sum_children extend(u_child)
-__global__ __launch_bounds__(256) void k_fwd_assemble(const i32 *__restrict__ list,
-                                                      const NodeInfo *__restrict__ nodes,
-                                                      const i32 *__restrict__ children,
-                                                      const i32 *__restrict__ lindx,
-                                                      const i32 *__restrict__ relmap,
-                                                      const double *b, i64 ldb, double *vwork,
-                                                      i64 vstride)
-{
-    const NodeInfo n = nodes[list[blockIdx.x]];
-    const int rhs = blockIdx.y, tid = threadIdx.x;
-    double *v = vwork + rhs * vstride + n.voff;
-    const double *bb = b + rhs * ldb;
-    const i32 *rows = lindx + n.lxoff;
-    for (int i = tid; i < n.F; i += 256) v[i] = (i < n.S) ? bb[rows[i]] : 0.0;
-    __syncthreads();
-    for (int ci = 0; ci < n.nchild; ++ci) {
-        const NodeInfo c = nodes[children[n.child0 + ci]];
-        const double *vc = vwork + rhs * vstride + c.voff + c.S;
-        const i32 *map = relmap + c.mapoff;
-        for (int i = tid; i < c.F - c.S; i += 256) v[map[i]] += vc[i];
-        __syncthreads();
-    }
-}
-
-// Forward dataflow: workgroup q < nblk solves diagonal block q after applying
-// blocks 0..q-1; workgroups nblk.. reduce the rectangle rows.  The L values a
-// workgroup needs are loaded before it waits, so only the dependent
-// arithmetic is on the critical path.
+// Workgroup local < nblk solves diagonal block local after applying blocks
+// 0..local-1; workgroups nblk.. compute the update vector of kSolveNB rows
+// below the diagonal part, applying every block as it finishes.
 template <typename T>
 __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ wide, int nwide,
                                                   const i64 *__restrict__ prefix,
                                                   const NodeInfo *__restrict__ nodes,
                                                   const i32 *__restrict__ lindx,
+                                                  const i64 *__restrict__ upd_ptr,
+                                                  const i32 *__restrict__ upd_src,
                                                   const T *__restrict__ pool,
                                                   const double *__restrict__ dvec, double *b,
                                                   i64 ldb, double *vwork, i64 vstride,
                                                   unsigned *flags, i64 nflags, unsigned epoch,
                                                   int *err)
 {
-    __shared__ double xs[kSolveNB];
-    __shared__ double part[4][64];
-    __shared__ T Lb[kSolveNB][kSolveNB + 1];
-    const int q = find_wide(prefix, nwide, blockIdx.x);
+    __shared__ double xs[NB];
+    __shared__ double ys[NB];
+    __shared__ double part[2][NB];
+    __shared__ T Ld[kPacked + 1];  // packed lower diagonal block, by columns; Ld[kZero] = 0
+    const int q = find_desc(prefix, nwide, blockIdx.x);
     const WideDesc w = wide[q];
     const int local = int(blockIdx.x - prefix[q]);
     const NodeInfo n = nodes[w.node];
     const int rhs = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int S = n.S, F = n.F;
-    double *v = vwork + rhs * vstride + n.voff;
+    double *vw = vwork + rhs * vstride;
+    double *v = vw + n.voff;
+    double *bb = b + rhs * ldb;
+    const i32 *rows = lindx + n.lxoff;
     const T *L = pool + n.foff;
     const i64 ld = n.ld;
+    const i64 *up = upd_ptr + n.voff;
     unsigned *fl = flags + rhs * nflags + w.flag0;
-    if (local < w.nblk) {
-        const int r0 = local * kSolveNB;
-        const int nr = min(kSolveNB, S - r0);
-        const int r = r0 + lane;
+    const bool diag = local < w.nblk;
+    const int r0 = diag ? local * NB : S + (local - w.nblk) * NB;
+    const int nr = min(NB, (diag ? S : F) - r0);
+    const int ri = tid & (NB - 1), h = tid >> 7;
+    const int row = r0 + ri;
+    const bool rv = ri < nr;
+    // right-hand side of my rows: B (own columns) plus the children's updates
+    if (h == 0) {
+        double y = 0.0;
+        if (rv) {
+            y = (row < S) ? bb[rows[row]] : 0.0;
+            for (i64 k = up[row]; k < up[row + 1]; ++k) y += vw[upd_src[k]];
+        }
+        ys[ri] = y;
+    }
+    // destinations and pivots of the two block rows wave 0 writes at the end
+    i32 dst[2] = {0, 0};
+    double dpiv[2] = {1.0, 1.0};
+    if (diag && wave == 0) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int i = min(lane + 64 * k, nr - 1);
+            dst[k] = rows[r0 + i];
+            dpiv[k] = dvec[n.first + r0 + i];
+        }
+    }
+    if (diag) {
         for (int idx = tid; idx < nr * nr; idx += 256) {
             const int rr = idx % nr, cc = idx / nr;
-            if (rr > cc) Lb[rr][cc] = L[(r0 + rr) + i64(r0 + cc) * ld];
+            if (rr > cc) Ld[packed_col(cc) + (rr - cc)] = L[(r0 + rr) + i64(r0 + cc) * ld];
         }
-        double acc = 0.0;
-        for (int blk = 0; blk < local; ++blk) {
-            T lv[16];
-            const T *Lc = L + r + i64(blk * kSolveNB + wave * 16) * ld;
+        if (tid == 0) Ld[kZero] = T(0);
+    }
+    const int nprev = diag ? local : w.nblk;
+    double acc = 0.0;
+    for (int p = 0; p < nprev; ++p) {
+        const int c0 = p * NB;
+        const int nc = min(NB, S - c0);
+        // loads are unconditional and unmasked so that they all stay in
+        // flight: rows past the block read row r0 (their results are not
+        // stored) and columns past the block read its last column (xs is
+        // zero there)
+        T lv[NH];
+        const T *Lr = L + (rv ? row : r0) + i64(c0) * ld;
 #pragma unroll
-            for (int c = 0; c < 16; ++c) lv[c] = (lane < nr) ? Lc[c * ld] : T(0);
-            wait_ready(fl + blk, epoch, err);
-            if (tid < kSolveNB) xs[tid] = v[blk * kSolveNB + tid];  // full block: blk < nblk-1
-            __syncthreads();
-#pragma unroll
-            for (int c = 0; c < 16; ++c) acc += double(lv[c]) * xs[wave * 16 + c];
-            __syncthreads();
-        }
-        part[wave][lane] = acc;
+        for (int c = 0; c < NH; ++c) lv[c] = Lr[i64(min(h * NH + c, nc - 1)) * ld];
+        wait_ready(fl + p, epoch, err);
+        if (tid < NB) xs[tid] = (tid < nc) ? load_shared(v + c0 + tid) : 0.0;
         __syncthreads();
-        if (wave == 0) {
-            double x = 0.0;
-            if (lane < nr)
-                x = v[r] - (((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane]);
-            for (int c = 0; c < nr - 1; ++c) {
-                const double xc = __shfl(x, c);
-                if (lane > c && lane < nr) x -= double(Lb[lane][c]) * xc;
-            }
-            if (lane < nr) {
-                v[r] = x;
-                b[rhs * ldb + lindx[n.lxoff + r]] = x / dvec[n.first + r];
-            }
-            publish_ready(fl + local, epoch);
+#pragma unroll
+        for (int c = 0; c < NH; ++c) acc += double(lv[c]) * xs[h * NH + c];
+        __syncthreads();
+    }
+    part[h][ri] = acc;
+    __syncthreads();
+    if (!diag) {
+        if (h == 0 && rv) v[row] = ys[ri] - (part[0][ri] + part[1][ri]);  // update for the parent
+        return;
+    }
+    // Diagonal block: four 32 x 32 sub-blocks in sequence.  Wave w keeps row
+    // 32w + lane (lane < 32) of its unit-lower sub-block in registers and
+    // solves it with a readlane chain; the waves below then apply it to their
+    // rows from the packed block in LDS.
+    if (tid < NB) ys[tid] = (tid < nr) ? ys[tid] - (part[0][tid] + part[1][tid]) : 0.0;
+    // Lanes 32..63 mirror lanes 0..31 (lane & 31) and do no work.
+    const int sr = 32 * wave + (lane & 31);  // block row of this lane (lanes < 32 work)
+    const bool sw = lane < 32 && sr < nr;
+    T lsub[32];
+#pragma unroll
+    for (int c = 0; c < 32; ++c) {
+        const int cc = 32 * wave + c;
+        lsub[c] = Ld[(sw && c < (lane & 31)) ? packed_col(cc) + sr - cc : kZero];
+    }
+    __syncthreads();
+    for (int j = 0; j < 4 && 32 * j < nr; ++j) {
+        if (wave == j) {
+            double x = sw ? ys[sr] : 0.0;
+#pragma unroll
+            for (int c = 0; c < 32; ++c) x -= double(lsub[c]) * lane_bcast(x, c);
+            if (lane < 32) ys[sr] = x;
         }
-    } else {
-        const int r = S + (local - w.nblk) * kSolveRows + tid;
-        double s = (r < F) ? v[r] : 0.0;
-        for (int blk = 0; blk < w.nblk; ++blk) {
-            wait_ready(fl + blk, epoch, err);
-            if (tid < kSolveNB)
-                xs[tid] = (blk * kSolveNB + tid < S) ? v[blk * kSolveNB + tid] : 0.0;
-            __syncthreads();
-            const int nc = min(kSolveNB, S - blk * kSolveNB);
-            if (r < F) {
-                const T *Lc = L + r + i64(blk * kSolveNB) * ld;
-                double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-                int c = 0;
-                for (; c + 3 < nc; c += 4) {
-                    s0 += double(Lc[c * ld]) * xs[c];
-                    s1 += double(Lc[(c + 1) * ld]) * xs[c + 1];
-                    s2 += double(Lc[(c + 2) * ld]) * xs[c + 2];
-                    s3 += double(Lc[(c + 3) * ld]) * xs[c + 3];
-                }
-                for (; c < nc; ++c) s0 += double(Lc[c * ld]) * xs[c];
-                s -= (s0 + s1) + (s2 + s3);
+        __syncthreads();
+        if (wave > j && sw) {
+            double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+            for (int c = 0; c < 32; c += 2) {
+                const int cc = 32 * j + c;
+                const T a0 = Ld[cc < nr ? packed_col(cc) + sr - cc : kZero];
+                const T a1 = Ld[cc + 1 < nr ? packed_col(cc + 1) + sr - cc - 1 : kZero];
+                s0 += double(a0) * ys[cc];
+                s1 += double(a1) * ys[cc + 1];
             }
-            __syncthreads();
+            ys[sr] -= s0 + s1;
         }
-        if (r < F) v[r] = s;
+        __syncthreads();
+    }
+    if (wave == 0) {
+        // results for the blocks that wait on this one first, then B
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+            if (lane + 64 * k < nr) store_shared(v + r0 + lane + 64 * k, ys[lane + 64 * k]);
+        publish_ready(fl + local, epoch);
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+            if (lane + 64 * k < nr) bb[dst[k]] = ys[lane + 64 * k] / dpiv[k];
     }
 }
 
-__global__ __launch_bounds__(256) void k_bwd_assemble(const i32 *__restrict__ list,
-                                                      const NodeInfo *__restrict__ nodes,
-                                                      const i32 *__restrict__ lindx,
-                                                      const double *b, i64 ldb, double *vwork,
-                                                      i64 vstride)
+// ---------------------------------------------------------------------------
+// wide supernodes, backward
+// ---------------------------------------------------------------------------
+// Partial sums of L21^T x2: workgroup (blk, k) multiplies rows [S+k*NB, +NB)
+// of the columns of block blk by the solution at those rows (final: they
+// belong to ancestors).  Two threads per column, each walking half the rows.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_rect(const WideDesc *__restrict__ wide, int nwide,
+                                                  const i64 *__restrict__ prefix,
+                                                  const NodeInfo *__restrict__ nodes,
+                                                  const i32 *__restrict__ lindx,
+                                                  const T *__restrict__ pool,
+                                                  const double *__restrict__ b, i64 ldb,
+                                                  double *partials, i64 npart)
 {
-    const NodeInfo n = nodes[list[blockIdx.x]];
-    const int rhs = blockIdx.y;
-    double *v = vwork + rhs * vstride + n.voff;
-    const double *bb = b + rhs * ldb;
-    const i32 *rows = lindx + n.lxoff;
-    for (int i = threadIdx.x; i < n.F; i += 256) v[i] = bb[rows[i]];
+    __shared__ double xs[NB];
+    __shared__ double part[2][NB];
+    const int q = find_desc(prefix, nwide, blockIdx.x);
+    const WideDesc w = wide[q];
+    const int t = int(blockIdx.x - prefix[q]);
+    const int blk = t / w.nrect, k = t % w.nrect;
+    const NodeInfo n = nodes[w.node];
+    const int rhs = blockIdx.y, tid = threadIdx.x;
+    const int S = n.S, F = n.F;
+    const int c0 = blk * NB, nc = min(NB, S - c0);
+    const int r0 = S + k * NB, nr = min(NB, F - r0);
+    const T *L = pool + n.foff;
+    const i64 ld = n.ld;
+    if (tid < NB) xs[tid] = (tid < nr) ? b[rhs * ldb + lindx[n.lxoff + r0 + tid]] : 0.0;
+    __syncthreads();
+    const int ci = tid & (NB - 1), h = tid >> 7;
+    // unconditional loads (columns past the block read its last column and
+    // are not stored; rows past the chunk read its last row, xs is zero)
+    T lv[NH];
+    const T *Lc = L + r0 + i64(c0 + min(ci, nc - 1)) * ld;
+#pragma unroll
+    for (int r = 0; r < NH; ++r) lv[r] = Lc[min(h * NH + r, nr - 1)];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+    for (int r = 0; r < NH; r += 4) {
+        a0 += double(lv[r]) * xs[h * NH + r];
+        a1 += double(lv[r + 1]) * xs[h * NH + r + 1];
+        a2 += double(lv[r + 2]) * xs[h * NH + r + 2];
+        a3 += double(lv[r + 3]) * xs[h * NH + r + 3];
+    }
+    part[h][ci] = (a0 + a1) + (a2 + a3);
+    __syncthreads();
+    if (tid < nc)
+        partials[rhs * npart + w.pbase + (i64(blk) * w.nrect + k) * NB + tid] =
+            part[0][tid] + part[1][tid];
 }
 
-// Backward dataflow: workgroup q handles block nblk-1-q; it first applies the
-// rectangle rows (known on entry), then the later blocks as they finish.
-// Lane = column of the block; the four waves interleave the rows and their
-// partial sums are combined in a fixed order.
+// Workgroup local handles block nblk-1-local: y = B - partial sums of the
+// rows below, minus the later blocks' products as they finish, then
+// L11^T x = y on the block by wave 0 (two columns per lane).
 template <typename T>
 __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ wide, int nwide,
                                                   const i64 *__restrict__ prefix,
                                                   const NodeInfo *__restrict__ nodes,
                                                   const i32 *__restrict__ lindx,
                                                   const T *__restrict__ pool, double *b, i64 ldb,
-                                                  double *vwork, i64 vstride, unsigned *flags,
-                                                  i64 nflags, unsigned epoch, int *err)
+                                                  double *vwork, i64 vstride,
+                                                  const double *__restrict__ partials, i64 npart,
+                                                  unsigned *flags, i64 nflags, unsigned epoch,
+                                                  int *err)
 {
-    __shared__ double part[4][64];
-    __shared__ double xr[kSolveNB];
-    __shared__ T Lb[kSolveNB][kSolveNB + 1];
-    const int q = find_wide(prefix, nwide, blockIdx.x);
+    __shared__ double xs[NB];
+    __shared__ double ys[NB];
+    __shared__ double part[2][NB];
+    __shared__ T Lr[kPacked + 1];  // packed lower diagonal block, by rows; Lr[kZero] = 0
+    const int q = find_desc(prefix, nwide, blockIdx.x);
     const WideDesc w = wide[q];
-    const int blkid = w.nblk - 1 - int(blockIdx.x - prefix[q]);
+    const int blk = w.nblk - 1 - int(blockIdx.x - prefix[q]);
     const NodeInfo n = nodes[w.node];
     const int rhs = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int S = n.S, F = n.F;
+    const int S = n.S;
     double *v = vwork + rhs * vstride + n.voff;
+    double *bb = b + rhs * ldb;
+    const i32 *rows = lindx + n.lxoff;
     const T *L = pool + n.foff;
     const i64 ld = n.ld;
     unsigned *fl = flags + rhs * nflags + w.flag0;
-    const int c0 = blkid * kSolveNB;
-    const int nc = min(kSolveNB, S - c0);
+    const int c0 = blk * NB, nc = min(NB, S - c0);
     for (int idx = tid; idx < nc * nc; idx += 256) {
         const int rr = idx % nc, cc = idx / nc;
-        if (rr > cc) Lb[rr][cc] = L[(c0 + rr) + i64(c0 + cc) * ld];
+        if (rr > cc) Lr[rr * (rr + 1) / 2 + cc] = L[(c0 + rr) + i64(c0 + cc) * ld];
     }
-    const bool cv = lane < nc;
-    const T *Lcol = L + i64(c0 + lane) * ld;
-    double acc = 0.0;
-    if (cv) {
-        // wave w takes the rows of its quarter; 4 independent partial sums
-        const int nrw = F - S, per = (nrw + 3) / 4;
-        const int ra = S + wave * per, rz = min(F, ra + per);
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-        int r = ra;
-        for (; r + 3 < rz; r += 4) {
-            a0 += double(Lcol[r]) * v[r];
-            a1 += double(Lcol[r + 1]) * v[r + 1];
-            a2 += double(Lcol[r + 2]) * v[r + 2];
-            a3 += double(Lcol[r + 3]) * v[r + 3];
-        }
-        for (; r < rz; ++r) a0 += double(Lcol[r]) * v[r];
-        acc = (a0 + a1) + (a2 + a3);
-    }
-    for (int blk = w.nblk - 1; blk > blkid; --blk) {
-        T lv[16];
-        const int rb = blk * kSolveNB + wave * 16;
-#pragma unroll
-        for (int k = 0; k < 16; ++k) lv[k] = (cv && rb + k < S) ? Lcol[rb + k] : T(0);
-        wait_ready(fl + blk, epoch, err);
-        if (tid < kSolveNB) xr[tid] = (blk * kSolveNB + tid < S) ? v[blk * kSolveNB + tid] : 0.0;
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < 16; ++k) acc += double(lv[k]) * xr[wave * 16 + k];
-        __syncthreads();
-    }
-    part[wave][lane] = acc;
-    __syncthreads();
+    if (tid == 0) Lr[kZero] = T(0);
+    i32 dst[2] = {0, 0};
     if (wave == 0) {
-        const int r = c0 + lane;
-        double x = cv ? v[r] - (((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane])
-                      : 0.0;
-        for (int rr = nc - 1; rr > 0; --rr) {
-            const double xv = __shfl(x, rr);
-            if (lane < rr) x -= double(Lb[rr][lane]) * xv;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) dst[k] = rows[c0 + min(lane + 64 * k, nc - 1)];
+    }
+    const int ci = tid & (NB - 1), h = tid >> 7;
+    // products with the rows below the diagonal part: the two halves of the
+    // workgroup sum the even and the odd row chunks' partials
+    {
+        const double *ps =
+            partials + rhs * npart + w.pbase + i64(blk) * w.nrect * NB + min(ci, nc - 1);
+        part[h][ci] = strided_sum(ps, NB, h, 2, w.nrect);
+    }
+    __syncthreads();
+    if (tid < NB) ys[tid] = (tid < nc) ? bb[rows[c0 + tid]] - (part[0][tid] + part[1][tid]) : 0.0;
+    __syncthreads();
+    double acc = 0.0;
+    for (int p = w.nblk - 1; p > blk; --p) {
+        const int rp0 = p * NB, np = min(NB, S - rp0);
+        // unconditional, unmasked loads (see k_fwd_wide): columns past the
+        // block read its last column (not stored), rows past block p read
+        // its last row (xs is zero there)
+        T lv[NH];
+        const T *Lc = L + rp0 + i64(c0 + min(ci, nc - 1)) * ld;
+#pragma unroll
+        for (int r = 0; r < NH; ++r) lv[r] = Lc[min(h * NH + r, np - 1)];
+        wait_ready(fl + p, epoch, err);
+        if (tid < NB) xs[tid] = (tid < np) ? load_shared(v + rp0 + tid) : 0.0;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < NH; ++r) acc += double(lv[r]) * xs[h * NH + r];
+        __syncthreads();
+    }
+    part[h][ci] = acc;
+    __syncthreads();
+    // Diagonal block, L11^T x = y: four 32 x 32 sub-blocks from the last.
+    // Wave w keeps column 32w + lane (lane < 32) of its sub-block in
+    // registers and solves it with a readlane chain; the waves to the left
+    // then apply it to their columns from the packed block in LDS.
+    if (tid < NB) ys[tid] = (tid < nc) ? ys[tid] - (part[0][tid] + part[1][tid]) : 0.0;
+    // Lanes 32..63 mirror lanes 0..31 (lane & 31) and do no work.
+    const int sc = 32 * wave + (lane & 31);  // block column of this lane (lanes < 32 work)
+    const bool sw = lane < 32 && sc < nc;
+    T lcol[32];
+#pragma unroll
+    for (int r = 0; r < 32; ++r) {
+        const int rr = 32 * wave + r;
+        lcol[r] = Lr[(sw && r > (lane & 31) && rr < nc) ? rr * (rr + 1) / 2 + sc : kZero];
+    }
+    __syncthreads();
+    for (int j = 3; j >= 0; --j) {
+        if (32 * j >= nc) continue;
+        if (wave == j) {
+            double x = sw ? ys[sc] : 0.0;
+#pragma unroll
+            for (int r = 31; r > 0; --r) x -= double(lcol[r]) * lane_bcast(x, r);
+            if (lane < 32) ys[sc] = x;
         }
-        if (cv) {
-            v[r] = x;
-            b[rhs * ldb + lindx[n.lxoff + r]] = x;
+        __syncthreads();
+        if (wave < j && sw) {
+            double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+            for (int r = 0; r < 32; r += 2) {
+                const int rr = 32 * j + r;
+                const T a0 = Lr[rr < nc ? rr * (rr + 1) / 2 + sc : kZero];
+                const T a1 = Lr[rr + 1 < nc ? (rr + 1) * (rr + 2) / 2 + sc : kZero];
+                s0 += double(a0) * ys[rr];
+                s1 += double(a1) * ys[rr + 1];
+            }
+            ys[sc] -= s0 + s1;
         }
-        publish_ready(fl + blkid, epoch);
+        __syncthreads();
+    }
+    if (wave == 0) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+            if (lane + 64 * k < nc) store_shared(v + c0 + lane + 64 * k, ys[lane + 64 * k]);
+        publish_ready(fl + blk, epoch);
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+            if (lane + 64 * k < nc) bb[dst[k]] = ys[lane + 64 * k];
     }
 }
 
@@ -427,6 +611,14 @@ static int ensure_solve_buffers(sprgpu_handle *h, i64 nrhs)
         d.flags_cap = nf;
         d.epoch = 0;  // every flag is zero again
     }
+    const i64 np = std::max<i64>(p.npart, 1) * nrhs;
+    if (d.partials_cap < np) {
+        if (d.partials) SPRGPU_HIP(hipFree(d.partials));
+        d.partials = nullptr;
+        SPRGPU_HIP(hipMalloc(&d.partials, sizeof(double) * np));
+        h->device_bytes += sizeof(double) * (np - d.partials_cap);
+        d.partials_cap = np;
+    }
     if (!d.spin_error) {
         SPRGPU_HIP(hipMalloc(&d.spin_error, sizeof(int)));
         SPRGPU_HIP(hipMemset(d.spin_error, 0, sizeof(int)));
@@ -445,39 +637,46 @@ static int solve_impl(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which)
     const i64 vstride = p.vwork_elems;
     const unsigned ur = unsigned(nrhs);
     const i64 nflags = std::max<i64>(p.nflags, 1);
+    const i64 npart = std::max<i64>(p.npart, 1);
+    auto narrow_threads = [&](int l) { return p.small_maxF[l] <= 128 ? 64u : 256u; };
     if (which & 1) {
         const unsigned ep = ++d.epoch;
         for (int l = 0; l < p.nlevels; ++l) {
             const unsigned ns = unsigned(p.solve_small_ptr[l + 1] - p.solve_small_ptr[l]);
-            if (ns)
-                k_fwd_small<T><<<dim3(ns, ur), 256, 0, s>>>(
-                    d.solve_small + p.solve_small_ptr[l], d.nodes, d.children, d.lindx, d.relmap,
-                    pool, d.dvec, b, ldb, d.vwork, vstride);
-            const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
-            if (nw) {
-                k_fwd_assemble<<<dim3(unsigned(nw), ur), 256, 0, s>>>(
-                    d.wide_nodes + w0, d.nodes, d.children, d.lindx, d.relmap, b, ldb, d.vwork,
-                    vstride);
-                k_fwd_wide<T><<<dim3(unsigned(p.wide_fwd_wgs[l]), ur), 256, 0, s>>>(
-                    d.wide + w0, nw, d.wide_fwd_prefix + w0, d.nodes, d.lindx, pool, d.dvec, b,
-                    ldb, d.vwork, vstride, d.flags, nflags, ep, d.spin_error);
+            if (ns) {
+                const int mf = p.small_maxF[l], ms = p.small_maxS[l];
+                const size_t shm = sizeof(double) * mf + sizeof(T) * ms * ms;
+                k_fwd_narrow<T><<<dim3(ns, ur), narrow_threads(l), shm, s>>>(
+                    d.solve_small + p.solve_small_ptr[l], mf, d.nodes, d.lindx, d.upd_ptr,
+                    d.upd_src, pool, d.dvec, b, ldb, d.vwork, vstride);
             }
+            const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
+            if (nw)
+                k_fwd_wide<T><<<dim3(unsigned(p.wide_fwd_wgs[l]), ur), 256, 0, s>>>(
+                    d.wide + w0, nw, d.wide_fwd_prefix + w0, d.nodes, d.lindx, d.upd_ptr,
+                    d.upd_src, pool, d.dvec, b, ldb, d.vwork, vstride, d.flags, nflags, ep,
+                    d.spin_error);
         }
     }
     if (which & 2) {
         const unsigned ep = ++d.epoch;
         for (int l = p.nlevels - 1; l >= 0; --l) {
-            const unsigned ns = unsigned(p.solve_small_ptr[l + 1] - p.solve_small_ptr[l]);
-            if (ns)
-                k_bwd_small<T><<<dim3(ns, ur), 256, 0, s>>>(d.solve_small + p.solve_small_ptr[l],
-                                                            d.nodes, d.lindx, pool, b, ldb);
             const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
             if (nw) {
-                k_bwd_assemble<<<dim3(unsigned(nw), ur), 256, 0, s>>>(
-                    d.wide_nodes + w0, d.nodes, d.lindx, b, ldb, d.vwork, vstride);
+                if (p.wide_rect_wgs[l] > 0)
+                    k_bwd_rect<T><<<dim3(unsigned(p.wide_rect_wgs[l]), ur), 256, 0, s>>>(
+                        d.wide + w0, nw, d.wide_rect_prefix + w0, d.nodes, d.lindx, pool, b, ldb,
+                        d.partials, npart);
                 k_bwd_wide<T><<<dim3(unsigned(p.wide_bwd_wgs[l]), ur), 256, 0, s>>>(
                     d.wide + w0, nw, d.wide_bwd_prefix + w0, d.nodes, d.lindx, pool, b, ldb,
-                    d.vwork, vstride, d.flags, nflags, ep, d.spin_error);
+                    d.vwork, vstride, d.partials, npart, d.flags, nflags, ep, d.spin_error);
+            }
+            const unsigned ns = unsigned(p.solve_small_ptr[l + 1] - p.solve_small_ptr[l]);
+            if (ns) {
+                const int mf = p.small_maxF[l], ms = p.small_maxS[l];
+                const size_t shm = sizeof(double) * (mf + 16 + 256) + sizeof(T) * ms * ms;
+                k_bwd_narrow<T><<<dim3(ns, ur), narrow_threads(l), shm, s>>>(
+                    d.solve_small + p.solve_small_ptr[l], mf, d.nodes, d.lindx, pool, b, ldb);
             }
         }
     }
