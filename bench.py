@@ -10,10 +10,11 @@ value = factor flops (the reference's RINFO(3) count) / seconds of one
 chosen-form step, in TFLOP/s.
 
 Set-up outside the timed region, as the host application does it: the mesh,
-the reference SPR symbolic phase and SPRADM assembly (oracle/_ref, the
-reference's own code), the upload of SM and b to HBM, and the residual
-pattern.  The CPU baseline is the reference's SPRFC1 + SPRFS1 + SPRBS1 on the
-20^3 case, on one host core.
+the reference SPR symbolic phase and SPRADM assembly (sprgpu.host, the
+reference's own routines compiled into lib/libspr_host.so), the upload of SM
+and b to HBM, and the residual pattern.  The CPU baseline (the only use of
+oracle/) is the reference's SPRFC1 + SPRFS1 + SPRBS1 on the 20^3 case, on
+one host core.
 """
 import argparse
 import json
@@ -24,7 +25,7 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
-sys.path[:0] = [os.path.join(ROOT, "sam-lib_amd"), os.path.join(ROOT, "oracle")]
+sys.path.insert(0, os.path.join(ROOT, "sam-lib_amd"))
 
 METRIC = "factor+solve s & TFLOP/s, 40^3 hex8: FP64 vs FP32+IR at 1e-10 residual"
 PEAK = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense MFMA TFLOP/s
@@ -36,25 +37,25 @@ def log(msg):
 
 
 def build_problem(n):
-    import refspr
-    from sprgpu import fem
+    from sprgpu import fem, host
 
     model = fem.hex8_grid(n, e_sigma=0.5, e_seed=1)
     rng = np.random.default_rng(1)
     rng.normal(0.0, 0.5, model.nel)           # the moduli drawn by hex8_grid
     f = rng.uniform(-1.0, 1.0, model.ndof)    # then the load vector
     t = time.perf_counter()
-    sysm = refspr.symbolic(model)
-    sm = refspr.assemble(model, sysm)
-    b = refspr.loads_to_equations(sysm, f)[:, 0].copy()
+    sysm = host.symbolic(model)
+    sm = host.assemble(model, sysm)
+    b = host.loads_to_equations(sysm, f)[:, 0].copy()
     log(f"set-up {n}^3: neq={sysm.neq} nsuper={sysm.nsuper} nzerol={sysm.nzerol} "
-        f"mfront={sysm.mfront} flops={refspr.factor_flops(sysm):.4e} ({time.perf_counter() - t:.1f}s)")
+        f"mfront={sysm.mfront} flops={host.factor_flops(sysm):.4e} ({time.perf_counter() - t:.1f}s)")
     return model, sysm, sm, b
 
 
 def cpu_baseline():
     """Reference SPRFC1/SPRFS1/SPRBS1 on the 20^3 case (E=1, nu=0.3, loads seed 0), 1 core."""
-    import refspr
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refspr  # the reference build, oracle/_ref: this leg only
     from sprgpu import fem
 
     model = fem.hex8_grid(20)
@@ -90,13 +91,13 @@ def main():
         sys.exit(2)
 
     import torch
-    import refspr
+    from sprgpu import host
     from sprgpu.device import SprFactor
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
     model, sysm, sm, b = build_problem(args.n)
-    flops = refspr.factor_flops(sysm)
+    flops = host.factor_flops(sysm)
     t = time.perf_counter()
     sm_dev = torch.from_numpy(sm).to(dev)
     b_dev = torch.from_numpy(b).to(dev).view(1, -1)
