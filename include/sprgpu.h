@@ -134,6 +134,28 @@ int sprgpu_spmv(sprgpu_handle *handle, const double *x_dev, double *y_dev, sprgp
 int sprgpu_solve_ir(sprgpu_handle *handle, const double *b_dev, double *x_dev, sprgpu_int ld,
                     sprgpu_int nrhs, double rtol, int maxit, sprgpu_ir_info *info);
 
+/* Adjoint gradients (no counterpart in the reference, which has no adjoint;
+ * lambda = A^-1 dL/du comes from sprgpu_solve / sprgpu_solve_ir with the same
+ * factor, A being symmetric).  All arrays are device memory, vectors in SPR
+ * equation order.
+ *
+ * sprgpu_element_grad: for K = sum_e p_e K0_e, grad[e] = -lambda_e^T K0_e x_e
+ * for nel elements of nedof (<= 64) dofs each.  elem_eq[e*nedof + i] is the
+ * 0-based equation of element dof i or -1 (no equation: a fixed dof).  K0 is
+ * k0 + k0_index[e]*nedof*nedof, column-major (k0_index NULL: one matrix
+ * shared by all elements). */
+int sprgpu_element_grad(sprgpu_handle *handle, sprgpu_int nel, sprgpu_int nedof,
+                        const sprgpu_int *elem_eq, const double *k0,
+                        const sprgpu_int *k0_index, const double *lambda, const double *x,
+                        double *grad);
+
+/* sprgpu_pattern_grad: dL/dA on the stored lower triangle in SM layout:
+ * grad_sm(NEQ + LNZ position of (i,j)) = -(lambda_i x_j + lambda_j x_i) for
+ * i > j and -lambda_i x_i on the diagonal; grad_sm(1:NEQ) is not written.
+ * grad_sm holds NEQ+NZEROL doubles. */
+int sprgpu_pattern_grad(sprgpu_handle *handle, const double *lambda, const double *x,
+                        double *grad_sm);
+
 /* Copies the factor into SM layout (NEQ+NZEROL doubles): d in SM(1:NEQ) and
  * the packed trapezoids with d on the diagonal and l_ij below.  sm_out may be
  * host or device memory. */

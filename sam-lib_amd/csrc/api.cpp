@@ -43,7 +43,7 @@ static void free_all(sprgpu_handle *h)
                     d.csr_src,  d.csr_val,    d.irwork,      d.level_nodes,
                     d.solve_small, d.wide,     d.wide_fwd_prefix, d.wide_bwd_prefix,
                     d.wide_rect_prefix, d.upd_ptr, d.upd_src, d.flags, d.partials,
-                    d.spin_error};
+                    d.spin_error, d.col_node};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     d = DeviceBuffers{};
@@ -194,6 +194,20 @@ int sprgpu_solve_ir(sprgpu_handle *h, const double *b, double *x, sprgpu_int ld,
 {
     if (!h || !b || !x || ld < h->plan.neq) return -2;
     return run_solve_ir(h, b, x, ld, nrhs, rtol, maxit, info);
+}
+
+int sprgpu_element_grad(sprgpu_handle *h, sprgpu_int nel, sprgpu_int nedof,
+                        const sprgpu_int *elem_eq, const double *k0, const sprgpu_int *k0_index,
+                        const double *lambda, const double *x, double *grad)
+{
+    if (!h || !elem_eq || !k0 || !lambda || !x || !grad || nel < 0) return -2;
+    return run_element_grad(h, nel, int(nedof), elem_eq, k0, k0_index, lambda, x, grad);
+}
+
+int sprgpu_pattern_grad(sprgpu_handle *h, const double *lambda, const double *x, double *grad_sm)
+{
+    if (!h || !lambda || !x || !grad_sm) return -2;
+    return run_pattern_grad(h, lambda, x, grad_sm + h->plan.neq);
 }
 
 int sprgpu_get_factor(sprgpu_handle *h, double *sm_out)

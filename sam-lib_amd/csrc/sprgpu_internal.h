@@ -192,6 +192,7 @@ struct DeviceBuffers {
     i64 partials_cap = 0;
     unsigned epoch = 0;
     int *spin_error = nullptr;  // set when a dataflow wait gave up
+    i32 *col_node = nullptr;    // supernode of each equation (adjoint pattern gradient)
     i64 irwork_cap = 0;
 };
 
@@ -234,4 +235,8 @@ int run_spmv(sprgpu_handle *h, const double *x, double *y, i64 ld, i64 nrhs);
 int run_solve_ir(sprgpu_handle *h, const double *b, double *x, i64 ld, i64 nrhs, double rtol,
                  int maxit, sprgpu_ir_info *info);
 int upload_pattern(sprgpu_handle *h, const CsrPattern &pat);
+// adjoint.hip
+int run_element_grad(sprgpu_handle *h, i64 nel, int nedof, const i64 *elem_eq, const double *k0,
+                     const i64 *k0_index, const double *lam, const double *x, double *grad);
+int run_pattern_grad(sprgpu_handle *h, const double *lam, const double *x, double *glnz);
 }  // namespace sprgpu

@@ -53,6 +53,8 @@ SIGNATURES = {
     "sprgpu_device_bytes": (I64, [P]),
     "sprgpu_plan_summary": (C.c_int, [P, P, P, P]),
     "sprgpu_profile_factor": (C.c_int, [P, P, P, C.c_int, P]),
+    "sprgpu_element_grad": (C.c_int, [P, I64, I64, P, P, P, P, P, P]),
+    "sprgpu_pattern_grad": (C.c_int, [P, P, P, P]),
     "sprgpu_last_error": (C.c_char_p, []),
 }
 

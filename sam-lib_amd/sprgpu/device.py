@@ -117,3 +117,28 @@ class SprFactor:
         else:
             check(lib().sprgpu_get_factor(self._h, _ptr(out)), "sprgpu_get_factor")
         return out
+
+    def element_grad(self, elem_eq_dev, k0_dev, k0_index_dev, lam_dev, x_dev, out=None):
+        """grad[e] = -lambda_e^T K0_e x_e on the device (sprgpu_element_grad).
+
+        elem_eq_dev: (nel, nedof) int64 equations (-1: no equation); k0_dev:
+        (nmat, nedof, nedof) float64 column-major matrices; k0_index_dev: (nel,)
+        int64 or None (one shared matrix)."""
+        torch = self._torch
+        nel, nedof = elem_eq_dev.shape
+        if out is None:
+            out = torch.empty(nel, dtype=torch.float64, device=lam_dev.device)
+        kidx = C.c_void_p(0) if k0_index_dev is None else _ptr(k0_index_dev)
+        check(lib().sprgpu_element_grad(self._h, nel, nedof, _ptr(elem_eq_dev), _ptr(k0_dev), kidx,
+                                        _ptr(lam_dev), _ptr(x_dev), _ptr(out)),
+              "sprgpu_element_grad")
+        return out
+
+    def pattern_grad(self, lam_dev, x_dev, out=None):
+        """dL/dA on the stored lower triangle in SM layout (sprgpu_pattern_grad)."""
+        torch = self._torch
+        if out is None:
+            out = torch.zeros(self.neq + self.nzerol, dtype=torch.float64, device=lam_dev.device)
+        check(lib().sprgpu_pattern_grad(self._h, _ptr(lam_dev), _ptr(x_dev), _ptr(out)),
+              "sprgpu_pattern_grad")
+        return out

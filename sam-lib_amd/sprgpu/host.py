@@ -218,3 +218,46 @@ def equations_to_dofs(sysm: SprSystem, x_eq: np.ndarray) -> np.ndarray:
 def factor_flops(sysm: SprSystem) -> float:
     """RINFO(3): the reference's factor flop count (sprtrs.f:1961-1969)."""
     return float(sysm.rinfo[2])
+
+
+def lnz_positions(sysm: SprSystem, rows: np.ndarray, cols: np.ndarray) -> np.ndarray:
+    """0-based SM indices of the lower-triangle entries (rows[k], cols[k]).
+
+    rows/cols are 0-based SPR equations with rows >= cols.  The entry of
+    column c lies in the supernode holding c, in the packed trapezoid that
+    starts at XLNZ: column k of a supernode of order F starts at
+    OFFSET(F,k) = k*F - k*(k-1)/2 and holds the rows LINDX(k..F-1)
+    (sprsmb.f:512-575, sprfc1.f:201-203); SM(1:NEQ) precedes LNZ.
+    """
+    rows = np.asarray(rows, dtype=np.int64)
+    cols = np.asarray(cols, dtype=np.int64)
+    if np.any(rows < cols):
+        raise ValueError("lnz_positions takes lower-triangle entries (row >= col)")
+    xsuper = sysm.xsuper() - 1
+    xlindx = sysm.xlindx() - 1
+    xlnz = sysm.xlnz() - 1
+    lindx = sysm.lindx() - 1
+    s = np.searchsorted(xsuper, cols, side="right") - 1
+    k = cols - xsuper[s]
+    F = xlindx[s + 1] - xlindx[s]
+    p = np.empty_like(rows)
+    for j in np.unique(s):  # rows are searched in the supernode's sorted list
+        m = s == j
+        seg = lindx[xlindx[j]:xlindx[j + 1]]
+        p[m] = np.searchsorted(seg, rows[m])
+        if np.any(seg[np.minimum(p[m], seg.size - 1)] != rows[m]):
+            raise ValueError("entry outside the symbolic pattern")
+    return sysm.neq + xlnz[s] + k * F - k * (k - 1) // 2 + (p - k)
+
+
+def element_equations(model, sysm: SprSystem) -> np.ndarray:
+    """(nel, nedof) 0-based equations of each element's dofs, -1 where a dof
+    has none (fixed).  All elements must have the same dof count."""
+    nedof = int(model.nedof[0])
+    if np.any(model.nedof != nedof):
+        raise ValueError("element_equations needs a uniform element dof count")
+    eq = np.empty((model.nel, nedof), dtype=np.int64)
+    for e in range(model.nel):
+        eq[e] = sysm.meqn[model.element_dofs(e)] - 1
+    eq[eq < 0] = -1
+    return eq
