@@ -84,11 +84,18 @@ def main():
     ap.add_argument("--maxit", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel timing pass")
+    ap.add_argument("--config", default="factor_solve",
+                    choices=["factor_solve", "adjoint"],
+                    help="factor_solve: the headline (40^3, FP64 vs FP32+IR); adjoint: "
+                         "dL/dE of the compliance at 40^3 (E seed 4)")
+    ap.add_argument("--fd", type=int, default=64, help="adjoint: elements checked by finite differences")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path's multi-GPU "
               "sharding is a design note, not a measured configuration.", file=sys.stderr)
         sys.exit(2)
+    if args.config == "adjoint":
+        return bench_adjoint(args)
 
     import torch
     from sprgpu import host
@@ -238,6 +245,95 @@ def main():
         result["cpu_baseline"] = cpu_baseline()
         log(f"cpu baseline {result['cpu_baseline']['value']*1e3:.2f} GFLOP/s")
 
+    print(json.dumps(result), flush=True)
+
+
+def bench_adjoint(args):
+    """Compliance L = f^T u on hex8 n^3, E log-normal(0,0.5) seed 4, loads U[-1,1]
+    drawn after the moduli.  Forward: FP32 factor + FP64 refinement (rtol);
+    backward: lambda from the same factor, then dL/dE_e on the device.  Timed
+    per pass with the host assembly outside.  Checked against cancellation-free
+    central differences (h = 1e-6 E_e) on --fd random elements (seed 4) and,
+    when tests/golden holds it, the reference's FP64 CPU adjoint."""
+    import torch
+    from sprgpu import fem
+    from sprgpu.adjoint import ElasticSystem, spr_solve
+
+    n, seed = args.n, 4
+    model = fem.hex8_grid(n, e_sigma=0.5, e_seed=seed)
+    rng = np.random.default_rng(seed)
+    rng.normal(0.0, 0.5, model.nel)
+    f = rng.uniform(-1.0, 1.0, model.ndof)
+    t = time.perf_counter()
+    sysobj = ElasticSystem(model, precision="fp32", rtol=args.rtol, maxit=args.maxit)
+    E = model.escale.copy()
+    sysobj.assemble(E)
+    log(f"adjoint set-up {n}^3: neq={sysobj.neq} ({time.perf_counter() - t:.1f}s)")
+    ft = torch.from_numpy(f).cuda()
+    f_eq = sysobj.to_eq(ft)
+
+    def forward():
+        sysobj.factor(E, sysobj.sm)
+        return sysobj.solve_eq(f_eq)
+
+    def backward(x_eq):
+        lam = sysobj.solve_eq(f_eq)  # dL/du = f
+        return sysobj.element_grad(lam, x_eq)
+
+    x_eq = forward()
+    g = backward(x_eq)
+    torch.cuda.synchronize()
+    tf = tb = 0.0
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        x_eq = forward()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        g = backward(x_eq)
+        torch.cuda.synchronize()
+        tf += t1 - t0
+        tb += time.perf_counter() - t1
+    tf /= args.steps
+    tb /= args.steps
+    L = float(torch.dot(f_eq, x_eq))
+    gh = g.cpu().numpy()
+    euler = abs(float(np.dot(E, gh)) + L) / abs(L)
+    # the autograd binding gives the same gradient
+    p = torch.from_numpy(E.copy()).cuda().requires_grad_(True)
+    u = spr_solve(p, ft, sysobj)
+    torch.dot(ft, u).backward()
+    autograd_diff = float((p.grad - g).abs().max() / g.abs().max())
+    log(f"forward {tf*1e3:.1f} ms, backward {tb*1e3:.1f} ms, L={L:.10e}, euler {euler:.2e}")
+    # finite differences
+    sel = np.random.default_rng(seed).choice(model.nel, args.fd, replace=False)
+    t = time.perf_counter()
+    fd_rel = []
+    for e in sel:
+        fd = sysobj.central_difference(E, f_eq, int(e), h_rel=1e-6)
+        fd_rel.append(abs(fd - gh[e]) / abs(gh[e]))
+    log(f"finite differences on {len(sel)} elements: max rel {max(fd_rel):.2e} "
+        f"({time.perf_counter() - t:.1f}s)")
+    ref = {}
+    path = os.path.join(ROOT, "tests", "golden", f"hex8_{n}_adjoint_reference.npz")
+    if os.path.exists(path):
+        d = np.load(path, allow_pickle=False)
+        ref["grad_max_rel"] = float(np.abs(gh - d["grad"]).max() / np.abs(d["grad"]).max())
+        ref["compliance_rel"] = abs(L - float(d["compliance"])) / abs(float(d["compliance"]))
+        xr = d["x"]
+        ref["x_rel"] = float(np.linalg.norm(x_eq.cpu().numpy() - xr) / np.linalg.norm(xr))
+    result = {
+        "metric": f"adjoint dL/dE_e of the compliance, {n}^3 hex8 (E seed 4), FP32 factor + FP64 IR",
+        "value": tf + tb, "unit": "s (forward + backward)", "n_gpus": 1, "steps": args.steps,
+        "higher_is_better": False, "data": "synthetic",
+        "config": {"workload": f"hex8 {n}^3, E log-normal(0,0.5) seed 4, loads U[-1,1]",
+                   "neq": sysobj.neq, "elements": model.nel, "rtol": args.rtol},
+        "forward_s": tf, "backward_s": tb, "compliance": L,
+        "euler_identity_rel": euler, "autograd_vs_kernel_rel": autograd_diff,
+        "fd": {"elements": int(len(sel)), "h_rel": 1e-6, "max_rel_diff": float(max(fd_rel)),
+               "median_rel_diff": float(np.median(fd_rel))},
+        "vs_fp64_cpu_adjoint": ref or None,
+    }
+    sysobj.close()
     print(json.dumps(result), flush=True)
 
 
