@@ -36,6 +36,7 @@ static void free_all(sprgpu_handle *h)
 {
     DeviceBuffers &d = h->d;
     void *ptrs[] = {d.nodes,    d.children,   d.lindx,       d.relmap,     d.init_tasks,
+                    d.copy_tasks,
                     d.ea_tasks, d.small_nodes, d.panel_desc, d.panel_prefix, d.gemm_desc,
                     d.gemm_prefix, d.pool,     d.dvec,       d.adiag,      d.dcor,
                     d.status,   d.stats,      d.vwork,       d.csr_rowptr, d.csr_col,
@@ -83,6 +84,7 @@ int sprgpu_create(const sprgpu_int *mspar, const sprgpu_int *mtrees, const sprgp
     if ((rc = upload(d.nodes, p.nodes, B)) || (rc = upload(d.children, p.children, B)) ||
         (rc = upload(d.lindx, p.lindx, B)) || (rc = upload(d.relmap, p.relmap, B)) ||
         (rc = upload(d.init_tasks, p.init_tasks, B)) || (rc = upload(d.ea_tasks, p.ea_tasks, B)) ||
+        (rc = upload(d.copy_tasks, p.copy_tasks, B)) ||
         (rc = upload(d.small_nodes, p.small_nodes, B)) ||
         (rc = upload(d.panel_desc, p.panel_desc, B)) ||
         (rc = upload(d.panel_prefix, p.panel_prefix, B)) ||

@@ -185,57 +185,97 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask *__restrict__ t
 }
 
 // ---------------------------------------------------------------------------
-// small fronts: everything in LDS, one workgroup per front
+// small fronts (F <= kSmallF): assembly and factor in LDS, one wavefront each
 // ---------------------------------------------------------------------------
+// The workgroup (64 threads, lane = front row) builds its front in LDS: the
+// supernode's trapezoid from SM (as k_front_init), then the children's
+// update blocks added in SIBL order (as k_extend_add), then eliminates the S
+// columns (SPRF17/LMMY18's recurrence: l = w * (1/d), a_rc -= w_r l_c) and
+// writes the lower triangle back to the pool, where the parent's extend-add
+// and the solves read it.  Dynamic LDS: F x (F+1) elements of T (F = the
+// launch's largest front) plus S pivots' a_jj and one column.
 template <typename T>
-__global__ __launch_bounds__(256) void k_small(const i32 *__restrict__ list,
-                                               const NodeInfo *__restrict__ nodes,
-                                               T *__restrict__ pool, double *dvec,
-                                               const double *__restrict__ adiag, double *dcor,
-                                               FactorStatus *st, PivotArgs pa)
+__global__ __launch_bounds__(64) void k_small(const i32 *__restrict__ list, int maxF,
+                                              const NodeInfo *__restrict__ nodes,
+                                              const i32 *__restrict__ children,
+                                              const i32 *__restrict__ relmap,
+                                              const double *__restrict__ lnz,
+                                              T *__restrict__ pool, double *dvec, double *adiag,
+                                              double *dcor, FactorStatus *st, PivotArgs pa)
 {
-    constexpr int LDM = kSmallF + 1;
-    __shared__ T M[kSmallF * LDM];
-    __shared__ T lcol[kSmallF];
-    __shared__ T dsh;
+    extern __shared__ double smem_small[];
+    const int LDM = maxF + 1;
+    T *M = reinterpret_cast<T *>(smem_small);       // M[r + c*LDM], lower triangle
+    double *ash = smem_small + (maxF * LDM * sizeof(T) + 7) / 8;  // a_jj of the S pivots
+    T *lcol = reinterpret_cast<T *>(ash + maxF);    // the scaled column of the step
     const NodeInfo n = nodes[list[blockIdx.x]];
-    const int F = n.F, S = n.S, tid = threadIdx.x;
-    T *fr = pool + n.foff;
-    for (int idx = tid; idx < F * F; idx += 256) {
-        const int r = idx % F, c = idx / F;
-        if (r >= c) M[r + c * LDM] = fr[r + i64(c) * n.ld];
-    }
-    __syncthreads();
-    for (int j = 0; j < S; ++j) {
-        if (tid == 0) {
-            const int eq = n.first + j;
-            const double d_in = double(M[j + j * LDM]);
-            const double a = adiag[eq];
-            const double d = pivot_value(d_in, a, pa);
-            pivot_record(d_in, d, a, eq, pa, dvec, dcor, st);
-            M[j + j * LDM] = T(d);
-            dsh = T(d);
-        }
-        __syncthreads();
-        const T inv = T(1) / dsh;
-        for (int r = j + 1 + tid; r < F; r += 256) lcol[r] = M[r + j * LDM] * inv;
-        __syncthreads();
-        const int m = F - j - 1;
-        for (int idx = tid; idx < m * m; idx += 256) {
-            const int rr = idx % m, cc = idx / m;
-            if (rr >= cc) {
-                const int r = j + 1 + rr, c = j + 1 + cc;
-                M[r + c * LDM] -= M[r + j * LDM] * lcol[c];
+    const int F = n.F, S = n.S, r = threadIdx.x;
+    const bool rv = r < F;
+    // 1. trapezoid from SM, zero elsewhere
+    unsigned long long nnz = 0;
+    for (int c = 0; c < F; ++c) {
+        if (!rv || r < c) continue;
+        double v = 0.0;
+        if (c < S) {
+            v = lnz[n.xlnz + i64(c) * F - (i64(c) * (c - 1)) / 2 + (r - c)];
+            nnz += (v != 0.0);
+            if (r == c) {
+                const int eq = n.first + c;
+                if (pa.mode == 3) {
+                    dcor[2 * i64(eq)] = 0.0;
+                    dcor[2 * i64(eq) + 1] = 0.0;
+                }
+                if (v == 0.0) {
+                    if (pa.mode == 3) {  // sprfc3.f:560-575
+                        v = 1.0;
+                        dcor[2 * i64(eq) + 1] = 1.0;
+                    } else {             // sprfc1.f:473-480
+                        atomicAdd(&st->nzero, 1ull);
+                        atomicMin(&st->first_zero, (unsigned long long)eq);
+                    }
+                }
+                adiag[eq] = v;
+                ash[c] = v;
             }
         }
-        __syncthreads();
-        for (int r = j + 1 + tid; r < F; r += 256) M[r + j * LDM] = lcol[r];
+        M[r + c * LDM] = T(v);
+    }
+    for (int o = 32; o > 0; o >>= 1) nnz += __shfl_down(nnz, o);
+    if (r == 0 && nnz) atomicAdd(&st->nnz_seen, nnz);
+    __syncthreads();
+    // 2. children's update blocks, one child at a time in SIBL order
+    for (int k = 0; k < n.nchild; ++k) {
+        const NodeInfo c = nodes[children[n.child0 + k]];
+        const i32 *map = relmap + c.mapoff;
+        const int U = c.F - c.S;
+        if (r < U) {
+            const int pr = map[r];
+            const T *src = pool + c.foff + c.S + r + i64(c.S) * c.ld;
+            for (int j = 0; j <= r; ++j) M[pr + map[j] * LDM] += src[i64(j) * c.ld];
+        }
         __syncthreads();
     }
-    for (int idx = tid; idx < F * F; idx += 256) {
-        const int r = idx % F, c = idx / F;
-        if (r >= c) fr[r + i64(c) * n.ld] = M[r + c * LDM];
+    // 3. eliminate the S columns
+    for (int j = 0; j < S; ++j) {
+        const double d_in = double(M[j + j * LDM]);
+        const double d = pivot_value(d_in, ash[j], pa);
+        if (r == 0) pivot_record(d_in, d, ash[j], n.first + j, pa, dvec, dcor, st);
+        const T inv = T(1) / T(d);
+        const T w = (rv && r > j) ? M[r + j * LDM] : T(0);
+        const T l = w * inv;
+        if (rv && r > j) lcol[r] = l;
+        __syncthreads();
+        if (rv && r > j) {
+            for (int c = j + 1; c <= r; ++c) M[r + c * LDM] -= w * lcol[c];
+            M[r + j * LDM] = l;
+        }
+        if (r == j) M[j + j * LDM] = T(d);
+        __syncthreads();
     }
+    // 4. lower triangle back to the pool
+    T *fr = pool + n.foff;
+    if (rv)
+        for (int c = 0; c <= r; ++c) fr[r + i64(c) * n.ld] = M[r + c * LDM];
 }
 
 // ---------------------------------------------------------------------------
@@ -700,11 +740,17 @@ static int factor_impl(sprgpu_handle *h, const double *sm_dev, int mode, const d
             k_extend_add<T><<<blocks, 256, 0, s>>>(d.ea_tasks + L.off, d.nodes, d.relmap, pool);
             mark(P_EA, 0.0);
             break;
-        case L_SMALL:
-            k_small<T><<<blocks, 256, 0, s>>>(d.small_nodes + L.off, d.nodes, pool, d.dvec,
-                                              d.adiag, d.dcor, d.status, pa);
+        case L_SMALL: {
+            // L.tile holds the launch's largest front
+            const int mf = L.tile;
+            const size_t shm = (size_t(mf) * (mf + 1) * sizeof(T) + 7) / 8 * 8 +
+                               sizeof(double) * mf + sizeof(T) * mf;
+            k_small<T><<<blocks, 64, shm, s>>>(d.small_nodes + L.off, mf, d.nodes, d.children,
+                                               d.relmap, lnz, pool, d.dvec, d.adiag, d.dcor,
+                                               d.status, pa);
             mark(P_SMALL, 0.0);
             break;
+        }
         case L_PANEL:
             k_panel<T><<<blocks, 256, 0, s>>>(d.panel_desc + L.off, int(L.count),
                                               d.panel_prefix + L.off, d.nodes, pool, d.dvec,
@@ -842,15 +888,15 @@ int copy_factor_to_sm(sprgpu_handle *h, double *sm_out)
                p.nodes[node_limit].first + p.nodes[node_limit].S <= diag_limit)
             ++node_limit;
     }
-    const unsigned nt = unsigned(p.init_tasks.size());
+    const unsigned nt = unsigned(p.copy_tasks.size());
     if (nt) {
         if (h->precision == SPRGPU_FP32)
             k_copy_out<float><<<nt, 256, 0, h->stream>>>(
-                h->d.init_tasks, h->d.nodes, static_cast<const float *>(h->d.pool), sm_out,
+                h->d.copy_tasks, h->d.nodes, static_cast<const float *>(h->d.pool), sm_out,
                 p.neq, node_limit);
         else
             k_copy_out<double><<<nt, 256, 0, h->stream>>>(
-                h->d.init_tasks, h->d.nodes, static_cast<const double *>(h->d.pool), sm_out,
+                h->d.copy_tasks, h->d.nodes, static_cast<const double *>(h->d.pool), sm_out,
                 p.neq, node_limit);
     }
     k_copy_diag<<<256, 256, 0, h->stream>>>(h->d.dvec, h->d.adiag, sm_out, p.neq, diag_limit);
@@ -861,14 +907,14 @@ int copy_factor_to_sm(sprgpu_handle *h, double *sm_out)
 int load_factor_from_sm(sprgpu_handle *h, const double *sm_dev)
 {
     const Plan &p = h->plan;
-    const unsigned nt = unsigned(p.init_tasks.size());
+    const unsigned nt = unsigned(p.copy_tasks.size());
     if (nt) {
         if (h->precision == SPRGPU_FP32)
             k_load_factor<float><<<nt, 256, 0, h->stream>>>(
-                h->d.init_tasks, h->d.nodes, static_cast<float *>(h->d.pool), sm_dev, p.neq);
+                h->d.copy_tasks, h->d.nodes, static_cast<float *>(h->d.pool), sm_dev, p.neq);
         else
             k_load_factor<double><<<nt, 256, 0, h->stream>>>(
-                h->d.init_tasks, h->d.nodes, static_cast<double *>(h->d.pool), sm_dev, p.neq);
+                h->d.copy_tasks, h->d.nodes, static_cast<double *>(h->d.pool), sm_dev, p.neq);
     }
     SPRGPU_HIP(hipMemcpyAsync(h->d.dvec, sm_dev, sizeof(double) * p.neq,
                               hipMemcpyDeviceToDevice, h->stream));

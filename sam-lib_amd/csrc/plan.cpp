@@ -37,6 +37,7 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
                        Plan &p)
 {
     p.big_tile = (precision == SPRGPU_FP32) ? kTileBig : kTileSmall;
+    p.small_f = kSmallF;
     auto M = [&](int k) { return mspar[k - 1]; };
     if (M(1) < 0) return "MSPAR(1) < 0: the symbolic phase did not complete";
     p.neq = M(8);
@@ -143,6 +144,10 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
     // schedule
     // ----------------------------------------------------------------------
     p.init_tasks.clear();
+    p.copy_tasks.clear();
+    for (i64 j = 0; j < ns; ++j)
+        for (i32 c = 0; c < p.nodes[j].S; c += kInitCols)
+            p.copy_tasks.push_back(ColTask{i32(j), c, std::min(p.nodes[j].S, c + kInitCols), 0});
     p.ea_tasks.clear();
     p.small_nodes.clear();
     p.panel_desc.clear();
@@ -187,21 +192,27 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
     for (int l = 0; l < p.nlevels; ++l) {
         const i32 *lv = &p.level_nodes[p.level_ptr[l]];
         const i64 nl = p.level_ptr[l + 1] - p.level_ptr[l];
+        // Fronts with F <= small_f are assembled and factored in LDS by one
+        // launch (k_small); the others are initialised, extend-added and
+        // factored in blocks.
+        std::vector<i32> small, big;
+        for (i64 q = 0; q < nl; ++q)
+            (p.nodes[lv[q]].F <= p.small_f ? small : big).push_back(lv[q]);
         // 1. front initialisation
         i64 off = i64(p.init_tasks.size());
-        for (i64 q = 0; q < nl; ++q) {
-            const NodeInfo &n = p.nodes[lv[q]];
+        for (i32 j : big) {
+            const NodeInfo &n = p.nodes[j];
             for (i32 c = 0; c < n.F; c += kInitCols)
-                p.init_tasks.push_back(ColTask{lv[q], c, std::min(n.F, c + kInitCols), 0});
+                p.init_tasks.push_back(ColTask{j, c, std::min(n.F, c + kInitCols), 0});
         }
         add_launch(L_INIT, l, off, i64(p.init_tasks.size()) - off, i64(p.init_tasks.size()) - off);
-        // 2. extend-add, one round per sibling index
+        // 2. extend-add into the blocked fronts, one round per sibling index
         i32 maxch = 0;
-        for (i64 q = 0; q < nl; ++q) maxch = std::max(maxch, p.nodes[lv[q]].nchild);
+        for (i32 j : big) maxch = std::max(maxch, p.nodes[j].nchild);
         for (i32 r = 0; r < maxch; ++r) {
             off = i64(p.ea_tasks.size());
-            for (i64 q = 0; q < nl; ++q) {
-                const NodeInfo &n = p.nodes[lv[q]];
+            for (i32 j : big) {
+                const NodeInfo &n = p.nodes[j];
                 if (r >= n.nchild) continue;
                 const i32 ch = p.children[n.child0 + r];
                 const i32 U = p.nodes[ch].F - p.nodes[ch].S;
@@ -210,17 +221,15 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
             }
             add_launch(L_EA, l, off, i64(p.ea_tasks.size()) - off, i64(p.ea_tasks.size()) - off);
         }
-        // 3. small fronts in LDS
+        // 3. small fronts: assembly and factor in LDS (tile = largest F)
         off = i64(p.small_nodes.size());
-        std::vector<i32> big;
-        for (i64 q = 0; q < nl; ++q) {
-            if (p.nodes[lv[q]].F <= kSmallF)
-                p.small_nodes.push_back(lv[q]);
-            else
-                big.push_back(lv[q]);
+        i32 maxf_small = 0;
+        for (i32 j : small) {
+            p.small_nodes.push_back(j);
+            maxf_small = std::max(maxf_small, p.nodes[j].F);
         }
         add_launch(L_SMALL, l, off, i64(p.small_nodes.size()) - off,
-                   i64(p.small_nodes.size()) - off);
+                   i64(p.small_nodes.size()) - off, 0.0, maxf_small);
         // 4. blocked fronts: panel steps with trailing updates inside the panel,
         //    then one Schur-complement update of the U block with K = S.
         i32 maxsteps = 0;

@@ -25,7 +25,7 @@ constexpr int kFrontLdAlign = 16; // front leading dimensions are multiples of t
 constexpr int kPanelNB = 32;     // columns per panel step of the blocked front factor
 constexpr int kPanelRows = 256;  // panel rows per workgroup (one row per thread)
 constexpr int kPanelOuter = 256; // outer block of the two-level panel blocking
-constexpr int kSmallF = 64;      // fronts with F <= kSmallF are factored in LDS
+constexpr int kSmallF = 64;      // fronts with F <= kSmallF can be factored in LDS (one lane a row)
 constexpr int kInitCols = 8;     // front columns per workgroup in the init kernel
 constexpr int kEaCols = 8;       // child columns per workgroup in extend-add
 constexpr int kSolveSmallS = 64;   // narrow supernodes: S <= this ...
@@ -97,13 +97,15 @@ struct Plan {
     i64 pool_elems = 0, vwork_elems = 0;
     i32 nlevels = 0;
     i32 big_tile = 0;               // MFMA tile of the rank-kPanelOuter and Schur updates
+    i32 small_f = 0;                // fronts with F <= small_f are factored in LDS (<= kSmallF)
     std::vector<NodeInfo> nodes;
     std::vector<i32> children;
     std::vector<i32> lindx;         // 0-based equations, NOFSUB
     std::vector<i32> relmap;        // per child: its U rows -> parent local row
     std::vector<i32> level_ptr, level_nodes;  // nodes grouped by height
     // factor schedule
-    std::vector<ColTask> init_tasks;
+    std::vector<ColTask> init_tasks;   // front initialisation of the blocked fronts
+    std::vector<ColTask> copy_tasks;   // the S columns of every supernode (factor <-> SM)
     std::vector<EaTask> ea_tasks;
     std::vector<i32> small_nodes;
     std::vector<PanelDesc> panel_desc;
@@ -157,7 +159,7 @@ struct FactorStatus {
 struct DeviceBuffers {
     NodeInfo *nodes = nullptr;
     i32 *children = nullptr, *lindx = nullptr, *relmap = nullptr;
-    ColTask *init_tasks = nullptr;
+    ColTask *init_tasks = nullptr, *copy_tasks = nullptr;
     EaTask *ea_tasks = nullptr;
     i32 *small_nodes = nullptr;
     i32 *level_nodes = nullptr;
