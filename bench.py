@@ -85,9 +85,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel timing pass")
     ap.add_argument("--config", default="factor_solve",
-                    choices=["factor_solve", "adjoint"],
+                    choices=["factor_solve", "adjoint", "frames"],
                     help="factor_solve: the headline (40^3, FP64 vs FP32+IR); adjoint: "
-                         "dL/dE of the compliance at 40^3 (E seed 4)")
+                         "dL/dE of the compliance at 40^3 (E seed 4); frames: 512 beam "
+                         "frames, FP64 factor+solve in one graph")
     ap.add_argument("--fd", type=int, default=64, help="adjoint: elements checked by finite differences")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -96,6 +97,8 @@ def main():
         sys.exit(2)
     if args.config == "adjoint":
         return bench_adjoint(args)
+    if args.config == "frames":
+        return bench_frames(args)
 
     import torch
     from sprgpu import host
@@ -334,6 +337,131 @@ def bench_adjoint(args):
         "vs_fp64_cpu_adjoint": ref or None,
     }
     sysobj.close()
+    print(json.dumps(result), flush=True)
+
+
+def _frame_ref_solve(item):
+    """Reference SPRSOL (factor + solve) of one frame: (seconds, x)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import refspr  # the reference build, oracle/_ref: CPU-baseline leg only
+
+    sysm, sm, b = item
+    x = np.asfortranarray(b[:, None].copy())
+    ierr, secs, _ = refspr.sol(3, sysm, sm.copy(), x)
+    assert ierr == 0
+    return secs, x[:, 0]
+
+
+def bench_frames(args):
+    """512 random 3D Euler-Bernoulli frames (node counts U[80,900], seed 3, one
+    node clamped each, one U[-1,1] RHS each) merged into one forest: FP64
+    factor + solve of all of them as one captured hipGraph, with the schedule
+    parameters chosen by sprgpu_autotune.  Checked against the reference's
+    SPRSOL per frame, which is also the CPU baseline (one core, and all host
+    cores with one frame per process)."""
+    import ctypes as C
+    from multiprocessing import Pool
+
+    import torch
+    from sprgpu import _lib, fem, host
+    from sprgpu.device import SprFactor
+
+    nsys = 512
+    rng = np.random.default_rng(3)
+    nodes = rng.integers(80, 901, nsys)
+    t = time.perf_counter()
+    models = [fem.beam_frame(int(n), rng) for n in nodes]
+    systems = [host.symbolic(m) for m in models]
+    sms = [host.assemble(m, s) for m, s in zip(models, systems)]
+    bs = [host.loads_to_equations(s, fem.uniform_loads(m.ndof, rng))[:, 0].copy()
+          for m, s in zip(models, systems)]
+    merged = host.merge_systems(systems)
+    sm = merged.merge_sm(sms)
+    b = np.concatenate(bs)
+    log(f"frames set-up: {nsys} systems, NEQ {min(s.neq for s in systems)}..{max(s.neq for s in systems)}, "
+        f"total {merged.neq}, NZEROL {merged.nzerol} ({time.perf_counter() - t:.1f}s)")
+    dev = torch.device("cuda:0")
+    sm_dev = torch.from_numpy(sm).to(dev)
+    b_dev = torch.from_numpy(b).to(dev).view(1, -1)
+    x_dev = torch.empty_like(b_dev)
+    h = SprFactor(merged.mspar, merged.mtrees, merged.msifa, "fp64")
+    L = _lib.lib()
+    report = np.zeros(256)
+    x_dev.copy_(b_dev)
+    t = time.perf_counter()
+    _lib.check(L.sprgpu_autotune(h._h, C.c_void_p(sm_dev.data_ptr()), C.c_void_p(x_dev.data_ptr()),
+                                 merged.neq, 1, report.ctypes.data_as(C.c_void_p)), "autotune")
+    torch.cuda.synchronize()
+    params = (C.c_int * 4)()
+    L.sprgpu_plan_params(h._h, params)
+    ncand = int(report[0])
+    cands = report[1:1 + 5 * ncand].reshape(ncand, 5)
+    log(f"autotune: {ncand} candidates in {time.perf_counter() - t:.1f}s, chose small_f={params[0]} "
+        f"waves={params[1]} narrow={params[2]}; ms best {cands[:, 4].min():.3f} worst {cands[:, 4].max():.3f}")
+
+    def run():
+        x_dev.copy_(b_dev)
+        _lib.check(L.sprgpu_factor_solve_graph(h._h, C.c_void_p(sm_dev.data_ptr()),
+                                               C.c_void_p(x_dev.data_ptr()), merged.neq, 1),
+                   "factor_solve_graph")
+
+    for _ in range(max(args.warmup, 1)):
+        run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run()
+    torch.cuda.synchronize()
+    tg = (time.perf_counter() - t0) / args.steps
+    info = h.factor_info()
+    assert info["ierr"] == 0, info
+    xg = x_dev.view(-1).cpu().numpy()
+    # eager sequence for comparison (same kernels, launched one by one)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        x_dev.copy_(b_dev)
+        h.factor(sm_dev, info=False)
+        h.solve(x_dev)
+    torch.cuda.synchronize()
+    te = (time.perf_counter() - t0) / args.steps
+    assert torch.equal(x_dev.view(-1).cpu(), torch.from_numpy(xg)), "graph and eager differ"
+    log(f"graph {tg*1e3:.2f} ms per batch, eager {te*1e3:.2f} ms")
+    result = {
+        "metric": "512 beam frames (500-5400 dofs each): FP64 factor+solve of the batch",
+        "value": tg, "unit": "s per batch", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "512 3D Euler-Bernoulli frames, nodes U[80,900], seed 3, "
+                               "one U[-1,1] RHS each, merged into one forest",
+                   "neq_total": merged.neq, "nzerol_total": merged.nzerol,
+                   "systems": nsys},
+        "graph_s": tg, "eager_s": te,
+        "autotune": {"candidates": ncand, "chosen": {"small_f": params[0], "waves": params[1],
+                                                     "narrow_threads": params[2]},
+                     "best_ms": float(cands[:, 4].min()), "worst_ms": float(cands[:, 4].max())},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        items = list(zip(systems, sms, bs))
+        t0 = time.perf_counter()
+        outs = [_frame_ref_solve(it) for it in items]
+        t1 = time.perf_counter() - t0
+        ncores = min(16, os.cpu_count() or 1)
+        t0 = time.perf_counter()
+        with Pool(ncores) as pool:
+            pool.map(_frame_ref_solve, items, chunksize=8)
+        tp = time.perf_counter() - t0
+        xr = np.concatenate([o[1] for o in outs])
+        errs = [np.linalg.norm(a - r) / np.linalg.norm(r)
+                for a, r in zip(merged.split(xg), merged.split(xr))]
+        result["max_rel_err_vs_reference"] = float(max(errs))
+        result["cpu_baseline"] = {
+            "value": t1, "unit": "s per batch", "cores": 1, "kind": "reference",
+            "sample": f"all {nsys} frames, reference SPRSOL(IOP=3) one after another on one core "
+                      f"(sum of per-call {sum(o[0] for o in outs):.2f}s)",
+            "all_cores": {"value": tp, "cores": ncores}}
+        log(f"reference: {t1:.2f}s on one core, {tp:.2f}s on {ncores} processes; "
+            f"max rel err {max(errs):.2e}")
+    h.close()
     print(json.dumps(result), flush=True)
 
 

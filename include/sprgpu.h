@@ -185,6 +185,28 @@ int sprgpu_profile_factor(sprgpu_handle *handle, const double *sm_dev, const dou
 int sprgpu_plan_summary(const sprgpu_int *mspar, const sprgpu_int *mtrees,
                         const sprgpu_int *msifa, double *out);
 
+/* Factor (SPRFC1 pivot rules, the handle's precision) followed by the
+ * forward and backward solves of B(LDB,NRHS), as one hipGraph: captured on
+ * the first call and replayed by later calls with the same sm_dev, b_dev, ldb
+ * and nrhs (their contents may change; other pointers capture a new graph).
+ * No counterpart in the reference; for callers that repeat factor+solve on
+ * new values of one structure (batches of small systems, time stepping).
+ * Errors of the factor are read afterwards with sprgpu_factor_info_get. */
+int sprgpu_factor_solve_graph(sprgpu_handle *handle, const double *sm_dev, double *b_dev,
+                              sprgpu_int ldb, sprgpu_int nrhs);
+
+/* Chooses the schedule parameters (small-front threshold, fronts per
+ * workgroup, narrow-solve workgroup size, FP32 update tile) for this
+ * structure by timing factor + solve on sm_dev / b_dev for each candidate,
+ * and keeps the fastest.  b_dev is overwritten.  report (NULL or 256 doubles):
+ * report[0] = candidates, then per candidate small_f, waves, narrow threads,
+ * tile, milliseconds.  No counterpart in the reference. */
+int sprgpu_autotune(sprgpu_handle *handle, const double *sm_dev, double *b_dev, sprgpu_int ldb,
+                    sprgpu_int nrhs, double *report);
+/* The parameters in use: out[0..3] = small_f, fronts per workgroup, narrow
+ * threads (0 = by size), update tile. */
+int sprgpu_plan_params(const sprgpu_handle *handle, int *out);
+
 /* Last error message (thread-local, empty if none). */
 const char *sprgpu_last_error(void);
 

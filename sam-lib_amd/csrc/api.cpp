@@ -44,10 +44,85 @@ static void free_all(sprgpu_handle *h)
                     d.csr_src,  d.csr_val,    d.irwork,      d.level_nodes,
                     d.solve_small, d.wide,     d.wide_fwd_prefix, d.wide_bwd_prefix,
                     d.wide_rect_prefix, d.upd_ptr, d.upd_src, d.flags, d.partials,
-                    d.spin_error, d.col_node};
+                    d.spin_error, d.col_node, d.epoch};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     d = DeviceBuffers{};
+}
+
+
+// Device copies of everything the planner derives (descriptor and index
+// arrays); replaced as a whole when the plan is rebuilt.
+static void free_plan_buffers(sprgpu_handle *h)
+{
+    DeviceBuffers &d = h->d;
+    void **ptrs[] = {(void **)&d.nodes,         (void **)&d.children,
+                     (void **)&d.lindx,         (void **)&d.relmap,
+                     (void **)&d.init_tasks,    (void **)&d.copy_tasks,
+                     (void **)&d.ea_tasks,      (void **)&d.small_nodes,
+                     (void **)&d.panel_desc,    (void **)&d.panel_prefix,
+                     (void **)&d.gemm_desc,     (void **)&d.gemm_prefix,
+                     (void **)&d.level_nodes,   (void **)&d.solve_small,
+                     (void **)&d.wide,          (void **)&d.wide_fwd_prefix,
+                     (void **)&d.wide_bwd_prefix, (void **)&d.wide_rect_prefix,
+                     (void **)&d.upd_ptr,       (void **)&d.upd_src,
+                     (void **)&d.col_node};
+    for (void **q : ptrs) {
+        if (*q) (void)hipFree(*q);
+        *q = nullptr;
+    }
+}
+
+static int upload_plan(sprgpu_handle *h)
+{
+    const Plan &p = h->plan;
+    DeviceBuffers &d = h->d;
+    sprgpu_int &B = h->device_bytes;
+    int rc = 0;
+    if ((rc = upload(d.nodes, p.nodes, B)) || (rc = upload(d.children, p.children, B)) ||
+        (rc = upload(d.lindx, p.lindx, B)) || (rc = upload(d.relmap, p.relmap, B)) ||
+        (rc = upload(d.init_tasks, p.init_tasks, B)) || (rc = upload(d.ea_tasks, p.ea_tasks, B)) ||
+        (rc = upload(d.copy_tasks, p.copy_tasks, B)) ||
+        (rc = upload(d.small_nodes, p.small_nodes, B)) ||
+        (rc = upload(d.panel_desc, p.panel_desc, B)) ||
+        (rc = upload(d.panel_prefix, p.panel_prefix, B)) ||
+        (rc = upload(d.gemm_desc, p.gemm_desc, B)) ||
+        (rc = upload(d.gemm_prefix, p.gemm_prefix, B)) ||
+        (rc = upload(d.level_nodes, p.level_nodes, B)) ||
+        (rc = upload(d.solve_small, p.solve_small, B)) ||
+        (rc = upload(d.wide, p.wide, B)) ||
+        (rc = upload(d.wide_fwd_prefix, p.wide_fwd_prefix, B)) ||
+        (rc = upload(d.wide_bwd_prefix, p.wide_bwd_prefix, B)) ||
+        (rc = upload(d.wide_rect_prefix, p.wide_rect_prefix, B)) ||
+        (rc = upload(d.upd_ptr, p.upd_ptr, B)) || (rc = upload(d.upd_src, p.upd_src, B)))
+        return rc;
+    return 0;
+}
+
+static void drop_graph(sprgpu_handle *h)
+{
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+    h->graph_exec = nullptr;
+    h->graph_sm = nullptr;
+    h->graph_b = nullptr;
+}
+
+// Rebuilds the plan with new parameters (same symbolic input, same data
+// layout) and replaces the device descriptor arrays.
+static int replan(sprgpu_handle *h, const PlanParams &params)
+{
+    Plan np;
+    std::string err = build_plan(h->mspar.data(), h->mtrees.data(), h->msifa.data(), h->precision,
+                                 params, np);
+    if (!err.empty()) {
+        set_error(err);
+        return -2;
+    }
+    SPRGPU_HIP(hipStreamSynchronize(h->stream));
+    drop_graph(h);
+    free_plan_buffers(h);
+    h->plan = std::move(np);
+    return upload_plan(h);
 }
 
 }  // namespace sprgpu
@@ -71,35 +146,29 @@ int sprgpu_create(const sprgpu_int *mspar, const sprgpu_int *mtrees, const sprgp
     }
     std::unique_ptr<sprgpu_handle> h(new sprgpu_handle);
     h->precision = precision;
-    std::string err = build_plan(mspar, mtrees, msifa, precision, h->plan);
+    std::string err = build_plan(mspar, mtrees, msifa, precision, PlanParams{}, h->plan);
     if (!err.empty()) {
         set_error(err);
         return -2;
     }
+    {
+        // the parts of the symbolic arrays the planner reads (to re-plan later)
+        auto M = [&](int k) { return mspar[k - 1]; };
+        const sprgpu_int ns = M(11);
+        const sprgpu_int nt = std::max(M(45) - 1 + 3 * (ns + 1), M(46) - 1 + 2 * ns + 1);
+        const sprgpu_int nf = std::max(M(47) - 1 + M(15), M(48) - 1 + ns + 1);
+        h->mspar.assign(mspar, mspar + 58);
+        h->mtrees.assign(mtrees, mtrees + nt);
+        h->msifa.assign(msifa, msifa + nf);
+    }
     SPRGPU_HIP(hipGetDevice(&h->device));
     const Plan &p = h->plan;
     DeviceBuffers &d = h->d;
-    sprgpu_int &B = h->device_bytes;
-    int rc = 0;
-    if ((rc = upload(d.nodes, p.nodes, B)) || (rc = upload(d.children, p.children, B)) ||
-        (rc = upload(d.lindx, p.lindx, B)) || (rc = upload(d.relmap, p.relmap, B)) ||
-        (rc = upload(d.init_tasks, p.init_tasks, B)) || (rc = upload(d.ea_tasks, p.ea_tasks, B)) ||
-        (rc = upload(d.copy_tasks, p.copy_tasks, B)) ||
-        (rc = upload(d.small_nodes, p.small_nodes, B)) ||
-        (rc = upload(d.panel_desc, p.panel_desc, B)) ||
-        (rc = upload(d.panel_prefix, p.panel_prefix, B)) ||
-        (rc = upload(d.gemm_desc, p.gemm_desc, B)) ||
-        (rc = upload(d.gemm_prefix, p.gemm_prefix, B)) ||
-        (rc = upload(d.level_nodes, p.level_nodes, B)) ||
-        (rc = upload(d.solve_small, p.solve_small, B)) ||
-        (rc = upload(d.wide, p.wide, B)) ||
-        (rc = upload(d.wide_fwd_prefix, p.wide_fwd_prefix, B)) ||
-        (rc = upload(d.wide_bwd_prefix, p.wide_bwd_prefix, B)) ||
-        (rc = upload(d.wide_rect_prefix, p.wide_rect_prefix, B)) ||
-        (rc = upload(d.upd_ptr, p.upd_ptr, B)) || (rc = upload(d.upd_src, p.upd_src, B))) {
+    if (int rc = upload_plan(h.get())) {
         free_all(h.get());
         return rc;
     }
+    sprgpu_int &B = h->device_bytes;
     const size_t esz = precision == SPRGPU_FP32 ? sizeof(float) : sizeof(double);
     hipError_t e = hipMalloc(&d.pool, esz * size_t(p.pool_elems));
     if (e == hipSuccess) e = hipMalloc(&d.dvec, sizeof(double) * p.neq);
@@ -126,8 +195,121 @@ int sprgpu_destroy(sprgpu_handle *h)
 {
     if (!h) return 0;
     (void)hipDeviceSynchronize();
+    drop_graph(h);
+    if (h->capture_stream) (void)hipStreamDestroy(h->capture_stream);
     free_all(h);
     delete h;
+    return 0;
+}
+
+int sprgpu_factor_solve_graph(sprgpu_handle *h, const double *sm_dev, double *b_dev,
+                              sprgpu_int ldb, sprgpu_int nrhs)
+{
+    if (!h || !sm_dev || !b_dev || nrhs < 1 || ldb < h->plan.neq) return -2;
+    const bool same = h->graph_exec && h->graph_sm == sm_dev && h->graph_b == b_dev &&
+                      h->graph_ldb == ldb && h->graph_nrhs == nrhs;
+    if (!same) {
+        drop_graph(h);
+        const double tol[4] = {0.0, 0.0, 0.0, 0.0};
+        // every buffer the sequence needs exists before capturing (the
+        // factor's are allocated with the handle)
+        if (int rc = prepare_solve(h, nrhs)) return rc;
+        SPRGPU_HIP(hipStreamSynchronize(h->stream));
+        if (!h->capture_stream)
+            SPRGPU_HIP(hipStreamCreateWithFlags(&h->capture_stream, hipStreamNonBlocking));
+        hipStream_t user = h->stream;
+        h->stream = h->capture_stream;
+        hipGraph_t g = nullptr;
+        SPRGPU_HIP(hipStreamBeginCapture(h->capture_stream, hipStreamCaptureModeThreadLocal));
+        int rc = run_factor(h, sm_dev, 1, tol);
+        if (rc == 0) rc = run_solve(h, b_dev, ldb, nrhs, 3);
+        const hipError_t e = hipStreamEndCapture(h->capture_stream, &g);
+        h->stream = user;
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+        const hipError_t ei = hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            h->graph_exec = nullptr;
+            return hip_fail(ei, "hipGraphInstantiate");
+        }
+        h->graph_sm = sm_dev;
+        h->graph_b = b_dev;
+        h->graph_ldb = ldb;
+        h->graph_nrhs = nrhs;
+    }
+    SPRGPU_HIP(hipGraphLaunch(h->graph_exec, h->stream));
+    h->factored = true;
+    h->last_sm_dev = sm_dev;
+    return 0;
+}
+
+int sprgpu_autotune(sprgpu_handle *h, const double *sm_dev, double *b_dev, sprgpu_int ldb,
+                    sprgpu_int nrhs, double *report)
+{
+    if (!h || !sm_dev || !b_dev || nrhs < 1 || ldb < h->plan.neq) return -2;
+    std::vector<PlanParams> cand;
+    const int tiles[2] = {kTileSmall, kTileBig};
+    for (int sf : {32, 48, 64})
+        for (int w : {1, 2, 4})
+            for (int nt : {64, 256})
+                for (int t = 0; t < (h->precision == SPRGPU_FP32 ? 2 : 1); ++t) {
+                    PlanParams pp;
+                    pp.small_f = sf;
+                    pp.small_waves = w;
+                    pp.narrow_threads = nt;
+                    pp.big_tile = (h->precision == SPRGPU_FP32) ? tiles[t] : 0;
+                    cand.push_back(pp);
+                }
+    const double tol[4] = {0.0, 0.0, 0.0, 0.0};
+    hipEvent_t e0, e1;
+    SPRGPU_HIP(hipEventCreate(&e0));
+    SPRGPU_HIP(hipEventCreate(&e1));
+    double best = 1e300;
+    size_t ibest = 0;
+    int rc = 0;
+    for (size_t i = 0; i < cand.size() && rc == 0; ++i) {
+        if ((rc = replan(h, cand[i]))) break;
+        if ((rc = run_factor(h, sm_dev, 1, tol)) || (rc = run_solve(h, b_dev, ldb, nrhs, 3))) break;
+        float ms = 0.f;
+        (void)hipEventRecord(e0, h->stream);
+        for (int k = 0; k < 3 && rc == 0; ++k)
+            if (!(rc = run_factor(h, sm_dev, 1, tol))) rc = run_solve(h, b_dev, ldb, nrhs, 3);
+        (void)hipEventRecord(e1, h->stream);
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = hip_fail(hipGetLastError(), "autotune timing");
+            break;
+        }
+        const double t = ms / 3.0;
+        if (report && 5 * i + 5 < 256) {
+            report[1 + 5 * i] = cand[i].small_f;
+            report[2 + 5 * i] = cand[i].small_waves;
+            report[3 + 5 * i] = cand[i].narrow_threads;
+            report[4 + 5 * i] = cand[i].big_tile;
+            report[5 + 5 * i] = t;
+        }
+        if (t < best) {
+            best = t;
+            ibest = i;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (report) report[0] = double(cand.size());
+    if (rc) return rc;
+    return replan(h, cand[ibest]);
+}
+
+int sprgpu_plan_params(const sprgpu_handle *h, int *out)
+{
+    if (!h || !out) return -2;
+    out[0] = h->plan.params.small_f;
+    out[1] = h->plan.params.small_waves;
+    out[2] = h->plan.params.narrow_threads;
+    out[3] = h->plan.big_tile;
     return 0;
 }
 
@@ -247,7 +429,7 @@ int sprgpu_plan_summary(const sprgpu_int *mspar, const sprgpu_int *mtrees,
 {
     if (!mspar || !mtrees || !msifa || !out) return -2;
     Plan p;
-    std::string err = build_plan(mspar, mtrees, msifa, SPRGPU_FP64, p);
+    std::string err = build_plan(mspar, mtrees, msifa, SPRGPU_FP64, PlanParams{}, p);
     if (!err.empty()) {
         set_error(err);
         return -2;

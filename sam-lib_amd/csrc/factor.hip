@@ -187,29 +187,48 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask *__restrict__ t
 // ---------------------------------------------------------------------------
 // small fronts (F <= kSmallF): assembly and factor in LDS, one wavefront each
 // ---------------------------------------------------------------------------
-// The workgroup (64 threads, lane = front row) builds its front in LDS: the
-// supernode's trapezoid from SM (as k_front_init), then the children's
-// update blocks added in SIBL order (as k_extend_add), then eliminates the S
-// columns (SPRF17/LMMY18's recurrence: l = w * (1/d), a_rc -= w_r l_c) and
-// writes the lower triangle back to the pool, where the parent's extend-add
-// and the solves read it.  Dynamic LDS: F x (F+1) elements of T (F = the
-// launch's largest front) plus S pivots' a_jj and one column.
+// Each wavefront (lane = front row) builds its front in LDS: the supernode's
+// trapezoid from SM (as k_front_init), then the children's update blocks
+// added in SIBL order (as k_extend_add), then eliminates the S columns
+// (SPRF17/LMMY18's recurrence: l = w * (1/d), a_rc -= w_r l_c) and writes the
+// lower triangle back to the pool, where the parent's extend-add and the
+// solves read it.  A workgroup holds `waves` fronts, one per wavefront; the
+// wavefronts synchronise only among their own lanes.  Dynamic LDS per
+// wavefront: F x (F+1) elements of T (F = the launch's largest front) plus
+// the S pivots' a_jj and one column.
+__device__ __forceinline__ size_t small_front_bytes(int maxF, size_t esz)
+{
+    return ((size_t(maxF) * (maxF + 1) * esz + 7) / 8 * 8 + 8 * size_t(maxF) + esz * maxF + 15) /
+           16 * 16;
+}
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename T>
-__global__ __launch_bounds__(64) void k_small(const i32 *__restrict__ list, int maxF,
-                                              const NodeInfo *__restrict__ nodes,
-                                              const i32 *__restrict__ children,
-                                              const i32 *__restrict__ relmap,
-                                              const double *__restrict__ lnz,
-                                              T *__restrict__ pool, double *dvec, double *adiag,
-                                              double *dcor, FactorStatus *st, PivotArgs pa)
+__global__ __launch_bounds__(256) void k_small(const i32 *__restrict__ list, int count, int maxF,
+                                               const NodeInfo *__restrict__ nodes,
+                                               const i32 *__restrict__ children,
+                                               const i32 *__restrict__ relmap,
+                                               const double *__restrict__ lnz,
+                                               T *__restrict__ pool, double *dvec, double *adiag,
+                                               double *dcor, FactorStatus *st, PivotArgs pa)
 {
     extern __shared__ double smem_small[];
+    const int wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+    const int q = blockIdx.x * waves + wave;
+    if (q >= count) return;
     const int LDM = maxF + 1;
-    T *M = reinterpret_cast<T *>(smem_small);       // M[r + c*LDM], lower triangle
-    double *ash = smem_small + (maxF * LDM * sizeof(T) + 7) / 8;  // a_jj of the S pivots
-    T *lcol = reinterpret_cast<T *>(ash + maxF);    // the scaled column of the step
-    const NodeInfo n = nodes[list[blockIdx.x]];
-    const int F = n.F, S = n.S, r = threadIdx.x;
+    char *base = reinterpret_cast<char *>(smem_small) + wave * small_front_bytes(maxF, sizeof(T));
+    T *M = reinterpret_cast<T *>(base);  // M[r + c*LDM], lower triangle
+    double *ash = reinterpret_cast<double *>(base + (size_t(maxF) * LDM * sizeof(T) + 7) / 8 * 8);
+    T *lcol = reinterpret_cast<T *>(ash + maxF);  // the scaled column of the step
+    const NodeInfo n = nodes[list[q]];
+    const int F = n.F, S = n.S, r = threadIdx.x & 63;
     const bool rv = r < F;
     // 1. trapezoid from SM, zero elsewhere
     unsigned long long nnz = 0;
@@ -242,7 +261,7 @@ __global__ __launch_bounds__(64) void k_small(const i32 *__restrict__ list, int 
     }
     for (int o = 32; o > 0; o >>= 1) nnz += __shfl_down(nnz, o);
     if (r == 0 && nnz) atomicAdd(&st->nnz_seen, nnz);
-    __syncthreads();
+    wave_sync();
     // 2. children's update blocks, one child at a time in SIBL order
     for (int k = 0; k < n.nchild; ++k) {
         const NodeInfo c = nodes[children[n.child0 + k]];
@@ -253,7 +272,7 @@ __global__ __launch_bounds__(64) void k_small(const i32 *__restrict__ list, int 
             const T *src = pool + c.foff + c.S + r + i64(c.S) * c.ld;
             for (int j = 0; j <= r; ++j) M[pr + map[j] * LDM] += src[i64(j) * c.ld];
         }
-        __syncthreads();
+        wave_sync();
     }
     // 3. eliminate the S columns
     for (int j = 0; j < S; ++j) {
@@ -264,13 +283,13 @@ __global__ __launch_bounds__(64) void k_small(const i32 *__restrict__ list, int 
         const T w = (rv && r > j) ? M[r + j * LDM] : T(0);
         const T l = w * inv;
         if (rv && r > j) lcol[r] = l;
-        __syncthreads();
+        wave_sync();
         if (rv && r > j) {
             for (int c = j + 1; c <= r; ++c) M[r + c * LDM] -= w * lcol[c];
             M[r + j * LDM] = l;
         }
         if (r == j) M[j + j * LDM] = T(d);
-        __syncthreads();
+        wave_sync();
     }
     // 4. lower triangle back to the pool
     T *fr = pool + n.foff;
@@ -742,12 +761,13 @@ static int factor_impl(sprgpu_handle *h, const double *sm_dev, int mode, const d
             break;
         case L_SMALL: {
             // L.tile holds the launch's largest front
-            const int mf = L.tile;
-            const size_t shm = (size_t(mf) * (mf + 1) * sizeof(T) + 7) / 8 * 8 +
-                               sizeof(double) * mf + sizeof(T) * mf;
-            k_small<T><<<blocks, 64, shm, s>>>(d.small_nodes + L.off, mf, d.nodes, d.children,
-                                               d.relmap, lnz, pool, d.dvec, d.adiag, d.dcor,
-                                               d.status, pa);
+            const int mf = L.tile, waves = p.params.small_waves;
+            const size_t per = ((size_t(mf) * (mf + 1) * sizeof(T) + 7) / 8 * 8 +
+                                8 * size_t(mf) + sizeof(T) * mf + 15) / 16 * 16;
+            const unsigned nb = unsigned((L.count + waves - 1) / waves);
+            k_small<T><<<nb, 64 * waves, per * waves, s>>>(
+                d.small_nodes + L.off, int(L.count), mf, d.nodes, d.children, d.relmap, lnz, pool,
+                d.dvec, d.adiag, d.dcor, d.status, pa);
             mark(P_SMALL, 0.0);
             break;
         }

@@ -34,10 +34,19 @@ inline i64 round_up(i64 x, i64 m) { return (x + m - 1) / m * m; }
 }  // namespace
 
 std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, int precision,
-                       Plan &p)
+                       const PlanParams &params, Plan &p)
 {
-    p.big_tile = (precision == SPRGPU_FP32) ? kTileBig : kTileSmall;
-    p.small_f = kSmallF;
+    p.params = params;
+    if (params.small_f < 0 || params.small_f > kSmallF) return "small_f must be in [0, 64]";
+    if (params.small_waves < 1 || params.small_waves > 4) return "small_waves must be in [1, 4]";
+    if (params.big_tile != 0 && params.big_tile != kTileSmall && params.big_tile != kTileBig)
+        return "big_tile must be 0, 64 or 128";
+    if (params.narrow_threads != 0 && params.narrow_threads != 64 &&
+        params.narrow_threads != 128 && params.narrow_threads != 256)
+        return "narrow_threads must be 0, 64, 128 or 256";
+    p.big_tile = params.big_tile ? params.big_tile
+                                 : (precision == SPRGPU_FP32 ? kTileBig : kTileSmall);
+    p.small_f = params.small_f;
     auto M = [&](int k) { return mspar[k - 1]; };
     if (M(1) < 0) return "MSPAR(1) < 0: the symbolic phase did not complete";
     p.neq = M(8);

@@ -92,7 +92,17 @@ struct Launch {
     i32 outer;       // L_GEMM: 1 for rank-kPanelOuter and Schur updates, 0 for rank-kPanelNB
 };
 
+// Tunable schedule parameters (chosen by sprgpu_autotune or left at their
+// defaults).  Changing them changes launches, not the data layout.
+struct PlanParams {
+    i32 small_f = kSmallF;    // fronts with F <= small_f are assembled and factored in LDS
+    i32 small_waves = 1;      // such fronts per workgroup (one wavefront each)
+    i32 big_tile = 0;         // MFMA tile of rank-kPanelOuter/Schur updates (0: by precision)
+    i32 narrow_threads = 0;   // workgroup size of the narrow solves (0: by front size)
+};
+
 struct Plan {
+    PlanParams params;
     i64 neq = 0, nsuper = 0, nofsub = 0, nzerol = 0, mfront = 0, maxsup = 0;
     i64 pool_elems = 0, vwork_elems = 0;
     i32 nlevels = 0;
@@ -135,7 +145,7 @@ struct Plan {
 // for a factor in the given precision (SPRGPU_FP64 / SPRGPU_FP32: the MFMA
 // update tiles differ).  Returns an empty string or an error message.
 std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, int precision,
-                       Plan &plan);
+                       const PlanParams &params, Plan &plan);
 
 // Pattern of A used by the FP64 residual: CSR over all rows, with the
 // position of each value in the packed LNZ array.
@@ -192,7 +202,7 @@ struct DeviceBuffers {
     i64 flags_cap = 0;
     double *partials = nullptr; // backward partial sums (npart * nrhs)
     i64 partials_cap = 0;
-    unsigned epoch = 0;
+    unsigned *epoch = nullptr; // sweep counter the ready flags are compared with
     int *spin_error = nullptr;  // set when a dataflow wait gave up
     i32 *col_node = nullptr;    // supernode of each equation (adjoint pattern gradient)
     i64 irwork_cap = 0;
@@ -212,6 +222,14 @@ struct sprgpu_handle {
     double tol[4] = {0, 0, 0, 0};
     const double *last_sm_dev = nullptr;  // input of the last factorization
     sprgpu_int device_bytes = 0;
+    // symbolic input, kept to re-plan (sprgpu_autotune)
+    std::vector<sprgpu_int> mspar, mtrees, msifa;
+    // captured factor + solve (sprgpu_factor_solve_graph)
+    hipGraphExec_t graph_exec = nullptr;
+    hipStream_t capture_stream = nullptr;
+    const double *graph_sm = nullptr;
+    double *graph_b = nullptr;
+    sprgpu_int graph_ldb = 0, graph_nrhs = 0;
 };
 
 namespace sprgpu {
@@ -232,6 +250,7 @@ int profile_factor(sprgpu_handle *h, const double *sm_dev, int mode, const doubl
                    double *out);
 // solve.hip
 int run_solve(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which);
+int prepare_solve(sprgpu_handle *h, i64 nrhs);  // allocates the solve buffers for nrhs
 int check_solve_error(sprgpu_handle *h);
 int run_spmv(sprgpu_handle *h, const double *x, double *y, i64 ld, i64 nrhs);
 int run_solve_ir(sprgpu_handle *h, const double *b, double *x, i64 ld, i64 nrhs, double rtol,

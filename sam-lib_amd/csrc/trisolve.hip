@@ -84,6 +84,14 @@ __device__ __forceinline__ void publish_ready(unsigned *flag, unsigned epoch)
         __hip_atomic_store(flag, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The epoch of a sweep lives in device memory and is advanced by a kernel
+// before the sweep, so that a captured sequence (hipGraph) replays with a new
+// epoch each time; flags start at zero and only ever take epoch values.
+__global__ void k_epoch_bump(unsigned *epoch)
+{
+    if (threadIdx.x == 0) *epoch += 1;
+}
+
 // Offset of column c in a packed lower-triangular NB x NB block stored by
 // columns (column c holds rows c..NB-1).
 __device__ __forceinline__ int packed_col(int c) { return c * NB - c * (c - 1) / 2; }
@@ -287,13 +295,14 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
                                                   const T *__restrict__ pool,
                                                   const double *__restrict__ dvec, double *b,
                                                   i64 ldb, double *vwork, i64 vstride,
-                                                  unsigned *flags, i64 nflags, unsigned epoch,
-                                                  int *err)
+                                                  unsigned *flags, i64 nflags,
+                                                  const unsigned *epoch_ptr, int *err)
 {
     __shared__ double xs[NB];
     __shared__ double ys[NB];
     __shared__ double part[2][NB];
-    __shared__ T Ld[kPacked + 1];  // packed lower diagonal block, by columns; Ld[kZero] = 0
+    __shared__ T Ld[kPacked + 1];
+    const unsigned epoch = *epoch_ptr;  // packed lower diagonal block, by columns; Ld[kZero] = 0
     const int q = find_desc(prefix, nwide, blockIdx.x);
     const WideDesc w = wide[q];
     const int local = int(blockIdx.x - prefix[q]);
@@ -479,13 +488,14 @@ __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ w
                                                   const T *__restrict__ pool, double *b, i64 ldb,
                                                   double *vwork, i64 vstride,
                                                   const double *__restrict__ partials, i64 npart,
-                                                  unsigned *flags, i64 nflags, unsigned epoch,
-                                                  int *err)
+                                                  unsigned *flags, i64 nflags,
+                                                  const unsigned *epoch_ptr, int *err)
 {
     __shared__ double xs[NB];
     __shared__ double ys[NB];
     __shared__ double part[2][NB];
-    __shared__ T Lr[kPacked + 1];  // packed lower diagonal block, by rows; Lr[kZero] = 0
+    __shared__ T Lr[kPacked + 1];
+    const unsigned epoch = *epoch_ptr;  // packed lower diagonal block, by rows; Lr[kZero] = 0
     const int q = find_desc(prefix, nwide, blockIdx.x);
     const WideDesc w = wide[q];
     const int blk = w.nblk - 1 - int(blockIdx.x - prefix[q]);
@@ -609,7 +619,8 @@ static int ensure_solve_buffers(sprgpu_handle *h, i64 nrhs)
         SPRGPU_HIP(hipMalloc(&d.flags, sizeof(unsigned) * nf));
         SPRGPU_HIP(hipMemset(d.flags, 0, sizeof(unsigned) * nf));
         d.flags_cap = nf;
-        d.epoch = 0;  // every flag is zero again
+        if (!d.epoch) SPRGPU_HIP(hipMalloc(&d.epoch, sizeof(unsigned)));
+        SPRGPU_HIP(hipMemset(d.epoch, 0, sizeof(unsigned)));  // every flag is zero again
     }
     const i64 np = std::max<i64>(p.npart, 1) * nrhs;
     if (d.partials_cap < np) {
@@ -638,9 +649,12 @@ static int solve_impl(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which)
     const unsigned ur = unsigned(nrhs);
     const i64 nflags = std::max<i64>(p.nflags, 1);
     const i64 npart = std::max<i64>(p.npart, 1);
-    auto narrow_threads = [&](int l) { return p.small_maxF[l] <= 128 ? 64u : 256u; };
+    auto narrow_threads = [&](int l) {
+        if (p.params.narrow_threads) return unsigned(p.params.narrow_threads);
+        return p.small_maxF[l] <= 128 ? 64u : 256u;
+    };
     if (which & 1) {
-        const unsigned ep = ++d.epoch;
+        k_epoch_bump<<<1, 64, 0, s>>>(d.epoch);
         for (int l = 0; l < p.nlevels; ++l) {
             const unsigned ns = unsigned(p.solve_small_ptr[l + 1] - p.solve_small_ptr[l]);
             if (ns) {
@@ -654,12 +668,12 @@ static int solve_impl(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which)
             if (nw)
                 k_fwd_wide<T><<<dim3(unsigned(p.wide_fwd_wgs[l]), ur), 256, 0, s>>>(
                     d.wide + w0, nw, d.wide_fwd_prefix + w0, d.nodes, d.lindx, d.upd_ptr,
-                    d.upd_src, pool, d.dvec, b, ldb, d.vwork, vstride, d.flags, nflags, ep,
+                    d.upd_src, pool, d.dvec, b, ldb, d.vwork, vstride, d.flags, nflags, d.epoch,
                     d.spin_error);
         }
     }
     if (which & 2) {
-        const unsigned ep = ++d.epoch;
+        k_epoch_bump<<<1, 64, 0, s>>>(d.epoch);
         for (int l = p.nlevels - 1; l >= 0; --l) {
             const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
             if (nw) {
@@ -669,7 +683,7 @@ static int solve_impl(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which)
                         d.partials, npart);
                 k_bwd_wide<T><<<dim3(unsigned(p.wide_bwd_wgs[l]), ur), 256, 0, s>>>(
                     d.wide + w0, nw, d.wide_bwd_prefix + w0, d.nodes, d.lindx, pool, b, ldb,
-                    d.vwork, vstride, d.partials, npart, d.flags, nflags, ep, d.spin_error);
+                    d.vwork, vstride, d.partials, npart, d.flags, nflags, d.epoch, d.spin_error);
             }
             const unsigned ns = unsigned(p.solve_small_ptr[l + 1] - p.solve_small_ptr[l]);
             if (ns) {
@@ -683,6 +697,8 @@ static int solve_impl(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which)
     SPRGPU_HIP(hipGetLastError());
     return 0;
 }
+
+int prepare_solve(sprgpu_handle *h, i64 nrhs) { return ensure_solve_buffers(h, nrhs); }
 
 int run_solve(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which)
 {

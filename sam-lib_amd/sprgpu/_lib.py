@@ -55,6 +55,9 @@ SIGNATURES = {
     "sprgpu_profile_factor": (C.c_int, [P, P, P, C.c_int, P]),
     "sprgpu_element_grad": (C.c_int, [P, I64, I64, P, P, P, P, P, P]),
     "sprgpu_pattern_grad": (C.c_int, [P, P, P, P]),
+    "sprgpu_factor_solve_graph": (C.c_int, [P, P, P, I64, I64]),
+    "sprgpu_autotune": (C.c_int, [P, P, P, I64, I64, P]),
+    "sprgpu_plan_params": (C.c_int, [P, P]),
     "sprgpu_last_error": (C.c_char_p, []),
 }
 

@@ -261,3 +261,94 @@ def element_equations(model, sysm: SprSystem) -> np.ndarray:
         eq[e] = sysm.meqn[model.element_dofs(e)] - 1
     eq[eq < 0] = -1
     return eq
+
+
+@dataclass
+class MergedSystem:
+    """Several independent SPR systems as one forest (block-diagonal A).
+
+    The arrays follow the SPR layout the drop-in consumes (sprfc1.f:161-203):
+    supernodes, equations, LINDX entries and LNZ positions of system k are
+    shifted by the totals of systems 0..k-1, and each system's roots stay
+    roots.  The GPU plan batches all supernodes of one tree height, so every
+    launch covers all systems.
+    """
+
+    mspar: np.ndarray
+    mtrees: np.ndarray
+    msifa: np.ndarray
+    neq_off: np.ndarray     # (k+1,) equation offsets
+    nzl_off: np.ndarray     # (k+1,) LNZ offsets
+    neq: int
+    nzerol: int
+
+    def merge_sm(self, sms) -> np.ndarray:
+        """SM of the forest: all diagonals, then all LNZ parts, in system order."""
+        k = len(sms)
+        out = np.empty(self.neq + self.nzerol)
+        for i in range(k):
+            n = self.neq_off[i + 1] - self.neq_off[i]
+            out[self.neq_off[i]:self.neq_off[i + 1]] = sms[i][:n]
+            out[self.neq + self.nzl_off[i]:self.neq + self.nzl_off[i + 1]] = sms[i][n:]
+        return out
+
+    def split(self, v: np.ndarray):
+        """Per-system slices of a vector in forest equation order."""
+        return [v[self.neq_off[i]:self.neq_off[i + 1]] for i in range(len(self.neq_off) - 1)]
+
+
+def merge_systems(systems) -> MergedSystem:
+    """Merges SprSystems into one forest-structured system (see MergedSystem)."""
+    ns_tot = sum(s.nsuper for s in systems)
+    neq_off = np.zeros(len(systems) + 1, dtype=I8)
+    nzl_off = np.zeros(len(systems) + 1, dtype=I8)
+    nof_tot = 0
+    maxsup = mfront = 0
+    xsuper, xsibl, sibl, xlindx, supsup, lindx, xlnz = [], [], [], [], [], [], []
+    ns0 = nsib0 = nof0 = 0
+    for i, s in enumerate(systems):
+        neq_off[i + 1] = neq_off[i] + s.neq
+        nzl_off[i + 1] = nzl_off[i] + s.nzerol
+        ns = s.nsuper
+        nofsub = int(s.mspar[14])
+        maxsup = max(maxsup, int(s.mspar[12]))
+        mfront = max(mfront, int(s.mspar[13]))
+        xs = s.xsuper()
+        xsuper.append(xs[:-1] + neq_off[i])
+        xsb = s.xsibl()
+        nch = int(xsb[ns] - xsb[0])
+        xsibl.append(xsb[:-1] - xsb[0] + 1 + nsib0)
+        sibl.append(s.sibl()[xsb[0] - 1:xsb[0] - 1 + nch] + ns0)
+        xl = s.xlindx()
+        xlindx.append(xl[:-1] - xl[0] + 1 + nof0)
+        p = int(s.mspar[45]) - 1 + ns + 1
+        supsup.append(s.mtrees[p:p + ns])
+        lindx.append(s.lindx() + neq_off[i])
+        xlnz.append(s.xlnz()[:-1] + nzl_off[i])
+        ns0 += ns
+        nsib0 += nch
+        nof0 += nofsub
+        nof_tot += nofsub
+    neq, nzerol = int(neq_off[-1]), int(nzl_off[-1])
+    xsuper = np.concatenate(xsuper + [np.array([neq + 1], dtype=I8)])
+    xsibl = np.concatenate(xsibl + [np.array([nsib0 + 1], dtype=I8)])
+    sibl = np.concatenate(sibl + [np.zeros(ns_tot + 1 - nsib0, dtype=I8)])
+    xlindx = np.concatenate(xlindx + [np.array([nof_tot + 1], dtype=I8)])
+    supsup = np.concatenate(supsup)
+    lindx = np.concatenate(lindx)
+    xlnz = np.concatenate(xlnz + [np.array([nzerol + 1], dtype=I8)])
+    mtrees = np.concatenate([xsuper, xsibl, sibl, xlindx, supsup])
+    msifa = np.concatenate([lindx, xlnz])
+    mspar = np.zeros(NSPAR, dtype=I8)
+    mspar[0] = 4                      # symbolic phase complete
+    mspar[7] = neq
+    mspar[10] = ns_tot
+    mspar[12] = maxsup
+    mspar[13] = mfront
+    mspar[14] = nof_tot
+    mspar[15] = nzerol
+    mspar[44] = 1                              # XSUPER
+    mspar[45] = 3 * (ns_tot + 1) + 1           # XLINDX
+    mspar[46] = 1                              # LINDX
+    mspar[47] = nof_tot + 1                    # XLNZ
+    return MergedSystem(mspar, mtrees, msifa, neq_off, nzl_off, neq, nzerol)

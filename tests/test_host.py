@@ -43,3 +43,31 @@ def test_element_equations_mark_fixed_dofs():
     # element 0 touches the clamped face z=0 with its first four nodes
     assert np.all(eq[0, :12] == -1) and np.all(eq[0, 12:] >= 0)
     assert eq.max() == sysm.neq - 1
+
+
+def test_merged_forest_solves_each_system():
+    """The forest of several systems (host.merge_systems) gives each system's
+    own solution bit for bit with the oracle, and plans without error."""
+    import ctypes as C
+
+    import oracle
+    from sprgpu import _lib, fem, host
+
+    rng = np.random.default_rng(3)
+    models = [fem.beam_frame(int(n), rng) for n in (30, 45, 25)] + [fem.hex8_grid(2)]
+    systems = [host.symbolic(m) for m in models]
+    sms = [host.assemble(m, s) for m, s in zip(models, systems)]
+    merged = host.merge_systems(systems)
+    out = np.zeros(10)
+    assert _lib.lib().sprgpu_plan_summary(*(a.ctypes.data_as(C.c_void_p) for a in
+                                            (merged.mspar, merged.mtrees, merged.msifa, out))) == 0
+    assert out[0] == merged.neq == sum(s.neq for s in systems)
+    b = np.random.default_rng(0).normal(size=(merged.neq, 1))
+    xm = np.asfortranarray(b.copy())
+    ierr, _ = oracle.sprsol(3, merged.mspar.copy(), merged.mtrees, merged.msifa,
+                            merged.merge_sm(sms), xm, tol=[0, 0, 0])
+    assert ierr == 0
+    for k, (s, sm, x) in enumerate(zip(systems, sms, merged.split(xm[:, 0]))):
+        bi = np.asfortranarray(b[merged.neq_off[k]:merged.neq_off[k + 1]].copy())
+        e, _ = oracle.sprsol(3, s.mspar.copy(), s.mtrees, s.msifa, sm.copy(), bi, tol=[0, 0, 0])
+        assert e == 0 and np.array_equal(bi[:, 0], x)
