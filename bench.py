@@ -29,6 +29,7 @@ sys.path.insert(0, os.path.join(ROOT, "sam-lib_amd"))
 
 METRIC = "factor+solve s & TFLOP/s, 40^3 hex8: FP64 vs FP32+IR at 1e-10 residual"
 PEAK = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense MFMA TFLOP/s
+GEMM_F32_TRAFFIC_40 = 433.9e6  # measured HBM bytes per k_gemm<float,128> launch, 40^3
 HBM_PEAK = 8000.0  # GB/s
 
 
@@ -85,10 +86,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-profile", action="store_true", help="skip the per-kernel timing pass")
     ap.add_argument("--config", default="factor_solve",
-                    choices=["factor_solve", "adjoint", "frames"],
+                    choices=["factor_solve", "adjoint", "frames", "rhs128", "quad4", "hbm_copy"],
                     help="factor_solve: the headline (40^3, FP64 vs FP32+IR); adjoint: "
                          "dL/dE of the compliance at 40^3 (E seed 4); frames: 512 beam "
-                         "frames, FP64 factor+solve in one graph")
+                         "frames, FP64 factor+solve in one graph; rhs128: 40^3 with 128 "
+                         "N(0,1) RHS; quad4: 1000^2 plane stress with 128 RHS; hbm_copy: a "
+                         "1 GiB device copy (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--fd", type=int, default=64, help="adjoint: elements checked by finite differences")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -99,6 +102,20 @@ def main():
         return bench_adjoint(args)
     if args.config == "frames":
         return bench_frames(args)
+    if args.config in ("rhs128", "quad4"):
+        return bench_many_rhs(args)
+    if args.config == "hbm_copy":
+        import torch
+
+        n = 1 << 27  # 1 GiB of float64
+        a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(args.steps):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"config": "hbm_copy", "bytes_read_per_copy": 8 * n,
+                          "bytes_written_per_copy": 8 * n, "copies": args.steps}), flush=True)
+        return
 
     import torch
     from sprgpu import host
@@ -233,10 +250,15 @@ def main():
 
         result["roofline"] = mfma_roofline(
             "fp32", "mfma_update", "k_gemm<float,128> (rank-256 and Schur updates, one factorization)")
+        # HBM bytes per launch of k_gemm<float,128> on this workload, from
+        # rocprofv3 FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB copy
+        # (profiles/k_gemm_f32_128_hex8_40_hbm_traffic.txt); None at other sizes
+        if args.n == 40:
+            result["roofline"]["traffic"] = GEMM_F32_TRAFFIC_40
         result["roofline_inner_fp32"] = mfma_roofline(
             "fp32", "mfma_inner", "k_gemm<float,64> (rank-32 updates, one factorization)")
         result["roofline_fp64"] = mfma_roofline(
-            "fp64", "mfma_update", "k_gemm<double,128> (rank-256 and Schur updates)")
+            "fp64", "mfma_update", "k_gemm<double,64> (rank-256 and Schur updates)")
         result["roofline_inner_fp64"] = mfma_roofline(
             "fp64", "mfma_inner", "k_gemm<double,64> (rank-32 updates)")
         result["kernel_classes_ms"] = {k: {c: round(v[c]["ms"], 3) for c in classes}
@@ -337,6 +359,92 @@ def bench_adjoint(args):
         "vs_fp64_cpu_adjoint": ref or None,
     }
     sysobj.close()
+    print(json.dumps(result), flush=True)
+
+
+def bench_many_rhs(args):
+    """128 right-hand sides, columns i.i.d. N(0,1) from seed 2 (natural dof
+    order), on the 40^3 hex8 matrix (rhs128) or on quad4 plane stress 1000^2
+    (E=1, nu=0.25, thickness 1, left edge clamped; quad4).  FP32 factor +
+    FP64 refinement of every column to rtol in one batched solve launch per
+    kernel, beside the plain FP64 factor + solve."""
+    import torch
+    from sprgpu import fem, host
+    from sprgpu.device import SprFactor
+
+    nrhs = 128
+    t = time.perf_counter()
+    if args.config == "rhs128":
+        model, sysm, sm, _ = build_problem(args.n)
+        label = f"hex8 {args.n}^3, E log-normal(0,0.5) seed 1"
+    else:
+        n = 1000 if args.n == 40 else args.n
+        model = fem.quad4_grid(n)
+        sysm = host.symbolic(model)
+        sm = host.assemble(model, sysm)
+        label = f"quad4 plane stress {n}^2, E=1, nu=0.25, t=1, x=0 clamped"
+    rng = np.random.default_rng(2)
+    B = host.loads_to_equations(sysm, fem.normal_loads(model.ndof, rng, nrhs))
+    flops = host.factor_flops(sysm)
+    log(f"{label}: neq={sysm.neq} nzerol={sysm.nzerol} flops={flops:.3e} "
+        f"({time.perf_counter() - t:.1f}s)")
+    dev = torch.device("cuda:0")
+    sm_dev = torch.from_numpy(sm).to(dev)
+    b_dev = torch.from_numpy(np.ascontiguousarray(B.T)).to(dev)  # (nrhs, neq) = B(LDB, NRHS)
+    h32 = SprFactor(sysm.mspar, sysm.mtrees, sysm.msifa, "fp32")
+    h32.set_pattern(sm)
+    x = torch.empty_like(b_dev)
+    tol = (0.0, 0.0, 0.0, 0.0)
+
+    def timed(fn, k):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = None
+        for _ in range(k):
+            out = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / k, out
+
+    def step32():
+        h32.factor(sm_dev, tol, info=False)
+        return h32.solve_ir(b_dev, x, rtol=args.rtol, maxit=args.maxit)
+
+    step32()
+    t32, ir = timed(step32, args.steps)
+    tf32, _ = timed(lambda: h32.factor(sm_dev, tol, info=False), args.steps)
+    r = torch.empty_like(b_dev)
+    h32.spmv(x, r)
+    relres = ((b_dev - r).norm(dim=1) / b_dev.norm(dim=1)).max().item()
+    log(f"fp32+IR: {t32*1e3:.1f} ms per step (factor {tf32*1e3:.1f}), {ir}, max relres {relres:.2e}")
+    h32.close()
+    h64 = SprFactor(sysm.mspar, sysm.mtrees, sysm.msifa, "fp64")
+    x64 = torch.empty_like(b_dev)
+
+    def step64():
+        h64.factor(sm_dev, tol, info=False)
+        x64.copy_(b_dev)
+        h64.solve(x64)
+
+    step64()
+    t64, _ = timed(step64, args.steps)
+    tf64, _ = timed(lambda: h64.factor(sm_dev, tol, info=False), args.steps)
+    h64.set_pattern(sm)
+    h64.spmv(x64, r)
+    relres64 = ((b_dev - r).norm(dim=1) / b_dev.norm(dim=1)).max().item()
+    h64.close()
+    log(f"fp64: {t64*1e3:.1f} ms per step (factor {tf64*1e3:.1f}), max relres {relres64:.2e}")
+    solve32 = t32 - tf32
+    result = {
+        "metric": f"factor + {nrhs}-RHS solve, {label}: FP32+IR vs FP64",
+        "value": nrhs / solve32, "unit": "RHS/s (FP32+IR solve phase)", "n_gpus": 1,
+        "steps": args.steps, "higher_is_better": True, "dtype": "f32 factor, f64 IR",
+        "data": "synthetic",
+        "config": {"workload": f"{label}, {nrhs} RHS N(0,1) seed 2", "neq": sysm.neq,
+                   "nzerol": sysm.nzerol, "factor_flops": flops, "rtol": args.rtol},
+        "fp32_ir": {"step_s": t32, "factor_s": tf32, "solve_ir_s": solve32,
+                    "iterations": int(ir["iterations"]), "max_relres": relres},
+        "fp64": {"step_s": t64, "factor_s": tf64, "solve_s": t64 - tf64, "max_relres": relres64},
+    }
     print(json.dumps(result), flush=True)
 
 
