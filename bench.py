@@ -500,7 +500,7 @@ def bench_frames(args):
     _lib.check(L.sprgpu_autotune(h._h, C.c_void_p(sm_dev.data_ptr()), C.c_void_p(x_dev.data_ptr()),
                                  merged.neq, 1, report.ctypes.data_as(C.c_void_p)), "autotune")
     torch.cuda.synchronize()
-    params = (C.c_int * 4)()
+    params = (C.c_int * 5)()
     L.sprgpu_plan_params(h._h, params)
     ncand = int(report[0])
     cands = report[1:1 + 5 * ncand].reshape(ncand, 5)

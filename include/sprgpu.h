@@ -170,9 +170,11 @@ sprgpu_int sprgpu_device_bytes(const sprgpu_handle *handle);
 /* sprgpu_factor with a HIP event after every kernel (slower; for
  * measurement).  out[3*c+0..2] (c < 8, so out holds 24 values) =
  * milliseconds, kernel launches, algorithmic flops for kernel class c:
- * 0 front init, 1 extend-add, 2 LDS small fronts, 3 diagonal-block
- * write-back, 4 panel steps, 5 MFMA rank-256 and Schur updates, 6 other,
- * 7 MFMA rank-32 updates inside a 256-column block. */
+ * 0 front init, 1 extend-add, 2 LDS small fronts, 3 diagonal blocks
+ * (256-column outer blocks; with panel steps, the write-back), 4 rows below
+ * them (outer-block TRSM; or the panel steps), 5 MFMA rank-256 and Schur
+ * updates, 6 other, 7 MFMA rank-32 updates inside a 256-column block (panel
+ * steps only). */
 int sprgpu_profile_factor(sprgpu_handle *handle, const double *sm_dev, const double *tol,
                           int mode, double *out);
 
@@ -180,7 +182,7 @@ int sprgpu_profile_factor(sprgpu_handle *handle, const double *sm_dev, const dou
  * touching a GPU and reports it, for checking and sizing.  out[0..9] =
  * NEQ, NSUPER, tree heights, kernel launches per factorization, front-pool
  * elements, solve-workspace elements per RHS, fronts factored in LDS,
- * panel steps, MFMA update tiles, factor flops (RINFO(3) formula,
+ * outer blocks (or panel steps), MFMA update tiles, factor flops (RINFO(3) formula,
  * sprtrs.f:1961-1969).  Returns 0 or -2 (structure not supported). */
 int sprgpu_plan_summary(const sprgpu_int *mspar, const sprgpu_int *mtrees,
                         const sprgpu_int *msifa, double *out);
@@ -203,9 +205,14 @@ int sprgpu_factor_solve_graph(sprgpu_handle *handle, const double *sm_dev, doubl
  * tile, milliseconds.  No counterpart in the reference. */
 int sprgpu_autotune(sprgpu_handle *handle, const double *sm_dev, double *b_dev, sprgpu_int ldb,
                     sprgpu_int nrhs, double *report);
-/* The parameters in use: out[0..3] = small_f, fronts per workgroup, narrow
- * threads (0 = by size), update tile. */
+/* The parameters in use: out[0..4] = small_f, fronts per workgroup, narrow
+ * threads (0 = by size), update tile, blocked-front scheme (1 = 256-column
+ * outer blocks, 0 = 32-column panel steps). */
 int sprgpu_plan_params(const sprgpu_handle *handle, int *out);
+/* Re-plans with the given parameters, in the order of sprgpu_plan_params
+ * (update tile 0 = by precision).  Returns 0 or -2 (a value out of range).
+ * No counterpart in the reference; for tuning and comparisons. */
+int sprgpu_set_plan_params(sprgpu_handle *handle, const int *params);
 
 /* Last error message (thread-local, empty if none). */
 const char *sprgpu_last_error(void);

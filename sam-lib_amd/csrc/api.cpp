@@ -310,7 +310,20 @@ int sprgpu_plan_params(const sprgpu_handle *h, int *out)
     out[1] = h->plan.params.small_waves;
     out[2] = h->plan.params.narrow_threads;
     out[3] = h->plan.big_tile;
+    out[4] = h->plan.params.block_factor;
     return 0;
+}
+
+int sprgpu_set_plan_params(sprgpu_handle *h, const int *params)
+{
+    if (!h || !params) return -2;
+    PlanParams pp;
+    pp.small_f = params[0];
+    pp.small_waves = params[1];
+    pp.narrow_threads = params[2];
+    pp.big_tile = params[3];
+    pp.block_factor = params[4];
+    return replan(h, pp);
 }
 
 int sprgpu_set_stream(sprgpu_handle *h, void *stream)
@@ -437,7 +450,7 @@ int sprgpu_plan_summary(const sprgpu_int *mspar, const sprgpu_int *mtrees,
     i64 tiles = 0, steps = 0;
     for (const Launch &L : p.launches) {
         if (L.kind == L_GEMM) tiles += L.blocks;
-        if (L.kind == L_PANEL) steps += L.count;
+        if (L.kind == L_PANEL || L.kind == L_BDIAG) steps += L.count;
     }
     out[0] = double(p.neq);
     out[1] = double(p.nsuper);

@@ -297,19 +297,173 @@ __global__ __launch_bounds__(256) void k_small(const i32 *__restrict__ list, int
         for (int c = 0; c <= r; ++c) fr[r + i64(c) * n.ld] = M[r + c * LDM];
 }
 
+// Bit-mask selects: x where the mask is all ones, zero where it is zero.  The
+// masks come from integer arithmetic on the lane index, so no comparison
+// results (scalar register pairs) are created; the compiler would otherwise
+// evaluate one per loop step up front and spill them.
+__device__ __forceinline__ int mask_ge(int a, int b) { return (b - 1 - a) >> 31; }  // a >= b
+__device__ __forceinline__ float mask_sel(float x, int m)
+{
+    return __builtin_bit_cast(float, __builtin_bit_cast(int, x) & m);
+}
+__device__ __forceinline__ double mask_sel(double x, int m)
+{
+    return __builtin_bit_cast(double, __builtin_bit_cast(long long, x) & (long long)m);
+}
+template <typename T> __device__ __forceinline__ T mask_blend(T a, T b, int m)  // m ? a : b
+{
+    return T(mask_sel(a, m) + mask_sel(b, ~m));
+}
+
+// Reciprocal from the hardware estimate refined by Newton steps (one for
+// FP32, two for FP64); zero and infinite inputs give the estimate itself
+// (inf and 0, as 1/d would).
+__device__ __forceinline__ float recip(float d)
+{
+    const float r = __builtin_amdgcn_rcpf(d);
+    const float e = __builtin_fmaf(-d, r, 1.0f);
+    const float x = __builtin_fmaf(r, e, r);
+    return __builtin_isfinite(x) ? x : r;
+}
+__device__ __forceinline__ double recip(double d)
+{
+    const double r = __builtin_amdgcn_rcp(d);
+    double x = __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
+    x = __builtin_fma(x, __builtin_fma(-d, x, 1.0), x);
+    return __builtin_isfinite(x) ? x : r;
+}
+
+// LDL' of an nb x nb diagonal block (nb <= kPanelNB; entry (r, c) of its
+// lower triangle at src[r*srs + c*scs]) by one wavefront, right-looking over
+// the columns (SPRF17's recurrence l_rj = (a_rj - sum_k l_rk d_k l_jk) / d_j,
+// one column at a time).  Lane r + 32h holds columns 16h..16h+15 of row r in
+// registers; entries right of a row's diagonal are scratch.  At step j the
+// half that holds column j writes it to LDS (rows above j as zeros) and lane
+// j writes its row of X (below), and every lane reads the pivot, its own
+// entry, its 16 columns and X's row j back as broadcasts, then updates its
+// entries unmasked (the zeros leave finished columns unchanged).  Alongside,
+// each lane keeps its row of X = L11^-1, built as the product of the
+// elimination steps (I - l_j e_j^T) in the same loop: X(r, :) -= l_rj X(j, :).
+// The pivots are recorded (when record is set) by each lane for its own row
+// after the loop.  Bm = L11^-T D^-1 = X^T D^-1 (upper triangular, zero
+// outside nb x nb) is left in LDS.  The factored rows (l_rc for c < r, d_r at
+// c = r) go to out[r*rs + c*cs] when out is set, with zeros right of the
+// diagonal if zero_upper; the rows of Bm go to bm_out + r*kPanelNB when
+// bm_out is set (rows r < nb).  lsh holds kLshSize elements of LDS scratch
+// (16-byte aligned).  Returns lane r's pivot d_r (zero for r >= nb; lanes
+// 32..63 return the value of lane - 32).
+constexpr int kLshSize = 7 * kPanelNB;  // LDS scratch elements
+
+template <typename T>
+__device__ __forceinline__ T diag_block_factor(const T *src, i64 srs, i64 scs, int nb, int eq0,
+                                               const double *__restrict__ adiag, bool record,
+                                               const PivotArgs &pa, double *dvec, double *dcor,
+                                               FactorStatus *st, T (*Bm)[kPanelNB + 1], T *lsh,
+                                               T *out, i64 rs, i64 cs, bool zero_upper, T *bm_out)
+{
+    constexpr int NB = kPanelNB, NH = NB / 2;
+    static_assert(NB == 32, "two 16-column halves per row");
+    typedef T vec_t __attribute__((ext_vector_type(16 / sizeof(T))));
+    constexpr int VE = 16 / sizeof(T);
+    const int lane = opaque(int(threadIdx.x & 63));
+    const int r = lane & 31, h = lane >> 5, c0 = NH * h;
+    T *pd = lsh;                 // [NB] pivots d_j
+    T *pi = lsh + NB;            // [NB] 1/d_j
+    T *pin = lsh + 2 * NB;       // [NB] pivots before the replacement rule
+    T rq[NH], xq[NH];
+#pragma unroll
+    for (int q = 0; q < NH; ++q) {
+        rq[q] = (r < nb && c0 + q <= r) ? src[r * srs + (c0 + q) * scs] : T(0);
+        xq[q] = mask_sel(T(1), mask_ge(r, c0 + q) & mask_ge(c0 + q, r));
+    }
+    const double avr = (r < nb) ? adiag[eq0 + r] : 0.0;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        if (j < nb) {
+            // column j and row j of X, double-buffered: the step after next
+            // overwrites them, after every lane's reads of this step are done
+            T *wcol = lsh + (3 + 2 * (j & 1)) * NB, *xrow = wcol + NB;
+            if (h == (j >> 4)) wcol[r] = mask_sel(rq[j & 15], mask_ge(r, j));
+            if (r == j) {
+#pragma unroll
+                for (int v = 0; v < NH / VE; ++v) {
+                    vec_t x;
+#pragma unroll
+                    for (int e = 0; e < VE; ++e) x[e] = xq[VE * v + e];
+                    *reinterpret_cast<vec_t *>(xrow + c0 + VE * v) = x;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const T dj = wcol[j];
+            const T wr = wcol[r];
+            T wc[NH], xj[NH];
+#pragma unroll
+            for (int v = 0; v < NH / VE; ++v) {
+                const vec_t x = *reinterpret_cast<const vec_t *>(wcol + c0 + VE * v);
+                const vec_t y = *reinterpret_cast<const vec_t *>(xrow + c0 + VE * v);
+#pragma unroll
+                for (int e = 0; e < VE; ++e) {
+                    wc[VE * v + e] = x[e];
+                    xj[VE * v + e] = y[e];
+                }
+            }
+            // all reads of the step in flight before the first use
+            __builtin_amdgcn_sched_barrier(0);
+            const double dp = pivot_value(double(dj), lane_bcast(avr, j), pa);
+            const T d = T(dp);
+            const T di = recip(d);
+            const T l = mask_sel(wr * di, mask_ge(r, j + 1));  // rows r > j only
+#pragma unroll
+            for (int q = 0; q < NH; ++q) {
+                rq[q] -= l * wc[q];
+                xq[q] -= l * xj[q];
+            }
+            if (h == (j >> 4)) {
+                const int eq = mask_ge(r, j) & mask_ge(j, r);  // r == j
+                rq[j & 15] = mask_blend(d, l, eq);
+            }
+            if (lane == 0) {
+                pd[j] = d;
+                pi[j] = di;
+                pin[j] = dj;
+            }
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const T dr = (r < nb) ? pd[r] : T(0);
+    const T dir = (r < nb) ? pi[r] : T(0);
+    if (record && h == 0 && r < nb)
+        pivot_record(double(pin[r]), double(dr), avr, eq0 + r, pa, dvec, dcor, st);
+    if (out && r < nb) {
+#pragma unroll
+        for (int q = 0; q < NH; ++q) {
+            const int c = c0 + q;
+            if (c <= r || zero_upper) out[r * rs + c * cs] = mask_sel(rq[q], mask_ge(r, c));
+        }
+    }
+    // Bm[i][c] = X[c][i] / d_c: lane c writes column c (zero for c >= nb)
+#pragma unroll
+    for (int q = 0; q < NH; ++q) {
+        const T v = xq[q] * dir;
+        Bm[c0 + q][r] = v;
+        if (bm_out && c0 + q < nb && r < nb) bm_out[i64(c0 + q) * NB + r] = v;
+    }
+    return dr;
+}
+
 // ---------------------------------------------------------------------------
 // panel step: factor the nb x nb diagonal block at column k0 and solve the
 // rows below it (L21 = A21 L11^-T D^-1)
 // ---------------------------------------------------------------------------
 // Panel step: columns [k0, k0+nb) of a blocked front, nb <= kPanelNB.
-// Every workgroup factors the nb x nb diagonal block itself: one wavefront,
-// lane r holding row r in registers, right-looking over the columns with the
-// column entries broadcast by readlane (SPRF17's recurrence
-// l_rj = (a_rj - sum_k l_rk d_k l_jk) / d_j, applied one column at a time).
-// The same wavefront inverts the unit lower factor L11 (lane c builds column
-// c of L11^-1), and the workgroup computes its kPanelRows rows below the
-// block as the product L21 = A21 (L11^-T D^-1) on MFMA, one 64-row slice per
-// wavefront.  The A21 loads are issued before the diagonal factor so their
+// Every workgroup factors the nb x nb diagonal block itself
+// (diag_block_factor, one wavefront), and the workgroup computes its
+// kPanelRows rows below the block as the product L21 = A21 (L11^-T D^-1) on
+// MFMA, one 64-row slice per wavefront.  The A21 loads are issued before the diagonal factor so their
 // latency overlaps it.  Workgroup 0 records the pivots and stores the factored
 // block in a scratch array indexed by equation; the diagonal blocks are
 // written into the fronts after the whole factorization (k_diag_writeback),
@@ -326,6 +480,7 @@ __global__ __launch_bounds__(256) void k_panel(const PanelDesc *__restrict__ des
     static_assert(NB == 32 && kPanelRows == 256,
                   "one lane per diagonal-block row; four 64-row MFMA slices per workgroup");
     __shared__ T Bm[NB][NB + 1];  // L11^-T D^-1 (upper triangular), zero outside nb x nb
+    __shared__ __attribute__((aligned(16))) T Lsh[kLshSize];
     const int q = find_desc(prefix, ndesc, blockIdx.x);
     const PanelDesc pd = descs[q];
     const int rb = int(blockIdx.x - prefix[q]);
@@ -346,59 +501,11 @@ __global__ __launch_bounds__(256) void k_panel(const PanelDesc *__restrict__ des
             const int row = rw + ti * 16 + (lane & 15), col = ks * 4 + (lane >> 4);
             a[ti][ks] = (row < n.F && col < nb) ? fr[row + (k0 + col) * ld] : T(0);
         }
-    if (wave == 0) {
-        const int r = lane;  // lanes >= nb hold zero rows
-        T row[NB];
-#pragma unroll
-        for (int c = 0; c < NB; ++c)
-            row[c] = (r < nb && c <= r) ? fr[(k0 + r) + (k0 + c) * ld] : T(0);
-        const double avr = (r < nb) ? adiag[n.first + k0 + r] : 0.0;
-        T dinvr = T(0);  // lane j: 1/d_j
-        // After step j, lane r > j holds l_rj in row[j] and lane j holds d_j;
-        // entries right of a lane's diagonal are scratch and never read.
-#pragma unroll
-        for (int j = 0; j < NB; ++j) {
-            if (j < nb) {
-                const T dj = lane_bcast(row[j], j);
-                const double av = lane_bcast(avr, j);
-                const int eq = n.first + k0 + j;
-                const double dp = pivot_value(double(dj), av, pa);
-                if (rb == 0 && r == 0) pivot_record(double(dj), dp, av, eq, pa, dvec, dcor, st);
-                const T d = T(dp);
-                const T di = T(1) / d;
-                const T w = row[j];  // lane r > j: a_rj - sum_{k<j} l_rk d_k l_jk = l_rj d_j
-                const T l = w * di;
-#pragma unroll
-                for (int k = j + 1; k < NB; ++k) {
-                    const T wk = lane_bcast(w, k);
-                    if (r > j) row[k] -= l * wk;
-                }
-                row[j] = (r > j) ? l : ((r == j) ? d : row[j]);
-                if (r == j) dinvr = di;
-            }
-        }
-        if (rb == 0 && r < nb) {
-            T *out = dscr + i64(n.first + k0 + r) * NB;
-#pragma unroll
-            for (int c = 0; c < NB; ++c) out[c] = (c <= r) ? row[c] : T(0);
-        }
-        // lane c: column c of M = L11^-1, m_i = delta_ic - sum_{k<i} l_ik m_k
-        T m[NB];
-#pragma unroll
-        for (int i = 0; i < NB; ++i) m[i] = (i == r) ? T(1) : T(0);
-#pragma unroll
-        for (int i = 1; i < NB; ++i) {
-            if (i < nb) {
-#pragma unroll
-                for (int k = 0; k < i; ++k) m[i] -= lane_bcast(row[k], i) * m[k];
-            }
-        }
-        // Bm[i][c] = M[c][i] / d_c: lane i writes row i
-        if (r < NB) {
-#pragma unroll
-            for (int c = 0; c < NB; ++c) Bm[r][c] = m[c] * lane_bcast(dinvr, c);
-        }
-    }
+    if (wave == 0)
+        diag_block_factor<T>(fr + k0 + k0 * ld, 1, ld, nb, n.first + k0, adiag, rb == 0, pa, dvec,
+                             dcor, st, Bm,
+                             Lsh, rb == 0 ? dscr + i64(n.first + k0) * NB : nullptr, NB, 1, true,
+                             nullptr);
     __syncthreads();
     if (!active) return;
     typename Mfma<T>::acc_t acc[4][2];
@@ -447,6 +554,252 @@ __global__ __launch_bounds__(256) void k_diag_writeback(const PanelDesc *__restr
 }
 
 // ---------------------------------------------------------------------------
+// outer-block factor: a kPanelOuter-wide column block of a blocked front
+// ---------------------------------------------------------------------------
+// The block [ob, ob+nbw) (descriptor k0 = ob, nb = nbw) is factored by two
+// launches instead of nbw/kPanelNB panel steps with rank-kPanelNB updates:
+//   k_block_diag   one workgroup per front factors the nbw x nbw diagonal
+//                  block: kPanelNB-column sub-steps of diag_block_factor,
+//                  the sub-panel rows below inside the block (MFMA with Bm)
+//                  and the rank-kPanelNB update of the rest of the block.
+//                  Each sub-step's Bm goes to a scratch array indexed by
+//                  equation.
+//   k_block_trsm   the rows below the block, 16 per wavefront, left-looking
+//                  over the sub-steps: L(r, jb) = (A(r, jb) - sum_{kb<jb}
+//                  L(r, kb) D_kb L(jb, kb)^T) Bm_jb.
+// The operations are those of the panel steps (same recurrences, same Bm
+// products); only the order of the rank-kPanelNB updates of a row changes.
+constexpr int kBdWaves = 4;                                      // k_block_diag
+constexpr int kBdTiles = (kPanelOuter / 16) * (kPanelOuter / 16 + 1) / 2;
+constexpr int kBdSlots = (kBdTiles + kBdWaves - 1) / kBdWaves;  // 16x16 tiles per wavefront
+
+// (ti, tj) of tile t, column-major over the lower triangle of nt x nt tiles
+__device__ __forceinline__ void tri_tile(int t, int nt, int &ti, int &tj)
+{
+    tj = 0;
+    while (t >= nt - tj) {
+        t -= nt - tj;
+        ++tj;
+    }
+    ti = tj + t;
+}
+
+// The block's lower triangle lives in registers as 16x16 MFMA accumulator
+// tiles, tile t owned by wavefront t % kBdWaves (column-major order, so the
+// tiles of every sub-step's trailing update are spread evenly).  A sub-step
+// moves its two tile columns into LDS, wavefront 0 factors the diagonal
+// 32 x 32 part, all wavefronts multiply the rows below by Bm (writing L to
+// the front) and then update their own tiles from the L rows in LDS.
+template <typename T>
+__global__ __launch_bounds__(64 * kBdWaves) void k_block_diag(const PanelDesc *__restrict__ descs,
+                                                             const NodeInfo *__restrict__ nodes,
+                                                             T *__restrict__ pool, double *dvec,
+                                                             const double *__restrict__ adiag,
+                                                             double *dcor, FactorStatus *st,
+                                                             PivotArgs pa, T *bmscr)
+{
+    constexpr int NB = kPanelNB, W = kPanelOuter, LS = NB + 1;
+    typedef typename Mfma<T>::acc_t acc_t;
+    __shared__ T Bm[NB][NB + 1];
+    __shared__ __attribute__((aligned(16))) T Lsh[kLshSize];
+    __shared__ T Ls[W * LS];  // the sub-step's two tile columns, rows relative to k0
+    __shared__ T dsh[NB];
+    const PanelDesc pd = descs[blockIdx.x];
+    const NodeInfo n = nodes[pd.node];
+    T *fr = pool + n.foff + pd.k0 + pd.k0 * n.ld;  // block origin
+    const i64 ld = n.ld;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nbw = pd.nb, nt = (nbw + 15) / 16, ntiles = nt * (nt + 1) / 2;
+    const int lr = lane & 15;
+    int sti[kBdSlots], stj[kBdSlots];
+    acc_t acc[kBdSlots];
+#pragma unroll
+    for (int q = 0; q < kBdSlots; ++q) {
+        const int t = q * kBdWaves + wave;
+        sti[q] = -1;
+        stj[q] = -1;
+        acc[q] = acc_t{0, 0, 0, 0};
+        if (t < ntiles) {
+            tri_tile(t, nt, sti[q], stj[q]);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int row = sti[q] * 16 + Mfma<T>::row(lane, e), col = stj[q] * 16 + lr;
+                if (row < nbw && col < nbw && row >= col) acc[q][e] = fr[row + col * ld];
+            }
+        }
+    }
+    for (int j = 0; j * NB < nbw; ++j) {
+        const int k0 = j * NB;  // relative to the block
+        const int nb = min(NB, nbw - k0);
+        const int m = nbw - k0;  // rows of the sub-panel, diagonal block included
+        // lane-derived addresses are recomputed in every sub-step rather than
+        // kept in registers across the loop
+        const int lane = opaque(tid & 63), lr = lane & 15;
+        // 1. the sub-panel's tiles (tile columns 2j, 2j+1) to LDS
+#pragma unroll
+        for (int q = 0; q < kBdSlots; ++q) {
+            if (stj[q] == 2 * j || stj[q] == 2 * j + 1) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int row = sti[q] * 16 + Mfma<T>::row(lane, e) - k0;
+                    const int col = (stj[q] - 2 * j) * 16 + lr;
+                    Ls[row * LS + col] = acc[q][e];
+                }
+            }
+        }
+        __syncthreads();
+        // 2. the diagonal 32 x 32 block
+        if (wave == 0) {
+            const T dr = diag_block_factor<T>(Ls, LS, 1, nb, n.first + pd.k0 + k0, adiag, true, pa,
+                                              dvec, dcor, st, Bm, Lsh, fr + k0 + k0 * ld, 1, ld,
+                                              false, bmscr + i64(n.first + pd.k0 + k0) * NB);
+            if (lane < NB) dsh[lane] = dr;
+        }
+        __syncthreads();
+        // 3. rows below it: L = A Bm, 16-row slices dealt to the wavefronts
+        const int nsl = (m + 15) / 16;
+        for (int u = 2 + wave; u < nsl; u += kBdWaves) {
+            acc_t o[2] = {acc_t{0, 0, 0, 0}, acc_t{0, 0, 0, 0}};
+#pragma unroll
+            for (int ks = 0; ks < NB / 4; ++ks) {
+                const int k = ks * 4 + (lane >> 4);
+                const T a = Ls[(u * 16 + lr) * LS + k];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) o[h] = Mfma<T>::op(a, Bm[k][h * 16 + lr], o[h]);
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int row = u * 16 + Mfma<T>::row(lane, e), col = h * 16 + lr;
+                    const T v = (row < m && col < nb) ? o[h][e] : T(0);
+                    Ls[row * LS + col] = v;
+                    if (row < m && col < nb) fr[(k0 + row) + (k0 + col) * ld] = v;
+                }
+        }
+        __syncthreads();
+        // 4. rank-nb update of the tiles right of the sub-panel
+#pragma unroll
+        for (int q = 0; q < kBdSlots; ++q) {
+            if (stj[q] >= 2 * j + 2) {
+                const int ra = sti[q] * 16 - k0 + lr, rb = stj[q] * 16 - k0 + lr;
+#pragma unroll
+                for (int ks = 0; ks < NB / 4; ++ks) {
+                    const int k = ks * 4 + (lane >> 4);
+                    const T a = -Ls[ra * LS + k];
+                    const T b = Ls[rb * LS + k] * dsh[k];
+                    acc[q] = Mfma<T>::op(a, b, acc[q]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Rows below an outer block: each wavefront owns kTrsmRows = 16 rows and
+// keeps their L entries of the whole block in registers; a workgroup of four
+// shares the staging of each sub-step's operands in LDS.  Each product is
+// computed transposed, D(c, r) = sum_k X(c, k) Y(k, r) with the rows r in
+// the MFMA column index, so a lane's accumulator elements are L(r, c) for
+// the k-slots it supplies as an MFMA B operand: accumulator element e of lane
+// l holds column c = Mfma::row(l, e) of a 16-column half, and the same
+// element feeds the k index Mfma::row(l, e) of a later product.  The A
+// operands (d_k L(jb, kb)^T and Bm, staged in LDS) follow that k assignment.
+// The last sub-step of a block may be narrower than kPanelNB: its Bm rows
+// k >= nb are staged as zeros and addresses are clamped into the block.
+constexpr int kBtWaves = kTrsmWaves;
+
+template <typename T>
+__global__ __launch_bounds__(64 * kBtWaves) void k_block_trsm(const PanelDesc *__restrict__ descs,
+                                                             int ndesc,
+                                                             const i64 *__restrict__ prefix,
+                                                             const NodeInfo *__restrict__ nodes,
+                                                             T *__restrict__ pool,
+                                                             const double *__restrict__ dvec,
+                                                             const T *__restrict__ bmscr)
+{
+    constexpr int NB = kPanelNB, NJB = kPanelOuter / kPanelNB, LR = NB + 16 / int(sizeof(T));
+    static_assert(NB == 32 && kTrsmRows == 16, "two 16-column halves per sub-step");
+    typedef typename Mfma<T>::acc_t acc_t;
+    __shared__ __attribute__((aligned(16))) T Rs[(kPanelOuter - NB) * LR];  // d_k L(k0 + c, k)
+    __shared__ __attribute__((aligned(16))) T Bs[NB * LR];                   // Bm of the sub-step
+    const int q = find_desc(prefix, ndesc, blockIdx.x);
+    const PanelDesc pd = descs[q];
+    const int sb = int(blockIdx.x - prefix[q]) * kBtWaves + (threadIdx.x >> 6);
+    const NodeInfo n = nodes[pd.node];
+    T *fr = pool + n.foff;
+    const i64 ld = n.ld;
+    const int tid = threadIdx.x, lane = tid & 63, c = lane & 15;
+    const int bend = pd.k0 + pd.nb;
+    const int njb = (pd.nb + NB - 1) / NB;
+    const int row = bend + sb * kTrsmRows + c;  // row r = lane & 15 of the stripe
+    const bool rowok = row < n.F;
+    const T *frow = fr + (rowok ? row : n.F - 1);
+    int kk[4];  // k slot (within a 16-wide half) this lane supplies at step e
+#pragma unroll
+    for (int e = 0; e < 4; ++e) kk[e] = Mfma<T>::row(lane, e);
+    acc_t O[NJB][2];  // O[jb][h][e] = L(row, ob + 32 jb + 16 h + kk[e])
+#pragma unroll
+    for (int jb = 0; jb < NJB; ++jb) {
+        if (jb >= njb) break;
+        const int k0 = pd.k0 + jb * NB;
+        const int nb = min(NB, bend - k0);
+        // this wavefront's rows of the sub-step (issued before the staging)
+        acc_t acc[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc[h][e] = frow[i64(k0 + min(16 * h + kk[e], nb - 1)) * ld];
+        // stage d_k L(k0 + cc, k) for k in [ob, k0) and Bm (rows k >= nb zero)
+        __syncthreads();  // the previous sub-step's reads of Rs/Bs are done
+        for (int x = tid; x < NB * jb * NB; x += 64 * kBtWaves) {
+            const int cc = x % NB, kx = x / NB;
+            const int kcol = pd.k0 + kx;
+            Rs[kx * LR + cc] = T(dvec[n.first + kcol]) * fr[(k0 + min(cc, nb - 1)) + i64(kcol) * ld];
+        }
+        for (int x = tid; x < NB * NB; x += 64 * kBtWaves) {
+            const int cc = x % NB, k = x / NB;
+            Bs[k * LR + cc] = (k < nb) ? bmscr[i64(n.first + k0 + k) * NB + cc] : T(0);
+        }
+        __syncthreads();
+        // acc(c, r) -= sum_k d_k L(k0 + c, kb cols) L(r, kb cols) over kb < jb
+#pragma unroll
+        for (int kb = 0; kb < NJB - 1; ++kb) {
+            if (kb >= jb) break;
+#pragma unroll
+            for (int hk = 0; hk < 2; ++hk)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const T *rk = Rs + (kb * NB + 16 * hk + kk[e]) * LR;
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+                        acc[h] = Mfma<T>::op(-rk[16 * h + c], O[kb][hk][e], acc[h]);
+                }
+        }
+        // O(c, r) = sum_k Bm(k, c) acc(k, r)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) O[jb][h] = acc_t{0, 0, 0, 0};
+#pragma unroll
+        for (int hk = 0; hk < 2; ++hk)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const T *bk = Bs + (16 * hk + kk[e]) * LR;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) O[jb][h] = Mfma<T>::op(bk[16 * h + c], acc[hk][e], O[jb][h]);
+            }
+        if (rowok) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int col = 16 * h + kk[e];
+                    if (col < nb) fr[row + i64(k0 + col) * ld] = O[jb][h][e];
+                }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // MFMA update: C(i,j) -= sum_k L(i,k) d_k L(j,k) on lower tiles
 // ---------------------------------------------------------------------------
 // One workgroup computes a BM x BM tile of C; its four wavefronts own 2x2
@@ -462,10 +815,44 @@ __global__ __launch_bounds__(256) void k_diag_writeback(const PanelDesc *__restr
 // stage's MFMAs and stored into the other LDS buffer after them, so each
 // stage needs a single barrier.  Consecutive workgroups are dispatched
 // round-robin over the 8 XCDs; the block -> tile map gives each XCD a
-// contiguous range of tiles, so tiles that share L columns share an L2.
+// contiguous range of tiles in gemm_tile's order, so tiles that share L
+// blocks share an L2.
 // Tiles are anchored at r0 rounded down to kGemmAlign and front leading
 // dimensions are multiples of kFrontLdAlign, so all vectors are aligned.
 constexpr int kXcds = 8;
+constexpr int kSuperCols = 8;  // tile columns per super-column of the tile order
+
+// Tile (ti, tj) of the t-th tile of a lower trapezoid of nti x ntj tiles
+// (ti >= tj).  Tiles are ordered by super-columns of kSuperCols tile columns
+// and row by row inside one, so that consecutive tiles (which run together
+// on one XCD) share a row block of L and cycle over few column blocks: the
+// operand working set of an XCD stays within its L2.
+__device__ __forceinline__ void gemm_tile(i64 t, int nti, int ntj, int &ti, int &tj)
+{
+    constexpr int G = kSuperCols;
+    int c0 = 0;
+    for (;;) {
+        const int w = min(G, ntj - c0);
+        const i64 tri = i64(w) * (w + 1) / 2;
+        const i64 cnt = tri + i64(nti - c0 - w) * w;
+        if (t < cnt || c0 + w >= ntj) {
+            if (t < tri) {  // the triangle at the top: row k holds k + 1 tiles
+                int k = int((sqrt(8.0 * double(t) + 1.0) - 1.0) * 0.5);
+                while (i64(k) * (k + 1) / 2 > t) --k;
+                while (i64(k + 1) * (k + 2) / 2 <= t) ++k;
+                ti = c0 + k;
+                tj = c0 + int(t - i64(k) * (k + 1) / 2);
+            } else {
+                const i64 u = t - tri;
+                ti = c0 + w + int(u / w);
+                tj = c0 + int(u % w);
+            }
+            return;
+        }
+        t -= cnt;
+        c0 += w;
+    }
+}
 
 template <typename T> struct GemmCfg;
 template <> struct GemmCfg<float> { static constexpr int KB = 32; };
@@ -500,19 +887,8 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc *__restrict__ descs
 
     const int q = find_desc(prefix, ndesc, lin);
     const GemmDesc g = descs[q];
-    const i64 t = lin - prefix[q];
-    // tile (ti,tj), column-major over the lower trapezoid: column tj holds
-    // nti - tj tiles and starts at tj*nti - tj*(tj-1)/2
-    int tj;
-    {
-        const double a = double(g.nti) + 0.5;
-        tj = int(a - sqrt(a * a - 2.0 * double(t)));
-        tj = max(0, min(tj, g.ntj - 1));
-        auto start = [&](i64 x) { return x * g.nti - (x * (x - 1)) / 2; };
-        while (tj > 0 && start(tj) > t) --tj;
-        while (tj + 1 < g.ntj && start(tj + 1) <= t) ++tj;
-    }
-    const int ti = tj + int(t - (i64(tj) * g.nti - (i64(tj) * (tj - 1)) / 2));
+    int ti, tj;
+    gemm_tile(lin - prefix[q], g.nti, g.ntj, ti, tj);
     const int a0 = g.r0 / kGemmAlign * kGemmAlign;
     const int i0 = a0 + ti * BM, j0 = a0 + tj * BM;
     T *fr = pool + g.foff;
@@ -718,7 +1094,8 @@ __global__ void k_copy_diag(const double *dvec, const double *adiag, double *sm,
 // ---------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------
-// Kernel classes for the per-class timing of sprgpu_profile_factor.
+// Kernel classes for the per-class timing of sprgpu_profile_factor
+// (P_PDIAG: diagonal blocks, P_PROWS: rows below them).
 enum ProfClass { P_INIT = 0, P_EA, P_SMALL, P_PDIAG, P_PROWS, P_GEMM, P_OTHER, P_GEMM_INNER, P_NCLASS };
 
 // Launches the whole factorization.  With ev != nullptr an event is recorded
@@ -778,6 +1155,19 @@ static int factor_impl(sprgpu_handle *h, const double *sm_dev, int mode, const d
                                               static_cast<T *>(d.dscr));
             mark(P_PROWS, 0.0);
             break;
+        case L_BDIAG:
+            k_block_diag<T><<<blocks, 64 * kBdWaves, 0, s>>>(d.panel_desc + L.off, d.nodes, pool,
+                                                             d.dvec,
+                                                   d.adiag, d.dcor, d.status, pa,
+                                                   static_cast<T *>(d.dscr));
+            mark(P_PDIAG, 0.0);
+            break;
+        case L_BTRSM:
+            k_block_trsm<T><<<blocks, 64 * kBtWaves, 0, s>>>(d.panel_desc + L.off, int(L.count),
+                                                  d.panel_prefix + L.off, d.nodes, pool, d.dvec,
+                                                  static_cast<const T *>(d.dscr));
+            mark(P_PROWS, 0.0);
+            break;
         case L_GEMM:
             if (L.tile == kTileBig)
                 k_gemm<T, kTileBig><<<blocks, 256, 0, s>>>(d.gemm_desc + L.off, int(L.count),
@@ -789,7 +1179,7 @@ static int factor_impl(sprgpu_handle *h, const double *sm_dev, int mode, const d
             break;
         }
     }
-    if (!p.panel_desc.empty()) {
+    if (!p.params.block_factor && !p.panel_desc.empty()) {
         k_diag_writeback<T><<<unsigned(p.panel_desc.size()), 256, 0, s>>>(
             d.panel_desc, d.nodes, pool, static_cast<const T *>(d.dscr));
         mark(P_PDIAG, 0.0);

@@ -18,6 +18,14 @@ __device__ __forceinline__ double lane_bcast(double v, int l)
     return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
 }
 
+// v, hidden from the optimiser (no code): keeps a per-lane value from being
+// treated as loop-invariant, so comparisons with it stay where they are used.
+__device__ __forceinline__ int opaque(int v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 // Index of the descriptor whose launch-local prefix range holds b.
 __device__ __forceinline__ int find_desc(const int64_t *prefix, int n, int64_t b)
 {

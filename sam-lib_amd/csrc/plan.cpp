@@ -44,8 +44,8 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
     if (params.narrow_threads != 0 && params.narrow_threads != 64 &&
         params.narrow_threads != 128 && params.narrow_threads != 256)
         return "narrow_threads must be 0, 64, 128 or 256";
-    p.big_tile = params.big_tile ? params.big_tile
-                                 : (precision == SPRGPU_FP32 ? kTileBig : kTileSmall);
+    if (params.block_factor != 0 && params.block_factor != 1) return "block_factor must be 0 or 1";
+    p.big_tile = params.big_tile ? params.big_tile : kTileSmall;
     p.small_f = params.small_f;
     auto M = [&](int k) { return mspar[k - 1]; };
     if (M(1) < 0) return "MSPAR(1) < 0: the symbolic phase did not complete";
@@ -239,10 +239,60 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
         }
         add_launch(L_SMALL, l, off, i64(p.small_nodes.size()) - off,
                    i64(p.small_nodes.size()) - off, 0.0, maxf_small);
-        // 4. blocked fronts: panel steps with trailing updates inside the panel,
-        //    then one Schur-complement update of the U block with K = S.
+        // 4. blocked fronts: kPanelOuter-wide outer blocks, each factored by
+        //    k_block_diag + k_block_trsm and followed by a rank-kPanelOuter
+        //    update of the columns right of it; then one Schur-complement
+        //    update of the U block with K = S.
+        if (p.params.block_factor) {
+            i32 maxblk = 0;
+            for (i32 j : big) maxblk = std::max(maxblk, (p.nodes[j].S + kPanelOuter - 1) / kPanelOuter);
+            for (i32 b = 0; b < maxblk; ++b) {
+                const i32 ob = b * kPanelOuter, oend = ob + kPanelOuter;
+                off = i64(p.panel_desc.size());
+                i64 nstripes = 0;
+                for (i32 j : big) {
+                    const NodeInfo &n = p.nodes[j];
+                    if (ob >= n.S) continue;
+                    const i32 nbw = std::min(kPanelOuter, n.S - ob);
+                    const i32 rows = kTrsmRows * kTrsmWaves;
+                    const i32 ns = (n.F - (ob + nbw) + rows - 1) / rows;
+                    p.panel_desc.push_back(PanelDesc{j, ob, nbw, ns});
+                    p.panel_prefix.push_back(nstripes);
+                    nstripes += ns;
+                }
+                const i64 cnt = i64(p.panel_desc.size()) - off;
+                add_launch(L_BDIAG, l, off, cnt, cnt);
+                if (nstripes > 0) add_launch(L_BTRSM, l, off, cnt, nstripes);
+                // Updates by the blocks of a kOuterRank-wide group: after the
+                // group's last block, one rank-kOuterRank update of the columns
+                // right of the group; after an earlier block, a rank-kPanelOuter
+                // update of the group's remaining columns only.
+                const i32 g0 = ob / kOuterRank * kOuterRank, gend = g0 + kOuterRank;
+                const bool last_in_group = oend >= gend;
+                off = i64(p.gemm_desc.size());
+                i64 ntiles = 0;
+                double fl = 0.0;
+                for (i32 j : big) {
+                    const NodeInfo &n = p.nodes[j];
+                    if (oend >= n.S) continue;
+                    const GemmDesc g = last_in_group
+                        ? make_gemm(n, oend, n.F, n.S, g0, oend - g0, p.big_tile)
+                        : make_gemm(n, oend, n.F, std::min(n.S, gend), ob, kPanelOuter, kTileSmall);
+                    p.gemm_desc.push_back(g);
+                    p.gemm_prefix.push_back(ntiles);
+                    ntiles += tri_tiles(g.nti, g.ntj);
+                    fl += gemm_flops(g);
+                }
+                if (last_in_group)
+                    add_launch(L_GEMM, l, off, i64(p.gemm_desc.size()) - off, ntiles, fl, p.big_tile, 1);
+                else
+                    add_launch(L_GEMM, l, off, i64(p.gemm_desc.size()) - off, ntiles, fl, kTileSmall, 0);
+            }
+        }
+        //    (or: panel steps with trailing updates inside the outer block)
         i32 maxsteps = 0;
-        for (i32 j : big) maxsteps = std::max(maxsteps, (p.nodes[j].S + kPanelNB - 1) / kPanelNB);
+        if (!p.params.block_factor)
+            for (i32 j : big) maxsteps = std::max(maxsteps, (p.nodes[j].S + kPanelNB - 1) / kPanelNB);
         for (i32 s = 0; s < maxsteps; ++s) {
             off = i64(p.panel_desc.size());
             i64 nblk = 0;

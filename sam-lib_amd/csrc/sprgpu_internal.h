@@ -16,15 +16,17 @@ using i64 = int64_t;
 using i32 = int32_t;
 
 // Tile and block sizes shared by the planner and the kernels.
-constexpr int kTileSmall = 64;   // MFMA update tile for the rank-kPanelNB panel updates
-constexpr int kTileBig = 128;    // ... and for the rank-kPanelOuter and Schur updates in FP32
-                                 // (FP64 uses kTileSmall throughout: a 128 tile needs too
-                                 // many registers for two waves per SIMD)
+constexpr int kTileSmall = 64;   // MFMA update tile (the default for every update)
+constexpr int kTileBig = 128;    // larger tile, selectable for the rank-kOuterRank and Schur
+                                 // updates (sprgpu_set_plan_params, the autotuner)
 constexpr int kGemmAlign = 4;    // MFMA update tiles start on rows that are multiples of this
 constexpr int kFrontLdAlign = 16; // front leading dimensions are multiples of this (64 B in FP32)
 constexpr int kPanelNB = 32;     // columns per panel step of the blocked front factor
 constexpr int kPanelRows = 256;  // panel rows per workgroup (one row per thread)
-constexpr int kPanelOuter = 256; // outer block of the two-level panel blocking
+constexpr int kPanelOuter = 128; // outer block of the two-level panel blocking (block kernels)
+constexpr int kOuterRank = 256;  // rank of the trailing updates between outer blocks
+constexpr int kTrsmRows = 16;    // rows below an outer block per wavefront (k_block_trsm) ...
+constexpr int kTrsmWaves = 4;    // ... and wavefronts per workgroup
 constexpr int kSmallF = 64;      // fronts with F <= kSmallF can be factored in LDS (one lane a row)
 constexpr int kInitCols = 8;     // front columns per workgroup in the init kernel
 constexpr int kEaCols = 8;       // child columns per workgroup in extend-add
@@ -79,6 +81,8 @@ enum LaunchKind : i32 {
     L_SMALL = 2,
     L_PANEL = 3,
     L_GEMM = 4,
+    L_BDIAG = 5,  // outer-block diagonal factor (one workgroup per front)
+    L_BTRSM = 6,  // rows below an outer block (16 per wavefront)
 };
 
 // A launch of one kernel over a contiguous range of a task array.
@@ -99,6 +103,7 @@ struct PlanParams {
     i32 small_waves = 1;      // such fronts per workgroup (one wavefront each)
     i32 big_tile = 0;         // MFMA tile of rank-kPanelOuter/Schur updates (0: by precision)
     i32 narrow_threads = 0;   // workgroup size of the narrow solves (0: by front size)
+    i32 block_factor = 1;     // blocked fronts: 1 = outer-block kernels, 0 = panel steps
 };
 
 struct Plan {
@@ -118,7 +123,8 @@ struct Plan {
     std::vector<ColTask> copy_tasks;   // the S columns of every supernode (factor <-> SM)
     std::vector<EaTask> ea_tasks;
     std::vector<i32> small_nodes;
-    std::vector<PanelDesc> panel_desc;
+    std::vector<PanelDesc> panel_desc;  // panel steps, or outer blocks (k0 = ob, nb = width,
+                                        // nrb = k_block_trsm workgroups below the block)
     std::vector<i64> panel_prefix;  // per launch-local prefix of row blocks (exclusive)
     std::vector<GemmDesc> gemm_desc;
     std::vector<i64> gemm_prefix;   // per launch-local prefix of tiles (exclusive)

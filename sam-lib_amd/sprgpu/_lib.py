@@ -58,6 +58,7 @@ SIGNATURES = {
     "sprgpu_factor_solve_graph": (C.c_int, [P, P, P, I64, I64]),
     "sprgpu_autotune": (C.c_int, [P, P, P, I64, I64, P]),
     "sprgpu_plan_params": (C.c_int, [P, P]),
+    "sprgpu_set_plan_params": (C.c_int, [P, P]),
     "sprgpu_last_error": (C.c_char_p, []),
 }
 

@@ -66,6 +66,24 @@ class SprFactor:
     def device_bytes(self) -> int:
         return int(lib().sprgpu_device_bytes(self._h))
 
+    _PARAM_NAMES = ("small_f", "small_waves", "narrow_threads", "big_tile", "block_factor")
+
+    def plan_params(self) -> dict:
+        """Schedule parameters in use (sprgpu_plan_params)."""
+        out = (C.c_int * 5)()
+        check(lib().sprgpu_plan_params(self._h, out), "sprgpu_plan_params")
+        return dict(zip(self._PARAM_NAMES, list(out)))
+
+    def set_plan_params(self, **kw):
+        """Re-plan with some schedule parameters changed (sprgpu_set_plan_params)."""
+        cur = self.plan_params()
+        unknown = set(kw) - set(self._PARAM_NAMES)
+        if unknown:
+            raise ValueError(f"unknown plan parameters {sorted(unknown)}")
+        cur.update(kw)
+        arr = (C.c_int * 5)(*[int(cur[k]) for k in self._PARAM_NAMES])
+        check(lib().sprgpu_set_plan_params(self._h, arr), "sprgpu_set_plan_params")
+
     def set_pattern(self, sm_host: np.ndarray):
         sm_host = np.ascontiguousarray(sm_host, dtype=np.float64)
         check(lib().sprgpu_set_pattern(self._h, _np(sm_host)), "sprgpu_set_pattern")

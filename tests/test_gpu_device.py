@@ -138,3 +138,63 @@ def test_sprfc3_replacement_matches_reference():
     assert info["supeqt"] == g["mspar_out"][26]      # -number of replaced pivots
     assert info["numneg"] == g["mspar_out"][25]
     h.close()
+
+
+@pytest.mark.parametrize("mode", [1, 3])
+@pytest.mark.parametrize("block", [1, 0], ids=["outer_blocks", "panel_steps"])
+def test_pivot_rules_inside_blocked_fronts(block, mode):
+    """TOL(1) = 0.6 puts about 50 pivots of the wide fronts (the root has
+    S = 702: three 256-column outer blocks) under the threshold d/a_jj:
+    SPRFC1 stops at the first one (IERR -3, MSPAR(27) = its equation),
+    SPRFC3 replaces every one of them; both schedules must match the
+    oracle."""
+    torch = _torch()
+    from sprgpu.device import SprFactor
+
+    model, sysm, sm = make_system("hex8", 10, e_sigma=0.5, e_seed=1)
+    xs = sysm.xsuper()
+    assert np.diff(xs).max() > 256
+    tol = [0.6, 0.0, 0.0, 1.0]
+    ms_o, sm_o = sysm.mspar.copy(), sm.copy()
+    if mode == 1:
+        ierr_o, tol_o = oracle.sprsol(1, ms_o, sysm.mtrees, sysm.msifa, sm_o, tol=tol[:3])
+    else:
+        ierr_o, tol_o, dcor_o = oracle.sprfc3(ms_o, sysm.mtrees, sysm.msifa, sm_o, tol)
+    h = SprFactor(sysm.mspar, sysm.mtrees, sysm.msifa, "fp64")
+    h.set_plan_params(block_factor=block)
+    assert h.plan_params()["block_factor"] == block
+    info = h.factor(torch.from_numpy(sm).cuda(), tol=tuple(tol), mode=mode)
+    print(f"mode {mode} block {block}: gpu {info} oracle ierr {ierr_o} "
+          f"mspar26/27 {ms_o[25]} {ms_o[26]}", flush=True)
+    assert info["ierr"] == ierr_o == -3
+    assert info["supeqt"] == ms_o[26]
+    assert info["numneg"] == ms_o[25]
+    if mode == 3:
+        assert ms_o[26] < -20  # many replacements
+        f = h.get_factor()
+        assert np.max(np.abs(f[:sysm.neq] / sm_o[:sysm.neq] - 1)) < 1e-9
+        assert np.linalg.norm(f[sysm.neq:] - sm_o[sysm.neq:]) / np.linalg.norm(sm_o[sysm.neq:]) < 1e-9
+    h.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_outer_blocks_match_panel_steps(precision):
+    """The two blocked-front schedules compute the same factor (the rank-32
+    updates of a row are summed in a different order)."""
+    torch = _torch()
+    from sprgpu.device import SprFactor
+
+    model, sysm, sm = make_system("hex8", 12, e_sigma=0.5, e_seed=1)
+    sm_dev = torch.from_numpy(sm).cuda()
+    fs = []
+    for block in (0, 1):
+        h = SprFactor(sysm.mspar, sysm.mtrees, sysm.msifa, precision)
+        h.set_plan_params(block_factor=block)
+        info = h.factor(sm_dev)
+        assert info["ierr"] == 0
+        fs.append(h.get_factor())
+        h.close()
+    a, b = fs
+    tol = 1e-11 if precision == "fp64" else 2e-4
+    assert np.max(np.abs(a[:sysm.neq] / b[:sysm.neq] - 1)) < tol
+    assert np.linalg.norm(a[sysm.neq:] - b[sysm.neq:]) / np.linalg.norm(b[sysm.neq:]) < tol

@@ -62,7 +62,8 @@ def test_plan_is_consistent(kind, n):
     xs, xl = sysm.xsuper(), sysm.xlindx()
     F, S = np.diff(xl), np.diff(xs)
     assert out[6] == np.sum(F <= 64)
-    assert out[7] == np.sum(np.where(F > 64, (S + 31) // 32, 0))
+    # one diagonal-block launch slot per 128-column outer block of a blocked front
+    assert out[7] == np.sum(np.where(F > 64, (S + 127) // 128, 0))
 
 
 def test_plan_rejects_retained_supernodes():
