@@ -29,7 +29,7 @@ sys.path.insert(0, os.path.join(ROOT, "sam-lib_amd"))
 
 METRIC = "factor+solve s & TFLOP/s, 40^3 hex8: FP64 vs FP32+IR at 1e-10 residual"
 PEAK = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense MFMA TFLOP/s
-GEMM_F32_TRAFFIC_40 = 433.9e6  # measured HBM bytes per k_gemm<float,128> launch, 40^3
+GEMM_F32_TRAFFIC_40 = 333.1e6  # measured HBM bytes per k_gemm<float,64,1> launch, 40^3
 HBM_PEAK = 8000.0  # GB/s
 
 
@@ -232,7 +232,7 @@ def main():
 
         out = np.zeros(24)
         tolv = np.zeros(4)
-        classes = ["init", "extend_add", "small_lds", "diag_writeback", "panel", "mfma_update",
+        classes = ["init", "extend_add", "small_lds", "diag_block", "rows_below", "mfma_update",
                    "other", "mfma_inner"]
         per = {}
         for name, hh in (("fp32", h32), ("fp64", h64)):
@@ -249,18 +249,18 @@ def main():
                     "launches": g["launches"], "avg_launch_ms": g["ms"] / max(g["launches"], 1)}
 
         result["roofline"] = mfma_roofline(
-            "fp32", "mfma_update", "k_gemm<float,128> (rank-256 and Schur updates, one factorization)")
-        # HBM bytes per launch of k_gemm<float,128> on this workload, from
+            "fp32", "mfma_update", "k_gemm<float,64,1> (rank-256 and Schur updates, one factorization)")
+        # HBM bytes per launch of k_gemm<float,64,1> on this workload, from
         # rocprofv3 FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB copy
-        # (profiles/k_gemm_f32_128_hex8_40_hbm_traffic.txt); None at other sizes
+        # (profiles/k_gemm_f32_hex8_40_hbm_traffic.txt); None at other sizes
         if args.n == 40:
             result["roofline"]["traffic"] = GEMM_F32_TRAFFIC_40
         result["roofline_inner_fp32"] = mfma_roofline(
-            "fp32", "mfma_inner", "k_gemm<float,64> (rank-32 updates, one factorization)")
+            "fp32", "mfma_inner", "k_gemm<float,64,0> (rank-128 updates inside a 256-column group)")
         result["roofline_fp64"] = mfma_roofline(
-            "fp64", "mfma_update", "k_gemm<double,64> (rank-256 and Schur updates)")
+            "fp64", "mfma_update", "k_gemm<double,64,1> (rank-256 and Schur updates)")
         result["roofline_inner_fp64"] = mfma_roofline(
-            "fp64", "mfma_inner", "k_gemm<double,64> (rank-32 updates)")
+            "fp64", "mfma_inner", "k_gemm<double,64,0> (rank-128 updates inside a 256-column group)")
         result["kernel_classes_ms"] = {k: {c: round(v[c]["ms"], 3) for c in classes}
                                        for k, v in per.items()}
         log(f"profile: {result['kernel_classes_ms']}")

@@ -678,18 +678,21 @@ __global__ __launch_bounds__(64 * kBdWaves) void k_block_diag(const PanelDesc *_
                 }
         }
         __syncthreads();
-        // 4. rank-nb update of the tiles right of the sub-panel
+        // 4. rank-nb update of the tiles right of the sub-panel, summed apart
+        //    from the tile and subtracted once (for accuracy in FP32)
 #pragma unroll
         for (int q = 0; q < kBdSlots; ++q) {
             if (stj[q] >= 2 * j + 2) {
                 const int ra = sti[q] * 16 - k0 + lr, rb = stj[q] * 16 - k0 + lr;
+                acc_t u = acc_t{0, 0, 0, 0};
 #pragma unroll
                 for (int ks = 0; ks < NB / 4; ++ks) {
                     const int k = ks * 4 + (lane >> 4);
-                    const T a = -Ls[ra * LS + k];
+                    const T a = Ls[ra * LS + k];
                     const T b = Ls[rb * LS + k] * dsh[k];
-                    acc[q] = Mfma<T>::op(a, b, acc[q]);
+                    u = Mfma<T>::op(a, b, u);
                 }
+                acc[q] -= u;
             }
         }
         __syncthreads();
@@ -762,7 +765,9 @@ __global__ __launch_bounds__(64 * kBtWaves) void k_block_trsm(const PanelDesc *_
             Bs[k * LR + cc] = (k < nb) ? bmscr[i64(n.first + k0 + k) * NB + cc] : T(0);
         }
         __syncthreads();
-        // acc(c, r) -= sum_k d_k L(k0 + c, kb cols) L(r, kb cols) over kb < jb
+        // acc(c, r) -= sum_k d_k L(k0 + c, kb cols) L(r, kb cols) over kb < jb,
+        // the sum formed apart and subtracted once (for accuracy in FP32)
+        acc_t u[2] = {acc_t{0, 0, 0, 0}, acc_t{0, 0, 0, 0}};
 #pragma unroll
         for (int kb = 0; kb < NJB - 1; ++kb) {
             if (kb >= jb) break;
@@ -773,9 +778,11 @@ __global__ __launch_bounds__(64 * kBtWaves) void k_block_trsm(const PanelDesc *_
                     const T *rk = Rs + (kb * NB + 16 * hk + kk[e]) * LR;
 #pragma unroll
                     for (int h = 0; h < 2; ++h)
-                        acc[h] = Mfma<T>::op(-rk[16 * h + c], O[kb][hk][e], acc[h]);
+                        u[h] = Mfma<T>::op(rk[16 * h + c], O[kb][hk][e], u[h]);
                 }
         }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) acc[h] -= u[h];
         // O(c, r) = sum_k Bm(k, c) acc(k, r)
 #pragma unroll
         for (int h = 0; h < 2; ++h) O[jb][h] = acc_t{0, 0, 0, 0};
@@ -858,7 +865,10 @@ template <typename T> struct GemmCfg;
 template <> struct GemmCfg<float> { static constexpr int KB = 32; };
 template <> struct GemmCfg<double> { static constexpr int KB = 16; };
 
-template <typename T, int BM>
+// KIND does not change the code: 1 marks the rank-kOuterRank and Schur
+// updates, 0 the narrower updates inside an outer group, so that kernel
+// traces and counters report the two uses apart.
+template <typename T, int BM, int KIND>
 __global__ __launch_bounds__(256) void k_gemm(const GemmDesc *__restrict__ descs, int ndesc,
                                               const i64 *__restrict__ prefix,
                                               T *__restrict__ pool,
@@ -1170,11 +1180,14 @@ static int factor_impl(sprgpu_handle *h, const double *sm_dev, int mode, const d
             break;
         case L_GEMM:
             if (L.tile == kTileBig)
-                k_gemm<T, kTileBig><<<blocks, 256, 0, s>>>(d.gemm_desc + L.off, int(L.count),
-                                                           d.gemm_prefix + L.off, pool, d.dvec);
+                k_gemm<T, kTileBig, 1><<<blocks, 256, 0, s>>>(d.gemm_desc + L.off, int(L.count),
+                                                              d.gemm_prefix + L.off, pool, d.dvec);
+            else if (L.outer)
+                k_gemm<T, kTileSmall, 1><<<blocks, 256, 0, s>>>(
+                    d.gemm_desc + L.off, int(L.count), d.gemm_prefix + L.off, pool, d.dvec);
             else
-                k_gemm<T, kTileSmall><<<blocks, 256, 0, s>>>(d.gemm_desc + L.off, int(L.count),
-                                                             d.gemm_prefix + L.off, pool, d.dvec);
+                k_gemm<T, kTileSmall, 0><<<blocks, 256, 0, s>>>(
+                    d.gemm_desc + L.off, int(L.count), d.gemm_prefix + L.off, pool, d.dvec);
             mark(L.outer ? P_GEMM : P_GEMM_INNER, L.flops);
             break;
         }

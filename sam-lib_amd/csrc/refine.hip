@@ -22,11 +22,13 @@ __global__ __launch_bounds__(256) void k_spmv(const i64 *__restrict__ rowptr,
                                               const double *__restrict__ val,
                                               const double *__restrict__ x, double *y,
                                               const double *__restrict__ bvec, i64 n, i64 ld,
-                                              double beta)
+                                              double beta, int nrhs)
 {
-    const i64 row = (i64(blockIdx.x) * 256 + threadIdx.x) >> 4;
+    // the right-hand side varies fastest over the workgroups, so the rows'
+    // matrix entries are read once from memory for all of them
+    const i64 row = (i64(blockIdx.x / nrhs) * 256 + threadIdx.x) >> 4;
     const int sub = threadIdx.x & 15;
-    const int rhs = blockIdx.y;
+    const int rhs = blockIdx.x % nrhs;
     const double *xx = x + rhs * ld;
     double s = 0.0;
     if (row < n) {
@@ -167,13 +169,26 @@ static int refresh_values(sprgpu_handle *h)
     return 0;
 }
 
+// y = b + beta A x for nrhs columns (b may be null), in chunks of right-hand
+// sides that keep a launch within 2^23 workgroups
+static void launch_spmv(sprgpu_handle *h, const double *x, double *y, const double *b, i64 ld,
+                        i64 nrhs, double beta)
+{
+    const i64 n = h->plan.neq;
+    const i64 nb = (n * 16 + 255) / 256;
+    const i64 chunk = std::max<i64>(1, std::min<i64>(nrhs, (i64(1) << 23) / nb));
+    for (i64 r0 = 0; r0 < nrhs; r0 += chunk) {
+        const i64 nr = std::min(chunk, nrhs - r0);
+        k_spmv<<<unsigned(nb * nr), 256, 0, h->stream>>>(
+            h->d.csr_rowptr, h->d.csr_col, h->d.csr_val, x + r0 * ld, y + r0 * ld,
+            b ? b + r0 * ld : nullptr, n, ld, beta, int(nr));
+    }
+}
+
 int run_spmv(sprgpu_handle *h, const double *x, double *y, i64 ld, i64 nrhs)
 {
     if (int rc = refresh_values(h)) return rc;
-    const i64 n = h->plan.neq;
-    dim3 grid(unsigned((n * 16 + 255) / 256), unsigned(nrhs));
-    k_spmv<<<grid, 256, 0, h->stream>>>(h->d.csr_rowptr, h->d.csr_col, h->d.csr_val, x, y,
-                                        nullptr, n, ld, 1.0);
+    launch_spmv(h, x, y, nullptr, ld, nrhs, 1.0);
     SPRGPU_HIP(hipGetLastError());
     return 0;
 }
@@ -217,16 +232,16 @@ int run_solve_ir(sprgpu_handle *h, const double *b, double *x, i64 ld, i64 nrhs,
     // x = A^-1 b
     k_copy_cols<<<gcol, 256, 0, s>>>(x, ld, b, ld, n);
     if (int rc = run_solve(h, x, ld, nrhs, 3)) return rc;
-    dim3 gsp(unsigned((n * 16 + 255) / 256), unsigned(nrhs));
+
     int it = 0;
     double worst = 0.0;
     for (;;) {
         // r = b - A x
-        k_spmv<<<gsp, 256, 0, s>>>(d.csr_rowptr, d.csr_col, d.csr_val, x, r, b, n, n, -1.0);
         if (ld != n) {
             set_error("solve_ir requires ld == neq");
             return -1;
         }
+        launch_spmv(h, x, r, b, n, nrhs, -1.0);
         if (int rc = col_norms(h, r, n, n, nrhs, part, nrm, rnorm)) return rc;
         if (int rc = check_solve_error(h)) return rc;
         worst = 0.0;

@@ -131,14 +131,14 @@ __global__ __launch_bounds__(256) void k_fwd_narrow(const i32 *__restrict__ list
                                                     const i32 *__restrict__ upd_src,
                                                     const T *__restrict__ pool,
                                                     const double *__restrict__ dvec, double *b,
-                                                    i64 ldb, double *vwork, i64 vstride)
+                                                    i64 ldb, double *vwork, i64 vstride, int nrhs)
 {
     extern __shared__ double smem[];
     __shared__ double vx[64];  // x of the S columns, zero past S
     double *v = smem;
     T *Lb = reinterpret_cast<T *>(smem + maxF);  // Lb[c*S + r] = L(r,c), r > c
-    const NodeInfo n = nodes[list[blockIdx.x]];
-    const int rhs = blockIdx.y, tid = threadIdx.x, nt = blockDim.x, lane = tid & 63;
+    const NodeInfo n = nodes[list[blockIdx.x / nrhs]];
+    const int rhs = blockIdx.x % nrhs, tid = threadIdx.x, nt = blockDim.x, lane = tid & 63;
     const int S = n.S, F = n.F;
     double *vw = vwork + rhs * vstride;
     double *bb = b + rhs * ldb;
@@ -209,14 +209,14 @@ __global__ __launch_bounds__(256) void k_bwd_narrow(const i32 *__restrict__ list
                                                     const NodeInfo *__restrict__ nodes,
                                                     const i32 *__restrict__ lindx,
                                                     const T *__restrict__ pool, double *b,
-                                                    i64 ldb)
+                                                    i64 ldb, int nrhs)
 {
     extern __shared__ double smem[];
     double *v = smem;                 // F entries and 16 zeros
     double *part = smem + maxF + 16;
     T *Lb = reinterpret_cast<T *>(smem + maxF + 16 + 256);  // Lb[r*S + c] = L(r,c), r > c
-    const NodeInfo n = nodes[list[blockIdx.x]];
-    const int rhs = blockIdx.y, tid = threadIdx.x, nt = blockDim.x, lane = tid & 63;
+    const NodeInfo n = nodes[list[blockIdx.x / nrhs]];
+    const int rhs = blockIdx.x % nrhs, tid = threadIdx.x, nt = blockDim.x, lane = tid & 63;
     const int wave = tid >> 6, nw = nt >> 6;
     const int S = n.S, F = n.F;
     double *bb = b + rhs * ldb;
@@ -296,18 +296,19 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
                                                   const double *__restrict__ dvec, double *b,
                                                   i64 ldb, double *vwork, i64 vstride,
                                                   unsigned *flags, i64 nflags,
-                                                  const unsigned *epoch_ptr, int *err)
+                                                  const unsigned *epoch_ptr, int *err, int nrhs)
 {
     __shared__ double xs[NB];
     __shared__ double ys[NB];
     __shared__ double part[2][NB];
     __shared__ T Ld[kPacked + 1];
     const unsigned epoch = *epoch_ptr;  // packed lower diagonal block, by columns; Ld[kZero] = 0
-    const int q = find_desc(prefix, nwide, blockIdx.x);
+    const i64 bid = blockIdx.x / nrhs;
+    const int q = find_desc(prefix, nwide, bid);
     const WideDesc w = wide[q];
-    const int local = int(blockIdx.x - prefix[q]);
+    const int local = int(bid - prefix[q]);
     const NodeInfo n = nodes[w.node];
-    const int rhs = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int rhs = blockIdx.x % nrhs, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int S = n.S, F = n.F;
     double *vw = vwork + rhs * vstride;
     double *v = vw + n.voff;
@@ -438,16 +439,17 @@ __global__ __launch_bounds__(256) void k_bwd_rect(const WideDesc *__restrict__ w
                                                   const i32 *__restrict__ lindx,
                                                   const T *__restrict__ pool,
                                                   const double *__restrict__ b, i64 ldb,
-                                                  double *partials, i64 npart)
+                                                  double *partials, i64 npart, int nrhs)
 {
     __shared__ double xs[NB];
     __shared__ double part[2][NB];
-    const int q = find_desc(prefix, nwide, blockIdx.x);
+    const i64 bid = blockIdx.x / nrhs;
+    const int q = find_desc(prefix, nwide, bid);
     const WideDesc w = wide[q];
-    const int t = int(blockIdx.x - prefix[q]);
+    const int t = int(bid - prefix[q]);
     const int blk = t / w.nrect, k = t % w.nrect;
     const NodeInfo n = nodes[w.node];
-    const int rhs = blockIdx.y, tid = threadIdx.x;
+    const int rhs = blockIdx.x % nrhs, tid = threadIdx.x;
     const int S = n.S, F = n.F;
     const int c0 = blk * NB, nc = min(NB, S - c0);
     const int r0 = S + k * NB, nr = min(NB, F - r0);
@@ -489,18 +491,19 @@ __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ w
                                                   double *vwork, i64 vstride,
                                                   const double *__restrict__ partials, i64 npart,
                                                   unsigned *flags, i64 nflags,
-                                                  const unsigned *epoch_ptr, int *err)
+                                                  const unsigned *epoch_ptr, int *err, int nrhs)
 {
     __shared__ double xs[NB];
     __shared__ double ys[NB];
     __shared__ double part[2][NB];
     __shared__ T Lr[kPacked + 1];
     const unsigned epoch = *epoch_ptr;  // packed lower diagonal block, by rows; Lr[kZero] = 0
-    const int q = find_desc(prefix, nwide, blockIdx.x);
+    const i64 bid = blockIdx.x / nrhs;
+    const int q = find_desc(prefix, nwide, bid);
     const WideDesc w = wide[q];
-    const int blk = w.nblk - 1 - int(blockIdx.x - prefix[q]);
+    const int blk = w.nblk - 1 - int(bid - prefix[q]);
     const NodeInfo n = nodes[w.node];
-    const int rhs = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int rhs = blockIdx.x % nrhs, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int S = n.S;
     double *v = vwork + rhs * vstride + n.voff;
     double *bb = b + rhs * ldb;
@@ -660,16 +663,16 @@ static int solve_impl(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which)
             if (ns) {
                 const int mf = p.small_maxF[l], ms = p.small_maxS[l];
                 const size_t shm = sizeof(double) * mf + sizeof(T) * ms * ms;
-                k_fwd_narrow<T><<<dim3(ns, ur), narrow_threads(l), shm, s>>>(
+                k_fwd_narrow<T><<<ns * ur, narrow_threads(l), shm, s>>>(
                     d.solve_small + p.solve_small_ptr[l], mf, d.nodes, d.lindx, d.upd_ptr,
-                    d.upd_src, pool, d.dvec, b, ldb, d.vwork, vstride);
+                    d.upd_src, pool, d.dvec, b, ldb, d.vwork, vstride, int(nrhs));
             }
             const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
             if (nw)
-                k_fwd_wide<T><<<dim3(unsigned(p.wide_fwd_wgs[l]), ur), 256, 0, s>>>(
+                k_fwd_wide<T><<<unsigned(p.wide_fwd_wgs[l]) * ur, 256, 0, s>>>(
                     d.wide + w0, nw, d.wide_fwd_prefix + w0, d.nodes, d.lindx, d.upd_ptr,
                     d.upd_src, pool, d.dvec, b, ldb, d.vwork, vstride, d.flags, nflags, d.epoch,
-                    d.spin_error);
+                    d.spin_error, int(nrhs));
         }
     }
     if (which & 2) {
@@ -678,19 +681,21 @@ static int solve_impl(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which)
             const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
             if (nw) {
                 if (p.wide_rect_wgs[l] > 0)
-                    k_bwd_rect<T><<<dim3(unsigned(p.wide_rect_wgs[l]), ur), 256, 0, s>>>(
+                    k_bwd_rect<T><<<unsigned(p.wide_rect_wgs[l]) * ur, 256, 0, s>>>(
                         d.wide + w0, nw, d.wide_rect_prefix + w0, d.nodes, d.lindx, pool, b, ldb,
-                        d.partials, npart);
-                k_bwd_wide<T><<<dim3(unsigned(p.wide_bwd_wgs[l]), ur), 256, 0, s>>>(
+                        d.partials, npart, int(nrhs));
+                k_bwd_wide<T><<<unsigned(p.wide_bwd_wgs[l]) * ur, 256, 0, s>>>(
                     d.wide + w0, nw, d.wide_bwd_prefix + w0, d.nodes, d.lindx, pool, b, ldb,
-                    d.vwork, vstride, d.partials, npart, d.flags, nflags, d.epoch, d.spin_error);
+                    d.vwork, vstride, d.partials, npart, d.flags, nflags, d.epoch, d.spin_error,
+                    int(nrhs));
             }
             const unsigned ns = unsigned(p.solve_small_ptr[l + 1] - p.solve_small_ptr[l]);
             if (ns) {
                 const int mf = p.small_maxF[l], ms = p.small_maxS[l];
                 const size_t shm = sizeof(double) * (mf + 16 + 256) + sizeof(T) * ms * ms;
-                k_bwd_narrow<T><<<dim3(ns, ur), narrow_threads(l), shm, s>>>(
-                    d.solve_small + p.solve_small_ptr[l], mf, d.nodes, d.lindx, pool, b, ldb);
+                k_bwd_narrow<T><<<ns * ur, narrow_threads(l), shm, s>>>(
+                    d.solve_small + p.solve_small_ptr[l], mf, d.nodes, d.lindx, pool, b, ldb,
+                    int(nrhs));
             }
         }
     }
@@ -711,8 +716,24 @@ int run_solve(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which)
         set_error("at most 65535 right-hand sides per call");
         return -1;
     }
-    return (h->precision == SPRGPU_FP32) ? solve_impl<float>(h, b, ldb, nrhs, which)
-                                         : solve_impl<double>(h, b, ldb, nrhs, which);
+    // The right-hand side is the fastest-varying workgroup index (the
+    // workgroups of one supernode for all right-hand sides run together and
+    // share its factor through the caches); a launch holds at most 2^23
+    // workgroups, so large batches go in chunks.
+    const Plan &p = h->plan;
+    i64 maxb = 1;
+    for (int l = 0; l < p.nlevels; ++l)
+        maxb = std::max({maxb, i64(p.solve_small_ptr[l + 1] - p.solve_small_ptr[l]),
+                         p.wide_fwd_wgs[l], p.wide_bwd_wgs[l], p.wide_rect_wgs[l]});
+    const i64 chunk = std::max<i64>(1, std::min<i64>(nrhs, (i64(1) << 23) / maxb));
+    for (i64 r0 = 0; r0 < nrhs; r0 += chunk) {
+        const i64 nr = std::min(chunk, nrhs - r0);
+        const int rc = (h->precision == SPRGPU_FP32)
+                           ? solve_impl<float>(h, b + r0 * ldb, ldb, nr, which)
+                           : solve_impl<double>(h, b + r0 * ldb, ldb, nr, which);
+        if (rc) return rc;
+    }
+    return 0;
 }
 
 int check_solve_error(sprgpu_handle *h)
