@@ -124,30 +124,45 @@ __global__ __launch_bounds__(256) void k_front_init(const ColTask *__restrict__ 
             // packed column c of the trapezoid starts at row c:
             // offset c*F - c*(c-1)/2 (OFFSET(F,c) in sprfc1.f)
             const double *src = lnz + n.xlnz + i64(c) * n.F - (i64(c) * (c - 1)) / 2;
-            for (int r = c + threadIdx.x; r < n.F; r += blockDim.x) {
-                double v = src[r - c];
-                nnz += (v != 0.0);
-                if (r == c) {
-                    const int eq = n.first + c;
-                    if (mode == 3) {
-                        dcor[2 * i64(eq)] = 0.0;
-                        dcor[2 * i64(eq) + 1] = 0.0;
-                    }
-                    if (v == 0.0) {
-                        if (mode == 3) {  // sprfc3.f:560-575
-                            v = 1.0;
-                            dcor[2 * i64(eq) + 1] = 1.0;
-                        } else {          // sprfc1.f:473-480
-                            atomicAdd(&st->nzero, 1ull);
-                            atomicMin(&st->first_zero, (unsigned long long)eq);
-                        }
-                    }
-                    adiag[eq] = v;
+            int r = c + 1 + threadIdx.x;
+            // rows below the diagonal, eight loads in flight per thread
+            for (; r + 7 * 256 < n.F; r += 8 * 256) {
+                double v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = src[r + 256 * u - c];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    nnz += (v[u] != 0.0);
+                    col[r + 256 * u] = T(v[u]);
                 }
+            }
+            for (; r < n.F; r += 256) {
+                const double v = src[r - c];
+                nnz += (v != 0.0);
                 col[r] = T(v);
             }
+            if (threadIdx.x == 0) {  // the diagonal: a_jj, zero pivots, DCOR
+                double v = src[0];
+                nnz += (v != 0.0);
+                const int eq = n.first + c;
+                if (mode == 3) {
+                    dcor[2 * i64(eq)] = 0.0;
+                    dcor[2 * i64(eq) + 1] = 0.0;
+                }
+                if (v == 0.0) {
+                    if (mode == 3) {  // sprfc3.f:560-575
+                        v = 1.0;
+                        dcor[2 * i64(eq) + 1] = 1.0;
+                    } else {          // sprfc1.f:473-480
+                        atomicAdd(&st->nzero, 1ull);
+                        atomicMin(&st->first_zero, (unsigned long long)eq);
+                    }
+                }
+                adiag[eq] = v;
+                col[c] = T(v);
+            }
         } else {
-            for (int r = c + threadIdx.x; r < n.F; r += blockDim.x) col[r] = T(0);
+            for (int r = c + threadIdx.x; r < n.F; r += 256) col[r] = T(0);
         }
     }
     // block reduction of the non-zero count
@@ -180,7 +195,24 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask *__restrict__ t
         const int pj = map[jc];
         const T *src = pool + c.foff + i64(c.S + jc) * c.ld + c.S;
         T *dst = pool + p.foff + i64(pj) * p.ld;
-        for (int i = jc + lane; i < U; i += 64) dst[map[i]] += src[i];
+        // eight rows per lane at a time: the map and child loads, then the
+        // parent loads, then the stores, so each group costs two memory
+        // round trips instead of three per row
+        int i = jc + lane;
+        for (; i + 7 * 64 < U; i += 8 * 64) {
+            int m[8];
+            T v[8], d[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                m[u] = map[i + 64 * u];
+                v[u] = src[i + 64 * u];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) d[u] = dst[m[u]];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) dst[m[u]] = d[u] + v[u];
+        }
+        for (; i < U; i += 64) dst[map[i]] += src[i];
     }
 }
 
