@@ -187,6 +187,17 @@ int sprgpu_create(const sprgpu_int *mspar, const sprgpu_int *mtrees, const sprgp
         return hip_fail(hipGetLastError(), "device memset");
     }
     B += esz * p.pool_elems + sizeof(double) * 4 * p.neq + sizeof(FactorStatus) + 8 * 4 * 257;
+    // the factorization's second stream and its two events (default priority)
+    if (hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev[1], hipEventDisableTiming) != hipSuccess) {
+        const hipError_t e = hipGetLastError();
+        for (hipEvent_t &x : h->ev)
+            if (x) (void)hipEventDestroy(x);
+        if (h->stream2) (void)hipStreamDestroy(h->stream2);
+        free_all(h.get());
+        return hip_fail(e, "stream/event creation");
+    }
     *handle = h.release();
     return 0;
 }
@@ -197,6 +208,9 @@ int sprgpu_destroy(sprgpu_handle *h)
     (void)hipDeviceSynchronize();
     drop_graph(h);
     if (h->capture_stream) (void)hipStreamDestroy(h->capture_stream);
+    for (hipEvent_t &x : h->ev)
+        if (x) (void)hipEventDestroy(x);
+    if (h->stream2) (void)hipStreamDestroy(h->stream2);
     free_all(h);
     delete h;
     return 0;
@@ -447,15 +461,16 @@ int sprgpu_plan_summary(const sprgpu_int *mspar, const sprgpu_int *mtrees,
         set_error(err);
         return -2;
     }
-    i64 tiles = 0, steps = 0;
+    i64 tiles = 0, steps = 0, kernels = 0;
     for (const Launch &L : p.launches) {
         if (L.kind == L_GEMM) tiles += L.blocks;
         if (L.kind == L_PANEL || L.kind == L_BDIAG) steps += L.count;
+        if (L.kind != L_RECORD && L.kind != L_WAIT) ++kernels;
     }
     out[0] = double(p.neq);
     out[1] = double(p.nsuper);
     out[2] = double(p.nlevels);
-    out[3] = double(p.launches.size());
+    out[3] = double(kernels);
     out[4] = double(p.pool_elems);
     out[5] = double(p.vwork_elems);
     out[6] = double(p.small_nodes.size());

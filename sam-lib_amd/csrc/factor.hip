@@ -329,24 +329,6 @@ __global__ __launch_bounds__(256) void k_small(const i32 *__restrict__ list, int
         for (int c = 0; c <= r; ++c) fr[r + i64(c) * n.ld] = M[r + c * LDM];
 }
 
-// Bit-mask selects: x where the mask is all ones, zero where it is zero.  The
-// masks come from integer arithmetic on the lane index, so no comparison
-// results (scalar register pairs) are created; the compiler would otherwise
-// evaluate one per loop step up front and spill them.
-__device__ __forceinline__ int mask_ge(int a, int b) { return (b - 1 - a) >> 31; }  // a >= b
-__device__ __forceinline__ float mask_sel(float x, int m)
-{
-    return __builtin_bit_cast(float, __builtin_bit_cast(int, x) & m);
-}
-__device__ __forceinline__ double mask_sel(double x, int m)
-{
-    return __builtin_bit_cast(double, __builtin_bit_cast(long long, x) & (long long)m);
-}
-template <typename T> __device__ __forceinline__ T mask_blend(T a, T b, int m)  // m ? a : b
-{
-    return T(mask_sel(a, m) + mask_sel(b, ~m));
-}
-
 // Reciprocal from the hardware estimate refined by Newton steps (one for
 // FP32, two for FP64); zero and infinite inputs give the estimate itself
 // (inf and 0, as 1/d would).
@@ -366,25 +348,27 @@ __device__ __forceinline__ double recip(double d)
 }
 
 // LDL' of an nb x nb diagonal block (nb <= kPanelNB; entry (r, c) of its
-// lower triangle at src[r*srs + c*scs]) by one wavefront, right-looking over
-// the columns (SPRF17's recurrence l_rj = (a_rj - sum_k l_rk d_k l_jk) / d_j,
-// one column at a time).  Lane r + 32h holds columns 16h..16h+15 of row r in
+// lower triangle at src[r*srs + c*scs]) by a workgroup of kChainThreads,
+// right-looking over the columns (SPRF17's recurrence
+// l_rj = (a_rj - sum_k l_rk d_k l_jk) / d_j, one column at a time).  Thread
+// t holds entries (r, 4g..4g+3) of the block, r = t / 8, g = t % 8, in
 // registers; entries right of a row's diagonal are scratch.  At step j the
-// half that holds column j writes it to LDS (rows above j as zeros) and lane
-// j writes its row of X (below), and every lane reads the pivot, its own
-// entry, its 16 columns and X's row j back as broadcasts, then updates its
-// entries unmasked (the zeros leave finished columns unchanged).  Alongside,
-// each lane keeps its row of X = L11^-1, built as the product of the
-// elimination steps (I - l_j e_j^T) in the same loop: X(r, :) -= l_rj X(j, :).
-// The pivots are recorded (when record is set) by each lane for its own row
+// threads holding column j write it to LDS (rows above j as zeros) and the
+// threads of row j write their part of row j of X = L11^-1; after one
+// barrier every thread reads the pivot, its row's entry and the four
+// column-j values and X(j, :) values it needs, and updates its entries
+// unmasked (the zeros leave finished columns unchanged).  X is built as the
+// product of the elimination steps, X(r, :) -= l_rj X(j, :), in the same
+// loop.  The pivots are recorded (when record is set) by one thread per row
 // after the loop.  Bm = L11^-T D^-1 = X^T D^-1 (upper triangular, zero
 // outside nb x nb) is left in LDS.  The factored rows (l_rc for c < r, d_r at
 // c = r) go to out[r*rs + c*cs] when out is set, with zeros right of the
 // diagonal if zero_upper; the rows of Bm go to bm_out + r*kPanelNB when
-// bm_out is set (rows r < nb).  lsh holds kLshSize elements of LDS scratch
-// (16-byte aligned).  Returns lane r's pivot d_r (zero for r >= nb; lanes
-// 32..63 return the value of lane - 32).
-constexpr int kLshSize = 7 * kPanelNB;  // LDS scratch elements
+// bm_out is set (rows r < nb).  lsh holds kLshSize elements of T and then
+// kPanelNB doubles of LDS scratch (16-byte aligned).  Returns the pivot d_r of the thread's row (zero for
+// r >= nb).  Every thread of the workgroup calls it.
+constexpr int kChainThreads = 256;
+constexpr int kLshSize = 7 * kPanelNB;  // LDS scratch elements of T (plus kPanelNB doubles)
 
 template <typename T>
 __device__ __forceinline__ T diag_block_factor(const T *src, i64 srs, i64 scs, int nb, int eq0,
@@ -393,94 +377,80 @@ __device__ __forceinline__ T diag_block_factor(const T *src, i64 srs, i64 scs, i
                                                FactorStatus *st, T (*Bm)[kPanelNB + 1], T *lsh,
                                                T *out, i64 rs, i64 cs, bool zero_upper, T *bm_out)
 {
-    constexpr int NB = kPanelNB, NH = NB / 2;
-    static_assert(NB == 32, "two 16-column halves per row");
-    typedef T vec_t __attribute__((ext_vector_type(16 / sizeof(T))));
-    constexpr int VE = 16 / sizeof(T);
-    const int lane = opaque(int(threadIdx.x & 63));
-    const int r = lane & 31, h = lane >> 5, c0 = NH * h;
+    constexpr int NB = kPanelNB, Q = 4;
+    static_assert(NB * NB == kChainThreads * Q, "four entries of the block per thread");
+    typedef T vec_t __attribute__((ext_vector_type(Q)));
+    const int t = threadIdx.x, r = t >> 3, g = t & 7, c0 = Q * g;
     T *pd = lsh;                 // [NB] pivots d_j
     T *pi = lsh + NB;            // [NB] 1/d_j
     T *pin = lsh + 2 * NB;       // [NB] pivots before the replacement rule
-    T rq[NH], xq[NH];
+    double *ash = reinterpret_cast<double *>(lsh + 7 * NB);  // [NB] a_jj
+    if (t < nb) ash[t] = adiag[eq0 + t];
+    T a[Q], x[Q];
 #pragma unroll
-    for (int q = 0; q < NH; ++q) {
-        rq[q] = (r < nb && c0 + q <= r) ? src[r * srs + (c0 + q) * scs] : T(0);
-        xq[q] = mask_sel(T(1), mask_ge(r, c0 + q) & mask_ge(c0 + q, r));
+    for (int q = 0; q < Q; ++q) {
+        a[q] = (r < nb && c0 + q <= r) ? src[r * srs + (c0 + q) * scs] : T(0);
+        x[q] = (c0 + q == r) ? T(1) : T(0);
     }
-    const double avr = (r < nb) ? adiag[eq0 + r] : 0.0;
+    for (int j = 0; j < nb; ++j) {
+        // column j and row j of X, double-buffered: a buffer is rewritten two
+        // steps later, after every thread has passed the next barrier
+        T *wcol = lsh + (3 + 2 * (j & 1)) * NB, *xrow = wcol + NB;
+        if (g == (j >> 2)) {
+            T v = a[0];
 #pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        if (j < nb) {
-            // column j and row j of X, double-buffered: the step after next
-            // overwrites them, after every lane's reads of this step are done
-            T *wcol = lsh + (3 + 2 * (j & 1)) * NB, *xrow = wcol + NB;
-            if (h == (j >> 4)) wcol[r] = mask_sel(rq[j & 15], mask_ge(r, j));
-            if (r == j) {
+            for (int q = 1; q < Q; ++q)
+                if ((j & 3) == q) v = a[q];
+            wcol[r] = (r >= j) ? v : T(0);
+        }
+        if (r == j) {
+            vec_t xv;
 #pragma unroll
-                for (int v = 0; v < NH / VE; ++v) {
-                    vec_t x;
+            for (int q = 0; q < Q; ++q) xv[q] = x[q];
+            *reinterpret_cast<vec_t *>(xrow + c0) = xv;
+        }
+        __syncthreads();
+        const T dj = wcol[j];
+        const T wr = wcol[r];
+        const vec_t wc = *reinterpret_cast<const vec_t *>(wcol + c0);
+        const vec_t xj = *reinterpret_cast<const vec_t *>(xrow + c0);
+        const double dp = pivot_value(double(dj), ash[j], pa);
+        const T d = T(dp);
+        const T di = recip(d);
+        const T l = (r > j) ? wr * di : T(0);
 #pragma unroll
-                    for (int e = 0; e < VE; ++e) x[e] = xq[VE * v + e];
-                    *reinterpret_cast<vec_t *>(xrow + c0 + VE * v) = x;
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            const T dj = wcol[j];
-            const T wr = wcol[r];
-            T wc[NH], xj[NH];
+        for (int q = 0; q < Q; ++q) {
+            a[q] -= l * wc[q];
+            x[q] -= l * xj[q];
+        }
+        if (g == (j >> 2)) {
 #pragma unroll
-            for (int v = 0; v < NH / VE; ++v) {
-                const vec_t x = *reinterpret_cast<const vec_t *>(wcol + c0 + VE * v);
-                const vec_t y = *reinterpret_cast<const vec_t *>(xrow + c0 + VE * v);
-#pragma unroll
-                for (int e = 0; e < VE; ++e) {
-                    wc[VE * v + e] = x[e];
-                    xj[VE * v + e] = y[e];
-                }
-            }
-            // all reads of the step in flight before the first use
-            __builtin_amdgcn_sched_barrier(0);
-            const double dp = pivot_value(double(dj), lane_bcast(avr, j), pa);
-            const T d = T(dp);
-            const T di = recip(d);
-            const T l = mask_sel(wr * di, mask_ge(r, j + 1));  // rows r > j only
-#pragma unroll
-            for (int q = 0; q < NH; ++q) {
-                rq[q] -= l * wc[q];
-                xq[q] -= l * xj[q];
-            }
-            if (h == (j >> 4)) {
-                const int eq = mask_ge(r, j) & mask_ge(j, r);  // r == j
-                rq[j & 15] = mask_blend(d, l, eq);
-            }
-            if (lane == 0) {
-                pd[j] = d;
-                pi[j] = di;
-                pin[j] = dj;
-            }
+            for (int q = 0; q < Q; ++q)
+                if ((j & 3) == q) a[q] = (r == j) ? d : l;
+        }
+        if (t == 0) {
+            pd[j] = d;
+            pi[j] = di;
+            pin[j] = dj;
         }
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __syncthreads();
     const T dr = (r < nb) ? pd[r] : T(0);
     const T dir = (r < nb) ? pi[r] : T(0);
-    if (record && h == 0 && r < nb)
-        pivot_record(double(pin[r]), double(dr), avr, eq0 + r, pa, dvec, dcor, st);
+    if (record && g == 0 && r < nb)
+        pivot_record(double(pin[r]), double(dr), ash[r], eq0 + r, pa, dvec, dcor, st);
     if (out && r < nb) {
 #pragma unroll
-        for (int q = 0; q < NH; ++q) {
+        for (int q = 0; q < Q; ++q) {
             const int c = c0 + q;
-            if (c <= r || zero_upper) out[r * rs + c * cs] = mask_sel(rq[q], mask_ge(r, c));
+            if (c <= r || zero_upper) out[r * rs + c * cs] = (c <= r) ? a[q] : T(0);
         }
     }
-    // Bm[i][c] = X[c][i] / d_c: lane c writes column c (zero for c >= nb)
+    // Bm[i][c] = X[c][i] / d_c: the threads of row c write column c (zero
+    // for c >= nb)
 #pragma unroll
-    for (int q = 0; q < NH; ++q) {
-        const T v = xq[q] * dir;
+    for (int q = 0; q < Q; ++q) {
+        const T v = x[q] * dir;
         Bm[c0 + q][r] = v;
         if (bm_out && c0 + q < nb && r < nb) bm_out[i64(c0 + q) * NB + r] = v;
     }
@@ -512,7 +482,7 @@ __global__ __launch_bounds__(256) void k_panel(const PanelDesc *__restrict__ des
     static_assert(NB == 32 && kPanelRows == 256,
                   "one lane per diagonal-block row; four 64-row MFMA slices per workgroup");
     __shared__ T Bm[NB][NB + 1];  // L11^-T D^-1 (upper triangular), zero outside nb x nb
-    __shared__ __attribute__((aligned(16))) T Lsh[kLshSize];
+    __shared__ __attribute__((aligned(16))) T Lsh[kLshSize + kPanelNB * sizeof(double) / sizeof(T)];
     const int q = find_desc(prefix, ndesc, blockIdx.x);
     const PanelDesc pd = descs[q];
     const int rb = int(blockIdx.x - prefix[q]);
@@ -533,11 +503,9 @@ __global__ __launch_bounds__(256) void k_panel(const PanelDesc *__restrict__ des
             const int row = rw + ti * 16 + (lane & 15), col = ks * 4 + (lane >> 4);
             a[ti][ks] = (row < n.F && col < nb) ? fr[row + (k0 + col) * ld] : T(0);
         }
-    if (wave == 0)
-        diag_block_factor<T>(fr + k0 + k0 * ld, 1, ld, nb, n.first + k0, adiag, rb == 0, pa, dvec,
-                             dcor, st, Bm,
-                             Lsh, rb == 0 ? dscr + i64(n.first + k0) * NB : nullptr, NB, 1, true,
-                             nullptr);
+    diag_block_factor<T>(fr + k0 + k0 * ld, 1, ld, nb, n.first + k0, adiag, rb == 0, pa, dvec,
+                         dcor, st, Bm, Lsh, rb == 0 ? dscr + i64(n.first + k0) * NB : nullptr, NB,
+                         1, true, nullptr);
     __syncthreads();
     if (!active) return;
     typename Mfma<T>::acc_t acc[4][2];
@@ -602,6 +570,7 @@ __global__ __launch_bounds__(256) void k_diag_writeback(const PanelDesc *__restr
 // The operations are those of the panel steps (same recurrences, same Bm
 // products); only the order of the rank-kPanelNB updates of a row changes.
 constexpr int kBdWaves = 4;                                      // k_block_diag
+static_assert(64 * kBdWaves == kChainThreads, "the diagonal chain uses the whole workgroup");
 constexpr int kBdTiles = (kPanelOuter / 16) * (kPanelOuter / 16 + 1) / 2;
 constexpr int kBdSlots = (kBdTiles + kBdWaves - 1) / kBdWaves;  // 16x16 tiles per wavefront
 
@@ -633,7 +602,7 @@ __global__ __launch_bounds__(64 * kBdWaves) void k_block_diag(const PanelDesc *_
     constexpr int NB = kPanelNB, W = kPanelOuter, LS = NB + 1;
     typedef typename Mfma<T>::acc_t acc_t;
     __shared__ T Bm[NB][NB + 1];
-    __shared__ __attribute__((aligned(16))) T Lsh[kLshSize];
+    __shared__ __attribute__((aligned(16))) T Lsh[kLshSize + kPanelNB * sizeof(double) / sizeof(T)];
     __shared__ T Ls[W * LS];  // the sub-step's two tile columns, rows relative to k0
     __shared__ T dsh[NB];
     const PanelDesc pd = descs[blockIdx.x];
@@ -681,11 +650,11 @@ __global__ __launch_bounds__(64 * kBdWaves) void k_block_diag(const PanelDesc *_
         }
         __syncthreads();
         // 2. the diagonal 32 x 32 block
-        if (wave == 0) {
+        {
             const T dr = diag_block_factor<T>(Ls, LS, 1, nb, n.first + pd.k0 + k0, adiag, true, pa,
                                               dvec, dcor, st, Bm, Lsh, fr + k0 + k0 * ld, 1, ld,
                                               false, bmscr + i64(n.first + pd.k0 + k0) * NB);
-            if (lane < NB) dsh[lane] = dr;
+            if ((tid & 7) == 0) dsh[tid >> 3] = dr;
         }
         __syncthreads();
         // 3. rows below it: L = A Bm, 16-row slices dealt to the wavefronts
@@ -1166,16 +1135,31 @@ static int factor_impl(sprgpu_handle *h, const double *sm_dev, int mode, const d
     mark(P_OTHER, 0.0);
     k_status_reset<<<1, 64, 0, s>>>(d.status);
     mark(P_OTHER, 0.0);
+    // The plan's far updates go to a second stream, ordered by events
+    // (L_RECORD, L_WAIT); the profiling pass runs everything on one stream.
+    const bool two = (ev == nullptr);
+    if (two) {
+        SPRGPU_HIP(hipEventRecord(h->ev[0], s));  // fork
+        SPRGPU_HIP(hipStreamWaitEvent(h->stream2, h->ev[0], 0));
+        SPRGPU_HIP(hipEventRecord(h->ev[1], h->stream2));
+    }
     for (const Launch &L : p.launches) {
         const unsigned blocks = unsigned(L.blocks);
+        const hipStream_t ls = (two && L.stream) ? h->stream2 : s;
         switch (L.kind) {
+        case L_RECORD:
+            if (two) SPRGPU_HIP(hipEventRecord(h->ev[L.stream], ls));
+            break;
+        case L_WAIT:
+            if (two) SPRGPU_HIP(hipStreamWaitEvent(ls, h->ev[1 - L.stream], 0));
+            break;
         case L_INIT:
-            k_front_init<T><<<blocks, 256, 0, s>>>(d.init_tasks + L.off, d.nodes, lnz, pool,
+            k_front_init<T><<<blocks, 256, 0, ls>>>(d.init_tasks + L.off, d.nodes, lnz, pool,
                                                    d.adiag, d.dcor, d.status, mode);
             mark(P_INIT, 0.0);
             break;
         case L_EA:
-            k_extend_add<T><<<blocks, 256, 0, s>>>(d.ea_tasks + L.off, d.nodes, d.relmap, pool);
+            k_extend_add<T><<<blocks, 256, 0, ls>>>(d.ea_tasks + L.off, d.nodes, d.relmap, pool);
             mark(P_EA, 0.0);
             break;
         case L_SMALL: {
@@ -1184,45 +1168,49 @@ static int factor_impl(sprgpu_handle *h, const double *sm_dev, int mode, const d
             const size_t per = ((size_t(mf) * (mf + 1) * sizeof(T) + 7) / 8 * 8 +
                                 8 * size_t(mf) + sizeof(T) * mf + 15) / 16 * 16;
             const unsigned nb = unsigned((L.count + waves - 1) / waves);
-            k_small<T><<<nb, 64 * waves, per * waves, s>>>(
+            k_small<T><<<nb, 64 * waves, per * waves, ls>>>(
                 d.small_nodes + L.off, int(L.count), mf, d.nodes, d.children, d.relmap, lnz, pool,
                 d.dvec, d.adiag, d.dcor, d.status, pa);
             mark(P_SMALL, 0.0);
             break;
         }
         case L_PANEL:
-            k_panel<T><<<blocks, 256, 0, s>>>(d.panel_desc + L.off, int(L.count),
+            k_panel<T><<<blocks, 256, 0, ls>>>(d.panel_desc + L.off, int(L.count),
                                               d.panel_prefix + L.off, d.nodes, pool, d.dvec,
                                               d.adiag, d.dcor, d.status, pa,
                                               static_cast<T *>(d.dscr));
             mark(P_PROWS, 0.0);
             break;
         case L_BDIAG:
-            k_block_diag<T><<<blocks, 64 * kBdWaves, 0, s>>>(d.panel_desc + L.off, d.nodes, pool,
+            k_block_diag<T><<<blocks, 64 * kBdWaves, 0, ls>>>(d.panel_desc + L.off, d.nodes, pool,
                                                              d.dvec,
                                                    d.adiag, d.dcor, d.status, pa,
                                                    static_cast<T *>(d.dscr));
             mark(P_PDIAG, 0.0);
             break;
         case L_BTRSM:
-            k_block_trsm<T><<<blocks, 64 * kBtWaves, 0, s>>>(d.panel_desc + L.off, int(L.count),
+            k_block_trsm<T><<<blocks, 64 * kBtWaves, 0, ls>>>(d.panel_desc + L.off, int(L.count),
                                                   d.panel_prefix + L.off, d.nodes, pool, d.dvec,
                                                   static_cast<const T *>(d.dscr));
             mark(P_PROWS, 0.0);
             break;
         case L_GEMM:
             if (L.tile == kTileBig)
-                k_gemm<T, kTileBig, 1><<<blocks, 256, 0, s>>>(d.gemm_desc + L.off, int(L.count),
+                k_gemm<T, kTileBig, 1><<<blocks, 256, 0, ls>>>(d.gemm_desc + L.off, int(L.count),
                                                               d.gemm_prefix + L.off, pool, d.dvec);
             else if (L.outer)
-                k_gemm<T, kTileSmall, 1><<<blocks, 256, 0, s>>>(
+                k_gemm<T, kTileSmall, 1><<<blocks, 256, 0, ls>>>(
                     d.gemm_desc + L.off, int(L.count), d.gemm_prefix + L.off, pool, d.dvec);
             else
-                k_gemm<T, kTileSmall, 0><<<blocks, 256, 0, s>>>(
+                k_gemm<T, kTileSmall, 0><<<blocks, 256, 0, ls>>>(
                     d.gemm_desc + L.off, int(L.count), d.gemm_prefix + L.off, pool, d.dvec);
             mark(L.outer ? P_GEMM : P_GEMM_INNER, L.flops);
             break;
         }
+    }
+    if (two) {  // join
+        SPRGPU_HIP(hipEventRecord(h->ev[1], h->stream2));
+        SPRGPU_HIP(hipStreamWaitEvent(s, h->ev[1], 0));
     }
     if (!p.params.block_factor && !p.panel_desc.empty()) {
         k_diag_writeback<T><<<unsigned(p.panel_desc.size()), 256, 0, s>>>(

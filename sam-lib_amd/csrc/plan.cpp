@@ -166,9 +166,13 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
     p.launches.clear();
 
     auto add_launch = [&](LaunchKind kind, int level, i64 off, i64 count, i64 blocks,
-                          double flops = 0.0, i32 tile = 0, i32 outer = 0) {
+                          double flops = 0.0, i32 tile = 0, i32 outer = 0, i32 stream = 0) {
         if (count > 0 && blocks > 0)
-            p.launches.push_back(Launch{kind, level, off, count, blocks, flops, tile, outer});
+            p.launches.push_back(
+                Launch{kind, level, off, count, blocks, flops, tile, outer, stream});
+    };
+    auto add_sync = [&](LaunchKind kind, int level, i32 stream) {
+        p.launches.push_back(Launch{kind, level, 0, 0, 0, 0.0, 0, 0, stream});
     };
     // useful flops of a GEMM descriptor: 2*kn per entry (i,j) of the lower
     // trapezoid rows [r0,rend) x cols [r0,cend), i >= j
@@ -263,31 +267,60 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
                 const i64 cnt = i64(p.panel_desc.size()) - off;
                 add_launch(L_BDIAG, l, off, cnt, cnt);
                 if (nstripes > 0) add_launch(L_BTRSM, l, off, cnt, nstripes);
-                // Updates by the blocks of a kOuterRank-wide group: after the
-                // group's last block, one rank-kOuterRank update of the columns
-                // right of the group; after an earlier block, a rank-kPanelOuter
-                // update of the group's remaining columns only.
+                // Updates by the blocks of a kOuterRank-wide group.  After an
+                // earlier block of the group: a rank-kPanelOuter update of the
+                // group's remaining columns.  After its last block: the
+                // rank-kOuterRank update of the next group's columns ("near",
+                // on the handle's stream) and of the columns beyond it
+                // ("far", on the second stream).  The far update runs beside
+                // the next group's diagonal blocks, which do not touch its
+                // columns; the next near update, which does, waits for it.
                 const i32 g0 = ob / kOuterRank * kOuterRank, gend = g0 + kOuterRank;
                 const bool last_in_group = oend >= gend;
-                off = i64(p.gemm_desc.size());
-                i64 ntiles = 0;
-                double fl = 0.0;
-                for (i32 j : big) {
-                    const NodeInfo &n = p.nodes[j];
-                    if (oend >= n.S) continue;
-                    const GemmDesc g = last_in_group
-                        ? make_gemm(n, oend, n.F, n.S, g0, oend - g0, p.big_tile)
-                        : make_gemm(n, oend, n.F, std::min(n.S, gend), ob, kPanelOuter, kTileSmall);
-                    p.gemm_desc.push_back(g);
-                    p.gemm_prefix.push_back(ntiles);
-                    ntiles += tri_tiles(g.nti, g.ntj);
-                    fl += gemm_flops(g);
+                auto gemm_launch = [&](i32 stream, auto range) {
+                    const i64 o = i64(p.gemm_desc.size());
+                    i64 ntiles = 0;
+                    double fl = 0.0;
+                    for (i32 j : big) {
+                        const NodeInfo &n = p.nodes[j];
+                        i32 r0, cend;
+                        if (!range(n, r0, cend)) continue;
+                        const i32 kn = last_in_group ? oend - g0 : kPanelOuter;
+                        const i32 tile = last_in_group ? p.big_tile : kTileSmall;
+                        const GemmDesc g = make_gemm(n, r0, n.F, cend, last_in_group ? g0 : ob, kn, tile);
+                        p.gemm_desc.push_back(g);
+                        p.gemm_prefix.push_back(ntiles);
+                        ntiles += tri_tiles(g.nti, g.ntj);
+                        fl += gemm_flops(g);
+                    }
+                    add_launch(L_GEMM, l, o, i64(p.gemm_desc.size()) - o, ntiles, fl,
+                               last_in_group ? p.big_tile : kTileSmall, last_in_group ? 1 : 0,
+                               stream);
+                };
+                if (!last_in_group) {
+                    gemm_launch(0, [&](const NodeInfo &n, i32 &r0, i32 &cend) {
+                        r0 = oend;
+                        cend = std::min(n.S, gend);
+                        return oend < n.S;
+                    });
+                } else {
+                    add_sync(L_RECORD, l, 0);  // the group's L columns are final
+                    add_sync(L_WAIT, l, 0);    // the previous far update is done
+                    gemm_launch(0, [&](const NodeInfo &n, i32 &r0, i32 &cend) {
+                        r0 = oend;
+                        cend = std::min(n.S, oend + kOuterRank);
+                        return oend < n.S;
+                    });
+                    add_sync(L_WAIT, l, 1);
+                    gemm_launch(1, [&](const NodeInfo &n, i32 &r0, i32 &cend) {
+                        r0 = oend + kOuterRank;
+                        cend = n.S;
+                        return r0 < n.S;
+                    });
+                    add_sync(L_RECORD, l, 1);
                 }
-                if (last_in_group)
-                    add_launch(L_GEMM, l, off, i64(p.gemm_desc.size()) - off, ntiles, fl, p.big_tile, 1);
-                else
-                    add_launch(L_GEMM, l, off, i64(p.gemm_desc.size()) - off, ntiles, fl, kTileSmall, 0);
             }
+            add_sync(L_WAIT, l, 0);  // every far update of the level is done
         }
         //    (or: panel steps with trailing updates inside the outer block)
         i32 maxsteps = 0;

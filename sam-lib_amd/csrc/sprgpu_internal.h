@@ -81,8 +81,10 @@ enum LaunchKind : i32 {
     L_SMALL = 2,
     L_PANEL = 3,
     L_GEMM = 4,
-    L_BDIAG = 5,  // outer-block diagonal factor (one workgroup per front)
-    L_BTRSM = 6,  // rows below an outer block (16 per wavefront)
+    L_BDIAG = 5,   // outer-block diagonal factor (one workgroup per front)
+    L_BTRSM = 6,   // rows below an outer block (16 per wavefront)
+    L_RECORD = 7,  // no kernel: record the event of stream `stream`
+    L_WAIT = 8,    // no kernel: stream `stream` waits for the other stream's event
 };
 
 // A launch of one kernel over a contiguous range of a task array.
@@ -93,7 +95,8 @@ struct Launch {
     i64 blocks;      // grid size
     double flops;    // algorithmic flops of the launch (multiply-add = 2)
     i32 tile;        // MFMA update tile edge (L_GEMM)
-    i32 outer;       // L_GEMM: 1 for rank-kPanelOuter and Schur updates, 0 for rank-kPanelNB
+    i32 outer;       // L_GEMM: 1 for rank-kOuterRank and Schur updates, 0 for narrower ones
+    i32 stream = 0;  // 0: the handle's stream; 1: the second stream of the bulk updates
 };
 
 // Tunable schedule parameters (chosen by sprgpu_autotune or left at their
@@ -230,6 +233,11 @@ struct sprgpu_handle {
     sprgpu_int device_bytes = 0;
     // symbolic input, kept to re-plan (sprgpu_autotune)
     std::vector<sprgpu_int> mspar, mtrees, msifa;
+    // second stream of the factorization (the rank-kOuterRank updates of the
+    // columns two groups ahead and beyond run there, beside the next group's
+    // diagonal blocks) and one event per stream
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
     // captured factor + solve (sprgpu_factor_solve_graph)
     hipGraphExec_t graph_exec = nullptr;
     hipStream_t capture_stream = nullptr;
