@@ -122,8 +122,14 @@ __device__ __forceinline__ double strided_sum(const double *p, i64 stride, int k
 // ---------------------------------------------------------------------------
 // narrow supernodes: S <= kSolveSmallS, F <= kSolveSmallF
 // ---------------------------------------------------------------------------
-// Dynamic LDS: v[maxF] (FP64), then L11 (S x S, T).
-template <typename T>
+// A workgroup serves R right-hand sides (rhs0 = (block % nrb) * R, ...): it
+// reads each L value once and applies it to all R of them, each with the
+// same sequence of operations as a single right-hand side.  Right-hand
+// sides past nrhs (in the last group) load the last valid one's data and
+// store nothing.
+//
+// Dynamic LDS: v[R][maxF] (FP64), then L11 (S x S, T).
+template <typename T, int R>
 __global__ __launch_bounds__(256) void k_fwd_narrow(const i32 *__restrict__ list, int maxF,
                                                     const NodeInfo *__restrict__ nodes,
                                                     const i32 *__restrict__ lindx,
@@ -134,33 +140,41 @@ __global__ __launch_bounds__(256) void k_fwd_narrow(const i32 *__restrict__ list
                                                     i64 ldb, double *vwork, i64 vstride, int nrhs)
 {
     extern __shared__ double smem[];
-    __shared__ double vx[64];  // x of the S columns, zero past S
-    double *v = smem;
-    T *Lb = reinterpret_cast<T *>(smem + maxF);  // Lb[c*S + r] = L(r,c), r > c
-    const NodeInfo n = nodes[list[blockIdx.x / nrhs]];
-    const int rhs = blockIdx.x % nrhs, tid = threadIdx.x, nt = blockDim.x, lane = tid & 63;
+    __shared__ double vx[R][64];  // x of the S columns, zero past S
+    const int nrb = (nrhs + R - 1) / R;
+    double *v = smem;  // v[r * maxF + i]
+    T *Lb = reinterpret_cast<T *>(smem + R * maxF);  // Lb[c*S + r] = L(r,c), r > c
+    const NodeInfo n = nodes[list[blockIdx.x / nrb]];
+    const int rhs0 = int(blockIdx.x % nrb) * R;
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
     const int S = n.S, F = n.F;
-    double *vw = vwork + rhs * vstride;
-    double *bb = b + rhs * ldb;
     const i32 *rows = lindx + n.lxoff;
     const T *L = pool + n.foff;
     const i64 ld = n.ld;
     const i64 *up = upd_ptr + n.voff;
     for (int i = tid; i < F; i += nt) {
-        double s = (i < S) ? bb[rows[i]] : 0.0;
-        for (i64 k = up[i]; k < up[i + 1]; ++k) s += vw[upd_src[k]];
-        v[i] = s;
+        const i64 k0 = up[i], k1 = up[i + 1];
+        const i32 gi = (i < S) ? rows[i] : 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const i64 rh = min(rhs0 + r, nrhs - 1);
+            const double *vw = vwork + rh * vstride;
+            double s = (i < S) ? b[rh * ldb + gi] : 0.0;
+            for (i64 k = k0; k < k1; ++k) s += vw[upd_src[k]];
+            v[r * maxF + i] = s;
+        }
     }
     for (int idx = tid; idx < S * S; idx += nt) {
         const int r = idx % S, c = idx / S;
         if (r > c) Lb[idx] = L[r + c * ld];
     }
     __syncthreads();
-    if (tid < 64) {
-        // unit lower L11: lane r holds x_r; column c is final when reached
-        // columns are read from LDS eight at a time (in bounds for every
-        // lane) ahead of the dependent readlane/FMA chain
-        double x = (lane < S) ? v[lane] : 0.0;
+    // unit lower L11, one right-hand side per wavefront at a time: lane r
+    // holds x_r; column c is final when reached.  Columns are read from LDS
+    // eight at a time (in bounds for every lane) ahead of the dependent
+    // readlane/FMA chain.
+    for (int q = wave; q < R; q += nw) {
+        double x = (lane < S) ? v[q * maxF + lane] : 0.0;
         const int lc = min(lane, S - 1);
         for (int cb = 0; cb + 1 < S; cb += 8) {
             T la[8];
@@ -175,36 +189,41 @@ __global__ __launch_bounds__(256) void k_fwd_narrow(const i32 *__restrict__ list
                 }
             }
         }
-        if (lane < S) {
-            v[lane] = x;
-            bb[rows[lane]] = x / dvec[n.first + lane];
-        }
+        if (lane < S && rhs0 + q < nrhs) b[i64(rhs0 + q) * ldb + rows[lane]] = x / dvec[n.first + lane];
+        vx[q][lane] = (lane < S) ? x : 0.0;
     }
-    if (tid < 64) vx[tid] = (tid < S) ? v[tid] : 0.0;
     __syncthreads();
-    // update vector for the parent: s_i = v_i - sum_{c<S} L(i,c) x_c, the
+    // update vectors for the parent: s_i = v_i - sum_{c<S} L(i,c) x_c, the
     // columns in batches of 16 (clamped loads, vx is zero past S)
     for (int i = S + tid; i < F; i += nt) {
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        double s0[R], s1[R], s2[R], s3[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) s0[q] = s1[q] = s2[q] = s3[q] = 0.0;
         const T *Li = L + i;
         for (int cb = 0; cb < S; cb += 16) {
             T lv[16];
 #pragma unroll
             for (int u = 0; u < 16; ++u) lv[u] = Li[i64(min(cb + u, S - 1)) * ld];
 #pragma unroll
-            for (int u = 0; u < 16; u += 4) {
-                s0 += double(lv[u]) * vx[cb + u];
-                s1 += double(lv[u + 1]) * vx[cb + u + 1];
-                s2 += double(lv[u + 2]) * vx[cb + u + 2];
-                s3 += double(lv[u + 3]) * vx[cb + u + 3];
-            }
+            for (int q = 0; q < R; ++q)
+#pragma unroll
+                for (int u = 0; u < 16; u += 4) {
+                    s0[q] += double(lv[u]) * vx[q][cb + u];
+                    s1[q] += double(lv[u + 1]) * vx[q][cb + u + 1];
+                    s2[q] += double(lv[u + 2]) * vx[q][cb + u + 2];
+                    s3[q] += double(lv[u + 3]) * vx[q][cb + u + 3];
+                }
         }
-        vw[n.voff + i] = v[i] - ((s0 + s1) + (s2 + s3));
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+            if (rhs0 + q < nrhs)
+                vwork[i64(rhs0 + q) * vstride + n.voff + i] =
+                    v[q * maxF + i] - ((s0[q] + s1[q]) + (s2[q] + s3[q]));
     }
 }
 
-// Dynamic LDS: v[maxF] (FP64), part[4*64] (FP64), then L11 (S x S, T).
-template <typename T>
+// Dynamic LDS: v[R][maxF + 16] (FP64), part[R][4*64] (FP64), then L11 (S x S, T).
+template <typename T, int R>
 __global__ __launch_bounds__(256) void k_bwd_narrow(const i32 *__restrict__ list, int maxF,
                                                     const NodeInfo *__restrict__ nodes,
                                                     const i32 *__restrict__ lindx,
@@ -212,18 +231,26 @@ __global__ __launch_bounds__(256) void k_bwd_narrow(const i32 *__restrict__ list
                                                     i64 ldb, int nrhs)
 {
     extern __shared__ double smem[];
-    double *v = smem;                 // F entries and 16 zeros
-    double *part = smem + maxF + 16;
-    T *Lb = reinterpret_cast<T *>(smem + maxF + 16 + 256);  // Lb[r*S + c] = L(r,c), r > c
-    const NodeInfo n = nodes[list[blockIdx.x / nrhs]];
-    const int rhs = blockIdx.x % nrhs, tid = threadIdx.x, nt = blockDim.x, lane = tid & 63;
-    const int wave = tid >> 6, nw = nt >> 6;
+    const int VS = maxF + 16;
+    double *v = smem;                      // v[r * VS + i]: F entries and 16 zeros
+    double *part = smem + R * VS;          // part[r * 256 + wave * 64 + lane]
+    T *Lb = reinterpret_cast<T *>(part + R * 256);  // Lb[r*S + c] = L(r,c), r > c
+    const int nrb = (nrhs + R - 1) / R;
+    const NodeInfo n = nodes[list[blockIdx.x / nrb]];
+    const int rhs0 = int(blockIdx.x % nrb) * R;
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
     const int S = n.S, F = n.F;
-    double *bb = b + rhs * ldb;
     const i32 *rows = lindx + n.lxoff;
     const T *L = pool + n.foff;
     const i64 ld = n.ld;
-    for (int i = tid; i < F + 16; i += nt) v[i] = (i < F) ? bb[rows[i]] : 0.0;
+    for (int i = tid; i < F + 16; i += nt) {
+        const i32 gi = (i < F) ? rows[i] : 0;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const i64 rh = min(rhs0 + q, nrhs - 1);
+            v[q * VS + i] = (i < F) ? b[rh * ldb + gi] : 0.0;
+        }
+    }
     for (int idx = tid; idx < S * S; idx += nt) {
         const int r = idx % S, c = idx / S;
         if (r > c) Lb[r * S + c] = L[r + c * ld];
@@ -231,37 +258,41 @@ __global__ __launch_bounds__(256) void k_bwd_narrow(const i32 *__restrict__ list
     __syncthreads();
     // x1 -= L21^T x2: lane = column, the waves split the rows in multiples
     // of 16 that are read 16 at a time (clamped loads, v is zero past F)
-    double acc = 0.0;
     {
         const T *Lcol = L + i64(min(lane, S - 1)) * ld;
         const int per = ((F - S + nw - 1) / nw + 15) / 16 * 16;
         const int ra = S + wave * per, rz = min(F, ra + per);
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        double a0[R], a1[R], a2[R], a3[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) a0[q] = a1[q] = a2[q] = a3[q] = 0.0;
         for (int r = ra; r < rz; r += 16) {
             T lv[16];
 #pragma unroll
             for (int u = 0; u < 16; ++u) lv[u] = Lcol[min(r + u, F - 1)];
 #pragma unroll
-            for (int u = 0; u < 16; u += 4) {
-                a0 += double(lv[u]) * v[r + u];
-                a1 += double(lv[u + 1]) * v[r + u + 1];
-                a2 += double(lv[u + 2]) * v[r + u + 2];
-                a3 += double(lv[u + 3]) * v[r + u + 3];
-            }
+            for (int q = 0; q < R; ++q)
+#pragma unroll
+                for (int u = 0; u < 16; u += 4) {
+                    a0[q] += double(lv[u]) * v[q * VS + r + u];
+                    a1[q] += double(lv[u + 1]) * v[q * VS + r + u + 1];
+                    a2[q] += double(lv[u + 2]) * v[q * VS + r + u + 2];
+                    a3[q] += double(lv[u + 3]) * v[q * VS + r + u + 3];
+                }
         }
-        acc = (a0 + a1) + (a2 + a3);
+#pragma unroll
+        for (int q = 0; q < R; ++q) part[q * 256 + wave * 64 + lane] = (a0[q] + a1[q]) + (a2[q] + a3[q]);
     }
-    part[wave * 64 + lane] = acc;
     __syncthreads();
-    if (tid < 64) {
-        // L11^T x1 = y1 by columns from the last: x_c -= l_rc x_r for c < r
+    // L11^T x1 = y1 by columns from the last, one right-hand side per
+    // wavefront at a time: x_c -= l_rc x_r for c < r
+    const int lc = min(lane, S - 1);
+    for (int q = wave; q < R; q += nw) {
         double x = 0.0;
         if (lane < S) {
-            double sp = part[lane];
-            for (int k = 1; k < nw; ++k) sp += part[k * 64 + lane];
-            x = v[lane] - sp;
+            double sp = part[q * 256 + lane];
+            for (int k = 1; k < nw; ++k) sp += part[q * 256 + k * 64 + lane];
+            x = v[q * VS + lane] - sp;
         }
-        const int lc = min(lane, S - 1);
         for (int rb = S - 1; rb > 0; rb -= 8) {
             T la[8];
 #pragma unroll
@@ -275,7 +306,7 @@ __global__ __launch_bounds__(256) void k_bwd_narrow(const i32 *__restrict__ list
                 }
             }
         }
-        if (lane < S) bb[rows[lane]] = x;
+        if (lane < S && rhs0 + q < nrhs) b[i64(rhs0 + q) * ldb + rows[lane]] = x;
     }
 }
 
@@ -284,8 +315,10 @@ __global__ __launch_bounds__(256) void k_bwd_narrow(const i32 *__restrict__ list
 // ---------------------------------------------------------------------------
 // Workgroup local < nblk solves diagonal block local after applying blocks
 // 0..local-1; workgroups nblk.. compute the update vector of kSolveNB rows
-// below the diagonal part, applying every block as it finishes.
-template <typename T>
+// below the diagonal part, applying every block as it finishes.  As in the
+// narrow kernels, a workgroup serves R right-hand sides; its ready flags are
+// those of its group of right-hand sides.
+template <typename T, int R>
 __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ wide, int nwide,
                                                   const i64 *__restrict__ prefix,
                                                   const NodeInfo *__restrict__ nodes,
@@ -298,26 +331,28 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
                                                   unsigned *flags, i64 nflags,
                                                   const unsigned *epoch_ptr, int *err, int nrhs)
 {
-    __shared__ double xs[NB];
-    __shared__ double ys[NB];
-    __shared__ double part[2][NB];
+    __shared__ double xs[R][NB];
+    __shared__ double ys[R][NB];
+    __shared__ double part[2][R][NB];
     __shared__ T Ld[kPacked + 1];
     const unsigned epoch = *epoch_ptr;  // packed lower diagonal block, by columns; Ld[kZero] = 0
-    const i64 bid = blockIdx.x / nrhs;
+    const int nrb = (nrhs + R - 1) / R;
+    const i64 bid = blockIdx.x / nrb;
+    const int rg = int(blockIdx.x % nrb), rhs0 = rg * R;
     const int q = find_desc(prefix, nwide, bid);
     const WideDesc w = wide[q];
     const int local = int(bid - prefix[q]);
     const NodeInfo n = nodes[w.node];
-    const int rhs = blockIdx.x % nrhs, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int S = n.S, F = n.F;
-    double *vw = vwork + rhs * vstride;
-    double *v = vw + n.voff;
-    double *bb = b + rhs * ldb;
     const i32 *rows = lindx + n.lxoff;
     const T *L = pool + n.foff;
     const i64 ld = n.ld;
     const i64 *up = upd_ptr + n.voff;
-    unsigned *fl = flags + rhs * nflags + w.flag0;
+    unsigned *fl = flags + rg * nflags + w.flag0;
+    i64 vo[R];  // this group's vectors of the supernode (the last valid one past nrhs)
+#pragma unroll
+    for (int k = 0; k < R; ++k) vo[k] = i64(min(rhs0 + k, nrhs - 1)) * vstride + n.voff;
     const bool diag = local < w.nblk;
     const int r0 = diag ? local * NB : S + (local - w.nblk) * NB;
     const int nr = min(NB, (diag ? S : F) - r0);
@@ -326,12 +361,19 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
     const bool rv = ri < nr;
     // right-hand side of my rows: B (own columns) plus the children's updates
     if (h == 0) {
-        double y = 0.0;
-        if (rv) {
-            y = (row < S) ? bb[rows[row]] : 0.0;
-            for (i64 k = up[row]; k < up[row + 1]; ++k) y += vw[upd_src[k]];
+        const i64 k0 = rv ? up[row] : 0, k1 = rv ? up[row + 1] : 0;
+        const i32 gi = (rv && row < S) ? rows[row] : 0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const i64 rh = min(rhs0 + k, nrhs - 1);
+            double y = 0.0;
+            if (rv) {
+                y = (row < S) ? b[rh * ldb + gi] : 0.0;
+                const double *vw = vwork + rh * vstride;
+                for (i64 j = k0; j < k1; ++j) y += vw[upd_src[j]];
+            }
+            ys[k][ri] = y;
         }
-        ys[ri] = y;
     }
     // destinations and pivots of the two block rows wave 0 writes at the end
     i32 dst[2] = {0, 0};
@@ -352,7 +394,9 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
         if (tid == 0) Ld[kZero] = T(0);
     }
     const int nprev = diag ? local : w.nblk;
-    double acc = 0.0;
+    double acc[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) acc[k] = 0.0;
     for (int p = 0; p < nprev; ++p) {
         const int c0 = p * NB;
         const int nc = min(NB, S - c0);
@@ -365,23 +409,37 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
 #pragma unroll
         for (int c = 0; c < NH; ++c) lv[c] = Lr[i64(min(h * NH + c, nc - 1)) * ld];
         wait_ready(fl + p, epoch, err);
-        if (tid < NB) xs[tid] = (tid < nc) ? load_shared(v + c0 + tid) : 0.0;
+        if (tid < NB) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) xs[k][tid] = (tid < nc) ? load_shared(vwork + vo[k] + c0 + tid) : 0.0;
+        }
         __syncthreads();
 #pragma unroll
-        for (int c = 0; c < NH; ++c) acc += double(lv[c]) * xs[h * NH + c];
+        for (int c = 0; c < NH; ++c)
+#pragma unroll
+            for (int k = 0; k < R; ++k) acc[k] += double(lv[c]) * xs[k][h * NH + c];
         __syncthreads();
     }
-    part[h][ri] = acc;
+#pragma unroll
+    for (int k = 0; k < R; ++k) part[h][k][ri] = acc[k];
     __syncthreads();
     if (!diag) {
-        if (h == 0 && rv) v[row] = ys[ri] - (part[0][ri] + part[1][ri]);  // update for the parent
+        if (h == 0 && rv) {  // update for the parent
+#pragma unroll
+            for (int k = 0; k < R; ++k)
+                if (rhs0 + k < nrhs) vwork[vo[k] + row] = ys[k][ri] - (part[0][k][ri] + part[1][k][ri]);
+        }
         return;
     }
     // Diagonal block: four 32 x 32 sub-blocks in sequence.  Wave w keeps row
     // 32w + lane (lane < 32) of its unit-lower sub-block in registers and
     // solves it with a readlane chain; the waves below then apply it to their
     // rows from the packed block in LDS.
-    if (tid < NB) ys[tid] = (tid < nr) ? ys[tid] - (part[0][tid] + part[1][tid]) : 0.0;
+    if (tid < NB) {
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            ys[k][tid] = (tid < nr) ? ys[k][tid] - (part[0][k][tid] + part[1][k][tid]) : 0.0;
+    }
     // Lanes 32..63 mirror lanes 0..31 (lane & 31) and do no work.
     const int sr = 32 * wave + (lane & 31);  // block row of this lane (lanes < 32 work)
     const bool sw = lane < 32 && sr < nr;
@@ -394,35 +452,49 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
     __syncthreads();
     for (int j = 0; j < 4 && 32 * j < nr; ++j) {
         if (wave == j) {
-            double x = sw ? ys[sr] : 0.0;
 #pragma unroll
-            for (int c = 0; c < 32; ++c) x -= double(lsub[c]) * lane_bcast(x, c);
-            if (lane < 32) ys[sr] = x;
+            for (int k = 0; k < R; ++k) {
+                double x = sw ? ys[k][sr] : 0.0;
+#pragma unroll
+                for (int c = 0; c < 32; ++c) x -= double(lsub[c]) * lane_bcast(x, c);
+                if (lane < 32) ys[k][sr] = x;
+            }
         }
         __syncthreads();
         if (wave > j && sw) {
-            double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-            for (int c = 0; c < 32; c += 2) {
-                const int cc = 32 * j + c;
-                const T a0 = Ld[cc < nr ? packed_col(cc) + sr - cc : kZero];
-                const T a1 = Ld[cc + 1 < nr ? packed_col(cc + 1) + sr - cc - 1 : kZero];
-                s0 += double(a0) * ys[cc];
-                s1 += double(a1) * ys[cc + 1];
+            for (int k = 0; k < R; ++k) {
+                double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+                for (int c = 0; c < 32; c += 2) {
+                    const int cc = 32 * j + c;
+                    const T a0 = Ld[cc < nr ? packed_col(cc) + sr - cc : kZero];
+                    const T a1 = Ld[cc + 1 < nr ? packed_col(cc + 1) + sr - cc - 1 : kZero];
+                    s0 += double(a0) * ys[k][cc];
+                    s1 += double(a1) * ys[k][cc + 1];
+                }
+                ys[k][sr] -= s0 + s1;
             }
-            ys[sr] -= s0 + s1;
         }
         __syncthreads();
     }
     if (wave == 0) {
         // results for the blocks that wait on this one first, then B
 #pragma unroll
-        for (int k = 0; k < 2; ++k)
-            if (lane + 64 * k < nr) store_shared(v + r0 + lane + 64 * k, ys[lane + 64 * k]);
+        for (int i = 0; i < 2; ++i)
+            if (lane + 64 * i < nr) {
+#pragma unroll
+                for (int k = 0; k < R; ++k)
+                    if (rhs0 + k < nrhs) store_shared(vwork + vo[k] + r0 + lane + 64 * i, ys[k][lane + 64 * i]);
+            }
         publish_ready(fl + local, epoch);
 #pragma unroll
-        for (int k = 0; k < 2; ++k)
-            if (lane + 64 * k < nr) bb[dst[k]] = ys[lane + 64 * k] / dpiv[k];
+        for (int i = 0; i < 2; ++i)
+            if (lane + 64 * i < nr) {
+#pragma unroll
+                for (int k = 0; k < R; ++k)
+                    if (rhs0 + k < nrhs) b[i64(rhs0 + k) * ldb + dst[i]] = ys[k][lane + 64 * i] / dpiv[i];
+            }
     }
 }
 
@@ -432,7 +504,7 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
 // Partial sums of L21^T x2: workgroup (blk, k) multiplies rows [S+k*NB, +NB)
 // of the columns of block blk by the solution at those rows (final: they
 // belong to ancestors).  Two threads per column, each walking half the rows.
-template <typename T>
+template <typename T, int R>
 __global__ __launch_bounds__(256) void k_bwd_rect(const WideDesc *__restrict__ wide, int nwide,
                                                   const i64 *__restrict__ prefix,
                                                   const NodeInfo *__restrict__ nodes,
@@ -441,21 +513,28 @@ __global__ __launch_bounds__(256) void k_bwd_rect(const WideDesc *__restrict__ w
                                                   const double *__restrict__ b, i64 ldb,
                                                   double *partials, i64 npart, int nrhs)
 {
-    __shared__ double xs[NB];
-    __shared__ double part[2][NB];
-    const i64 bid = blockIdx.x / nrhs;
+    __shared__ double xs[R][NB];
+    __shared__ double part[2][R][NB];
+    const int nrb = (nrhs + R - 1) / R;
+    const i64 bid = blockIdx.x / nrb;
+    const int rhs0 = int(blockIdx.x % nrb) * R;
     const int q = find_desc(prefix, nwide, bid);
     const WideDesc w = wide[q];
     const int t = int(bid - prefix[q]);
     const int blk = t / w.nrect, k = t % w.nrect;
     const NodeInfo n = nodes[w.node];
-    const int rhs = blockIdx.x % nrhs, tid = threadIdx.x;
+    const int tid = threadIdx.x;
     const int S = n.S, F = n.F;
     const int c0 = blk * NB, nc = min(NB, S - c0);
     const int r0 = S + k * NB, nr = min(NB, F - r0);
     const T *L = pool + n.foff;
     const i64 ld = n.ld;
-    if (tid < NB) xs[tid] = (tid < nr) ? b[rhs * ldb + lindx[n.lxoff + r0 + tid]] : 0.0;
+    if (tid < NB) {
+        const i32 gi = lindx[n.lxoff + r0 + min(tid, nr - 1)];
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+            xs[j][tid] = (tid < nr) ? b[i64(min(rhs0 + j, nrhs - 1)) * ldb + gi] : 0.0;
+    }
     __syncthreads();
     const int ci = tid & (NB - 1), h = tid >> 7;
     // unconditional loads (columns past the block read its last column and
@@ -464,25 +543,32 @@ __global__ __launch_bounds__(256) void k_bwd_rect(const WideDesc *__restrict__ w
     const T *Lc = L + r0 + i64(c0 + min(ci, nc - 1)) * ld;
 #pragma unroll
     for (int r = 0; r < NH; ++r) lv[r] = Lc[min(h * NH + r, nr - 1)];
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
 #pragma unroll
-    for (int r = 0; r < NH; r += 4) {
-        a0 += double(lv[r]) * xs[h * NH + r];
-        a1 += double(lv[r + 1]) * xs[h * NH + r + 1];
-        a2 += double(lv[r + 2]) * xs[h * NH + r + 2];
-        a3 += double(lv[r + 3]) * xs[h * NH + r + 3];
+    for (int j = 0; j < R; ++j) {
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+        for (int r = 0; r < NH; r += 4) {
+            a0 += double(lv[r]) * xs[j][h * NH + r];
+            a1 += double(lv[r + 1]) * xs[j][h * NH + r + 1];
+            a2 += double(lv[r + 2]) * xs[j][h * NH + r + 2];
+            a3 += double(lv[r + 3]) * xs[j][h * NH + r + 3];
+        }
+        part[h][j][ci] = (a0 + a1) + (a2 + a3);
     }
-    part[h][ci] = (a0 + a1) + (a2 + a3);
     __syncthreads();
-    if (tid < nc)
-        partials[rhs * npart + w.pbase + (i64(blk) * w.nrect + k) * NB + tid] =
-            part[0][tid] + part[1][tid];
+    if (tid < nc) {
+#pragma unroll
+        for (int j = 0; j < R; ++j)
+            if (rhs0 + j < nrhs)
+                partials[i64(rhs0 + j) * npart + w.pbase + (i64(blk) * w.nrect + k) * NB + tid] =
+                    part[0][j][tid] + part[1][j][tid];
+    }
 }
 
 // Workgroup local handles block nblk-1-local: y = B - partial sums of the
 // rows below, minus the later blocks' products as they finish, then
 // L11^T x = y on the block by wave 0 (two columns per lane).
-template <typename T>
+template <typename T, int R>
 __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ wide, int nwide,
                                                   const i64 *__restrict__ prefix,
                                                   const NodeInfo *__restrict__ nodes,
@@ -493,24 +579,27 @@ __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ w
                                                   unsigned *flags, i64 nflags,
                                                   const unsigned *epoch_ptr, int *err, int nrhs)
 {
-    __shared__ double xs[NB];
-    __shared__ double ys[NB];
-    __shared__ double part[2][NB];
+    __shared__ double xs[R][NB];
+    __shared__ double ys[R][NB];
+    __shared__ double part[2][R][NB];
     __shared__ T Lr[kPacked + 1];
     const unsigned epoch = *epoch_ptr;  // packed lower diagonal block, by rows; Lr[kZero] = 0
-    const i64 bid = blockIdx.x / nrhs;
+    const int nrb = (nrhs + R - 1) / R;
+    const i64 bid = blockIdx.x / nrb;
+    const int rg = int(blockIdx.x % nrb), rhs0 = rg * R;
     const int q = find_desc(prefix, nwide, bid);
     const WideDesc w = wide[q];
     const int blk = w.nblk - 1 - int(bid - prefix[q]);
     const NodeInfo n = nodes[w.node];
-    const int rhs = blockIdx.x % nrhs, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int S = n.S;
-    double *v = vwork + rhs * vstride + n.voff;
-    double *bb = b + rhs * ldb;
     const i32 *rows = lindx + n.lxoff;
     const T *L = pool + n.foff;
     const i64 ld = n.ld;
-    unsigned *fl = flags + rhs * nflags + w.flag0;
+    unsigned *fl = flags + rg * nflags + w.flag0;
+    i64 vo[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) vo[k] = i64(min(rhs0 + k, nrhs - 1)) * vstride + n.voff;
     const int c0 = blk * NB, nc = min(NB, S - c0);
     for (int idx = tid; idx < nc * nc; idx += 256) {
         const int rr = idx % nc, cc = idx / nc;
@@ -525,15 +614,25 @@ __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ w
     const int ci = tid & (NB - 1), h = tid >> 7;
     // products with the rows below the diagonal part: the two halves of the
     // workgroup sum the even and the odd row chunks' partials
-    {
-        const double *ps =
-            partials + rhs * npart + w.pbase + i64(blk) * w.nrect * NB + min(ci, nc - 1);
-        part[h][ci] = strided_sum(ps, NB, h, 2, w.nrect);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const double *ps = partials + i64(min(rhs0 + k, nrhs - 1)) * npart + w.pbase +
+                           i64(blk) * w.nrect * NB + min(ci, nc - 1);
+        part[h][k][ci] = strided_sum(ps, NB, h, 2, w.nrect);
     }
     __syncthreads();
-    if (tid < NB) ys[tid] = (tid < nc) ? bb[rows[c0 + tid]] - (part[0][tid] + part[1][tid]) : 0.0;
+    if (tid < NB) {
+        const i32 gi = rows[c0 + min(tid, nc - 1)];
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            ys[k][tid] = (tid < nc)
+                             ? b[i64(min(rhs0 + k, nrhs - 1)) * ldb + gi] - (part[0][k][tid] + part[1][k][tid])
+                             : 0.0;
+    }
     __syncthreads();
-    double acc = 0.0;
+    double acc[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) acc[k] = 0.0;
     for (int p = w.nblk - 1; p > blk; --p) {
         const int rp0 = p * NB, np = min(NB, S - rp0);
         // unconditional, unmasked loads (see k_fwd_wide): columns past the
@@ -544,19 +643,29 @@ __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ w
 #pragma unroll
         for (int r = 0; r < NH; ++r) lv[r] = Lc[min(h * NH + r, np - 1)];
         wait_ready(fl + p, epoch, err);
-        if (tid < NB) xs[tid] = (tid < np) ? load_shared(v + rp0 + tid) : 0.0;
+        if (tid < NB) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) xs[k][tid] = (tid < np) ? load_shared(vwork + vo[k] + rp0 + tid) : 0.0;
+        }
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < NH; ++r) acc += double(lv[r]) * xs[h * NH + r];
+        for (int r = 0; r < NH; ++r)
+#pragma unroll
+            for (int k = 0; k < R; ++k) acc[k] += double(lv[r]) * xs[k][h * NH + r];
         __syncthreads();
     }
-    part[h][ci] = acc;
+#pragma unroll
+    for (int k = 0; k < R; ++k) part[h][k][ci] = acc[k];
     __syncthreads();
     // Diagonal block, L11^T x = y: four 32 x 32 sub-blocks from the last.
     // Wave w keeps column 32w + lane (lane < 32) of its sub-block in
     // registers and solves it with a readlane chain; the waves to the left
     // then apply it to their columns from the packed block in LDS.
-    if (tid < NB) ys[tid] = (tid < nc) ? ys[tid] - (part[0][tid] + part[1][tid]) : 0.0;
+    if (tid < NB) {
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            ys[k][tid] = (tid < nc) ? ys[k][tid] - (part[0][k][tid] + part[1][k][tid]) : 0.0;
+    }
     // Lanes 32..63 mirror lanes 0..31 (lane & 31) and do no work.
     const int sc = 32 * wave + (lane & 31);  // block column of this lane (lanes < 32 work)
     const bool sw = lane < 32 && sc < nc;
@@ -570,34 +679,48 @@ __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ w
     for (int j = 3; j >= 0; --j) {
         if (32 * j >= nc) continue;
         if (wave == j) {
-            double x = sw ? ys[sc] : 0.0;
 #pragma unroll
-            for (int r = 31; r > 0; --r) x -= double(lcol[r]) * lane_bcast(x, r);
-            if (lane < 32) ys[sc] = x;
+            for (int k = 0; k < R; ++k) {
+                double x = sw ? ys[k][sc] : 0.0;
+#pragma unroll
+                for (int r = 31; r > 0; --r) x -= double(lcol[r]) * lane_bcast(x, r);
+                if (lane < 32) ys[k][sc] = x;
+            }
         }
         __syncthreads();
         if (wave < j && sw) {
-            double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-            for (int r = 0; r < 32; r += 2) {
-                const int rr = 32 * j + r;
-                const T a0 = Lr[rr < nc ? rr * (rr + 1) / 2 + sc : kZero];
-                const T a1 = Lr[rr + 1 < nc ? (rr + 1) * (rr + 2) / 2 + sc : kZero];
-                s0 += double(a0) * ys[rr];
-                s1 += double(a1) * ys[rr + 1];
+            for (int k = 0; k < R; ++k) {
+                double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+                for (int r = 0; r < 32; r += 2) {
+                    const int rr = 32 * j + r;
+                    const T a0 = Lr[rr < nc ? rr * (rr + 1) / 2 + sc : kZero];
+                    const T a1 = Lr[rr + 1 < nc ? (rr + 1) * (rr + 2) / 2 + sc : kZero];
+                    s0 += double(a0) * ys[k][rr];
+                    s1 += double(a1) * ys[k][rr + 1];
+                }
+                ys[k][sc] -= s0 + s1;
             }
-            ys[sc] -= s0 + s1;
         }
         __syncthreads();
     }
     if (wave == 0) {
 #pragma unroll
-        for (int k = 0; k < 2; ++k)
-            if (lane + 64 * k < nc) store_shared(v + c0 + lane + 64 * k, ys[lane + 64 * k]);
+        for (int i = 0; i < 2; ++i)
+            if (lane + 64 * i < nc) {
+#pragma unroll
+                for (int k = 0; k < R; ++k)
+                    if (rhs0 + k < nrhs) store_shared(vwork + vo[k] + c0 + lane + 64 * i, ys[k][lane + 64 * i]);
+            }
         publish_ready(fl + blk, epoch);
 #pragma unroll
-        for (int k = 0; k < 2; ++k)
-            if (lane + 64 * k < nc) bb[dst[k]] = ys[lane + 64 * k];
+        for (int i = 0; i < 2; ++i)
+            if (lane + 64 * i < nc) {
+#pragma unroll
+                for (int k = 0; k < R; ++k)
+                    if (rhs0 + k < nrhs) b[i64(rhs0 + k) * ldb + dst[i]] = ys[k][lane + 64 * i];
+            }
     }
 }
 
@@ -640,6 +763,75 @@ static int ensure_solve_buffers(sprgpu_handle *h, i64 nrhs)
     return 0;
 }
 
+// Right-hand sides per workgroup: kSolveGroup when a call has at least that
+// many (the wide kernels always, the narrow ones while their LDS stays within
+// 64 KB, else 4 or 1), one otherwise.
+constexpr int kSolveGroup = 8;
+constexpr size_t kNarrowLds = 64 * 1024;
+
+template <typename T>
+static size_t fwd_narrow_lds(int R, int mf, int ms)
+{
+    return sizeof(double) * size_t(R) * mf + sizeof(T) * size_t(ms) * ms;
+}
+template <typename T>
+static size_t bwd_narrow_lds(int R, int mf, int ms)
+{
+    return sizeof(double) * size_t(R) * (mf + 16 + 256) + sizeof(T) * size_t(ms) * ms;
+}
+
+template <typename T, int R>
+static void fwd_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs, bool narrow)
+{
+    const Plan &p = h->plan;
+    DeviceBuffers &d = h->d;
+    const T *pool = static_cast<const T *>(d.pool);
+    const unsigned groups = unsigned((nrhs + R - 1) / R);
+    if (narrow) {
+        const unsigned ns = unsigned(p.solve_small_ptr[l + 1] - p.solve_small_ptr[l]);
+        const int mf = p.small_maxF[l], ms = p.small_maxS[l];
+        const unsigned nt = p.params.narrow_threads ? unsigned(p.params.narrow_threads)
+                                                    : (mf <= 128 ? 64u : 256u);
+        k_fwd_narrow<T, R><<<ns * groups, nt, fwd_narrow_lds<T>(R, mf, ms), h->stream>>>(
+            d.solve_small + p.solve_small_ptr[l], mf, d.nodes, d.lindx, d.upd_ptr, d.upd_src, pool,
+            d.dvec, b, ldb, d.vwork, p.vwork_elems, int(nrhs));
+    } else {
+        const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
+        k_fwd_wide<T, R><<<unsigned(p.wide_fwd_wgs[l]) * groups, 256, 0, h->stream>>>(
+            d.wide + w0, nw, d.wide_fwd_prefix + w0, d.nodes, d.lindx, d.upd_ptr, d.upd_src, pool,
+            d.dvec, b, ldb, d.vwork, p.vwork_elems, d.flags, std::max<i64>(p.nflags, 1), d.epoch,
+            d.spin_error, int(nrhs));
+    }
+}
+
+template <typename T, int R>
+static void bwd_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs, bool narrow)
+{
+    const Plan &p = h->plan;
+    DeviceBuffers &d = h->d;
+    const T *pool = static_cast<const T *>(d.pool);
+    const unsigned groups = unsigned((nrhs + R - 1) / R);
+    const i64 npart = std::max<i64>(p.npart, 1);
+    if (narrow) {
+        const unsigned ns = unsigned(p.solve_small_ptr[l + 1] - p.solve_small_ptr[l]);
+        const int mf = p.small_maxF[l], ms = p.small_maxS[l];
+        const unsigned nt = p.params.narrow_threads ? unsigned(p.params.narrow_threads)
+                                                    : (mf <= 128 ? 64u : 256u);
+        k_bwd_narrow<T, R><<<ns * groups, nt, bwd_narrow_lds<T>(R, mf, ms), h->stream>>>(
+            d.solve_small + p.solve_small_ptr[l], mf, d.nodes, d.lindx, pool, b, ldb, int(nrhs));
+    } else {
+        const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
+        if (p.wide_rect_wgs[l] > 0)
+            k_bwd_rect<T, R><<<unsigned(p.wide_rect_wgs[l]) * groups, 256, 0, h->stream>>>(
+                d.wide + w0, nw, d.wide_rect_prefix + w0, d.nodes, d.lindx, pool, b, ldb,
+                d.partials, npart, int(nrhs));
+        k_bwd_wide<T, R><<<unsigned(p.wide_bwd_wgs[l]) * groups, 256, 0, h->stream>>>(
+            d.wide + w0, nw, d.wide_bwd_prefix + w0, d.nodes, d.lindx, pool, b, ldb, d.vwork,
+            p.vwork_elems, d.partials, npart, d.flags, std::max<i64>(p.nflags, 1), d.epoch,
+            d.spin_error, int(nrhs));
+    }
+}
+
 template <typename T>
 static int solve_impl(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which)
 {
@@ -647,55 +839,44 @@ static int solve_impl(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which)
     DeviceBuffers &d = h->d;
     hipStream_t s = h->stream;
     if (int rc = ensure_solve_buffers(h, nrhs)) return rc;
-    const T *pool = static_cast<const T *>(d.pool);
-    const i64 vstride = p.vwork_elems;
-    const unsigned ur = unsigned(nrhs);
-    const i64 nflags = std::max<i64>(p.nflags, 1);
-    const i64 npart = std::max<i64>(p.npart, 1);
-    auto narrow_threads = [&](int l) {
-        if (p.params.narrow_threads) return unsigned(p.params.narrow_threads);
-        return p.small_maxF[l] <= 128 ? 64u : 256u;
+    const bool grouped = nrhs >= kSolveGroup;
+    // group size of a level's narrow launch (fwd: dir 0, bwd: dir 1)
+    auto narrow_group = [&](int l, int dir) {
+        if (!grouped) return 1;
+        const int mf = p.small_maxF[l], ms = p.small_maxS[l];
+        for (int R : {kSolveGroup, 4}) {
+            const size_t need = dir ? bwd_narrow_lds<T>(R, mf, ms) : fwd_narrow_lds<T>(R, mf, ms);
+            if (need <= kNarrowLds) return R;
+        }
+        return 1;
     };
     if (which & 1) {
         k_epoch_bump<<<1, 64, 0, s>>>(d.epoch);
         for (int l = 0; l < p.nlevels; ++l) {
-            const unsigned ns = unsigned(p.solve_small_ptr[l + 1] - p.solve_small_ptr[l]);
-            if (ns) {
-                const int mf = p.small_maxF[l], ms = p.small_maxS[l];
-                const size_t shm = sizeof(double) * mf + sizeof(T) * ms * ms;
-                k_fwd_narrow<T><<<ns * ur, narrow_threads(l), shm, s>>>(
-                    d.solve_small + p.solve_small_ptr[l], mf, d.nodes, d.lindx, d.upd_ptr,
-                    d.upd_src, pool, d.dvec, b, ldb, d.vwork, vstride, int(nrhs));
+            if (p.solve_small_ptr[l + 1] > p.solve_small_ptr[l]) {
+                const int R = narrow_group(l, 0);
+                if (R == kSolveGroup) fwd_level<T, kSolveGroup>(h, l, b, ldb, nrhs, true);
+                else if (R == 4) fwd_level<T, 4>(h, l, b, ldb, nrhs, true);
+                else fwd_level<T, 1>(h, l, b, ldb, nrhs, true);
             }
-            const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
-            if (nw)
-                k_fwd_wide<T><<<unsigned(p.wide_fwd_wgs[l]) * ur, 256, 0, s>>>(
-                    d.wide + w0, nw, d.wide_fwd_prefix + w0, d.nodes, d.lindx, d.upd_ptr,
-                    d.upd_src, pool, d.dvec, b, ldb, d.vwork, vstride, d.flags, nflags, d.epoch,
-                    d.spin_error, int(nrhs));
+            if (p.wide_ptr[l + 1] > p.wide_ptr[l]) {
+                if (grouped) fwd_level<T, kSolveGroup>(h, l, b, ldb, nrhs, false);
+                else fwd_level<T, 1>(h, l, b, ldb, nrhs, false);
+            }
         }
     }
     if (which & 2) {
         k_epoch_bump<<<1, 64, 0, s>>>(d.epoch);
         for (int l = p.nlevels - 1; l >= 0; --l) {
-            const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
-            if (nw) {
-                if (p.wide_rect_wgs[l] > 0)
-                    k_bwd_rect<T><<<unsigned(p.wide_rect_wgs[l]) * ur, 256, 0, s>>>(
-                        d.wide + w0, nw, d.wide_rect_prefix + w0, d.nodes, d.lindx, pool, b, ldb,
-                        d.partials, npart, int(nrhs));
-                k_bwd_wide<T><<<unsigned(p.wide_bwd_wgs[l]) * ur, 256, 0, s>>>(
-                    d.wide + w0, nw, d.wide_bwd_prefix + w0, d.nodes, d.lindx, pool, b, ldb,
-                    d.vwork, vstride, d.partials, npart, d.flags, nflags, d.epoch, d.spin_error,
-                    int(nrhs));
+            if (p.wide_ptr[l + 1] > p.wide_ptr[l]) {
+                if (grouped) bwd_level<T, kSolveGroup>(h, l, b, ldb, nrhs, false);
+                else bwd_level<T, 1>(h, l, b, ldb, nrhs, false);
             }
-            const unsigned ns = unsigned(p.solve_small_ptr[l + 1] - p.solve_small_ptr[l]);
-            if (ns) {
-                const int mf = p.small_maxF[l], ms = p.small_maxS[l];
-                const size_t shm = sizeof(double) * (mf + 16 + 256) + sizeof(T) * ms * ms;
-                k_bwd_narrow<T><<<ns * ur, narrow_threads(l), shm, s>>>(
-                    d.solve_small + p.solve_small_ptr[l], mf, d.nodes, d.lindx, pool, b, ldb,
-                    int(nrhs));
+            if (p.solve_small_ptr[l + 1] > p.solve_small_ptr[l]) {
+                const int R = narrow_group(l, 1);
+                if (R == kSolveGroup) bwd_level<T, kSolveGroup>(h, l, b, ldb, nrhs, true);
+                else if (R == 4) bwd_level<T, 4>(h, l, b, ldb, nrhs, true);
+                else bwd_level<T, 1>(h, l, b, ldb, nrhs, true);
             }
         }
     }

@@ -108,6 +108,26 @@ def test_many_rhs_match_single():
     h.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_grouped_rhs_match_single_with_wide_supernodes(precision):
+    """11 right-hand sides (a group of 8 and a partial group of 3 in every
+    solve kernel) on hex8 12^3, whose top supernodes take the wide dataflow
+    path: every column equals its own single-RHS solve bit for bit."""
+    torch = _torch()
+    model, sysm, sm = make_system("hex8", 12, e_sigma=0.5, e_seed=1)
+    assert np.diff(sysm.xsuper()).max() > 128
+    h, sm_dev = _handle(sysm, sm, precision)
+    assert h.factor(sm_dev)["ierr"] == 0
+    B = torch.randn(11, sysm.neq, dtype=torch.float64, device="cuda")
+    X = B.clone()
+    h.solve(X)
+    for k in range(11):
+        xk = B[k:k + 1].clone()
+        h.solve(xk)
+        assert torch.equal(xk[0], X[k]), k
+    h.close()
+
+
 def _golden(name):
     with np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False) as z:
         return {k: z[k] for k in z.files}
