@@ -155,7 +155,7 @@ __global__ __launch_bounds__(256) void k_fwd_narrow(const i32 *__restrict__ list
     for (int i = tid; i < F; i += nt) {
         const i64 k0 = up[i], k1 = up[i + 1];
         const i32 gi = (i < S) ? rows[i] : 0;
-#pragma unroll
+#pragma unroll 4
         for (int r = 0; r < R; ++r) {
             const i64 rh = min(rhs0 + r, nrhs - 1);
             const double *vw = vwork + rh * vstride;
@@ -195,34 +195,37 @@ __global__ __launch_bounds__(256) void k_fwd_narrow(const i32 *__restrict__ list
     __syncthreads();
     // update vectors for the parent: s_i = v_i - sum_{c<S} L(i,c) x_c, the
     // columns in batches of 16 (clamped loads, vx is zero past S)
+    constexpr int RS = R < 8 ? R : 8;  // right-hand sides accumulated at a time
     for (int i = S + tid; i < F; i += nt) {
-        double s0[R], s1[R], s2[R], s3[R];
-#pragma unroll
-        for (int q = 0; q < R; ++q) s0[q] = s1[q] = s2[q] = s3[q] = 0.0;
         const T *Li = L + i;
-        for (int cb = 0; cb < S; cb += 16) {
-            T lv[16];
+        for (int q0 = 0; q0 < R; q0 += RS) {
+            double s0[RS], s1[RS], s2[RS], s3[RS];
 #pragma unroll
-            for (int u = 0; u < 16; ++u) lv[u] = Li[i64(min(cb + u, S - 1)) * ld];
+            for (int q = 0; q < RS; ++q) s0[q] = s1[q] = s2[q] = s3[q] = 0.0;
+            for (int cb = 0; cb < S; cb += 16) {
+                T lv[16];
 #pragma unroll
-            for (int q = 0; q < R; ++q)
+                for (int u = 0; u < 16; ++u) lv[u] = Li[i64(min(cb + u, S - 1)) * ld];
 #pragma unroll
-                for (int u = 0; u < 16; u += 4) {
-                    s0[q] += double(lv[u]) * vx[q][cb + u];
-                    s1[q] += double(lv[u + 1]) * vx[q][cb + u + 1];
-                    s2[q] += double(lv[u + 2]) * vx[q][cb + u + 2];
-                    s3[q] += double(lv[u + 3]) * vx[q][cb + u + 3];
-                }
+                for (int q = 0; q < RS; ++q)
+#pragma unroll
+                    for (int u = 0; u < 16; u += 4) {
+                        s0[q] += double(lv[u]) * vx[q0 + q][cb + u];
+                        s1[q] += double(lv[u + 1]) * vx[q0 + q][cb + u + 1];
+                        s2[q] += double(lv[u + 2]) * vx[q0 + q][cb + u + 2];
+                        s3[q] += double(lv[u + 3]) * vx[q0 + q][cb + u + 3];
+                    }
+            }
+#pragma unroll
+            for (int q = 0; q < RS; ++q)
+                if (rhs0 + q0 + q < nrhs)
+                    vwork[i64(rhs0 + q0 + q) * vstride + n.voff + i] =
+                        v[(q0 + q) * maxF + i] - ((s0[q] + s1[q]) + (s2[q] + s3[q]));
         }
-#pragma unroll
-        for (int q = 0; q < R; ++q)
-            if (rhs0 + q < nrhs)
-                vwork[i64(rhs0 + q) * vstride + n.voff + i] =
-                    v[q * maxF + i] - ((s0[q] + s1[q]) + (s2[q] + s3[q]));
     }
 }
 
-// Dynamic LDS: v[R][maxF + 16] (FP64), part[R][4*64] (FP64), then L11 (S x S, T).
+// Dynamic LDS: v[R][maxF + 16] (FP64), part[R][waves*64] (FP64), then L11 (S x S, T).
 template <typename T, int R>
 __global__ __launch_bounds__(256) void k_bwd_narrow(const i32 *__restrict__ list, int maxF,
                                                     const NodeInfo *__restrict__ nodes,
@@ -231,14 +234,14 @@ __global__ __launch_bounds__(256) void k_bwd_narrow(const i32 *__restrict__ list
                                                     i64 ldb, int nrhs)
 {
     extern __shared__ double smem[];
-    const int VS = maxF + 16;
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
+    const int VS = maxF + 16, PS = nt;     // PS: partial sums per right-hand side
     double *v = smem;                      // v[r * VS + i]: F entries and 16 zeros
-    double *part = smem + R * VS;          // part[r * 256 + wave * 64 + lane]
-    T *Lb = reinterpret_cast<T *>(part + R * 256);  // Lb[r*S + c] = L(r,c), r > c
+    double *part = smem + R * VS;          // part[r * PS + wave * 64 + lane]
+    T *Lb = reinterpret_cast<T *>(part + R * PS);  // Lb[r*S + c] = L(r,c), r > c
     const int nrb = (nrhs + R - 1) / R;
     const NodeInfo n = nodes[list[blockIdx.x / nrb]];
     const int rhs0 = int(blockIdx.x % nrb) * R;
-    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
     const int S = n.S, F = n.F;
     const i32 *rows = lindx + n.lxoff;
     const T *L = pool + n.foff;
@@ -262,25 +265,29 @@ __global__ __launch_bounds__(256) void k_bwd_narrow(const i32 *__restrict__ list
         const T *Lcol = L + i64(min(lane, S - 1)) * ld;
         const int per = ((F - S + nw - 1) / nw + 15) / 16 * 16;
         const int ra = S + wave * per, rz = min(F, ra + per);
-        double a0[R], a1[R], a2[R], a3[R];
+        constexpr int RS = R < 8 ? R : 8;  // right-hand sides accumulated at a time
+        for (int q0 = 0; q0 < R; q0 += RS) {
+            double a0[RS], a1[RS], a2[RS], a3[RS];
 #pragma unroll
-        for (int q = 0; q < R; ++q) a0[q] = a1[q] = a2[q] = a3[q] = 0.0;
-        for (int r = ra; r < rz; r += 16) {
-            T lv[16];
+            for (int q = 0; q < RS; ++q) a0[q] = a1[q] = a2[q] = a3[q] = 0.0;
+            for (int r = ra; r < rz; r += 16) {
+                T lv[16];
 #pragma unroll
-            for (int u = 0; u < 16; ++u) lv[u] = Lcol[min(r + u, F - 1)];
+                for (int u = 0; u < 16; ++u) lv[u] = Lcol[min(r + u, F - 1)];
 #pragma unroll
-            for (int q = 0; q < R; ++q)
+                for (int q = 0; q < RS; ++q)
 #pragma unroll
-                for (int u = 0; u < 16; u += 4) {
-                    a0[q] += double(lv[u]) * v[q * VS + r + u];
-                    a1[q] += double(lv[u + 1]) * v[q * VS + r + u + 1];
-                    a2[q] += double(lv[u + 2]) * v[q * VS + r + u + 2];
-                    a3[q] += double(lv[u + 3]) * v[q * VS + r + u + 3];
-                }
+                    for (int u = 0; u < 16; u += 4) {
+                        a0[q] += double(lv[u]) * v[(q0 + q) * VS + r + u];
+                        a1[q] += double(lv[u + 1]) * v[(q0 + q) * VS + r + u + 1];
+                        a2[q] += double(lv[u + 2]) * v[(q0 + q) * VS + r + u + 2];
+                        a3[q] += double(lv[u + 3]) * v[(q0 + q) * VS + r + u + 3];
+                    }
+            }
+#pragma unroll
+            for (int q = 0; q < RS; ++q)
+                part[(q0 + q) * PS + wave * 64 + lane] = (a0[q] + a1[q]) + (a2[q] + a3[q]);
         }
-#pragma unroll
-        for (int q = 0; q < R; ++q) part[q * 256 + wave * 64 + lane] = (a0[q] + a1[q]) + (a2[q] + a3[q]);
     }
     __syncthreads();
     // L11^T x1 = y1 by columns from the last, one right-hand side per
@@ -289,8 +296,8 @@ __global__ __launch_bounds__(256) void k_bwd_narrow(const i32 *__restrict__ list
     for (int q = wave; q < R; q += nw) {
         double x = 0.0;
         if (lane < S) {
-            double sp = part[q * 256 + lane];
-            for (int k = 1; k < nw; ++k) sp += part[q * 256 + k * 64 + lane];
+            double sp = part[q * PS + lane];
+            for (int k = 1; k < nw; ++k) sp += part[q * PS + k * 64 + lane];
             x = v[q * VS + lane] - sp;
         }
         for (int rb = S - 1; rb > 0; rb -= 8) {
@@ -770,98 +777,120 @@ constexpr int kSolveGroup = 8;
 constexpr size_t kNarrowLds = 64 * 1024;
 
 template <typename T>
-static size_t fwd_narrow_lds(int R, int mf, int ms)
+static size_t fwd_narrow_lds(int R, int mf, int ms, int nt)
 {
+    (void)nt;
     return sizeof(double) * size_t(R) * mf + sizeof(T) * size_t(ms) * ms;
 }
 template <typename T>
-static size_t bwd_narrow_lds(int R, int mf, int ms)
+static size_t bwd_narrow_lds(int R, int mf, int ms, int nt)
 {
-    return sizeof(double) * size_t(R) * (mf + 16 + 256) + sizeof(T) * size_t(ms) * ms;
+    return sizeof(double) * size_t(R) * (mf + 16 + nt) + sizeof(T) * size_t(ms) * ms;
+}
+// workgroup size of a level's narrow launches
+static unsigned narrow_threads(const Plan &p, int l)
+{
+    if (p.params.narrow_threads) return unsigned(p.params.narrow_threads);
+    return p.small_maxF[l] <= 128 ? 64u : 256u;
+}
+// Right-hand sides per workgroup of a level's narrow launches: the larger of
+// kSolveGroup and 4 whose LDS fits in kNarrowLds, or 1 below kSolveGroup
+// right-hand sides.  (Larger groups on the smallest fronts were measured
+// slower: one wavefront's chains and gathers for them run in sequence.)
+template <typename T>
+static int narrow_group(const Plan &p, int l, i64 nrhs)
+{
+    if (nrhs < kSolveGroup) return 1;
+    const int mf = p.small_maxF[l], ms = p.small_maxS[l], nt = int(narrow_threads(p, l));
+    for (int R : {kSolveGroup, 4}) {
+        if (std::max(fwd_narrow_lds<T>(R, mf, ms, nt), bwd_narrow_lds<T>(R, mf, ms, nt)) <=
+            kNarrowLds)
+            return R;
+    }
+    return 1;
 }
 
 template <typename T, int R>
-static void fwd_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs, bool narrow)
+static void fwd_narrow_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs)
 {
     const Plan &p = h->plan;
     DeviceBuffers &d = h->d;
     const T *pool = static_cast<const T *>(d.pool);
     const unsigned groups = unsigned((nrhs + R - 1) / R);
-    if (narrow) {
-        const unsigned ns = unsigned(p.solve_small_ptr[l + 1] - p.solve_small_ptr[l]);
-        const int mf = p.small_maxF[l], ms = p.small_maxS[l];
-        const unsigned nt = p.params.narrow_threads ? unsigned(p.params.narrow_threads)
-                                                    : (mf <= 128 ? 64u : 256u);
-        k_fwd_narrow<T, R><<<ns * groups, nt, fwd_narrow_lds<T>(R, mf, ms), h->stream>>>(
-            d.solve_small + p.solve_small_ptr[l], mf, d.nodes, d.lindx, d.upd_ptr, d.upd_src, pool,
-            d.dvec, b, ldb, d.vwork, p.vwork_elems, int(nrhs));
-    } else {
-        const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
-        k_fwd_wide<T, R><<<unsigned(p.wide_fwd_wgs[l]) * groups, 256, 0, h->stream>>>(
-            d.wide + w0, nw, d.wide_fwd_prefix + w0, d.nodes, d.lindx, d.upd_ptr, d.upd_src, pool,
-            d.dvec, b, ldb, d.vwork, p.vwork_elems, d.flags, std::max<i64>(p.nflags, 1), d.epoch,
-            d.spin_error, int(nrhs));
-    }
+    const unsigned ns = unsigned(p.solve_small_ptr[l + 1] - p.solve_small_ptr[l]);
+    const int mf = p.small_maxF[l], ms = p.small_maxS[l];
+    const unsigned nt = narrow_threads(p, l);
+    k_fwd_narrow<T, R><<<ns * groups, nt, fwd_narrow_lds<T>(R, mf, ms, nt), h->stream>>>(
+        d.solve_small + p.solve_small_ptr[l], mf, d.nodes, d.lindx, d.upd_ptr, d.upd_src, pool,
+        d.dvec, b, ldb, d.vwork, p.vwork_elems, int(nrhs));
 }
 
 template <typename T, int R>
-static void bwd_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs, bool narrow)
+static void fwd_wide_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs)
+{
+    const Plan &p = h->plan;
+    DeviceBuffers &d = h->d;
+    const T *pool = static_cast<const T *>(d.pool);
+    const unsigned groups = unsigned((nrhs + R - 1) / R);
+    const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
+    k_fwd_wide<T, R><<<unsigned(p.wide_fwd_wgs[l]) * groups, 256, 0, h->stream>>>(
+        d.wide + w0, nw, d.wide_fwd_prefix + w0, d.nodes, d.lindx, d.upd_ptr, d.upd_src, pool,
+        d.dvec, b, ldb, d.vwork, p.vwork_elems, d.flags, std::max<i64>(p.nflags, 1), d.epoch,
+        d.spin_error, int(nrhs));
+}
+
+template <typename T, int R>
+static void bwd_narrow_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs)
+{
+    const Plan &p = h->plan;
+    DeviceBuffers &d = h->d;
+    const T *pool = static_cast<const T *>(d.pool);
+    const unsigned groups = unsigned((nrhs + R - 1) / R);
+    const unsigned ns = unsigned(p.solve_small_ptr[l + 1] - p.solve_small_ptr[l]);
+    const int mf = p.small_maxF[l], ms = p.small_maxS[l];
+    const unsigned nt = narrow_threads(p, l);
+    k_bwd_narrow<T, R><<<ns * groups, nt, bwd_narrow_lds<T>(R, mf, ms, nt), h->stream>>>(
+        d.solve_small + p.solve_small_ptr[l], mf, d.nodes, d.lindx, pool, b, ldb, int(nrhs));
+}
+
+template <typename T, int R>
+static void bwd_wide_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs)
 {
     const Plan &p = h->plan;
     DeviceBuffers &d = h->d;
     const T *pool = static_cast<const T *>(d.pool);
     const unsigned groups = unsigned((nrhs + R - 1) / R);
     const i64 npart = std::max<i64>(p.npart, 1);
-    if (narrow) {
-        const unsigned ns = unsigned(p.solve_small_ptr[l + 1] - p.solve_small_ptr[l]);
-        const int mf = p.small_maxF[l], ms = p.small_maxS[l];
-        const unsigned nt = p.params.narrow_threads ? unsigned(p.params.narrow_threads)
-                                                    : (mf <= 128 ? 64u : 256u);
-        k_bwd_narrow<T, R><<<ns * groups, nt, bwd_narrow_lds<T>(R, mf, ms), h->stream>>>(
-            d.solve_small + p.solve_small_ptr[l], mf, d.nodes, d.lindx, pool, b, ldb, int(nrhs));
-    } else {
-        const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
-        if (p.wide_rect_wgs[l] > 0)
-            k_bwd_rect<T, R><<<unsigned(p.wide_rect_wgs[l]) * groups, 256, 0, h->stream>>>(
-                d.wide + w0, nw, d.wide_rect_prefix + w0, d.nodes, d.lindx, pool, b, ldb,
-                d.partials, npart, int(nrhs));
-        k_bwd_wide<T, R><<<unsigned(p.wide_bwd_wgs[l]) * groups, 256, 0, h->stream>>>(
-            d.wide + w0, nw, d.wide_bwd_prefix + w0, d.nodes, d.lindx, pool, b, ldb, d.vwork,
-            p.vwork_elems, d.partials, npart, d.flags, std::max<i64>(p.nflags, 1), d.epoch,
-            d.spin_error, int(nrhs));
-    }
+    const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
+    if (p.wide_rect_wgs[l] > 0)
+        k_bwd_rect<T, R><<<unsigned(p.wide_rect_wgs[l]) * groups, 256, 0, h->stream>>>(
+            d.wide + w0, nw, d.wide_rect_prefix + w0, d.nodes, d.lindx, pool, b, ldb, d.partials,
+            npart, int(nrhs));
+    k_bwd_wide<T, R><<<unsigned(p.wide_bwd_wgs[l]) * groups, 256, 0, h->stream>>>(
+        d.wide + w0, nw, d.wide_bwd_prefix + w0, d.nodes, d.lindx, pool, b, ldb, d.vwork,
+        p.vwork_elems, d.partials, npart, d.flags, std::max<i64>(p.nflags, 1), d.epoch,
+        d.spin_error, int(nrhs));
 }
 
 template <typename T>
-static int solve_impl(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which)
+static int solve_chunk(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which)
 {
     const Plan &p = h->plan;
     DeviceBuffers &d = h->d;
     hipStream_t s = h->stream;
-    if (int rc = ensure_solve_buffers(h, nrhs)) return rc;
     const bool grouped = nrhs >= kSolveGroup;
-    // group size of a level's narrow launch (fwd: dir 0, bwd: dir 1)
-    auto narrow_group = [&](int l, int dir) {
-        if (!grouped) return 1;
-        const int mf = p.small_maxF[l], ms = p.small_maxS[l];
-        for (int R : {kSolveGroup, 4}) {
-            const size_t need = dir ? bwd_narrow_lds<T>(R, mf, ms) : fwd_narrow_lds<T>(R, mf, ms);
-            if (need <= kNarrowLds) return R;
-        }
-        return 1;
-    };
     if (which & 1) {
         k_epoch_bump<<<1, 64, 0, s>>>(d.epoch);
         for (int l = 0; l < p.nlevels; ++l) {
             if (p.solve_small_ptr[l + 1] > p.solve_small_ptr[l]) {
-                const int R = narrow_group(l, 0);
-                if (R == kSolveGroup) fwd_level<T, kSolveGroup>(h, l, b, ldb, nrhs, true);
-                else if (R == 4) fwd_level<T, 4>(h, l, b, ldb, nrhs, true);
-                else fwd_level<T, 1>(h, l, b, ldb, nrhs, true);
+                const int R = narrow_group<T>(p, l, nrhs);
+                if (R == kSolveGroup) fwd_narrow_level<T, kSolveGroup>(h, l, b, ldb, nrhs);
+                else if (R == 4) fwd_narrow_level<T, 4>(h, l, b, ldb, nrhs);
+                else fwd_narrow_level<T, 1>(h, l, b, ldb, nrhs);
             }
             if (p.wide_ptr[l + 1] > p.wide_ptr[l]) {
-                if (grouped) fwd_level<T, kSolveGroup>(h, l, b, ldb, nrhs, false);
-                else fwd_level<T, 1>(h, l, b, ldb, nrhs, false);
+                if (grouped) fwd_wide_level<T, kSolveGroup>(h, l, b, ldb, nrhs);
+                else fwd_wide_level<T, 1>(h, l, b, ldb, nrhs);
             }
         }
     }
@@ -869,18 +898,42 @@ static int solve_impl(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which)
         k_epoch_bump<<<1, 64, 0, s>>>(d.epoch);
         for (int l = p.nlevels - 1; l >= 0; --l) {
             if (p.wide_ptr[l + 1] > p.wide_ptr[l]) {
-                if (grouped) bwd_level<T, kSolveGroup>(h, l, b, ldb, nrhs, false);
-                else bwd_level<T, 1>(h, l, b, ldb, nrhs, false);
+                if (grouped) bwd_wide_level<T, kSolveGroup>(h, l, b, ldb, nrhs);
+                else bwd_wide_level<T, 1>(h, l, b, ldb, nrhs);
             }
             if (p.solve_small_ptr[l + 1] > p.solve_small_ptr[l]) {
-                const int R = narrow_group(l, 1);
-                if (R == kSolveGroup) bwd_level<T, kSolveGroup>(h, l, b, ldb, nrhs, true);
-                else if (R == 4) bwd_level<T, 4>(h, l, b, ldb, nrhs, true);
-                else bwd_level<T, 1>(h, l, b, ldb, nrhs, true);
+                const int R = narrow_group<T>(p, l, nrhs);
+                if (R == kSolveGroup) bwd_narrow_level<T, kSolveGroup>(h, l, b, ldb, nrhs);
+                else if (R == 4) bwd_narrow_level<T, 4>(h, l, b, ldb, nrhs);
+                else bwd_narrow_level<T, 1>(h, l, b, ldb, nrhs);
             }
         }
     }
     SPRGPU_HIP(hipGetLastError());
+    return 0;
+}
+
+// A launch holds at most 2^23 workgroups: large batches go in chunks of
+// right-hand sides sized with each level's group size.
+template <typename T>
+static int solve_impl(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which)
+{
+    const Plan &p = h->plan;
+    if (int rc = ensure_solve_buffers(h, nrhs)) return rc;
+    constexpr i64 kMaxWgs = i64(1) << 23;
+    i64 chunk = nrhs;
+    for (int l = 0; l < p.nlevels; ++l) {
+        const i64 ns = p.solve_small_ptr[l + 1] - p.solve_small_ptr[l];
+        const i64 wmax = std::max({p.wide_fwd_wgs[l], p.wide_bwd_wgs[l], p.wide_rect_wgs[l]});
+        const i64 rn = ns > 0 ? narrow_group<T>(p, l, nrhs) : 1;  // group of the narrow launches
+        const i64 rw = (nrhs >= kSolveGroup) ? kSolveGroup : 1;
+        if (ns > 0) chunk = std::min(chunk, std::max<i64>(1, kMaxWgs / ns) * rn);
+        if (wmax > 0) chunk = std::min(chunk, std::max<i64>(1, kMaxWgs / wmax) * rw);
+    }
+    for (i64 r0 = 0; r0 < nrhs; r0 += chunk) {
+        if (int rc = solve_chunk<T>(h, b + r0 * ldb, ldb, std::min(chunk, nrhs - r0), which))
+            return rc;
+    }
     return 0;
 }
 
@@ -897,24 +950,11 @@ int run_solve(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which)
         set_error("at most 65535 right-hand sides per call");
         return -1;
     }
-    // The right-hand side is the fastest-varying workgroup index (the
-    // workgroups of one supernode for all right-hand sides run together and
-    // share its factor through the caches); a launch holds at most 2^23
-    // workgroups, so large batches go in chunks.
-    const Plan &p = h->plan;
-    i64 maxb = 1;
-    for (int l = 0; l < p.nlevels; ++l)
-        maxb = std::max({maxb, i64(p.solve_small_ptr[l + 1] - p.solve_small_ptr[l]),
-                         p.wide_fwd_wgs[l], p.wide_bwd_wgs[l], p.wide_rect_wgs[l]});
-    const i64 chunk = std::max<i64>(1, std::min<i64>(nrhs, (i64(1) << 23) / maxb));
-    for (i64 r0 = 0; r0 < nrhs; r0 += chunk) {
-        const i64 nr = std::min(chunk, nrhs - r0);
-        const int rc = (h->precision == SPRGPU_FP32)
-                           ? solve_impl<float>(h, b + r0 * ldb, ldb, nr, which)
-                           : solve_impl<double>(h, b + r0 * ldb, ldb, nr, which);
-        if (rc) return rc;
-    }
-    return 0;
+    // The group of right-hand sides is the fastest-varying workgroup index
+    // (the workgroups of one supernode for all groups run together and share
+    // its factor through the caches).
+    return (h->precision == SPRGPU_FP32) ? solve_impl<float>(h, b, ldb, nrhs, which)
+                                         : solve_impl<double>(h, b, ldb, nrhs, which);
 }
 
 int check_solve_error(sprgpu_handle *h)
