@@ -336,7 +336,8 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
                                                   const double *__restrict__ dvec, double *b,
                                                   i64 ldb, double *vwork, i64 vstride,
                                                   unsigned *flags, i64 nflags,
-                                                  const unsigned *epoch_ptr, int *err, int nrhs)
+                                                  const unsigned *epoch_ptr, int *err, int nrhs,
+                                                  const T *__restrict__ bmscr)
 {
     __shared__ double xs[R][NB];
     __shared__ double ys[R][NB];
@@ -400,6 +401,23 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
         }
         if (tid == 0) Ld[kZero] = T(0);
     }
+    // Diagonal solve operands: with one right-hand side (the latency-bound
+    // case) loaded ahead of the waits, so that their latency stays off the
+    // chain of blocks; with a group, after the loop (registers).  Lanes
+    // 32..63 mirror lanes 0..31 (lane & 31) and do no work there.
+    const int sr = 32 * wave + (lane & 31);  // block row of this lane (lanes < 32 work)
+    const bool sw = diag && lane < 32 && sr < nr;
+    const int nbw = min(32, nr - 32 * wave);  // rows of this wave's sub-block
+    T lsub[32];
+    double dsr = 0.0;
+    auto load_bm = [&]() {
+        // column lane of the sub-block's Bm (rows past the sub-block zero)
+        const T *bm = bmscr + i64(n.first + r0 + 32 * wave) * kPanelNB + (lane & 31);
+#pragma unroll
+        for (int c = 0; c < 32; ++c) lsub[c] = (sw && c < nbw) ? bm[c * kPanelNB] : T(0);
+        dsr = sw ? dvec[n.first + r0 + sr] : 0.0;
+    };
+    if (R == 1 && bmscr) load_bm();
     const int nprev = diag ? local : w.nblk;
     double acc[R];
 #pragma unroll
@@ -438,32 +456,60 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
         }
         return;
     }
-    // Diagonal block: four 32 x 32 sub-blocks in sequence.  Wave w keeps row
-    // 32w + lane (lane < 32) of its unit-lower sub-block in registers and
-    // solves it with a readlane chain; the waves below then apply it to their
-    // rows from the packed block in LDS.
+    // Diagonal block: four 32 x 32 sub-blocks in sequence.  Wave w solves
+    // its sub-block, lane i < 32 holding row 32w + i: with the factor's
+    // inverses (bmscr: Bm = L11^-T D^-1 of every 32-column sub-block, so
+    // L11^-1 y = D Bm^T y) as a product x_i = d_i sum_c Bm(c, i) y_c, else by
+    // a readlane chain over the row of L11 in registers.  The waves below
+    // then add its products with their rows of the packed block in LDS to
+    // their pending sums, registers with one right-hand side (pend), LDS for
+    // a group (registers there would become hoisted loads and spills), so
+    // that one barrier per sub-block orders the steps.
+    if (bmscr) {
+        if (R != 1) load_bm();
+    } else {
+#pragma unroll
+        for (int c = 0; c < 32; ++c) {
+            const int cc = 32 * wave + c;
+            lsub[c] = Ld[(sw && c < (lane & 31)) ? packed_col(cc) + sr - cc : kZero];
+        }
+    }
     if (tid < NB) {
 #pragma unroll
         for (int k = 0; k < R; ++k)
             ys[k][tid] = (tid < nr) ? ys[k][tid] - (part[0][k][tid] + part[1][k][tid]) : 0.0;
     }
-    // Lanes 32..63 mirror lanes 0..31 (lane & 31) and do no work.
-    const int sr = 32 * wave + (lane & 31);  // block row of this lane (lanes < 32 work)
-    const bool sw = lane < 32 && sr < nr;
-    T lsub[32];
+    __syncthreads();
+    if (tid < NB) {  // part[0] holds a group's pending sums from here
 #pragma unroll
-    for (int c = 0; c < 32; ++c) {
-        const int cc = 32 * wave + c;
-        lsub[c] = Ld[(sw && c < (lane & 31)) ? packed_col(cc) + sr - cc : kZero];
+        for (int k = 0; k < R; ++k) part[0][k][tid] = 0.0;
     }
     __syncthreads();
+    double pend[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) pend[k] = 0.0;
     for (int j = 0; j < 4 && 32 * j < nr; ++j) {
         if (wave == j) {
 #pragma unroll
             for (int k = 0; k < R; ++k) {
-                double x = sw ? ys[k][sr] : 0.0;
+                double x;
+                if (bmscr) {
+                    // the wave's own LDS stores precede its loads (in order per wave)
+                    if (lane < 32) ys[k][sr] -= (R == 1 ? pend[k] : part[0][k][sr]);
+                    __builtin_amdgcn_wave_barrier();
+                    double a0 = 0.0, a1 = 0.0;
 #pragma unroll
-                for (int c = 0; c < 32; ++c) x -= double(lsub[c]) * lane_bcast(x, c);
+                    for (int c = 0; c < 32; c += 2) {
+                        a0 += double(lsub[c]) * ys[k][32 * j + c];
+                        a1 += double(lsub[c + 1]) * ys[k][32 * j + c + 1];
+                    }
+                    x = dsr * (a0 + a1);
+                } else {
+                    x = (sw ? ys[k][sr] : 0.0) - (R == 1 ? pend[k] : part[0][k][sr]);
+#pragma unroll
+                    for (int c = 0; c < 32; ++c) x -= double(lsub[c]) * lane_bcast(x, c);
+                }
+                __builtin_amdgcn_wave_barrier();
                 if (lane < 32) ys[k][sr] = x;
             }
         }
@@ -480,10 +526,12 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
                     s0 += double(a0) * ys[k][cc];
                     s1 += double(a1) * ys[k][cc + 1];
                 }
-                ys[k][sr] -= s0 + s1;
+                if constexpr (R == 1)
+                    pend[k] += s0 + s1;
+                else
+                    part[0][k][sr] += s0 + s1;
             }
         }
-        __syncthreads();
     }
     if (wave == 0) {
         // results for the blocks that wait on this one first, then B
@@ -584,7 +632,9 @@ __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ w
                                                   double *vwork, i64 vstride,
                                                   const double *__restrict__ partials, i64 npart,
                                                   unsigned *flags, i64 nflags,
-                                                  const unsigned *epoch_ptr, int *err, int nrhs)
+                                                  const unsigned *epoch_ptr, int *err, int nrhs,
+                                                  const T *__restrict__ bmscr,
+                                                  const double *__restrict__ dvec)
 {
     __shared__ double xs[R][NB];
     __shared__ double ys[R][NB];
@@ -631,66 +681,126 @@ __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ w
     if (tid < NB) {
         const i32 gi = rows[c0 + min(tid, nc - 1)];
 #pragma unroll
-        for (int k = 0; k < R; ++k)
+        for (int k = 0; k < R; ++k) {
             ys[k][tid] = (tid < nc)
                              ? b[i64(min(rhs0 + k, nrhs - 1)) * ldb + gi] - (part[0][k][tid] + part[1][k][tid])
                              : 0.0;
+            part[0][k][tid] = 0.0;  // a group's pending sums in the diagonal solve
+        }
     }
-    __syncthreads();
-    double acc[R];
+    // Diagonal solve operands: with one right-hand side (the latency-bound
+    // case) loaded ahead of the waits, so that their latency stays off the
+    // chain of blocks; with a group, after the loop (registers).  Lanes
+    // 32..63 mirror lanes 0..31 (lane & 31) and do no work there.
+    const int sc = 32 * wave + (lane & 31);  // block column of this lane (lanes < 32 work)
+    const bool sw = lane < 32 && sc < nc;
+    const int nbw = min(32, nc - 32 * wave);  // columns of this wave's sub-block
+    T lcol[32];
+    double dsc = 0.0;  // d of this lane's column (with the inverses)
+    auto load_bm = [&]() {
+        // row lane of the sub-block's Bm (columns past the sub-block zero)
+        const T *bm = bmscr + i64(n.first + c0 + sc) * kPanelNB;
 #pragma unroll
-    for (int k = 0; k < R; ++k) acc[k] = 0.0;
+        for (int r = 0; r < 32; ++r) lcol[r] = (sw && r < nbw) ? bm[r] : T(0);
+        dsc = sw ? dvec[n.first + c0 + sc] : 0.0;
+    };
+    if (R == 1 && bmscr) load_bm();
+    __syncthreads();
+    // Products with the later blocks, L(p, blk)^T x_p: the thread sums
+    // columns gc + 8j (j < 4) over rows 8q + gs (q < 16) of every block, so
+    // that a load instruction reads 8 rows of 8 columns (8 cache lines where
+    // a lane per column would touch 64 and be bound by address processing);
+    // the eight lanes of a column are added at the end (xor 1, 2, 4).
+    const int gs = lane & 7, gc = 32 * wave + (lane >> 3);
+    double acc[4][R];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int k = 0; k < R; ++k) acc[j][k] = 0.0;
     for (int p = w.nblk - 1; p > blk; --p) {
         const int rp0 = p * NB, np = min(NB, S - rp0);
         // unconditional, unmasked loads (see k_fwd_wide): columns past the
         // block read its last column (not stored), rows past block p read
         // its last row (xs is zero there)
-        T lv[NH];
-        const T *Lc = L + rp0 + i64(c0 + min(ci, nc - 1)) * ld;
+        T lv[4][NB / 8];
 #pragma unroll
-        for (int r = 0; r < NH; ++r) lv[r] = Lc[min(h * NH + r, np - 1)];
+        for (int j = 0; j < 4; ++j) {
+            const T *Lc = L + rp0 + i64(c0 + min(gc + 8 * j, nc - 1)) * ld;
+#pragma unroll
+            for (int q = 0; q < NB / 8; ++q) lv[j][q] = Lc[min(8 * q + gs, np - 1)];
+        }
         wait_ready(fl + p, epoch, err);
         if (tid < NB) {
 #pragma unroll
-            for (int k = 0; k < R; ++k) xs[k][tid] = (tid < np) ? load_shared(vwork + vo[k] + rp0 + tid) : 0.0;
+            for (int k = 0; k < R; ++k)
+                xs[k][tid] = (tid < np) ? load_shared(vwork + vo[k] + rp0 + tid) : 0.0;
         }
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < NH; ++r)
+        for (int j = 0; j < 4; ++j)
 #pragma unroll
-            for (int k = 0; k < R; ++k) acc[k] += double(lv[r]) * xs[k][h * NH + r];
+            for (int q = 0; q < NB / 8; ++q)
+#pragma unroll
+                for (int k = 0; k < R; ++k) acc[j][k] += double(lv[j][q]) * xs[k][8 * q + gs];
         __syncthreads();
     }
 #pragma unroll
-    for (int k = 0; k < R; ++k) part[h][k][ci] = acc[k];
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            double v = acc[j][k];
+            v += __shfl_xor(v, 1);
+            v += __shfl_xor(v, 2);
+            v += __shfl_xor(v, 4);
+            if (gs == 0 && gc + 8 * j < nc) ys[k][gc + 8 * j] -= v;
+        }
+    }
     __syncthreads();
     // Diagonal block, L11^T x = y: four 32 x 32 sub-blocks from the last.
-    // Wave w keeps column 32w + lane (lane < 32) of its sub-block in
-    // registers and solves it with a readlane chain; the waves to the left
-    // then apply it to their columns from the packed block in LDS.
-    if (tid < NB) {
+    // Wave w solves its sub-block, lane i < 32 holding column 32w + i: with
+    // the factor's inverses (L11^-T y = Bm D y) as a product
+    // x_i = sum_r Bm(i, r) d_r y_r, else by a readlane chain over the column
+    // of L11 in registers.  The waves to the left then add its products with
+    // their columns of the packed block in LDS to their pending sums,
+    // registers with one right-hand side (pend), LDS for a group (registers
+    // there would become hoisted loads and spills), so that one barrier per
+    // sub-block orders the steps.
+    if (bmscr) {
+        if (R != 1) load_bm();
+    } else {
 #pragma unroll
-        for (int k = 0; k < R; ++k)
-            ys[k][tid] = (tid < nc) ? ys[k][tid] - (part[0][k][tid] + part[1][k][tid]) : 0.0;
+        for (int r = 0; r < 32; ++r) {
+            const int rr = 32 * wave + r;
+            lcol[r] = Lr[(sw && r > (lane & 31) && rr < nc) ? rr * (rr + 1) / 2 + sc : kZero];
+        }
     }
-    // Lanes 32..63 mirror lanes 0..31 (lane & 31) and do no work.
-    const int sc = 32 * wave + (lane & 31);  // block column of this lane (lanes < 32 work)
-    const bool sw = lane < 32 && sc < nc;
-    T lcol[32];
+    double pend[R];
 #pragma unroll
-    for (int r = 0; r < 32; ++r) {
-        const int rr = 32 * wave + r;
-        lcol[r] = Lr[(sw && r > (lane & 31) && rr < nc) ? rr * (rr + 1) / 2 + sc : kZero];
-    }
-    __syncthreads();
+    for (int k = 0; k < R; ++k) pend[k] = 0.0;
     for (int j = 3; j >= 0; --j) {
         if (32 * j >= nc) continue;
         if (wave == j) {
 #pragma unroll
             for (int k = 0; k < R; ++k) {
-                double x = sw ? ys[k][sc] : 0.0;
+                double x;
+                if (bmscr) {
+                    // d_r y_r of the sub-block; the wave's own LDS stores
+                    // precede its loads (in order per wave)
+                    if (lane < 32) ys[k][sc] = dsc * (ys[k][sc] - (R == 1 ? pend[k] : part[0][k][sc]));
+                    __builtin_amdgcn_wave_barrier();
+                    double a0 = 0.0, a1 = 0.0;
 #pragma unroll
-                for (int r = 31; r > 0; --r) x -= double(lcol[r]) * lane_bcast(x, r);
+                    for (int r = 0; r < 32; r += 2) {
+                        a0 += double(lcol[r]) * ys[k][32 * j + r];
+                        a1 += double(lcol[r + 1]) * ys[k][32 * j + r + 1];
+                    }
+                    x = a0 + a1;
+                } else {
+                    x = (sw ? ys[k][sc] : 0.0) - (R == 1 ? pend[k] : part[0][k][sc]);
+#pragma unroll
+                    for (int r = 31; r > 0; --r) x -= double(lcol[r]) * lane_bcast(x, r);
+                }
+                __builtin_amdgcn_wave_barrier();
                 if (lane < 32) ys[k][sc] = x;
             }
         }
@@ -707,10 +817,12 @@ __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ w
                     s0 += double(a0) * ys[k][rr];
                     s1 += double(a1) * ys[k][rr + 1];
                 }
-                ys[k][sc] -= s0 + s1;
+                if constexpr (R == 1)
+                    pend[k] += s0 + s1;
+                else
+                    part[0][k][sc] += s0 + s1;
             }
         }
-        __syncthreads();
     }
     if (wave == 0) {
 #pragma unroll
@@ -810,6 +922,15 @@ static int narrow_group(const Plan &p, int l, i64 nrhs)
     return 1;
 }
 
+// The factor's 32 x 32 diagonal-block inverses, when the outer-block scheme
+// computed them (in the panel-step scheme the scratch holds the factored
+// blocks instead and the solves use readlane chains).
+template <typename T>
+static const T *solve_inverses(sprgpu_handle *h)
+{
+    return h->plan.params.block_factor ? static_cast<const T *>(h->d.dscr) : nullptr;
+}
+
 template <typename T, int R>
 static void fwd_narrow_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs)
 {
@@ -836,7 +957,7 @@ static void fwd_wide_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs
     k_fwd_wide<T, R><<<unsigned(p.wide_fwd_wgs[l]) * groups, 256, 0, h->stream>>>(
         d.wide + w0, nw, d.wide_fwd_prefix + w0, d.nodes, d.lindx, d.upd_ptr, d.upd_src, pool,
         d.dvec, b, ldb, d.vwork, p.vwork_elems, d.flags, std::max<i64>(p.nflags, 1), d.epoch,
-        d.spin_error, int(nrhs));
+        d.spin_error, int(nrhs), solve_inverses<T>(h));
 }
 
 template <typename T, int R>
@@ -869,7 +990,7 @@ static void bwd_wide_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs
     k_bwd_wide<T, R><<<unsigned(p.wide_bwd_wgs[l]) * groups, 256, 0, h->stream>>>(
         d.wide + w0, nw, d.wide_bwd_prefix + w0, d.nodes, d.lindx, pool, b, ldb, d.vwork,
         p.vwork_elems, d.partials, npart, d.flags, std::max<i64>(p.nflags, 1), d.epoch,
-        d.spin_error, int(nrhs));
+        d.spin_error, int(nrhs), solve_inverses<T>(h), d.dvec);
 }
 
 template <typename T>
