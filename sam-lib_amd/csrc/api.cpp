@@ -40,7 +40,7 @@ static void free_all(sprgpu_handle *h)
                     d.ea_tasks, d.small_nodes, d.panel_desc, d.panel_prefix, d.gemm_desc,
                     d.gemm_prefix, d.pool,     d.dvec,       d.adiag,      d.dcor,
                     d.status,   d.stats,      d.vwork,       d.csr_rowptr, d.csr_col,
-                    d.dscr,
+                    d.dscr,     d.dinv,
                     d.csr_src,  d.csr_val,    d.irwork,      d.level_nodes,
                     d.solve_small, d.wide,     d.wide_fwd_prefix, d.wide_bwd_prefix,
                     d.wide_rect_prefix, d.upd_ptr, d.upd_src, d.flags, d.partials,
@@ -177,6 +177,8 @@ int sprgpu_create(const sprgpu_int *mspar, const sprgpu_int *mtrees, const sprgp
     if (e == hipSuccess) e = hipMalloc(&d.status, sizeof(FactorStatus));
     if (e == hipSuccess) e = hipMalloc(&d.stats, sizeof(double) * 4 * 257);
     if (e == hipSuccess) e = hipMalloc(&d.dscr, esz * size_t(p.neq) * kPanelNB);
+    const size_t ninv = size_t(std::max<i64>(p.nflags, 1)) * 2 * kInvLayout;
+    if (e == hipSuccess) e = hipMalloc(&d.dinv, esz * ninv);
     if (e != hipSuccess) {
         free_all(h.get());
         return hip_fail(e, "device allocation");
@@ -186,7 +188,8 @@ int sprgpu_create(const sprgpu_int *mspar, const sprgpu_int *mtrees, const sprgp
         free_all(h.get());
         return hip_fail(hipGetLastError(), "device memset");
     }
-    B += esz * p.pool_elems + sizeof(double) * 4 * p.neq + sizeof(FactorStatus) + 8 * 4 * 257;
+    B += esz * p.pool_elems + sizeof(double) * 4 * p.neq + sizeof(FactorStatus) + 8 * 4 * 257 +
+         esz * (size_t(p.neq) * kPanelNB + ninv);
     // the factorization's second stream and its two events (default priority)
     if (hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&h->ev[0], hipEventDisableTiming) != hipSuccess ||

@@ -1217,6 +1217,8 @@ static int factor_impl(sprgpu_handle *h, const double *sm_dev, int mode, const d
             d.panel_desc, d.nodes, pool, static_cast<const T *>(d.dscr));
         mark(P_PDIAG, 0.0);
     }
+    if (int rc = compute_block_inverses(h)) return rc;
+    mark(P_OTHER, 0.0);
     k_stats_partial<<<kStatBlocks, 256, 0, s>>>(d.dvec, d.adiag, p.neq, d.status, d.stats);
     k_stats_final<<<1, 64, 0, s>>>(d.stats, kStatBlocks);
     SPRGPU_HIP(hipGetLastError());
@@ -1362,6 +1364,7 @@ int load_factor_from_sm(sprgpu_handle *h, const double *sm_dev)
     SPRGPU_HIP(hipMemcpyAsync(h->d.dvec, sm_dev, sizeof(double) * p.neq,
                               hipMemcpyDeviceToDevice, h->stream));
     SPRGPU_HIP(hipGetLastError());
+    if (int rc = compute_block_inverses(h)) return rc;
     h->factored = true;
     return 0;
 }

@@ -18,6 +18,29 @@ __device__ __forceinline__ double lane_bcast(double v, int l)
     return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
 }
 
+// v of the lane that DPP control CTRL selects (quad_perm, row_half_mirror ...),
+// per 32-bit half.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = unsigned(__builtin_amdgcn_update_dpp(0, int(unsigned(u)), CTRL, 0xF, 0xF, false));
+    const unsigned hi = unsigned(__builtin_amdgcn_update_dpp(0, int(unsigned(u >> 32)), CTRL, 0xF, 0xF, false));
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
+// Sum of v over each aligned group of 8 lanes, in every lane of the group:
+// lane pairs (xor 1), then quads (xor 2), then the two quads (each lane adds
+// its mirror in the group, which holds the other quad's sum); a + b and
+// b + a round alike, so all eight lanes get the same bits.
+__device__ __forceinline__ double sum8(double v)
+{
+    v += dpp_move<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+    v += dpp_move<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+    v += dpp_move<0x141>(v);  // row_half_mirror
+    return v;
+}
+
 // v, hidden from the optimiser (no code): keeps a per-lane value from being
 // treated as loop-invariant, so comparisons with it stay where they are used.
 __device__ __forceinline__ int opaque(int v)

@@ -33,6 +33,10 @@ constexpr int kEaCols = 8;       // child columns per workgroup in extend-add
 constexpr int kSolveSmallS = 64;   // narrow supernodes: S <= this ...
 constexpr int kSolveSmallF = 3072; // ... and F <= this: one workgroup, vector in LDS
 constexpr int kSolveNB = 128;      // column block (and rectangle row chunk) of the wide solve
+// The inverse of a wide diagonal block is kept in two zero-padded layouts of
+// kInvLayout entries each (the forward and the backward sweep's), i.e. the
+// lower triangle's three 64 x 64 quadrants per layout.
+constexpr int kInvLayout = 3 * kSolveNB * kSolveNB / 4;
 
 // A wide supernode in the dataflow solves: its diagonal part is split into
 // nblk blocks of kSolveNB columns (one workgroup each, ordered so that every
@@ -193,6 +197,7 @@ struct DeviceBuffers {
     FactorStatus *status = nullptr;
     double *stats = nullptr;    // reduction scratch for TOL(2)/TOL(3)/NUMNEG
     void *dscr = nullptr;       // factored diagonal blocks of the panel steps, T[NEQ*kPanelNB]
+    void *dinv = nullptr;       // L11^-1 of every wide-solve column block, T[nflags*2*kInvLayout]
     double *vwork = nullptr;    // solve workspace (vwork_elems * nrhs_cap)
     i64 vwork_cap = 0;
     // residual pattern
@@ -260,6 +265,9 @@ int run_factor(sprgpu_handle *h, const double *sm_dev, int mode, const double *t
 int read_factor_info(sprgpu_handle *h, sprgpu_factor_info *info);
 int copy_factor_to_sm(sprgpu_handle *h, double *sm_dev_out);
 int load_factor_from_sm(sprgpu_handle *h, const double *sm_dev);
+// Inverses of the wide-solve diagonal blocks from the factor in the pool
+// (after every factorization and factor upload), on the handle's stream.
+int compute_block_inverses(sprgpu_handle *h);
 int profile_factor(sprgpu_handle *h, const double *sm_dev, int mode, const double *tol,
                    double *out);
 // solve.hip

@@ -13,10 +13,11 @@
 // * Wide supernodes are split into kSolveNB-column blocks, one workgroup per
 //   block, plus (forward) one workgroup per kSolveNB rows below the diagonal
 //   part.  A block's workgroup loads the L values of each block it depends
-//   on before waiting for that block's ready flag, applies it as a GEMV with
-//   two threads per row, and finally solves its own 128 x 128 unit-triangular
-//   block as four 32 x 32 sub-blocks: a readlane chain over registers on one
-//   wavefront per sub-block, the updates between sub-blocks in parallel.
+//   on before waiting for that block's ready flag and applies them as a
+//   GEMV (forward: two threads per row; backward, the transposed product:
+//   eight lanes per column, so that loads stay coalesced), then solves its
+//   own 128 x 128 unit-triangular block as one more GEMV with the block's
+//   inverse, computed after the factorization (k_block_inverse).
 //   Every workgroup only waits for lower-indexed workgroups of the same
 //   launch, which the hardware dispatches first, and each wait is bounded.
 //   In the backward sweep the products with the rows below the diagonal part
@@ -38,7 +39,7 @@ namespace sprgpu {
 constexpr unsigned long long kSpinLimit = 1ull << 24;
 constexpr int NB = kSolveNB;
 constexpr int NH = NB / 2;  // columns (forward) or rows (backward) per thread of a block GEMV
-static_assert(NB == 128, "two threads per row of a block; one 32-row sub-block per wavefront");
+static_assert(NB == 128, "two threads per row (column) of a block GEMV, 64 entries each");
 
 // Block results cross workgroups (and XCDs, whose L2 caches are not kept
 // coherent with each other) as agent-scope atomic stores and loads, which
@@ -92,14 +93,18 @@ __global__ void k_epoch_bump(unsigned *epoch)
     if (threadIdx.x == 0) *epoch += 1;
 }
 
-// Offset of column c in a packed lower-triangular NB x NB block stored by
-// columns (column c holds rows c..NB-1).
-__device__ __forceinline__ int packed_col(int c) { return c * NB - c * (c - 1) / 2; }
-// The packed blocks in LDS carry one extra zero entry: a read of an entry
-// outside the block selects this address instead of masking the loaded value
-// (a select on a loaded value makes the compiler wait for each load).
-constexpr int kPacked = NB * (NB + 1) / 2;
-constexpr int kZero = kPacked;
+// Layouts of a diagonal block's inverse X (zero above the diagonal and past
+// the block), the lower triangle's three 64 x 64 quadrants each:
+//   forward:  columns 0..63 over all rows, X(r, c) at c*NB + r, then columns
+//             64..127 over rows 64..127 at NB*NB/2 + (c-64)*64 + (r-64);
+//   backward: rows 0..63 over columns 0..63, X(r, c) at r*64 + c, then rows
+//             64..127 over all columns at NB*NB/4 + (r-64)*NB + c.
+// A block GEMV walks a uniform index (column, resp. row) while the lanes take
+// consecutive rows (columns): every read is a scalar offset from the lane's
+// base, with consecutive lanes on consecutive addresses, and no entry outside
+// the stored quadrants is ever needed (the one wavefront whose range is
+// entirely above the diagonal skips).
+static_assert(kInvLayout == 3 * NB * NB / 4, "three quadrants per layout");
 
 // sum of p[k*stride] for k = k0, k0+step, ... < n in that order; the loads
 // are issued eight at a time (a loop with a run-time trip count would wait
@@ -337,13 +342,13 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
                                                   i64 ldb, double *vwork, i64 vstride,
                                                   unsigned *flags, i64 nflags,
                                                   const unsigned *epoch_ptr, int *err, int nrhs,
-                                                  const T *__restrict__ bmscr)
+                                                  const T *__restrict__ dinv)
 {
     __shared__ double xs[R][NB];
     __shared__ double ys[R][NB];
     __shared__ double part[2][R][NB];
-    __shared__ T Ld[kPacked + 1];
-    const unsigned epoch = *epoch_ptr;  // packed lower diagonal block, by columns; Ld[kZero] = 0
+    __shared__ T Lf[kInvLayout];
+    const unsigned epoch = *epoch_ptr;  // L11^-1, forward layout
     const int nrb = (nrhs + R - 1) / R;
     const i64 bid = blockIdx.x / nrb;
     const int rg = int(blockIdx.x % nrb), rhs0 = rg * R;
@@ -395,29 +400,9 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
         }
     }
     if (diag) {
-        for (int idx = tid; idx < nr * nr; idx += 256) {
-            const int rr = idx % nr, cc = idx / nr;
-            if (rr > cc) Ld[packed_col(cc) + (rr - cc)] = L[(r0 + rr) + i64(r0 + cc) * ld];
-        }
-        if (tid == 0) Ld[kZero] = T(0);
+        const T *inv = dinv + (w.flag0 + local) * i64(2 * kInvLayout);
+        for (int idx = tid; idx < kInvLayout; idx += 256) Lf[idx] = inv[idx];
     }
-    // Diagonal solve operands: with one right-hand side (the latency-bound
-    // case) loaded ahead of the waits, so that their latency stays off the
-    // chain of blocks; with a group, after the loop (registers).  Lanes
-    // 32..63 mirror lanes 0..31 (lane & 31) and do no work there.
-    const int sr = 32 * wave + (lane & 31);  // block row of this lane (lanes < 32 work)
-    const bool sw = diag && lane < 32 && sr < nr;
-    const int nbw = min(32, nr - 32 * wave);  // rows of this wave's sub-block
-    T lsub[32];
-    double dsr = 0.0;
-    auto load_bm = [&]() {
-        // column lane of the sub-block's Bm (rows past the sub-block zero)
-        const T *bm = bmscr + i64(n.first + r0 + 32 * wave) * kPanelNB + (lane & 31);
-#pragma unroll
-        for (int c = 0; c < 32; ++c) lsub[c] = (sw && c < nbw) ? bm[c * kPanelNB] : T(0);
-        dsr = sw ? dvec[n.first + r0 + sr] : 0.0;
-    };
-    if (R == 1 && bmscr) load_bm();
     const int nprev = diag ? local : w.nblk;
     double acc[R];
 #pragma unroll
@@ -456,83 +441,33 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
         }
         return;
     }
-    // Diagonal block: four 32 x 32 sub-blocks in sequence.  Wave w solves
-    // its sub-block, lane i < 32 holding row 32w + i: with the factor's
-    // inverses (bmscr: Bm = L11^-T D^-1 of every 32-column sub-block, so
-    // L11^-1 y = D Bm^T y) as a product x_i = d_i sum_c Bm(c, i) y_c, else by
-    // a readlane chain over the row of L11 in registers.  The waves below
-    // then add its products with their rows of the packed block in LDS to
-    // their pending sums, registers with one right-hand side (pend), LDS for
-    // a group (registers there would become hoisted loads and spills), so
-    // that one barrier per sub-block orders the steps.
-    if (bmscr) {
-        if (R != 1) load_bm();
-    } else {
-#pragma unroll
-        for (int c = 0; c < 32; ++c) {
-            const int cc = 32 * wave + c;
-            lsub[c] = Ld[(sw && c < (lane & 31)) ? packed_col(cc) + sr - cc : kZero];
-        }
-    }
+    // Diagonal block: x = L11^-1 y with the block's inverse in LDS, thread
+    // (ri, h) summing columns [64h, 64h + 64) of row ri (h is uniform in a
+    // wavefront: the column offsets are scalar arithmetic).
+    const int hs = __builtin_amdgcn_readfirstlane(h);
     if (tid < NB) {
 #pragma unroll
         for (int k = 0; k < R; ++k)
             ys[k][tid] = (tid < nr) ? ys[k][tid] - (part[0][k][tid] + part[1][k][tid]) : 0.0;
     }
     __syncthreads();
-    if (tid < NB) {  // part[0] holds a group's pending sums from here
+    // forward layout: columns [64hs, 64hs + 64) of row ri, stride cs
+    const bool above = hs == 1 && wave == 2;  // rows 0..63 of columns 64..127: zero
+    const T *Lrow = Lf + (hs == 0 ? ri : NB * NB / 2 + (ri - NH));
+    const int cs = hs == 0 ? NB : NH;
 #pragma unroll
-        for (int k = 0; k < R; ++k) part[0][k][tid] = 0.0;
+    for (int k = 0; k < R; ++k) {
+        double a0 = 0.0, a1 = 0.0;
+        if (!above) {
+#pragma unroll 16
+            for (int c = 0; c < NH; c += 2) {
+                a0 += double(Lrow[c * cs]) * ys[k][hs * NH + c];
+                a1 += double(Lrow[(c + 1) * cs]) * ys[k][hs * NH + c + 1];
+            }
+        }
+        part[h][k][ri] = a0 + a1;
     }
     __syncthreads();
-    double pend[R];
-#pragma unroll
-    for (int k = 0; k < R; ++k) pend[k] = 0.0;
-    for (int j = 0; j < 4 && 32 * j < nr; ++j) {
-        if (wave == j) {
-#pragma unroll
-            for (int k = 0; k < R; ++k) {
-                double x;
-                if (bmscr) {
-                    // the wave's own LDS stores precede its loads (in order per wave)
-                    if (lane < 32) ys[k][sr] -= (R == 1 ? pend[k] : part[0][k][sr]);
-                    __builtin_amdgcn_wave_barrier();
-                    double a0 = 0.0, a1 = 0.0;
-#pragma unroll
-                    for (int c = 0; c < 32; c += 2) {
-                        a0 += double(lsub[c]) * ys[k][32 * j + c];
-                        a1 += double(lsub[c + 1]) * ys[k][32 * j + c + 1];
-                    }
-                    x = dsr * (a0 + a1);
-                } else {
-                    x = (sw ? ys[k][sr] : 0.0) - (R == 1 ? pend[k] : part[0][k][sr]);
-#pragma unroll
-                    for (int c = 0; c < 32; ++c) x -= double(lsub[c]) * lane_bcast(x, c);
-                }
-                __builtin_amdgcn_wave_barrier();
-                if (lane < 32) ys[k][sr] = x;
-            }
-        }
-        __syncthreads();
-        if (wave > j && sw) {
-#pragma unroll
-            for (int k = 0; k < R; ++k) {
-                double s0 = 0.0, s1 = 0.0;
-#pragma unroll
-                for (int c = 0; c < 32; c += 2) {
-                    const int cc = 32 * j + c;
-                    const T a0 = Ld[cc < nr ? packed_col(cc) + sr - cc : kZero];
-                    const T a1 = Ld[cc + 1 < nr ? packed_col(cc + 1) + sr - cc - 1 : kZero];
-                    s0 += double(a0) * ys[k][cc];
-                    s1 += double(a1) * ys[k][cc + 1];
-                }
-                if constexpr (R == 1)
-                    pend[k] += s0 + s1;
-                else
-                    part[0][k][sr] += s0 + s1;
-            }
-        }
-    }
     if (wave == 0) {
         // results for the blocks that wait on this one first, then B
 #pragma unroll
@@ -540,7 +475,9 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
             if (lane + 64 * i < nr) {
 #pragma unroll
                 for (int k = 0; k < R; ++k)
-                    if (rhs0 + k < nrhs) store_shared(vwork + vo[k] + r0 + lane + 64 * i, ys[k][lane + 64 * i]);
+                    if (rhs0 + k < nrhs)
+                        store_shared(vwork + vo[k] + r0 + lane + 64 * i,
+                                     part[0][k][lane + 64 * i] + part[1][k][lane + 64 * i]);
             }
         publish_ready(fl + local, epoch);
 #pragma unroll
@@ -548,7 +485,9 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
             if (lane + 64 * i < nr) {
 #pragma unroll
                 for (int k = 0; k < R; ++k)
-                    if (rhs0 + k < nrhs) b[i64(rhs0 + k) * ldb + dst[i]] = ys[k][lane + 64 * i] / dpiv[i];
+                    if (rhs0 + k < nrhs)
+                        b[i64(rhs0 + k) * ldb + dst[i]] =
+                            (part[0][k][lane + 64 * i] + part[1][k][lane + 64 * i]) / dpiv[i];
             }
     }
 }
@@ -633,14 +572,13 @@ __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ w
                                                   const double *__restrict__ partials, i64 npart,
                                                   unsigned *flags, i64 nflags,
                                                   const unsigned *epoch_ptr, int *err, int nrhs,
-                                                  const T *__restrict__ bmscr,
-                                                  const double *__restrict__ dvec)
+                                                  const T *__restrict__ dinv)
 {
     __shared__ double xs[R][NB];
     __shared__ double ys[R][NB];
     __shared__ double part[2][R][NB];
-    __shared__ T Lr[kPacked + 1];
-    const unsigned epoch = *epoch_ptr;  // packed lower diagonal block, by rows; Lr[kZero] = 0
+    __shared__ T Lb[kInvLayout];
+    const unsigned epoch = *epoch_ptr;  // L11^-1, backward layout
     const int nrb = (nrhs + R - 1) / R;
     const i64 bid = blockIdx.x / nrb;
     const int rg = int(blockIdx.x % nrb), rhs0 = rg * R;
@@ -658,11 +596,8 @@ __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ w
 #pragma unroll
     for (int k = 0; k < R; ++k) vo[k] = i64(min(rhs0 + k, nrhs - 1)) * vstride + n.voff;
     const int c0 = blk * NB, nc = min(NB, S - c0);
-    for (int idx = tid; idx < nc * nc; idx += 256) {
-        const int rr = idx % nc, cc = idx / nc;
-        if (rr > cc) Lr[rr * (rr + 1) / 2 + cc] = L[(c0 + rr) + i64(c0 + cc) * ld];
-    }
-    if (tid == 0) Lr[kZero] = T(0);
+    const T *inv = dinv + (w.flag0 + blk) * i64(2 * kInvLayout) + kInvLayout;
+    for (int idx = tid; idx < kInvLayout; idx += 256) Lb[idx] = inv[idx];
     i32 dst[2] = {0, 0};
     if (wave == 0) {
 #pragma unroll
@@ -681,36 +616,17 @@ __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ w
     if (tid < NB) {
         const i32 gi = rows[c0 + min(tid, nc - 1)];
 #pragma unroll
-        for (int k = 0; k < R; ++k) {
+        for (int k = 0; k < R; ++k)
             ys[k][tid] = (tid < nc)
                              ? b[i64(min(rhs0 + k, nrhs - 1)) * ldb + gi] - (part[0][k][tid] + part[1][k][tid])
                              : 0.0;
-            part[0][k][tid] = 0.0;  // a group's pending sums in the diagonal solve
-        }
     }
-    // Diagonal solve operands: with one right-hand side (the latency-bound
-    // case) loaded ahead of the waits, so that their latency stays off the
-    // chain of blocks; with a group, after the loop (registers).  Lanes
-    // 32..63 mirror lanes 0..31 (lane & 31) and do no work there.
-    const int sc = 32 * wave + (lane & 31);  // block column of this lane (lanes < 32 work)
-    const bool sw = lane < 32 && sc < nc;
-    const int nbw = min(32, nc - 32 * wave);  // columns of this wave's sub-block
-    T lcol[32];
-    double dsc = 0.0;  // d of this lane's column (with the inverses)
-    auto load_bm = [&]() {
-        // row lane of the sub-block's Bm (columns past the sub-block zero)
-        const T *bm = bmscr + i64(n.first + c0 + sc) * kPanelNB;
-#pragma unroll
-        for (int r = 0; r < 32; ++r) lcol[r] = (sw && r < nbw) ? bm[r] : T(0);
-        dsc = sw ? dvec[n.first + c0 + sc] : 0.0;
-    };
-    if (R == 1 && bmscr) load_bm();
-    __syncthreads();
+    __syncthreads();  // ys complete before the column sums below update it
     // Products with the later blocks, L(p, blk)^T x_p: the thread sums
     // columns gc + 8j (j < 4) over rows 8q + gs (q < 16) of every block, so
     // that a load instruction reads 8 rows of 8 columns (8 cache lines where
     // a lane per column would touch 64 and be bound by address processing);
-    // the eight lanes of a column are added at the end (xor 1, 2, 4).
+    // the eight lanes of a column are added at the end (sum8).
     const int gs = lane & 7, gc = 32 * wave + (lane >> 3);
     double acc[4][R];
 #pragma unroll
@@ -748,89 +664,41 @@ __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ w
     for (int j = 0; j < 4; ++j) {
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            double v = acc[j][k];
-            v += __shfl_xor(v, 1);
-            v += __shfl_xor(v, 2);
-            v += __shfl_xor(v, 4);
+            const double v = sum8(acc[j][k]);
             if (gs == 0 && gc + 8 * j < nc) ys[k][gc + 8 * j] -= v;
         }
     }
     __syncthreads();
-    // Diagonal block, L11^T x = y: four 32 x 32 sub-blocks from the last.
-    // Wave w solves its sub-block, lane i < 32 holding column 32w + i: with
-    // the factor's inverses (L11^-T y = Bm D y) as a product
-    // x_i = sum_r Bm(i, r) d_r y_r, else by a readlane chain over the column
-    // of L11 in registers.  The waves to the left then add its products with
-    // their columns of the packed block in LDS to their pending sums,
-    // registers with one right-hand side (pend), LDS for a group (registers
-    // there would become hoisted loads and spills), so that one barrier per
-    // sub-block orders the steps.
-    if (bmscr) {
-        if (R != 1) load_bm();
-    } else {
+    // Diagonal block: x = L11^-T y with the block's inverse in LDS, thread
+    // (ci, h) summing rows [64h, 64h + 64) of column ci (h is uniform in a
+    // wavefront: the row offsets are scalar arithmetic).
+    const int hs = __builtin_amdgcn_readfirstlane(h);
+    // backward layout: rows [64hs, 64hs + 64) of column ci, stride rs
+    const bool above = hs == 0 && wave == 1;  // columns 64..127 of rows 0..63: zero
+    const T *Lcol = Lb + (hs == 0 ? ci : NB * NB / 4 + ci);
+    const int rs = hs == 0 ? NH : NB;
 #pragma unroll
-        for (int r = 0; r < 32; ++r) {
-            const int rr = 32 * wave + r;
-            lcol[r] = Lr[(sw && r > (lane & 31) && rr < nc) ? rr * (rr + 1) / 2 + sc : kZero];
-        }
-    }
-    double pend[R];
-#pragma unroll
-    for (int k = 0; k < R; ++k) pend[k] = 0.0;
-    for (int j = 3; j >= 0; --j) {
-        if (32 * j >= nc) continue;
-        if (wave == j) {
-#pragma unroll
-            for (int k = 0; k < R; ++k) {
-                double x;
-                if (bmscr) {
-                    // d_r y_r of the sub-block; the wave's own LDS stores
-                    // precede its loads (in order per wave)
-                    if (lane < 32) ys[k][sc] = dsc * (ys[k][sc] - (R == 1 ? pend[k] : part[0][k][sc]));
-                    __builtin_amdgcn_wave_barrier();
-                    double a0 = 0.0, a1 = 0.0;
-#pragma unroll
-                    for (int r = 0; r < 32; r += 2) {
-                        a0 += double(lcol[r]) * ys[k][32 * j + r];
-                        a1 += double(lcol[r + 1]) * ys[k][32 * j + r + 1];
-                    }
-                    x = a0 + a1;
-                } else {
-                    x = (sw ? ys[k][sc] : 0.0) - (R == 1 ? pend[k] : part[0][k][sc]);
-#pragma unroll
-                    for (int r = 31; r > 0; --r) x -= double(lcol[r]) * lane_bcast(x, r);
-                }
-                __builtin_amdgcn_wave_barrier();
-                if (lane < 32) ys[k][sc] = x;
+    for (int k = 0; k < R; ++k) {
+        double a0 = 0.0, a1 = 0.0;
+        if (!above) {
+#pragma unroll 16
+            for (int r = 0; r < NH; r += 2) {
+                a0 += double(Lcol[r * rs]) * ys[k][hs * NH + r];
+                a1 += double(Lcol[(r + 1) * rs]) * ys[k][hs * NH + r + 1];
             }
         }
-        __syncthreads();
-        if (wave < j && sw) {
-#pragma unroll
-            for (int k = 0; k < R; ++k) {
-                double s0 = 0.0, s1 = 0.0;
-#pragma unroll
-                for (int r = 0; r < 32; r += 2) {
-                    const int rr = 32 * j + r;
-                    const T a0 = Lr[rr < nc ? rr * (rr + 1) / 2 + sc : kZero];
-                    const T a1 = Lr[rr + 1 < nc ? (rr + 1) * (rr + 2) / 2 + sc : kZero];
-                    s0 += double(a0) * ys[k][rr];
-                    s1 += double(a1) * ys[k][rr + 1];
-                }
-                if constexpr (R == 1)
-                    pend[k] += s0 + s1;
-                else
-                    part[0][k][sc] += s0 + s1;
-            }
-        }
+        part[h][k][ci] = a0 + a1;
     }
+    __syncthreads();
     if (wave == 0) {
 #pragma unroll
         for (int i = 0; i < 2; ++i)
             if (lane + 64 * i < nc) {
 #pragma unroll
                 for (int k = 0; k < R; ++k)
-                    if (rhs0 + k < nrhs) store_shared(vwork + vo[k] + c0 + lane + 64 * i, ys[k][lane + 64 * i]);
+                    if (rhs0 + k < nrhs)
+                        store_shared(vwork + vo[k] + c0 + lane + 64 * i,
+                                     part[0][k][lane + 64 * i] + part[1][k][lane + 64 * i]);
             }
         publish_ready(fl + blk, epoch);
 #pragma unroll
@@ -838,8 +706,117 @@ __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ w
             if (lane + 64 * i < nc) {
 #pragma unroll
                 for (int k = 0; k < R; ++k)
-                    if (rhs0 + k < nrhs) b[i64(rhs0 + k) * ldb + dst[i]] = ys[k][lane + 64 * i];
+                    if (rhs0 + k < nrhs)
+                        b[i64(rhs0 + k) * ldb + dst[i]] = part[0][k][lane + 64 * i] + part[1][k][lane + 64 * i];
             }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// inverses of the wide diagonal blocks
+// ---------------------------------------------------------------------------
+// X = L11^-1 of column block blk of wide supernode w (unit lower triangular,
+// nc <= NB), computed in double by block rows of 32 and stored as T in the
+// forward and the backward layout (zero above the diagonal and past nc) at
+// dinv + 2 kInvLayout (flag0 + blk).  Block row I: A) T = -L(I, <I) X(<I, :) for the
+// columns left of the block row (the identity for its own columns), B)
+// X(I, :) = L(I, I)^-1 T by substitution, one column per thread.
+template <typename T>
+__global__ __launch_bounds__(256) void k_block_inverse(const WideDesc *__restrict__ wide, int nwide,
+                                                       const NodeInfo *__restrict__ nodes,
+                                                       const T *__restrict__ pool, T *dinv)
+{
+    __shared__ double X[NB][NB + 1];  // X[c][r]; odd stride: a column walk is conflict-free
+    __shared__ double Lii[32][33];    // the block row's diagonal 32 x 32 block of L
+    const i64 b = blockIdx.x;
+    int lo = 0, hi = nwide - 1;  // the supernode whose blocks hold b
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (wide[mid].flag0 <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const WideDesc w = wide[lo];
+    const int blk = int(b - w.flag0);
+    const NodeInfo n = nodes[w.node];
+    const T *L = pool + n.foff;
+    const i64 ld = n.ld;
+    const int c0 = blk * NB, nc = min(NB, n.S - c0);
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < NB * (NB + 1); idx += 256) (&X[0][0])[idx] = 0.0;
+    const int r = tid & 31, cg = tid >> 5;  // step A: row r of the block row, columns cg + 8m
+    for (int I = 0; 32 * I < nc; ++I) {
+        const int rI = 32 * I + r;
+        // the diagonal block of L (zero past nc and on and above the diagonal)
+        for (int idx = tid; idx < 32 * 32; idx += 256) {
+            const int i = idx & 31, k = idx >> 5;
+            const int gi = 32 * I + i, gk = 32 * I + k;
+            Lii[i][k] = (i > k && gi < nc) ? double(L[(c0 + gi) + i64(c0 + gk) * ld]) : 0.0;
+        }
+        __syncthreads();  // X rows above the block row and Lii complete
+        if (I > 0) {
+            double acc[12];
+#pragma unroll
+            for (int m = 0; m < 12; ++m) acc[m] = 0.0;
+            const T *Lr = L + (c0 + min(rI, nc - 1)) + i64(c0) * ld;
+            for (int k = 0; k < 32 * I; ++k) {
+                const double a = double(Lr[i64(k) * ld]);
+#pragma unroll
+                for (int m = 0; m < 12; ++m) acc[m] += a * X[cg + 8 * m][k];  // zero for k < c
+            }
+            if (rI < nc) {
+#pragma unroll
+                for (int m = 0; m < 12; ++m)
+                    if (cg + 8 * m < 32 * I) X[cg + 8 * m][rI] = -acc[m];
+            }
+        }
+        __syncthreads();
+        // B: columns c < 32I + 32 of the block row, one per thread
+        const int c = tid;
+        if (c < min(nc, 32 * I + 32)) {
+            double x[32];
+#pragma unroll
+            for (int i = 0; i < 32; ++i) x[i] = X[c][32 * I + i] + (c == 32 * I + i ? 1.0 : 0.0);
+            // right-looking, a column of Lii per step (zero on and above its
+            // diagonal, so every row takes the update); a rolled loop keeps
+            // the Lii loads from being hoisted all at once
+#pragma unroll 1
+            for (int k = 0; k < 31; ++k) {
+                double xk = 0.0;
+#pragma unroll
+                for (int i = 0; i < 32; ++i) xk = (i == k) ? x[i] : xk;
+#pragma unroll
+                for (int i = 0; i < 32; ++i) x[i] -= Lii[i][k] * xk;
+            }
+#pragma unroll
+            for (int i = 0; i < 32; ++i)
+                if (32 * I + i < nc) X[c][32 * I + i] = x[i];
+        }
+        __syncthreads();
+    }
+    // both layouts (X is zero above the diagonal; rows past nc were never
+    // written: zero)
+    T *out = dinv + (w.flag0 + blk) * i64(2 * kInvLayout);
+    for (int idx = tid; idx < kInvLayout; idx += 256) {
+        int rr, cc;
+        if (idx < NB * NB / 2) {
+            cc = idx >> 7, rr = idx & (NB - 1);
+        } else {
+            const int t = idx - NB * NB / 2;
+            cc = NH + (t >> 6), rr = NH + (t & (NH - 1));
+        }
+        out[idx] = T(X[cc][rr]);
+    }
+    for (int idx = tid; idx < kInvLayout; idx += 256) {
+        int rr, cc;
+        if (idx < NB * NB / 4) {
+            rr = idx >> 6, cc = idx & (NH - 1);
+        } else {
+            const int t = idx - NB * NB / 4;
+            rr = NH + (t >> 7), cc = t & (NB - 1);
+        }
+        out[kInvLayout + idx] = T(X[cc][rr]);
     }
 }
 
@@ -926,9 +903,9 @@ static int narrow_group(const Plan &p, int l, i64 nrhs)
 // computed them (in the panel-step scheme the scratch holds the factored
 // blocks instead and the solves use readlane chains).
 template <typename T>
-static const T *solve_inverses(sprgpu_handle *h)
+static const T *block_inverses(sprgpu_handle *h)
 {
-    return h->plan.params.block_factor ? static_cast<const T *>(h->d.dscr) : nullptr;
+    return static_cast<const T *>(h->d.dinv);
 }
 
 template <typename T, int R>
@@ -957,7 +934,7 @@ static void fwd_wide_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs
     k_fwd_wide<T, R><<<unsigned(p.wide_fwd_wgs[l]) * groups, 256, 0, h->stream>>>(
         d.wide + w0, nw, d.wide_fwd_prefix + w0, d.nodes, d.lindx, d.upd_ptr, d.upd_src, pool,
         d.dvec, b, ldb, d.vwork, p.vwork_elems, d.flags, std::max<i64>(p.nflags, 1), d.epoch,
-        d.spin_error, int(nrhs), solve_inverses<T>(h));
+        d.spin_error, int(nrhs), block_inverses<T>(h));
 }
 
 template <typename T, int R>
@@ -990,7 +967,7 @@ static void bwd_wide_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs
     k_bwd_wide<T, R><<<unsigned(p.wide_bwd_wgs[l]) * groups, 256, 0, h->stream>>>(
         d.wide + w0, nw, d.wide_bwd_prefix + w0, d.nodes, d.lindx, pool, b, ldb, d.vwork,
         p.vwork_elems, d.partials, npart, d.flags, std::max<i64>(p.nflags, 1), d.epoch,
-        d.spin_error, int(nrhs), solve_inverses<T>(h), d.dvec);
+        d.spin_error, int(nrhs), block_inverses<T>(h));
 }
 
 template <typename T>
@@ -1059,6 +1036,22 @@ static int solve_impl(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which)
 }
 
 int prepare_solve(sprgpu_handle *h, i64 nrhs) { return ensure_solve_buffers(h, nrhs); }
+
+int compute_block_inverses(sprgpu_handle *h)
+{
+    const Plan &p = h->plan;
+    if (p.nflags == 0) return 0;
+    if (h->precision == SPRGPU_FP32)
+        k_block_inverse<float><<<unsigned(p.nflags), 256, 0, h->stream>>>(
+            h->d.wide, int(p.wide.size()), h->d.nodes, static_cast<const float *>(h->d.pool),
+            static_cast<float *>(h->d.dinv));
+    else
+        k_block_inverse<double><<<unsigned(p.nflags), 256, 0, h->stream>>>(
+            h->d.wide, int(p.wide.size()), h->d.nodes, static_cast<const double *>(h->d.pool),
+            static_cast<double *>(h->d.dinv));
+    SPRGPU_HIP(hipGetLastError());
+    return 0;
+}
 
 int run_solve(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which)
 {

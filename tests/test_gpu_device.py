@@ -112,9 +112,7 @@ def test_many_rhs_match_single():
 def test_grouped_rhs_match_single_with_wide_supernodes(precision):
     """11 right-hand sides (a group of 8 and a partial group of 3 in every
     solve kernel) on hex8 12^3, whose top supernodes take the wide dataflow
-    path: every column equals its own single-RHS solve to 1e-13 relative
-    (the one-column backward kernel sums its block products in another
-    order; the groups of 8 and of 3 give identical bits)."""
+    path: every column equals its own single-RHS solve bit for bit."""
     torch = _torch()
     model, sysm, sm = make_system("hex8", 12, e_sigma=0.5, e_seed=1)
     assert np.diff(sysm.xsuper()).max() > 128
@@ -126,10 +124,8 @@ def test_grouped_rhs_match_single_with_wide_supernodes(precision):
     for k in range(11):
         xk = B[k:k + 1].clone()
         h.solve(xk)
-        rel = float(torch.linalg.norm(xk[0] - X[k]) / torch.linalg.norm(X[k]))
-        assert rel < 1e-13, (k, rel)
-    # the same columns in groups of 8 and in one call of 11: identical bits
-    X8 = B[:8].clone()
+        assert torch.equal(xk[0], X[k]), k
+    X8 = B[:8].clone()  # one full group
     h.solve(X8)
     assert torch.equal(X8, X[:8])
     h.close()
