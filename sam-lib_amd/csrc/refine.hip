@@ -42,7 +42,10 @@ __global__ __launch_bounds__(256) void k_spmv(const i64 *__restrict__ rowptr,
 }
 
 // gathers the CSR values from the packed assembled matrix and counts the
-// non-zeros found on or below the diagonal (each packed entry once)
+// non-zeros found on or below the diagonal (each packed entry once); sixteen
+// lanes per row, so that a row's index and value reads are coalesced and
+// many gathers are in flight
+constexpr int kGatherLanes = 16;
 __global__ __launch_bounds__(256) void k_gather_vals(const i64 *__restrict__ rowptr,
                                                      const i32 *__restrict__ col,
                                                      const i64 *__restrict__ src,
@@ -51,8 +54,12 @@ __global__ __launch_bounds__(256) void k_gather_vals(const i64 *__restrict__ row
                                                      unsigned long long *count)
 {
     unsigned long long c = 0;
-    for (i64 row = i64(blockIdx.x) * 256 + threadIdx.x; row < n; row += i64(gridDim.x) * 256) {
-        for (i64 k = rowptr[row]; k < rowptr[row + 1]; ++k) {
+    const int g = threadIdx.x % kGatherLanes;
+    const i64 stride = i64(gridDim.x) * (256 / kGatherLanes);
+    for (i64 row = i64(blockIdx.x) * (256 / kGatherLanes) + threadIdx.x / kGatherLanes; row < n;
+         row += stride) {
+        const i64 k1 = rowptr[row + 1];
+        for (i64 k = rowptr[row] + g; k < k1; k += kGatherLanes) {
             const double v = lnz[src[k]];
             val[k] = v;
             c += (v != 0.0 && col[k] <= row);
@@ -153,7 +160,10 @@ static int refresh_values(sprgpu_handle *h)
     }
     unsigned long long *cnt = reinterpret_cast<unsigned long long *>(&d.status->pad[0]);
     SPRGPU_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), h->stream));
-    k_gather_vals<<<1024, 256, 0, h->stream>>>(d.csr_rowptr, d.csr_col, d.csr_src,
+    const i64 rows_per_wg = 256 / kGatherLanes;
+    const unsigned gwg =
+        unsigned(std::max<i64>(1, std::min<i64>((h->plan.neq + rows_per_wg - 1) / rows_per_wg, 16384)));
+    k_gather_vals<<<gwg, 256, 0, h->stream>>>(d.csr_rowptr, d.csr_col, d.csr_src,
                                                h->last_sm_dev + h->plan.neq, d.csr_val,
                                                h->plan.neq, cnt);
     SPRGPU_HIP(hipGetLastError());

@@ -31,7 +31,9 @@ constexpr int kSmallF = 64;      // fronts with F <= kSmallF can be factored in 
 constexpr int kInitCols = 8;     // front columns per workgroup in the init kernel
 constexpr int kEaCols = 8;       // child columns per workgroup in extend-add
 constexpr int kSolveSmallS = 64;   // narrow supernodes: S <= this ...
-constexpr int kSolveSmallF = 3072; // ... and F <= this: one workgroup, vector in LDS
+constexpr int kSolveSmallF = 1024; // ... and F <= this: one workgroup, vector in LDS (taller
+                                   // fronts take the wide path, whose rows below split
+                                   // into parallel chunks)
 constexpr int kSolveNB = 128;      // column block (and rectangle row chunk) of the wide solve
 // The inverse of a wide diagonal block is kept in two zero-padded layouts of
 // kInvLayout entries each (the forward and the backward sweep's), i.e. the
