@@ -43,7 +43,7 @@ static void free_all(sprgpu_handle *h)
                     d.dscr,     d.dinv,
                     d.csr_src,  d.csr_val,    d.irwork,      d.level_nodes,
                     d.solve_small, d.wide,     d.wide_fwd_prefix, d.wide_bwd_prefix,
-                    d.wide_rect_prefix, d.upd_ptr, d.upd_src, d.flags, d.partials,
+                    d.wide_rect_prefix, d.wide_frect_prefix, d.upd_ptr, d.upd_src, d.flags, d.partials,
                     d.spin_error, d.col_node, d.epoch};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -65,6 +65,7 @@ static void free_plan_buffers(sprgpu_handle *h)
                      (void **)&d.level_nodes,   (void **)&d.solve_small,
                      (void **)&d.wide,          (void **)&d.wide_fwd_prefix,
                      (void **)&d.wide_bwd_prefix, (void **)&d.wide_rect_prefix,
+                     (void **)&d.wide_frect_prefix,
                      (void **)&d.upd_ptr,       (void **)&d.upd_src,
                      (void **)&d.col_node};
     for (void **q : ptrs) {
@@ -94,6 +95,7 @@ static int upload_plan(sprgpu_handle *h)
         (rc = upload(d.wide_fwd_prefix, p.wide_fwd_prefix, B)) ||
         (rc = upload(d.wide_bwd_prefix, p.wide_bwd_prefix, B)) ||
         (rc = upload(d.wide_rect_prefix, p.wide_rect_prefix, B)) ||
+        (rc = upload(d.wide_frect_prefix, p.wide_frect_prefix, B)) ||
         (rc = upload(d.upd_ptr, p.upd_ptr, B)) || (rc = upload(d.upd_src, p.upd_src, B)))
         return rc;
     return 0;

@@ -404,6 +404,7 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
     p.wide_fwd_prefix.clear();
     p.wide_bwd_prefix.clear();
     p.wide_rect_prefix.clear();
+    p.wide_frect_prefix.clear();
     p.solve_small_ptr.assign(p.nlevels + 1, 0);
     p.small_maxF.assign(p.nlevels, 0);
     p.small_maxS.assign(p.nlevels, 0);
@@ -411,10 +412,11 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
     p.wide_fwd_wgs.assign(p.nlevels, 0);
     p.wide_bwd_wgs.assign(p.nlevels, 0);
     p.wide_rect_wgs.assign(p.nlevels, 0);
+    p.wide_frect_wgs.assign(p.nlevels, 0);
     p.nflags = 0;
     p.npart = 0;
     for (int l = 0; l < p.nlevels; ++l) {
-        i64 wf = 0, wb = 0, wr = 0;
+        i64 wf = 0, wb = 0, wr = 0, wfr = 0;
         for (i32 q = p.level_ptr[l]; q < p.level_ptr[l + 1]; ++q) {
             const i32 j = p.level_nodes[q];
             const NodeInfo &n = p.nodes[j];
@@ -435,9 +437,11 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
             p.wide_fwd_prefix.push_back(wf);
             p.wide_bwd_prefix.push_back(wb);
             p.wide_rect_prefix.push_back(wr);
+            p.wide_frect_prefix.push_back(wfr);
             wf += w.nblk + w.nrect;
             wb += w.nblk;
             wr += i64(w.nblk) * w.nrect;
+            wfr += w.nrect;
             p.nflags += w.nblk;
             p.npart += i64(w.nblk) * w.nrect * kSolveNB;
         }
@@ -446,6 +450,7 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
         p.wide_fwd_wgs[l] = wf;
         p.wide_bwd_wgs[l] = wb;
         p.wide_rect_wgs[l] = wr;
+        p.wide_frect_wgs[l] = wfr;
     }
     // gather form of the forward assembly (inverse of the child maps)
     if (p.vwork_elems >= (i64(1) << 31)) return "solve workspace too large for 32-bit indices";

@@ -145,8 +145,10 @@ struct Plan {
     std::vector<i32> small_maxF, small_maxS;          // per level, for the LDS size
     std::vector<WideDesc> wide;                       // grouped by level
     std::vector<i64> wide_fwd_prefix, wide_bwd_prefix, wide_rect_prefix;  // launch-local
+    std::vector<i64> wide_frect_prefix;  // launch-local, the forward rectangle chunks alone
     std::vector<i32> wide_ptr;                        // nlevels+1
     std::vector<i64> wide_fwd_wgs, wide_bwd_wgs, wide_rect_wgs;  // workgroups per level
+    std::vector<i64> wide_frect_wgs;
     i64 nflags = 0;                                   // ready flags per right-hand side
     i64 npart = 0;                                    // backward partial sums per RHS
     // forward assembly as a gather: entry i of the solve workspace (node
@@ -212,6 +214,7 @@ struct DeviceBuffers {
     i32 *solve_small = nullptr;
     WideDesc *wide = nullptr;
     i64 *wide_fwd_prefix = nullptr, *wide_bwd_prefix = nullptr, *wide_rect_prefix = nullptr;
+    i64 *wide_frect_prefix = nullptr;
     i64 *upd_ptr = nullptr;
     i32 *upd_src = nullptr;
     unsigned *flags = nullptr;  // ready flags (nflags * nrhs), compared with an epoch

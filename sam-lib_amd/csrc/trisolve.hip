@@ -492,6 +492,192 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
     }
 }
 
+// The rows below the diagonal part of the wide supernodes as a launch of
+// their own, after the supernodes' block chains (k_fwd_wide over the
+// diagonal blocks only): used for groups of right-hand sides, where a wide
+// group (R up to 32) reads L21 fewer times and nothing waits.  Each
+// workgroup takes kSolveNB rows of one supernode and the same operations per
+// right-hand side as the rectangle workgroups of k_fwd_wide, so the update
+// vectors are identical bit for bit.
+template <typename T, int R>
+__global__ __launch_bounds__(256) void k_fwd_rect(const WideDesc *__restrict__ wide, int nwide,
+                                                  const i64 *__restrict__ prefix,
+                                                  const NodeInfo *__restrict__ nodes,
+                                                  const i64 *__restrict__ upd_ptr,
+                                                  const i32 *__restrict__ upd_src,
+                                                  const T *__restrict__ pool, double *vwork,
+                                                  i64 vstride, int nrhs)
+{
+    __shared__ double xs[R][NB];
+    __shared__ double ys[R][NB];
+    __shared__ double part[2][R][NB];
+    const int nrb = (nrhs + R - 1) / R;
+    const i64 bid = blockIdx.x / nrb;
+    const int rhs0 = int(blockIdx.x % nrb) * R;
+    const int q = find_desc(prefix, nwide, bid);
+    const WideDesc w = wide[q];
+    const int chunk = int(bid - prefix[q]);
+    const NodeInfo n = nodes[w.node];
+    const int tid = threadIdx.x;
+    const int S = n.S, F = n.F;
+    const T *L = pool + n.foff;
+    const i64 ld = n.ld;
+    const i64 *up = upd_ptr + n.voff;
+    i64 vo[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) vo[k] = i64(min(rhs0 + k, nrhs - 1)) * vstride + n.voff;
+    const int r0 = S + chunk * NB;
+    const int nr = min(NB, F - r0);
+    const int ri = tid & (NB - 1), h = tid >> 7;
+    const int row = r0 + ri;
+    const bool rv = ri < nr;
+    // the children's updates of my rows (no B entries below the diagonal part)
+    if (h == 0) {
+        const i64 k0 = rv ? up[row] : 0, k1 = rv ? up[row + 1] : 0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const double *vw = vwork + i64(min(rhs0 + k, nrhs - 1)) * vstride;
+            double y = 0.0;
+            for (i64 j = k0; j < k1; ++j) y += vw[upd_src[j]];
+            ys[k][ri] = y;
+        }
+    }
+    double acc[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) acc[k] = 0.0;
+    for (int p = 0; p < w.nblk; ++p) {
+        const int c0 = p * NB;
+        const int nc = min(NB, S - c0);
+        // unconditional loads as in k_fwd_wide (rows past the chunk read row
+        // r0, columns past the block its last column: xs is zero there)
+        T lv[NH];
+        const T *Lr = L + (rv ? row : r0) + i64(c0) * ld;
+#pragma unroll
+        for (int c = 0; c < NH; ++c) lv[c] = Lr[i64(min(h * NH + c, nc - 1)) * ld];
+        if (tid < NB) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) xs[k][tid] = (tid < nc) ? vwork[vo[k] + c0 + tid] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < NH; ++c)
+#pragma unroll
+            for (int k = 0; k < R; ++k) acc[k] += double(lv[c]) * xs[k][h * NH + c];
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) part[h][k][ri] = acc[k];
+    __syncthreads();
+    if (h == 0 && rv) {  // update for the parent
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            if (rhs0 + k < nrhs) vwork[vo[k] + row] = ys[k][ri] - (part[0][k][ri] + part[1][k][ri]);
+    }
+}
+
+// k_fwd_rect for groups of kRectWide right-hand sides, register-blocked:
+// thread (h, rg, ks) takes rows rg + 32i (i < 4) and right-hand sides
+// 8ks..8ks+7 of column half h, so that each x value loaded from LDS feeds
+// four rows.  Every (row, right-hand side) sum runs over the same columns in
+// the same order as in k_fwd_rect and k_fwd_wide, so the update vectors are
+// identical bit for bit.  The gathered children's updates go to the rows'
+// vector slots first and are read back at the end (LDS: x, then the half
+// sums, 64 KB).
+constexpr int kRectWide = 32;
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_rect_wide(const WideDesc *__restrict__ wide, int nwide,
+                                                       const i64 *__restrict__ prefix,
+                                                       const NodeInfo *__restrict__ nodes,
+                                                       const i64 *__restrict__ upd_ptr,
+                                                       const i32 *__restrict__ upd_src,
+                                                       const T *__restrict__ pool, double *vwork,
+                                                       i64 vstride, int nrhs)
+{
+    constexpr int R = kRectWide, KS = 8, CH = 16;  // RHS per thread, columns per load chunk
+    __shared__ double sbuf[2 * R * NB];             // xs[R][NB] in the loop, then part[2][R][NB]
+    double(*xs)[NB] = reinterpret_cast<double(*)[NB]>(sbuf);
+    double(*part)[R][NB] = reinterpret_cast<double(*)[R][NB]>(sbuf);
+    const int nrb = (nrhs + R - 1) / R;
+    const i64 bid = blockIdx.x / nrb;
+    const int rhs0 = int(blockIdx.x % nrb) * R;
+    const int q = find_desc(prefix, nwide, bid);
+    const WideDesc w = wide[q];
+    const int chunk = int(bid - prefix[q]);
+    const NodeInfo n = nodes[w.node];
+    const int tid = threadIdx.x;
+    const int S = n.S, F = n.F;
+    const T *L = pool + n.foff;
+    const i64 ld = n.ld;
+    const i64 *up = upd_ptr + n.voff;
+    const int r0 = S + chunk * NB;
+    const int nr = min(NB, F - r0);
+    // the children's updates of row ri, into its slot of every vector
+    {
+        const int ri = tid & (NB - 1);
+        if (tid < NB && ri < nr) {
+            const i64 k0 = up[r0 + ri], k1 = up[r0 + ri + 1];
+            for (int k = 0; k < R; ++k) {
+                if (rhs0 + k >= nrhs) break;
+                double *vw = vwork + i64(rhs0 + k) * vstride;
+                double y = 0.0;
+                for (i64 j = k0; j < k1; ++j) y += vw[upd_src[j]];
+                vw[n.voff + r0 + ri] = y;
+            }
+        }
+    }
+    const int h = tid >> 7, rg = tid & 31, ks = (tid >> 5) & 3;
+    i64 rofs[4];  // rows past the chunk read row r0 (their sums are not stored)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) rofs[i] = (rg + 32 * i < nr) ? r0 + rg + 32 * i : r0;
+    double acc[4][KS];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int k = 0; k < KS; ++k) acc[i][k] = 0.0;
+    for (int p = 0; p < w.nblk; ++p) {
+        const int c0 = p * NB;
+        const int nc = min(NB, S - c0);
+        if (tid < NB) {
+            for (int k = 0; k < R; ++k) {
+                const i64 rh = min(rhs0 + k, nrhs - 1);
+                xs[k][tid] = (tid < nc) ? vwork[rh * vstride + n.voff + c0 + tid] : 0.0;
+            }
+        }
+        __syncthreads();
+        for (int cc = 0; cc < NH / CH; ++cc) {
+            const int cb = h * NH + cc * CH;  // first column of the chunk (in the block)
+            T lv[4][CH];
+#pragma unroll
+            for (int c = 0; c < CH; ++c) {
+                const i64 co = i64(c0 + min(cb + c, nc - 1)) * ld;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) lv[i][c] = L[rofs[i] + co];
+            }
+#pragma unroll
+            for (int c = 0; c < CH; ++c)
+#pragma unroll
+                for (int k = 0; k < KS; ++k) {
+                    const double xv = xs[KS * ks + k][cb + c];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) acc[i][k] += double(lv[i][c]) * xv;
+                }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int k = 0; k < KS; ++k) part[h][KS * ks + k][rg + 32 * i] = acc[i][k];
+    __syncthreads();
+    if (tid < NB && tid < nr) {  // update for the parent
+        for (int k = 0; k < R; ++k) {
+            if (rhs0 + k >= nrhs) break;
+            double *slot = vwork + i64(rhs0 + k) * vstride + n.voff + r0 + tid;
+            *slot = *slot - (part[0][k][tid] + part[1][k][tid]);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // wide supernodes, backward
 // ---------------------------------------------------------------------------
@@ -923,18 +1109,40 @@ static void fwd_narrow_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nr
         d.dvec, b, ldb, d.vwork, p.vwork_elems, int(nrhs));
 }
 
+// The level's wide supernodes forward: the block chains with their rectangle
+// rows in the same launch (fused), or the block chains alone.
 template <typename T, int R>
-static void fwd_wide_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs)
+static void fwd_wide_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs, bool fused)
 {
     const Plan &p = h->plan;
     DeviceBuffers &d = h->d;
     const T *pool = static_cast<const T *>(d.pool);
     const unsigned groups = unsigned((nrhs + R - 1) / R);
     const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
-    k_fwd_wide<T, R><<<unsigned(p.wide_fwd_wgs[l]) * groups, 256, 0, h->stream>>>(
-        d.wide + w0, nw, d.wide_fwd_prefix + w0, d.nodes, d.lindx, d.upd_ptr, d.upd_src, pool,
-        d.dvec, b, ldb, d.vwork, p.vwork_elems, d.flags, std::max<i64>(p.nflags, 1), d.epoch,
-        d.spin_error, int(nrhs), block_inverses<T>(h));
+    const i64 wgs = fused ? p.wide_fwd_wgs[l] : p.wide_bwd_wgs[l];
+    const i64 *prefix = fused ? d.wide_fwd_prefix : d.wide_bwd_prefix;  // all blocks, or the chains'
+    k_fwd_wide<T, R><<<unsigned(wgs) * groups, 256, 0, h->stream>>>(
+        d.wide + w0, nw, prefix + w0, d.nodes, d.lindx, d.upd_ptr, d.upd_src, pool, d.dvec, b,
+        ldb, d.vwork, p.vwork_elems, d.flags, std::max<i64>(p.nflags, 1), d.epoch, d.spin_error,
+        int(nrhs), block_inverses<T>(h));
+}
+
+template <typename T, int R>
+static void fwd_rect_level(sprgpu_handle *h, int l, i64 nrhs)
+{
+    const Plan &p = h->plan;
+    DeviceBuffers &d = h->d;
+    if (p.wide_frect_wgs[l] == 0) return;
+    const unsigned groups = unsigned((nrhs + R - 1) / R);
+    const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
+    if constexpr (R == kRectWide)
+        k_fwd_rect_wide<T><<<unsigned(p.wide_frect_wgs[l]) * groups, 256, 0, h->stream>>>(
+            d.wide + w0, nw, d.wide_frect_prefix + w0, d.nodes, d.upd_ptr, d.upd_src,
+            static_cast<const T *>(d.pool), d.vwork, p.vwork_elems, int(nrhs));
+    else
+        k_fwd_rect<T, R><<<unsigned(p.wide_frect_wgs[l]) * groups, 256, 0, h->stream>>>(
+            d.wide + w0, nw, d.wide_frect_prefix + w0, d.nodes, d.upd_ptr, d.upd_src,
+            static_cast<const T *>(d.pool), d.vwork, p.vwork_elems, int(nrhs));
 }
 
 template <typename T, int R>
@@ -960,15 +1168,28 @@ static void bwd_wide_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs
     const unsigned groups = unsigned((nrhs + R - 1) / R);
     const i64 npart = std::max<i64>(p.npart, 1);
     const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
-    if (p.wide_rect_wgs[l] > 0)
-        k_bwd_rect<T, R><<<unsigned(p.wide_rect_wgs[l]) * groups, 256, 0, h->stream>>>(
-            d.wide + w0, nw, d.wide_rect_prefix + w0, d.nodes, d.lindx, pool, b, ldb, d.partials,
-            npart, int(nrhs));
     k_bwd_wide<T, R><<<unsigned(p.wide_bwd_wgs[l]) * groups, 256, 0, h->stream>>>(
         d.wide + w0, nw, d.wide_bwd_prefix + w0, d.nodes, d.lindx, pool, b, ldb, d.vwork,
         p.vwork_elems, d.partials, npart, d.flags, std::max<i64>(p.nflags, 1), d.epoch,
         d.spin_error, int(nrhs), block_inverses<T>(h));
 }
+
+template <typename T, int R>
+static void bwd_rect_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs)
+{
+    const Plan &p = h->plan;
+    DeviceBuffers &d = h->d;
+    if (p.wide_rect_wgs[l] == 0) return;
+    const unsigned groups = unsigned((nrhs + R - 1) / R);
+    const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
+    k_bwd_rect<T, R><<<unsigned(p.wide_rect_wgs[l]) * groups, 256, 0, h->stream>>>(
+        d.wide + w0, nw, d.wide_rect_prefix + w0, d.nodes, d.lindx, static_cast<const T *>(d.pool),
+        b, ldb, d.partials, std::max<i64>(p.npart, 1), int(nrhs));
+}
+
+// Right-hand sides per workgroup of the rectangle launches (no waits, L21
+// read once per group): kRectGroup when a call has that many.
+constexpr int kRectGroup = kRectWide;
 
 template <typename T>
 static int solve_chunk(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which)
@@ -987,8 +1208,13 @@ static int solve_chunk(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which
                 else fwd_narrow_level<T, 1>(h, l, b, ldb, nrhs);
             }
             if (p.wide_ptr[l + 1] > p.wide_ptr[l]) {
-                if (grouped) fwd_wide_level<T, kSolveGroup>(h, l, b, ldb, nrhs);
-                else fwd_wide_level<T, 1>(h, l, b, ldb, nrhs);
+                if (grouped) {  // the chains, then the rectangles in wider groups
+                    fwd_wide_level<T, kSolveGroup>(h, l, b, ldb, nrhs, false);
+                    if (nrhs >= kRectGroup) fwd_rect_level<T, kRectGroup>(h, l, nrhs);
+                    else fwd_rect_level<T, kSolveGroup>(h, l, nrhs);
+                } else {
+                    fwd_wide_level<T, 1>(h, l, b, ldb, nrhs, true);
+                }
             }
         }
     }
@@ -996,6 +1222,9 @@ static int solve_chunk(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which
         k_epoch_bump<<<1, 64, 0, s>>>(d.epoch);
         for (int l = p.nlevels - 1; l >= 0; --l) {
             if (p.wide_ptr[l + 1] > p.wide_ptr[l]) {
+                if (nrhs >= kRectGroup) bwd_rect_level<T, kRectGroup>(h, l, b, ldb, nrhs);
+                else if (grouped) bwd_rect_level<T, kSolveGroup>(h, l, b, ldb, nrhs);
+                else bwd_rect_level<T, 1>(h, l, b, ldb, nrhs);
                 if (grouped) bwd_wide_level<T, kSolveGroup>(h, l, b, ldb, nrhs);
                 else bwd_wide_level<T, 1>(h, l, b, ldb, nrhs);
             }

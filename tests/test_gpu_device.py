@@ -110,24 +110,28 @@ def test_many_rhs_match_single():
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_grouped_rhs_match_single_with_wide_supernodes(precision):
-    """11 right-hand sides (a group of 8 and a partial group of 3 in every
-    solve kernel) on hex8 12^3, whose top supernodes take the wide dataflow
-    path: every column equals its own single-RHS solve bit for bit."""
+    """Right-hand sides in groups on hex8 12^3, whose top supernodes take the
+    wide dataflow path: 37 of them (rectangle launches: a group of 32 and a
+    partial group of 5; block chains and narrow supernodes: four groups of 8
+    and a partial group of 5), 11 (groups of 8 throughout, the forward
+    rectangles in their own launch) and single columns (the forward
+    rectangles fused with the chains).  Every column equals its own
+    single-RHS solve bit for bit."""
     torch = _torch()
     model, sysm, sm = make_system("hex8", 12, e_sigma=0.5, e_seed=1)
     assert np.diff(sysm.xsuper()).max() > 128
     h, sm_dev = _handle(sysm, sm, precision)
     assert h.factor(sm_dev)["ierr"] == 0
-    B = torch.randn(11, sysm.neq, dtype=torch.float64, device="cuda")
+    B = torch.randn(37, sysm.neq, dtype=torch.float64, device="cuda")
     X = B.clone()
     h.solve(X)
-    for k in range(11):
+    for k in range(37):
         xk = B[k:k + 1].clone()
         h.solve(xk)
         assert torch.equal(xk[0], X[k]), k
-    X8 = B[:8].clone()  # one full group
-    h.solve(X8)
-    assert torch.equal(X8, X[:8])
+    X11 = B[:11].clone()
+    h.solve(X11)
+    assert torch.equal(X11, X[:11])
     h.close()
 
 
