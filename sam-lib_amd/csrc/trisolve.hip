@@ -683,7 +683,12 @@ __global__ __launch_bounds__(256) void k_fwd_rect_wide(const WideDesc *__restric
 // ---------------------------------------------------------------------------
 // Partial sums of L21^T x2: workgroup (blk, k) multiplies rows [S+k*NB, +NB)
 // of the columns of block blk by the solution at those rows (final: they
-// belong to ancestors).  Two threads per column, each walking half the rows.
+// belong to ancestors).  Same mapping as the block products of k_bwd_wide:
+// the thread sums columns gc + 8j (j < 4) over rows 8q + gs (q < 16), so a
+// load instruction reads 8 rows of 8 columns, and the eight lanes of a
+// column are added with sum8.  The L values stay in registers while the
+// right-hand sides pass in groups of at most 8; every column's sum takes the
+// same operations whatever R is.
 template <typename T, int R>
 __global__ __launch_bounds__(256) void k_bwd_rect(const WideDesc *__restrict__ wide, int nwide,
                                                   const i64 *__restrict__ prefix,
@@ -693,8 +698,9 @@ __global__ __launch_bounds__(256) void k_bwd_rect(const WideDesc *__restrict__ w
                                                   const double *__restrict__ b, i64 ldb,
                                                   double *partials, i64 npart, int nrhs)
 {
+    constexpr int KS = R < 8 ? R : 8;  // right-hand sides per pass
+    static_assert(R % KS == 0, "whole passes");
     __shared__ double xs[R][NB];
-    __shared__ double part[2][R][NB];
     const int nrb = (nrhs + R - 1) / R;
     const i64 bid = blockIdx.x / nrb;
     const int rhs0 = int(blockIdx.x % nrb) * R;
@@ -703,7 +709,7 @@ __global__ __launch_bounds__(256) void k_bwd_rect(const WideDesc *__restrict__ w
     const int t = int(bid - prefix[q]);
     const int blk = t / w.nrect, k = t % w.nrect;
     const NodeInfo n = nodes[w.node];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int S = n.S, F = n.F;
     const int c0 = blk * NB, nc = min(NB, S - c0);
     const int r0 = S + k * NB, nr = min(NB, F - r0);
@@ -715,33 +721,41 @@ __global__ __launch_bounds__(256) void k_bwd_rect(const WideDesc *__restrict__ w
         for (int j = 0; j < R; ++j)
             xs[j][tid] = (tid < nr) ? b[i64(min(rhs0 + j, nrhs - 1)) * ldb + gi] : 0.0;
     }
-    __syncthreads();
-    const int ci = tid & (NB - 1), h = tid >> 7;
     // unconditional loads (columns past the block read its last column and
     // are not stored; rows past the chunk read its last row, xs is zero)
-    T lv[NH];
-    const T *Lc = L + r0 + i64(c0 + min(ci, nc - 1)) * ld;
+    const int gs = lane & 7, gc = 32 * wave + (lane >> 3);
+    T lv[4][NB / 8];
 #pragma unroll
-    for (int r = 0; r < NH; ++r) lv[r] = Lc[min(h * NH + r, nr - 1)];
+    for (int j = 0; j < 4; ++j) {
+        const T *Lc = L + r0 + i64(c0 + min(gc + 8 * j, nc - 1)) * ld;
 #pragma unroll
-    for (int j = 0; j < R; ++j) {
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-#pragma unroll
-        for (int r = 0; r < NH; r += 4) {
-            a0 += double(lv[r]) * xs[j][h * NH + r];
-            a1 += double(lv[r + 1]) * xs[j][h * NH + r + 1];
-            a2 += double(lv[r + 2]) * xs[j][h * NH + r + 2];
-            a3 += double(lv[r + 3]) * xs[j][h * NH + r + 3];
-        }
-        part[h][j][ci] = (a0 + a1) + (a2 + a3);
+        for (int qq = 0; qq < NB / 8; ++qq) lv[j][qq] = Lc[min(8 * qq + gs, nr - 1)];
     }
     __syncthreads();
-    if (tid < nc) {
+    double *dst = partials + w.pbase + (i64(blk) * w.nrect + k) * NB;
+#pragma unroll 1
+    for (int g = 0; g < R; g += KS) {
+        double acc[4][KS];
 #pragma unroll
-        for (int j = 0; j < R; ++j)
-            if (rhs0 + j < nrhs)
-                partials[i64(rhs0 + j) * npart + w.pbase + (i64(blk) * w.nrect + k) * NB + tid] =
-                    part[0][j][tid] + part[1][j][tid];
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < KS; ++r) acc[j][r] = 0.0;
+#pragma unroll 4
+        for (int qq = 0; qq < NB / 8; ++qq)
+#pragma unroll
+            for (int r = 0; r < KS; ++r) {
+                const double xv = xs[g + r][8 * qq + gs];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[j][r] += double(lv[j][qq]) * xv;
+            }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < KS; ++r) {
+                const double v = sum8(acc[j][r]);
+                if (gs == 0 && gc + 8 * j < nc && rhs0 + g + r < nrhs)
+                    dst[i64(rhs0 + g + r) * npart + gc + 8 * j] = v;
+            }
     }
 }
 
