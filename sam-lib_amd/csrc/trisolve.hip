@@ -1084,8 +1084,9 @@ static unsigned narrow_threads(const Plan &p, int l)
 }
 // Right-hand sides per workgroup of a level's narrow launches: the larger of
 // kSolveGroup and 4 whose LDS fits in kNarrowLds, or 1 below kSolveGroup
-// right-hand sides.  (Larger groups on the smallest fronts were measured
-// slower: one wavefront's chains and gathers for them run in sequence.)
+// right-hand sides.  (Larger groups were measured slower, 32 on quad4 1000^2
+// with 128 right-hand sides by 1.6x: a workgroup's gathers and triangular
+// chains for its right-hand sides run in sequence.)
 template <typename T>
 static int narrow_group(const Plan &p, int l, i64 nrhs)
 {
@@ -1099,9 +1100,7 @@ static int narrow_group(const Plan &p, int l, i64 nrhs)
     return 1;
 }
 
-// The factor's 32 x 32 diagonal-block inverses, when the outer-block scheme
-// computed them (in the panel-step scheme the scratch holds the factored
-// blocks instead and the solves use readlane chains).
+// The inverses of the wide solve's 128 x 128 diagonal blocks (k_block_inverse).
 template <typename T>
 static const T *block_inverses(sprgpu_handle *h)
 {
