@@ -201,6 +201,17 @@ def test_pivot_rules_inside_blocked_fronts(block, mode):
         f = h.get_factor()
         assert np.max(np.abs(f[:sysm.neq] / sm_o[:sysm.neq] - 1)) < 1e-9
         assert np.linalg.norm(f[sysm.neq:] - sm_o[sysm.neq:]) / np.linalg.norm(sm_o[sysm.neq:]) < 1e-9
+        # solves with the replaced-pivot factor (the wide blocks through their
+        # explicit inverses) against the oracle's solve with its own factor
+        b = np.random.default_rng(7).uniform(-1, 1, (sysm.neq, 2))
+        x_o = np.asfortranarray(b.copy())
+        ierr_s, _ = oracle.sprsol(4, ms_o, sysm.mtrees, sysm.msifa, sm_o, x_o)  # FS + BS
+        assert ierr_s == 0
+        x = torch.from_numpy(np.ascontiguousarray(b.T)).cuda()
+        h.solve(x)
+        rel = np.linalg.norm(x.cpu().numpy().T - x_o) / np.linalg.norm(x_o)
+        print(f"solve with replaced pivots: rel {rel:.2e}", flush=True)
+        assert rel < 1e-8, rel
     h.close()
 
 
