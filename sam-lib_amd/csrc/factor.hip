@@ -357,7 +357,8 @@ __device__ __forceinline__ double recip(double d)
 // threads of row j write their part of row j of X = L11^-1; after one
 // barrier every thread reads the pivot, its row's entry and the four
 // column-j values and X(j, :) values it needs, and updates its entries
-// unmasked (the zeros leave finished columns unchanged).  X is built as the
+// unmasked (the zeros leave finished columns unchanged).  Steps go two
+// columns per barrier (see the loop).  X is built as the
 // product of the elimination steps, X(r, :) -= l_rj X(j, :), in the same
 // loop.  The pivots are recorded (when record is set) by one thread per row
 // after the loop.  Bm = L11^-T D^-1 = X^T D^-1 (upper triangular, zero
@@ -368,7 +369,7 @@ __device__ __forceinline__ double recip(double d)
 // kPanelNB doubles of LDS scratch (16-byte aligned).  Returns the pivot d_r of the thread's row (zero for
 // r >= nb).  Every thread of the workgroup calls it.
 constexpr int kChainThreads = 256;
-constexpr int kLshSize = 7 * kPanelNB;  // LDS scratch elements of T (plus kPanelNB doubles)
+constexpr int kLshSize = 11 * kPanelNB;  // LDS scratch elements of T (plus kPanelNB doubles)
 
 template <typename T>
 __device__ __forceinline__ T diag_block_factor(const T *src, i64 srs, i64 scs, int nb, int eq0,
@@ -384,7 +385,7 @@ __device__ __forceinline__ T diag_block_factor(const T *src, i64 srs, i64 scs, i
     T *pd = lsh;                 // [NB] pivots d_j
     T *pi = lsh + NB;            // [NB] 1/d_j
     T *pin = lsh + 2 * NB;       // [NB] pivots before the replacement rule
-    double *ash = reinterpret_cast<double *>(lsh + 7 * NB);  // [NB] a_jj
+    double *ash = reinterpret_cast<double *>(lsh + 11 * NB);  // [NB] a_jj
     if (t < nb) ash[t] = adiag[eq0 + t];
     T a[Q], x[Q];
 #pragma unroll
@@ -392,10 +393,96 @@ __device__ __forceinline__ T diag_block_factor(const T *src, i64 srs, i64 scs, i
         a[q] = (r < nb && c0 + q <= r) ? src[r * srs + (c0 + q) * scs] : T(0);
         x[q] = (c0 + q == r) ? T(1) : T(0);
     }
-    for (int j = 0; j < nb; ++j) {
-        // column j and row j of X, double-buffered: a buffer is rewritten two
-        // steps later, after every thread has passed the next barrier
-        T *wcol = lsh + (3 + 2 * (j & 1)) * NB, *xrow = wcol + NB;
+    // Two columns per barrier: after columns j and j+1 and rows j and j+1 of
+    // X are in LDS, every thread takes step j and then step j+1 from the
+    // column j+1 and X row j+1 that step j leaves, computed with the same
+    // operations as the threads holding them would (so the factor is the
+    // same as one column per barrier).  Buffers alternate per pair: one is
+    // rewritten two pairs later, after every thread has passed the next
+    // barrier.
+    int j = 0;
+    for (; j + 1 < nb; j += 2) {
+        T *wa = lsh + (3 + 4 * ((j >> 1) & 1)) * NB, *wb = wa + NB, *xa = wb + NB, *xb = xa + NB;
+        if (g == (j >> 2)) {  // j is even: both columns belong to group j >> 2
+            const bool hi = (j & 3) == 2;
+            const T va = hi ? a[2] : a[0], vb = hi ? a[3] : a[1];
+            wa[r] = (r >= j) ? va : T(0);
+            wb[r] = (r >= j + 1) ? vb : T(0);
+        }
+        if (r == j || r == j + 1) {
+            vec_t xv;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) xv[q] = x[q];
+            *reinterpret_cast<vec_t *>((r == j ? xa : xb) + c0) = xv;
+        }
+        __syncthreads();
+        // step j
+        const T dj = wa[j];
+        const T aj1 = wa[j + 1];  // A(j+1, j)
+        const T dj1r = wb[j + 1];
+        const T wra = wa[r], wrb = wb[r];
+        const vec_t wca = *reinterpret_cast<const vec_t *>(wa + c0);
+        const vec_t wcb = *reinterpret_cast<const vec_t *>(wb + c0);
+        const vec_t xja = *reinterpret_cast<const vec_t *>(xa + c0);
+        const vec_t xjb = *reinterpret_cast<const vec_t *>(xb + c0);
+        const double dp = pivot_value(double(dj), ash[j], pa);
+        const T d = T(dp);
+        const T di = recip(d);
+        const T la = (r > j) ? wra * di : T(0);
+        // column j+1 and row j+1 of X after step j
+        const T lj1 = aj1 * di;  // l(j+1, j)
+        T dj1 = dj1r;
+        dj1 -= lj1 * aj1;
+        T wrb1 = wrb;
+        wrb1 -= la * aj1;
+        vec_t wcb1, xjb1;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int c = c0 + q;
+            const T lc = (c > j) ? wca[q] * di : T(0);
+            T w = wcb[q];
+            w -= lc * aj1;
+            wcb1[q] = (c >= j + 1) ? w : T(0);
+            T xv = xjb[q];
+            xv -= lj1 * xja[q];
+            xjb1[q] = xv;
+        }
+        // step j+1
+        const double dp1 = pivot_value(double(dj1), ash[j + 1], pa);
+        const T d1 = T(dp1);
+        const T di1 = recip(d1);
+        const T lb = (r > j + 1) ? wrb1 * di1 : T(0);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            a[q] -= la * wca[q];
+            x[q] -= la * xja[q];
+        }
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            a[q] -= lb * wcb1[q];
+            x[q] -= lb * xjb1[q];
+        }
+        if (g == (j >> 2)) {
+            const T fa = (r == j) ? d : la, fb = (r == j + 1) ? d1 : lb;
+            if ((j & 3) == 2) {
+                a[2] = fa;
+                a[3] = fb;
+            } else {
+                a[0] = fa;
+                a[1] = fb;
+            }
+        }
+        if (t == 0) {
+            pd[j] = d;
+            pi[j] = di;
+            pin[j] = dj;
+            pd[j + 1] = d1;
+            pi[j + 1] = di1;
+            pin[j + 1] = dj1;
+        }
+    }
+    if (j < nb) {  // an odd last column, one step
+        T *wcol = lsh + (3 + 4 * ((j >> 1) & 1)) * NB, *xrow = wcol + 2 * NB;
         if (g == (j >> 2)) {
             T v = a[0];
 #pragma unroll
