@@ -245,8 +245,9 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
                    i64(p.small_nodes.size()) - off, 0.0, maxf_small);
         // 4. blocked fronts: kPanelOuter-wide outer blocks, each factored by
         //    k_block_diag + k_block_trsm and followed by a rank-kPanelOuter
-        //    update of the columns right of it; then one Schur-complement
-        //    update of the U block with K = S.
+        //    update of the columns right of it; the update (U) block takes
+        //    each kOuterRank-wide group's update with the group's far update,
+        //    and a last partial group's at the end of the level.
         if (p.params.block_factor) {
             i32 maxblk = 0;
             for (i32 j : big) maxblk = std::max(maxblk, (p.nodes[j].S + kPanelOuter - 1) / kPanelOuter);
@@ -271,10 +272,12 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
                 // earlier block of the group: a rank-kPanelOuter update of the
                 // group's remaining columns.  After its last block: the
                 // rank-kOuterRank update of the next group's columns ("near",
-                // on the handle's stream) and of the columns beyond it
-                // ("far", on the second stream).  The far update runs beside
-                // the next group's diagonal blocks, which do not touch its
-                // columns; the next near update, which does, waits for it.
+                // on the handle's stream) and of the columns beyond it,
+                // through the U block ("far", on the second stream).  The far
+                // update runs beside the next group's diagonal blocks, which
+                // do not touch its columns; the next near update, which does,
+                // waits for it, and nothing else reads the U block before the
+                // level's end.
                 const i32 g0 = ob / kOuterRank * kOuterRank, gend = g0 + kOuterRank;
                 const bool last_in_group = oend >= gend;
                 auto gemm_launch = [&](i32 stream, auto range) {
@@ -285,7 +288,8 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
                         const NodeInfo &n = p.nodes[j];
                         i32 r0, cend;
                         if (!range(n, r0, cend)) continue;
-                        const i32 kn = last_in_group ? oend - g0 : kPanelOuter;
+                        // a group that runs past S ends at S
+                        const i32 kn = last_in_group ? std::min(oend, n.S) - g0 : kPanelOuter;
                         const i32 tile = last_in_group ? p.big_tile : kTileSmall;
                         const GemmDesc g = make_gemm(n, r0, n.F, cend, last_in_group ? g0 : ob, kn, tile);
                         p.gemm_desc.push_back(g);
@@ -313,9 +317,9 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
                     });
                     add_sync(L_WAIT, l, 1);
                     gemm_launch(1, [&](const NodeInfo &n, i32 &r0, i32 &cend) {
-                        r0 = oend + kOuterRank;
-                        cend = n.S;
-                        return r0 < n.S;
+                        r0 = std::min(oend + kOuterRank, n.S);
+                        cend = n.F;
+                        return ob < n.S && r0 < n.F;
                     });
                     add_sync(L_RECORD, l, 1);
                 }
@@ -380,14 +384,24 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
                 add_launch(L_GEMM, l, off, i64(p.gemm_desc.size()) - off, ntiles, fl, p.big_tile, 1);
             }
         }
-        // Schur complement of every blocked front of this level
+        // Schur complement of every blocked front of this level: with the
+        // panel steps in one update with K = S; with outer blocks only the
+        // columns of a last group that ended before its kOuterRank-th column
+        // (the complete groups' far updates took the rest)
         off = i64(p.gemm_desc.size());
         i64 ntiles = 0;
         double fl = 0.0;
         for (i32 j : big) {
             const NodeInfo &n = p.nodes[j];
             if (n.F == n.S) continue;
-            const GemmDesc g = make_gemm(n, n.S, n.F, n.F, 0, n.S, p.big_tile);
+            i32 k0 = 0;
+            if (p.params.block_factor) {
+                const i32 ob = (n.S - 1) / kPanelOuter * kPanelOuter;
+                const i32 g0 = ob / kOuterRank * kOuterRank;
+                if (ob + kPanelOuter >= g0 + kOuterRank) continue;  // ended with a group update
+                k0 = g0;
+            }
+            const GemmDesc g = make_gemm(n, n.S, n.F, n.F, k0, n.S - k0, p.big_tile);
             p.gemm_desc.push_back(g);
             p.gemm_prefix.push_back(ntiles);
             ntiles += tri_tiles(g.nti, g.ntj);
