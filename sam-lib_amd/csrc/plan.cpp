@@ -275,9 +275,9 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
                 // on the handle's stream) and of the columns beyond it,
                 // through the U block ("far", on the second stream).  The far
                 // update runs beside the next group's diagonal blocks, which
-                // do not touch its columns; the next near update, which does,
-                // waits for it, and nothing else reads the U block before the
-                // level's end.
+                // do not touch its columns; the next near update waits for
+                // the part of it that covers the same columns, and nothing
+                // else reads the U block before the level's end.
                 const i32 g0 = ob / kOuterRank * kOuterRank, gend = g0 + kOuterRank;
                 const bool last_in_group = oend >= gend;
                 auto gemm_launch = [&](i32 stream, auto range) {
@@ -316,14 +316,23 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
                         return oend < n.S;
                     });
                     add_sync(L_WAIT, l, 1);
+                    // far, in two launches: the group after next (which the
+                    // next near update overwrites, so it waits for this part
+                    // alone), then everything beyond it
                     gemm_launch(1, [&](const NodeInfo &n, i32 &r0, i32 &cend) {
-                        r0 = std::min(oend + kOuterRank, n.S);
+                        r0 = oend + kOuterRank;
+                        cend = std::min(n.S, oend + 2 * kOuterRank);
+                        return r0 < n.S;
+                    });
+                    add_sync(L_RECORD, l, 1);
+                    gemm_launch(1, [&](const NodeInfo &n, i32 &r0, i32 &cend) {
+                        r0 = std::min(oend + 2 * kOuterRank, n.S);
                         cend = n.F;
                         return ob < n.S && r0 < n.F;
                     });
-                    add_sync(L_RECORD, l, 1);
                 }
             }
+            add_sync(L_RECORD, l, 1);
             add_sync(L_WAIT, l, 0);  // every far update of the level is done
         }
         //    (or: panel steps with trailing updates inside the outer block)
