@@ -46,11 +46,56 @@ def build_problem(n):
     f = rng.uniform(-1.0, 1.0, model.ndof)    # then the load vector
     t = time.perf_counter()
     sysm = host.symbolic(model)
+    t_sym = time.perf_counter() - t
+    t = time.perf_counter()
     sm = host.assemble(model, sysm)
+    t_asm = time.perf_counter() - t
     b = host.loads_to_equations(sysm, f)[:, 0].copy()
     log(f"set-up {n}^3: neq={sysm.neq} nsuper={sysm.nsuper} nzerol={sysm.nzerol} "
-        f"mfront={sysm.mfront} flops={host.factor_flops(sysm):.4e} ({time.perf_counter() - t:.1f}s)")
-    return model, sysm, sm, b
+        f"mfront={sysm.mfront} flops={host.factor_flops(sysm):.4e} (symbolic {t_sym:.1f}s, "
+        f"host SPRADM loop {t_asm:.1f}s)")
+    return model, sysm, sm, b, t_asm
+
+
+def bench_assembly(model, sysm, sm, h, reps, t_host):
+    """Element assembly of the same SM on the device (sprgpu_set_assembly once,
+    then sprgpu_assemble): bit equality with the host SPRADM loop's SM, and the
+    time per assembly from HIP events on the handle's stream (the clearing of
+    SM included).  Algorithmic bytes: the cleared SM, per LNZ slot with element
+    entries its position, range and result (24 B), per element entry its map
+    word, element offset, scale and matrix value (32 B)."""
+    import torch
+    from sprgpu import host
+
+    t = time.perf_counter()
+    mp, me = host.element_meen(model, sysm)
+    h.set_assembly(mp, me)
+    t_map = time.perf_counter() - t
+    em = torch.from_numpy(model.em).cuda()
+    off = torch.from_numpy(model.em_off).cuda()
+    sc = torch.from_numpy(model.escale).cuda()
+    out = torch.empty(sysm.neq + sysm.nzerol, dtype=torch.float64, device="cuda")
+    h.assemble(em, out, off, sc)
+    torch.cuda.synchronize()
+    same = bool(torch.equal(out.view(torch.int64).cpu(), torch.from_numpy(sm).view(torch.int64)))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        h.assemble(em, out, off, sc)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del out
+    nf = np.add.reduceat(me > 0, mp[:-1]).astype(np.int64)
+    ncontrib = int(np.sum(nf * (nf + 1) // 2))
+    nslots = int(np.count_nonzero(sm[sysm.neq:]))
+    nbytes = 8 * (sysm.neq + sysm.nzerol) + 24 * nslots + 32 * ncontrib
+    log(f"device assembly {ms:.3f} ms (bit-equal to SPRADM: {same}; map {t_map:.1f}s; "
+        f"host SPRADM loop {t_host:.1f}s)")
+    return {"device_ms": ms, "bit_equal_to_host_spradm": same, "map_setup_s": t_map,
+            "host_spradm_loop_s": t_host, "element_entries": ncontrib, "lnz_slots": nslots,
+            "algorithmic_bytes": nbytes, "achieved_GBps": nbytes / (ms * 1e-3) / 1e9,
+            "frac_hbm": nbytes / (ms * 1e-3) / 8e12}
 
 
 def cpu_baseline():
@@ -123,7 +168,7 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
-    model, sysm, sm, b = build_problem(args.n)
+    model, sysm, sm, b, t_host_asm = build_problem(args.n)
     flops = host.factor_flops(sysm)
     t = time.perf_counter()
     sm_dev = torch.from_numpy(sm).to(dev)
@@ -226,6 +271,10 @@ def main():
         "cpu_baseline": None,
     }
 
+    # element assembly on the device (after the timed steps: it replaces the
+    # FP64 handle's residual pattern by the map's, which that handle no longer uses)
+    result["assembly"] = bench_assembly(model, sysm, sm, h64, max(args.steps, 5), t_host_asm)
+
     if not args.no_profile:
         import ctypes as C
         from sprgpu import _lib
@@ -277,7 +326,7 @@ def bench_adjoint(args):
     """Compliance L = f^T u on hex8 n^3, E log-normal(0,0.5) seed 4, loads U[-1,1]
     drawn after the moduli.  Forward: FP32 factor + FP64 refinement (rtol);
     backward: lambda from the same factor, then dL/dE_e on the device.  Timed
-    per pass with the host assembly outside.  Checked against cancellation-free
+    per pass; the forward pass includes the device assembly of K(E).  Checked against cancellation-free
     central differences (h = 1e-6 E_e) on --fd random elements (seed 4) and,
     when tests/golden holds it, the reference's FP64 CPU adjoint."""
     import torch
@@ -298,7 +347,7 @@ def bench_adjoint(args):
     f_eq = sysobj.to_eq(ft)
 
     def forward():
-        sysobj.factor(E, sysobj.sm)
+        sysobj.factor(E)  # device assembly + factor
         return sysobj.solve_eq(f_eq)
 
     def backward(x_eq):
@@ -375,7 +424,7 @@ def bench_many_rhs(args):
     nrhs = 128
     t = time.perf_counter()
     if args.config == "rhs128":
-        model, sysm, sm, _ = build_problem(args.n)
+        model, sysm, sm, _, _ = build_problem(args.n)
         label = f"hex8 {args.n}^3, E log-normal(0,0.5) seed 1"
     else:
         n = 1000 if args.n == 40 else args.n

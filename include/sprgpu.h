@@ -109,6 +109,33 @@ int sprgpu_set_stream(sprgpu_handle *handle, void *stream);
  * outside this pattern became non-zero and rebuild it if one did. */
 int sprgpu_set_pattern(sprgpu_handle *handle, const double *sm_host);
 
+/* Element assembly on the device, replacing a host loop that calls SPRADM
+ * (src/SPR/spradm.f:5-166) once per element with the free-dof loop of its
+ * SPRAD2 (spradm.f:818-859) and SPRAD1's slot table (spradm.f:514-740).
+ *
+ * sprgpu_set_assembly (host arrays, once per connectivity): element e
+ * (0-based, in the order SPRADM would be called) has the dofs whose 1-based
+ * SPR equations are meen[meen_ptr[e] .. meen_ptr[e+1]) in element-matrix
+ * order, 0 for a dof without one: SPREEQ's MEEN (spradm.f:169-221).  Builds
+ * the map from element entries to LNZ slots and sets the residual pattern
+ * to the slots that receive an entry (so sprgpu_set_pattern is not needed).
+ * Returns -2 for a constrained dof (MEEN < 0: its TTCC couplings,
+ * spradm.f:861-1002, stay with the host SPRADM), an equation beyond NEQ, or
+ * an element coupling two equations outside the symbolic pattern.
+ *
+ * sprgpu_assemble (device arrays, asynchronous on the handle's stream):
+ * SM(NEQ+1:) receives sum_e EM_e(I,J) over I <= J, in SPRADM's order (by
+ * element, then J, then I), so the result equals the host loop's bit for
+ * bit.  EM_e is column-major nedof_e x nedof_e at em_dev + em_off_dev[e]
+ * (em_off_dev NULL: the matrices follow each other), multiplied by
+ * escale_dev[e] when escale_dev is not NULL, as the host loop does before it
+ * calls SPRADM.  accumulate 0: SM(1:NEQ+NZEROL) is cleared first (the
+ * caller's zeroing before the loop); 1: added to the SM given. */
+int sprgpu_set_assembly(sprgpu_handle *handle, sprgpu_int nel, const sprgpu_int *meen_ptr,
+                        const sprgpu_int *meen);
+int sprgpu_assemble(sprgpu_handle *handle, const double *em_dev, const sprgpu_int *em_off_dev,
+                    const double *escale_dev, double *sm_dev, int accumulate);
+
 /* Numeric factorization of SM given in device memory (NEQ+NZEROL doubles, the
  * SPRADM layout; not modified).  tol points to host TOL(1..4) (TOL(4) is read
  * only when mode==3).  mode 1 = SPRFC1 semantics, 3 = SPRFC3 semantics.

@@ -73,3 +73,22 @@ def sprprm(a, b, mspar, mtrees, msifa, iflag=1, c=None):
     lib().oracle_sprprm(_p(a), _p(b), _p(c), _p(wa), _p(mspar), _p(mtrees), _p(msifa),
                         _i(m), _i(n), _i(1), _i(iflag), _i(-1), C.byref(ierr))
     return int(ierr.value), c
+
+
+def assemble_free(mspar, mtrees, msifa, meen_ptr, meen, em, em_off, escale=None, sm=None):
+    """Restated SPRADM element loop (free dofs): SM(NEQ+1:) += EM_e per element.
+
+    Returns (rc, sm); sm is a zero SM(NEQ+NZEROL) unless one is given (it is
+    then updated in place)."""
+    neq, nzerol = int(mspar[7]), int(mspar[15])
+    sm = np.zeros(neq + nzerol) if sm is None else sm
+    meen_ptr = np.ascontiguousarray(meen_ptr, dtype=I8)
+    meen = np.ascontiguousarray(meen, dtype=I8)
+    em = np.ascontiguousarray(em, dtype=np.float64)
+    em_off = np.ascontiguousarray(em_off, dtype=I8)
+    sc = None if escale is None else np.ascontiguousarray(escale, dtype=np.float64)
+    f = lib().oracle_assemble_free
+    f.restype = C.c_int64
+    rc = f(_p(mspar), _p(mtrees), _p(msifa), C.c_int64(meen_ptr.size - 1), _p(meen_ptr),
+           _p(meen), _p(em), _p(em_off), _p(sc) if sc is not None else None, _p(sm))
+    return int(rc), sm

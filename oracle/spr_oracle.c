@@ -690,3 +690,70 @@ void oracle_sprprm(const double *a, const double *b, double *c, double *wa, fint
         }
     }
 }
+
+/* ------------------------------------------------------------------------- */
+/* Element assembly: SPRAD2's free-dof loop (spradm.f:818-859), element by    */
+/* element as a host loop over SPRADM (spradm.f:5-166) leaves it.             */
+/* ------------------------------------------------------------------------- */
+/* 1-based LNZ position of entry (row r, column c), r >= c, 1-based SPR
+ * equations: the slot that SPRAD1's PSM table assigns (spradm.f:514-740).
+ * Column c of supernode js is column k = c - XSUPER(js) of its packed
+ * trapezoid, which starts at XLNZ(js) + OFFSET(F,k) and holds the rows
+ * LINDX(XLINDX(js)+k ..) (sprsmb.f:512-575).  Returns 0 if r is not in the
+ * column's row list. */
+static fint lnz_position(const spr_view *v, const fint *col_sup, fint r, fint c)
+{
+    const fint js = col_sup[c - 1];
+    const fint k = c - F1(v->xsuper, js);
+    const fint f = F1(v->xlindx, js + 1) - F1(v->xlindx, js);
+    fint lo = F1(v->xlindx, js) + k, hi = F1(v->xlindx, js + 1) - 1;
+    while (lo <= hi) {
+        const fint mid = lo + (hi - lo) / 2;
+        const fint e = F1(v->lindx, mid);
+        if (e == r) return F1(v->xlnz, js) + OFFSET(f, k) + (mid - F1(v->xlindx, js) - k);
+        if (e < r)
+            lo = mid + 1;
+        else
+            hi = mid - 1;
+    }
+    return 0;
+}
+
+fint oracle_assemble_free(const fint *mspar, const fint *mtrees, const fint *msifa, fint nel,
+                          const fint *meen_ptr, const fint *meen, const double *em,
+                          const fint *em_off, const double *escale, double *sm)
+{
+    spr_view v;
+    make_view(mspar, mtrees, msifa, &v);
+    fint *col_sup = (fint *)malloc(sizeof(fint) * (size_t)(v.neq > 0 ? v.neq : 1));
+    if (!col_sup) return -1;
+    for (fint js = 1; js <= v.nsuper; ++js)
+        for (fint c = F1(v.xsuper, js); c < F1(v.xsuper, js + 1); ++c) col_sup[c - 1] = js;
+    double *lnz = sm + v.neq;  /* SPRADM passes SM(NEQ+1) (spradm.f:161) */
+    fint rc = 0;
+    for (fint e = 0; e < nel && rc == 0; ++e) {
+        const fint *mn = meen + meen_ptr[e];
+        const fint nd = meen_ptr[e + 1] - meen_ptr[e];
+        const double *ek = em + em_off[e];
+        for (fint j = 1; j <= nd && rc == 0; ++j) {
+            const fint jeq = F1(mn, j);
+            if (jeq < 1) continue;
+            for (fint i = 1; i <= j; ++i) {
+                const fint ieq = F1(mn, i);
+                if (ieq < 1) continue;
+                const fint r = ieq > jeq ? ieq : jeq, c = ieq > jeq ? jeq : ieq;
+                const fint k = lnz_position(&v, col_sup, r, c);
+                if (k == 0) {
+                    rc = -(e + 1);
+                    break;
+                }
+                /* the host loop scales the base matrix before the call */
+                const double x = ek[(i - 1) + (j - 1) * nd];
+                const double emij = escale ? escale[e] * x : x;
+                F1(lnz, k) = F1(lnz, k) + emij;
+            }
+        }
+    }
+    free(col_sup);
+    return rc;
+}

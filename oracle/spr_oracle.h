@@ -13,6 +13,8 @@
  *   oracle_sprsol  <- SPRSOL  src/SPR/sprsol.f:5-6
  *   oracle_sprprm  <- SPRPRM  src/SPR/sprprm.f:5-7 (C = +-A*B on the unfactored
  *                              SPR layout; used as the residual checker)
+ *   oracle_assemble_free <- SPRADM called once per element (spradm.f:5-166),
+ *                              its SPRAD2 free-dof loop (spradm.f:818-859)
  * The MVMULT/MMMULT callback arguments of SPRFC1/SPRFC3 are fixed to the
  * restated LMVY18/LMMY18 (what SPRSOL passes, sprsol.f:29-35).
  */
@@ -42,4 +44,16 @@ void oracle_sprprm(const double *a, const double *b, double *c, double *wa, fint
                    const fint *mtrees, const fint *msifa, const fint *m, const fint *n,
                    const fint *ksa, const fint *iflag, const fint *lpu, fint *ierr);
 
+/* SM(NEQ+1:) += EM_e(I,J) for every element e = 0..nel-1 in order, I <= J
+ * with MEEN(I), MEEN(J) >= 1 (SPRAD2's loop, spradm.f:818-859), at the LNZ
+ * slot of (max(IEQ,JEQ), min(IEQ,JEQ)) (SPRAD1's PSM table,
+ * spradm.f:514-740).  Element e's dofs have the 1-based equations
+ * meen[meen_ptr[e] .. meen_ptr[e+1]) (SPREEQ's MEEN: 0 = no equation; the
+ * constrained dofs, MEEN < 0, are skipped as the free loop skips them).
+ * EM_e = escale[e] * em[em_off[e] ..] (escale NULL: no scaling),
+ * column-major.  Returns 0, -(e+1) if element e couples two equations outside
+ * the symbolic pattern, or -1 on allocation failure. */
+fint oracle_assemble_free(const fint *mspar, const fint *mtrees, const fint *msifa, fint nel,
+                          const fint *meen_ptr, const fint *meen, const double *em,
+                          const fint *em_off, const double *escale, double *sm);
 #endif

@@ -47,6 +47,7 @@ static void free_all(sprgpu_handle *h)
                     d.spin_error, d.col_node, d.epoch};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    free_assembly(h);
     d = DeviceBuffers{};
 }
 
@@ -358,6 +359,20 @@ int sprgpu_set_pattern(sprgpu_handle *h, const double *sm_host)
     CsrPattern pat;
     build_csr_pattern(h->plan, sm_host + h->plan.neq, pat);
     return upload_pattern(h, pat);
+}
+
+int sprgpu_set_assembly(sprgpu_handle *h, sprgpu_int nel, const sprgpu_int *meen_ptr,
+                        const sprgpu_int *meen)
+{
+    if (!h || nel < 0 || (nel > 0 && (!meen_ptr || !meen))) return -2;
+    return set_assembly(h, nel, meen_ptr, meen);
+}
+
+int sprgpu_assemble(sprgpu_handle *h, const double *em_dev, const sprgpu_int *em_off_dev,
+                    const double *escale_dev, double *sm_dev, int accumulate)
+{
+    if (!h || !sm_dev || (!em_dev && h->d.asm_nel > 0)) return -2;
+    return run_assemble(h, em_dev, em_off_dev, escale_dev, sm_dev, accumulate);
 }
 
 int sprgpu_factor(sprgpu_handle *h, const double *sm_dev, const double *tol, int mode,

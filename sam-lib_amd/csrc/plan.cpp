@@ -505,7 +505,10 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
 // contributing its sub-diagonal entries to later rows (their lower parts)
 // and, in its own row, the diagonal and the upper part.  Threads own
 // contiguous column ranges, so per-thread counts keep the order.
-void build_csr_pattern(const Plan &p, const double *lnz, CsrPattern &pat)
+// keep(pos) selects the packed positions that enter the pattern; they are
+// visited in increasing order within each thread's range of supernodes
+template <class Keep>
+static void csr_pattern_impl(const Plan &p, Keep keep, CsrPattern &pat)
 {
     const i64 neq = p.neq, ns = p.nsuper;
     unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -534,7 +537,7 @@ void build_csr_pattern(const Plan &p, const double *lnz, CsrPattern &pat)
             for (i32 col = 0; col < n.S; ++col) {
                 const i32 ceq = n.first + col;
                 for (i32 r = col; r < n.F; ++r, ++pos) {
-                    if (lnz[pos] == 0.0) continue;
+                    if (!keep(pos)) continue;
                     nzl[t]++;
                     c[ceq]++;
                     if (r != col) c[rows[r]]++;
@@ -576,7 +579,7 @@ void build_csr_pattern(const Plan &p, const double *lnz, CsrPattern &pat)
             for (i32 col = 0; col < n.S; ++col) {
                 const i32 ceq = n.first + col;
                 for (i32 r = col; r < n.F; ++r, ++pos) {
-                    if (lnz[pos] == 0.0) continue;
+                    if (!keep(pos)) continue;
                     // row ceq, column rows[r] (upper part incl. diagonal)
                     i64 k = cu[ceq]++;
                     pat.col[k] = rows[r];
@@ -596,6 +599,18 @@ void build_csr_pattern(const Plan &p, const double *lnz, CsrPattern &pat)
         for (unsigned t = 0; t < nt; ++t) th.emplace_back(fill, t);
         for (auto &x : th) x.join();
     }
+}
+
+void build_csr_pattern(const Plan &p, const double *lnz, CsrPattern &pat)
+{
+    csr_pattern_impl(p, [lnz](i64 pos) { return lnz[pos] != 0.0; }, pat);
+}
+
+void build_csr_pattern_marked(const Plan &p, const std::vector<uint64_t> &marked, CsrPattern &pat)
+{
+    csr_pattern_impl(
+        p, [&marked](i64 pos) { return ((marked[size_t(pos) >> 6] >> (pos & 63)) & 1u) != 0; },
+        pat);
 }
 
 }  // namespace sprgpu

@@ -173,6 +173,9 @@ struct CsrPattern {
     i64 nnz_lower = 0;        // non-zeros in the packed lower triangle
 };
 void build_csr_pattern(const Plan &plan, const double *lnz, CsrPattern &pat);
+// the same from a bit set over LNZ positions (bit pos set: in the pattern)
+void build_csr_pattern_marked(const Plan &plan, const std::vector<uint64_t> &marked,
+                              CsrPattern &pat);
 
 struct FactorStatus {
     unsigned long long first_fail;   // smallest failing equation (UINT64_MAX = none)
@@ -225,6 +228,12 @@ struct DeviceBuffers {
     int *spin_error = nullptr;  // set when a dataflow wait gave up
     i32 *col_node = nullptr;    // supernode of each equation (adjoint pattern gradient)
     i64 irwork_cap = 0;
+    // element assembly map: LNZ positions that receive element entries
+    // (ascending), the range of each one's contributions, and the
+    // contributions in SPRADM order, each (element << 32) | (I + J*nedof)
+    i64 *asm_tpos = nullptr, *asm_tptr = nullptr, *asm_src = nullptr;
+    i64 *asm_emoff = nullptr;   // default element-matrix offsets (consecutive)
+    i64 asm_ntgt = 0, asm_ncontrib = 0, asm_nel = 0;
 };
 
 }  // namespace sprgpu
@@ -287,4 +296,9 @@ int upload_pattern(sprgpu_handle *h, const CsrPattern &pat);
 int run_element_grad(sprgpu_handle *h, i64 nel, int nedof, const i64 *elem_eq, const double *k0,
                      const i64 *k0_index, const double *lam, const double *x, double *grad);
 int run_pattern_grad(sprgpu_handle *h, const double *lam, const double *x, double *glnz);
+// assemble.hip
+int set_assembly(sprgpu_handle *h, i64 nel, const i64 *meen_ptr, const i64 *meen);
+int run_assemble(sprgpu_handle *h, const double *em, const i64 *em_off, const double *escale,
+                 double *sm, int accumulate);
+void free_assembly(sprgpu_handle *h);
 }  // namespace sprgpu

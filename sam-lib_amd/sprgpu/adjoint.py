@@ -7,9 +7,10 @@ plain FP64).  The backward pass reuses the same factor: lambda = K^-1 dL/du
 (K is symmetric), then dL/dp_e = -lambda_e^T K0_e u_e on the device
 (sprgpu_element_grad) and dL/df = lambda.  The reference has no adjoint.
 
-Assembly stays on the host, as in the reference flow: the reference SPRADM
-(sprgpu.host) builds SM for the current parameters, which is uploaded once
-per forward pass.
+Assembly runs on the device (sprgpu_assemble): the unit-parameter element
+matrices stay in HBM and SM for the current parameters is summed there, bit
+for bit what the reference SPRADM loop gives (EM_e = p_e * K0_e, then the
+element sums in SPRADM's order), so a forward pass moves no matrix over PCIe.
 """
 from __future__ import annotations
 
@@ -50,8 +51,12 @@ class ElasticSystem:
         self.free_dofs = torch.from_numpy(free).to(self.dev)
         self.free_eqs = torch.from_numpy(self.sysm.meqn[free] - 1).to(self.dev)
         self.p_factored = None  # parameters of the factor currently held by the handle
+        # element assembly map (also the residual pattern) and the device inputs
+        meen_ptr, meen = host.element_meen(model, self.sysm)
+        self.h.set_assembly(meen_ptr, meen)
+        self.em_dev = torch.from_numpy(np.ascontiguousarray(model.em)).to(self.dev)
+        self.em_off_dev = torch.from_numpy(np.asarray(model.em_off, dtype=np.int64)).to(self.dev)
         self.sm = None
-        self.pattern_set = False
 
     # -- vector maps between natural dof order and SPR equation order ------
     def to_eq(self, v_dof):
@@ -65,19 +70,14 @@ class ElasticSystem:
         return u
 
     # -- factor and solves ----------------------------------------------------
-    def assemble(self, p: np.ndarray):
-        """SM for parameters p (host SPRADM), uploaded to the device."""
-        m = self.model
-        saved = m.escale
-        m.escale = np.asarray(p, dtype=np.float64)
-        try:
-            sm = host.assemble(m, self.sysm)
-        finally:
-            m.escale = saved
-        if not self.pattern_set:
-            self.h.set_pattern(sm)
-            self.pattern_set = True
-        self.sm = self.torch.from_numpy(sm).to(self.dev)
+    def assemble(self, p):
+        """SM for parameters p (numpy or a device tensor), assembled on the device."""
+        torch = self.torch
+        p_dev = torch.as_tensor(p, dtype=torch.float64).to(self.dev)
+        if self.sm is None:
+            self.sm = torch.empty(self.sysm.neq + self.sysm.nzerol, dtype=torch.float64,
+                                  device=self.dev)
+        self.h.assemble(self.em_dev, self.sm, self.em_off_dev, p_dev)
         return self.sm
 
     def factor(self, p: np.ndarray, sm_dev=None):

@@ -88,6 +88,24 @@ class SprFactor:
         sm_host = np.ascontiguousarray(sm_host, dtype=np.float64)
         check(lib().sprgpu_set_pattern(self._h, _np(sm_host)), "sprgpu_set_pattern")
 
+    def set_assembly(self, meen_ptr: np.ndarray, meen: np.ndarray):
+        """Element connectivity for ``assemble``: 1-based equations (SPREEQ's
+        MEEN, 0 = none) of element e's dofs at meen[meen_ptr[e]:meen_ptr[e+1]].
+        Also sets the residual pattern."""
+        meen_ptr = np.ascontiguousarray(meen_ptr, dtype=np.int64)
+        meen = np.ascontiguousarray(meen, dtype=np.int64)
+        check(lib().sprgpu_set_assembly(self._h, meen_ptr.size - 1, _np(meen_ptr), _np(meen)),
+              "sprgpu_set_assembly")
+
+    def assemble(self, em_dev, sm_dev, em_off_dev=None, escale_dev=None,
+                 accumulate: bool = False):
+        """SM (device, NEQ+NZEROL float64) from the element matrices on the
+        device, bit for bit what a host loop of SPRADM gives."""
+        opt = (lambda t: None if t is None else _ptr(t))  # optional arrays: NULL
+        check(lib().sprgpu_assemble(self._h, _ptr(em_dev), opt(em_off_dev), opt(escale_dev),
+                                    _ptr(sm_dev), int(bool(accumulate))), "sprgpu_assemble")
+        return sm_dev
+
     def factor(self, sm_dev, tol=(0.0, 0.0, 0.0, 0.0), mode: int = 1, info: bool = True):
         tolv = np.zeros(4)
         tolv[:len(tol)] = tol

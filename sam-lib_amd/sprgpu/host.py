@@ -263,6 +263,28 @@ def element_equations(model, sysm: SprSystem) -> np.ndarray:
     return eq
 
 
+def element_meen(model, sysm: SprSystem):
+    """(meen_ptr, meen): element e's dofs have the 1-based SPR equations
+    meen[meen_ptr[e]:meen_ptr[e+1]] in element-matrix order, 0 where a dof has
+    none.  This is the MEEN array that SPREEQ builds for each SPRADM call
+    (spradm.f:169-221, MEEN(I) = MEQN(IDOF))."""
+    nedof = np.asarray(model.nedof, dtype=I8)
+    meen_ptr = np.zeros(model.nel + 1, dtype=I8)
+    np.cumsum(nedof, out=meen_ptr[1:])
+    if np.all(nedof == nedof[0]) and model.nel:
+        nn = int(model.mpmnpc[1] - model.mpmnpc[0])
+        if np.all(np.diff(model.mpmnpc) == nn):
+            # uniform elements: every node has nedof/nn dofs, in MADOF order
+            nodes = (model.mmnpc - 1).reshape(model.nel, nn)
+            per = int(nedof[0]) // nn
+            first = model.madof[nodes] - 1
+            if np.all(model.madof[nodes + 1] - 1 - first == per):
+                dofs = (first[:, :, None] + np.arange(per)).reshape(model.nel, -1)
+                return meen_ptr, np.ascontiguousarray(sysm.meqn[dofs].ravel(), dtype=I8)
+    meen = np.concatenate([sysm.meqn[model.element_dofs(e)] for e in range(model.nel)])
+    return meen_ptr, np.ascontiguousarray(meen, dtype=I8)
+
+
 @dataclass
 class MergedSystem:
     """Several independent SPR systems as one forest (block-diagonal A).
