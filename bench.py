@@ -29,7 +29,7 @@ sys.path.insert(0, os.path.join(ROOT, "sam-lib_amd"))
 
 METRIC = "factor+solve s & TFLOP/s, 40^3 hex8: FP64 vs FP32+IR at 1e-10 residual"
 PEAK = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense MFMA TFLOP/s
-GEMM_F32_TRAFFIC_40 = 227.1e6  # measured HBM bytes per k_gemm<float,64,1> launch, 40^3
+GEMM_F32_TRAFFIC_40 = 168.6e6  # measured HBM bytes per k_gemm<float,64,1> launch, 40^3
 HBM_PEAK = 8000.0  # GB/s
 
 
