@@ -156,6 +156,23 @@ int sprgpu_solve(sprgpu_handle *handle, double *b_dev, sprgpu_int ldb, sprgpu_in
 int sprgpu_spmv(sprgpu_handle *handle, const double *x_dev, double *y_dev, sprgpu_int ld,
                 sprgpu_int nrhs);
 
+/* Replaces SPRPRM (src/SPR/sprprm.f:5-7) on device arrays: for each of the N
+ * columns of B(M,N), IFLAG +-1: C = +-A B; +-2: C = C +- A B; 88 (N = 1):
+ * WA = A B and C is not referenced.  KSA > 0: A is a symmetric matrix in the
+ * SPR layout, the NZEROL doubles of LNZ (SM(NEQ+1:), as SPRSMM passes it);
+ * its non-zeros must lie on the handle's pattern (sprgpu_set_pattern or
+ * sprgpu_set_assembly; checked, -1 if not).  KSA <= 0: A is a diagonal of
+ * NEQ doubles.  wa_dev (M doubles, may be NULL unless IFLAG = 88) receives
+ * the last column's +-A b, as the reference's WA does.  Each C(r) is summed
+ * in the reference's order (row r in ascending column order, every product
+ * rounded before it is added), so the result equals SPRPRM's, up to the sign
+ * of a zero sum.  Returns 0, -2 (IFLAG, N, missing C/WA, or the PERI2E
+ * permutation, MSPAR(58) >= 1, not supported), -3 (KSA <= 0 with PERI2E),
+ * -4 (M < NEQ) as the reference's IERR, or -1 (pattern). */
+int sprgpu_sprprm(sprgpu_handle *handle, const double *a_dev, const double *b_dev,
+                  double *c_dev, double *wa_dev, sprgpu_int m, sprgpu_int n, sprgpu_int ksa,
+                  sprgpu_int iflag);
+
 /* Solves A X = B with the current factor plus FP64 iterative refinement
  * until max_col ||B - A X|| / ||B|| <= rtol or maxit steps. */
 int sprgpu_solve_ir(sprgpu_handle *handle, const double *b_dev, double *x_dev, sprgpu_int ld,

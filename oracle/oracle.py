@@ -61,8 +61,9 @@ def sprfc3(mspar, mtrees, msifa, sm, tol):
     return int(ierr.value), tolv, dcor.reshape(neq, 2).T
 
 
-def sprprm(a, b, mspar, mtrees, msifa, iflag=1, c=None):
-    """Restated SPRPRM: C = +-A*B (|iflag|=1) or C +-= A*B (|iflag|=2)."""
+def sprprm(a, b, mspar, mtrees, msifa, iflag=1, c=None, ksa=1):
+    """Restated SPRPRM: C = +-A*B (|iflag|=1) or C +-= A*B (|iflag|=2); A is
+    the SPR-layout lower triangle (ksa > 0) or a diagonal vector (ksa <= 0)."""
     b = np.asfortranarray(b, dtype=np.float64)
     if b.ndim == 1:
         b = b[:, None]
@@ -71,7 +72,7 @@ def sprprm(a, b, mspar, mtrees, msifa, iflag=1, c=None):
     wa = np.zeros(m)
     ierr = C.c_int64(0)
     lib().oracle_sprprm(_p(a), _p(b), _p(c), _p(wa), _p(mspar), _p(mtrees), _p(msifa),
-                        _i(m), _i(n), _i(1), _i(iflag), _i(-1), C.byref(ierr))
+                        _i(m), _i(n), _i(ksa), _i(iflag), _i(-1), C.byref(ierr))
     return int(ierr.value), c
 
 

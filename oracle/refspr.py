@@ -77,3 +77,20 @@ def sol(iop: int, sysm: SprSystem, sm: np.ndarray, b: np.ndarray | None, tol=Non
     return int(ierr), secs.value, tolv
 
 
+
+
+def prm(a, b, mspar, mtrees, msifa, iflag=1, ksa=1, c=None):
+    """Reference SPRPRM (sprprm.f:5-7): C = +-A B or C = C +- A B.  Returns
+    (ierr, c, wa); wa is the reference's work array on exit (the last
+    column's +-A b, or the result for IFLAG = 88)."""
+    b = np.asfortranarray(b, dtype=np.float64)
+    if b.ndim == 1:
+        b = b[:, None]
+    m, n = b.shape
+    c = np.zeros((m, n), order="F") if c is None else c
+    wa = np.zeros(m)
+    ierr = C.c_int64(0)
+    lib().sprprm_(_p(np.ascontiguousarray(a, dtype=np.float64)), _p(b), _p(c), _p(wa),
+                  _p(mspar), _p(mtrees), _p(msifa), _i(m), _i(n), _i(ksa), _i(iflag), _i(-1),
+                  C.byref(ierr))
+    return int(ierr.value), c, wa

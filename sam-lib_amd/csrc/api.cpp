@@ -44,7 +44,7 @@ static void free_all(sprgpu_handle *h)
                     d.csr_src,  d.csr_val,    d.irwork,      d.level_nodes,
                     d.solve_small, d.wide,     d.wide_fwd_prefix, d.wide_bwd_prefix,
                     d.wide_rect_prefix, d.wide_frect_prefix, d.upd_ptr, d.upd_src, d.flags, d.partials,
-                    d.spin_error, d.col_node, d.epoch};
+                    d.spin_error, d.col_node, d.epoch, d.prm_count};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     free_assembly(h);
@@ -418,6 +418,13 @@ int sprgpu_spmv(sprgpu_handle *h, const double *x, double *y, sprgpu_int ld, spr
 {
     if (!h || !x || !y || ld < h->plan.neq) return -2;
     return run_spmv(h, x, y, ld, nrhs);
+}
+
+int sprgpu_sprprm(sprgpu_handle *h, const double *a_dev, const double *b_dev, double *c_dev,
+                  double *wa_dev, sprgpu_int m, sprgpu_int n, sprgpu_int ksa, sprgpu_int iflag)
+{
+    if (!h || !a_dev || !b_dev) return -2;
+    return run_sprprm(h, a_dev, b_dev, c_dev, wa_dev, m, n, ksa, iflag);
 }
 
 int sprgpu_solve_ir(sprgpu_handle *h, const double *b, double *x, sprgpu_int ld,

@@ -195,6 +195,158 @@ static void launch_spmv(sprgpu_handle *h, const double *x, double *y, const doub
     }
 }
 
+// ---------------------------------------------------------------------------
+// SPRPRM on the device (sprprm.f:5-238, its product loop SPRPRQ at :243-326)
+// ---------------------------------------------------------------------------
+// SPRPRQ walks the packed columns in order and adds ALFA*(a_ij*b) to C(i)
+// and C(j) as it meets each entry, starting from C = 0.  Each C(r) therefore
+// receives the entries of row r of the symmetric A in ascending column order:
+// a_rj for j < r from the columns before r, then a_rr and the a_ir, i > r,
+// of column r.  The CSR pattern holds every row in that order, so one lane
+// per row reproduces the reference's sums bit for bit (entries outside the
+// pattern are zero and add nothing).
+__global__ __launch_bounds__(256) void k_prm_rows(const i64 *__restrict__ rowptr,
+                                                  const i32 *__restrict__ col,
+                                                  const i64 *__restrict__ src,
+                                                  const double *__restrict__ a,
+                                                  const double *__restrict__ b, double *c,
+                                                  double *wa, i64 n, i64 ld, double alfa,
+                                                  int accumulate, int write_c, int nrhs)
+{
+    // product rounded, then added, as the reference's separate operations
+#pragma clang fp contract(off)
+    const i64 row = i64(blockIdx.x) * 256 + threadIdx.x;
+    const int rhs = blockIdx.y;
+    if (row >= n) return;
+    const double *bb = b + rhs * ld;
+    double s = 0.0;
+    const i64 k1 = rowptr[row + 1];
+    for (i64 k = rowptr[row]; k < k1; ++k) {
+        const double p = a[src[k]] * bb[col[k]];
+        s = s + alfa * p;
+    }
+    if (write_c) {
+        double *cc = c + rhs * ld + row;
+        *cc = accumulate ? *cc + s : s;
+    }
+    if (wa && rhs == nrhs - 1) wa[row] = s;
+}
+
+// KSA <= 0: A is a diagonal held as a vector, C(j) = ALFA*(A(j)*B(j))
+__global__ __launch_bounds__(256) void k_prm_diag(const double *__restrict__ a,
+                                                  const double *__restrict__ b, double *c,
+                                                  double *wa, i64 n, i64 ld, double alfa,
+                                                  int accumulate, int write_c, int nrhs)
+{
+#pragma clang fp contract(off)
+    const i64 row = i64(blockIdx.x) * 256 + threadIdx.x;
+    const int rhs = blockIdx.y;
+    if (row >= n) return;
+    const double p = a[row] * b[rhs * ld + row];
+    const double s = alfa * p;
+    if (write_c) {
+        double *cc = c + rhs * ld + row;
+        *cc = accumulate ? *cc + s : s;
+    }
+    if (wa && rhs == nrhs - 1) wa[row] = s;
+}
+
+// non-zeros of a packed matrix (all of it, and on the pattern's lower entries)
+__global__ __launch_bounds__(256) void k_count_nonzero(const double *__restrict__ a, i64 n,
+                                                       unsigned long long *count)
+{
+    unsigned long long c = 0;
+    for (i64 i = i64(blockIdx.x) * 256 + threadIdx.x; i < n; i += i64(gridDim.x) * 256)
+        c += (a[i] != 0.0);
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+}
+
+__global__ __launch_bounds__(256) void k_count_pattern(const i64 *__restrict__ rowptr,
+                                                       const i32 *__restrict__ col,
+                                                       const i64 *__restrict__ src,
+                                                       const double *__restrict__ a, i64 n,
+                                                       unsigned long long *count)
+{
+    unsigned long long c = 0;
+    for (i64 row = i64(blockIdx.x) * 256 + threadIdx.x; row < n; row += i64(gridDim.x) * 256)
+        for (i64 k = rowptr[row]; k < rowptr[row + 1]; ++k)
+            c += (col[k] <= row && a[src[k]] != 0.0);
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+}
+
+int run_sprprm(sprgpu_handle *h, const double *a, const double *b, double *c, double *wa,
+               i64 m, i64 n, i64 ksa, i64 iflag)
+{
+    DeviceBuffers &d = h->d;
+    const i64 neq = h->plan.neq;
+    const bool only_wa = (iflag == 88 && n == 1);
+    if (!only_wa && iflag != 1 && iflag != -1 && iflag != 2 && iflag != -2) {
+        set_error("sprprm: IFLAG must be +-1, +-2, or 88 with N = 1");
+        return -2;
+    }
+    const bool peri2e = h->mspar.size() >= 58 && h->mspar[57] >= 1;
+    if (ksa <= 0 && peri2e) {
+        set_error("sprprm: a diagonal A (KSA <= 0) with the PERI2E permutation");
+        return -3;
+    }
+    if (m < neq) {
+        set_error("sprprm: M < NEQ (retained equations are not supported)");
+        return -4;
+    }
+    if (ksa > 0 && peri2e) {
+        set_error("sprprm: the PERI2E permutation (MSPAR(58) >= 1) is not supported");
+        return -2;
+    }
+    if (n < 1 || (only_wa && !wa) || (!only_wa && !c)) {
+        set_error("sprprm: N must be positive; C (or WA for IFLAG 88) is required");
+        return -2;
+    }
+    hipStream_t s = h->stream;
+    const double alfa = iflag < 0 ? -1.0 : 1.0;
+    const int accumulate = (iflag == 2 || iflag == -2);
+    if (ksa > 0) {
+        if (!h->has_pattern) {
+            set_error("sprprm: no pattern; call sprgpu_set_pattern or sprgpu_set_assembly");
+            return -1;
+        }
+        // every non-zero of A must lie on the pattern
+        if (!d.prm_count) SPRGPU_HIP(hipMalloc(&d.prm_count, 2 * sizeof(unsigned long long)));
+        SPRGPU_HIP(hipMemsetAsync(d.prm_count, 0, 2 * sizeof(unsigned long long), s));
+        const i64 nz = h->plan.nzerol;
+        k_count_nonzero<<<unsigned(std::max<i64>(1, std::min<i64>((nz + 255) / 256, 8192))), 256,
+                          0, s>>>(a, nz, d.prm_count);
+        k_count_pattern<<<unsigned(std::max<i64>(1, std::min<i64>((neq + 255) / 256, 4096))),
+                          256, 0, s>>>(d.csr_rowptr, d.csr_col, d.csr_src, a, neq,
+                                       d.prm_count + 1);
+        unsigned long long cnt[2];
+        SPRGPU_HIP(hipMemcpyAsync(cnt, d.prm_count, sizeof(cnt), hipMemcpyDeviceToHost, s));
+        SPRGPU_HIP(hipStreamSynchronize(s));
+        if (cnt[0] != cnt[1]) {
+            set_error("sprprm: A has non-zeros outside the pattern; call sprgpu_set_pattern "
+                      "with A");
+            return -1;
+        }
+    }
+    const unsigned gx = unsigned((neq + 255) / 256);
+    for (i64 r0 = 0; r0 < n; r0 += 65535) {
+        const i64 nr = std::min<i64>(65535, n - r0);
+        dim3 g(gx, unsigned(nr));
+        double *wa_chunk = (r0 + nr == n) ? wa : nullptr;
+        if (ksa > 0)
+            k_prm_rows<<<g, 256, 0, s>>>(d.csr_rowptr, d.csr_col, d.csr_src, a, b + r0 * m,
+                                         only_wa ? nullptr : c + r0 * m, wa_chunk, neq, m, alfa,
+                                         accumulate, only_wa ? 0 : 1, int(nr));
+        else
+            k_prm_diag<<<g, 256, 0, s>>>(a, b + r0 * m, only_wa ? nullptr : c + r0 * m,
+                                         wa_chunk, neq, m, alfa, accumulate, only_wa ? 0 : 1,
+                                         int(nr));
+    }
+    SPRGPU_HIP(hipGetLastError());
+    return 0;
+}
+
 int run_spmv(sprgpu_handle *h, const double *x, double *y, i64 ld, i64 nrhs)
 {
     if (int rc = refresh_values(h)) return rc;

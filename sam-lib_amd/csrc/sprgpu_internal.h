@@ -226,6 +226,7 @@ struct DeviceBuffers {
     i64 partials_cap = 0;
     unsigned *epoch = nullptr; // sweep counter the ready flags are compared with
     int *spin_error = nullptr;  // set when a dataflow wait gave up
+    unsigned long long *prm_count = nullptr;  // non-zero counts of a product's A (2)
     i32 *col_node = nullptr;    // supernode of each equation (adjoint pattern gradient)
     i64 irwork_cap = 0;
     // element assembly map: LNZ positions that receive element entries
@@ -292,6 +293,8 @@ int run_spmv(sprgpu_handle *h, const double *x, double *y, i64 ld, i64 nrhs);
 int run_solve_ir(sprgpu_handle *h, const double *b, double *x, i64 ld, i64 nrhs, double rtol,
                  int maxit, sprgpu_ir_info *info);
 int upload_pattern(sprgpu_handle *h, const CsrPattern &pat);
+int run_sprprm(sprgpu_handle *h, const double *a, const double *b, double *c, double *wa,
+               i64 m, i64 n, i64 ksa, i64 iflag);
 // adjoint.hip
 int run_element_grad(sprgpu_handle *h, i64 nel, int nedof, const i64 *elem_eq, const double *k0,
                      const i64 *k0_index, const double *lam, const double *x, double *grad);

@@ -135,6 +135,21 @@ class SprFactor:
               "sprgpu_spmv")
         return y_dev
 
+    def sprprm(self, a_dev, b_dev, c_dev=None, iflag: int = 1, ksa: int = 1, wa_dev=None):
+        """SPRPRM on the device: C = +-A B (iflag +-1), C +-= A B (+-2), or
+        WA = A B (88, one column).  a_dev: LNZ (NZEROL) or a diagonal (NEQ,
+        ksa <= 0); b_dev, c_dev: (nrhs, neq) float64."""
+        torch = self._torch
+        b2 = b_dev if b_dev.dim() == 2 else b_dev.view(1, -1)
+        if c_dev is None and iflag != 88:
+            c_dev = torch.zeros_like(b2)
+        c2 = None if c_dev is None else (c_dev if c_dev.dim() == 2 else c_dev.view(1, -1))
+        opt = (lambda t: None if t is None else _ptr(t))
+        check(lib().sprgpu_sprprm(self._h, _ptr(a_dev), _ptr(b2), opt(c2), opt(wa_dev),
+                                  b2.shape[1], b2.shape[0], int(ksa), int(iflag)),
+              "sprgpu_sprprm")
+        return c_dev if iflag != 88 else wa_dev
+
     def solve_ir(self, b_dev, x_dev, rtol: float = 1e-10, maxit: int = 20) -> dict:
         b2 = b_dev if b_dev.dim() == 2 else b_dev.view(1, -1)
         x2 = x_dev if x_dev.dim() == 2 else x_dev.view(1, -1)

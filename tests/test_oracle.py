@@ -208,6 +208,40 @@ def test_sprprm_equals_dense_product():
     np.testing.assert_array_equal(c2, -c)
 
 
+@pytest.mark.parametrize("kind,n,kw", [("hex8", 5, dict(e_sigma=0.5, e_seed=1)),
+                                       ("quad4", 10, {}), ("frame", 50, dict(seed=3))])
+def test_sprprm_bitexact_vs_reference(kind, n, kw):
+    """The restated SPRPRM equals the reference's (oracle/_ref) bit for bit for
+    every IFLAG and for a diagonal A (KSA <= 0), and fails for a planted
+    change of the summation order (the column's own entries first)."""
+    import refspr
+
+    model, sysm, sm = make_system(kind, n, **kw)
+    rng = np.random.default_rng(6)
+    x = rng.normal(size=(sysm.neq, 3))
+    a = sm[sysm.neq:]
+    for ksa in (1, 0):
+        aa = a if ksa > 0 else rng.normal(size=sysm.neq)
+        for iflag in (1, -1, 2, -2):
+            c0 = rng.normal(size=(sysm.neq, 3))
+            ierr_r, c_r, wa_r = refspr.prm(aa, x, sysm.mspar.copy(), sysm.mtrees, sysm.msifa,
+                                           iflag=iflag, ksa=ksa, c=np.asfortranarray(c0.copy()))
+            c_o = np.asfortranarray(c0.copy())
+            ierr_o, c_o = oracle.sprprm(aa, x, sysm.mspar.copy(), sysm.mtrees, sysm.msifa,
+                                        iflag=iflag, c=c_o, ksa=ksa)
+            assert ierr_r == ierr_o == 0
+            assert np.array_equal(c_o.view(np.int64), c_r.view(np.int64)), (ksa, iflag)
+    # planted mistake: each row summed with its own column's entries first
+    A = dense_equation_matrix(model, sysm)
+    ierr_r, c_r, _ = refspr.prm(a, x, sysm.mspar.copy(), sysm.mtrees, sysm.msifa)
+    w = np.zeros_like(x)
+    for r in range(sysm.neq):
+        cols = np.nonzero(A[r])[0]
+        for cc in np.concatenate([cols[cols >= r], cols[cols < r]]):
+            w[r] += A[r, cc] * x[cc]
+    assert not np.array_equal(w, c_r)
+
+
 def test_planted_mistake_is_caught():
     """A dropped Schur term in the factor must break the dense-LDL' check."""
     import refspr
