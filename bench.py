@@ -57,6 +57,32 @@ def build_problem(n):
     return model, sysm, sm, b, t_asm
 
 
+def bench_product(sysm, h, sm_dev, x, b_dev, reps):
+    """SPRPRM on the device (sprgpu_sprprm) with the assembled A: A x for the
+    FP64 solution must give b back to the solve's residual (a size-independent
+    check of the full product), and the time per call from HIP events (the
+    check that A's non-zeros lie on the pattern included, and apart).
+    Algorithmic bytes of the row sums: per stored entry of the full symmetric
+    pattern its column index, slot and value and the x entry (28 B), plus the
+    row pointers and C (16 B per row); the check reads LNZ once (8 B per slot)."""
+    import torch
+
+    a = sm_dev[sysm.neq:]
+    c = torch.empty_like(x)
+    h.sprprm(a, x, c, iflag=1)
+    torch.cuda.synchronize()
+    rel = float(torch.linalg.norm(c - b_dev) / torch.linalg.norm(b_dev))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        h.sprprm(a, x, c, iflag=1)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    log(f"device SPRPRM {ms:.3f} ms per call (pattern check included), |Ax - b|/|b| = {rel:.2e}")
+    return {"ms_per_call": ms, "relres_of_fp64_solution": rel}
+
+
 def bench_assembly(model, sysm, sm, h, reps, t_host):
     """Element assembly of the same SM on the device (sprgpu_set_assembly once,
     then sprgpu_assemble): bit equality with the host SPRADM loop's SM, and the
@@ -274,6 +300,7 @@ def main():
     # element assembly on the device (after the timed steps: it replaces the
     # FP64 handle's residual pattern by the map's, which that handle no longer uses)
     result["assembly"] = bench_assembly(model, sysm, sm, h64, max(args.steps, 5), t_host_asm)
+    result["sprprm"] = bench_product(sysm, h64, sm_dev, x64, b_dev, max(args.steps, 5))
 
     if not args.no_profile:
         import ctypes as C

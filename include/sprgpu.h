@@ -161,8 +161,8 @@ int sprgpu_spmv(sprgpu_handle *handle, const double *x_dev, double *y_dev, sprgp
  * WA = A B and C is not referenced.  KSA > 0: A is a symmetric matrix in the
  * SPR layout, the NZEROL doubles of LNZ (SM(NEQ+1:), as SPRSMM passes it);
  * its non-zeros must lie on the handle's pattern (sprgpu_set_pattern or
- * sprgpu_set_assembly; checked, -1 if not).  KSA <= 0: A is a diagonal of
- * NEQ doubles.  wa_dev (M doubles, may be NULL unless IFLAG = 88) receives
+ * sprgpu_set_assembly; checked after the product: -1, and C not valid, if
+ * not).  KSA <= 0: A is a diagonal of NEQ doubles.  wa_dev (M doubles, may be NULL unless IFLAG = 88) receives
  * the last column's +-A b, as the reference's WA does.  Each C(r) is summed
  * in the reference's order (row r in ascending column order, every product
  * rounded before it is added), so the result equals SPRPRM's, up to the sign

@@ -205,31 +205,57 @@ static void launch_spmv(sprgpu_handle *h, const double *x, double *y, const doub
 // of column r.  The CSR pattern holds every row in that order, so one lane
 // per row reproduces the reference's sums bit for bit (entries outside the
 // pattern are zero and add nothing).
+constexpr int kPrmBatch = 8;  // entries whose loads are issued together
 __global__ __launch_bounds__(256) void k_prm_rows(const i64 *__restrict__ rowptr,
                                                   const i32 *__restrict__ col,
                                                   const i64 *__restrict__ src,
                                                   const double *__restrict__ a,
                                                   const double *__restrict__ b, double *c,
                                                   double *wa, i64 n, i64 ld, double alfa,
-                                                  int accumulate, int write_c, int nrhs)
+                                                  int accumulate, int write_c, int nrhs,
+                                                  unsigned long long *count)
 {
     // product rounded, then added, as the reference's separate operations
 #pragma clang fp contract(off)
     const i64 row = i64(blockIdx.x) * 256 + threadIdx.x;
     const int rhs = blockIdx.y;
-    if (row >= n) return;
-    const double *bb = b + rhs * ld;
-    double s = 0.0;
-    const i64 k1 = rowptr[row + 1];
-    for (i64 k = rowptr[row]; k < k1; ++k) {
-        const double p = a[src[k]] * bb[col[k]];
-        s = s + alfa * p;
+    unsigned long long nz = 0;
+    if (row < n) {
+        const double *bb = b + rhs * ld;
+        double s = 0.0;
+        const i64 k0 = rowptr[row], k1 = rowptr[row + 1];
+        for (i64 kb = k0; kb < k1; kb += kPrmBatch) {
+            // the batch's loads first (clamped to the row's last entry), then
+            // the products added one by one in the row's order
+            double av[kPrmBatch], bv[kPrmBatch];
+            i32 cv[kPrmBatch];
+#pragma unroll
+            for (int q = 0; q < kPrmBatch; ++q) {
+                const i64 k = min(kb + q, k1 - 1);
+                cv[q] = col[k];
+                av[q] = a[src[k]];
+                bv[q] = bb[cv[q]];
+            }
+#pragma unroll
+            for (int q = 0; q < kPrmBatch; ++q) {
+                if (kb + q < k1) {
+                    const double p = av[q] * bv[q];
+                    s = s + alfa * p;
+                    nz += (cv[q] <= row && av[q] != 0.0);
+                }
+            }
+        }
+        if (write_c) {
+            double *cc = c + rhs * ld + row;
+            *cc = accumulate ? *cc + s : s;
+        }
+        if (wa && rhs == nrhs - 1) wa[row] = s;
     }
-    if (write_c) {
-        double *cc = c + rhs * ld + row;
-        *cc = accumulate ? *cc + s : s;
+    // non-zeros of A on the pattern's lower entries (first column's blocks)
+    if (count && rhs == 0) {
+        for (int o = 32; o > 0; o >>= 1) nz += __shfl_down(nz, o);
+        if ((threadIdx.x & 63) == 0 && nz) atomicAdd(count, nz);
     }
-    if (wa && rhs == nrhs - 1) wa[row] = s;
 }
 
 // KSA <= 0: A is a diagonal held as a vector, C(j) = ALFA*(A(j)*B(j))
@@ -251,27 +277,23 @@ __global__ __launch_bounds__(256) void k_prm_diag(const double *__restrict__ a,
     if (wa && rhs == nrhs - 1) wa[row] = s;
 }
 
-// non-zeros of a packed matrix (all of it, and on the pattern's lower entries)
+// non-zeros of a packed matrix, 16-byte loads over its aligned part
 __global__ __launch_bounds__(256) void k_count_nonzero(const double *__restrict__ a, i64 n,
                                                        unsigned long long *count)
 {
     unsigned long long c = 0;
-    for (i64 i = i64(blockIdx.x) * 256 + threadIdx.x; i < n; i += i64(gridDim.x) * 256)
-        c += (a[i] != 0.0);
-    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o);
-    if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
-}
-
-__global__ __launch_bounds__(256) void k_count_pattern(const i64 *__restrict__ rowptr,
-                                                       const i32 *__restrict__ col,
-                                                       const i64 *__restrict__ src,
-                                                       const double *__restrict__ a, i64 n,
-                                                       unsigned long long *count)
-{
-    unsigned long long c = 0;
-    for (i64 row = i64(blockIdx.x) * 256 + threadIdx.x; row < n; row += i64(gridDim.x) * 256)
-        for (i64 k = rowptr[row]; k < rowptr[row + 1]; ++k)
-            c += (col[k] <= row && a[src[k]] != 0.0);
+    const i64 t = i64(blockIdx.x) * 256 + threadIdx.x, stride = i64(gridDim.x) * 256;
+    const i64 head = (reinterpret_cast<uintptr_t>(a) & 15) ? 1 : 0;
+    const i64 n2 = (n - head) / 2;
+    const double2 *q = reinterpret_cast<const double2 *>(a + head);
+    for (i64 i = t; i < n2; i += stride) {
+        const double2 v = q[i];
+        c += (v.x != 0.0) + (v.y != 0.0);
+    }
+    if (t == 0) {
+        if (head && n > 0) c += (a[0] != 0.0);
+        if (head + 2 * n2 < n) c += (a[n - 1] != 0.0);
+    }
     for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o);
     if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
 }
@@ -311,23 +333,13 @@ int run_sprprm(sprgpu_handle *h, const double *a, const double *b, double *c, do
             set_error("sprprm: no pattern; call sprgpu_set_pattern or sprgpu_set_assembly");
             return -1;
         }
-        // every non-zero of A must lie on the pattern
+        // every non-zero of A must lie on the pattern: A's non-zeros are
+        // counted here and those on the pattern by the product kernel
         if (!d.prm_count) SPRGPU_HIP(hipMalloc(&d.prm_count, 2 * sizeof(unsigned long long)));
         SPRGPU_HIP(hipMemsetAsync(d.prm_count, 0, 2 * sizeof(unsigned long long), s));
         const i64 nz = h->plan.nzerol;
-        k_count_nonzero<<<unsigned(std::max<i64>(1, std::min<i64>((nz + 255) / 256, 8192))), 256,
-                          0, s>>>(a, nz, d.prm_count);
-        k_count_pattern<<<unsigned(std::max<i64>(1, std::min<i64>((neq + 255) / 256, 4096))),
-                          256, 0, s>>>(d.csr_rowptr, d.csr_col, d.csr_src, a, neq,
-                                       d.prm_count + 1);
-        unsigned long long cnt[2];
-        SPRGPU_HIP(hipMemcpyAsync(cnt, d.prm_count, sizeof(cnt), hipMemcpyDeviceToHost, s));
-        SPRGPU_HIP(hipStreamSynchronize(s));
-        if (cnt[0] != cnt[1]) {
-            set_error("sprprm: A has non-zeros outside the pattern; call sprgpu_set_pattern "
-                      "with A");
-            return -1;
-        }
+        k_count_nonzero<<<unsigned(std::max<i64>(1, std::min<i64>((nz / 2 + 255) / 256, 8192))),
+                          256, 0, s>>>(a, nz, d.prm_count);
     }
     const unsigned gx = unsigned((neq + 255) / 256);
     for (i64 r0 = 0; r0 < n; r0 += 65535) {
@@ -337,13 +349,24 @@ int run_sprprm(sprgpu_handle *h, const double *a, const double *b, double *c, do
         if (ksa > 0)
             k_prm_rows<<<g, 256, 0, s>>>(d.csr_rowptr, d.csr_col, d.csr_src, a, b + r0 * m,
                                          only_wa ? nullptr : c + r0 * m, wa_chunk, neq, m, alfa,
-                                         accumulate, only_wa ? 0 : 1, int(nr));
+                                         accumulate, only_wa ? 0 : 1, int(nr),
+                                         r0 == 0 ? d.prm_count + 1 : nullptr);
         else
             k_prm_diag<<<g, 256, 0, s>>>(a, b + r0 * m, only_wa ? nullptr : c + r0 * m,
                                          wa_chunk, neq, m, alfa, accumulate, only_wa ? 0 : 1,
                                          int(nr));
     }
     SPRGPU_HIP(hipGetLastError());
+    if (ksa > 0) {
+        unsigned long long cnt[2];
+        SPRGPU_HIP(hipMemcpyAsync(cnt, d.prm_count, sizeof(cnt), hipMemcpyDeviceToHost, s));
+        SPRGPU_HIP(hipStreamSynchronize(s));
+        if (cnt[0] != cnt[1]) {
+            set_error("sprprm: A has non-zeros outside the pattern (C is not valid); call "
+                      "sprgpu_set_pattern with A");
+            return -1;
+        }
+    }
     return 0;
 }
 
