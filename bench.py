@@ -80,7 +80,30 @@ def bench_product(sysm, h, sm_dev, x, b_dev, reps):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
     log(f"device SPRPRM {ms:.3f} ms per call (pattern check included), |Ax - b|/|b| = {rel:.2e}")
-    return {"ms_per_call": ms, "relres_of_fp64_solution": rel}
+    # SPRTMM with the stored lower triangle L of the same A (fill included):
+    # L^T x + L x - diag(A) x must equal A x
+    from sprgpu import host
+
+    diag_pos = host.lnz_positions(sysm, np.arange(sysm.neq), np.arange(sysm.neq)) - sysm.neq
+    dvec = a[torch.from_numpy(diag_pos).cuda()]
+    out = {}
+    for iflag in (1, 2):
+        t = h.sprtmm(a, x, iflag=iflag)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            h.sprtmm(a, x, t, iflag=iflag)
+        e1.record()
+        torch.cuda.synchronize()
+        out[iflag] = (t, e0.elapsed_time(e1) / reps)
+    d = h.sprtmm(dvec, x, iflag=1, ksu=0)
+    sym = out[1][0] + out[2][0] - d
+    rel_tmm = float(torch.linalg.norm(sym - c) / torch.linalg.norm(c))
+    log(f"device SPRTMM L^T x {out[1][1]:.3f} ms, L x {out[2][1]:.3f} ms; "
+        f"|L^T x + L x - D x - A x|/|A x| = {rel_tmm:.2e}")
+    return {"ms_per_call": ms, "relres_of_fp64_solution": rel,
+            "sprtmm_upper_ms": out[1][1], "sprtmm_lower_ms": out[2][1],
+            "sprtmm_vs_sprprm_rel": rel_tmm, "lnz_bytes": 8 * sysm.nzerol}
 
 
 def bench_assembly(model, sysm, sm, h, reps, t_host):

@@ -173,6 +173,19 @@ int sprgpu_sprprm(sprgpu_handle *handle, const double *a_dev, const double *b_de
                   double *c_dev, double *wa_dev, sprgpu_int m, sprgpu_int n, sprgpu_int ksa,
                   sprgpu_int iflag);
 
+/* Replaces SPRTMM (src/SPR/sprtmm.f:5-7) on device arrays.  With L the stored
+ * lower triangle, diagonal included, of U (NZEROL doubles in the LNZ layout:
+ * an assembled matrix or a factor), for each of the N columns of B(M,N):
+ * IFLAG 1: C = L^T B; 2: C = L B; 11 / 22 (N = 1): the same into WA only.
+ * KSU <= 0: U is a diagonal of NEQ doubles.  wa_dev (M doubles; may be NULL
+ * unless IFLAG is 11 or 22) receives the last column, as the reference's WA.
+ * Every stored entry is used, in the reference's order, so C equals
+ * SPRTMM's bit for bit.  Returns 0, -2 (IFLAG, N, missing C/WA, or PERI2E,
+ * MSPAR(58) >= 1, not supported) or -3 (M < NEQ) as the reference's IERR. */
+int sprgpu_sprtmm(sprgpu_handle *handle, const double *u_dev, const double *b_dev,
+                  double *c_dev, double *wa_dev, sprgpu_int m, sprgpu_int n, sprgpu_int ksu,
+                  sprgpu_int iflag);
+
 /* Solves A X = B with the current factor plus FP64 iterative refinement
  * until max_col ||B - A X|| / ||B|| <= rtol or maxit steps. */
 int sprgpu_solve_ir(sprgpu_handle *handle, const double *b_dev, double *x_dev, sprgpu_int ld,

@@ -93,3 +93,20 @@ def assemble_free(mspar, mtrees, msifa, meen_ptr, meen, em, em_off, escale=None,
     rc = f(_p(mspar), _p(mtrees), _p(msifa), C.c_int64(meen_ptr.size - 1), _p(meen_ptr),
            _p(meen), _p(em), _p(em_off), _p(sc) if sc is not None else None, _p(sm))
     return int(rc), sm
+
+
+def sprtmm(u, b, mspar, mtrees, msifa, iflag=1, ksu=1):
+    """Restated SPRTMM: IFLAG 1 C = L^T B, 2 C = L B (L the stored lower
+    triangle of U); 11/22 with one column: the result in WA.  Returns
+    (ierr, c, wa)."""
+    b = np.asfortranarray(b, dtype=np.float64)
+    if b.ndim == 1:
+        b = b[:, None]
+    m, n = b.shape
+    c = np.zeros((m, n), order="F")
+    wa = np.zeros(m)
+    ierr = C.c_int64(0)
+    lib().oracle_sprtmm(_p(np.ascontiguousarray(u, dtype=np.float64)), _p(b), _p(c), _p(wa),
+                        _p(mspar), _p(mtrees), _p(msifa), _i(m), _i(n), _i(ksu), _i(iflag),
+                        _i(-1), C.byref(ierr))
+    return int(ierr.value), c, wa

@@ -94,3 +94,18 @@ def prm(a, b, mspar, mtrees, msifa, iflag=1, ksa=1, c=None):
                   _p(mspar), _p(mtrees), _p(msifa), _i(m), _i(n), _i(ksa), _i(iflag), _i(-1),
                   C.byref(ierr))
     return int(ierr.value), c, wa
+
+
+def tmm(u, b, mspar, mtrees, msifa, iflag=1, ksu=1):
+    """Reference SPRTMM (sprtmm.f:5-7).  Returns (ierr, c, wa)."""
+    b = np.asfortranarray(b, dtype=np.float64)
+    if b.ndim == 1:
+        b = b[:, None]
+    m, n = b.shape
+    c = np.zeros((m, n), order="F")
+    wa = np.zeros(m)
+    ierr = C.c_int64(0)
+    lib().sprtmm_(_p(np.ascontiguousarray(u, dtype=np.float64)), _p(b), _p(c), _p(wa),
+                  _p(mspar), _p(mtrees), _p(msifa), _i(m), _i(n), _i(ksu), _i(iflag), _i(-1),
+                  C.byref(ierr))
+    return int(ierr.value), c, wa

@@ -757,3 +757,71 @@ fint oracle_assemble_free(const fint *mspar, const fint *mtrees, const fint *msi
     free(col_sup);
     return rc;
 }
+
+/* ------------------------------------------------------------------------- */
+/* SPRTMM (sprtmm.f:5-241) with its product loop SPRTMQ (sprtmm.f:242-344).   */
+/* ------------------------------------------------------------------------- */
+void oracle_sprtmm(const double *u, const double *b, double *c, double *wa, fint *mspar,
+                   const fint *mtrees, const fint *msifa, const fint *m, const fint *n,
+                   const fint *ksu, const fint *iflag, const fint *lpu, fint *ierr)
+{
+    (void)lpu;
+    if (F1(mspar, 1) < 0) {
+        F1(mspar, 1) = -10;
+        *ierr = -1;
+        return;
+    }
+    *ierr = 0;
+    const int avoidc = (*iflag == 11 || *iflag == 22) && *n == 1;
+    if (!avoidc && *iflag != 1 && *iflag != 2) {
+        F1(mspar, 1) = -10;
+        *ierr = -2;
+        return;
+    }
+    spr_view v;
+    make_view(mspar, mtrees, msifa, &v);
+    fint neq = v.neq;
+    if (*m < neq) {
+        if (F1(mspar, 57) == 1) {
+            neq = *m;
+        } else {
+            F1(mspar, 1) = -10;
+            *ierr = -3;
+            return;
+        }
+    }
+    const int upper = (*iflag == 1 || *iflag == 11);
+    for (fint in = 0; in < *n; ++in) {
+        const double *bb = b + in * (*m);
+        if (*ksu > 0) {
+            for (fint i = 1; i <= neq; ++i) F1(wa, i) = 0.0;
+            for (fint js = 1; js <= v.nsuper; ++js) {
+                const fint fstvar = F1(v.xsuper, js), lstvar = F1(v.xsuper, js + 1) - 1;
+                fint istrt = F1(v.xlindx, js);
+                const fint istop = F1(v.xlindx, js + 1) - 1;
+                fint iplnz = F1(v.xlnz, js) - 1;
+                for (fint j = fstvar; j <= lstvar; ++j) {
+                    iplnz += 1;
+                    if (j <= neq) {
+                        F1(wa, j) = F1(wa, j) + F1(u, iplnz) * F1(bb, j);
+                        for (fint i = istrt + 1; i <= istop; ++i) {
+                            iplnz += 1;
+                            const fint ii = F1(v.lindx, i);
+                            if (ii <= neq) {
+                                if (upper)
+                                    F1(wa, j) = F1(wa, j) + F1(u, iplnz) * F1(bb, ii);
+                                else
+                                    F1(wa, ii) = F1(wa, ii) + F1(u, iplnz) * F1(bb, j);
+                            }
+                        }
+                    }
+                    istrt += 1;
+                }
+            }
+        } else {
+            for (fint j = 1; j <= neq; ++j) F1(wa, j) = F1(u, j) * F1(bb, j);
+        }
+        if (!avoidc)
+            for (fint jn = 1; jn <= *m; ++jn) c[in * (*m) + jn - 1] = F1(wa, jn);
+    }
+}

@@ -13,6 +13,8 @@
  *   oracle_sprsol  <- SPRSOL  src/SPR/sprsol.f:5-6
  *   oracle_sprprm  <- SPRPRM  src/SPR/sprprm.f:5-7 (C = +-A*B on the unfactored
  *                              SPR layout; used as the residual checker)
+ *   oracle_sprtmm  <- SPRTMM  src/SPR/sprtmm.f:5-241 (C = U B or U^T B with the
+ *                              stored lower triangle)
  *   oracle_assemble_free <- SPRADM called once per element (spradm.f:5-166),
  *                              its SPRAD2 free-dof loop (spradm.f:818-859)
  * The MVMULT/MMMULT callback arguments of SPRFC1/SPRFC3 are fixed to the
@@ -56,4 +58,11 @@ void oracle_sprprm(const double *a, const double *b, double *c, double *wa, fint
 fint oracle_assemble_free(const fint *mspar, const fint *mtrees, const fint *msifa, fint nel,
                           const fint *meen_ptr, const fint *meen, const double *em,
                           const fint *em_off, const double *escale, double *sm);
+/* SPRTMM(U,B,C,WA,MSPAR,MTREES,MSIFA,M,N,KSU,IFLAG,LPU,IERR): with L the
+ * stored lower triangle of U (diagonal included), IFLAG 1: C = L^T B, 2:
+ * C = L B; 11/22 (N = 1): the same into WA only (SPRTMQ, sprtmm.f:242-344).
+ * KSU <= 0: U is a diagonal.  MSPAR(58)>=1 (PERI2E, SPRTMR) is not restated. */
+void oracle_sprtmm(const double *u, const double *b, double *c, double *wa, fint *mspar,
+                   const fint *mtrees, const fint *msifa, const fint *m, const fint *n,
+                   const fint *ksu, const fint *iflag, const fint *lpu, fint *ierr);
 #endif

@@ -81,17 +81,25 @@ int run_element_grad(sprgpu_handle *h, i64 nel, int nedof, const i64 *elem_eq, c
     return 0;
 }
 
-int run_pattern_grad(sprgpu_handle *h, const double *lam, const double *x, double *glnz)
+int upload_col_node(sprgpu_handle *h)
 {
     const Plan &p = h->plan;
     if (!h->d.col_node) {
-        std::vector<i32> cn(p.neq);
+        std::vector<i32> cn(std::max<i64>(p.neq, 1));
         for (i64 j = 0; j < p.nsuper; ++j)
             for (i32 k = 0; k < p.nodes[j].S; ++k) cn[p.nodes[j].first + k] = i32(j);
-        SPRGPU_HIP(hipMalloc(&h->d.col_node, sizeof(i32) * p.neq));
-        SPRGPU_HIP(hipMemcpy(h->d.col_node, cn.data(), sizeof(i32) * p.neq, hipMemcpyHostToDevice));
+        SPRGPU_HIP(hipMalloc(&h->d.col_node, sizeof(i32) * cn.size()));
+        SPRGPU_HIP(hipMemcpy(h->d.col_node, cn.data(), sizeof(i32) * cn.size(),
+                             hipMemcpyHostToDevice));
         h->device_bytes += sizeof(i32) * p.neq;
     }
+    return 0;
+}
+
+int run_pattern_grad(sprgpu_handle *h, const double *lam, const double *x, double *glnz)
+{
+    const Plan &p = h->plan;
+    if (int rc = upload_col_node(h)) return rc;
     k_pattern_grad<<<unsigned(p.neq), 256, 0, h->stream>>>(h->d.col_node, h->d.nodes, h->d.lindx,
                                                             lam, x, glnz);
     SPRGPU_HIP(hipGetLastError());

@@ -44,7 +44,8 @@ static void free_all(sprgpu_handle *h)
                     d.csr_src,  d.csr_val,    d.irwork,      d.level_nodes,
                     d.solve_small, d.wide,     d.wide_fwd_prefix, d.wide_bwd_prefix,
                     d.wide_rect_prefix, d.wide_frect_prefix, d.upd_ptr, d.upd_src, d.flags, d.partials,
-                    d.spin_error, d.col_node, d.epoch, d.prm_count};
+                    d.spin_error, d.col_node, d.epoch, d.prm_count,
+                    d.tmm_rptr, d.tmm_s, d.tmm_p};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     free_assembly(h);
@@ -68,7 +69,8 @@ static void free_plan_buffers(sprgpu_handle *h)
                      (void **)&d.wide_bwd_prefix, (void **)&d.wide_rect_prefix,
                      (void **)&d.wide_frect_prefix,
                      (void **)&d.upd_ptr,       (void **)&d.upd_src,
-                     (void **)&d.col_node};
+                     (void **)&d.col_node,      (void **)&d.tmm_rptr,
+                     (void **)&d.tmm_s,         (void **)&d.tmm_p};
     for (void **q : ptrs) {
         if (*q) (void)hipFree(*q);
         *q = nullptr;
@@ -425,6 +427,13 @@ int sprgpu_sprprm(sprgpu_handle *h, const double *a_dev, const double *b_dev, do
 {
     if (!h || !a_dev || !b_dev) return -2;
     return run_sprprm(h, a_dev, b_dev, c_dev, wa_dev, m, n, ksa, iflag);
+}
+
+int sprgpu_sprtmm(sprgpu_handle *h, const double *u_dev, const double *b_dev, double *c_dev,
+                  double *wa_dev, sprgpu_int m, sprgpu_int n, sprgpu_int ksu, sprgpu_int iflag)
+{
+    if (!h || !u_dev || !b_dev) return -2;
+    return run_sprtmm(h, u_dev, b_dev, c_dev, wa_dev, m, n, ksu, iflag);
 }
 
 int sprgpu_solve_ir(sprgpu_handle *h, const double *b, double *x, sprgpu_int ld,

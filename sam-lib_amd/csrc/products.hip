@@ -1,0 +1,200 @@
+// SPRTMM on the device: products with the stored lower triangle of an SPR
+// matrix (sprtmm.f:5-241, its product loop SPRTMQ at sprtmm.f:242-344).
+//
+// With L the stored lower triangle (diagonal included) of U in the LNZ
+// layout, IFLAG 1 gives C = L^T B and IFLAG 2 gives C = L B.  SPRTMQ walks
+// the packed columns in order from C = 0:
+// * IFLAG 1 adds column J's entries to C(J), the diagonal first and then the
+//   rows below in LINDX order: each C(J) is a sum over one packed column,
+//   which is contiguous in LNZ;
+// * IFLAG 2 adds L(r,J) B(J) to C(r) as it meets column J: each C(r) is a
+//   sum over row r of L in ascending column order.  Row r's entries lie in
+//   the supernodes whose row lists hold r, columns 0..p of supernode s when r
+//   is at position p of its list; a row index at supernode granularity
+//   (NOFSUB entries, built on the host once) gives them in that order.
+// One lane per column or row adds the products, each rounded before it is
+// added, in the reference's order, so C equals SPRTMM's bit for bit; every
+// stored entry is used, fill included.  The loads of eight entries are issued
+// before their products are added.
+#include <algorithm>
+
+#include "sprgpu_internal.h"
+
+namespace sprgpu {
+
+constexpr int kTmmBatch = 8;
+
+__global__ __launch_bounds__(256) void k_tmm_cols(const i32 *__restrict__ col_node,
+                                                  const NodeInfo *__restrict__ nodes,
+                                                  const i32 *__restrict__ lindx,
+                                                  const double *__restrict__ u,
+                                                  const double *__restrict__ b, double *c,
+                                                  double *wa, i64 n, i64 ld, int write_c,
+                                                  int nrhs)
+{
+#pragma clang fp contract(off)
+    const i64 j = i64(blockIdx.x) * 256 + threadIdx.x;
+    const int rhs = blockIdx.y;
+    if (j >= n) return;
+    const NodeInfo nd = nodes[col_node[j]];
+    const i64 k = j - nd.first;
+    const double *uc = u + nd.xlnz + k * nd.F - k * (k - 1) / 2;
+    const i32 *rows = lindx + nd.lxoff + k;
+    const i64 len = nd.F - k;
+    const double *bb = b + rhs * ld;
+    double s = 0.0;
+    for (i64 q0 = 0; q0 < len; q0 += kTmmBatch) {
+        double uv[kTmmBatch], bv[kTmmBatch];
+#pragma unroll
+        for (int q = 0; q < kTmmBatch; ++q) {
+            const i64 t = min(q0 + q, len - 1);
+            uv[q] = uc[t];
+            bv[q] = bb[rows[t]];
+        }
+#pragma unroll
+        for (int q = 0; q < kTmmBatch; ++q)
+            if (q0 + q < len) {
+                const double p = uv[q] * bv[q];
+                s = s + p;
+            }
+    }
+    if (write_c) c[rhs * ld + j] = s;
+    if (wa && rhs == nrhs - 1) wa[j] = s;
+}
+
+__global__ __launch_bounds__(256) void k_tmm_rows(const i64 *__restrict__ rptr,
+                                                  const i32 *__restrict__ rs,
+                                                  const i32 *__restrict__ rp,
+                                                  const NodeInfo *__restrict__ nodes,
+                                                  const double *__restrict__ u,
+                                                  const double *__restrict__ b, double *c,
+                                                  double *wa, i64 n, i64 ld, int write_c,
+                                                  int nrhs)
+{
+#pragma clang fp contract(off)
+    const i64 r = i64(blockIdx.x) * 256 + threadIdx.x;
+    const int rhs = blockIdx.y;
+    if (r >= n) return;
+    const double *bb = b + rhs * ld;
+    double s = 0.0;
+    for (i64 e = rptr[r]; e < rptr[r + 1]; ++e) {
+        const NodeInfo nd = nodes[rs[e]];
+        const i64 p = rp[e];
+        const i64 kend = min<i64>(nd.S, p + 1);  // columns 0..p hold row r
+        // column k's entry of row r: xlnz + k*F - k*(k-1)/2 + (p - k)
+        for (i64 k0 = 0; k0 < kend; k0 += kTmmBatch) {
+            double uv[kTmmBatch], bv[kTmmBatch];
+#pragma unroll
+            for (int q = 0; q < kTmmBatch; ++q) {
+                const i64 k = min(k0 + q, kend - 1);
+                uv[q] = u[nd.xlnz + k * nd.F - k * (k - 1) / 2 + (p - k)];
+                bv[q] = bb[nd.first + k];
+            }
+#pragma unroll
+            for (int q = 0; q < kTmmBatch; ++q)
+                if (k0 + q < kend) {
+                    const double pr = uv[q] * bv[q];
+                    s = s + pr;
+                }
+        }
+    }
+    if (write_c) c[rhs * ld + r] = s;
+    if (wa && rhs == nrhs - 1) wa[r] = s;
+}
+
+// KSU <= 0: U is a diagonal held as a vector, C(J) = U(J)*B(J)
+__global__ __launch_bounds__(256) void k_tmm_diag(const double *__restrict__ u,
+                                                  const double *__restrict__ b, double *c,
+                                                  double *wa, i64 n, i64 ld, int write_c,
+                                                  int nrhs)
+{
+    const i64 j = i64(blockIdx.x) * 256 + threadIdx.x;
+    const int rhs = blockIdx.y;
+    if (j >= n) return;
+    const double s = u[j] * b[rhs * ld + j];
+    if (write_c) c[rhs * ld + j] = s;
+    if (wa && rhs == nrhs - 1) wa[j] = s;
+}
+
+static int upload_row_index(sprgpu_handle *h)
+{
+    DeviceBuffers &d = h->d;
+    if (d.tmm_rptr) return 0;
+    const Plan &p = h->plan;
+    std::vector<i64> rptr(p.neq + 1, 0);
+    for (i64 k = 0; k < p.nofsub; ++k) rptr[p.lindx[k] + 1]++;
+    for (i64 r = 0; r < p.neq; ++r) rptr[r + 1] += rptr[r];
+    std::vector<i32> rs(std::max<i64>(p.nofsub, 1)), rp(std::max<i64>(p.nofsub, 1));
+    std::vector<i64> cur(rptr.begin(), rptr.end() - 1);
+    for (i64 s = 0; s < p.nsuper; ++s) {  // supernodes ascending: each row's list in order
+        const NodeInfo &n = p.nodes[s];
+        for (i32 q = 0; q < n.F; ++q) {
+            const i64 o = cur[p.lindx[n.lxoff + q]]++;
+            rs[o] = i32(s);
+            rp[o] = q;
+        }
+    }
+    SPRGPU_HIP(hipMalloc(&d.tmm_rptr, sizeof(i64) * rptr.size()));
+    SPRGPU_HIP(hipMalloc(&d.tmm_s, sizeof(i32) * rs.size()));
+    SPRGPU_HIP(hipMalloc(&d.tmm_p, sizeof(i32) * rp.size()));
+    SPRGPU_HIP(hipMemcpy(d.tmm_rptr, rptr.data(), sizeof(i64) * rptr.size(),
+                         hipMemcpyHostToDevice));
+    SPRGPU_HIP(hipMemcpy(d.tmm_s, rs.data(), sizeof(i32) * rs.size(), hipMemcpyHostToDevice));
+    SPRGPU_HIP(hipMemcpy(d.tmm_p, rp.data(), sizeof(i32) * rp.size(), hipMemcpyHostToDevice));
+    h->device_bytes += sizeof(i64) * rptr.size() + 2 * sizeof(i32) * p.nofsub;
+    return 0;
+}
+
+int run_sprtmm(sprgpu_handle *h, const double *u, const double *b, double *c, double *wa,
+               i64 m, i64 n, i64 ksu, i64 iflag)
+{
+    const Plan &p = h->plan;
+    const i64 neq = p.neq;
+    const bool only_wa = (iflag == 11 || iflag == 22) && n == 1;
+    if (!only_wa && iflag != 1 && iflag != 2) {
+        set_error("sprtmm: IFLAG must be 1, 2, or 11/22 with N = 1");
+        return -2;
+    }
+    if (m < neq) {
+        set_error("sprtmm: M < NEQ (retained equations are not supported)");
+        return -3;
+    }
+    if (h->mspar.size() >= 58 && h->mspar[57] >= 1) {
+        set_error("sprtmm: the PERI2E permutation (MSPAR(58) >= 1) is not supported");
+        return -2;
+    }
+    if (n < 1 || (only_wa && !wa) || (!only_wa && !c)) {
+        set_error("sprtmm: N must be positive; C (or WA for IFLAG 11/22) is required");
+        return -2;
+    }
+    const bool upper = (iflag == 1 || iflag == 11);
+    if (ksu > 0) {
+        if (upper) {
+            if (int rc = upload_col_node(h)) return rc;
+        } else if (int rc = upload_row_index(h)) {
+            return rc;
+        }
+    }
+    hipStream_t s = h->stream;
+    DeviceBuffers &d = h->d;
+    const unsigned gx = unsigned(std::max<i64>(1, (neq + 255) / 256));
+    for (i64 r0 = 0; r0 < n; r0 += 65535) {
+        const i64 nr = std::min<i64>(65535, n - r0);
+        const dim3 g(gx, unsigned(nr));
+        double *cc = only_wa ? nullptr : c + r0 * m;
+        double *wc = (r0 + nr == n) ? wa : nullptr;
+        const int wr = only_wa ? 0 : 1;
+        if (ksu <= 0)
+            k_tmm_diag<<<g, 256, 0, s>>>(u, b + r0 * m, cc, wc, neq, m, wr, int(nr));
+        else if (upper)
+            k_tmm_cols<<<g, 256, 0, s>>>(d.col_node, d.nodes, d.lindx, u, b + r0 * m, cc, wc,
+                                         neq, m, wr, int(nr));
+        else
+            k_tmm_rows<<<g, 256, 0, s>>>(d.tmm_rptr, d.tmm_s, d.tmm_p, d.nodes, u, b + r0 * m,
+                                         cc, wc, neq, m, wr, int(nr));
+    }
+    SPRGPU_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace sprgpu

@@ -228,6 +228,11 @@ struct DeviceBuffers {
     int *spin_error = nullptr;  // set when a dataflow wait gave up
     unsigned long long *prm_count = nullptr;  // non-zero counts of a product's A (2)
     i32 *col_node = nullptr;    // supernode of each equation (adjoint pattern gradient)
+    // row index of the stored lower triangle by supernode (SPRTMM's L B):
+    // row r appears in supernode tmm_s[k] at row position tmm_p[k],
+    // k in [tmm_rptr[r], tmm_rptr[r+1]), supernodes ascending
+    i64 *tmm_rptr = nullptr;
+    i32 *tmm_s = nullptr, *tmm_p = nullptr;
     i64 irwork_cap = 0;
     // element assembly map: LNZ positions that receive element entries
     // (ascending), the range of each one's contributions, and the
@@ -299,6 +304,10 @@ int run_sprprm(sprgpu_handle *h, const double *a, const double *b, double *c, do
 int run_element_grad(sprgpu_handle *h, i64 nel, int nedof, const i64 *elem_eq, const double *k0,
                      const i64 *k0_index, const double *lam, const double *x, double *grad);
 int run_pattern_grad(sprgpu_handle *h, const double *lam, const double *x, double *glnz);
+int upload_col_node(sprgpu_handle *h);  // supernode of every equation, once
+// products.hip
+int run_sprtmm(sprgpu_handle *h, const double *u, const double *b, double *c, double *wa,
+               i64 m, i64 n, i64 ksu, i64 iflag);
 // assemble.hip
 int set_assembly(sprgpu_handle *h, i64 nel, const i64 *meen_ptr, const i64 *meen);
 int run_assemble(sprgpu_handle *h, const double *em, const i64 *em_off, const double *escale,

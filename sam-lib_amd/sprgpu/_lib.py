@@ -49,6 +49,7 @@ SIGNATURES = {
     "sprgpu_solve": (C.c_int, [P, P, I64, I64, C.c_int]),
     "sprgpu_spmv": (C.c_int, [P, P, P, I64, I64]),
     "sprgpu_sprprm": (C.c_int, [P, P, P, P, P, I64, I64, I64, I64]),
+    "sprgpu_sprtmm": (C.c_int, [P, P, P, P, P, I64, I64, I64, I64]),
     "sprgpu_solve_ir": (C.c_int, [P, P, P, I64, I64, C.c_double, C.c_int, C.POINTER(IrInfo)]),
     "sprgpu_get_factor": (C.c_int, [P, P]),
     "sprgpu_neq": (I64, [P]),

@@ -150,6 +150,21 @@ class SprFactor:
               "sprgpu_sprprm")
         return c_dev if iflag != 88 else wa_dev
 
+    def sprtmm(self, u_dev, b_dev, c_dev=None, iflag: int = 1, ksu: int = 1, wa_dev=None):
+        """SPRTMM on the device: C = L^T B (iflag 1) or L B (2) with L the stored
+        lower triangle of u_dev (LNZ, or a diagonal when ksu <= 0); 11/22 with
+        one column give the result in wa_dev only."""
+        torch = self._torch
+        b2 = b_dev if b_dev.dim() == 2 else b_dev.view(1, -1)
+        if c_dev is None and iflag in (1, 2):
+            c_dev = torch.empty_like(b2)
+        c2 = None if c_dev is None else (c_dev if c_dev.dim() == 2 else c_dev.view(1, -1))
+        opt = (lambda t: None if t is None else _ptr(t))
+        check(lib().sprgpu_sprtmm(self._h, _ptr(u_dev), _ptr(b2), opt(c2), opt(wa_dev),
+                                  b2.shape[1], b2.shape[0], int(ksu), int(iflag)),
+              "sprgpu_sprtmm")
+        return c_dev if iflag in (1, 2) else wa_dev
+
     def solve_ir(self, b_dev, x_dev, rtol: float = 1e-10, maxit: int = 20) -> dict:
         b2 = b_dev if b_dev.dim() == 2 else b_dev.view(1, -1)
         x2 = x_dev if x_dev.dim() == 2 else x_dev.view(1, -1)

@@ -109,3 +109,43 @@ def test_sprprm_on_the_assembly_pattern_and_the_pattern_check():
     with pytest.raises(SprGpuError, match="IFLAG"):
         h.sprprm(_dev(a3), _dev(x.T), c_dev, iflag=3)
     h.close()
+
+
+@pytest.mark.parametrize("kind,n,kw", [("hex8", 10, dict(e_sigma=0.5, e_seed=1)),
+                                       ("quad4", 30, {}), ("frame", 300, dict(seed=3))])
+def test_sprtmm_matches_oracle(kind, n, kw):
+    """SPRTMM with the stored lower triangle of the assembled matrix and of
+    the factor (fill included): bit for bit equal to the oracle's SPRTMM,
+    which tests/test_oracle.py pins to the reference."""
+    from sprgpu.device import SprFactor
+
+    torch = _torch()
+    model, sysm, sm = make_system(kind, n, **kw)
+    fac = sm.copy()
+    assert oracle.sprsol(1, sysm.mspar.copy(), sysm.mtrees, sysm.msifa, fac)[0] == 0
+    h = SprFactor(sysm.mspar, sysm.mtrees, sysm.msifa, "fp64")
+    rng = np.random.default_rng(4)
+    x = rng.normal(size=(sysm.neq, 3))
+    x_dev = _dev(x.T)
+
+    def bits(a):
+        return np.ascontiguousarray(a).view(np.int64)
+
+    for u in (sm[sysm.neq:], fac[sysm.neq:], rng.normal(size=sysm.neq)):
+        ksu = 1 if u.size == sysm.nzerol else 0
+        u_dev = _dev(u)
+        for iflag in (1, 2):
+            wa = torch.empty(sysm.neq, dtype=torch.float64, device="cuda")
+            c = h.sprtmm(u_dev, x_dev, iflag=iflag, ksu=ksu, wa_dev=wa)
+            ierr, c_o, wa_o = oracle.sprtmm(u, x, sysm.mspar.copy(), sysm.mtrees, sysm.msifa,
+                                            iflag, ksu)
+            assert ierr == 0
+            assert np.array_equal(bits(c.cpu().numpy().T), bits(c_o)), (ksu, iflag)
+            assert np.array_equal(bits(wa.cpu().numpy()), bits(wa_o))
+        for iflag in (11, 22):
+            wa = torch.empty(sysm.neq, dtype=torch.float64, device="cuda")
+            h.sprtmm(u_dev, x_dev[:1], None, iflag=iflag, ksu=ksu, wa_dev=wa)
+            ierr, _, wa_o = oracle.sprtmm(u, x[:, :1], sysm.mspar.copy(), sysm.mtrees,
+                                          sysm.msifa, iflag, ksu)
+            assert ierr == 0 and np.array_equal(bits(wa.cpu().numpy()), bits(wa_o))
+    h.close()

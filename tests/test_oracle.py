@@ -242,6 +242,51 @@ def test_sprprm_bitexact_vs_reference(kind, n, kw):
     assert not np.array_equal(w, c_r)
 
 
+@pytest.mark.parametrize("kind,n,kw", [("hex8", 5, dict(e_sigma=0.5, e_seed=1)),
+                                       ("quad4", 10, {}), ("frame", 50, dict(seed=3))])
+def test_sprtmm_bitexact_vs_reference(kind, n, kw):
+    """The restated SPRTMM equals the reference's bit for bit (IFLAG 1, 2, 11,
+    22, a diagonal U), on the unfactored matrix and on the factor; it equals
+    L^T B and L B of the dense lower triangle; a planted change (the
+    diagonal term added last in IFLAG 1) is caught."""
+    import refspr
+
+    model, sysm, sm = make_system(kind, n, **kw)
+    rng = np.random.default_rng(7)
+    x = rng.normal(size=(sysm.neq, 2))
+    fac = sm.copy()
+    assert oracle.sprsol(1, sysm.mspar.copy(), sysm.mtrees, sysm.msifa, fac)[0] == 0
+    for u in (sm[sysm.neq:], fac[sysm.neq:]):
+        for ksu, uu in ((1, u), (0, rng.normal(size=sysm.neq))):
+            for iflag in (1, 2):
+                r = refspr.tmm(uu, x, sysm.mspar.copy(), sysm.mtrees, sysm.msifa, iflag, ksu)
+                o = oracle.sprtmm(uu, x, sysm.mspar.copy(), sysm.mtrees, sysm.msifa, iflag, ksu)
+                assert r[0] == o[0] == 0
+                assert np.array_equal(r[1].view(np.int64), o[1].view(np.int64)), (ksu, iflag)
+                assert np.array_equal(r[2].view(np.int64), o[2].view(np.int64))
+            for iflag in (11, 22):
+                r = refspr.tmm(uu, x[:, :1], sysm.mspar.copy(), sysm.mtrees, sysm.msifa, iflag, ksu)
+                o = oracle.sprtmm(uu, x[:, :1], sysm.mspar.copy(), sysm.mtrees, sysm.msifa, iflag,
+                                  ksu)
+                assert r[0] == o[0] == 0
+                assert np.array_equal(r[2].view(np.int64), o[2].view(np.int64))
+    # mathematics: the dense lower triangle of the unfactored matrix
+    A = dense_equation_matrix(model, sysm)
+    Lt = np.tril(A)
+    _, c1, _ = oracle.sprtmm(sm[sysm.neq:], x, sysm.mspar.copy(), sysm.mtrees, sysm.msifa, 1)
+    _, c2, _ = oracle.sprtmm(sm[sysm.neq:], x, sysm.mspar.copy(), sysm.mtrees, sysm.msifa, 2)
+    tol = 1e-12 * np.abs(A).max() * np.abs(x).max()
+    np.testing.assert_allclose(c1, Lt.T @ x, rtol=1e-12, atol=tol)
+    np.testing.assert_allclose(c2, Lt @ x, rtol=1e-12, atol=tol)
+    # planted mistake: column sums with the diagonal term last
+    w = np.zeros(sysm.neq)
+    for j in range(sysm.neq):
+        rows = np.nonzero(Lt[:, j])[0]
+        for r_ in np.concatenate([rows[rows > j], rows[rows == j]]):
+            w[j] += Lt[r_, j] * x[r_, 0]
+    assert not np.array_equal(w, c1[:, 0])
+
+
 def test_planted_mistake_is_caught():
     """A dropped Schur term in the factor must break the dense-LDL' check."""
     import refspr
