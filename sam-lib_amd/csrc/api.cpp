@@ -45,7 +45,7 @@ static void free_all(sprgpu_handle *h)
                     d.solve_small, d.wide,     d.wide_fwd_prefix, d.wide_bwd_prefix,
                     d.wide_rect_prefix, d.wide_frect_prefix, d.upd_ptr, d.upd_src, d.flags, d.partials,
                     d.spin_error, d.col_node, d.epoch, d.prm_count,
-                    d.tmm_rptr, d.tmm_s, d.tmm_p};
+                    d.tmm_rptr, d.tmm_segs, d.tmm_long};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     free_assembly(h);
@@ -70,7 +70,7 @@ static void free_plan_buffers(sprgpu_handle *h)
                      (void **)&d.wide_frect_prefix,
                      (void **)&d.upd_ptr,       (void **)&d.upd_src,
                      (void **)&d.col_node,      (void **)&d.tmm_rptr,
-                     (void **)&d.tmm_s,         (void **)&d.tmm_p};
+                     (void **)&d.tmm_segs,      (void **)&d.tmm_long};
     for (void **q : ptrs) {
         if (*q) (void)hipFree(*q);
         *q = nullptr;

@@ -67,6 +67,10 @@ struct NodeInfo {
 
 // Work descriptors consumed by the kernels (built once per plan).
 struct ColTask { i32 node, c0, c1, pad; };        // front columns [c0,c1)
+// one supernode's part of a row of L: columns 0..kend-1 of the supernode
+// (first equation `first`, order F); column k's entry at base + k*(F-1) -
+// k*(k-1)/2 in LNZ
+struct TmmSeg { i64 base; i32 F, first, kend, pad; };
 struct EaTask { i32 child, c0, c1, pad; };        // child update columns [c0,c1)
 struct PanelDesc { i32 node, k0, nb, nrb; };       // panel step at column k0, nrb row blocks
 // Lower-trapezoidal tile set: rows [r0,rend) x cols [r0,cend) (square tiles
@@ -229,10 +233,12 @@ struct DeviceBuffers {
     unsigned long long *prm_count = nullptr;  // non-zero counts of a product's A (2)
     i32 *col_node = nullptr;    // supernode of each equation (adjoint pattern gradient)
     // row index of the stored lower triangle by supernode (SPRTMM's L B):
-    // row r appears in supernode tmm_s[k] at row position tmm_p[k],
-    // k in [tmm_rptr[r], tmm_rptr[r+1]), supernodes ascending
+    // row r's segments tmm_segs[k], k in [tmm_rptr[r], tmm_rptr[r+1]),
+    // supernodes ascending
     i64 *tmm_rptr = nullptr;
-    i32 *tmm_s = nullptr, *tmm_p = nullptr;
+    TmmSeg *tmm_segs = nullptr;
+    i32 *tmm_long = nullptr;    // columns summed by a wavefront each (SPRTMM's L^T B)
+    i64 tmm_nlong = 0;
     i64 irwork_cap = 0;
     // element assembly map: LNZ positions that receive element entries
     // (ascending), the range of each one's contributions, and the
