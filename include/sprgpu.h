@@ -138,7 +138,14 @@ int sprgpu_assemble(sprgpu_handle *handle, const double *em_dev, const sprgpu_in
 
 /* Numeric factorization of SM given in device memory (NEQ+NZEROL doubles, the
  * SPRADM layout; not modified).  tol points to host TOL(1..4) (TOL(4) is read
- * only when mode==3).  mode 1 = SPRFC1 semantics, 3 = SPRFC3 semantics.
+ * only when mode==3).  mode 1 = SPRFC1 semantics, 3 = SPRFC3 semantics,
+ * 2 = SPRFC2 (src/SPR/sprfc2.f:5-7, the CC' form: a pivot fails when
+ * d <= TOL(1)*|a_jj|, sign included, sprfc2.f:934-1037; IERR, SUPEQT, TOL(2:3)
+ * and MSPAR(26) as SPRFC2 reports them).  After a mode-2 factorization,
+ * sprgpu_solve with which 1 is SPRFS2 (C^-1 B), 2 is SPRBS2 (C^-T B), 3 both,
+ * and sprgpu_get_factor gives C in SM layout.  The CC' factor is C = L
+ * sqrt(D) of the LDL' factorization, so it agrees with the reference's to
+ * rounding, not bit for bit.
  * info may be NULL (then the call returns without synchronizing). */
 int sprgpu_factor(sprgpu_handle *handle, const double *sm_dev, const double *tol, int mode,
                   sprgpu_factor_info *info);

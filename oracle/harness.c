@@ -36,3 +36,19 @@ fint spr_sol_timed(fint iop, fint *mspar, fint *mtrees, fint *msifa, double *sm,
     *seconds = now_s() - t0;
     return ierr;
 }
+
+/* SPRFC2 (sprfc2.f:5-7) with the CC' kernels SPRSOL-style callers pass:
+ * CMVY18 / CMMY18 (cmvy18.f:5, cmmy18.f:5). */
+extern void cmvy18_(void);
+extern void cmmy18_(void);
+extern void sprfc2_(fint *mspar, fint *mtrees, fint *msifa, double *sm, double *tol,
+                    fint *iwork, double *rwork, const fint *lpu, fint *ierr,
+                    void (*mvmult)(void), void (*mmmult)(void));
+
+fint spr_fc2(fint *mspar, fint *mtrees, fint *msifa, double *sm, double *tol, fint *iwork,
+             double *rwork, fint lpu)
+{
+    fint ierr = 0;
+    sprfc2_(mspar, mtrees, msifa, sm, tol, iwork, rwork, &lpu, &ierr, cmvy18_, cmmy18_);
+    return ierr;
+}

@@ -109,3 +109,38 @@ def tmm(u, b, mspar, mtrees, msifa, iflag=1, ksu=1):
                   _p(mspar), _p(mtrees), _p(msifa), _i(m), _i(n), _i(ksu), _i(iflag), _i(-1),
                   C.byref(ierr))
     return int(ierr.value), c, wa
+
+
+def fc2(sysm: SprSystem, sm: np.ndarray, tol=(0.0, 0.0, 0.0), mspar: np.ndarray | None = None):
+    """Reference SPRFC2 (sprfc2.f:5-7), the CC' factorization, with CMVY18 /
+    CMMY18.  ``sm`` and ``mspar`` are modified in place.  Returns (ierr, tol)."""
+    mspar = sysm.mspar if mspar is None else mspar
+    mfront = int(mspar[13])
+    iwork = np.zeros(max(mfront + 1, int(mspar[12])) + 1, dtype=I8)
+    rwork = np.zeros(max(int(mspar[16]), mfront) + 1)
+    tolv = np.asarray(tol, dtype=np.float64).copy()
+    f = lib().spr_fc2
+    f.restype = C.c_int64
+    ierr = f(_p(mspar), _p(sysm.mtrees), _p(sysm.msifa), _p(sm), _p(tolv), _p(iwork), _p(rwork),
+             C.c_int64(-1))
+    return int(ierr), tolv
+
+
+def fs2_bs2(which: str, sysm: SprSystem, sm: np.ndarray, b: np.ndarray,
+            mspar: np.ndarray | None = None):
+    """Reference SPRFS2 (C y = b, sprfs2.f:5) or SPRBS2 (C^T x = y, sprbs2.f:5)
+    in place on Fortran-ordered b (neq, nrhs).  Returns ierr."""
+    mspar = sysm.mspar if mspar is None else mspar
+    neq, mfront = int(mspar[7]), int(mspar[13])
+    assert b.flags.f_contiguous and b.dtype == np.float64
+    nrhs = b.shape[1] if b.ndim == 2 else 1
+    rwork = np.zeros(mfront + 1)
+    ierr = C.c_int64(0)
+    if which == "fs":
+        iwork = np.zeros(max(int(mspar[12]), mfront) + 1, dtype=I8)
+        lib().sprfs2_(_p(mspar), _p(sysm.mtrees), _p(sysm.msifa), _p(sm), _p(b), _i(neq),
+                      _i(nrhs), _p(iwork), _p(rwork), _i(-1), C.byref(ierr))
+    else:
+        lib().sprbs2_(_p(mspar), _p(sysm.mtrees), _p(sysm.msifa), _p(sm), _p(b), _i(neq),
+                      _i(nrhs), _p(rwork), _i(-1), C.byref(ierr))
+    return int(ierr.value)

@@ -381,13 +381,18 @@ int sprgpu_factor(sprgpu_handle *h, const double *sm_dev, const double *tol, int
                   sprgpu_factor_info *info)
 {
     if (!h || !sm_dev || !tol) return -2;
-    if (mode != 1 && mode != 3) {
-        set_error("mode must be 1 (SPRFC1) or 3 (SPRFC3)");
+    if (mode != 1 && mode != 2 && mode != 3) {
+        set_error("mode must be 1 (SPRFC1), 2 (SPRFC2) or 3 (SPRFC3)");
         return -2;
     }
-    double t4[4] = {tol[0], 0.0, 0.0, mode == 3 ? tol[3] : 0.0};
-    int rc = run_factor(h, sm_dev, mode, t4);
+    // SPRFC2: the LDL' factorization with every pivot computed, then its test
+    double t4[4] = {mode == 2 ? 0.0 : tol[0], 0.0, 0.0, mode == 3 ? tol[3] : 0.0};
+    int rc = run_factor(h, sm_dev, mode == 2 ? 1 : mode, t4);
     if (rc) return rc;
+    if (mode == 2) {
+        h->mode = 2;
+        h->cc_tol1 = tol[0];
+    }
     if (info) return read_factor_info(h, info);
     return 0;
 }
@@ -413,6 +418,15 @@ int sprgpu_factor_info_get(sprgpu_handle *h, sprgpu_factor_info *info)
 int sprgpu_solve(sprgpu_handle *h, double *b, sprgpu_int ldb, sprgpu_int nrhs, int which)
 {
     if (!h || !b || ldb < h->plan.neq || which < 1 || which > 3) return -2;
+    if (h->mode == 2 && which != 3) {
+        // C^-1 b = sqrt(D) D^-1 L^-1 b; C^-T y = L^-T D^-1/2 y
+        if (which == 1) {
+            if (int rc = run_solve(h, b, ldb, nrhs, 1)) return rc;
+            return cc_scale(h, b, ldb, nrhs, 0);
+        }
+        if (int rc = cc_scale(h, b, ldb, nrhs, 1)) return rc;
+        return run_solve(h, b, ldb, nrhs, 2);
+    }
     return run_solve(h, b, ldb, nrhs, which);
 }
 
@@ -471,13 +485,18 @@ int sprgpu_get_factor(sprgpu_handle *h, double *sm_out)
     (void)hipGetLastError();
     const Plan &p = h->plan;
     const size_t bytes = sizeof(double) * size_t(p.neq + p.nzerol);
-    if (on_device) return copy_factor_to_sm(h, sm_out);
+    if (on_device) {
+        int rc = copy_factor_to_sm(h, sm_out);
+        if (rc == 0 && h->mode == 2) rc = cc_factor_layout(h, sm_out);
+        return rc;
+    }
     double *tmp = nullptr;
     SPRGPU_HIP(hipMalloc(&tmp, bytes));
     // start from the input so that entries the factor does not own keep it
     if (h->last_sm_dev)
         SPRGPU_HIP(hipMemcpyAsync(tmp, h->last_sm_dev, bytes, hipMemcpyDeviceToDevice, h->stream));
     int rc = copy_factor_to_sm(h, tmp);
+    if (rc == 0 && h->mode == 2) rc = cc_factor_layout(h, tmp);
     if (rc == 0) {
         hipError_t e = hipMemcpyAsync(sm_out, tmp, bytes, hipMemcpyDeviceToHost, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);

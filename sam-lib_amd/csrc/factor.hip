@@ -1352,6 +1352,7 @@ int profile_factor(sprgpu_handle *h, const double *sm_dev, int mode, const doubl
 
 int read_factor_info(sprgpu_handle *h, sprgpu_factor_info *info)
 {
+    if (h->mode == 2) return cc_factor_info(h, info);
     FactorStatus st;
     double agg[4];
     SPRGPU_HIP(hipMemcpyAsync(&st, h->d.status, sizeof(st), hipMemcpyDeviceToHost, h->stream));

@@ -258,7 +258,8 @@ struct sprgpu_handle {
     int device = 0;
     bool factored = false;
     bool has_pattern = false;
-    int mode = 1;
+    int mode = 1;               // 1 SPRFC1, 2 SPRFC2 (CC' from the LDL' factor), 3 SPRFC3
+    double cc_tol1 = 0.0;       // TOL(1) of the last SPRFC2-mode factorization
     double tol[4] = {0, 0, 0, 0};
     const double *last_sm_dev = nullptr;  // input of the last factorization
     sprgpu_int device_bytes = 0;
@@ -311,6 +312,10 @@ int run_element_grad(sprgpu_handle *h, i64 nel, int nedof, const i64 *elem_eq, c
                      const i64 *k0_index, const double *lam, const double *x, double *grad);
 int run_pattern_grad(sprgpu_handle *h, const double *lam, const double *x, double *glnz);
 int upload_col_node(sprgpu_handle *h);  // supernode of every equation, once
+// cholesky.hip: the CC' form (SPRFC2/FS2/BS2) on the LDL' factor
+int cc_factor_info(sprgpu_handle *h, sprgpu_factor_info *info);
+int cc_scale(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int inverse);
+int cc_factor_layout(sprgpu_handle *h, double *sm_dev);
 // products.hip
 int run_sprtmm(sprgpu_handle *h, const double *u, const double *b, double *c, double *wa,
                i64 m, i64 n, i64 ksu, i64 iflag);

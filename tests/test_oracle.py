@@ -303,3 +303,44 @@ def test_planted_mistake_is_caught():
     C = np.linalg.cholesky(A)
     assert np.allclose(d, np.diag(C) ** 2, rtol=1e-12)
     assert not np.allclose(d_wrong, np.diag(C) ** 2, rtol=1e-12)
+
+
+def test_reference_cct_is_ldlt_scaled():
+    """The reference's SPRFC2 factor C equals L sqrt(D) from its SPRFC1 factor
+    (C C' = L D L'), and SPRFS2 / SPRBS2 are C^-1 and C^-T: the relations the
+    GPU's CC' form is built on (checked here on the reference itself)."""
+    import refspr
+
+    model, sysm, sm = make_system("hex8", 5, e_sigma=0.5, e_seed=1)
+    n = sysm.neq
+    cc, ms = sm.copy(), sysm.mspar.copy()
+    ierr, tol_c = refspr.fc2(sysm, cc, mspar=ms)
+    assert ierr == 0 and ms[0] >= 0
+    ld, ms1 = sm.copy(), sysm.mspar.copy()
+    ierr1, _, tol_l = refspr.sol(1, sysm, ld, None, mspar=ms1)
+    assert ierr1 == 0
+    d = ld[:n]
+    np.testing.assert_allclose(cc[:n], np.sqrt(d), rtol=1e-13)
+    xs, xl = sysm.xsuper() - 1, sysm.xlindx() - 1
+    scale = np.empty(sysm.nzerol)
+    pos = 0
+    for s in range(sysm.nsuper):
+        f = xl[s + 1] - xl[s]
+        for k in range(xs[s + 1] - xs[s]):
+            j = xs[s] + k
+            scale[pos:pos + f - k] = np.sqrt(d[j])
+            scale[pos] = 1.0 / np.sqrt(d[j])          # diagonal: d -> sqrt(d)
+            pos += f - k
+    np.testing.assert_allclose(cc[n:], ld[n:] * scale, rtol=1e-11,
+                               atol=1e-13 * np.abs(cc[n:]).max())
+    np.testing.assert_allclose(tol_c[1:], tol_l[1:], rtol=1e-12)
+    b = np.asfortranarray(np.random.default_rng(3).uniform(-1, 1, (n, 2)))
+    y = b.copy(order="F")
+    assert refspr.fs2_bs2("fs", sysm, cc, y, mspar=ms) == 0
+    y1 = b.copy(order="F")
+    assert oracle.sprsol(5, ms1.copy(), sysm.mtrees, sysm.msifa, ld, y1)[0] == 0  # forward only
+    np.testing.assert_allclose(y, np.sqrt(d)[:, None] * y1, rtol=1e-11, atol=1e-13)
+    assert refspr.fs2_bs2("bs", sysm, cc, y, mspar=ms) == 0
+    x = b.copy(order="F")
+    assert oracle.sprsol(3, sysm.mspar.copy(), sysm.mtrees, sysm.msifa, sm.copy(), x)[0] == 0
+    np.testing.assert_allclose(y, x, rtol=1e-10, atol=1e-12 * np.abs(x).max())
