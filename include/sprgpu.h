@@ -60,6 +60,23 @@ void sprgpu_sprbs1(sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa, dou
                    double *b, const sprgpu_int *ldb, const sprgpu_int *nrhs, double *rwork,
                    const sprgpu_int *lpu, sprgpu_int *ierr);
 
+/* Replace SPRFC2 (src/SPR/sprfc2.f:5-7), SPRFS2 (sprfs2.f:5-7) and SPRBS2
+ * (sprbs2.f:5-6), the CC' siblings: SM in, C out in the packed layout (c_jj
+ * on the diagonal and in SM(1:NEQ)); B <- C^-1 B; B <- C^-T B.  A pivot
+ * fails when d <= TOL(1)*|a_jj| (sprfc2.f:934-1037).  MSPAR(1) = 6 after the
+ * factor and 8 after a solve, as the reference sets them.  C is L sqrt(D) of
+ * the GPU's LDL' factorization: equal to the reference's to rounding.  A C
+ * in SM computed elsewhere is taken by SPRFS2/SPRBS2 as it is. */
+void sprgpu_sprfc2(sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa, double *sm,
+                   double *tol, sprgpu_int *iwork, double *rwork, const sprgpu_int *lpu,
+                   sprgpu_int *ierr, void (*mvmult)(void), void (*mmmult)(void));
+void sprgpu_sprfs2(sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa, double *sm,
+                   double *b, const sprgpu_int *ldb, const sprgpu_int *nrhs, sprgpu_int *iwork,
+                   double *rwork, const sprgpu_int *lpu, sprgpu_int *ierr);
+void sprgpu_sprbs2(sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa, double *sm,
+                   double *b, const sprgpu_int *ldb, const sprgpu_int *nrhs, double *rwork,
+                   const sprgpu_int *lpu, sprgpu_int *ierr);
+
 /* Replaces SPRSOL (src/SPR/sprsol.f:5-6): IOP<=3 factor, 2<=IOP<=5 forward,
  * IOP>=3 and IOP!=5 backward. */
 void sprgpu_sprsol(const sprgpu_int *iop, sprgpu_int *mspar, sprgpu_int *mtrees,

@@ -45,6 +45,33 @@ __global__ __launch_bounds__(256) void k_cc_factor_layout(const i32 *__restrict_
     for (i64 t = 1; t < nd.F - k; ++t) col[t] *= c;
 }
 
+// C in SM layout (a CC' factor computed elsewhere) to the LDL' layout:
+// d_j = c_jj^2, l_ij = c_ij / c_jj
+__global__ __launch_bounds__(256) void k_cc_to_ldl_layout(const i32 *__restrict__ col_node,
+                                                          const NodeInfo *__restrict__ nodes,
+                                                          double *sm, i64 n)
+{
+    const i64 j = i64(blockIdx.x) * 256 + threadIdx.x;
+    if (j >= n) return;
+    const NodeInfo nd = nodes[col_node[j]];
+    const i64 k = j - nd.first;
+    double *col = sm + n + nd.xlnz + k * nd.F - k * (k - 1) / 2;
+    const double c = col[0];
+    sm[j] = c * c;
+    col[0] = c * c;
+    for (i64 t = 1; t < nd.F - k; ++t) col[t] /= c;
+}
+
+int cc_to_ldl_layout(sprgpu_handle *h, double *sm_dev)
+{
+    if (int rc = upload_col_node(h)) return rc;
+    const i64 n = h->plan.neq;
+    k_cc_to_ldl_layout<<<unsigned((n + 255) / 256), 256, 0, h->stream>>>(h->d.col_node,
+                                                                          h->d.nodes, sm_dev, n);
+    SPRGPU_HIP(hipGetLastError());
+    return 0;
+}
+
 int cc_factor_info(sprgpu_handle *h, sprgpu_factor_info *info)
 {
     // the LDL' statistics of the TOL(1) = 0 factorization

@@ -121,15 +121,16 @@ void factor_dropin(int mode, sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *
                    double *sm, double *tol, double *dcor, const sprgpu_int *lpu,
                    sprgpu_int *ierr, const char *name)
 {
-    // stage protocol (sprfc1.f:140-148)
+    // stage protocol (sprfc1.f:140-148; SPRFC2 uses 6, sprfc2.f)
+    const sprgpu_int stage = mode == 2 ? 6 : 5;
     if (mspar[0] < 0) {
-        mspar[0] = -5;
+        mspar[0] = -stage;
         *ierr = -1;
         set_error("error from a previous SPR stage is not cleared");
         report(lpu, name, *ierr);
         return;
     }
-    mspar[0] = 5;
+    mspar[0] = stage;
     *ierr = 0;
     if (!lengths_ok_factor(mspar)) {
         mspar[0] = -2;
@@ -156,6 +157,7 @@ void factor_dropin(int mode, sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *
     if (he != hipSuccess) rc = hip_fail(he, "upload SM");
     if (!rc) rc = sprgpu_factor(h, e->d_sm, t4, mode, &info);
     if (!rc) rc = copy_factor_to_sm(h, e->d_sm);
+    if (!rc && mode == 2 && info.ierr == 0) rc = cc_factor_layout(h, e->d_sm);
     if (!rc) {
         he = hipMemcpy(sm, e->d_sm, sizeof(double) * n, hipMemcpyDeviceToHost);
         if (he != hipSuccess) rc = hip_fail(he, "download SM");
@@ -182,19 +184,21 @@ void factor_dropin(int mode, sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *
     report(lpu, name, *ierr);
 }
 
-// Solve driver shared by SPRFS1 and SPRBS1.  which = 1 forward, 2 backward.
-void solve_dropin(int which, sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa,
+// Solve driver shared by SPRFS1/SPRBS1 and, with cc, SPRFS2/SPRBS2 (the CC'
+// factor C in SM).  which = 1 forward, 2 backward.
+void solve_dropin(int which, bool cc, sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa,
                   double *sm, double *b, const sprgpu_int *ldb, const sprgpu_int *nrhs,
                   const sprgpu_int *lpu, sprgpu_int *ierr, const char *name)
 {
+    const sprgpu_int stage = cc ? 8 : 7;  // sprfs1.f, sprfs2.f:140-145
     if (mspar[0] < 0) {
-        mspar[0] = -7;
+        mspar[0] = -stage;
         *ierr = -1;
         set_error("error from a previous SPR stage is not cleared");
         report(lpu, name, *ierr);
         return;
     }
-    mspar[0] = 7;
+    mspar[0] = stage;
     *ierr = 0;
     const i64 neq = mspar[7];
     if (mspar[56] == 1 && *ldb < neq) {
@@ -230,8 +234,10 @@ void solve_dropin(int which, sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *
         // take it from SM
         const i64 n = neq + h->plan.nzerol;
         he = hipMemcpy(e->d_sm, sm, sizeof(double) * n, hipMemcpyHostToDevice);
-        if (he == hipSuccess) rc = load_factor_from_sm(h, e->d_sm);
-        else rc = hip_fail(he, "upload SM");
+        if (he != hipSuccess) rc = hip_fail(he, "upload SM");
+        if (!rc && cc) rc = cc_to_ldl_layout(h, e->d_sm);  // C = L sqrt(D) to L, D
+        if (!rc) rc = load_factor_from_sm(h, e->d_sm);
+        if (!rc) h->mode = cc ? 2 : 1;
     }
     if (!rc && e->b_cap < neq * k) {
         if (e->d_b) (void)hipFree(e->d_b);
@@ -255,7 +261,7 @@ void solve_dropin(int which, sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *
         }
         if (he != hipSuccess) rc = hip_fail(he, "upload B");
     }
-    if (!rc) rc = run_solve(h, e->d_b, neq, k, which);
+    if (!rc) rc = sprgpu_solve(h, e->d_b, neq, k, which);
     if (!rc) rc = check_solve_error(h);
     if (!rc) {
         if (perm) {
@@ -308,7 +314,7 @@ void sprgpu_sprfs1(sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa, dou
 {
     (void)iwork;
     (void)rwork;
-    solve_dropin(1, mspar, mtrees, msifa, sm, b, ldb, nrhs, lpu, ierr, "SPRFS1");
+    solve_dropin(1, false, mspar, mtrees, msifa, sm, b, ldb, nrhs, lpu, ierr, "SPRFS1");
 }
 
 void sprgpu_sprbs1(sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa, double *sm,
@@ -316,7 +322,35 @@ void sprgpu_sprbs1(sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa, dou
                    const sprgpu_int *lpu, sprgpu_int *ierr)
 {
     (void)rwork;
-    solve_dropin(2, mspar, mtrees, msifa, sm, b, ldb, nrhs, lpu, ierr, "SPRBS1");
+    solve_dropin(2, false, mspar, mtrees, msifa, sm, b, ldb, nrhs, lpu, ierr, "SPRBS1");
+}
+
+void sprgpu_sprfc2(sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa, double *sm,
+                   double *tol, sprgpu_int *iwork, double *rwork, const sprgpu_int *lpu,
+                   sprgpu_int *ierr, void (*mvmult)(void), void (*mmmult)(void))
+{
+    (void)iwork;
+    (void)rwork;
+    (void)mvmult;
+    (void)mmmult;
+    factor_dropin(2, mspar, mtrees, msifa, sm, tol, nullptr, lpu, ierr, "SPRFC2");
+}
+
+void sprgpu_sprfs2(sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa, double *sm,
+                   double *b, const sprgpu_int *ldb, const sprgpu_int *nrhs, sprgpu_int *iwork,
+                   double *rwork, const sprgpu_int *lpu, sprgpu_int *ierr)
+{
+    (void)iwork;
+    (void)rwork;
+    solve_dropin(1, true, mspar, mtrees, msifa, sm, b, ldb, nrhs, lpu, ierr, "SPRFS2");
+}
+
+void sprgpu_sprbs2(sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa, double *sm,
+                   double *b, const sprgpu_int *ldb, const sprgpu_int *nrhs, double *rwork,
+                   const sprgpu_int *lpu, sprgpu_int *ierr)
+{
+    (void)rwork;
+    solve_dropin(2, true, mspar, mtrees, msifa, sm, b, ldb, nrhs, lpu, ierr, "SPRBS2");
 }
 
 void sprgpu_sprsol(const sprgpu_int *iop, sprgpu_int *mspar, sprgpu_int *mtrees,

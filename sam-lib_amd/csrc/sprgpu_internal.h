@@ -316,6 +316,7 @@ int upload_col_node(sprgpu_handle *h);  // supernode of every equation, once
 int cc_factor_info(sprgpu_handle *h, sprgpu_factor_info *info);
 int cc_scale(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int inverse);
 int cc_factor_layout(sprgpu_handle *h, double *sm_dev);
+int cc_to_ldl_layout(sprgpu_handle *h, double *sm_dev);
 // products.hip
 int run_sprtmm(sprgpu_handle *h, const double *u, const double *b, double *c, double *wa,
                i64 m, i64 n, i64 ksu, i64 iflag);

@@ -36,6 +36,9 @@ SIGNATURES = {
     "sprgpu_sprfc3": (None, [P] * 10 + [P, P]),
     "sprgpu_sprfs1": (None, [P] * 11),
     "sprgpu_sprbs1": (None, [P] * 10),
+    "sprgpu_sprfc2": (None, [P] * 9 + [P, P]),
+    "sprgpu_sprfs2": (None, [P] * 11),
+    "sprgpu_sprbs2": (None, [P] * 10),
     "sprgpu_sprsol": (None, [P] * 13),
     "sprgpu_release": (None, [P]),
     "sprgpu_create": (C.c_int, [P, P, P, C.c_int, C.POINTER(P)]),

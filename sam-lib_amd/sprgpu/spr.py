@@ -92,6 +92,35 @@ def sprbs1(mspar, mtrees, msifa, sm, b, lpu: int = -1) -> int:
     return int(ierr.value)
 
 
+def sprfc2(mspar, mtrees, msifa, sm, tol, lpu: int = -1) -> int:
+    _check_int(mspar, mtrees, msifa)
+    _check_real(sm, tol)
+    ierr = C.c_int64(0)
+    lib().sprgpu_sprfc2(_p(mspar), _p(mtrees), _p(msifa), _p(sm), _p(tol), None, None,
+                        _ref(lpu), C.byref(ierr), None, None)
+    return int(ierr.value)
+
+
+def sprfs2(mspar, mtrees, msifa, sm, b, lpu: int = -1) -> int:
+    _check_int(mspar, mtrees, msifa)
+    _check_real(sm, b)
+    b, ldb, nrhs = _b_view(b)
+    ierr = C.c_int64(0)
+    lib().sprgpu_sprfs2(_p(mspar), _p(mtrees), _p(msifa), _p(sm), _p(b), _ref(ldb), _ref(nrhs),
+                        None, None, _ref(lpu), C.byref(ierr))
+    return int(ierr.value)
+
+
+def sprbs2(mspar, mtrees, msifa, sm, b, lpu: int = -1) -> int:
+    _check_int(mspar, mtrees, msifa)
+    _check_real(sm, b)
+    b, ldb, nrhs = _b_view(b)
+    ierr = C.c_int64(0)
+    lib().sprgpu_sprbs2(_p(mspar), _p(mtrees), _p(msifa), _p(sm), _p(b), _ref(ldb), _ref(nrhs),
+                        None, _ref(lpu), C.byref(ierr))
+    return int(ierr.value)
+
+
 def sprsol(iop, mspar, mtrees, msifa, sm, b, tol, lpu: int = -1) -> int:
     _check_int(mspar, mtrees, msifa)
     _check_real(sm, tol)

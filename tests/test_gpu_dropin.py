@@ -15,6 +15,15 @@ from conftest import make_system
 
 pytestmark = pytest.mark.gpu
 
+
+@pytest.fixture(autouse=True, scope="module")
+def _torch_runtime_first():
+    """torch brings up its HIP runtime before the library's first call, so the
+    tests after these ones (which use torch) find the GPU whatever the order."""
+    import torch
+
+    assert torch.cuda.is_available()
+
 # relative tolerance of FP64 solutions against the oracle (factor computed in
 # a different elimination order; the bound is kappa * eps with margin)
 SOL_RTOL = 1e-9
@@ -79,3 +88,41 @@ def test_factor_then_separate_solves():
     assert spr.sprbs1(ms, sysm.mtrees, sysm.msifa, sm, x) == 0
     spr.release(sm)
     assert np.linalg.norm(x - x_o) / np.linalg.norm(x_o) < SOL_RTOL
+
+
+def test_cct_dropin_matches_reference():
+    """SPRFC2, SPRFS2 and SPRBS2 through the drop-in against the reference's
+    own routines (oracle/_ref): stage codes, TOL, C in SM, both solves; and
+    the reference's C (factored elsewhere) solved by the GPU's SPRFS2/SPRBS2."""
+    import refspr
+    from sprgpu import spr
+
+    model, sysm, sm = make_system("hex8", 6, e_sigma=0.5, e_seed=1)
+    n = sysm.neq
+    b = np.asfortranarray(np.random.default_rng(2).uniform(-1, 1, (n, 2)))
+    cc_r, ms_r = sm.copy(), sysm.mspar.copy()
+    ierr_r, tol_r = refspr.fc2(sysm, cc_r, mspar=ms_r)
+    y_r = b.copy(order="F")
+    assert ierr_r == 0 and refspr.fs2_bs2("fs", sysm, cc_r, y_r, mspar=ms_r) == 0
+    x_r = y_r.copy(order="F")
+    assert refspr.fs2_bs2("bs", sysm, cc_r, x_r, mspar=ms_r) == 0
+    cc, ms = sm.copy(), sysm.mspar.copy()
+    tol = np.zeros(3)
+    assert spr.sprfc2(ms, sysm.mtrees, sysm.msifa, cc, tol) == 0
+    assert ms[0] == 6
+    np.testing.assert_allclose(tol[1:], tol_r[1:], rtol=1e-9)
+    assert np.max(np.abs(cc[:n] / cc_r[:n] - 1)) < 1e-9
+    assert np.linalg.norm(cc[n:] - cc_r[n:]) / np.linalg.norm(cc_r[n:]) < 1e-9
+    y = b.copy(order="F")
+    assert spr.sprfs2(ms, sysm.mtrees, sysm.msifa, cc, y) == 0 and ms[0] == 8
+    assert np.linalg.norm(y - y_r) / np.linalg.norm(y_r) < 1e-9
+    assert spr.sprbs2(ms, sysm.mtrees, sysm.msifa, cc, y) == 0
+    assert np.linalg.norm(y - x_r) / np.linalg.norm(x_r) < 1e-9
+    spr.release(cc)
+    # the reference's factor, new to the drop-in: taken from SM as C
+    ms2 = ms_r.copy()
+    y2 = b.copy(order="F")
+    assert spr.sprfs2(ms2, sysm.mtrees, sysm.msifa, cc_r, y2) == 0
+    assert spr.sprbs2(ms2, sysm.mtrees, sysm.msifa, cc_r, y2) == 0
+    spr.release(cc_r)
+    assert np.linalg.norm(y2 - x_r) / np.linalg.norm(x_r) < 1e-9
