@@ -210,6 +210,24 @@ int sprgpu_sprtmm(sprgpu_handle *handle, const double *u_dev, const double *b_de
                   double *c_dev, double *wa_dev, sprgpu_int m, sprgpu_int n, sprgpu_int ksu,
                   sprgpu_int iflag);
 
+/* The task of SPRLAN (src/SPR/sprlan.f:5-945) on the device: the NEVAL
+ * eigenvalues of (A - lambda B) q = 0 closest to SHIFT and, for the first
+ * NEVEC of them, B-orthonormal eigenvectors.  a_sm_dev: A as SM
+ * (NEQ+NZEROL, assembled); b_dev: B as LNZ (NZEROL, ksb > 0; its non-zeros
+ * on the handle's pattern) or a diagonal (NEQ, ksb <= 0).  The handle (FP64)
+ * factors A - SHIFT B and keeps that factor afterwards.  Lanczos on
+ * (A - SHIFT B)^-1 B with full reorthogonalisation in the B inner product,
+ * at most MAXLAN steps (NEVAL < MAXLAN <= NEQ), stopping when the NEVAL
+ * estimates EVERR(i) = |beta_m s_mi| / theta_i^2 are all at most
+ * TOL |EVL(i)|.  evl, everr: host arrays of NEVAL, nearest to SHIFT first;
+ * v_dev: NEQ x NEVEC column-major, device; nconv: how many converged.
+ * SPRLAN's option arrays (MIP, MOP), restarts and other reorthogonalisation
+ * schemes are not offered.  Returns 0, -2 (arguments, FP32 handle), -1 (no
+ * pattern for a sparse B) or -3 (A - SHIFT B singular). */
+int sprgpu_lanczos(sprgpu_handle *handle, const double *a_sm_dev, const double *b_dev, int ksb,
+                   double shift, sprgpu_int neval, sprgpu_int nevec, sprgpu_int maxlan,
+                   double tol, double *evl, double *everr, double *v_dev, sprgpu_int *nconv);
+
 /* Solves A X = B with the current factor plus FP64 iterative refinement
  * until max_col ||B - A X|| / ||B|| <= rtol or maxit steps. */
 int sprgpu_solve_ir(sprgpu_handle *handle, const double *b_dev, double *x_dev, sprgpu_int ld,

@@ -450,6 +450,16 @@ int sprgpu_sprtmm(sprgpu_handle *h, const double *u_dev, const double *b_dev, do
     return run_sprtmm(h, u_dev, b_dev, c_dev, wa_dev, m, n, ksu, iflag);
 }
 
+int sprgpu_lanczos(sprgpu_handle *h, const double *a_sm_dev, const double *b_dev, int ksb,
+                   double shift, sprgpu_int neval, sprgpu_int nevec, sprgpu_int maxlan, double tol,
+                   double *evl, double *everr, double *v_dev, sprgpu_int *nconv)
+{
+    if (!h || !a_sm_dev || !b_dev || !evl || !everr || !nconv || (nevec > 0 && !v_dev))
+        return -2;
+    return run_lanczos(h, a_sm_dev, b_dev, ksb, shift, neval, nevec, maxlan, tol, evl, everr,
+                       v_dev, nconv);
+}
+
 int sprgpu_solve_ir(sprgpu_handle *h, const double *b, double *x, sprgpu_int ld,
                     sprgpu_int nrhs, double rtol, int maxit, sprgpu_ir_info *info)
 {

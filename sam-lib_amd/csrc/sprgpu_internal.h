@@ -317,6 +317,10 @@ int cc_factor_info(sprgpu_handle *h, sprgpu_factor_info *info);
 int cc_scale(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int inverse);
 int cc_factor_layout(sprgpu_handle *h, double *sm_dev);
 int cc_to_ldl_layout(sprgpu_handle *h, double *sm_dev);
+// lanczos.hip
+int run_lanczos(sprgpu_handle *h, const double *a_sm, const double *b, int ksb, double shift,
+                i64 neval, i64 nevec, i64 maxlan, double tol, double *evl, double *everr,
+                double *v, i64 *nconv);
 // products.hip
 int run_sprtmm(sprgpu_handle *h, const double *u, const double *b, double *c, double *wa,
                i64 m, i64 n, i64 ksu, i64 iflag);

@@ -53,6 +53,8 @@ SIGNATURES = {
     "sprgpu_spmv": (C.c_int, [P, P, P, I64, I64]),
     "sprgpu_sprprm": (C.c_int, [P, P, P, P, P, I64, I64, I64, I64]),
     "sprgpu_sprtmm": (C.c_int, [P, P, P, P, P, I64, I64, I64, I64]),
+    "sprgpu_lanczos": (C.c_int, [P, P, P, C.c_int, C.c_double, I64, I64, I64, C.c_double, P, P,
+                                  P, P]),
     "sprgpu_solve_ir": (C.c_int, [P, P, P, I64, I64, C.c_double, C.c_int, C.POINTER(IrInfo)]),
     "sprgpu_get_factor": (C.c_int, [P, P]),
     "sprgpu_neq": (I64, [P]),

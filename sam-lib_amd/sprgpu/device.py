@@ -165,6 +165,20 @@ class SprFactor:
               "sprgpu_sprtmm")
         return c_dev if iflag in (1, 2) else wa_dev
 
+    def lanczos(self, a_sm_dev, b_dev, shift: float, neval: int, nevec: int = 0,
+                maxlan: int | None = None, tol: float = 1e-10, ksb: int = 1):
+        """Eigenvalues of (A - lambda B) q = 0 nearest to shift (sprgpu_lanczos).
+        Returns (evl, everr, v (neq x nevec tensor or None), nconv)."""
+        torch = self._torch
+        maxlan = maxlan or min(self.neq, max(6 * neval, neval + 40))
+        evl, everr = np.zeros(neval), np.zeros(neval)
+        v = torch.empty(max(nevec, 1), self.neq, dtype=torch.float64, device=a_sm_dev.device)
+        nconv = C.c_int64(0)
+        check(lib().sprgpu_lanczos(self._h, _ptr(a_sm_dev), _ptr(b_dev), int(ksb), float(shift),
+                                   int(neval), int(nevec), int(maxlan), float(tol), _np(evl),
+                                   _np(everr), _ptr(v), C.byref(nconv)), "sprgpu_lanczos")
+        return evl, everr, (v[:nevec] if nevec else None), int(nconv.value)
+
     def solve_ir(self, b_dev, x_dev, rtol: float = 1e-10, maxit: int = 20) -> dict:
         b2 = b_dev if b_dev.dim() == 2 else b_dev.view(1, -1)
         x2 = x_dev if x_dev.dim() == 2 else x_dev.view(1, -1)
