@@ -106,6 +106,33 @@ def bench_product(sysm, h, sm_dev, x, b_dev, reps):
             "sprtmm_vs_sprprm_rel": rel_tmm, "lnz_bytes": 8 * sysm.nzerol}
 
 
+def bench_lanczos(model, sysm, h, sm_dev, neval=10):
+    """The NEVAL lowest modes of K q = lambda M q with a lumped mass (1/8 of
+    each adjacent unit-density element per free dof), shift 0, by
+    sprgpu_lanczos on the FP64 handle: the time (factor of K included), the
+    converged count, and |K v - lambda M v| / |K v| of the first mode from
+    the device SPRPRM products."""
+    import torch
+    from sprgpu import host
+
+    mp, me = host.element_meen(model, sysm)
+    mass = np.zeros(sysm.neq)
+    np.add.at(mass, me[me > 0] - 1, 1.0 / 8.0)
+    m_dev = torch.from_numpy(mass).cuda()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    evl, everr, v, nconv = h.lanczos(sm_dev, m_dev, 0.0, neval, nevec=1, ksb=0)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t
+    kv = h.sprprm(sm_dev[sysm.neq:], v)
+    mv = h.sprprm(m_dev, v, ksa=0)
+    res = float(torch.linalg.norm(kv - evl[0] * mv) / torch.linalg.norm(kv))
+    log(f"Lanczos: {neval} lowest modes in {secs:.3f} s, {nconv} converged, "
+        f"lambda_1 {evl[0]:.6e}, residual {res:.2e}")
+    return {"seconds": secs, "neval": neval, "nconv": nconv, "eigenvalues": evl.tolist(),
+            "first_mode_residual": res}
+
+
 def bench_assembly(model, sysm, sm, h, reps, t_host):
     """Element assembly of the same SM on the device (sprgpu_set_assembly once,
     then sprgpu_assemble): bit equality with the host SPRADM loop's SM, and the
@@ -324,6 +351,7 @@ def main():
     # FP64 handle's residual pattern by the map's, which that handle no longer uses)
     result["assembly"] = bench_assembly(model, sysm, sm, h64, max(args.steps, 5), t_host_asm)
     result["sprprm"] = bench_product(sysm, h64, sm_dev, x64, b_dev, max(args.steps, 5))
+    result["lanczos"] = bench_lanczos(model, sysm, h64, sm_dev)
 
     if not args.no_profile:
         import ctypes as C
