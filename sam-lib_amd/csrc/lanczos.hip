@@ -193,6 +193,9 @@ int run_lanczos(sprgpu_handle *h, const double *a_sm, const double *b, int ksb, 
     double *sm_s = nullptr, *Q = nullptr, *BQ = nullptr, *w = nullptr, *bw = nullptr,
            *part = nullptr, *coef = nullptr;
     auto release = [&]() {
+        (void)hipStreamSynchronize(s);
+        // the handle keeps the factor of A - SHIFT B, not its freed input
+        if (h->last_sm_dev == sm_s) h->last_sm_dev = nullptr;
         for (double *q : {sm_s, Q, BQ, w, bw, part, coef})
             if (q) (void)hipFree(q);
     };
@@ -244,10 +247,11 @@ int run_lanczos(sprgpu_handle *h, const double *a_sm, const double *b, int ksb, 
     auto dots = [&](const double *X, int ncol, const double *y) -> int {
         k_lan_dots<<<dim3(kLanDotBlocks, unsigned(ncol)), 256, 0, s>>>(X, n, y, n, part);
         k_lan_dots_final<<<unsigned((ncol + 63) / 64), 64, 0, s>>>(part, coef, ncol);
-        SPRGPU_HIP(hipMemcpyAsync(hdots.data(), coef, sizeof(double) * ncol,
-                                  hipMemcpyDeviceToHost, s));
-        SPRGPU_HIP(hipStreamSynchronize(s));
-        return 0;
+        // errors are returned; the caller releases the work space
+        hipError_t e = hipMemcpyAsync(hdots.data(), coef, sizeof(double) * ncol,
+                                      hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        return e == hipSuccess ? 0 : hip_fail(e, "lanczos: dot products");
     };
     // start: q0 = r / |r|_B
     k_lan_start<<<gn, 256, 0, s>>>(Q, n);
@@ -337,8 +341,6 @@ int run_lanczos(sprgpu_handle *h, const double *a_sm, const double *b, int ksb, 
         LAN_HIP(hipStreamSynchronize(s));
     }
     LAN_HIP(hipGetLastError());
-    // the handle keeps the factor of A - SHIFT B; its input is freed here
-    h->last_sm_dev = nullptr;
     release();
 #undef LAN_HIP
     return 0;
