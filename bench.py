@@ -15,6 +15,12 @@ reference's own routines compiled into lib/libspr_host.so), the upload of SM
 and b to HBM, and the residual pattern.  The CPU baseline (the only use of
 oracle/) is the reference's SPRFC1 + SPRFS1 + SPRBS1 on the 20^3 case, on
 one host core.
+
+After the timed steps, on the same 40^3 matrix and outside `value`: the
+device element assembly (bit equality with the host SPRADM loop's SM and its
+time), the SPR-layout products (SPRPRM: A x of the FP64 solution against b;
+SPRTMM: L^T x + L x - D x against A x) and the shift-invert Lanczos leg (the
+10 lowest modes with a lumped mass), each with its own JSON object.
 """
 import argparse
 import json
