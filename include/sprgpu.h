@@ -170,6 +170,14 @@ int sprgpu_factor(sprgpu_handle *handle, const double *sm_dev, const double *tol
 /* Reads the factor statistics of the last factorization (synchronizes). */
 int sprgpu_factor_info_get(sprgpu_handle *handle, sprgpu_factor_info *info);
 
+/* DCOR(2,NEQ) of the last mode-3 (SPRFC3) factorization, the array SPRFC3
+ * returns (sprfc3.f:5-7, filled at sprfc3.f:560-575 and :1083-1099):
+ * DCOR(1,j) the pivot before its replacement (or a negative pivot), DCOR(2,j)
+ * the replacement (1 for a zero diagonal), zero elsewhere.  dcor_out (2*NEQ
+ * doubles) may be host or device memory.  Returns -1 if the last
+ * factorization was not in mode 3. */
+int sprgpu_get_dcor(sprgpu_handle *handle, double *dcor_out);
+
 /* Forward (D^-1 L^-1) and/or backward (L^-T) substitution in place on device
  * B(LDB,NRHS) in FP64 (the factor may be FP32).  which: 1 forward, 2
  * backward, 3 both. */

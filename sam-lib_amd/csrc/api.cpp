@@ -229,7 +229,8 @@ int sprgpu_factor_solve_graph(sprgpu_handle *h, const double *sm_dev, double *b_
 {
     if (!h || !sm_dev || !b_dev || nrhs < 1 || ldb < h->plan.neq) return -2;
     const bool same = h->graph_exec && h->graph_sm == sm_dev && h->graph_b == b_dev &&
-                      h->graph_ldb == ldb && h->graph_nrhs == nrhs;
+                      h->graph_ldb == ldb && h->graph_nrhs == nrhs &&
+                      h->graph_gen == h->buf_gen;
     if (!same) {
         drop_graph(h);
         const double tol[4] = {0.0, 0.0, 0.0, 0.0};
@@ -262,6 +263,7 @@ int sprgpu_factor_solve_graph(sprgpu_handle *h, const double *sm_dev, double *b_
         h->graph_b = b_dev;
         h->graph_ldb = ldb;
         h->graph_nrhs = nrhs;
+        h->graph_gen = h->buf_gen;
     }
     SPRGPU_HIP(hipGraphLaunch(h->graph_exec, h->stream));
     h->factored = true;
@@ -413,6 +415,19 @@ int sprgpu_factor_info_get(sprgpu_handle *h, sprgpu_factor_info *info)
         return -1;
     }
     return read_factor_info(h, info);
+}
+
+int sprgpu_get_dcor(sprgpu_handle *h, double *dcor_out)
+{
+    if (!h || !dcor_out) return -2;
+    if (!h->factored || h->mode != 3) {
+        set_error("DCOR exists after a mode-3 (SPRFC3) factorization only");
+        return -1;
+    }
+    SPRGPU_HIP(hipMemcpyAsync(dcor_out, h->d.dcor, sizeof(double) * 2 * size_t(h->plan.neq),
+                              hipMemcpyDefault, h->stream));
+    SPRGPU_HIP(hipStreamSynchronize(h->stream));
+    return 0;
 }
 
 int sprgpu_solve(sprgpu_handle *h, double *b, sprgpu_int ldb, sprgpu_int nrhs, int which)

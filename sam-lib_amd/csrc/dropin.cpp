@@ -10,7 +10,13 @@
 //   sprgpu_sprsol  <- SPRSOL  sprsol.f:5-52
 // The factor stays on the GPU between calls, keyed by the SM address; SM on
 // the host always receives the factor too, so reference routines that read
-// it afterwards (SPREXD, SPRDAD, ...) still work.
+// it afterwards (SPREXD, SPRDAD, ...) still work.  Each cached factor keeps a
+// signature of the host SM it was written to (a hash of SM(1:NEQ), the
+// pivots, and of a strided sample of LNZ); a solve whose SM no longer
+// matches it (a factor computed elsewhere, a new SM at a reused address)
+// takes the factor from SM again.  At most kMaxEntries SM arrays are cached;
+// the least recently used one is released first.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -28,10 +34,35 @@ struct Entry {
     double *d_b = nullptr;    // device right-hand sides
     i64 b_cap = 0;
     std::vector<i64> key;     // fingerprint of the symbolic structure
+    unsigned long long sig = 0;  // signature of the host SM holding the device factor
+    bool sig_valid = false;
+    unsigned long long last_use = 0;
 };
+
+constexpr size_t kMaxEntries = 4;
+constexpr i64 kSigSamples = 4096;
 
 std::mutex g_mu;
 std::map<const double *, Entry> g_cache;
+unsigned long long g_clock = 0;
+
+// Signature of a factor in SM: FNV-1a over the bits of SM(1:NEQ) and of
+// kSigSamples entries of LNZ spread evenly over it.
+unsigned long long factor_signature(const double *sm, i64 neq, i64 nzerol)
+{
+    unsigned long long h = 1469598103934665603ull;
+    auto mix = [&](double v) {
+        unsigned long long u;
+        std::memcpy(&u, &v, sizeof(u));
+        h ^= u;
+        h *= 1099511628211ull;
+    };
+    for (i64 i = 0; i < neq; ++i) mix(sm[i]);
+    const i64 step = std::max<i64>(1, nzerol / kSigSamples);
+    for (i64 k = 0; k < nzerol; k += step) mix(sm[neq + k]);
+    if (nzerol > 0) mix(sm[neq + nzerol - 1]);
+    return h;
+}
 
 std::vector<i64> fingerprint(const i64 *mspar, const i64 *mtrees, const i64 *msifa)
 {
@@ -69,9 +100,20 @@ int get_entry(const double *sm, const i64 *mspar, const i64 *mtrees, const i64 *
     *fresh = false;
     if (it != g_cache.end() && it->second.key == key) {
         out = &it->second;
+        out->last_use = ++g_clock;
         return 0;
     }
-    if (it != g_cache.end()) release_entry(it->second);
+    if (it != g_cache.end()) {
+        release_entry(it->second);
+        g_cache.erase(it);
+    }
+    while (g_cache.size() >= kMaxEntries) {  // evict the least recently used
+        auto lru = g_cache.begin();
+        for (auto q = g_cache.begin(); q != g_cache.end(); ++q)
+            if (q->second.last_use < lru->second.last_use) lru = q;
+        release_entry(lru->second);
+        g_cache.erase(lru);
+    }
     Entry e;
     e.key = key;
     int rc = sprgpu_create(mspar, mtrees, msifa, SPRGPU_FP64, &e.h);
@@ -82,6 +124,7 @@ int get_entry(const double *sm, const i64 *mspar, const i64 *mtrees, const i64 *
         set_error("device allocation of SM failed");
         return -5;
     }
+    e.last_use = ++g_clock;
     g_cache[sm] = e;
     out = &g_cache[sm];
     *fresh = true;
@@ -162,6 +205,10 @@ void factor_dropin(int mode, sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *
         he = hipMemcpy(sm, e->d_sm, sizeof(double) * n, hipMemcpyDeviceToHost);
         if (he != hipSuccess) rc = hip_fail(he, "download SM");
     }
+    if (!rc) {
+        e->sig = factor_signature(sm, neq, h->plan.nzerol);
+        e->sig_valid = true;
+    }
     if (!rc && mode == 3 && dcor) {
         he = hipMemcpy(dcor, h->d.dcor, sizeof(double) * 2 * neq, hipMemcpyDeviceToHost);
         if (he != hipSuccess) rc = hip_fail(he, "download DCOR");
@@ -229,15 +276,22 @@ void solve_dropin(int which, bool cc, sprgpu_int *mspar, sprgpu_int *mtrees, spr
     }
     sprgpu_handle *h = e->h;
     hipError_t he = hipSuccess;
-    if (fresh || !h->factored) {
-        // the factor was computed elsewhere (e.g. by the reference SPRFC1):
-        // take it from SM
+    const unsigned long long sig = factor_signature(sm, neq, h->plan.nzerol);
+    if (fresh || !h->factored || !e->sig_valid || e->sig != sig) {
+        // the factor was computed elsewhere (e.g. by the reference SPRFC1),
+        // or SM changed since the device factor was written to it: take the
+        // factor from SM
         const i64 n = neq + h->plan.nzerol;
+        e->sig_valid = false;
         he = hipMemcpy(e->d_sm, sm, sizeof(double) * n, hipMemcpyHostToDevice);
         if (he != hipSuccess) rc = hip_fail(he, "upload SM");
         if (!rc && cc) rc = cc_to_ldl_layout(h, e->d_sm);  // C = L sqrt(D) to L, D
         if (!rc) rc = load_factor_from_sm(h, e->d_sm);
-        if (!rc) h->mode = cc ? 2 : 1;
+        if (!rc) {
+            h->mode = cc ? 2 : 1;
+            e->sig = sig;
+            e->sig_valid = true;
+        }
     }
     if (!rc && e->b_cap < neq * k) {
         if (e->d_b) (void)hipFree(e->d_b);

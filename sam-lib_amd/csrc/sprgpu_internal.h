@@ -276,6 +276,11 @@ struct sprgpu_handle {
     const double *graph_sm = nullptr;
     double *graph_b = nullptr;
     sprgpu_int graph_ldb = 0, graph_nrhs = 0;
+    // Generation of the device buffers a captured graph may reference: bumped
+    // whenever one of them is freed and allocated again (the solve workspace
+    // growing for more right-hand sides); a graph captured under another
+    // generation holds stale pointers and is captured again.
+    unsigned long long buf_gen = 0, graph_gen = 0;
 };
 
 namespace sprgpu {

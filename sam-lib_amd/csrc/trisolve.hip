@@ -1028,6 +1028,7 @@ static int ensure_solve_buffers(sprgpu_handle *h, i64 nrhs)
     DeviceBuffers &d = h->d;
     const Plan &p = h->plan;
     if (d.vwork_cap < nrhs) {
+        ++h->buf_gen;
         if (d.vwork) SPRGPU_HIP(hipFree(d.vwork));
         d.vwork = nullptr;
         SPRGPU_HIP(hipMalloc(&d.vwork, sizeof(double) * size_t(p.vwork_elems) * size_t(nrhs)));
@@ -1036,6 +1037,7 @@ static int ensure_solve_buffers(sprgpu_handle *h, i64 nrhs)
     }
     const i64 nf = std::max<i64>(p.nflags, 1) * nrhs;
     if (d.flags_cap < nf) {
+        ++h->buf_gen;
         if (d.flags) SPRGPU_HIP(hipFree(d.flags));
         d.flags = nullptr;
         SPRGPU_HIP(hipMalloc(&d.flags, sizeof(unsigned) * nf));
@@ -1046,6 +1048,7 @@ static int ensure_solve_buffers(sprgpu_handle *h, i64 nrhs)
     }
     const i64 np = std::max<i64>(p.npart, 1) * nrhs;
     if (d.partials_cap < np) {
+        ++h->buf_gen;
         if (d.partials) SPRGPU_HIP(hipFree(d.partials));
         d.partials = nullptr;
         SPRGPU_HIP(hipMalloc(&d.partials, sizeof(double) * np));

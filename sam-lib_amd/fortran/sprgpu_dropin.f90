@@ -58,6 +58,10 @@ module sprgpu_c
       integer(c_int64_t) :: mspar(*), mtrees(*), msifa(*), ldb, nrhs, lpu, ierr
       real(c_double) :: sm(*), b(*), rwork(*)
     end subroutine
+    subroutine c_release(sm) bind(C, name='sprgpu_release')
+      import :: c_double
+      real(c_double) :: sm(*)
+    end subroutine
   end interface
 contains
   ! Message on unit LPU in the spirit of SPRER1 (silent if LPU < 0).
@@ -163,3 +167,13 @@ subroutine SPRSOL(IOP, MSPAR, MTREES, MSIFA, SM, B, LDB, NRHS, TOL, IWORK, RWORK
     call sprgpu_message('SPRBS1', LPU, IERR)
   end if
 end subroutine SPRSOL
+
+! Releases the device factor the drop-in layer keeps for SM (no counterpart in
+! the reference, which keeps nothing between calls): for a caller that is
+! done with SM, or that will reuse its memory for another model.
+subroutine SPRGPU_RELEASE(SM)
+  use sprgpu_c
+  implicit none
+  real(c_double) :: SM(*)
+  call c_release(SM)
+end subroutine SPRGPU_RELEASE

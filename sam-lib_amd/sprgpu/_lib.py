@@ -57,6 +57,7 @@ SIGNATURES = {
                                   P, P]),
     "sprgpu_solve_ir": (C.c_int, [P, P, P, I64, I64, C.c_double, C.c_int, C.POINTER(IrInfo)]),
     "sprgpu_get_factor": (C.c_int, [P, P]),
+    "sprgpu_get_dcor": (C.c_int, [P, P]),
     "sprgpu_neq": (I64, [P]),
     "sprgpu_nzerol": (I64, [P]),
     "sprgpu_device_bytes": (I64, [P]),

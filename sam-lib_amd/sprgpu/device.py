@@ -117,6 +117,12 @@ class SprFactor:
         check(lib().sprgpu_factor(self._h, _ptr(sm_dev), _np(tolv), mode, None), "sprgpu_factor")
         return None
 
+    def dcor(self) -> np.ndarray:
+        """DCOR(2,NEQ) of the last mode-3 factorization, as (neq, 2) (sprgpu_get_dcor)."""
+        out = np.zeros(2 * self.neq)
+        check(lib().sprgpu_get_dcor(self._h, _np(out)), "sprgpu_get_dcor")
+        return out.reshape(self.neq, 2)
+
     def factor_info(self) -> dict:
         fi = FactorInfo()
         check(lib().sprgpu_factor_info_get(self._h, C.byref(fi)), "sprgpu_factor_info_get")

@@ -181,6 +181,43 @@ def beam3d_stiffness(p0: np.ndarray, p1: np.ndarray, EA: float, EI: float, GJ: f
     return 0.5 * (K + K.T)
 
 
+def beam3d_stiffness_batch(p0: np.ndarray, p1: np.ndarray, EA: np.ndarray, EI: np.ndarray,
+                           GJ: np.ndarray) -> np.ndarray:
+    """beam3d_stiffness for many members at once: p0, p1 (n, 3), EA, EI, GJ
+    (n,); returns (n, 12, 12), member e equal to beam3d_stiffness(p0[e],
+    p1[e], EA[e], EI[e], GJ[e]) up to rounding."""
+    d = np.asarray(p1, float) - np.asarray(p0, float)
+    n = d.shape[0]
+    L = np.sqrt(np.einsum("ij,ij->i", d, d))
+    ex = d / L[:, None]
+    ref = np.eye(3)[np.argmin(np.abs(ex), axis=1)]
+    ey = np.cross(ref, ex)
+    ey /= np.sqrt(np.einsum("ij,ij->i", ey, ey))[:, None]
+    ez = np.cross(ex, ey)
+    R = np.stack([ex, ey, ez], axis=1)  # (n, 3, 3), rows ex, ey, ez
+    k = np.zeros((n, 12, 12))
+    a = EA / L
+    g = GJ / L
+    for (i, j, v) in ((0, 0, a), (6, 6, a), (0, 6, -a), (6, 0, -a),
+                      (3, 3, g), (9, 9, g), (3, 9, -g), (9, 3, -g)):
+        k[:, i, j] = v
+    c1, c2, c3, c4 = 12 * EI / L ** 3, 6 * EI / L ** 2, 4 * EI / L, 2 * EI / L
+    for (v1, r1, v2, r2, sg) in ((1, 5, 7, 11, 1.0), (2, 4, 8, 10, -1.0)):
+        idx = (v1, r1, v2, r2)
+        kb = np.stack([np.stack([c1, sg * c2, -c1, sg * c2], -1),
+                       np.stack([sg * c2, c3, -sg * c2, c4], -1),
+                       np.stack([-c1, -sg * c2, c1, -sg * c2], -1),
+                       np.stack([sg * c2, c4, -sg * c2, c3], -1)], 1)
+        for ii in range(4):
+            for jj in range(4):
+                k[:, idx[ii], idx[jj]] += kb[:, ii, jj]
+    T = np.zeros((n, 12, 12))
+    for b in range(4):
+        T[:, 3 * b:3 * b + 3, 3 * b:3 * b + 3] = R
+    K = np.matmul(np.matmul(T.transpose(0, 2, 1), k), T)
+    return 0.5 * (K + K.transpose(0, 2, 1))
+
+
 # --------------------------------------------------------------------------
 # Meshes
 # --------------------------------------------------------------------------
@@ -284,9 +321,9 @@ def beam_frame(nnodes: int, rng: np.random.Generator, extra_ratio: float = 0.5,
     nel = len(edges)
     EA = np.exp(rng.normal(0.0, 0.3, nel))
     EI = np.exp(rng.normal(0.0, 0.3, nel))
-    em = np.empty(nel * 144)
-    for e, (a, b) in enumerate(edges):
-        em[144 * e:144 * (e + 1)] = beam3d_stiffness(pos[a], pos[b], EA[e], EI[e], EI[e] / 1.3).ravel(order="F")
+    ea = np.array(edges, dtype=np.int64).reshape(-1, 2)
+    K = beam3d_stiffness_batch(pos[ea[:, 0]], pos[ea[:, 1]], EA, EI, EI / 1.3)
+    em = np.ascontiguousarray(K.transpose(0, 2, 1)).ravel()  # each matrix column-major
     madof = (6 * np.arange(nnodes + 1) + 1).astype(I8)
     msc = np.ones(6 * nnodes, dtype=I8)
     msc[:6] = 0

@@ -52,3 +52,18 @@ def gpu_available() -> bool:
         return torch.cuda.is_available()
     except Exception:
         return False
+
+
+def backward_error(sysm, sm, x, b):
+    """Normwise backward error ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf)
+    of a solution x (SPR equation order) of A x = b, A given by SM's LNZ part.
+    A x and the row sums of |A| come from the reference SPRPRM built from
+    source (oracle/_ref), the checker's own product."""
+    import refspr
+
+    lnz = sm[sysm.neq:]
+    _, ax, _ = refspr.prm(lnz, x, sysm.mspar, sysm.mtrees, sysm.msifa)
+    _, rows, _ = refspr.prm(np.abs(lnz), np.ones(sysm.neq), sysm.mspar, sysm.mtrees,
+                            sysm.msifa)
+    r = b - ax[:, 0]
+    return float(np.abs(r).max() / (rows.max() * np.abs(x).max() + np.abs(b).max()))

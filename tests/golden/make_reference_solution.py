@@ -22,7 +22,7 @@ from bench import build_problem  # noqa: E402
 
 
 def main(n):
-    model, sysm, sm, b = build_problem(n)
+    model, sysm, sm, b, _ = build_problem(n)
     x = np.asfortranarray(b[:, None].copy())
     t = time.perf_counter()
     ierr, secs, tol = refspr.sol(3, sysm, sm, x)

@@ -4,7 +4,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
-from conftest import gpu_available, make_system
+from conftest import backward_error, gpu_available, make_system
 
 pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not gpu_available(), reason="needs an MI355X")]
@@ -61,12 +61,21 @@ def test_graph_replays_match_eager_on_a_frame_forest():
     merged = host.merge_systems(systems)
     b = rng.uniform(-1, 1, merged.neq)
     x = _check_replays(merged.mspar, merged.mtrees, merged.msifa, merged.merge_sm(sms), b)
+    # per frame: a backward-error bar (the frames are ill-conditioned, so the
+    # forward difference to the oracle is bounded by kappa times it, not by a
+    # fixed number), and the oracle's own solution for comparison
     for k, (s, sm) in enumerate(zip(systems, sms)):
-        bi = np.asfortranarray(b[merged.neq_off[k]:merged.neq_off[k + 1]][:, None].copy())
+        bk = b[merged.neq_off[k]:merged.neq_off[k + 1]].copy()
+        bi = np.asfortranarray(bk[:, None].copy())
         assert oracle.sprsol(3, s.mspar.copy(), s.mtrees, s.msifa, sm.copy(), bi,
                              tol=[0, 0, 0])[0] == 0
         xk = merged.split(x)[k]
-        assert np.linalg.norm(xk - bi[:, 0]) / np.linalg.norm(bi[:, 0]) < 1e-9
+        be_g = backward_error(s, sm, xk, bk)
+        be_o = backward_error(s, sm, bi[:, 0], bk)
+        print(f"frame {k}: neq {s.neq} backward error {be_g:.2e} (oracle {be_o:.2e}), "
+              f"difference to the oracle {np.linalg.norm(xk - bi[:, 0]) / np.linalg.norm(bi[:, 0]):.2e}",
+              flush=True)
+        assert be_g <= 1e-14, (k, be_g, be_o)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
@@ -107,5 +116,39 @@ def test_autotune_keeps_results():
         h.solve(x1)
         rel = float(torch.linalg.norm(x1 - x0) / torch.linalg.norm(x0))
         assert rel < 1e-12, rel
+    finally:
+        h.close()
+
+
+def test_graph_recaptured_after_the_solve_buffers_grow():
+    """A graph captured with one right-hand side must not replay after an
+    eager solve with more right-hand sides reallocated the solve workspace
+    (its captured kernels would address freed memory): the next call
+    captures again, and its result equals the eager factor + solve."""
+    import torch
+    from sprgpu.device import SprFactor
+
+    _, sysm, sm = make_system("hex8", 12, e_sigma=0.5, e_seed=1)
+    dev = torch.device("cuda:0")
+    sm_dev = torch.from_numpy(sm).to(dev)
+    b = torch.from_numpy(np.random.default_rng(5).uniform(-1, 1, sysm.neq)).to(dev).view(1, -1)
+    h = SprFactor(sysm.mspar, sysm.mtrees, sysm.msifa, "fp64")
+    try:
+        x_graph = b.clone()
+        _graph_run(h, sm_dev, x_graph, sysm.neq)      # captured with 1 RHS
+        torch.cuda.synchronize()
+        B8 = torch.randn(8, sysm.neq, dtype=torch.float64, device=dev)
+        h.factor(sm_dev)
+        h.solve(B8)                                   # grows the solve workspace
+        torch.cuda.synchronize()
+        x_eager = b.clone()
+        h.factor(sm_dev)
+        h.solve(x_eager)
+        torch.cuda.synchronize()
+        for rep in range(2):
+            x_graph.copy_(b)
+            _graph_run(h, sm_dev, x_graph, sysm.neq)
+            torch.cuda.synchronize()
+            assert torch.equal(x_graph, x_eager), f"replay {rep}"
     finally:
         h.close()

@@ -34,7 +34,7 @@ def test_fortran_dropin_exports_reference_names():
     out = subprocess.run(["nm", "-D", "--defined-only", _lib.FLIB_PATH], capture_output=True,
                          text=True, check=True).stdout
     for n in ("sprfc1_", "sprfc2_", "sprfc3_", "sprfs1_", "sprfs2_", "sprbs1_", "sprbs2_",
-              "sprsol_"):
+              "sprsol_", "sprgpu_release_"):
         assert re.search(rf"\bT {n}$", out, flags=re.M), n
 
 
