@@ -87,6 +87,14 @@ void sprgpu_sprsol(const sprgpu_int *iop, sprgpu_int *mspar, sprgpu_int *mtrees,
 /* Releases the device factor that the drop-in layer keeps for SM. */
 void sprgpu_release(const double *sm);
 
+/* LIERR of the last drop-in factorization on the calling thread: the
+ * reference's internal error code behind IERR = -3 (sprfc1.f:250-262,
+ * sprfc2.f:244-255, sprfc3.f:250-262): -30 a pivot at or below TOL(1)|a_jj|
+ * (SPRFC1), -32 the same in SPRFC2, -31 pivots replaced (SPRFC3), -35 a zero
+ * diagonal found before the factorization; 0 otherwise.  The Fortran layer
+ * hands it to SPRER1 as the reference drivers do. */
+sprgpu_int sprgpu_dropin_lierr(void);
+
 /* ------------------------------------------------------------------------ */
 /* 2. Device-resident handle API                                            */
 /* ------------------------------------------------------------------------ */

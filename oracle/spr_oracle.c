@@ -503,18 +503,22 @@ static void sprbs6(fint supvar, fint lenvar, const double *lnz, double *locrhs)
 
 /* SPRFB1 gather (zero beyond NEQ) and SPRFB2 scatter by assignment
  * (sprfs1.f:636-774, :777-926) */
-static void sprfb1(fint neq, fint jslen, const fint *lindx, const double *glb, double *loc)
+/* SPRFB1 / SPRFB3 (sprfs1.f:777-926): gather, through PERI2E when given */
+static void sprfb1(fint neq, fint jslen, const fint *lindx, const fint *peri2e,
+                   const double *glb, double *loc)
 {
     for (fint k = 1; k <= jslen; ++k) {
-        const fint r = F1(lindx, k);
+        const fint r = peri2e ? F1(peri2e, F1(lindx, k)) : F1(lindx, k);
         F1(loc, k) = (r <= neq) ? F1(glb, r) : 0.0;
     }
 }
 
-static void sprfb2(fint neq, fint jslen, const fint *lindx, double *glb, const double *loc)
+/* SPRFB2 / SPRFB4 (sprfs1.f:636-774): scatter by assignment */
+static void sprfb2(fint neq, fint jslen, const fint *lindx, const fint *peri2e, double *glb,
+                   const double *loc)
 {
     for (fint k = 1; k <= jslen; ++k) {
-        const fint r = F1(lindx, k);
+        const fint r = peri2e ? F1(peri2e, F1(lindx, k)) : F1(lindx, k);
         if (r <= neq) F1(glb, r) = F1(loc, k);
     }
 }
@@ -554,6 +558,8 @@ void oracle_sprfs1(fint *mspar, const fint *mtrees, const fint *msifa, const dou
     double *loc = (double *)malloc((size_t)(v.mfront + 1) * sizeof(double));
     fint *first = (fint *)malloc((size_t)(v.maxsup + 1) * sizeof(fint));
     const fint lpar57 = F1(mspar, 57);
+    /* SPRFS4 (sprfs1.f:396-528) when MSPAR(58) >= 1 */
+    const fint *peri2e = (F1(mspar, 58) >= 1) ? msifa : NULL;
     for (fint irhs = 0; irhs < *nrhs; ++irhs) {
         double *glb = b + irhs * (*ldb);
         fint iplnz = 1;
@@ -564,9 +570,9 @@ void oracle_sprfs1(fint *mspar, const fint *mtrees, const fint *msifa, const dou
             const fint supsze =
                 (jslen * (jslen + 1) - (jslen - supvar) * (jslen - supvar + 1)) / 2;
             if (F1(v.supsup, js) <= 0 || lpar57 <= 0) {
-                sprfb1(neq, jslen, &F1(v.lindx, jsstrt), glb, loc);
+                sprfb1(neq, jslen, &F1(v.lindx, jsstrt), peri2e, glb, loc);
                 sprfs5(supvar, jslen, &F1(lnz, iplnz), loc, first);
-                sprfb2(neq, jslen, &F1(v.lindx, jsstrt), glb, loc);
+                sprfb2(neq, jslen, &F1(v.lindx, jsstrt), peri2e, glb, loc);
             }
             iplnz += supsze;
         }
@@ -588,6 +594,8 @@ void oracle_sprbs1(fint *mspar, const fint *mtrees, const fint *msifa, const dou
     const double *lnz = &F1(sm, v.neq + 1);
     double *loc = (double *)malloc((size_t)(v.mfront + 1) * sizeof(double));
     const fint lpar57 = F1(mspar, 57);
+    /* SPRBS5 (sprbs1.f:371-495) when MSPAR(58) >= 1 */
+    const fint *peri2e = (F1(mspar, 58) >= 1) ? msifa : NULL;
     for (fint irhs = 0; irhs < *nrhs; ++irhs) {
         double *glb = b + irhs * (*ldb);
         fint iplnz = v.nzerol + 1;
@@ -599,9 +607,9 @@ void oracle_sprbs1(fint *mspar, const fint *mtrees, const fint *msifa, const dou
                 (jslen * (jslen + 1) - (jslen - supvar) * (jslen - supvar + 1)) / 2;
             iplnz -= supsze;
             if (F1(v.supsup, js) <= 0 || lpar57 <= 0) {
-                sprfb1(neq, jslen, &F1(v.lindx, jsstrt), glb, loc);
+                sprfb1(neq, jslen, &F1(v.lindx, jsstrt), peri2e, glb, loc);
                 sprbs6(supvar, jslen, &F1(lnz, iplnz), loc);
-                sprfb2(neq, jslen, &F1(v.lindx, jsstrt), glb, loc);
+                sprfb2(neq, jslen, &F1(v.lindx, jsstrt), peri2e, glb, loc);
             }
         }
     }
@@ -640,6 +648,13 @@ void oracle_sprprm(const double *a, const double *b, double *c, double *wa, fint
         *ierr = -2;
         return;
     }
+    /* SPRPRR (sprprm.f:327-397) addresses B and C through PERI2E */
+    const fint *peri2e = (F1(mspar, 58) >= 1) ? msifa : NULL;
+    if (*ksa <= 0 && peri2e) {
+        F1(mspar, 1) = -9;
+        *ierr = -3;
+        return;
+    }
     spr_view v;
     make_view(mspar, mtrees, msifa, &v);
     fint neq = v.neq;
@@ -664,16 +679,19 @@ void oracle_sprprm(const double *a, const double *b, double *c, double *wa, fint
                 fint iplnz = F1(v.xlnz, js) - 1;
                 for (fint j = fstvar; j <= lstvar; ++j) {
                     iplnz += 1;
-                    if (j <= neq) {
-                        F1(wa, j) = F1(wa, j) + alfa * (F1(a, iplnz) * F1(bb, j));
+                    const fint jj = peri2e ? F1(peri2e, j) : j;
+                    if (jj <= neq) {
+                        F1(wa, jj) = F1(wa, jj) + alfa * (F1(a, iplnz) * F1(bb, jj));
                         for (fint i = istrt + 1; i <= istop; ++i) {
                             iplnz += 1;
-                            const fint ii = F1(v.lindx, i);
+                            const fint ii = peri2e ? F1(peri2e, F1(v.lindx, i)) : F1(v.lindx, i);
                             if (ii <= neq) {
-                                F1(wa, j) = F1(wa, j) + alfa * (F1(a, iplnz) * F1(bb, ii));
-                                F1(wa, ii) = F1(wa, ii) + alfa * (F1(a, iplnz) * F1(bb, j));
+                                F1(wa, jj) = F1(wa, jj) + alfa * (F1(a, iplnz) * F1(bb, ii));
+                                F1(wa, ii) = F1(wa, ii) + alfa * (F1(a, iplnz) * F1(bb, jj));
                             }
                         }
+                    } else if (peri2e && istop > istrt) {
+                        iplnz += istop - istrt;
                     }
                     istrt += 1;
                 }
@@ -791,9 +809,11 @@ void oracle_sprtmm(const double *u, const double *b, double *c, double *wa, fint
         }
     }
     const int upper = (*iflag == 1 || *iflag == 11);
+    /* SPRTMR (sprtmm.f:345-480): through PERI2E, U read as LNZ whatever KSU */
+    const fint *peri2e = (F1(mspar, 58) >= 1) ? msifa : NULL;
     for (fint in = 0; in < *n; ++in) {
         const double *bb = b + in * (*m);
-        if (*ksu > 0) {
+        if (*ksu > 0 || peri2e) {
             for (fint i = 1; i <= neq; ++i) F1(wa, i) = 0.0;
             for (fint js = 1; js <= v.nsuper; ++js) {
                 const fint fstvar = F1(v.xsuper, js), lstvar = F1(v.xsuper, js + 1) - 1;
@@ -802,18 +822,21 @@ void oracle_sprtmm(const double *u, const double *b, double *c, double *wa, fint
                 fint iplnz = F1(v.xlnz, js) - 1;
                 for (fint j = fstvar; j <= lstvar; ++j) {
                     iplnz += 1;
-                    if (j <= neq) {
-                        F1(wa, j) = F1(wa, j) + F1(u, iplnz) * F1(bb, j);
+                    const fint jj = peri2e ? F1(peri2e, j) : j;
+                    if (jj <= neq) {
+                        F1(wa, jj) = F1(wa, jj) + F1(u, iplnz) * F1(bb, jj);
                         for (fint i = istrt + 1; i <= istop; ++i) {
                             iplnz += 1;
-                            const fint ii = F1(v.lindx, i);
+                            const fint ii = peri2e ? F1(peri2e, F1(v.lindx, i)) : F1(v.lindx, i);
                             if (ii <= neq) {
                                 if (upper)
-                                    F1(wa, j) = F1(wa, j) + F1(u, iplnz) * F1(bb, ii);
+                                    F1(wa, jj) = F1(wa, jj) + F1(u, iplnz) * F1(bb, ii);
                                 else
-                                    F1(wa, ii) = F1(wa, ii) + F1(u, iplnz) * F1(bb, j);
+                                    F1(wa, ii) = F1(wa, ii) + F1(u, iplnz) * F1(bb, jj);
                             }
                         }
+                    } else if (peri2e && istop > istrt) {
+                        iplnz += istop - istrt;
                     }
                     istrt += 1;
                 }

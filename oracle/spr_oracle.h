@@ -41,7 +41,7 @@ void oracle_sprsol(const fint *iop, fint *mspar, const fint *mtrees, const fint 
                    fint *iwork, double *rwork, const fint *lpu, fint *ierr);
 /* SPRPRM(A,B,C,WA,MSPAR,MTREES,MSIFA,M,N,KSA,IFLAG,LPU,IERR) with A the
  * unfactored matrix SM(NEQ+1:) (KSA>0) or a diagonal (KSA<=0); IFLAG +-1:
- * C = +-AB, +-2: C = C +- AB.  MSPAR(58)>=1 (PERI2E) is not restated. */
+ * C = +-AB, +-2: C = C +- AB.  MSPAR(58)>=1: SPRPRR (sprprm.f:327-397). */
 void oracle_sprprm(const double *a, const double *b, double *c, double *wa, fint *mspar,
                    const fint *mtrees, const fint *msifa, const fint *m, const fint *n,
                    const fint *ksa, const fint *iflag, const fint *lpu, fint *ierr);
@@ -61,7 +61,7 @@ fint oracle_assemble_free(const fint *mspar, const fint *mtrees, const fint *msi
 /* SPRTMM(U,B,C,WA,MSPAR,MTREES,MSIFA,M,N,KSU,IFLAG,LPU,IERR): with L the
  * stored lower triangle of U (diagonal included), IFLAG 1: C = L^T B, 2:
  * C = L B; 11/22 (N = 1): the same into WA only (SPRTMQ, sprtmm.f:242-344).
- * KSU <= 0: U is a diagonal.  MSPAR(58)>=1 (PERI2E, SPRTMR) is not restated. */
+ * KSU <= 0: U is a diagonal.  MSPAR(58)>=1: SPRTMR (sprtmm.f:345-480). */
 void oracle_sprtmm(const double *u, const double *b, double *c, double *wa, fint *mspar,
                    const fint *mtrees, const fint *msifa, const fint *m, const fint *n,
                    const fint *ksu, const fint *iflag, const fint *lpu, fint *ierr);

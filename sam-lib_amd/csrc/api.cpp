@@ -45,7 +45,7 @@ static void free_all(sprgpu_handle *h)
                     d.solve_small, d.wide,     d.wide_fwd_prefix, d.wide_bwd_prefix,
                     d.wide_rect_prefix, d.wide_frect_prefix, d.upd_ptr, d.upd_src, d.flags, d.partials,
                     d.spin_error, d.col_node, d.epoch, d.prm_count,
-                    d.tmm_rptr, d.tmm_segs, d.tmm_long};
+                    d.tmm_rptr, d.tmm_segs, d.tmm_long, d.peri2e, d.permwork};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     free_assembly(h);
@@ -173,6 +173,12 @@ int sprgpu_create(const sprgpu_int *mspar, const sprgpu_int *mtrees, const sprgp
         free_all(h.get());
         return rc;
     }
+    if (mspar[57] >= 1) {  // PERI2E = MSIFA(1:NEQ) (sprsmb.f:181, sprabc.f:45-66)
+        if (int rc = upload_permutation(h.get(), msifa)) {
+            free_all(h.get());
+            return rc;
+        }
+    }
     sprgpu_int &B = h->device_bytes;
     const size_t esz = precision == SPRGPU_FP32 ? sizeof(float) : sizeof(double);
     hipError_t e = hipMalloc(&d.pool, esz * size_t(p.pool_elems));
@@ -237,6 +243,9 @@ int sprgpu_factor_solve_graph(sprgpu_handle *h, const double *sm_dev, double *b_
         // every buffer the sequence needs exists before capturing (the
         // factor's are allocated with the handle)
         if (int rc = prepare_solve(h, nrhs)) return rc;
+        double *w0 = nullptr, *w1 = nullptr;
+        if (h->d.peri2e)
+            if (int rc = perm_work(h, nrhs, &w0, &w1)) return rc;
         SPRGPU_HIP(hipStreamSynchronize(h->stream));
         if (!h->capture_stream)
             SPRGPU_HIP(hipStreamCreateWithFlags(&h->capture_stream, hipStreamNonBlocking));
@@ -245,7 +254,12 @@ int sprgpu_factor_solve_graph(sprgpu_handle *h, const double *sm_dev, double *b_
         hipGraph_t g = nullptr;
         SPRGPU_HIP(hipStreamBeginCapture(h->capture_stream, hipStreamCaptureModeThreadLocal));
         int rc = run_factor(h, sm_dev, 1, tol);
-        if (rc == 0) rc = run_solve(h, b_dev, ldb, nrhs, 3);
+        if (rc == 0 && !h->d.peri2e) rc = run_solve(h, b_dev, ldb, nrhs, 3);
+        if (rc == 0 && h->d.peri2e) {  // B in external order
+            if (!(rc = perm_gather(h, b_dev, ldb, w0, h->plan.neq, nrhs)) &&
+                !(rc = run_solve(h, w0, h->plan.neq, nrhs, 3)))
+                rc = perm_scatter(h, w0, h->plan.neq, b_dev, ldb, nrhs);
+        }
         const hipError_t e = hipStreamEndCapture(h->capture_stream, &g);
         h->stream = user;
         if (rc) {
@@ -430,9 +444,10 @@ int sprgpu_get_dcor(sprgpu_handle *h, double *dcor_out)
     return 0;
 }
 
-int sprgpu_solve(sprgpu_handle *h, double *b, sprgpu_int ldb, sprgpu_int nrhs, int which)
+// The solves in internal equation order.
+static int solve_internal(sprgpu_handle *h, double *b, sprgpu_int ldb, sprgpu_int nrhs,
+                          int which)
 {
-    if (!h || !b || ldb < h->plan.neq || which < 1 || which > 3) return -2;
     if (h->mode == 2 && which != 3) {
         // C^-1 b = sqrt(D) D^-1 L^-1 b; C^-T y = L^-T D^-1/2 y
         if (which == 1) {
@@ -445,24 +460,83 @@ int sprgpu_solve(sprgpu_handle *h, double *b, sprgpu_int ldb, sprgpu_int nrhs, i
     return run_solve(h, b, ldb, nrhs, which);
 }
 
+int sprgpu_solve(sprgpu_handle *h, double *b, sprgpu_int ldb, sprgpu_int nrhs, int which)
+{
+    if (!h || !b || ldb < h->plan.neq || which < 1 || which > 3) return -2;
+    if (!h->d.peri2e || nrhs <= 0) return solve_internal(h, b, ldb, nrhs, which);
+    // B in external order (SPRFS4 / SPRBS5 with SPRFB3 / SPRFB4)
+    if (!h->factored) {
+        set_error("solve called before a successful factorization");
+        return -1;
+    }
+    const i64 neq = h->plan.neq;
+    double *w0, *w1;
+    if (int rc = perm_work(h, nrhs, &w0, &w1)) return rc;
+    if (int rc = perm_gather(h, b, ldb, w0, neq, nrhs)) return rc;
+    if (int rc = solve_internal(h, w0, neq, nrhs, which)) return rc;
+    return perm_scatter(h, w0, neq, b, ldb, nrhs);
+}
+
 int sprgpu_spmv(sprgpu_handle *h, const double *x, double *y, sprgpu_int ld, sprgpu_int nrhs)
 {
     if (!h || !x || !y || ld < h->plan.neq) return -2;
-    return run_spmv(h, x, y, ld, nrhs);
+    if (!h->d.peri2e || nrhs <= 0) return run_spmv(h, x, y, ld, nrhs);
+    const i64 neq = h->plan.neq;
+    double *w0, *w1;
+    if (int rc = perm_work(h, nrhs, &w0, &w1)) return rc;
+    if (int rc = perm_gather(h, x, ld, w0, neq, nrhs)) return rc;
+    if (int rc = run_spmv(h, w0, w1, neq, nrhs)) return rc;
+    return perm_scatter(h, w1, neq, y, ld, nrhs);
 }
 
 int sprgpu_sprprm(sprgpu_handle *h, const double *a_dev, const double *b_dev, double *c_dev,
                   double *wa_dev, sprgpu_int m, sprgpu_int n, sprgpu_int ksa, sprgpu_int iflag)
 {
     if (!h || !a_dev || !b_dev) return -2;
-    return run_sprprm(h, a_dev, b_dev, c_dev, wa_dev, m, n, ksa, iflag);
+    if (!h->d.peri2e || ksa <= 0 || m < h->plan.neq || n < 1)
+        return run_sprprm(h, a_dev, b_dev, c_dev, wa_dev, m, n, ksa, iflag);
+    // SPRPRR (sprprm.f:327-397): B, C and WA in external order
+    const i64 neq = h->plan.neq;
+    const bool only_wa = (iflag == 88 && n == 1);
+    if ((only_wa && !wa_dev) || (!only_wa && !c_dev))
+        return run_sprprm(h, a_dev, b_dev, c_dev, wa_dev, m, n, ksa, iflag);  // its error
+    double *w0, *w1;
+    if (int rc = perm_work(h, n + 1, &w0, &w1)) return rc;
+    double *wa_int = w1 + neq * n;
+    if (int rc = perm_gather(h, b_dev, m, w0, neq, n)) return rc;
+    if (!only_wa && (iflag == 2 || iflag == -2))
+        if (int rc = perm_gather(h, c_dev, m, w1, neq, n)) return rc;
+    if (int rc = run_sprprm(h, a_dev, w0, only_wa ? nullptr : w1, wa_int, neq, n, ksa, iflag))
+        return rc;
+    if (!only_wa)
+        if (int rc = perm_scatter(h, w1, neq, c_dev, m, n)) return rc;
+    if (wa_dev) return perm_scatter(h, wa_int, neq, wa_dev, m, 1);
+    return 0;
 }
 
 int sprgpu_sprtmm(sprgpu_handle *h, const double *u_dev, const double *b_dev, double *c_dev,
                   double *wa_dev, sprgpu_int m, sprgpu_int n, sprgpu_int ksu, sprgpu_int iflag)
 {
     if (!h || !u_dev || !b_dev) return -2;
-    return run_sprtmm(h, u_dev, b_dev, c_dev, wa_dev, m, n, ksu, iflag);
+    if (!h->d.peri2e || m < h->plan.neq || n < 1)
+        return run_sprtmm(h, u_dev, b_dev, c_dev, wa_dev, m, n, ksu, iflag);
+    // SPRTMR (sprtmm.f:345-480): B, C and WA in external order; U is read as
+    // a stored lower triangle whatever KSU says, as SPRTMR reads it
+    const i64 neq = h->plan.neq;
+    const bool only_wa = (iflag == 11 || iflag == 22) && n == 1;
+    if ((only_wa && !wa_dev) || (!only_wa && !c_dev) ||
+        (!only_wa && iflag != 1 && iflag != 2))
+        return run_sprtmm(h, u_dev, b_dev, c_dev, wa_dev, m, n, 1, iflag);  // its error
+    double *w0, *w1;
+    if (int rc = perm_work(h, n + 1, &w0, &w1)) return rc;
+    double *wa_int = w1 + neq * n;
+    if (int rc = perm_gather(h, b_dev, m, w0, neq, n)) return rc;
+    if (int rc = run_sprtmm(h, u_dev, w0, only_wa ? nullptr : w1, wa_int, neq, n, 1, iflag))
+        return rc;
+    if (!only_wa)
+        if (int rc = perm_scatter(h, w1, neq, c_dev, m, n)) return rc;
+    if (wa_dev) return perm_scatter(h, wa_int, neq, wa_dev, m, 1);
+    return 0;
 }
 
 int sprgpu_lanczos(sprgpu_handle *h, const double *a_sm_dev, const double *b_dev, int ksb,
@@ -479,7 +553,13 @@ int sprgpu_solve_ir(sprgpu_handle *h, const double *b, double *x, sprgpu_int ld,
                     sprgpu_int nrhs, double rtol, int maxit, sprgpu_ir_info *info)
 {
     if (!h || !b || !x || ld < h->plan.neq) return -2;
-    return run_solve_ir(h, b, x, ld, nrhs, rtol, maxit, info);
+    if (!h->d.peri2e || nrhs <= 0) return run_solve_ir(h, b, x, ld, nrhs, rtol, maxit, info);
+    const i64 neq = h->plan.neq;
+    double *w0, *w1;
+    if (int rc = perm_work(h, nrhs, &w0, &w1)) return rc;
+    if (int rc = perm_gather(h, b, ld, w0, neq, nrhs)) return rc;
+    if (int rc = run_solve_ir(h, w0, w1, neq, nrhs, rtol, maxit, info)) return rc;
+    return perm_scatter(h, w1, neq, x, ld, nrhs);
 }
 
 int sprgpu_element_grad(sprgpu_handle *h, sprgpu_int nel, sprgpu_int nedof,

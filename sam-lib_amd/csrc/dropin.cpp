@@ -45,6 +45,12 @@ constexpr i64 kSigSamples = 4096;
 std::mutex g_mu;
 std::map<const double *, Entry> g_cache;
 unsigned long long g_clock = 0;
+// LIERR of the last drop-in factorization on this thread: the reference's
+// internal error code behind IERR = -3 (sprfc1.f:250-262, sprfc2.f:244-255,
+// sprfc3.f:250-262): -30 a pivot at or below TOL(1)|a_jj| (SPRFC1), -32 the
+// same in SPRFC2, -31 pivots replaced (SPRFC3), -35 a zero diagonal before
+// the factorization; 0 otherwise.  The Fortran layer passes it to SPRER1.
+thread_local i64 g_lierr = 0;
 
 // Signature of a factor in SM: FNV-1a over the bits of SM(1:NEQ) and of
 // kSigSamples entries of LNZ spread evenly over it.
@@ -166,6 +172,7 @@ void factor_dropin(int mode, sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *
 {
     // stage protocol (sprfc1.f:140-148; SPRFC2 uses 6, sprfc2.f)
     const sprgpu_int stage = mode == 2 ? 6 : 5;
+    g_lierr = 0;
     if (mspar[0] < 0) {
         mspar[0] = -stage;
         *ierr = -1;
@@ -224,6 +231,7 @@ void factor_dropin(int mode, sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *
     if (info.ierr == -3) {
         *ierr = -3;
         mspar[26] = info.supeqt;
+        g_lierr = (mode != 3 && info.nzero_diag > 0) ? -35 : mode == 3 ? -31 : mode == 2 ? -32 : -30;
     } else if (info.ierr == 3) {
         *ierr = 3;
     }
@@ -300,33 +308,19 @@ void solve_dropin(int which, bool cc, sprgpu_int *mspar, sprgpu_int *mtrees, spr
         if (he != hipSuccess) rc = hip_fail(he, "allocate B");
         else e->b_cap = neq * k;
     }
-    // PERI2E = MSIFA(1:NEQ) when MSPAR(58) >= 1 (sprfs1.f:192, SPRFB3/SPRFB4)
-    const bool perm = mspar[57] >= 1;
-    std::vector<double> tmp;
+    // With MSPAR(58) >= 1 B is in the caller's external order; the handle
+    // applies PERI2E on the device (SPRFS4 / SPRBS5 through SPRFB3 / SPRFB4,
+    // sprfs1.f:396-528, sprbs1.f:371-495)
     if (!rc) {
-        if (perm) {
-            tmp.resize(neq * k);
-            for (i64 c = 0; c < k; ++c)
-                for (i64 i = 0; i < neq; ++i) tmp[c * neq + i] = b[c * (*ldb) + msifa[i] - 1];
-            he = hipMemcpy(e->d_b, tmp.data(), sizeof(double) * neq * k, hipMemcpyHostToDevice);
-        } else {
-            he = hipMemcpy2D(e->d_b, sizeof(double) * neq, b, sizeof(double) * (*ldb),
-                             sizeof(double) * neq, k, hipMemcpyHostToDevice);
-        }
+        he = hipMemcpy2D(e->d_b, sizeof(double) * neq, b, sizeof(double) * (*ldb),
+                         sizeof(double) * neq, k, hipMemcpyHostToDevice);
         if (he != hipSuccess) rc = hip_fail(he, "upload B");
     }
     if (!rc) rc = sprgpu_solve(h, e->d_b, neq, k, which);
     if (!rc) rc = check_solve_error(h);
     if (!rc) {
-        if (perm) {
-            he = hipMemcpy(tmp.data(), e->d_b, sizeof(double) * neq * k, hipMemcpyDeviceToHost);
-            if (he == hipSuccess)
-                for (i64 c = 0; c < k; ++c)
-                    for (i64 i = 0; i < neq; ++i) b[c * (*ldb) + msifa[i] - 1] = tmp[c * neq + i];
-        } else {
-            he = hipMemcpy2D(b, sizeof(double) * (*ldb), e->d_b, sizeof(double) * neq,
-                             sizeof(double) * neq, k, hipMemcpyDeviceToHost);
-        }
+        he = hipMemcpy2D(b, sizeof(double) * (*ldb), e->d_b, sizeof(double) * neq,
+                         sizeof(double) * neq, k, hipMemcpyDeviceToHost);
         if (he != hipSuccess) rc = hip_fail(he, "download B");
     }
     if (rc) {
@@ -421,6 +415,8 @@ void sprgpu_sprsol(const sprgpu_int *iop, sprgpu_int *mspar, sprgpu_int *mtrees,
     if (*iop >= 3 && *iop != 5 && *ierr >= 0)
         sprgpu_sprbs1(mspar, mtrees, msifa, sm, b, ldb, nrhs, rwork, lpu, ierr);
 }
+
+sprgpu_int sprgpu_dropin_lierr(void) { return g_lierr; }
 
 void sprgpu_release(const double *sm)
 {

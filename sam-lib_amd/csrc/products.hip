@@ -211,10 +211,6 @@ int run_sprtmm(sprgpu_handle *h, const double *u, const double *b, double *c, do
         set_error("sprtmm: M < NEQ (retained equations are not supported)");
         return -3;
     }
-    if (h->mspar.size() >= 58 && h->mspar[57] >= 1) {
-        set_error("sprtmm: the PERI2E permutation (MSPAR(58) >= 1) is not supported");
-        return -2;
-    }
     if (n < 1 || (only_wa && !wa) || (!only_wa && !c)) {
         set_error("sprtmm: N must be positive; C (or WA for IFLAG 11/22) is required");
         return -2;

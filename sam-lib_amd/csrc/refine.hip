@@ -317,10 +317,6 @@ int run_sprprm(sprgpu_handle *h, const double *a, const double *b, double *c, do
         set_error("sprprm: M < NEQ (retained equations are not supported)");
         return -4;
     }
-    if (ksa > 0 && peri2e) {
-        set_error("sprprm: the PERI2E permutation (MSPAR(58) >= 1) is not supported");
-        return -2;
-    }
     if (n < 1 || (only_wa && !wa) || (!only_wa && !c)) {
         set_error("sprprm: N must be positive; C (or WA for IFLAG 88) is required");
         return -2;

@@ -246,6 +246,11 @@ struct DeviceBuffers {
     i64 *asm_tpos = nullptr, *asm_tptr = nullptr, *asm_src = nullptr;
     i64 *asm_emoff = nullptr;   // default element-matrix offsets (consecutive)
     i64 asm_ntgt = 0, asm_ncontrib = 0, asm_nel = 0;
+    // PERI2E (MSPAR(58) >= 1): internal -> external equation, 0-based, and
+    // the work space of the permuted entry points (two blocks)
+    i32 *peri2e = nullptr;
+    double *permwork = nullptr;
+    i64 permwork_cap = 0;
 };
 
 }  // namespace sprgpu
@@ -329,6 +334,13 @@ int run_lanczos(sprgpu_handle *h, const double *a_sm, const double *b, int ksb, 
 // products.hip
 int run_sprtmm(sprgpu_handle *h, const double *u, const double *b, double *c, double *wa,
                i64 m, i64 n, i64 ksu, i64 iflag);
+// permute.hip: the PERI2E permutation (MSPAR(58) >= 1)
+int upload_permutation(sprgpu_handle *h, const i64 *msifa);
+// two work blocks of NEQ x nrhs each
+int perm_work(sprgpu_handle *h, i64 nrhs, double **w0, double **w1);
+// dst(i, c) = src(PERI2E(i), c) and back, i < NEQ, c < nrhs
+int perm_gather(sprgpu_handle *h, const double *src, i64 lds, double *dst, i64 ldd, i64 nrhs);
+int perm_scatter(sprgpu_handle *h, const double *src, i64 lds, double *dst, i64 ldd, i64 nrhs);
 // assemble.hip
 int set_assembly(sprgpu_handle *h, i64 nel, const i64 *meen_ptr, const i64 *meen);
 int run_assemble(sprgpu_handle *h, const double *em, const i64 *em_off, const double *escale,

@@ -10,9 +10,23 @@
 ! run on the GPU.  Integers are INTEGER*8, as in the reference's default
 ! build (src/SPR/CMakeLists.txt:12-29).
 module sprgpu_c
-  use iso_c_binding, only: c_int64_t, c_double, c_funptr, c_null_funptr
+  use iso_c_binding, only: c_int64_t, c_double, c_funptr, c_null_funptr, c_ptr, c_char, &
+                           c_size_t, c_f_pointer
   implicit none
   interface
+    function c_lierr() bind(C, name='sprgpu_dropin_lierr')
+      import :: c_int64_t
+      integer(c_int64_t) :: c_lierr
+    end function
+    function c_last_error() bind(C, name='sprgpu_last_error')
+      import :: c_ptr
+      type(c_ptr) :: c_last_error
+    end function
+    function c_strlen(p) bind(C, name='strlen')
+      import :: c_ptr, c_size_t
+      type(c_ptr), value :: p
+      integer(c_size_t) :: c_strlen
+    end function
     subroutine c_sprfc1(mspar, mtrees, msifa, sm, tol, iwork, rwork, lpu, ierr, mv, mm) &
         bind(C, name='sprgpu_sprfc1')
       import :: c_int64_t, c_double, c_funptr
@@ -64,16 +78,97 @@ module sprgpu_c
     end subroutine
   end interface
 contains
-  ! Message on unit LPU in the spirit of SPRER1 (silent if LPU < 0).
-  subroutine sprgpu_message(name, lpu, ierr)
+  ! Messages on unit LPU as the reference drivers write them (silent if
+  ! LPU < 0): SPRER1, the reference's own message routine (src/SPR/sprer1.f,
+  ! linked from the host library with the symbolic phase), with the codes
+  ! the reference passes (sprfc1.f:143,234,257,264; sprfc2.f:136,227,250;
+  ! sprfc3.f:142,233,256,265; sprfs1.f:140,229; sprbs1.f:135,214), plus the
+  ! pivot diagnostic of IERR -3 and the SPRFC3 lists in the same layout as
+  ! the reference's, and one message of this layer for a HIP error (IERR -5).
+  subroutine factor_messages(name, mspar, tol, dcor, neq, lpu, ierr)
+    character(len=*), intent(in) :: name
+    integer(c_int64_t), intent(in) :: mspar(*), neq, lpu, ierr
+    real(c_double), intent(in) :: tol(*), dcor(2, *)
+    integer(c_int64_t) :: lierr, i
+    external :: SPRER1
+    if (ierr == -1) call SPRER1(30_c_int64_t, name, 0_c_int64_t, 0_c_int64_t, 0_c_int64_t, &
+                                lpu, ierr)
+    if (ierr == -2) call SPRER1(36_c_int64_t, name, 0_c_int64_t, 0_c_int64_t, 0_c_int64_t, &
+                                lpu, ierr)
+    if (ierr == -5) call hip_message(name, lpu)
+    if (ierr == -3) then
+      lierr = c_lierr()
+      if (lierr == -31) then
+        ! SPRFC3: SUPEQT counts the replaced pivots, MSPAR(27) = -SUPEQT
+        call SPRER1(40_c_int64_t, name, mspar(26), -mspar(27), lierr, lpu, ierr)
+        if (lpu >= 0) then
+          write(lpu, 700) tol(1), -mspar(27)
+          do i = 1, neq
+            if (dcor(2, i) /= 0.0d0) write(lpu, 710) i, dcor(1, i), dcor(2, i)
+          end do
+          write(lpu, '(A)') ''
+        end if
+      else
+        call SPRER1(40_c_int64_t, name, mspar(26), mspar(27), lierr, lpu, ierr)
+        if (lpu >= 0 .and. lierr == -30) write(lpu, 600) tol(1), tol(2), tol(3), &
+                                                    abs(tol(2) / tol(3))
+        if (lpu >= 0 .and. lierr == -32) write(lpu, 601) tol(1), tol(2), tol(3), &
+                                                    tol(2) / abs(tol(3))
+      end if
+    else if (ierr == 3) then
+      call SPRER1(60_c_int64_t, name, 0_c_int64_t, mspar(27), mspar(26), lpu, ierr)
+      if (lpu >= 0 .and. neq > 0) then  ! SPRFC3: its negative pivots
+        do i = 1, neq
+          if (dcor(1, i) < 0.0d0) write(lpu, 720) i, dcor(1, i)
+        end do
+        write(lpu, '(A)') ''
+      end if
+    end if
+600 format(5x, 'ZERO FACTOR THRESHOLD,   TOL(1) = ', 1pe12.5, &
+           / 5x, 'DIAGONAL ENTRY (PIVOT),   DIAG  = ', 1pe12.5, &
+           / 5x, 'PREVIOUS PIVOT,          ADIAG  = ', 1pe12.5, &
+           / 5x, 'DIAGONAL DECAY, ABS(DIAG/ADIAG) = ', 1pe12.5, ' < TOL(1)')
+601 format(5x, 'ZERO FACTOR THRESHOLD,   TOL(1) = ', 1pe12.5, &
+           / 5x, 'DIAGONAL ENTRY (PIVOT),   DIAG  = ', 1pe12.5, &
+           / 5x, 'PREVIOUS PIVOT,          ADIAG  = ', 1pe12.5, &
+           / 5x, 'DIAGONAL DECAY, DIAG/ABS(ADIAG) = ', 1pe12.5, ' < TOL(1)')
+700 format(5x, 'ZERO FACTOR THRESHOLD,   TOL(1) = ', 1pe12.5, &
+           / 5x, 'NUMBER OF SINGULARITIES FOUND   = ', i8, &
+           // 5x, 'EQUATION   DIAGONAL ENTRY  INSERTED NEW ENTRY')
+710 format(i10, 1p2e20.8)
+720 format(5x, 'EQUATION #', i8, ' :  DIAG =', 1pe16.8)
+  end subroutine
+
+  subroutine solve_messages(name, lpu, ierr)
     character(len=*), intent(in) :: name
     integer(c_int64_t), intent(in) :: lpu, ierr
-    if (lpu < 0 .or. ierr == 0) return
-    if (ierr > 0) then
-      write(lpu, '(/1X,A,A,A,I4)') '*** WARNING from ', name, ' (GPU): IERR =', ierr
-    else
-      write(lpu, '(/1X,A,A,A,I4)') '*** ERROR from ', name, ' (GPU): IERR =', ierr
-    end if
+    external :: SPRER1
+    if (ierr == -1) call SPRER1(30_c_int64_t, name, 0_c_int64_t, 0_c_int64_t, 0_c_int64_t, &
+                                lpu, ierr)
+    if (ierr == -2) call SPRER1(37_c_int64_t, name, 0_c_int64_t, 0_c_int64_t, 0_c_int64_t, &
+                                lpu, ierr)
+    if (ierr == -5) call hip_message(name, lpu)
+  end subroutine
+
+  ! IERR -5 (a HIP runtime error, no counterpart in the reference): the
+  ! library's reason (sprgpu_last_error)
+  subroutine hip_message(name, lpu)
+    character(len=*), intent(in) :: name
+    integer(c_int64_t), intent(in) :: lpu
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: txt(:)
+    integer :: n, i
+    character(len=256) :: line
+    if (lpu < 0) return
+    p = c_last_error()
+    n = int(min(c_strlen(p), 256_c_size_t))
+    call c_f_pointer(p, txt, [max(n, 1)])
+    line = ' '
+    do i = 1, n
+      line(i:i) = txt(i)
+    end do
+    write(lpu, '(///A,A/,4X,A,A)') ' *** ERROR RETURN FROM  S A M / SPR  LIBRARY ROUTINE : ', &
+      name, ' GPU RUNTIME ERROR, IERR = -5: ', line(1:max(n, 1))
   end subroutine
 end module sprgpu_c
 
@@ -85,7 +180,7 @@ subroutine SPRFC1(MSPAR, MTREES, MSIFA, SM, TOL, IWORK, RWORK, LPU, IERR, MVMULT
   external :: MVMULT, MMMULT
   call c_sprfc1(MSPAR, MTREES, MSIFA, SM, TOL, IWORK, RWORK, LPU, IERR, &
                 c_null_funptr, c_null_funptr)
-  call sprgpu_message('SPRFC1', LPU, IERR)
+  call factor_messages('SPRFC1', MSPAR, TOL, TOL, 0_c_int64_t, LPU, IERR)
 end subroutine SPRFC1
 
 subroutine SPRFC3(MSPAR, MTREES, MSIFA, SM, TOL, IWORK, RWORK, DCOR, LPU, IERR, MVMULT, MMMULT)
@@ -96,7 +191,7 @@ subroutine SPRFC3(MSPAR, MTREES, MSIFA, SM, TOL, IWORK, RWORK, DCOR, LPU, IERR, 
   external :: MVMULT, MMMULT
   call c_sprfc3(MSPAR, MTREES, MSIFA, SM, TOL, IWORK, RWORK, DCOR, LPU, IERR, &
                 c_null_funptr, c_null_funptr)
-  call sprgpu_message('SPRFC3', LPU, IERR)
+  call factor_messages('SPRFC3', MSPAR, TOL, DCOR, MSPAR(8), LPU, IERR)
 end subroutine SPRFC3
 
 subroutine SPRFS1(MSPAR, MTREES, MSIFA, SM, B, LDB, NRHS, IWORK, RWORK, LPU, IERR)
@@ -105,7 +200,7 @@ subroutine SPRFS1(MSPAR, MTREES, MSIFA, SM, B, LDB, NRHS, IWORK, RWORK, LPU, IER
   integer(c_int64_t) :: MSPAR(*), MTREES(*), MSIFA(*), IWORK(*), LDB, NRHS, LPU, IERR
   real(c_double) :: SM(*), B(*), RWORK(*)
   call c_sprfs1(MSPAR, MTREES, MSIFA, SM, B, LDB, NRHS, IWORK, RWORK, LPU, IERR)
-  call sprgpu_message('SPRFS1', LPU, IERR)
+  call solve_messages('SPRFS1', LPU, IERR)
 end subroutine SPRFS1
 
 subroutine SPRBS1(MSPAR, MTREES, MSIFA, SM, B, LDB, NRHS, RWORK, LPU, IERR)
@@ -114,7 +209,7 @@ subroutine SPRBS1(MSPAR, MTREES, MSIFA, SM, B, LDB, NRHS, RWORK, LPU, IERR)
   integer(c_int64_t) :: MSPAR(*), MTREES(*), MSIFA(*), LDB, NRHS, LPU, IERR
   real(c_double) :: SM(*), B(*), RWORK(*)
   call c_sprbs1(MSPAR, MTREES, MSIFA, SM, B, LDB, NRHS, RWORK, LPU, IERR)
-  call sprgpu_message('SPRBS1', LPU, IERR)
+  call solve_messages('SPRBS1', LPU, IERR)
 end subroutine SPRBS1
 
 subroutine SPRFC2(MSPAR, MTREES, MSIFA, SM, TOL, IWORK, RWORK, LPU, IERR, MVMULT, MMMULT)
@@ -125,7 +220,7 @@ subroutine SPRFC2(MSPAR, MTREES, MSIFA, SM, TOL, IWORK, RWORK, LPU, IERR, MVMULT
   external :: MVMULT, MMMULT
   call c_sprfc2(MSPAR, MTREES, MSIFA, SM, TOL, IWORK, RWORK, LPU, IERR, &
                 c_null_funptr, c_null_funptr)
-  call sprgpu_message('SPRFC2', LPU, IERR)
+  call factor_messages('SPRFC2', MSPAR, TOL, TOL, 0_c_int64_t, LPU, IERR)
 end subroutine SPRFC2
 
 subroutine SPRFS2(MSPAR, MTREES, MSIFA, SM, B, LDB, NRHS, IWORK, RWORK, LPU, IERR)
@@ -134,7 +229,7 @@ subroutine SPRFS2(MSPAR, MTREES, MSIFA, SM, B, LDB, NRHS, IWORK, RWORK, LPU, IER
   integer(c_int64_t) :: MSPAR(*), MTREES(*), MSIFA(*), IWORK(*), LDB, NRHS, LPU, IERR
   real(c_double) :: SM(*), B(*), RWORK(*)
   call c_sprfs2(MSPAR, MTREES, MSIFA, SM, B, LDB, NRHS, IWORK, RWORK, LPU, IERR)
-  call sprgpu_message('SPRFS2', LPU, IERR)
+  call solve_messages('SPRFS2', LPU, IERR)
 end subroutine SPRFS2
 
 subroutine SPRBS2(MSPAR, MTREES, MSIFA, SM, B, LDB, NRHS, RWORK, LPU, IERR)
@@ -143,7 +238,7 @@ subroutine SPRBS2(MSPAR, MTREES, MSIFA, SM, B, LDB, NRHS, RWORK, LPU, IERR)
   integer(c_int64_t) :: MSPAR(*), MTREES(*), MSIFA(*), LDB, NRHS, LPU, IERR
   real(c_double) :: SM(*), B(*), RWORK(*)
   call c_sprbs2(MSPAR, MTREES, MSIFA, SM, B, LDB, NRHS, RWORK, LPU, IERR)
-  call sprgpu_message('SPRBS2', LPU, IERR)
+  call solve_messages('SPRBS2', LPU, IERR)
 end subroutine SPRBS2
 
 ! sprsol.f:29-50 with the GPU factor and solves
@@ -156,15 +251,15 @@ subroutine SPRSOL(IOP, MSPAR, MTREES, MSIFA, SM, B, LDB, NRHS, TOL, IWORK, RWORK
   if (IOP <= 3) then
     call c_sprfc1(MSPAR, MTREES, MSIFA, SM, TOL, IWORK, RWORK, LPU, IERR, &
                   c_null_funptr, c_null_funptr)
-    call sprgpu_message('SPRFC1', LPU, IERR)
+    call factor_messages('SPRFC1', MSPAR, TOL, TOL, 0_c_int64_t, LPU, IERR)
   end if
   if (IOP >= 2 .and. IOP <= 5 .and. IERR >= 0) then
     call c_sprfs1(MSPAR, MTREES, MSIFA, SM, B, LDB, NRHS, IWORK, RWORK, LPU, IERR)
-    call sprgpu_message('SPRFS1', LPU, IERR)
+    call solve_messages('SPRFS1', LPU, IERR)
   end if
   if (IOP >= 3 .and. IOP /= 5 .and. IERR >= 0) then
     call c_sprbs1(MSPAR, MTREES, MSIFA, SM, B, LDB, NRHS, RWORK, LPU, IERR)
-    call sprgpu_message('SPRBS1', LPU, IERR)
+    call solve_messages('SPRBS1', LPU, IERR)
   end if
 end subroutine SPRSOL
 

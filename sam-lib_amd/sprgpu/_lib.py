@@ -41,6 +41,7 @@ SIGNATURES = {
     "sprgpu_sprbs2": (None, [P] * 10),
     "sprgpu_sprsol": (None, [P] * 13),
     "sprgpu_release": (None, [P]),
+    "sprgpu_dropin_lierr": (I64, []),
     "sprgpu_create": (C.c_int, [P, P, P, C.c_int, C.POINTER(P)]),
     "sprgpu_destroy": (C.c_int, [P]),
     "sprgpu_set_stream": (C.c_int, [P, P]),

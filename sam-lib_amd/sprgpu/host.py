@@ -105,8 +105,12 @@ class SprSystem:
         return self.msifa[p:p + self.nsuper + 1]
 
 
-def symbolic(model, nspar: int = NSPAR, cache_kb: int = 0) -> SprSystem:
-    """Run the reference symbolic phase (sprprp.f:5 ... sprpmp.f:5) on a SamModel."""
+def symbolic(model, nspar: int = NSPAR, cache_kb: int = 0, nperm: int = 0) -> SprSystem:
+    """Run the reference symbolic phase (sprprp.f:5 ... sprpmp.f:5) on a SamModel.
+
+    nperm = 2 reserves the PERI2E/PERE2I permutation arrays at the start of
+    MSIFA (MSPAR(58) = 2 before SPRSMB, sprsmb.f:181, sprpmp.f:47); fill them
+    with set_permutation (SPRABC) afterwards."""
     L = lib()
     mpar = model.mpar()
     ndof, nanod, nel = model.ndof, model.nanod, model.nel
@@ -128,6 +132,8 @@ def symbolic(model, nspar: int = NSPAR, cache_kb: int = 0) -> SprSystem:
         raise SprError(f"SPRPRP IERR={ierr.value}")
     if cache_kb:
         mspar[8] = cache_kb  # MSPAR(9): cache size for panelling (sprsmb.f:137)
+    if nperm:
+        mspar[57] = nperm    # MSPAR(58): permutation arrays first in MSIFA
     nmsica = int(mspar[34])
     ntrees = int(mspar[35])
     msica = np.zeros(nmsica, dtype=I8)
@@ -148,6 +154,8 @@ def symbolic(model, nspar: int = NSPAR, cache_kb: int = 0) -> SprSystem:
               _i(ndof), _i(niwork), _p(iwork), _p(rinfo), _i(LPU), C.byref(ierr))
     if ierr.value < 0:
         raise SprError(f"SPRTRS IERR={ierr.value}")
+    if nperm:
+        mspar[2] += nperm * int(mspar[7])  # MSIFA also holds the permutations
     msifa = np.zeros(int(mspar[2]), dtype=I8)
     iwork = np.zeros(3 * int(mspar[5]) + 2 * int(mspar[7]) + 1, dtype=I8)
     L.sprsmb_(_p(mspar), _p(msica), _p(mtrees), _p(msifa), _p(meqn), _p(iwork), _i(LPU),
@@ -158,6 +166,22 @@ def symbolic(model, nspar: int = NSPAR, cache_kb: int = 0) -> SprSystem:
     mdofnc = np.zeros(2 * neq + 1, dtype=I8)
     L.sprpmp_(_p(msica), _p(mtrees), _p(meqn), _p(mpar), _p(mspar), _p(mdofnc))
     return SprSystem(mpar, mspar, meqn, msica, mtrees, msifa, mdofnc, rinfo)
+
+
+def set_permutation(sysm: SprSystem, meqn_ext: np.ndarray) -> None:
+    """SPRABC (sprabc.f:5-66): PERI2E and PERE2I between the SPR equation
+    order (sysm.meqn) and the caller's external order meqn_ext (1-based
+    external equation of each dof, 0 where the dof has none), placed first
+    in MSIFA; sets MSPAR(58) = 2.  The system must come from
+    symbolic(..., nperm=2)."""
+    if int(sysm.mspar[46]) < 1 + 2 * sysm.neq:
+        raise SprError("MSIFA has no room for the permutations: use symbolic(..., nperm=2)")
+    meqn_ext = np.ascontiguousarray(meqn_ext, dtype=I8)
+    ierr = C.c_int64(0)
+    lib().sprabc_(_p(sysm.mspar), _p(sysm.msifa), _p(sysm.meqn), _p(meqn_ext),
+                  _i(sysm.meqn.size), _i(LPU), C.byref(ierr))
+    if ierr.value != 0:
+        raise SprError(f"SPRABC IERR={ierr.value}")
 
 
 def assemble(model, sysm: SprSystem) -> np.ndarray:
