@@ -17,6 +17,7 @@ oracle/) is the reference's SPRFC1 + SPRFS1 + SPRBS1 on the 20^3 case, on
 one host core.
 
 After the timed steps, on the same 40^3 matrix and outside `value`: the
+forward + backward sweeps alone per precision against the HBM roofline, the
 device element assembly (bit equality with the host SPRADM loop's SM and its
 time), the SPR-layout products (SPRPRM: A x of the FP64 solution against b;
 SPRTMM: L^T x + L x - D x against A x) and the shift-invert Lanczos leg (the
@@ -110,6 +111,42 @@ def bench_product(sysm, h, sm_dev, x, b_dev, reps):
     return {"ms_per_call": ms, "relres_of_fp64_solution": rel,
             "sprtmm_upper_ms": out[1][1], "sprtmm_lower_ms": out[2][1],
             "sprtmm_vs_sprprm_rel": rel_tmm, "lnz_bytes": 8 * sysm.nzerol}
+
+
+def bench_solves(sysm, handles, b_dev, reps):
+    """The forward + backward sweeps alone (the factor in HBM), one RHS, per
+    precision: time per pair from HIP events on the handle's stream, and the
+    HBM roofline: algorithmic bytes per sweep = LNZ once (NZEROL entries of
+    the factor's type) + the right-hand side read and written in FP64
+    (2 NEQ 8 B), two sweeps per pair, against 8 TB/s."""
+    import torch
+
+    out = {}
+    for name, h, esz in handles:
+        x = torch.empty_like(b_dev)
+        per = {}
+        for which, label in ((3, "pair"), (1, "forward"), (2, "backward")):
+            x.copy_(b_dev)
+            h.solve(x, which=which)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                x.copy_(b_dev)
+                h.solve(x, which=which)
+            e1.record()
+            torch.cuda.synchronize()
+            per[label] = e0.elapsed_time(e1) / reps
+        sweep = sysm.nzerol * esz + 2 * sysm.neq * 8
+        ach = 2 * sweep / (per["pair"] * 1e-3) / 1e9
+        out[name] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK, "traffic": None,
+                     "kernel": "forward + backward sweeps (k_fwd_*, k_bwd_*), 1 RHS",
+                     "algorithmic_bytes_per_pair": 2 * sweep, "pair_ms": per["pair"],
+                     "forward_ms": per["forward"], "backward_ms": per["backward"]}
+        log(f"solves {name}: pair {per['pair']:.3f} ms (fwd {per['forward']:.3f}, bwd "
+            f"{per['backward']:.3f}), {ach:.0f} GB/s = {ach / HBM_PEAK:.3f} of HBM")
+    return out
 
 
 def bench_lanczos(model, sysm, h, sm_dev, neval=10):
@@ -352,6 +389,11 @@ def main():
         "roofline": None,
         "cpu_baseline": None,
     }
+
+    # the sweeps alone against the HBM roofline
+    sol = bench_solves(sysm, (("fp64", h64, 8), ("fp32", h32, 4)), b_dev, max(args.steps, 10))
+    result["roofline_solve_fp64"] = sol["fp64"]
+    result["roofline_solve_fp32"] = sol["fp32"]
 
     # element assembly on the device (after the timed steps: it replaces the
     # FP64 handle's residual pattern by the map's, which that handle no longer uses)

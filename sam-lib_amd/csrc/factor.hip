@@ -366,7 +366,7 @@ __device__ __forceinline__ double recip(double d)
 // c = r) go to out[r*rs + c*cs] when out is set, with zeros right of the
 // diagonal if zero_upper; the rows of Bm go to bm_out + r*kPanelNB when
 // bm_out is set (rows r < nb).  lsh holds kLshSize elements of T and then
-// kPanelNB doubles of LDS scratch (16-byte aligned).  Returns the pivot d_r of the thread's row (zero for
+// 2 kPanelNB doubles of LDS scratch (16-byte aligned).  Returns the pivot d_r of the thread's row (zero for
 // r >= nb).  Every thread of the workgroup calls it.
 constexpr int kChainThreads = 256;
 constexpr int kLshSize = 11 * kPanelNB;  // LDS scratch elements of T (plus kPanelNB doubles)
@@ -386,6 +386,7 @@ __device__ __forceinline__ T diag_block_factor(const T *src, i64 srs, i64 scs, i
     T *pi = lsh + NB;            // [NB] 1/d_j
     T *pin = lsh + 2 * NB;       // [NB] pivots before the replacement rule
     double *ash = reinterpret_cast<double *>(lsh + 11 * NB);  // [NB] a_jj
+    double *pdd = ash + NB;      // [NB] pivots d_j in FP64 (recorded in D and DCOR)
     if (t < nb) ash[t] = adiag[eq0 + t];
     T a[Q], x[Q];
 #pragma unroll
@@ -476,9 +477,11 @@ __device__ __forceinline__ T diag_block_factor(const T *src, i64 srs, i64 scs, i
             pd[j] = d;
             pi[j] = di;
             pin[j] = dj;
+            pdd[j] = dp;
             pd[j + 1] = d1;
             pi[j + 1] = di1;
             pin[j + 1] = dj1;
+            pdd[j + 1] = dp1;
         }
     }
     if (j < nb) {  // an odd last column, one step
@@ -519,13 +522,17 @@ __device__ __forceinline__ T diag_block_factor(const T *src, i64 srs, i64 scs, i
             pd[j] = d;
             pi[j] = di;
             pin[j] = dj;
+            pdd[j] = dp;
         }
     }
     __syncthreads();
     const T dr = (r < nb) ? pd[r] : T(0);
     const T dir = (r < nb) ? pi[r] : T(0);
+    // The pivot's test, its replacement (SPRFC3) and its value in D and DCOR
+    // are FP64 whatever T is: an FP32 front uses d rounded to FP32 for its
+    // own columns only
     if (record && g == 0 && r < nb)
-        pivot_record(double(pin[r]), double(dr), ash[r], eq0 + r, pa, dvec, dcor, st);
+        pivot_record(double(pin[r]), pdd[r], ash[r], eq0 + r, pa, dvec, dcor, st);
     if (out && r < nb) {
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
@@ -569,7 +576,7 @@ __global__ __launch_bounds__(256) void k_panel(const PanelDesc *__restrict__ des
     static_assert(NB == 32 && kPanelRows == 256,
                   "one lane per diagonal-block row; four 64-row MFMA slices per workgroup");
     __shared__ T Bm[NB][NB + 1];  // L11^-T D^-1 (upper triangular), zero outside nb x nb
-    __shared__ __attribute__((aligned(16))) T Lsh[kLshSize + kPanelNB * sizeof(double) / sizeof(T)];
+    __shared__ __attribute__((aligned(16))) T Lsh[kLshSize + 2 * kPanelNB * sizeof(double) / sizeof(T)];
     const int q = find_desc(prefix, ndesc, blockIdx.x);
     const PanelDesc pd = descs[q];
     const int rb = int(blockIdx.x - prefix[q]);
@@ -689,7 +696,7 @@ __global__ __launch_bounds__(64 * kBdWaves) void k_block_diag(const PanelDesc *_
     constexpr int NB = kPanelNB, W = kPanelOuter, LS = NB + 1;
     typedef typename Mfma<T>::acc_t acc_t;
     __shared__ T Bm[NB][NB + 1];
-    __shared__ __attribute__((aligned(16))) T Lsh[kLshSize + kPanelNB * sizeof(double) / sizeof(T)];
+    __shared__ __attribute__((aligned(16))) T Lsh[kLshSize + 2 * kPanelNB * sizeof(double) / sizeof(T)];
     __shared__ T Ls[W * LS];  // the sub-step's two tile columns, rows relative to k0
     __shared__ T dsh[NB];
     const PanelDesc pd = descs[blockIdx.x];
