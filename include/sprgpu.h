@@ -192,6 +192,21 @@ int sprgpu_get_dcor(sprgpu_handle *handle, double *dcor_out);
 int sprgpu_solve(sprgpu_handle *handle, double *b_dev, sprgpu_int ldb, sprgpu_int nrhs,
                  int which);
 
+/* sprgpu_solve with no allocation (no counterpart in the reference): for
+ * callers that cannot allocate device memory during a call -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * Same arguments, semantics and results as sprgpu_solve (bit for bit), but
+ * the solve's work space (the per-supernode vector slots, ready flags and
+ * their epoch, partial sums, and the PERI2E copy of B) is taken from a
+ * static device array of 64 MiB that each GPU holds for the life of the
+ * process, instead of buffers the handle grows.  A call whose work space
+ * does not fit (about 50 MB per right-hand side at 40^3 hex8) returns -2
+ * with the sizes in sprgpu_last_error and touches nothing.  Every call on a
+ * GPU shares its array: calls that overlap on one GPU (other streams,
+ * threads or handles) race. */
+int sprgpu_solve_noalloc(sprgpu_handle *handle, double *b_dev, sprgpu_int ldb, sprgpu_int nrhs,
+                         int which);
+
 /* Y = A X with the assembled matrix of the last factorization, FP64. */
 int sprgpu_spmv(sprgpu_handle *handle, const double *x_dev, double *y_dev, sprgpu_int ld,
                 sprgpu_int nrhs);

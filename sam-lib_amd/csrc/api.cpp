@@ -167,6 +167,7 @@ int sprgpu_create(const sprgpu_int *mspar, const sprgpu_int *mtrees, const sprgp
         h->msifa.assign(msifa, msifa + nf);
     }
     SPRGPU_HIP(hipGetDevice(&h->device));
+    if (int rc = noalloc_scratch_init()) return rc;  // once per device
     const Plan &p = h->plan;
     DeviceBuffers &d = h->d;
     if (int rc = upload_plan(h.get())) {

@@ -62,6 +62,10 @@ int perm_work(sprgpu_handle *h, i64 nrhs, double **w0, double **w1)
 {
     DeviceBuffers &d = h->d;
     const i64 need = 2 * h->plan.neq * nrhs;
+    if (h->noalloc && d.permwork_cap < need) {
+        set_error("permutation work space larger than the scratch view");
+        return -2;
+    }
     if (d.permwork_cap < need) {
         ++h->buf_gen;
         if (d.permwork) SPRGPU_HIP(hipFree(d.permwork));

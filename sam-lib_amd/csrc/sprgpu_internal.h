@@ -286,6 +286,10 @@ struct sprgpu_handle {
     // growing for more right-hand sides); a graph captured under another
     // generation holds stale pointers and is captured again.
     unsigned long long buf_gen = 0, graph_gen = 0;
+    // Set by sprgpu_solve_noalloc while the solve buffers point into the
+    // library's static scratch: the buffer-growth paths then refuse instead
+    // of allocating.
+    bool noalloc = false;
 };
 
 namespace sprgpu {
@@ -334,6 +338,9 @@ int run_lanczos(sprgpu_handle *h, const double *a_sm, const double *b, int ksb, 
 // products.hip
 int run_sprtmm(sprgpu_handle *h, const double *u, const double *b, double *c, double *wa,
                i64 m, i64 n, i64 ksu, i64 iflag);
+// noalloc.hip: the static scratch of sprgpu_solve_noalloc, located once per
+// device at handle creation
+int noalloc_scratch_init();
 // permute.hip: the PERI2E permutation (MSPAR(58) >= 1)
 int upload_permutation(sprgpu_handle *h, const i64 *msifa);
 // two work blocks of NEQ x nrhs each

@@ -1027,6 +1027,14 @@ static int ensure_solve_buffers(sprgpu_handle *h, i64 nrhs)
 {
     DeviceBuffers &d = h->d;
     const Plan &p = h->plan;
+    if (h->noalloc) {  // the buffers are views of the static scratch, sized by the caller
+        if (d.vwork_cap < nrhs || d.flags_cap < std::max<i64>(p.nflags, 1) * nrhs ||
+            d.partials_cap < std::max<i64>(p.npart, 1) * nrhs || !d.epoch || !d.spin_error) {
+            set_error("solve workspace larger than the scratch view");
+            return -2;
+        }
+        return 0;
+    }
     if (d.vwork_cap < nrhs) {
         ++h->buf_gen;
         if (d.vwork) SPRGPU_HIP(hipFree(d.vwork));

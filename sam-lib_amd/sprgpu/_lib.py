@@ -51,6 +51,7 @@ SIGNATURES = {
     "sprgpu_factor": (C.c_int, [P, P, P, C.c_int, C.POINTER(FactorInfo)]),
     "sprgpu_factor_info_get": (C.c_int, [P, C.POINTER(FactorInfo)]),
     "sprgpu_solve": (C.c_int, [P, P, I64, I64, C.c_int]),
+    "sprgpu_solve_noalloc": (C.c_int, [P, P, I64, I64, C.c_int]),
     "sprgpu_spmv": (C.c_int, [P, P, P, I64, I64]),
     "sprgpu_sprprm": (C.c_int, [P, P, P, P, P, I64, I64, I64, I64]),
     "sprgpu_sprtmm": (C.c_int, [P, P, P, P, P, I64, I64, I64, I64]),
