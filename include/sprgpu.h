@@ -196,8 +196,8 @@ int sprgpu_solve(sprgpu_handle *handle, double *b_dev, sprgpu_int ldb, sprgpu_in
  * callers that cannot allocate device memory during a call -- code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
  * Same arguments, semantics and results as sprgpu_solve (bit for bit), but
- * the solve's work space (the per-supernode vector slots, ready flags and
- * their epoch, partial sums, and the PERI2E copy of B) is taken from a
+ * the solve's work space (the per-supernode vector slots, partial sums, a
+ * spin-error word and the PERI2E copy of B) is taken from a
  * static device array of 64 MiB that each GPU holds for the life of the
  * process, instead of buffers the handle grows.  A call whose work space
  * does not fit (about 50 MB per right-hand side at 40^3 hex8) returns -2
@@ -340,7 +340,8 @@ int sprgpu_autotune(sprgpu_handle *handle, const double *sm_dev, double *b_dev, 
  * outer blocks, 0 = 32-column panel steps). */
 int sprgpu_plan_params(const sprgpu_handle *handle, int *out);
 /* Re-plans with the given parameters, in the order of sprgpu_plan_params
- * (update tile 0 = by precision).  Returns 0 or -2 (a value out of range).
+ * (update tile 0 = per launch: 128 for FP32 launches of at least 4096 64-tiles,
+ * else 64).  Returns 0 or -2 (a value out of range).
  * No counterpart in the reference; for tuning and comparisons. */
 int sprgpu_set_plan_params(sprgpu_handle *handle, const int *params);
 

@@ -43,8 +43,8 @@ static void free_all(sprgpu_handle *h)
                     d.dscr,     d.dinv,
                     d.csr_src,  d.csr_val,    d.irwork,      d.level_nodes,
                     d.solve_small, d.wide,     d.wide_fwd_prefix, d.wide_bwd_prefix,
-                    d.wide_rect_prefix, d.wide_frect_prefix, d.upd_ptr, d.upd_src, d.flags, d.partials,
-                    d.spin_error, d.col_node, d.epoch, d.prm_count,
+                    d.wide_rect_prefix, d.wide_frect_prefix, d.upd_ptr, d.upd_src, d.partials,
+                    d.spin_error, d.col_node, d.prm_count,
                     d.tmm_rptr, d.tmm_segs, d.tmm_long, d.peri2e, d.permwork};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -291,7 +291,9 @@ int sprgpu_autotune(sprgpu_handle *h, const double *sm_dev, double *b_dev, sprgp
 {
     if (!h || !sm_dev || !b_dev || nrhs < 1 || ldb < h->plan.neq) return -2;
     std::vector<PlanParams> cand;
-    const int tiles[2] = {kTileSmall, kTileBig};
+    // FP32 tries the per-launch tile choice and the 128 tile everywhere
+    // (36 candidates: the report's 256 doubles hold 51)
+    const int tiles[2] = {0, kTileBig};
     for (int sf : {32, 48, 64})
         for (int w : {1, 2, 4})
             for (int nt : {64, 256})

@@ -6,9 +6,8 @@
 // HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
 // The solve kernels use, besides the factor and the caller's B, a vector
 // slot per supernode per right-hand side (the forward sweep's update
-// vectors), the dataflow ready flags and their sweep epoch, the backward
-// sweep's partial sums, a spin-error word and, with PERI2E, the permuted
-// copy of B.  Here all of them are views of `scratch`, a file-scope array
+// vectors and the wide supernodes' block results), the backward sweep's
+// partial sums, a spin-error word and, with PERI2E, the permuted copy of B.  Here all of them are views of `scratch`, a file-scope array
 // that every GPU the library runs on holds until the process ends
 // (kScratchBytes); its device address is looked up once per device when a
 // handle is created there.  Calls that do not fit return -2 and allocate
@@ -16,9 +15,9 @@
 //
 // All calls on one GPU share the array: two of them that overlap on that
 // GPU (different streams, threads or handles) race.  The array keeps the
-// last call's contents; the ready flags only ever take epoch values and the
-// epoch only grows, so stale flags never satisfy a new sweep, and the
-// spin-error word is cleared by a kernel at the start of every call.
+// last call's contents; every sweep marks its block result slots pending
+// before it starts, and the spin-error word is cleared by a kernel at the
+// start of every call.
 #include <mutex>
 
 #include "sprgpu_internal.h"
@@ -84,12 +83,9 @@ extern "C" int sprgpu_solve_noalloc(sprgpu_handle *h, double *b, sprgpu_int ldb,
         return -5;
     }
     const Plan &p = h->plan;
-    // layout: epoch | spin error | ready flags | partial sums | vector slots | PERI2E copies
-    const size_t o_epoch = 0;
-    const size_t o_err = align_up(o_epoch + sizeof(unsigned));
-    const size_t o_flags = align_up(o_err + sizeof(int));
-    const i64 nflags = std::max<i64>(p.nflags, 1) * nrhs;
-    const size_t o_part = align_up(o_flags + sizeof(unsigned) * size_t(nflags));
+    // layout: spin error | partial sums | vector slots | PERI2E copies
+    const size_t o_err = 0;
+    const size_t o_part = align_up(o_err + sizeof(int));
     const i64 npart = std::max<i64>(p.npart, 1) * nrhs;
     const size_t o_vw = align_up(o_part + sizeof(double) * size_t(npart));
     const i64 nvw = p.vwork_elems * nrhs;
@@ -104,10 +100,7 @@ extern "C" int sprgpu_solve_noalloc(sprgpu_handle *h, double *b, sprgpu_int ldb,
     }
     DeviceBuffers &d = h->d;
     DeviceBuffers saved = d;
-    d.epoch = reinterpret_cast<unsigned *>(base + o_epoch);
     d.spin_error = reinterpret_cast<int *>(base + o_err);
-    d.flags = reinterpret_cast<unsigned *>(base + o_flags);
-    d.flags_cap = nflags;
     d.partials = reinterpret_cast<double *>(base + o_part);
     d.partials_cap = npart;
     d.vwork = reinterpret_cast<double *>(base + o_vw);
@@ -119,10 +112,7 @@ extern "C" int sprgpu_solve_noalloc(sprgpu_handle *h, double *b, sprgpu_int ldb,
     const hipError_t e = hipGetLastError();
     int rc = (e == hipSuccess) ? sprgpu_solve(h, b, ldb, nrhs, which) : hip_fail(e, "k_clear_word");
     h->noalloc = false;
-    d.epoch = saved.epoch;
     d.spin_error = saved.spin_error;
-    d.flags = saved.flags;
-    d.flags_cap = saved.flags_cap;
     d.partials = saved.partials;
     d.partials_cap = saved.partials_cap;
     d.vwork = saved.vwork;

@@ -45,7 +45,15 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
         params.narrow_threads != 128 && params.narrow_threads != 256)
         return "narrow_threads must be 0, 64, 128 or 256";
     if (params.block_factor != 0 && params.block_factor != 1) return "block_factor must be 0 or 1";
-    p.big_tile = params.big_tile ? params.big_tile : kTileSmall;
+    p.big_tile = params.big_tile;  // 0: chosen per launch (below)
+    // Tile of a rank-kOuterRank / Schur update launch: the configured one,
+    // or with big_tile 0 in FP32 the 128 tile (32x32x2 MFMA kernel) for
+    // launches of at least kAutoBigTiles 64-tiles, where it runs faster, and
+    // the 64 tile for smaller ones, where it leaves fewer workgroups idle
+    auto outer_tile = [&](i64 tiles64) -> i32 {
+        if (p.big_tile) return p.big_tile;
+        return (precision == SPRGPU_FP32 && tiles64 >= kAutoBigTiles) ? kTileBig : kTileSmall;
+    };
     p.small_f = params.small_f;
     auto M = [&](int k) { return mspar[k - 1]; };
     if (M(1) < 0) return "MSPAR(1) < 0: the symbolic phase did not complete";
@@ -281,6 +289,16 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
                 const i32 g0 = ob / kOuterRank * kOuterRank, gend = g0 + kOuterRank;
                 const bool last_in_group = oend >= gend;
                 auto gemm_launch = [&](i32 stream, auto range) {
+                    // the launch's size in 64-tiles decides its tile
+                    i64 tiles64 = 0;
+                    for (i32 j : big) {
+                        i32 r0, cend;
+                        if (!range(p.nodes[j], r0, cend)) continue;
+                        const GemmDesc g = make_gemm(p.nodes[j], r0, p.nodes[j].F, cend, 0, 1,
+                                                     kTileSmall);
+                        tiles64 += tri_tiles(g.nti, g.ntj);
+                    }
+                    const i32 tile = last_in_group ? outer_tile(tiles64) : kTileSmall;
                     const i64 o = i64(p.gemm_desc.size());
                     i64 ntiles = 0;
                     double fl = 0.0;
@@ -290,16 +308,14 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
                         if (!range(n, r0, cend)) continue;
                         // a group that runs past S ends at S
                         const i32 kn = last_in_group ? std::min(oend, n.S) - g0 : kPanelOuter;
-                        const i32 tile = last_in_group ? p.big_tile : kTileSmall;
                         const GemmDesc g = make_gemm(n, r0, n.F, cend, last_in_group ? g0 : ob, kn, tile);
                         p.gemm_desc.push_back(g);
                         p.gemm_prefix.push_back(ntiles);
                         ntiles += tri_tiles(g.nti, g.ntj);
                         fl += gemm_flops(g);
                     }
-                    add_launch(L_GEMM, l, o, i64(p.gemm_desc.size()) - o, ntiles, fl,
-                               last_in_group ? p.big_tile : kTileSmall, last_in_group ? 1 : 0,
-                               stream);
+                    add_launch(L_GEMM, l, o, i64(p.gemm_desc.size()) - o, ntiles, fl, tile,
+                               last_in_group ? 1 : 0, stream);
                 };
                 if (!last_in_group) {
                     gemm_launch(0, [&](const NodeInfo &n, i32 &r0, i32 &cend) {
@@ -381,16 +397,17 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
                 off = i64(p.gemm_desc.size());
                 ntiles = 0;
                 fl = 0.0;
+                const i32 ptile = p.big_tile ? p.big_tile : kTileSmall;
                 for (i32 j : big) {
                     const NodeInfo &n = p.nodes[j];
                     if (oend >= n.S) continue;
-                    const GemmDesc g = make_gemm(n, oend, n.F, n.S, ob, kPanelOuter, p.big_tile);
+                    const GemmDesc g = make_gemm(n, oend, n.F, n.S, ob, kPanelOuter, ptile);
                     p.gemm_desc.push_back(g);
                     p.gemm_prefix.push_back(ntiles);
                     ntiles += tri_tiles(g.nti, g.ntj);
                     fl += gemm_flops(g);
                 }
-                add_launch(L_GEMM, l, off, i64(p.gemm_desc.size()) - off, ntiles, fl, p.big_tile, 1);
+                add_launch(L_GEMM, l, off, i64(p.gemm_desc.size()) - off, ntiles, fl, ptile, 1);
             }
         }
         // Schur complement of every blocked front of this level: with the
@@ -400,23 +417,37 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
         off = i64(p.gemm_desc.size());
         i64 ntiles = 0;
         double fl = 0.0;
-        for (i32 j : big) {
-            const NodeInfo &n = p.nodes[j];
-            if (n.F == n.S) continue;
-            i32 k0 = 0;
+        auto schur_k0 = [&](const NodeInfo &n, i32 &k0) {
+            k0 = 0;
+            if (n.F == n.S) return false;
             if (p.params.block_factor) {
                 const i32 ob = (n.S - 1) / kPanelOuter * kPanelOuter;
                 const i32 g0 = ob / kOuterRank * kOuterRank;
-                if (ob + kPanelOuter >= g0 + kOuterRank) continue;  // ended with a group update
+                if (ob + kPanelOuter >= g0 + kOuterRank) return false;  // ended with a group update
                 k0 = g0;
             }
-            const GemmDesc g = make_gemm(n, n.S, n.F, n.F, k0, n.S - k0, p.big_tile);
+            return true;
+        };
+        i64 stiles64 = 0;
+        for (i32 j : big) {
+            i32 k0;
+            if (!schur_k0(p.nodes[j], k0)) continue;
+            const NodeInfo &n = p.nodes[j];
+            const GemmDesc g = make_gemm(n, n.S, n.F, n.F, k0, 1, kTileSmall);
+            stiles64 += tri_tiles(g.nti, g.ntj);
+        }
+        const i32 stile = outer_tile(stiles64);
+        for (i32 j : big) {
+            const NodeInfo &n = p.nodes[j];
+            i32 k0;
+            if (!schur_k0(n, k0)) continue;
+            const GemmDesc g = make_gemm(n, n.S, n.F, n.F, k0, n.S - k0, stile);
             p.gemm_desc.push_back(g);
             p.gemm_prefix.push_back(ntiles);
             ntiles += tri_tiles(g.nti, g.ntj);
             fl += gemm_flops(g);
         }
-        add_launch(L_GEMM, l, off, i64(p.gemm_desc.size()) - off, ntiles, fl, p.big_tile, 1);
+        add_launch(L_GEMM, l, off, i64(p.gemm_desc.size()) - off, ntiles, fl, stile, 1);
     }
 
     // ----------------------------------------------------------------------

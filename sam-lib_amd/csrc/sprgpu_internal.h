@@ -19,6 +19,8 @@ using i32 = int32_t;
 constexpr int kTileSmall = 64;   // MFMA update tile (the default for every update)
 constexpr int kTileBig = 128;    // larger tile, selectable for the rank-kOuterRank and Schur
                                  // updates (sprgpu_set_plan_params, the autotuner)
+constexpr int kAutoBigTiles = 4096;  // FP32 update launches of at least this many 64-tiles
+                                     // take the 128 tile (big_tile 0)
 constexpr int kGemmAlign = 4;    // MFMA update tiles start on rows that are multiples of this
 constexpr int kFrontLdAlign = 16; // front leading dimensions are multiples of this (64 B in FP32)
 constexpr int kPanelNB = 32;     // columns per panel step of the blocked front factor
@@ -43,8 +45,9 @@ constexpr int kInvLayout = 3 * kSolveNB * kSolveNB / 4;
 // A wide supernode in the dataflow solves: its diagonal part is split into
 // nblk blocks of kSolveNB columns (one workgroup each, ordered so that every
 // workgroup only waits for lower-indexed ones) and its F-S rectangle rows
-// into nrect chunks of kSolveNB rows.  flag0 is its first ready flag, pbase
-// the offset of its nblk*nrect*kSolveNB backward partial sums.
+// into nrect chunks of kSolveNB rows.  flag0 is the index of its first block
+// among the blocks of all wide supernodes (block inverses, result slots),
+// pbase the offset of its nblk*nrect*kSolveNB backward partial sums.
 struct WideDesc {
     i32 node, nblk, nrect, pad;
     i64 flag0, pbase;
@@ -114,7 +117,8 @@ struct Launch {
 struct PlanParams {
     i32 small_f = kSmallF;    // fronts with F <= small_f are assembled and factored in LDS
     i32 small_waves = 1;      // such fronts per workgroup (one wavefront each)
-    i32 big_tile = 0;         // MFMA tile of rank-kPanelOuter/Schur updates (0: by precision)
+    i32 big_tile = 0;         // MFMA tile of rank-kOuterRank/Schur updates (0: per launch,
+                              // 128 for large FP32 launches, else 64)
     i32 narrow_threads = 0;   // workgroup size of the narrow solves (0: by front size)
     i32 block_factor = 1;     // blocked fronts: 1 = outer-block kernels, 0 = panel steps
 };
@@ -124,7 +128,8 @@ struct Plan {
     i64 neq = 0, nsuper = 0, nofsub = 0, nzerol = 0, mfront = 0, maxsup = 0;
     i64 pool_elems = 0, vwork_elems = 0;
     i32 nlevels = 0;
-    i32 big_tile = 0;               // MFMA tile of the rank-kPanelOuter and Schur updates
+    i32 big_tile = 0;               // configured tile of the rank-kOuterRank and Schur updates
+                                    // (0: per launch, Launch::tile)
     i32 small_f = 0;                // fronts with F <= small_f are factored in LDS (<= kSmallF)
     std::vector<NodeInfo> nodes;
     std::vector<i32> children;
@@ -153,7 +158,7 @@ struct Plan {
     std::vector<i32> wide_ptr;                        // nlevels+1
     std::vector<i64> wide_fwd_wgs, wide_bwd_wgs, wide_rect_wgs;  // workgroups per level
     std::vector<i64> wide_frect_wgs;
-    i64 nflags = 0;                                   // ready flags per right-hand side
+    i64 nflags = 0;                                   // blocks of all wide supernodes
     i64 npart = 0;                                    // backward partial sums per RHS
     // forward assembly as a gather: entry i of the solve workspace (node
     // n's local row i - n.voff) receives vwork[upd_src[k]] for k in
@@ -224,11 +229,8 @@ struct DeviceBuffers {
     i64 *wide_frect_prefix = nullptr;
     i64 *upd_ptr = nullptr;
     i32 *upd_src = nullptr;
-    unsigned *flags = nullptr;  // ready flags (nflags * nrhs), compared with an epoch
-    i64 flags_cap = 0;
     double *partials = nullptr; // backward partial sums (npart * nrhs)
     i64 partials_cap = 0;
-    unsigned *epoch = nullptr; // sweep counter the ready flags are compared with
     int *spin_error = nullptr;  // set when a dataflow wait gave up
     unsigned long long *prm_count = nullptr;  // non-zero counts of a product's A (2)
     i32 *col_node = nullptr;    // supernode of each equation (adjoint pattern gradient)

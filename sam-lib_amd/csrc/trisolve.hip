@@ -13,7 +13,7 @@
 // * Wide supernodes are split into kSolveNB-column blocks, one workgroup per
 //   block, plus (forward) one workgroup per kSolveNB rows below the diagonal
 //   part.  A block's workgroup loads the L values of each block it depends
-//   on before waiting for that block's ready flag and applies them as a
+//   on before waiting for that block's result and applies them as a
 //   GEMV (forward: two threads per row; backward, the transposed product:
 //   eight lanes per column, so that loads stay coalesced), then solves its
 //   own 128 x 128 unit-triangular block as one more GEMV with the block's
@@ -43,28 +43,14 @@ static_assert(NB == 128, "two threads per row (column) of a block GEMV, 64 entri
 
 // Block results cross workgroups (and XCDs, whose L2 caches are not kept
 // coherent with each other) as agent-scope atomic stores and loads, which
-// are coherent on their own; the publishing wavefront waits for its stores
-// to complete (s_waitcnt) before it stores the ready flag, and readers load
-// the results only after seeing the flag.  This avoids the agent-scope
-// fences, which write back or invalidate a whole L2.
-
-// One thread waits until *flag reaches epoch (bounded), then the workgroup
-// synchronises.  A wait that gives up sets *err and continues.
-__device__ __forceinline__ void wait_ready(const unsigned *flag, unsigned epoch, int *err)
-{
-    if (threadIdx.x == 0) {
-        unsigned long long spins = 0;
-        while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
-            __builtin_amdgcn_s_sleep(1);
-            if (++spins > kSpinLimit) {
-                atomicExch(err, 1);
-                break;
-            }
-        }
-    }
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
+// are coherent on their own.  Before a sweep every block result slot holds
+// kPending (k_mark_pending); a reader spins on each value it needs until it
+// differs, so a result is handed over by its own store, with no ready flag
+// and no wait for the stores to complete.  kPending is a signalling NaN:
+// every arithmetic result is a number or a quiet NaN, so no result equals
+// it.  This avoids the agent-scope fences, which write back or invalidate a
+// whole L2.
+constexpr unsigned long long kPending = 0xFFF7A5A5A5A5A5A5ull;
 
 __device__ __forceinline__ void store_shared(double *p, double x)
 {
@@ -75,22 +61,52 @@ __device__ __forceinline__ double load_shared(const double *p)
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Called by every lane of the wavefront that stored the block (store_shared).
-__device__ __forceinline__ void publish_ready(unsigned *flag, unsigned epoch)
+// The value at p once its writer has stored it, given a first look v
+// (bounded: a wait that gives up sets *err and returns 0).
+__device__ __forceinline__ double await_value(double v, const double *p, int *err)
 {
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_waitcnt(0);  // vmcnt = expcnt = lgkmcnt = 0: the stores have completed
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    if ((threadIdx.x & 63) == 0)
-        __hip_atomic_store(flag, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long spins = 0;
+    while (__builtin_bit_cast(unsigned long long, v) == kPending) {
+        __builtin_amdgcn_s_sleep(1);
+        if (++spins > kSpinLimit) {
+            atomicExch(err, 1);
+            return 0.0;
+        }
+        v = load_shared(p);
+    }
+    return v;
 }
 
-// The epoch of a sweep lives in device memory and is advanced by a kernel
-// before the sweep, so that a captured sequence (hipGraph) replays with a new
-// epoch each time; flags start at zero and only ever take epoch values.
-__global__ void k_epoch_bump(unsigned *epoch)
+// v(lane with h = 0) + v(lane with h = 1) of the lane pair (lane, lane ^ 1),
+// in both lanes, with the same operands in the same order.
+__device__ __forceinline__ double pair_sum(double v, int h)
 {
-    if (threadIdx.x == 0) *epoch += 1;
+    const double o = dpp_move<0xB1>(v);  // quad_perm [1, 0, 3, 2]
+    return h == 0 ? v + o : o + v;
+}
+
+// Every block result slot of the wide supernodes to kPending, for nrhs
+// vectors: workgroup (b, r) takes block b (of all supernodes' blocks, in
+// flag0 order) of vector r.
+__global__ __launch_bounds__(NB) void k_mark_pending(const WideDesc *__restrict__ wide, int nwide,
+                                                     const NodeInfo *__restrict__ nodes,
+                                                     double *vwork, i64 vstride)
+{
+    const i64 blkid = blockIdx.x;
+    int lo = 0, hi = nwide - 1;  // the supernode whose blocks hold blkid
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (wide[mid].flag0 <= blkid)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const WideDesc w = wide[lo];
+    const NodeInfo n = nodes[w.node];
+    const int c = int(blkid - w.flag0) * NB + int(threadIdx.x);
+    if (c < n.S)
+        vwork[i64(blockIdx.y) * vstride + n.voff + c] =
+            __builtin_bit_cast(double, kPending);
 }
 
 // Layouts of a diagonal block's inverse X (zero above the diagonal and past
@@ -323,13 +339,150 @@ __global__ __launch_bounds__(256) void k_bwd_narrow(const i32 *__restrict__ list
 }
 
 // ---------------------------------------------------------------------------
+// wide supernodes: block products
+// ---------------------------------------------------------------------------
+// Rows r..r+3 of an L column as one vector load (at any element alignment:
+// gfx950 loads a dwordx4 at 4-byte alignment).  Callers keep the four rows
+// inside the front: a group that starts at or past F is read at F - 4
+// instead (its values are not used), and one that runs past F starts on a
+// multiple of 4 (then its rows stay below the column's padded length ld) or
+// lies in a column with another one after it in the front.  Every load is
+// issued unconditionally, so all of a block's loads stay in flight together
+// (a second, clamped load path would make the compiler wait for the loads
+// of one path before those of the other).
+template <typename T>
+struct Rows4 {
+    T v[4];
+};
+typedef unsigned long long lanemask;
+template <typename T>
+__device__ __forceinline__ Rows4<T> load_rows4(const T *Lc, int r, int F)
+{
+    typedef T v4u __attribute__((ext_vector_type(4), aligned(sizeof(T))));
+    const v4u v = *reinterpret_cast<const v4u *>(Lc + (r < F ? r : F - 4));
+    Rows4<T> o;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o.v[i] = v[i];
+    return o;
+}
+
+// Forward: y_i -= sum_p L(i, block p) x_p over a chunk of kSolveNB rows
+// (rows r0 + 4g + i of the front).  Every forward kernel takes these
+// products the same way, so results do not depend on how right-hand sides
+// are grouped: thread (g, cg) = (tid / 8, tid % 8) takes rows 4g..4g+3 of
+// the chunk (a vector load per column) and columns 16cg..16cg+15 of the
+// block, sums its 16 products per row in column order, the eight lanes of
+// a row group add their sums (sum8), and lane cg = i < 4, the owner of row
+// 4g + i, adds the block's sum to the row's total, blocks in ascending
+// order.  Staged x is padded by 2 values per 16 (xs_at), so that the eight
+// column groups of a wavefront read distinct LDS banks.
+constexpr int kXsLd = NB + NB / 8;
+__device__ __forceinline__ int xs_at(int c) { return c + 2 * (c >> 4); }
+
+// (Rows past F give values that only reach rows past F, which are not
+// stored.)
+template <typename T>
+__device__ __forceinline__ void fwd_load_block(Rows4<T> (&dst)[16], const T *L, i64 ld, int F,
+                                               int S, int r, int cg, int p)
+{
+    const int c0 = p * NB, nc = min(NB, S - c0);
+#pragma unroll
+    for (int c = 0; c < 16; ++c) dst[c] = load_rows4(L + i64(c0 + min(16 * cg + c, nc - 1)) * ld, r, F);
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void fwd_block_sums(const Rows4<T> (&lv)[16],
+                                               const double (*__restrict__ xb)[kXsLd], int cg,
+                                               double (&tot)[R])
+{
+    constexpr int RS = R < 8 ? R : 8;  // right-hand sides per pass
+#pragma unroll
+    for (int k0 = 0; k0 < R; k0 += RS) {
+        double part[4][RS];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int k = 0; k < RS; ++k) part[i][k] = 0.0;
+#pragma unroll
+        for (int c = 0; c < 16; ++c)
+#pragma unroll
+            for (int k = 0; k < RS; ++k) {
+                const double xv = xb[k0 + k][xs_at(16 * cg + c)];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) part[i][k] += double(lv[c].v[i]) * xv;
+            }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int k = 0; k < RS; ++k) {
+                const double v = sum8(part[i][k]);
+                if (cg == i) tot[k0 + k] += v;
+            }
+    }
+}
+
+// Backward: y_c -= sum_p L(block p, c)^T x_p over the columns c of block
+// blk: thread (cq, rr) = (tid / 8, tid % 8) takes columns 4cq..4cq+3 and
+// rows 4rr + 32m + i (m, i < 4) of block p (a vector load per column and
+// m: the eight lanes of a column read 128 consecutive bytes), sums its 16
+// products per column in row order, the eight lanes add their sums (sum8),
+// and lane rr = j < 4, the owner of column 4cq + j, adds the block's sum to
+// the column's total, blocks in descending order.
+// Rows past F are summed with x = 0 but may hold anything (the front's
+// padding rows are never written), so a wavefront that reaches F zeroes
+// them first (a scalar branch around arithmetic only).
+template <typename T>
+__device__ __forceinline__ void bwd_load_block(Rows4<T> (&dst)[16], const T *L, i64 ld, int F,
+                                               int c0, int nc, int cq, int rr, int p)
+{
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const T *Lc = L + i64(c0 + min(4 * cq + j, nc - 1)) * ld;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) dst[4 * j + m] = load_rows4(Lc, p * NB + 4 * rr + 32 * m, F);
+    }
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void bwd_block_sums(Rows4<T> (&lv)[16],
+                                               const double (*__restrict__ xb)[NB], int rr,
+                                               int F, int p, double (&tot)[R])
+{
+    if (__builtin_amdgcn_ballot_w64(p * NB + 4 * rr + 99 >= F) != lanemask(0)) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (p * NB + 4 * rr + 32 * m + i >= F) lv[4 * j + m].v[i] = T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            double part = 0.0;
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    part += double(lv[4 * j + m].v[i]) * xb[k][4 * rr + 32 * m + i];
+            const double v = sum8(part);
+            if (rr == j) tot[k] += v;
+        }
+}
+
+// ---------------------------------------------------------------------------
 // wide supernodes, forward
 // ---------------------------------------------------------------------------
 // Workgroup local < nblk solves diagonal block local after applying blocks
 // 0..local-1; workgroups nblk.. compute the update vector of kSolveNB rows
 // below the diagonal part, applying every block as it finishes.  As in the
-// narrow kernels, a workgroup serves R right-hand sides; its ready flags are
-// those of its group of right-hand sides.
+// narrow kernels, a workgroup serves R right-hand sides.  Each step of the
+// block loop costs one barrier; the diagonal block costs one more (its
+// right-hand side, complete, to LDS for the product with its inverse,
+// thread (ri, h) = (tid / 2, tid % 2) taking row ri and column half h, the
+// halves added with one DPP exchange).
 template <typename T, int R>
 __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ wide, int nwide,
                                                   const i64 *__restrict__ prefix,
@@ -339,64 +492,49 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
                                                   const i32 *__restrict__ upd_src,
                                                   const T *__restrict__ pool,
                                                   const double *__restrict__ dvec, double *b,
-                                                  i64 ldb, double *vwork, i64 vstride,
-                                                  unsigned *flags, i64 nflags,
-                                                  const unsigned *epoch_ptr, int *err, int nrhs,
-                                                  const T *__restrict__ dinv)
+                                                  i64 ldb, double *vwork, i64 vstride, int *err,
+                                                  int nrhs, const T *__restrict__ dinv)
 {
-    __shared__ double xs[R][NB];
+    __shared__ double xs[4][R][kXsLd];
     __shared__ double ys[R][NB];
-    __shared__ double part[2][R][NB];
-    __shared__ T Lf[kInvLayout];
-    const unsigned epoch = *epoch_ptr;  // L11^-1, forward layout
+    __shared__ T Lf[kInvLayout];  // L11^-1, forward layout
     const int nrb = (nrhs + R - 1) / R;
     const i64 bid = blockIdx.x / nrb;
-    const int rg = int(blockIdx.x % nrb), rhs0 = rg * R;
+    const int rhs0 = int(blockIdx.x % nrb) * R;
     const int q = find_desc(prefix, nwide, bid);
     const WideDesc w = wide[q];
     const int local = int(bid - prefix[q]);
     const NodeInfo n = nodes[w.node];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x;
     const int S = n.S, F = n.F;
     const i32 *rows = lindx + n.lxoff;
     const T *L = pool + n.foff;
     const i64 ld = n.ld;
     const i64 *up = upd_ptr + n.voff;
-    unsigned *fl = flags + rg * nflags + w.flag0;
     i64 vo[R];  // this group's vectors of the supernode (the last valid one past nrhs)
 #pragma unroll
     for (int k = 0; k < R; ++k) vo[k] = i64(min(rhs0 + k, nrhs - 1)) * vstride + n.voff;
     const bool diag = local < w.nblk;
     const int r0 = diag ? local * NB : S + (local - w.nblk) * NB;
     const int nr = min(NB, (diag ? S : F) - r0);
-    const int ri = tid & (NB - 1), h = tid >> 7;
-    const int row = r0 + ri;
-    const bool rv = ri < nr;
-    // right-hand side of my rows: B (own columns) plus the children's updates
-    if (h == 0) {
-        const i64 k0 = rv ? up[row] : 0, k1 = rv ? up[row + 1] : 0;
-        const i32 gi = (rv && row < S) ? rows[row] : 0;
+    const int cg = tid & 7, g = tid >> 3;
+    // the row I own (lanes cg < 4): B (own columns) plus the children's updates
+    const int orow = 4 * g + cg;
+    const bool owner = cg < 4 && orow < nr;
+    double y[R], tot[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) y[k] = tot[k] = 0.0;
+    if (owner) {
+        const int row = r0 + orow;
+        const i64 k0 = up[row], k1 = up[row + 1];
+        const i32 gi = (row < S) ? rows[row] : 0;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const i64 rh = min(rhs0 + k, nrhs - 1);
-            double y = 0.0;
-            if (rv) {
-                y = (row < S) ? b[rh * ldb + gi] : 0.0;
-                const double *vw = vwork + rh * vstride;
-                for (i64 j = k0; j < k1; ++j) y += vw[upd_src[j]];
-            }
-            ys[k][ri] = y;
-        }
-    }
-    // destinations and pivots of the two block rows wave 0 writes at the end
-    i32 dst[2] = {0, 0};
-    double dpiv[2] = {1.0, 1.0};
-    if (diag && wave == 0) {
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const int i = min(lane + 64 * k, nr - 1);
-            dst[k] = rows[r0 + i];
-            dpiv[k] = dvec[n.first + r0 + i];
+            double v = (row < S) ? b[rh * ldb + gi] : 0.0;
+            const double *vw = vwork + rh * vstride;
+            for (i64 j = k0; j < k1; ++j) v += vw[upd_src[j]];
+            y[k] = v;
         }
     }
     if (diag) {
@@ -404,101 +542,107 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
         for (int idx = tid; idx < kInvLayout; idx += 256) Lf[idx] = inv[idx];
     }
     const int nprev = diag ? local : w.nblk;
-    double acc[R];
+    // Blocks are taken two at a time while both are available ahead of the
+    // last one (with one right-hand side; a group takes one at a time, two
+    // blocks' L values would not fit in registers): the looks at their x and
+    // the loads of their L values are issued together, so that a step costs
+    // one memory round trip and one barrier.  The last block (on the chain's
+    // critical path for a diagonal block) goes alone.  The staged x
+    // alternate between two pairs of LDS buffers, so no barrier is needed
+    // before they are rewritten.
+    constexpr int PB = R == 1 ? 2 : 1;  // blocks per step
+    Rows4<T> lv[PB][16];
+    int it = 0;
+    auto step = [&](int p, int nb) {
+        double xv[PB][R];
 #pragma unroll
-    for (int k = 0; k < R; ++k) acc[k] = 0.0;
-    for (int p = 0; p < nprev; ++p) {
-        const int c0 = p * NB;
-        const int nc = min(NB, S - c0);
-        // loads are unconditional and unmasked so that they all stay in
-        // flight: rows past the block read row r0 (their results are not
-        // stored) and columns past the block read its last column (xs is
-        // zero there)
-        T lv[NH];
-        const T *Lr = L + (rv ? row : r0) + i64(c0) * ld;
+        for (int u = 0; u < PB; ++u) {
+            const int c0 = (p + u) * NB, nc = min(NB, S - c0);
 #pragma unroll
-        for (int c = 0; c < NH; ++c) lv[c] = Lr[i64(min(h * NH + c, nc - 1)) * ld];
-        wait_ready(fl + p, epoch, err);
+            for (int k = 0; k < R; ++k)
+                xv[u][k] = (u < nb && tid < nc) ? load_shared(vwork + vo[k] + c0 + tid) : 0.0;
+        }
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+#pragma unroll
+        for (int u = 0; u < PB; ++u)
+            fwd_load_block(lv[u], L, ld, F, S, r0 + 4 * g, cg, min(p + u, nprev - 1));
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
         if (tid < NB) {
 #pragma unroll
-            for (int k = 0; k < R; ++k) xs[k][tid] = (tid < nc) ? load_shared(vwork + vo[k] + c0 + tid) : 0.0;
+            for (int u = 0; u < PB; ++u) {
+                const int c0 = (p + u) * NB, nc = min(NB, S - c0);
+#pragma unroll
+                for (int k = 0; k < R; ++k)
+                    xs[2 * (it & 1) + u][k][xs_at(tid)] =
+                        (u < nb && tid < nc) ? await_value(xv[u][k], vwork + vo[k] + c0 + tid, err) : 0.0;
+            }
         }
         __syncthreads();
 #pragma unroll
-        for (int c = 0; c < NH; ++c)
-#pragma unroll
-            for (int k = 0; k < R; ++k) acc[k] += double(lv[c]) * xs[k][h * NH + c];
-        __syncthreads();
-    }
-#pragma unroll
-    for (int k = 0; k < R; ++k) part[h][k][ri] = acc[k];
-    __syncthreads();
+        for (int u = 0; u < PB; ++u)
+            if (u < nb) fwd_block_sums<T, R>(lv[u], xs[2 * (it & 1) + u], cg, tot);
+        ++it;
+    };
+    int p = 0;
+    for (; p + PB < nprev; p += PB) step(p, PB);
+    for (; p < nprev; ++p) step(p, 1);
     if (!diag) {
-        if (h == 0 && rv) {  // update for the parent
+        if (owner) {  // update for the parent
 #pragma unroll
             for (int k = 0; k < R; ++k)
-                if (rhs0 + k < nrhs) vwork[vo[k] + row] = ys[k][ri] - (part[0][k][ri] + part[1][k][ri]);
+                if (rhs0 + k < nrhs) vwork[vo[k] + r0 + orow] = y[k] - tot[k];
         }
         return;
     }
     // Diagonal block: x = L11^-1 y with the block's inverse in LDS, thread
-    // (ri, h) summing columns [64h, 64h + 64) of row ri (h is uniform in a
-    // wavefront: the column offsets are scalar arithmetic).
-    const int hs = __builtin_amdgcn_readfirstlane(h);
-    if (tid < NB) {
+    // (ri, h) summing columns [64h, 64h + 64) of row ri; in the forward
+    // layout, columns 64..127 of rows 0..63 are zero and not stored.
+    if (cg < 4) {
 #pragma unroll
-        for (int k = 0; k < R; ++k)
-            ys[k][tid] = (tid < nr) ? ys[k][tid] - (part[0][k][tid] + part[1][k][tid]) : 0.0;
+        for (int k = 0; k < R; ++k) ys[k][orow] = owner ? y[k] - tot[k] : 0.0;
+    }
+    const int ri = tid >> 1, h = tid & 1;
+    i32 dst = 0;
+    double dpiv = 1.0;
+    if (h == 0 && ri < nr) {
+        dst = rows[r0 + ri];
+        dpiv = dvec[n.first + r0 + ri];
     }
     __syncthreads();
-    // forward layout: columns [64hs, 64hs + 64) of row ri, stride cs
-    const bool above = hs == 1 && wave == 2;  // rows 0..63 of columns 64..127: zero
-    const T *Lrow = Lf + (hs == 0 ? ri : NB * NB / 2 + (ri - NH));
-    const int cs = hs == 0 ? NB : NH;
+    const bool above = h == 1 && ri < NH;
+    const T *Lrow = Lf + (h == 0 ? ri : NB * NB / 2 + (ri - NH));
+    const int cs = h == 0 ? NB : NH;
+    double x[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         double a0 = 0.0, a1 = 0.0;
         if (!above) {
 #pragma unroll 16
             for (int c = 0; c < NH; c += 2) {
-                a0 += double(Lrow[c * cs]) * ys[k][hs * NH + c];
-                a1 += double(Lrow[(c + 1) * cs]) * ys[k][hs * NH + c + 1];
+                a0 += double(Lrow[c * cs]) * ys[k][h * NH + c];
+                a1 += double(Lrow[(c + 1) * cs]) * ys[k][h * NH + c + 1];
             }
         }
-        part[h][k][ri] = a0 + a1;
+        x[k] = pair_sum(a0 + a1, h);
     }
-    __syncthreads();
-    if (wave == 0) {
-        // results for the blocks that wait on this one first, then B
+    if (h == 0 && ri < nr) {
+        // the result for the blocks that wait on it first, then B
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
-            if (lane + 64 * i < nr) {
+        for (int k = 0; k < R; ++k)
+            if (rhs0 + k < nrhs) store_shared(vwork + vo[k] + r0 + ri, x[k]);
 #pragma unroll
-                for (int k = 0; k < R; ++k)
-                    if (rhs0 + k < nrhs)
-                        store_shared(vwork + vo[k] + r0 + lane + 64 * i,
-                                     part[0][k][lane + 64 * i] + part[1][k][lane + 64 * i]);
-            }
-        publish_ready(fl + local, epoch);
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-            if (lane + 64 * i < nr) {
-#pragma unroll
-                for (int k = 0; k < R; ++k)
-                    if (rhs0 + k < nrhs)
-                        b[i64(rhs0 + k) * ldb + dst[i]] =
-                            (part[0][k][lane + 64 * i] + part[1][k][lane + 64 * i]) / dpiv[i];
-            }
+        for (int k = 0; k < R; ++k)
+            if (rhs0 + k < nrhs) b[i64(rhs0 + k) * ldb + dst] = x[k] / dpiv;
     }
 }
 
 // The rows below the diagonal part of the wide supernodes as a launch of
 // their own, after the supernodes' block chains (k_fwd_wide over the
-// diagonal blocks only): used for groups of right-hand sides, where a wide
-// group (R up to 32) reads L21 fewer times and nothing waits.  Each
-// workgroup takes kSolveNB rows of one supernode and the same operations per
-// right-hand side as the rectangle workgroups of k_fwd_wide, so the update
-// vectors are identical bit for bit.
+// diagonal blocks only): used for groups of right-hand sides (R = 8 or 32),
+// where a group reads L21 once and nothing waits.  Each workgroup takes
+// kSolveNB rows of one supernode with the block products of k_fwd_wide, so
+// the update vectors are identical bit for bit.  The gathered children's
+// updates go to the rows' vector slots first and are read back at the end.
 template <typename T, int R>
 __global__ __launch_bounds__(256) void k_fwd_rect(const WideDesc *__restrict__ wide, int nwide,
                                                   const i64 *__restrict__ prefix,
@@ -508,9 +652,7 @@ __global__ __launch_bounds__(256) void k_fwd_rect(const WideDesc *__restrict__ w
                                                   const T *__restrict__ pool, double *vwork,
                                                   i64 vstride, int nrhs)
 {
-    __shared__ double xs[R][NB];
-    __shared__ double ys[R][NB];
-    __shared__ double part[2][R][NB];
+    __shared__ double xs[2][R][kXsLd];
     const int nrb = (nrhs + R - 1) / R;
     const i64 bid = blockIdx.x / nrb;
     const int rhs0 = int(blockIdx.x % nrb) * R;
@@ -523,157 +665,44 @@ __global__ __launch_bounds__(256) void k_fwd_rect(const WideDesc *__restrict__ w
     const T *L = pool + n.foff;
     const i64 ld = n.ld;
     const i64 *up = upd_ptr + n.voff;
-    i64 vo[R];
-#pragma unroll
-    for (int k = 0; k < R; ++k) vo[k] = i64(min(rhs0 + k, nrhs - 1)) * vstride + n.voff;
     const int r0 = S + chunk * NB;
     const int nr = min(NB, F - r0);
-    const int ri = tid & (NB - 1), h = tid >> 7;
-    const int row = r0 + ri;
-    const bool rv = ri < nr;
-    // the children's updates of my rows (no B entries below the diagonal part)
-    if (h == 0) {
-        const i64 k0 = rv ? up[row] : 0, k1 = rv ? up[row + 1] : 0;
-#pragma unroll
+    const int cg = tid & 7, g = tid >> 3;
+    const int orow = 4 * g + cg;
+    const bool owner = cg < 4 && orow < nr;
+    if (owner) {  // the children's updates of my row (no B entries below the diagonal part)
+        const i64 k0 = up[r0 + orow], k1 = up[r0 + orow + 1];
         for (int k = 0; k < R; ++k) {
-            const double *vw = vwork + i64(min(rhs0 + k, nrhs - 1)) * vstride;
-            double y = 0.0;
-            for (i64 j = k0; j < k1; ++j) y += vw[upd_src[j]];
-            ys[k][ri] = y;
+            if (rhs0 + k >= nrhs) break;
+            double *vw = vwork + i64(rhs0 + k) * vstride;
+            double v = 0.0;
+            for (i64 j = k0; j < k1; ++j) v += vw[upd_src[j]];
+            vw[n.voff + r0 + orow] = v;
         }
     }
-    double acc[R];
+    double tot[R];
 #pragma unroll
-    for (int k = 0; k < R; ++k) acc[k] = 0.0;
+    for (int k = 0; k < R; ++k) tot[k] = 0.0;
+    Rows4<T> lv[16];
     for (int p = 0; p < w.nblk; ++p) {
-        const int c0 = p * NB;
-        const int nc = min(NB, S - c0);
-        // unconditional loads as in k_fwd_wide (rows past the chunk read row
-        // r0, columns past the block its last column: xs is zero there)
-        T lv[NH];
-        const T *Lr = L + (rv ? row : r0) + i64(c0) * ld;
-#pragma unroll
-        for (int c = 0; c < NH; ++c) lv[c] = Lr[i64(min(h * NH + c, nc - 1)) * ld];
-        if (tid < NB) {
-#pragma unroll
-            for (int k = 0; k < R; ++k) xs[k][tid] = (tid < nc) ? vwork[vo[k] + c0 + tid] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int c = 0; c < NH; ++c)
-#pragma unroll
-            for (int k = 0; k < R; ++k) acc[k] += double(lv[c]) * xs[k][h * NH + c];
-        __syncthreads();
-    }
-#pragma unroll
-    for (int k = 0; k < R; ++k) part[h][k][ri] = acc[k];
-    __syncthreads();
-    if (h == 0 && rv) {  // update for the parent
-#pragma unroll
-        for (int k = 0; k < R; ++k)
-            if (rhs0 + k < nrhs) vwork[vo[k] + row] = ys[k][ri] - (part[0][k][ri] + part[1][k][ri]);
-    }
-}
-
-// k_fwd_rect for groups of kRectWide right-hand sides, register-blocked:
-// thread (h, rg, ks) takes rows rg + 32i (i < 4) and right-hand sides
-// 8ks..8ks+7 of column half h, so that each x value loaded from LDS feeds
-// four rows.  Every (row, right-hand side) sum runs over the same columns in
-// the same order as in k_fwd_rect and k_fwd_wide, so the update vectors are
-// identical bit for bit.  The gathered children's updates go to the rows'
-// vector slots first and are read back at the end (LDS: x, then the half
-// sums, 64 KB).
-constexpr int kRectWide = 32;
-template <typename T>
-__global__ __launch_bounds__(256) void k_fwd_rect_wide(const WideDesc *__restrict__ wide, int nwide,
-                                                       const i64 *__restrict__ prefix,
-                                                       const NodeInfo *__restrict__ nodes,
-                                                       const i64 *__restrict__ upd_ptr,
-                                                       const i32 *__restrict__ upd_src,
-                                                       const T *__restrict__ pool, double *vwork,
-                                                       i64 vstride, int nrhs)
-{
-    constexpr int R = kRectWide, KS = 8, CH = 16;  // RHS per thread, columns per load chunk
-    __shared__ double sbuf[2 * R * NB];             // xs[R][NB] in the loop, then part[2][R][NB]
-    double(*xs)[NB] = reinterpret_cast<double(*)[NB]>(sbuf);
-    double(*part)[R][NB] = reinterpret_cast<double(*)[R][NB]>(sbuf);
-    const int nrb = (nrhs + R - 1) / R;
-    const i64 bid = blockIdx.x / nrb;
-    const int rhs0 = int(blockIdx.x % nrb) * R;
-    const int q = find_desc(prefix, nwide, bid);
-    const WideDesc w = wide[q];
-    const int chunk = int(bid - prefix[q]);
-    const NodeInfo n = nodes[w.node];
-    const int tid = threadIdx.x;
-    const int S = n.S, F = n.F;
-    const T *L = pool + n.foff;
-    const i64 ld = n.ld;
-    const i64 *up = upd_ptr + n.voff;
-    const int r0 = S + chunk * NB;
-    const int nr = min(NB, F - r0);
-    // the children's updates of row ri, into its slot of every vector
-    {
-        const int ri = tid & (NB - 1);
-        if (tid < NB && ri < nr) {
-            const i64 k0 = up[r0 + ri], k1 = up[r0 + ri + 1];
-            for (int k = 0; k < R; ++k) {
-                if (rhs0 + k >= nrhs) break;
-                double *vw = vwork + i64(rhs0 + k) * vstride;
-                double y = 0.0;
-                for (i64 j = k0; j < k1; ++j) y += vw[upd_src[j]];
-                vw[n.voff + r0 + ri] = y;
-            }
-        }
-    }
-    const int h = tid >> 7, rg = tid & 31, ks = (tid >> 5) & 3;
-    i64 rofs[4];  // rows past the chunk read row r0 (their sums are not stored)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) rofs[i] = (rg + 32 * i < nr) ? r0 + rg + 32 * i : r0;
-    double acc[4][KS];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int k = 0; k < KS; ++k) acc[i][k] = 0.0;
-    for (int p = 0; p < w.nblk; ++p) {
-        const int c0 = p * NB;
-        const int nc = min(NB, S - c0);
+        const int c0 = p * NB, nc = min(NB, S - c0);
+        fwd_load_block(lv, L, ld, F, S, r0 + 4 * g, cg, p);
+        double(*xb)[kXsLd] = xs[p & 1];
         if (tid < NB) {
             for (int k = 0; k < R; ++k) {
                 const i64 rh = min(rhs0 + k, nrhs - 1);
-                xs[k][tid] = (tid < nc) ? vwork[rh * vstride + n.voff + c0 + tid] : 0.0;
+                xb[k][xs_at(tid)] = (tid < nc) ? vwork[rh * vstride + n.voff + c0 + tid] : 0.0;
             }
         }
         __syncthreads();
-        for (int cc = 0; cc < NH / CH; ++cc) {
-            const int cb = h * NH + cc * CH;  // first column of the chunk (in the block)
-            T lv[4][CH];
-#pragma unroll
-            for (int c = 0; c < CH; ++c) {
-                const i64 co = i64(c0 + min(cb + c, nc - 1)) * ld;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) lv[i][c] = L[rofs[i] + co];
-            }
-#pragma unroll
-            for (int c = 0; c < CH; ++c)
-#pragma unroll
-                for (int k = 0; k < KS; ++k) {
-                    const double xv = xs[KS * ks + k][cb + c];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) acc[i][k] += double(lv[i][c]) * xv;
-                }
-        }
-        __syncthreads();
+        fwd_block_sums<T, R>(lv, xb, cg, tot);
     }
+    if (owner) {  // update for the parent
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int k = 0; k < KS; ++k) part[h][KS * ks + k][rg + 32 * i] = acc[i][k];
-    __syncthreads();
-    if (tid < NB && tid < nr) {  // update for the parent
         for (int k = 0; k < R; ++k) {
             if (rhs0 + k >= nrhs) break;
-            double *slot = vwork + i64(rhs0 + k) * vstride + n.voff + r0 + tid;
-            *slot = *slot - (part[0][k][tid] + part[1][k][tid]);
+            double *slot = vwork + i64(rhs0 + k) * vstride + n.voff + r0 + orow;
+            *slot = *slot - tot[k];
         }
     }
 }
@@ -760,8 +789,12 @@ __global__ __launch_bounds__(256) void k_bwd_rect(const WideDesc *__restrict__ w
 }
 
 // Workgroup local handles block nblk-1-local: y = B - partial sums of the
-// rows below, minus the later blocks' products as they finish, then
-// L11^T x = y on the block by wave 0 (two columns per lane).
+// rows below, minus the later blocks' products as they finish (bwd_block_sums),
+// then x = L11^-T y on the block with its inverse.  Each step of the block
+// loop costs one barrier (as in k_fwd_wide); the diagonal block costs one
+// more.  For the product with the inverse, thread (ci, h) = (tid / 2,
+// tid % 2) takes column ci and row half h, and the two halves (adjacent
+// lanes) are added with one DPP exchange.
 template <typename T, int R>
 __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ wide, int nwide,
                                                   const i64 *__restrict__ prefix,
@@ -770,145 +803,117 @@ __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ w
                                                   const T *__restrict__ pool, double *b, i64 ldb,
                                                   double *vwork, i64 vstride,
                                                   const double *__restrict__ partials, i64 npart,
-                                                  unsigned *flags, i64 nflags,
-                                                  const unsigned *epoch_ptr, int *err, int nrhs,
-                                                  const T *__restrict__ dinv)
+                                                  int *err, int nrhs, const T *__restrict__ dinv)
 {
-    __shared__ double xs[R][NB];
+    __shared__ double xs[4][R][NB];
     __shared__ double ys[R][NB];
-    __shared__ double part[2][R][NB];
-    __shared__ T Lb[kInvLayout];
-    const unsigned epoch = *epoch_ptr;  // L11^-1, backward layout
+    __shared__ T Lb[kInvLayout];  // L11^-1, backward layout
     const int nrb = (nrhs + R - 1) / R;
     const i64 bid = blockIdx.x / nrb;
-    const int rg = int(blockIdx.x % nrb), rhs0 = rg * R;
+    const int rhs0 = int(blockIdx.x % nrb) * R;
     const int q = find_desc(prefix, nwide, bid);
     const WideDesc w = wide[q];
     const int blk = w.nblk - 1 - int(bid - prefix[q]);
     const NodeInfo n = nodes[w.node];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int S = n.S;
+    const int tid = threadIdx.x;
+    const int S = n.S, F = n.F;
     const i32 *rows = lindx + n.lxoff;
     const T *L = pool + n.foff;
     const i64 ld = n.ld;
-    unsigned *fl = flags + rg * nflags + w.flag0;
     i64 vo[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) vo[k] = i64(min(rhs0 + k, nrhs - 1)) * vstride + n.voff;
     const int c0 = blk * NB, nc = min(NB, S - c0);
     const T *inv = dinv + (w.flag0 + blk) * i64(2 * kInvLayout) + kInvLayout;
     for (int idx = tid; idx < kInvLayout; idx += 256) Lb[idx] = inv[idx];
-    i32 dst[2] = {0, 0};
-    if (wave == 0) {
-#pragma unroll
-        for (int k = 0; k < 2; ++k) dst[k] = rows[c0 + min(lane + 64 * k, nc - 1)];
-    }
-    const int ci = tid & (NB - 1), h = tid >> 7;
-    // products with the rows below the diagonal part: the two halves of the
-    // workgroup sum the even and the odd row chunks' partials
+    const int ci = tid >> 1, h = tid & 1;
+    const bool cv = ci < nc;
+    const i32 dst = rows[c0 + min(ci, nc - 1)];
+    // products with the rows below the diagonal part: the two lanes of a
+    // column sum the even and the odd row chunks' partials
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const double *ps = partials + i64(min(rhs0 + k, nrhs - 1)) * npart + w.pbase +
                            i64(blk) * w.nrect * NB + min(ci, nc - 1);
-        part[h][k][ci] = strided_sum(ps, NB, h, 2, w.nrect);
+        const double sp = pair_sum(strided_sum(ps, NB, h, 2, w.nrect), h);
+        if (h == 0)
+            ys[k][ci] = cv ? b[i64(min(rhs0 + k, nrhs - 1)) * ldb + dst] - sp : 0.0;
     }
-    __syncthreads();
-    if (tid < NB) {
-        const i32 gi = rows[c0 + min(tid, nc - 1)];
+    // products with the later blocks (bwd_block_sums), into the totals of the
+    // columns' owner lanes
+    const int rr = tid & 7, cq = tid >> 3;
+    double tot[R];
 #pragma unroll
-        for (int k = 0; k < R; ++k)
-            ys[k][tid] = (tid < nc)
-                             ? b[i64(min(rhs0 + k, nrhs - 1)) * ldb + gi] - (part[0][k][tid] + part[1][k][tid])
-                             : 0.0;
-    }
-    __syncthreads();  // ys complete before the column sums below update it
-    // Products with the later blocks, L(p, blk)^T x_p: the thread sums
-    // columns gc + 8j (j < 4) over rows 8q + gs (q < 16) of every block, so
-    // that a load instruction reads 8 rows of 8 columns (8 cache lines where
-    // a lane per column would touch 64 and be bound by address processing);
-    // the eight lanes of a column are added at the end (sum8).
-    const int gs = lane & 7, gc = 32 * wave + (lane >> 3);
-    double acc[4][R];
+    for (int k = 0; k < R; ++k) tot[k] = 0.0;
+    // Steps as in k_fwd_wide, blocks from the last one down.
+    constexpr int PB = R == 1 ? 2 : 1;  // blocks per step
+    Rows4<T> lv[PB][16];
+    int it = 0;
+    auto step = [&](int p, int nb) {  // blocks p, p - 1, ..., p - nb + 1
+        double xv[PB][R];
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int k = 0; k < R; ++k) acc[j][k] = 0.0;
-    for (int p = w.nblk - 1; p > blk; --p) {
-        const int rp0 = p * NB, np = min(NB, S - rp0);
-        // unconditional, unmasked loads (see k_fwd_wide): columns past the
-        // block read its last column (not stored), rows past block p read
-        // its last row (xs is zero there)
-        T lv[4][NB / 8];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const T *Lc = L + rp0 + i64(c0 + min(gc + 8 * j, nc - 1)) * ld;
-#pragma unroll
-            for (int q = 0; q < NB / 8; ++q) lv[j][q] = Lc[min(8 * q + gs, np - 1)];
-        }
-        wait_ready(fl + p, epoch, err);
-        if (tid < NB) {
+        for (int u = 0; u < PB; ++u) {
+            const int rp0 = (p - u) * NB, np = min(NB, S - rp0);
 #pragma unroll
             for (int k = 0; k < R; ++k)
-                xs[k][tid] = (tid < np) ? load_shared(vwork + vo[k] + rp0 + tid) : 0.0;
+                xv[u][k] = (u < nb && tid < np) ? load_shared(vwork + vo[k] + rp0 + tid) : 0.0;
+        }
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+#pragma unroll
+        for (int u = 0; u < PB; ++u)
+            bwd_load_block(lv[u], L, ld, F, c0, nc, cq, rr, max(p - u, blk + 1));
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        if (tid < NB) {
+#pragma unroll
+            for (int u = 0; u < PB; ++u) {
+                const int rp0 = (p - u) * NB, np = min(NB, S - rp0);
+#pragma unroll
+                for (int k = 0; k < R; ++k)
+                    xs[2 * (it & 1) + u][k][tid] =
+                        (u < nb && tid < np) ? await_value(xv[u][k], vwork + vo[k] + rp0 + tid, err) : 0.0;
+            }
         }
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int u = 0; u < PB; ++u)
+            if (u < nb) bwd_block_sums<T, R>(lv[u], xs[2 * (it & 1) + u], rr, F, p - u, tot);
+        ++it;
+    };
+    int p = w.nblk - 1;
+    for (; p - PB > blk; p -= PB) step(p, PB);
+    for (; p > blk; --p) step(p, 1);
+    __syncthreads();  // ys complete (its initial values) before the owners update it
+    if (rr < 4 && 4 * cq + rr < nc) {
 #pragma unroll
-            for (int q = 0; q < NB / 8; ++q)
-#pragma unroll
-                for (int k = 0; k < R; ++k) acc[j][k] += double(lv[j][q]) * xs[k][8 * q + gs];
-        __syncthreads();
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-#pragma unroll
-        for (int k = 0; k < R; ++k) {
-            const double v = sum8(acc[j][k]);
-            if (gs == 0 && gc + 8 * j < nc) ys[k][gc + 8 * j] -= v;
-        }
+        for (int k = 0; k < R; ++k) ys[k][4 * cq + rr] -= tot[k];
     }
     __syncthreads();
     // Diagonal block: x = L11^-T y with the block's inverse in LDS, thread
-    // (ci, h) summing rows [64h, 64h + 64) of column ci (h is uniform in a
-    // wavefront: the row offsets are scalar arithmetic).
-    const int hs = __builtin_amdgcn_readfirstlane(h);
-    // backward layout: rows [64hs, 64hs + 64) of column ci, stride rs
-    const bool above = hs == 0 && wave == 1;  // columns 64..127 of rows 0..63: zero
-    const T *Lcol = Lb + (hs == 0 ? ci : NB * NB / 4 + ci);
-    const int rs = hs == 0 ? NH : NB;
+    // (ci, h) summing rows [64h, 64h + 64) of column ci; in the backward
+    // layout, columns 64..127 of rows 0..63 are zero and not stored.
+    const bool above = h == 0 && ci >= NH;
+    const T *Lcol = Lb + (h == 0 ? ci : NB * NB / 4 + ci);
+    const int rs = h == 0 ? NH : NB;
+    double x[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         double a0 = 0.0, a1 = 0.0;
         if (!above) {
 #pragma unroll 16
             for (int r = 0; r < NH; r += 2) {
-                a0 += double(Lcol[r * rs]) * ys[k][hs * NH + r];
-                a1 += double(Lcol[(r + 1) * rs]) * ys[k][hs * NH + r + 1];
+                a0 += double(Lcol[r * rs]) * ys[k][h * NH + r];
+                a1 += double(Lcol[(r + 1) * rs]) * ys[k][h * NH + r + 1];
             }
         }
-        part[h][k][ci] = a0 + a1;
+        x[k] = pair_sum(a0 + a1, h);
     }
-    __syncthreads();
-    if (wave == 0) {
+    if (h == 0 && cv) {
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
-            if (lane + 64 * i < nc) {
+        for (int k = 0; k < R; ++k)
+            if (rhs0 + k < nrhs) store_shared(vwork + vo[k] + c0 + ci, x[k]);
 #pragma unroll
-                for (int k = 0; k < R; ++k)
-                    if (rhs0 + k < nrhs)
-                        store_shared(vwork + vo[k] + c0 + lane + 64 * i,
-                                     part[0][k][lane + 64 * i] + part[1][k][lane + 64 * i]);
-            }
-        publish_ready(fl + blk, epoch);
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-            if (lane + 64 * i < nc) {
-#pragma unroll
-                for (int k = 0; k < R; ++k)
-                    if (rhs0 + k < nrhs)
-                        b[i64(rhs0 + k) * ldb + dst[i]] = part[0][k][lane + 64 * i] + part[1][k][lane + 64 * i];
-            }
+        for (int k = 0; k < R; ++k)
+            if (rhs0 + k < nrhs) b[i64(rhs0 + k) * ldb + dst] = x[k];
     }
 }
 
@@ -1028,8 +1033,8 @@ static int ensure_solve_buffers(sprgpu_handle *h, i64 nrhs)
     DeviceBuffers &d = h->d;
     const Plan &p = h->plan;
     if (h->noalloc) {  // the buffers are views of the static scratch, sized by the caller
-        if (d.vwork_cap < nrhs || d.flags_cap < std::max<i64>(p.nflags, 1) * nrhs ||
-            d.partials_cap < std::max<i64>(p.npart, 1) * nrhs || !d.epoch || !d.spin_error) {
+        if (d.vwork_cap < nrhs || d.partials_cap < std::max<i64>(p.npart, 1) * nrhs ||
+            !d.spin_error) {
             set_error("solve workspace larger than the scratch view");
             return -2;
         }
@@ -1042,17 +1047,6 @@ static int ensure_solve_buffers(sprgpu_handle *h, i64 nrhs)
         SPRGPU_HIP(hipMalloc(&d.vwork, sizeof(double) * size_t(p.vwork_elems) * size_t(nrhs)));
         h->device_bytes += sizeof(double) * p.vwork_elems * (nrhs - d.vwork_cap);
         d.vwork_cap = nrhs;
-    }
-    const i64 nf = std::max<i64>(p.nflags, 1) * nrhs;
-    if (d.flags_cap < nf) {
-        ++h->buf_gen;
-        if (d.flags) SPRGPU_HIP(hipFree(d.flags));
-        d.flags = nullptr;
-        SPRGPU_HIP(hipMalloc(&d.flags, sizeof(unsigned) * nf));
-        SPRGPU_HIP(hipMemset(d.flags, 0, sizeof(unsigned) * nf));
-        d.flags_cap = nf;
-        if (!d.epoch) SPRGPU_HIP(hipMalloc(&d.epoch, sizeof(unsigned)));
-        SPRGPU_HIP(hipMemset(d.epoch, 0, sizeof(unsigned)));  // every flag is zero again
     }
     const i64 np = std::max<i64>(p.npart, 1) * nrhs;
     if (d.partials_cap < np) {
@@ -1147,8 +1141,7 @@ static void fwd_wide_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs
     const i64 *prefix = fused ? d.wide_fwd_prefix : d.wide_bwd_prefix;  // all blocks, or the chains'
     k_fwd_wide<T, R><<<unsigned(wgs) * groups, 256, 0, h->stream>>>(
         d.wide + w0, nw, prefix + w0, d.nodes, d.lindx, d.upd_ptr, d.upd_src, pool, d.dvec, b,
-        ldb, d.vwork, p.vwork_elems, d.flags, std::max<i64>(p.nflags, 1), d.epoch, d.spin_error,
-        int(nrhs), block_inverses<T>(h));
+        ldb, d.vwork, p.vwork_elems, d.spin_error, int(nrhs), block_inverses<T>(h));
 }
 
 template <typename T, int R>
@@ -1159,14 +1152,9 @@ static void fwd_rect_level(sprgpu_handle *h, int l, i64 nrhs)
     if (p.wide_frect_wgs[l] == 0) return;
     const unsigned groups = unsigned((nrhs + R - 1) / R);
     const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
-    if constexpr (R == kRectWide)
-        k_fwd_rect_wide<T><<<unsigned(p.wide_frect_wgs[l]) * groups, 256, 0, h->stream>>>(
-            d.wide + w0, nw, d.wide_frect_prefix + w0, d.nodes, d.upd_ptr, d.upd_src,
-            static_cast<const T *>(d.pool), d.vwork, p.vwork_elems, int(nrhs));
-    else
-        k_fwd_rect<T, R><<<unsigned(p.wide_frect_wgs[l]) * groups, 256, 0, h->stream>>>(
-            d.wide + w0, nw, d.wide_frect_prefix + w0, d.nodes, d.upd_ptr, d.upd_src,
-            static_cast<const T *>(d.pool), d.vwork, p.vwork_elems, int(nrhs));
+    k_fwd_rect<T, R><<<unsigned(p.wide_frect_wgs[l]) * groups, 256, 0, h->stream>>>(
+        d.wide + w0, nw, d.wide_frect_prefix + w0, d.nodes, d.upd_ptr, d.upd_src,
+        static_cast<const T *>(d.pool), d.vwork, p.vwork_elems, int(nrhs));
 }
 
 template <typename T, int R>
@@ -1194,8 +1182,7 @@ static void bwd_wide_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs
     const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
     k_bwd_wide<T, R><<<unsigned(p.wide_bwd_wgs[l]) * groups, 256, 0, h->stream>>>(
         d.wide + w0, nw, d.wide_bwd_prefix + w0, d.nodes, d.lindx, pool, b, ldb, d.vwork,
-        p.vwork_elems, d.partials, npart, d.flags, std::max<i64>(p.nflags, 1), d.epoch,
-        d.spin_error, int(nrhs), block_inverses<T>(h));
+        p.vwork_elems, d.partials, npart, d.spin_error, int(nrhs), block_inverses<T>(h));
 }
 
 template <typename T, int R>
@@ -1213,17 +1200,24 @@ static void bwd_rect_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs
 
 // Right-hand sides per workgroup of the rectangle launches (no waits, L21
 // read once per group): kRectGroup when a call has that many.
-constexpr int kRectGroup = kRectWide;
+constexpr int kRectGroup = 32;
+
+// Every block result slot of the wide supernodes to kPending, before a sweep.
+static void mark_pending(sprgpu_handle *h, i64 nrhs)
+{
+    const Plan &p = h->plan;
+    if (p.nflags == 0) return;
+    k_mark_pending<<<dim3(unsigned(p.nflags), unsigned(nrhs)), NB, 0, h->stream>>>(
+        h->d.wide, int(p.wide.size()), h->d.nodes, h->d.vwork, p.vwork_elems);
+}
 
 template <typename T>
 static int solve_chunk(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which)
 {
     const Plan &p = h->plan;
-    DeviceBuffers &d = h->d;
-    hipStream_t s = h->stream;
     const bool grouped = nrhs >= kSolveGroup;
     if (which & 1) {
-        k_epoch_bump<<<1, 64, 0, s>>>(d.epoch);
+        mark_pending(h, nrhs);
         for (int l = 0; l < p.nlevels; ++l) {
             if (p.solve_small_ptr[l + 1] > p.solve_small_ptr[l]) {
                 const int R = narrow_group<T>(p, l, nrhs);
@@ -1243,7 +1237,7 @@ static int solve_chunk(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which
         }
     }
     if (which & 2) {
-        k_epoch_bump<<<1, 64, 0, s>>>(d.epoch);
+        mark_pending(h, nrhs);
         for (int l = p.nlevels - 1; l >= 0; --l) {
             if (p.wide_ptr[l + 1] > p.wide_ptr[l]) {
                 if (nrhs >= kRectGroup) bwd_rect_level<T, kRectGroup>(h, l, b, ldb, nrhs);
