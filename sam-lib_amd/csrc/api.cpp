@@ -189,7 +189,7 @@ int sprgpu_create(const sprgpu_int *mspar, const sprgpu_int *mtrees, const sprgp
     if (e == hipSuccess) e = hipMalloc(&d.status, sizeof(FactorStatus));
     if (e == hipSuccess) e = hipMalloc(&d.stats, sizeof(double) * 4 * 257);
     if (e == hipSuccess) e = hipMalloc(&d.dscr, esz * size_t(p.neq) * kPanelNB);
-    const size_t ninv = size_t(std::max<i64>(p.nflags, 1)) * 2 * kInvLayout;
+    const size_t ninv = size_t(std::max<i64>(p.nflags, 1)) * kBlockSolveData;
     if (e == hipSuccess) e = hipMalloc(&d.dinv, esz * ninv);
     if (e != hipSuccess) {
         free_all(h.get());

@@ -402,8 +402,23 @@ __device__ __forceinline__ T diag_block_factor(const T *src, i64 srs, i64 scs, i
     // same as one column per barrier).  Buffers alternate per pair: one is
     // rewritten two pairs later, after every thread has passed the next
     // barrier.
-    int j = 0;
-    for (; j + 1 < nb; j += 2) {
+    // The pairs are unrolled (j is a constant in each copy, so the thread's
+    // entries of columns j, j+1 are named statically) and leave at nb.
+    // Outside SPRFC3 mode no pivot is replaced, and the FP64 test is skipped.
+    const bool replace = pa.mode == 3;
+    auto pivot = [&](T dj, double a, double &dp) -> T {
+        if (replace) {
+            dp = pivot_value(double(dj), a, pa);
+            return T(dp);
+        }
+        dp = double(dj);
+        return dj;
+    };
+    int jn = 0;  // the first column not yet taken
+#pragma unroll
+    for (int jj = 0; jj < NB / 2; ++jj) {
+        if (2 * jj + 1 >= nb) break;
+        const int j = 2 * jj;
         T *wa = lsh + (3 + 4 * ((j >> 1) & 1)) * NB, *wb = wa + NB, *xa = wb + NB, *xb = xa + NB;
         if (g == (j >> 2)) {  // j is even: both columns belong to group j >> 2
             const bool hi = (j & 3) == 2;
@@ -427,8 +442,8 @@ __device__ __forceinline__ T diag_block_factor(const T *src, i64 srs, i64 scs, i
         const vec_t wcb = *reinterpret_cast<const vec_t *>(wb + c0);
         const vec_t xja = *reinterpret_cast<const vec_t *>(xa + c0);
         const vec_t xjb = *reinterpret_cast<const vec_t *>(xb + c0);
-        const double dp = pivot_value(double(dj), ash[j], pa);
-        const T d = T(dp);
+        double dp;
+        const T d = pivot(dj, ash[j], dp);
         const T di = recip(d);
         const T la = (r > j) ? wra * di : T(0);
         // column j+1 and row j+1 of X after step j
@@ -450,8 +465,8 @@ __device__ __forceinline__ T diag_block_factor(const T *src, i64 srs, i64 scs, i
             xjb1[q] = xv;
         }
         // step j+1
-        const double dp1 = pivot_value(double(dj1), ash[j + 1], pa);
-        const T d1 = T(dp1);
+        double dp1;
+        const T d1 = pivot(dj1, ash[j + 1], dp1);
         const T di1 = recip(d1);
         const T lb = (r > j + 1) ? wrb1 * di1 : T(0);
 #pragma unroll
@@ -484,8 +499,9 @@ __device__ __forceinline__ T diag_block_factor(const T *src, i64 srs, i64 scs, i
             pin[j + 1] = dj1;
             pdd[j + 1] = dp1;
         }
+        jn = j + 2;
     }
-    if (j < nb) {  // an odd last column, one step
+    if (const int j = jn; j < nb) {  // an odd last column, one step
         T *wcol = lsh + (3 + 4 * ((j >> 1) & 1)) * NB, *xrow = wcol + 2 * NB;
         if (g == (j >> 2)) {
             T v = a[0];

@@ -41,6 +41,11 @@ constexpr int kSolveNB = 128;      // column block (and rectangle row chunk) of 
 // kInvLayout entries each (the forward and the backward sweep's), i.e. the
 // lower triangle's three 64 x 64 quadrants per layout.
 constexpr int kInvLayout = 3 * kSolveNB * kSolveNB / 4;
+// Per wide block: the inverse in both layouts, then M = L11^-1 L(block,
+// block - 1) (forward, blocks after the first) and M' = L11^-T L(block + 1,
+// block)^T (backward, blocks before the last), each kSolveNB x kSolveNB
+// column-major.
+constexpr int kBlockSolveData = 2 * kInvLayout + 2 * kSolveNB * kSolveNB;
 
 // A wide supernode in the dataflow solves: its diagonal part is split into
 // nblk blocks of kSolveNB columns (one workgroup each, ordered so that every
@@ -213,7 +218,7 @@ struct DeviceBuffers {
     FactorStatus *status = nullptr;
     double *stats = nullptr;    // reduction scratch for TOL(2)/TOL(3)/NUMNEG
     void *dscr = nullptr;       // factored diagonal blocks of the panel steps, T[NEQ*kPanelNB]
-    void *dinv = nullptr;       // L11^-1 of every wide-solve column block, T[nflags*2*kInvLayout]
+    void *dinv = nullptr;       // per wide-solve column block: inverses, M, M' (T[nflags*kBlockSolveData])
     double *vwork = nullptr;    // solve workspace (vwork_elems * nrhs_cap)
     i64 vwork_cap = 0;
     // residual pattern

@@ -497,6 +497,7 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
 {
     __shared__ double xs[4][R][kXsLd];
     __shared__ double ys[R][NB];
+    __shared__ double zs[R][NB];
     __shared__ T Lf[kInvLayout];  // L11^-1, forward layout
     const int nrb = (nrhs + R - 1) / R;
     const i64 bid = blockIdx.x / nrb;
@@ -521,6 +522,11 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
     // the row I own (lanes cg < 4): B (own columns) plus the children's updates
     const int orow = 4 * g + cg;
     const bool owner = cg < 4 && orow < nr;
+    // its destination and pivot (a diagonal block's rows; read here, before
+    // any wait, for the stores at the end)
+    const int orc = diag ? r0 + min(orow, nr - 1) : 0;
+    const i32 dst = rows[orc];
+    const double dpiv = dvec[n.first + orc];
     double y[R], tot[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) y[k] = tot[k] = 0.0;
@@ -537,11 +543,13 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
             y[k] = v;
         }
     }
+    const T *bdat = dinv + (w.flag0 + (diag ? local : 0)) * i64(kBlockSolveData);
     if (diag) {
-        const T *inv = dinv + (w.flag0 + local) * i64(2 * kInvLayout);
-        for (int idx = tid; idx < kInvLayout; idx += 256) Lf[idx] = inv[idx];
+        for (int idx = tid; idx < kInvLayout; idx += 256) Lf[idx] = bdat[idx];
     }
-    const int nprev = diag ? local : w.nblk;
+    // A diagonal block takes the blocks before the previous one here, and
+    // the previous one through M after its own inverse product
+    const int nprev = diag ? max(local - 1, 0) : w.nblk;
     // Blocks are taken two at a time while both are available ahead of the
     // last one (with one right-hand side; a group takes one at a time, two
     // blocks' L values would not fit in registers): the looks at their x and
@@ -594,25 +602,19 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
         }
         return;
     }
-    // Diagonal block: x = L11^-1 y with the block's inverse in LDS, thread
+    // Diagonal block: z = L11^-1 y with the block's inverse in LDS, thread
     // (ri, h) summing columns [64h, 64h + 64) of row ri; in the forward
     // layout, columns 64..127 of rows 0..63 are zero and not stored.
     if (cg < 4) {
 #pragma unroll
         for (int k = 0; k < R; ++k) ys[k][orow] = owner ? y[k] - tot[k] : 0.0;
     }
-    const int ri = tid >> 1, h = tid & 1;
-    i32 dst = 0;
-    double dpiv = 1.0;
-    if (h == 0 && ri < nr) {
-        dst = rows[r0 + ri];
-        dpiv = dvec[n.first + r0 + ri];
-    }
     __syncthreads();
+    const int ri = tid >> 1, h = tid & 1;
     const bool above = h == 1 && ri < NH;
     const T *Lrow = Lf + (h == 0 ? ri : NB * NB / 2 + (ri - NH));
     const int cs = h == 0 ? NB : NH;
-    double x[R];
+    double z[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         double a0 = 0.0, a1 = 0.0;
@@ -623,13 +625,55 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
                 a1 += double(Lrow[(c + 1) * cs]) * ys[k][h * NH + c + 1];
             }
         }
-        x[k] = pair_sum(a0 + a1, h);
+        z[k] = pair_sum(a0 + a1, h);
     }
-    if (h == 0 && ri < nr) {
-        // the result for the blocks that wait on it first, then B
+    if (local == 0) {  // the first block: x = z
+        if (h == 0 && ri < nr) {
+            const i32 zdst = rows[r0 + ri];
+            const double zpiv = dvec[n.first + r0 + ri];
+            // the result for the blocks that wait on it first, then B
+#pragma unroll
+            for (int k = 0; k < R; ++k)
+                if (rhs0 + k < nrhs) store_shared(vwork + vo[k] + r0 + ri, z[k]);
+#pragma unroll
+            for (int k = 0; k < R; ++k)
+                if (rhs0 + k < nrhs) b[i64(rhs0 + k) * ldb + zdst] = z[k] / zpiv;
+        }
+        return;
+    }
+    // x = z - M x_prev, M = L11^-1 L(block, previous block): the products
+    // are taken as block products (fwd_block_sums) with M for L, and the
+    // owner lane of a row stores its x
+    if (h == 0) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) zs[k][ri] = z[k];
+    }
+    // (the look at x_prev goes before M's loads, so that finding it stored
+    // does not wait for them)
+    const int cp = (local - 1) * NB;  // the previous block: full
+    double xv[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) xv[k] = (tid < NB) ? load_shared(vwork + vo[k] + cp + tid) : 0.0;
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    Rows4<T> mv[16];
+    fwd_load_block(mv, bdat + 2 * kInvLayout, NB, NB, NB, 4 * g, cg, 0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (tid < NB) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) xs[0][k][xs_at(tid)] = await_value(xv[k], vwork + vo[k] + cp + tid, err);
+    }
+    __syncthreads();
+    double mt[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) mt[k] = 0.0;
+    fwd_block_sums<T, R>(mv, xs[0], cg, mt);
+    if (owner) {
+        double x[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) x[k] = zs[k][orow] - mt[k];
 #pragma unroll
         for (int k = 0; k < R; ++k)
-            if (rhs0 + k < nrhs) store_shared(vwork + vo[k] + r0 + ri, x[k]);
+            if (rhs0 + k < nrhs) store_shared(vwork + vo[k] + r0 + orow, x[k]);
 #pragma unroll
         for (int k = 0; k < R; ++k)
             if (rhs0 + k < nrhs) b[i64(rhs0 + k) * ldb + dst] = x[k] / dpiv;
@@ -807,6 +851,7 @@ __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ w
 {
     __shared__ double xs[4][R][NB];
     __shared__ double ys[R][NB];
+    __shared__ double zs[R][NB];
     __shared__ T Lb[kInvLayout];  // L11^-1, backward layout
     const int nrb = (nrhs + R - 1) / R;
     const i64 bid = blockIdx.x / nrb;
@@ -824,11 +869,12 @@ __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ w
 #pragma unroll
     for (int k = 0; k < R; ++k) vo[k] = i64(min(rhs0 + k, nrhs - 1)) * vstride + n.voff;
     const int c0 = blk * NB, nc = min(NB, S - c0);
-    const T *inv = dinv + (w.flag0 + blk) * i64(2 * kInvLayout) + kInvLayout;
-    for (int idx = tid; idx < kInvLayout; idx += 256) Lb[idx] = inv[idx];
+    const T *bdat = dinv + (w.flag0 + blk) * i64(kBlockSolveData);
+    for (int idx = tid; idx < kInvLayout; idx += 256) Lb[idx] = bdat[kInvLayout + idx];
     const int ci = tid >> 1, h = tid & 1;
     const bool cv = ci < nc;
     const i32 dst = rows[c0 + min(ci, nc - 1)];
+    const i32 odst = rows[c0 + min(4 * (tid >> 3) + (tid & 7), nc - 1)];  // of the M' step's owner column
     // products with the rows below the diagonal part: the two lanes of a
     // column sum the even and the odd row chunks' partials
 #pragma unroll
@@ -879,22 +925,24 @@ __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ w
             if (u < nb) bwd_block_sums<T, R>(lv[u], xs[2 * (it & 1) + u], rr, F, p - u, tot);
         ++it;
     };
+    // blocks after the next one here; the next one through M' after the
+    // block's own inverse product
     int p = w.nblk - 1;
     for (; p - PB > blk; p -= PB) step(p, PB);
-    for (; p > blk; --p) step(p, 1);
+    for (; p > blk + 1; --p) step(p, 1);
     __syncthreads();  // ys complete (its initial values) before the owners update it
     if (rr < 4 && 4 * cq + rr < nc) {
 #pragma unroll
         for (int k = 0; k < R; ++k) ys[k][4 * cq + rr] -= tot[k];
     }
     __syncthreads();
-    // Diagonal block: x = L11^-T y with the block's inverse in LDS, thread
+    // Diagonal block: z = L11^-T y with the block's inverse in LDS, thread
     // (ci, h) summing rows [64h, 64h + 64) of column ci; in the backward
     // layout, columns 64..127 of rows 0..63 are zero and not stored.
     const bool above = h == 0 && ci >= NH;
     const T *Lcol = Lb + (h == 0 ? ci : NB * NB / 4 + ci);
     const int rs = h == 0 ? NH : NB;
-    double x[R];
+    double z[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         double a0 = 0.0, a1 = 0.0;
@@ -905,15 +953,57 @@ __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ w
                 a1 += double(Lcol[(r + 1) * rs]) * ys[k][h * NH + r + 1];
             }
         }
-        x[k] = pair_sum(a0 + a1, h);
+        z[k] = pair_sum(a0 + a1, h);
     }
-    if (h == 0 && cv) {
+    if (blk == w.nblk - 1) {  // the last block: x = z
+        if (h == 0 && cv) {
+#pragma unroll
+            for (int k = 0; k < R; ++k)
+                if (rhs0 + k < nrhs) store_shared(vwork + vo[k] + c0 + ci, z[k]);
+#pragma unroll
+            for (int k = 0; k < R; ++k)
+                if (rhs0 + k < nrhs) b[i64(rhs0 + k) * ldb + dst] = z[k];
+        }
+        return;
+    }
+    // x = z - M' x_next, M' = L11^-T L(next block, block)^T: block products
+    // as in the forward sweep (fwd_block_sums, M' for L, the rows of M' are
+    // the block's columns), the owner lane of a column stores its x
+    if (h == 0) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) zs[k][ci] = z[k];
+    }
+    const int g = tid >> 3, cg = tid & 7, oc = 4 * g + cg;
+    const bool owner = cg < 4 && oc < nc;
+    const int rn = (blk + 1) * NB, nn = min(NB, S - rn);  // the next block
+    double xv[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) xv[k] = (tid < nn) ? load_shared(vwork + vo[k] + rn + tid) : 0.0;
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    Rows4<T> mv[16];
+    fwd_load_block(mv, bdat + 2 * kInvLayout + NB * NB, NB, NB, NB, 4 * g, cg, 0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    double(*xm)[kXsLd] = reinterpret_cast<double(*)[kXsLd]>(&xs[0][0][0]);
+    if (tid < NB) {
 #pragma unroll
         for (int k = 0; k < R; ++k)
-            if (rhs0 + k < nrhs) store_shared(vwork + vo[k] + c0 + ci, x[k]);
+            xm[k][xs_at(tid)] = (tid < nn) ? await_value(xv[k], vwork + vo[k] + rn + tid, err) : 0.0;
+    }
+    __syncthreads();
+    double mt[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) mt[k] = 0.0;
+    fwd_block_sums<T, R>(mv, xm, cg, mt);
+    if (owner) {
+        double x[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) x[k] = zs[k][oc] - mt[k];
 #pragma unroll
         for (int k = 0; k < R; ++k)
-            if (rhs0 + k < nrhs) b[i64(rhs0 + k) * ldb + dst] = x[k];
+            if (rhs0 + k < nrhs) store_shared(vwork + vo[k] + c0 + oc, x[k]);
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            if (rhs0 + k < nrhs) b[i64(rhs0 + k) * ldb + odst] = x[k];
     }
 }
 
@@ -923,7 +1013,11 @@ __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ w
 // X = L11^-1 of column block blk of wide supernode w (unit lower triangular,
 // nc <= NB), computed in double by block rows of 32 and stored as T in the
 // forward and the backward layout (zero above the diagonal and past nc) at
-// dinv + 2 kInvLayout (flag0 + blk).  Block row I: A) T = -L(I, <I) X(<I, :) for the
+// dinv + kBlockSolveData (flag0 + blk), followed by M = X L(blk, blk - 1)
+// (blocks after the first) and M' = X^T L(blk + 1, blk)^T (blocks before
+// the last), column-major NB x NB, zero past the block's rows (columns):
+// with them a sweep applies the neighbouring block's result with one
+// product after everything else of the block is done.  Block row I: A) T = -L(I, <I) X(<I, :) for the
 // columns left of the block row (the identity for its own columns), B)
 // X(I, :) = L(I, I)^-1 T by substitution, one column per thread.
 template <typename T>
@@ -933,6 +1027,7 @@ __global__ __launch_bounds__(256) void k_block_inverse(const WideDesc *__restric
 {
     __shared__ double X[NB][NB + 1];  // X[c][r]; odd stride: a column walk is conflict-free
     __shared__ double Lii[32][33];    // the block row's diagonal 32 x 32 block of L
+    __shared__ double Lst[16][NB + 1];  // 16 rows (columns) of L for M (M')
     const i64 b = blockIdx.x;
     int lo = 0, hi = nwide - 1;  // the supernode whose blocks hold b
     while (lo < hi) {
@@ -1002,7 +1097,7 @@ __global__ __launch_bounds__(256) void k_block_inverse(const WideDesc *__restric
     }
     // both layouts (X is zero above the diagonal; rows past nc were never
     // written: zero)
-    T *out = dinv + (w.flag0 + blk) * i64(2 * kInvLayout);
+    T *out = dinv + (w.flag0 + blk) * i64(kBlockSolveData);
     for (int idx = tid; idx < kInvLayout; idx += 256) {
         int rr, cc;
         if (idx < NB * NB / 2) {
@@ -1022,6 +1117,56 @@ __global__ __launch_bounds__(256) void k_block_inverse(const WideDesc *__restric
             rr = NH + (t >> 7), cc = t & (NB - 1);
         }
         out[kInvLayout + idx] = T(X[cc][rr]);
+    }
+    // M(r, c) = sum_i X(r, i) L(c0 + i, c0 - NB + c): thread (r, half) takes
+    // 64 columns; 16 rows of L at a time in LDS (the previous block is full)
+    const int rm = tid & (NB - 1), half = tid >> 7;
+    if (blk > 0) {
+        double acc[NH];
+#pragma unroll
+        for (int c = 0; c < NH; ++c) acc[c] = 0.0;
+        for (int i0 = 0; i0 < nc; i0 += 16) {
+            __syncthreads();  // Lst free
+            for (int idx = tid; idx < 16 * NB; idx += 256) {
+                const int ii = idx & 15, c = idx >> 4;
+                Lst[ii][c] = (i0 + ii < nc) ? double(L[(c0 + i0 + ii) + i64(c0 - NB + c) * ld]) : 0.0;
+            }
+            __syncthreads();
+#pragma unroll 4
+            for (int ii = 0; ii < 16; ++ii) {
+                const double xv = X[i0 + ii][rm];
+#pragma unroll
+                for (int c = 0; c < NH; ++c) acc[c] += xv * Lst[ii][half * NH + c];
+            }
+        }
+        T *m = out + 2 * kInvLayout;
+#pragma unroll
+        for (int c = 0; c < NH; ++c) m[(half * NH + c) * NB + rm] = rm < nc ? T(acc[c]) : T(0);
+    }
+    // M'(c, r) = sum_i X(i, c) L(c0 + NB + r, c0 + i): thread (c, half) takes
+    // 64 rows r of the next block (np of them valid)
+    if (blk + 1 < w.nblk) {
+        const int np = min(NB, n.S - (c0 + NB));
+        double acc[NH];
+#pragma unroll
+        for (int r = 0; r < NH; ++r) acc[r] = 0.0;
+        for (int i0 = 0; i0 < nc; i0 += 16) {
+            __syncthreads();  // Lst free
+            for (int idx = tid; idx < 16 * NB; idx += 256) {
+                const int r = idx & (NB - 1), ii = idx >> 7;
+                Lst[ii][r] = (r < np && i0 + ii < nc) ? double(L[(c0 + NB + r) + i64(c0 + i0 + ii) * ld]) : 0.0;
+            }
+            __syncthreads();
+#pragma unroll 4
+            for (int ii = 0; ii < 16; ++ii) {
+                const double xv = X[rm][i0 + ii];
+#pragma unroll
+                for (int r = 0; r < NH; ++r) acc[r] += xv * Lst[ii][half * NH + r];
+            }
+        }
+        T *m = out + 2 * kInvLayout + NB * NB;
+#pragma unroll
+        for (int r = 0; r < NH; ++r) m[(half * NH + r) * NB + rm] = rm < nc ? T(acc[r]) : T(0);
     }
 }
 
