@@ -36,7 +36,7 @@ sys.path.insert(0, os.path.join(ROOT, "sam-lib_amd"))
 
 METRIC = "factor+solve s & TFLOP/s, 40^3 hex8: FP64 vs FP32+IR at 1e-10 residual"
 PEAK = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense MFMA TFLOP/s
-GEMM_F32_TRAFFIC_40 = 168.6e6  # measured HBM bytes per k_gemm<float,64,1> launch, 40^3
+GEMM_F32_TRAFFIC_40 = 470.5e6  # measured HBM bytes per k_gemm_x32<1> launch, 40^3
 HBM_PEAK = 8000.0  # GB/s
 
 
@@ -405,10 +405,10 @@ def main():
         import ctypes as C
         from sprgpu import _lib
 
-        out = np.zeros(24)
+        out = np.zeros(27)
         tolv = np.zeros(4)
         classes = ["init", "extend_add", "small_lds", "diag_block", "rows_below", "mfma_update",
-                   "other", "mfma_inner"]
+                   "other", "mfma_inner", "mfma_update_128"]
         per = {}
         for name, hh in (("fp32", h32), ("fp64", h64)):
             _lib.check(_lib.lib().sprgpu_profile_factor(
@@ -424,12 +424,16 @@ def main():
                     "launches": g["launches"], "avg_launch_ms": g["ms"] / max(g["launches"], 1)}
 
         result["roofline"] = mfma_roofline(
-            "fp32", "mfma_update", "k_gemm<float,64,1> (rank-256 and Schur updates, one factorization)")
-        # HBM bytes per launch of k_gemm<float,64,1> on this workload, from
+            "fp32", "mfma_update_128",
+            "k_gemm_x32<1> (rank-256 and Schur update launches of at least 4096 64-tiles, "
+            "128-row tiles, one factorization)")
+        # HBM bytes per launch of k_gemm_x32<1> on this workload, from
         # rocprofv3 FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB copy
         # (profiles/k_gemm_f32_hex8_40_hbm_traffic.txt); None at other sizes
         if args.n == 40:
             result["roofline"]["traffic"] = GEMM_F32_TRAFFIC_40
+        result["roofline_update64_fp32"] = mfma_roofline(
+            "fp32", "mfma_update", "k_gemm<float,64,1> (the smaller rank-256 and Schur update launches)")
         result["roofline_inner_fp32"] = mfma_roofline(
             "fp32", "mfma_inner", "k_gemm<float,64,0> (rank-128 updates inside a 256-column group)")
         result["roofline_fp64"] = mfma_roofline(

@@ -298,13 +298,13 @@ sprgpu_int sprgpu_nzerol(const sprgpu_handle *handle);
 sprgpu_int sprgpu_device_bytes(const sprgpu_handle *handle);
 
 /* sprgpu_factor with a HIP event after every kernel (slower; for
- * measurement).  out[3*c+0..2] (c < 8, so out holds 24 values) =
+ * measurement).  out[3*c+0..2] (c < 9, so out holds 27 values) =
  * milliseconds, kernel launches, algorithmic flops for kernel class c:
  * 0 front init, 1 extend-add, 2 LDS small fronts, 3 diagonal blocks
- * (256-column outer blocks; with panel steps, the write-back), 4 rows below
+ * (128-column outer blocks; with panel steps, the write-back), 4 rows below
  * them (outer-block TRSM; or the panel steps), 5 MFMA rank-256 and Schur
- * updates, 6 other, 7 MFMA rank-32 updates inside a 256-column block (panel
- * steps only). */
+ * updates on 64-row tiles, 6 other, 7 MFMA rank-128 updates inside a
+ * 256-column group, 8 MFMA rank-256 and Schur updates on 128-row tiles. */
 int sprgpu_profile_factor(sprgpu_handle *handle, const double *sm_dev, const double *tol,
                           int mode, double *out);
 

@@ -1354,7 +1354,9 @@ __global__ void k_copy_diag(const double *dvec, const double *adiag, double *sm,
 // ---------------------------------------------------------------------------
 // Kernel classes for the per-class timing of sprgpu_profile_factor
 // (P_PDIAG: diagonal blocks, P_PROWS: rows below them).
-enum ProfClass { P_INIT = 0, P_EA, P_SMALL, P_PDIAG, P_PROWS, P_GEMM, P_OTHER, P_GEMM_INNER, P_NCLASS };
+enum ProfClass {
+    P_INIT = 0, P_EA, P_SMALL, P_PDIAG, P_PROWS, P_GEMM, P_OTHER, P_GEMM_INNER, P_GEMM_BIG, P_NCLASS
+};
 
 // Launches the whole factorization.  With ev != nullptr an event is recorded
 // after every kernel and the kernel's class appended to cls, so consecutive
@@ -1456,7 +1458,7 @@ static int factor_impl(sprgpu_handle *h, const double *sm_dev, int mode, const d
             else
                 k_gemm<T, kTileSmall, 0><<<blocks, 256, 0, ls>>>(
                     d.gemm_desc + L.off, int(L.count), d.gemm_prefix + L.off, pool, d.dvec);
-            mark(L.outer ? P_GEMM : P_GEMM_INNER, L.flops);
+            mark(!L.outer ? P_GEMM_INNER : L.tile == kTileBig ? P_GEMM_BIG : P_GEMM, L.flops);
             break;
         }
     }
