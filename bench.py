@@ -236,7 +236,20 @@ def cpu_baseline():
             "kind": "reference",
             "sample": (f"20^3 hex8 (NEQ {sysm.neq}, RINFO(3) {flops:.3e}): full reference "
                        f"SPRFC1 {tf:.2f}s + SPRFS1/SPRBS1 {ts:.3f}s, flang -O2, one thread"),
-            "factor_s": tf, "solve_s": ts}
+            "factor_s": tf, "solve_s": ts, "cpu_model": _cpu_model(),
+            "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+
+
+def _cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), for the baseline's record."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def main():
