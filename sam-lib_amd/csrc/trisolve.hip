@@ -832,6 +832,230 @@ __global__ __launch_bounds__(256) void k_bwd_rect(const WideDesc *__restrict__ w
     }
 }
 
+// ---------------------------------------------------------------------------
+// wide supernodes, many right-hand sides: the rectangle products on MFMA
+// ---------------------------------------------------------------------------
+// With kMmGroup or more right-hand sides, the rectangle products of the wide
+// supernodes (forward: the update rows L21 x1; backward: the partial sums of
+// L21^T x2) are GEMMs over kMmGroup right-hand sides at a time on the FP64
+// 16x16x4 MFMA, so a workgroup reads its part of L once for all of them.  An
+// FP32 factor is widened as it is staged: the products stay FP64, as in the
+// other kernels, but are summed in another order, so these columns agree with
+// a solve of fewer right-hand sides to rounding, not bit for bit.
+// Both kernels form C(rhs, j) = sum_k X(k, rhs) Y(k, j) with the right-hand
+// sides as MFMA operand A (As[k][rhs]) and the factor as operand B
+// (Bs[k][j], j = a row of L21 forward, a column backward), so the 16 adjacent
+// j of an accumulator tile are adjacent in memory when it is stored.  A
+// workgroup computes kMmGroup right-hand sides x kMmCols j, wavefront w the 16
+// j from 16w; K runs in stages of kMmK staged in two LDS buffers: the next
+// stage's global loads are issued before this stage's MFMAs and written to
+// the other buffer after them, one barrier per stage.
+constexpr int kMmGroup = 64;            // right-hand sides per workgroup
+constexpr int kMmCols = 64;             // rows (forward) or columns (backward) of L21 per workgroup
+constexpr int kMmK = 32;                // K per stage
+constexpr int kMmLd = 80;               // LDS row stride (doubles): 16-lane groups on distinct banks
+constexpr int kMmTiles = kMmGroup / 16; // accumulator tiles per wavefront
+static_assert(kMmCols == 64 && NB % kMmCols == 0, "four wavefronts of 16, whole halves of a chunk");
+
+typedef double mm_acc_t __attribute__((ext_vector_type(4)));
+
+// The stage loop shared by both kernels: load(k0) issues the global loads of
+// the stage at k0 into registers, store(buf, k0) writes them to LDS.  The
+// accumulator element e of tile t of a lane is C(16t + lane/16 + 4e, 16w +
+// lane%16) (FP64 16x16x4: A[i][k] from lane i + 16k, B[k][j] from lane
+// j + 16k, element e of row lane/16 + 4e).
+template <typename LoadF, typename StoreF>
+__device__ __forceinline__ void mm_stages(double (*As)[kMmK][kMmLd], double (*Bs)[kMmK][kMmLd],
+                                          int nst, LoadF &&load, StoreF &&store,
+                                          mm_acc_t (&acc)[kMmTiles])
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int t = 0; t < kMmTiles; ++t) acc[t] = mm_acc_t{0.0, 0.0, 0.0, 0.0};
+    load(0);
+    store(0, 0);
+    __syncthreads();
+    for (int s = 0; s < nst; ++s) {
+        const int buf = s & 1;
+        const bool more = s + 1 < nst;
+        if (more) load((s + 1) * kMmK);
+#pragma unroll
+        for (int kk = 0; kk < kMmK / 4; ++kk) {
+            const int k = 4 * kk + (lane >> 4);
+            const double bv = Bs[buf][k][16 * wave + (lane & 15)];
+#pragma unroll
+            for (int t = 0; t < kMmTiles; ++t)
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(As[buf][k][16 * t + (lane & 15)], bv,
+                                                              acc[t], 0, 0, 0);
+        }
+        if (more) store(buf ^ 1, (s + 1) * kMmK);
+        __syncthreads();
+    }
+}
+
+// Forward: workgroup (chunk, half, group) computes rows [r0, r0 + kMmCols)
+// of a wide supernode's update vector (r0 = S + NB chunk + kMmCols half) for
+// kMmGroup right-hand sides: y = (the children's entries of the row, summed
+// in SIBL order) - sum_k L(r, k) x_k over the S columns.  Staging: thread
+// (tid / 8, tid % 8) loads rows 8 (tid % 8) .. + 7 of L column tid / 8 (two
+// vector loads, clamped as in load_rows4) and x_k, k = 4 (tid % 8) .. + 3,
+// of right-hand sides tid / 8 and tid / 8 + 32.  Columns past S read
+// column S - 1 against x = 0; rows past F are not stored.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_rect_mm(const WideDesc *__restrict__ wide, int nwide,
+                                                     const i64 *__restrict__ prefix,
+                                                     const NodeInfo *__restrict__ nodes,
+                                                     const i64 *__restrict__ upd_ptr,
+                                                     const i32 *__restrict__ upd_src,
+                                                     const T *__restrict__ pool, double *vwork,
+                                                     i64 vstride, int nrhs)
+{
+    __shared__ double As[2][kMmK][kMmLd];
+    __shared__ double Bs[2][kMmK][kMmLd];
+    const int nrg = (nrhs + kMmGroup - 1) / kMmGroup;
+    const i64 bid = blockIdx.x / nrg;
+    const int rhs0 = int(blockIdx.x % nrg) * kMmGroup;
+    const i64 cid = bid / (NB / kMmCols);
+    const int q = find_desc(prefix, nwide, cid);
+    const WideDesc w = wide[q];
+    const NodeInfo n = nodes[w.node];
+    const int S = n.S, F = n.F;
+    const int r0 = S + int(cid - prefix[q]) * NB + int(bid % (NB / kMmCols)) * kMmCols;
+    if (r0 >= F) return;  // the whole workgroup: a chunk's second half past F
+    const T *L = pool + n.foff;
+    const i64 ld = n.ld;
+    const int tid = threadIdx.x;
+    const int sc = tid >> 3, sr = 8 * (tid & 7), sk = 4 * (tid & 7);
+    Rows4<T> lv[2];
+    double xv[2][4];
+    auto load = [&](int k0) {
+        const T *Lc = L + i64(min(k0 + sc, S - 1)) * ld;
+        lv[0] = load_rows4(Lc, r0 + sr, F);
+        lv[1] = load_rows4(Lc, r0 + sr + 4, F);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const double *xp = vwork + i64(min(rhs0 + sc + 32 * h, nrhs - 1)) * vstride + n.voff;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) xv[h][i] = xp[min(k0 + sk + i, S - 1)];
+        }
+    };
+    auto store = [&](int buf, int k0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            Bs[buf][sc][sr + i] = double(lv[0].v[i]);
+            Bs[buf][sc][sr + 4 + i] = double(lv[1].v[i]);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                As[buf][sk + i][sc + 32 * h] =
+                    (k0 + sk + i < S && rhs0 + sc + 32 * h < nrhs) ? xv[h][i] : 0.0;
+    };
+    mm_acc_t acc[kMmTiles];
+    mm_stages(As, Bs, (S + kMmK - 1) / kMmK, load, store, acc);
+    const int lane = tid & 63;
+    const int r = r0 + 16 * (tid >> 6) + (lane & 15);
+    if (r < F) {
+        const i64 j0 = upd_ptr[n.voff + r], j1 = upd_ptr[n.voff + r + 1];
+#pragma unroll
+        for (int t = 0; t < kMmTiles; ++t)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int rh = rhs0 + 16 * t + (lane >> 4) + 4 * e;
+                if (rh < nrhs) {
+                    double *vw = vwork + i64(rh) * vstride;
+                    double v = 0.0;
+                    for (i64 j = j0; j < j1; ++j) v += vw[upd_src[j]];
+                    vw[n.voff + r] = v - acc[t][e];
+                }
+            }
+    }
+}
+
+// Backward: workgroup (blk, chunk, half, group) computes the partial sums
+// sum_r L(r, c) x_r over the NB rows of a rectangle chunk (as k_bwd_rect) for
+// the kMmCols columns c from NB blk + kMmCols half and kMmGroup right-hand
+// sides.  Staging: thread (tid / 4, tid % 4) loads rows 8 (tid % 4) .. + 7
+// of the stage in column tid / 4 (two vector loads, written to LDS
+// transposed) and thread (tid / 8, tid % 8) x at rows 4 (tid % 8) .. + 3
+// (through the row list) for right-hand sides tid / 8 and tid / 8 + 32.
+// Rows past the chunk are read clamped against x = 0; columns past the
+// block read its last column and are not stored.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bwd_rect_mm(const WideDesc *__restrict__ wide, int nwide,
+                                                     const i64 *__restrict__ prefix,
+                                                     const NodeInfo *__restrict__ nodes,
+                                                     const i32 *__restrict__ lindx,
+                                                     const T *__restrict__ pool,
+                                                     const double *__restrict__ b, i64 ldb,
+                                                     double *partials, i64 npart, int nrhs)
+{
+    __shared__ double As[2][kMmK][kMmLd];
+    __shared__ double Bs[2][kMmK][kMmLd];
+    const int nrg = (nrhs + kMmGroup - 1) / kMmGroup;
+    const i64 bid = blockIdx.x / nrg;
+    const int rhs0 = int(blockIdx.x % nrg) * kMmGroup;
+    const i64 pid = bid / (NB / kMmCols);
+    const int half = int(bid % (NB / kMmCols));
+    const int q = find_desc(prefix, nwide, pid);
+    const WideDesc w = wide[q];
+    const int t = int(pid - prefix[q]);
+    const int blk = t / w.nrect, kch = t % w.nrect;
+    const NodeInfo n = nodes[w.node];
+    const int S = n.S, F = n.F;
+    const int c0 = blk * NB + half * kMmCols, nc = min(kMmCols, S - c0);
+    if (nc <= 0) return;  // the whole workgroup: a block's second half past S
+    const int r0 = S + kch * NB, nr = min(NB, F - r0);
+    const T *L = pool + n.foff;
+    const i64 ld = n.ld;
+    const int tid = threadIdx.x;
+    const int lc = tid >> 2, lr = 8 * (tid & 3);
+    const int xc = tid >> 3, xk = 4 * (tid & 7);
+    const T *Lc = L + r0 + i64(c0 + min(lc, nc - 1)) * ld;
+    const i32 *rows = lindx + n.lxoff + r0;
+    Rows4<T> lv[2];
+    double xv[2][4];
+    auto load = [&](int k0) {
+        lv[0] = load_rows4(Lc, k0 + lr, F - r0);
+        lv[1] = load_rows4(Lc, k0 + lr + 4, F - r0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const i32 gi = rows[min(k0 + xk + i, nr - 1)];
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                xv[h][i] = b[i64(min(rhs0 + xc + 32 * h, nrhs - 1)) * ldb + gi];
+        }
+    };
+    auto store = [&](int buf, int k0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {  // rows past the chunk may be unwritten memory
+            Bs[buf][lr + i][lc] = (k0 + lr + i < nr) ? double(lv[0].v[i]) : 0.0;
+            Bs[buf][lr + 4 + i][lc] = (k0 + lr + 4 + i < nr) ? double(lv[1].v[i]) : 0.0;
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                As[buf][xk + i][xc + 32 * h] =
+                    (k0 + xk + i < nr && rhs0 + xc + 32 * h < nrhs) ? xv[h][i] : 0.0;
+    };
+    mm_acc_t acc[kMmTiles];
+    mm_stages(As, Bs, (nr + kMmK - 1) / kMmK, load, store, acc);
+    const int lane = tid & 63;
+    const int c = 16 * (tid >> 6) + (lane & 15);
+    if (c < nc) {
+        double *dst = partials + w.pbase + (i64(blk) * w.nrect + kch) * NB + half * kMmCols + c;
+#pragma unroll
+        for (int tt = 0; tt < kMmTiles; ++tt)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int rh = rhs0 + 16 * tt + (lane >> 4) + 4 * e;
+                if (rh < nrhs) dst[i64(rh) * npart] = acc[tt][e];
+            }
+    }
+}
+
 // Workgroup local handles block nblk-1-local: y = B - partial sums of the
 // rows below, minus the later blocks' products as they finish (bwd_block_sums),
 // then x = L11^-T y on the block with its inverse.  Each step of the block
@@ -1344,8 +1568,37 @@ static void bwd_rect_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs
 }
 
 // Right-hand sides per workgroup of the rectangle launches (no waits, L21
-// read once per group): kRectGroup when a call has that many.
+// read once per group): kRectGroup when a call has that many, and the MFMA
+// kernels (kMmGroup per workgroup) from kMmGroup right-hand sides on.
 constexpr int kRectGroup = 32;
+
+template <typename T>
+static void fwd_rect_mm_level(sprgpu_handle *h, int l, i64 nrhs)
+{
+    const Plan &p = h->plan;
+    DeviceBuffers &d = h->d;
+    if (p.wide_frect_wgs[l] == 0) return;
+    const unsigned groups = unsigned((nrhs + kMmGroup - 1) / kMmGroup);
+    const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
+    k_fwd_rect_mm<T><<<unsigned(p.wide_frect_wgs[l]) * (NB / kMmCols) * groups, 256, 0,
+                       h->stream>>>(d.wide + w0, nw, d.wide_frect_prefix + w0, d.nodes, d.upd_ptr,
+                                    d.upd_src, static_cast<const T *>(d.pool), d.vwork,
+                                    p.vwork_elems, int(nrhs));
+}
+
+template <typename T>
+static void bwd_rect_mm_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs)
+{
+    const Plan &p = h->plan;
+    DeviceBuffers &d = h->d;
+    if (p.wide_rect_wgs[l] == 0) return;
+    const unsigned groups = unsigned((nrhs + kMmGroup - 1) / kMmGroup);
+    const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
+    k_bwd_rect_mm<T><<<unsigned(p.wide_rect_wgs[l]) * (NB / kMmCols) * groups, 256, 0,
+                       h->stream>>>(d.wide + w0, nw, d.wide_rect_prefix + w0, d.nodes, d.lindx,
+                                    static_cast<const T *>(d.pool), b, ldb, d.partials,
+                                    std::max<i64>(p.npart, 1), int(nrhs));
+}
 
 // Every block result slot of the wide supernodes to kPending, before a sweep.
 static void mark_pending(sprgpu_handle *h, i64 nrhs)
@@ -1373,7 +1626,8 @@ static int solve_chunk(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which
             if (p.wide_ptr[l + 1] > p.wide_ptr[l]) {
                 if (grouped) {  // the chains, then the rectangles in wider groups
                     fwd_wide_level<T, kSolveGroup>(h, l, b, ldb, nrhs, false);
-                    if (nrhs >= kRectGroup) fwd_rect_level<T, kRectGroup>(h, l, nrhs);
+                    if (nrhs >= kMmGroup) fwd_rect_mm_level<T>(h, l, nrhs);
+                    else if (nrhs >= kRectGroup) fwd_rect_level<T, kRectGroup>(h, l, nrhs);
                     else fwd_rect_level<T, kSolveGroup>(h, l, nrhs);
                 } else {
                     fwd_wide_level<T, 1>(h, l, b, ldb, nrhs, true);
@@ -1385,7 +1639,8 @@ static int solve_chunk(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which
         mark_pending(h, nrhs);
         for (int l = p.nlevels - 1; l >= 0; --l) {
             if (p.wide_ptr[l + 1] > p.wide_ptr[l]) {
-                if (nrhs >= kRectGroup) bwd_rect_level<T, kRectGroup>(h, l, b, ldb, nrhs);
+                if (nrhs >= kMmGroup) bwd_rect_mm_level<T>(h, l, b, ldb, nrhs);
+                else if (nrhs >= kRectGroup) bwd_rect_level<T, kRectGroup>(h, l, b, ldb, nrhs);
                 else if (grouped) bwd_rect_level<T, kSolveGroup>(h, l, b, ldb, nrhs);
                 else bwd_rect_level<T, 1>(h, l, b, ldb, nrhs);
                 if (grouped) bwd_wide_level<T, kSolveGroup>(h, l, b, ldb, nrhs);

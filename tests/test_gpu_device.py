@@ -135,6 +135,39 @@ def test_grouped_rhs_match_single_with_wide_supernodes(precision):
     h.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_mfma_rectangles_with_many_rhs(precision):
+    """From 64 right-hand sides on, the wide supernodes' rectangle products
+    (forward L21 x1, backward L21^T x2) run as FP64 MFMA GEMMs over groups of
+    64: 70 columns on hex8 12^3 (a full group and a partial group of 6).  The
+    products are summed in another order than the per-column kernels, so a
+    column agrees with its own single-RHS solve to rounding (relative
+    difference at most 1e-11, measured about 1e-15), and every column
+    solves A x = b (relative residual at most 1e-12 in FP64; the FP32 factor's
+    columns to the factor's own accuracy, 1e-5)."""
+    torch = _torch()
+    model, sysm, sm = make_system("hex8", 12, e_sigma=0.5, e_seed=1)
+    assert np.diff(sysm.xsuper()).max() > 128
+    h, sm_dev = _handle(sysm, sm, precision)
+    assert h.factor(sm_dev)["ierr"] == 0
+    h.set_pattern(sm)
+    B = torch.randn(70, sysm.neq, dtype=torch.float64, device="cuda")
+    X = B.clone()
+    h.solve(X)
+    worst = 0.0
+    for k in (0, 1, 17, 63, 64, 69):
+        xk = B[k:k + 1].clone()
+        h.solve(xk)
+        worst = max(worst, ((xk[0] - X[k]).norm() / xk[0].norm()).item())
+    r = torch.empty_like(X)
+    h.spmv(X, r)
+    relres = ((B - r).norm(dim=1) / B.norm(dim=1)).max().item()
+    print(f"{precision}: worst column difference {worst:.2e}, max relres {relres:.2e}", flush=True)
+    h.close()
+    assert worst <= 1e-11
+    assert relres <= (1e-12 if precision == "fp64" else 1e-5)
+
+
 def _golden(name):
     with np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False) as z:
         return {k: z[k] for k in z.files}
