@@ -228,11 +228,11 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask *__restrict__ t
 // solves read it.  A workgroup holds `waves` fronts, one per wavefront; the
 // wavefronts synchronise only among their own lanes.  Dynamic LDS per
 // wavefront: F x (F+1) elements of T (F = the launch's largest front) plus
-// the S pivots' a_jj and one column.
-__device__ __forceinline__ size_t small_front_bytes(int maxF, size_t esz)
+// the pivots' a_jj and kSmallF multipliers.
+__host__ __device__ __forceinline__ size_t small_front_bytes(int maxF, size_t esz)
 {
-    return ((size_t(maxF) * (maxF + 1) * esz + 7) / 8 * 8 + 8 * size_t(maxF) + esz * maxF + 15) /
-           16 * 16;
+    return ((size_t(maxF) * (maxF + 1) * esz + 7) / 8 * 8 + 8 * size_t(maxF) + esz * kSmallF +
+            15) / 16 * 16;
 }
 
 __device__ __forceinline__ void wave_sync()
@@ -263,71 +263,111 @@ __global__ __launch_bounds__(256) void k_small(const i32 *__restrict__ list, int
     const NodeInfo n = nodes[list[q]];
     const int F = n.F, S = n.S, r = threadIdx.x & 63;
     const bool rv = r < F;
-    // 1. trapezoid from SM, zero elsewhere
+    // 1. trapezoid from SM, zero elsewhere.  The loads go eight columns at a
+    //    time, unconditionally (entries outside the trapezoid read its first
+    //    entry and are not used), so that eight are in flight together.
     unsigned long long nnz = 0;
-    for (int c = 0; c < F; ++c) {
-        if (!rv || r < c) continue;
-        double v = 0.0;
-        if (c < S) {
-            v = lnz[n.xlnz + i64(c) * F - (i64(c) * (c - 1)) / 2 + (r - c)];
-            nnz += (v != 0.0);
-            if (r == c) {
-                const int eq = n.first + c;
-                if (pa.mode == 3) {
-                    dcor[2 * i64(eq)] = 0.0;
-                    dcor[2 * i64(eq) + 1] = 0.0;
-                }
-                if (v == 0.0) {
-                    if (pa.mode == 3) {  // sprfc3.f:560-575
-                        v = 1.0;
-                        dcor[2 * i64(eq) + 1] = 1.0;
-                    } else {             // sprfc1.f:473-480
-                        atomicAdd(&st->nzero, 1ull);
-                        atomicMin(&st->first_zero, (unsigned long long)eq);
-                    }
-                }
-                adiag[eq] = v;
-                ash[c] = v;
-            }
+    constexpr int U8 = 8;
+    for (int c0 = 0; c0 < F; c0 += U8) {
+        double lv[U8];
+#pragma unroll
+        for (int u = 0; u < U8; ++u) {
+            const int c = c0 + u;
+            const bool in = rv && c < S && r >= c;
+            lv[u] = lnz[n.xlnz + (in ? i64(c) * F - (i64(c) * (c - 1)) / 2 + (r - c) : 0)];
         }
-        M[r + c * LDM] = T(v);
+#pragma unroll
+        for (int u = 0; u < U8; ++u) {
+            const int c = c0 + u;
+            if (c >= F || !rv || r < c) continue;
+            double v = 0.0;
+            if (c < S) {
+                v = lv[u];
+                nnz += (v != 0.0);
+                if (r == c) {
+                    const int eq = n.first + c;
+                    if (pa.mode == 3) {
+                        dcor[2 * i64(eq)] = 0.0;
+                        dcor[2 * i64(eq) + 1] = 0.0;
+                    }
+                    if (v == 0.0) {
+                        if (pa.mode == 3) {  // sprfc3.f:560-575
+                            v = 1.0;
+                            dcor[2 * i64(eq) + 1] = 1.0;
+                        } else {             // sprfc1.f:473-480
+                            atomicAdd(&st->nzero, 1ull);
+                            atomicMin(&st->first_zero, (unsigned long long)eq);
+                        }
+                    }
+                    adiag[eq] = v;
+                    ash[c] = v;
+                }
+            }
+            M[r + c * LDM] = T(v);
+        }
     }
     for (int o = 32; o > 0; o >>= 1) nnz += __shfl_down(nnz, o);
     if (r == 0 && nnz) atomicAdd(&st->nnz_seen, nnz);
     wave_sync();
-    // 2. children's update blocks, one child at a time in SIBL order
+    // 2. children's update blocks, one child at a time in SIBL order (each
+    //    entry of the front is added to at most once per child); the child's
+    //    entries and map values are loaded eight columns at a time,
+    //    unconditionally (clamped into the child's update block)
     for (int k = 0; k < n.nchild; ++k) {
         const NodeInfo c = nodes[children[n.child0 + k]];
         const i32 *map = relmap + c.mapoff;
         const int U = c.F - c.S;
-        if (r < U) {
-            const int pr = map[r];
-            const T *src = pool + c.foff + c.S + r + i64(c.S) * c.ld;
-            for (int j = 0; j <= r; ++j) M[pr + map[j] * LDM] += src[i64(j) * c.ld];
+        if (U <= 0) continue;  // (a child always has update rows)
+        const int rr = min(r, U - 1);
+        const int pr = map[rr];
+        const T *src = pool + c.foff + c.S + rr + i64(c.S) * c.ld;
+        for (int j0 = 0; j0 < U; j0 += U8) {
+            T sv[U8];
+            int mj[U8];
+#pragma unroll
+            for (int u = 0; u < U8; ++u) {
+                const int j = min(j0 + u, rr);
+                sv[u] = src[i64(j) * c.ld];
+                mj[u] = map[j];
+            }
+#pragma unroll
+            for (int u = 0; u < U8; ++u)
+                if (r < U && j0 + u <= r) M[pr + mj[u] * LDM] += sv[u];
         }
         wave_sync();
     }
-    // 3. eliminate the S columns
-    for (int j = 0; j < S; ++j) {
-        const double d_in = double(M[j + j * LDM]);
+    // 3. eliminate the S columns with the lane's row in registers (entries
+    //    right of its diagonal are scratch): step j takes the pivot from lane
+    //    j, the lanes below it scale their entry of column j and write the
+    //    multiplier to LDS, and every lane updates its entries right of
+    //    column j (w = 0 from the pivot row up).  Each entry of the lower
+    //    triangle takes the same operations as with the front in LDS.
+    T m[kSmallF];
+#pragma unroll
+    for (int c = 0; c < kSmallF; ++c) m[c] = (rv && c <= r) ? M[r + c * LDM] : T(0);
+#pragma clang loop unroll(full)
+    for (int j = 0; j < kSmallF; ++j) {
+        if (j >= S) continue;  // (a constant trip count: every copy's j is known)
+        const double d_in = double(lane_bcast(m[j], j));
         const double d = pivot_value(d_in, ash[j], pa);
         if (r == 0) pivot_record(d_in, d, ash[j], n.first + j, pa, dvec, dcor, st);
         const T inv = T(1) / T(d);
-        const T w = (rv && r > j) ? M[r + j * LDM] : T(0);
+        const T w = (rv && r > j) ? m[j] : T(0);
         const T l = w * inv;
         if (rv && r > j) lcol[r] = l;
         wave_sync();
-        if (rv && r > j) {
-            for (int c = j + 1; c <= r; ++c) M[r + c * LDM] -= w * lcol[c];
-            M[r + j * LDM] = l;
-        }
-        if (r == j) M[j + j * LDM] = T(d);
-        wave_sync();
+#pragma clang loop unroll(full)
+        for (int c = 1; c < kSmallF; ++c)
+            if (c > j) m[c] -= w * lcol[c];
+        if (r > j) m[j] = l;
+        if (r == j) m[j] = T(d);
+        wave_sync();  // lcol is rewritten by the next step
     }
     // 4. lower triangle back to the pool
     T *fr = pool + n.foff;
-    if (rv)
-        for (int c = 0; c <= r; ++c) fr[r + i64(c) * n.ld] = M[r + c * LDM];
+#pragma unroll
+    for (int c = 0; c < kSmallF; ++c)
+        if (rv && c <= r) fr[r + i64(c) * n.ld] = m[c];
 }
 
 // Reciprocal from the hardware estimate refined by Newton steps (one for
@@ -1414,8 +1454,7 @@ static int factor_impl(sprgpu_handle *h, const double *sm_dev, int mode, const d
         case L_SMALL: {
             // L.tile holds the launch's largest front
             const int mf = L.tile, waves = p.params.small_waves;
-            const size_t per = ((size_t(mf) * (mf + 1) * sizeof(T) + 7) / 8 * 8 +
-                                8 * size_t(mf) + sizeof(T) * mf + 15) / 16 * 16;
+            const size_t per = small_front_bytes(mf, sizeof(T));
             const unsigned nb = unsigned((L.count + waves - 1) / waves);
             k_small<T><<<nb, 64 * waves, per * waves, ls>>>(
                 d.small_nodes + L.off, int(L.count), mf, d.nodes, d.children, d.relmap, lnz, pool,
