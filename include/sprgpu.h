@@ -301,7 +301,8 @@ sprgpu_int sprgpu_device_bytes(const sprgpu_handle *handle);
  * measurement).  out[3*c+0..2] (c < 9, so out holds 27 values) =
  * milliseconds, kernel launches, algorithmic flops for kernel class c:
  * 0 front init, 1 extend-add, 2 LDS small fronts, 3 diagonal blocks
- * (128-column outer blocks; with panel steps, the write-back), 4 rows below
+ * (128-column outer blocks, with the rows below them when one launch does
+ * both; with panel steps, the write-back), 4 rows below
  * them (outer-block TRSM; or the panel steps), 5 MFMA rank-256 and Schur
  * updates on 64-row tiles, 6 other, 7 MFMA rank-128 updates inside a
  * 256-column group, 8 MFMA rank-256 and Schur updates on 128-row tiles. */
@@ -336,8 +337,9 @@ int sprgpu_factor_solve_graph(sprgpu_handle *handle, const double *sm_dev, doubl
 int sprgpu_autotune(sprgpu_handle *handle, const double *sm_dev, double *b_dev, sprgpu_int ldb,
                     sprgpu_int nrhs, double *report);
 /* The parameters in use: out[0..4] = small_f, fronts per workgroup, narrow
- * threads (0 = by size), update tile, blocked-front scheme (1 = 256-column
- * outer blocks, 0 = 32-column panel steps). */
+ * threads (0 = by size), update tile, blocked-front scheme (1 = 128-column
+ * outer blocks, the diagonal block and the rows below it in one launch;
+ * 2 = the same as two launches; 0 = 32-column panel steps). */
 int sprgpu_plan_params(const sprgpu_handle *handle, int *out);
 /* Re-plans with the given parameters, in the order of sprgpu_plan_params
  * (update tile 0 = per launch: 128 for FP32 launches of at least 4096 64-tiles,

@@ -40,7 +40,7 @@ static void free_all(sprgpu_handle *h)
                     d.ea_tasks, d.small_nodes, d.panel_desc, d.panel_prefix, d.gemm_desc,
                     d.gemm_prefix, d.pool,     d.dvec,       d.adiag,      d.dcor,
                     d.status,   d.stats,      d.vwork,       d.csr_rowptr, d.csr_col,
-                    d.dscr,     d.dinv,
+                    d.dscr,     d.dinv,       d.pub,
                     d.csr_src,  d.csr_val,    d.irwork,      d.level_nodes,
                     d.solve_small, d.wide,     d.wide_fwd_prefix, d.wide_bwd_prefix,
                     d.wide_rect_prefix, d.wide_frect_prefix, d.upd_ptr, d.upd_src, d.partials,
@@ -75,6 +75,29 @@ static void free_plan_buffers(sprgpu_handle *h)
         if (*q) (void)hipFree(*q);
         *q = nullptr;
     }
+}
+
+// The fused outer-block launches' publication areas (kPubElems per
+// descriptor of the plan), sized for the current plan.
+static int ensure_pub(sprgpu_handle *h)
+{
+    const Plan &p = h->plan;
+    DeviceBuffers &d = h->d;
+    bool fused = false;
+    for (const Launch &L : p.launches) fused |= (L.kind == L_BFUSED);
+    const i64 need = fused ? i64(p.panel_desc.size()) * kPubElems : 0;
+    const i64 esz = h->precision == SPRGPU_FP32 ? i64(sizeof(float)) : i64(sizeof(double));
+    if (need > d.pub_cap) {
+        if (d.pub) SPRGPU_HIP(hipFree(d.pub));
+        h->device_bytes -= esz * d.pub_cap;
+        d.pub = nullptr;
+        d.pub_cap = 0;
+        SPRGPU_HIP(hipMalloc(&d.pub, size_t(esz * need)));
+        h->device_bytes += esz * need;
+        d.pub_cap = need;
+    }
+    d.pub_elems = need;
+    return 0;
 }
 
 static int upload_plan(sprgpu_handle *h)
@@ -127,7 +150,8 @@ static int replan(sprgpu_handle *h, const PlanParams &params)
     drop_graph(h);
     free_plan_buffers(h);
     h->plan = std::move(np);
-    return upload_plan(h);
+    if (int rc = upload_plan(h)) return rc;
+    return ensure_pub(h);
 }
 
 }  // namespace sprgpu
@@ -194,6 +218,10 @@ int sprgpu_create(const sprgpu_int *mspar, const sprgpu_int *mtrees, const sprgp
     if (e != hipSuccess) {
         free_all(h.get());
         return hip_fail(e, "device allocation");
+    }
+    if (int rc = ensure_pub(h.get())) {
+        free_all(h.get());
+        return rc;
     }
     if (hipMemset(d.dcor, 0, sizeof(double) * 2 * p.neq) != hipSuccess ||
         hipMemset(d.status, 0, sizeof(FactorStatus)) != hipSuccess) {
@@ -627,7 +655,7 @@ int sprgpu_plan_summary(const sprgpu_int *mspar, const sprgpu_int *mtrees,
     i64 tiles = 0, steps = 0, kernels = 0;
     for (const Launch &L : p.launches) {
         if (L.kind == L_GEMM) tiles += L.blocks;
-        if (L.kind == L_PANEL || L.kind == L_BDIAG) steps += L.count;
+        if (L.kind == L_PANEL || L.kind == L_BDIAG || L.kind == L_BFUSED) steps += L.count;
         if (L.kind != L_RECORD && L.kind != L_WAIT) ++kernels;
     }
     out[0] = double(p.neq);

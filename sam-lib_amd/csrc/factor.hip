@@ -99,6 +99,7 @@ __global__ void k_status_reset(FactorStatus *st)
 {
     if (threadIdx.x == 0) {
         st->first_fail = ULLONG_MAX;
+        st->wait_timeout = 0;
         st->first_zero = ULLONG_MAX;
         st->nzero = 0;
         st->nrepl = 0;
@@ -406,7 +407,8 @@ __device__ __forceinline__ double recip(double d)
 // outside nb x nb) is left in LDS.  The factored rows (l_rc for c < r, d_r at
 // c = r) go to out[r*rs + c*cs] when out is set, with zeros right of the
 // diagonal if zero_upper; the rows of Bm go to bm_out + r*kPanelNB when
-// bm_out is set (rows r < nb).  lsh holds kLshSize elements of T and then
+// bm_out is set (rows r < nb), as agent-scope atomic stores if bm_shared
+// (read by other workgroups of the same launch).  lsh holds kLshSize elements of T and then
 // 2 kPanelNB doubles of LDS scratch (16-byte aligned).  Returns the pivot d_r of the thread's row (zero for
 // r >= nb).  Every thread of the workgroup calls it.
 constexpr int kChainThreads = 256;
@@ -417,7 +419,8 @@ __device__ __forceinline__ T diag_block_factor(const T *src, i64 srs, i64 scs, i
                                                const double *__restrict__ adiag, bool record,
                                                const PivotArgs &pa, double *dvec, double *dcor,
                                                FactorStatus *st, T (*Bm)[kPanelNB + 1], T *lsh,
-                                               T *out, i64 rs, i64 cs, bool zero_upper, T *bm_out)
+                                               T *out, i64 rs, i64 cs, bool zero_upper, T *bm_out,
+                                               bool bm_shared = false)
 {
     constexpr int NB = kPanelNB, Q = 4;
     static_assert(NB * NB == kChainThreads * Q, "four entries of the block per thread");
@@ -603,7 +606,13 @@ __device__ __forceinline__ T diag_block_factor(const T *src, i64 srs, i64 scs, i
     for (int q = 0; q < Q; ++q) {
         const T v = x[q] * dir;
         Bm[c0 + q][r] = v;
-        if (bm_out && c0 + q < nb && r < nb) bm_out[i64(c0 + q) * NB + r] = v;
+        if (bm_out && c0 + q < nb && r < nb) {
+            if (bm_shared)
+                __hip_atomic_store(bm_out + i64(c0 + q) * NB + r, v, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            else
+                bm_out[i64(c0 + q) * NB + r] = v;
+        }
     }
     return dr;
 }
@@ -707,23 +716,39 @@ __global__ __launch_bounds__(256) void k_diag_writeback(const PanelDesc *__restr
 // ---------------------------------------------------------------------------
 // outer-block factor: a kPanelOuter-wide column block of a blocked front
 // ---------------------------------------------------------------------------
-// The block [ob, ob+nbw) (descriptor k0 = ob, nb = nbw) is factored by two
-// launches instead of nbw/kPanelNB panel steps with rank-kPanelNB updates:
-//   k_block_diag   one workgroup per front factors the nbw x nbw diagonal
-//                  block: kPanelNB-column sub-steps of diag_block_factor,
-//                  the sub-panel rows below inside the block (MFMA with Bm)
-//                  and the rank-kPanelNB update of the rest of the block.
-//                  Each sub-step's Bm goes to a scratch array indexed by
-//                  equation.
-//   k_block_trsm   the rows below the block, 16 per wavefront, left-looking
-//                  over the sub-steps: L(r, jb) = (A(r, jb) - sum_{kb<jb}
-//                  L(r, kb) D_kb L(jb, kb)^T) Bm_jb.
+// The block [ob, ob+nbw) (descriptor k0 = ob, nb = nbw) is factored without
+// nbw/kPanelNB panel steps and their rank-kPanelNB updates:
+//   diagonal      one workgroup per front factors the nbw x nbw diagonal
+//                 block: kPanelNB-column sub-steps of diag_block_factor,
+//                 the sub-panel rows below inside the block (MFMA with Bm)
+//                 and the rank-kPanelNB update of the rest of the block.
+//   rows below    16 per wavefront, left-looking over the sub-steps:
+//                 L(r, jb) = (A(r, jb) - sum_{kb<jb} L(r, kb) D_kb
+//                 L(jb, kb)^T) Bm_jb.
 // The operations are those of the panel steps (same recurrences, same Bm
 // products); only the order of the rank-kPanelNB updates of a row changes.
-constexpr int kBdWaves = 4;                                      // k_block_diag
+// k_block_fused runs both in one launch: the diagonal workgroups first (one
+// per descriptor), then the row workgroups, which take sub-step jb as soon
+// as its operands are made, instead of after the whole block: Bm of jb and
+// d_k L(r, k) for the sub-step's rows r at the block's earlier columns k.
+// The diagonal workgroup stores them in the descriptor's publication area
+// as agent-scope atomic stores over a fill of pub_pending (a signalling
+// NaN, which no arithmetic produces), and a row workgroup reads each value
+// with agent-scope atomic loads until it differs from the fill: each value
+// is handed over by its own store, with no fence (an agent-scope fence
+// writes back or invalidates a whole L2, under the far updates' traffic).
+// A row workgroup waits only for a diagonal workgroup, which has a lower
+// index and is dispatched first; each wait is bounded (a wait that gives up
+// counts in the status, and the factorization reports an error).
+// k_block_diag and k_block_trsm are the two halves as separate launches
+// (the rows below then read Bm from a scratch array indexed by equation,
+// and the earlier columns from the front).
+constexpr int kBdWaves = 4;                                      // diagonal workgroup
 static_assert(64 * kBdWaves == kChainThreads, "the diagonal chain uses the whole workgroup");
 constexpr int kBdTiles = (kPanelOuter / 16) * (kPanelOuter / 16 + 1) / 2;
 constexpr int kBdSlots = (kBdTiles + kBdWaves - 1) / kBdWaves;  // 16x16 tiles per wavefront
+constexpr int kBtWaves = kTrsmWaves;                             // row workgroup
+static_assert(kBtWaves == kBdWaves, "one workgroup size for the fused launch");
 
 // (ti, tj) of tile t, column-major over the lower triangle of nt x nt tiles
 __device__ __forceinline__ void tri_tile(int t, int nt, int &ti, int &tj)
@@ -736,28 +761,97 @@ __device__ __forceinline__ void tri_tile(int t, int nt, int &ti, int &tj)
     ti = tj + t;
 }
 
+// Publication area of one descriptor (kPubElems elements of T): Bm of each
+// sub-step (kPanelNB x kPanelNB, [k][c]), then d_k L(r, k) for the rows r of
+// sub-steps 1.. (row r - kPanelNB of kPubRows) at the columns k < kPanelNB
+// * (r / kPanelNB) of the block.
+constexpr int kPubBm = (kPanelOuter / kPanelNB) * kPanelNB * kPanelNB;
+constexpr int kPubRows = kPanelOuter - kPanelNB;
+static_assert(kPubElems == kPubBm + kPubRows * kPubRows, "publication area layout");
+constexpr unsigned kPubSpinLimit = 1u << 22;
+
+template <typename T> struct PubBits;
+template <> struct PubBits<float> {
+    typedef unsigned type;
+    static constexpr type pending = 0xFFA5A5A5u;
+};
+template <> struct PubBits<double> {
+    typedef unsigned long long type;
+    static constexpr type pending = 0xFFF7A5A5A5A5A5A5ull;
+};
+template <typename T>
+__device__ __forceinline__ void pub_store(T *p, T x)
+{
+    __hip_atomic_store(p, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T pub_load(const T *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// The value at p once its writer has stored it, given a first look v
+// (bounded: a wait that gives up counts in the status, sets *dead so that
+// later waits of the thread give up at once, and returns 0).
+template <typename T>
+__device__ __forceinline__ T pub_take(const T *p, T v, FactorStatus *st, bool &dead)
+{
+    unsigned spins = 0;
+    while (__builtin_bit_cast(typename PubBits<T>::type, v) == PubBits<T>::pending) {
+        if (dead || ++spins > kPubSpinLimit) {
+            if (!dead) atomicAdd(&st->wait_timeout, 1ull);
+            dead = true;
+            return T(0);
+        }
+        __builtin_amdgcn_s_sleep(2);
+        v = pub_load(p);
+    }
+    return v;
+}
+
+// Every element of the publication areas to pub_pending, before the
+// factorization.
+template <typename T>
+__global__ __launch_bounds__(256) void k_pub_mark(T *pub, i64 n)
+{
+    typedef typename PubBits<T>::type B;
+    B *q = reinterpret_cast<B *>(pub);
+    for (i64 i = blockIdx.x * i64(256) + threadIdx.x; i < n; i += i64(gridDim.x) * 256)
+        q[i] = PubBits<T>::pending;
+}
+
+template <typename T>
+struct BdLds {
+    T Bm[kPanelNB][kPanelNB + 1];
+    alignas(16) T Lsh[kLshSize + 2 * kPanelNB * sizeof(double) / sizeof(T)];
+    T Ls[kPanelOuter * (kPanelNB + 1)];  // the sub-step's two tile columns, rows relative to k0
+    T dsh[kPanelNB];
+};
+// row stride of the row workgroup's operands (rows padded by 16 bytes)
+template <typename T>
+constexpr int bt_ld = kPanelNB + 16 / int(sizeof(T));
+template <typename T>
+struct BtLds {
+    alignas(16) T Rs[(kPanelOuter - kPanelNB) * bt_ld<T>];  // d_k L(k0 + c, k)
+    alignas(16) T Bs[kPanelNB * bt_ld<T>];                  // Bm of the sub-step
+};
+
 // The block's lower triangle lives in registers as 16x16 MFMA accumulator
 // tiles, tile t owned by wavefront t % kBdWaves (column-major order, so the
 // tiles of every sub-step's trailing update are spread evenly).  A sub-step
 // moves its two tile columns into LDS, wavefront 0 factors the diagonal
 // 32 x 32 part, all wavefronts multiply the rows below by Bm (writing L to
-// the front) and then update their own tiles from the L rows in LDS.
-template <typename T>
-__global__ __launch_bounds__(64 * kBdWaves) void k_block_diag(const PanelDesc *__restrict__ descs,
-                                                             const NodeInfo *__restrict__ nodes,
-                                                             T *__restrict__ pool, double *dvec,
-                                                             const double *__restrict__ adiag,
-                                                             double *dcor, FactorStatus *st,
-                                                             PivotArgs pa, T *bmscr)
+// the front) and then update their own tiles from the L rows in LDS.  With
+// PUB, each sub-step's Bm and the d_k L(r, k) of the rows below it in
+// the block go to the publication area as well (bmscr is not written).
+template <typename T, bool PUB>
+__device__ __forceinline__ void block_diag_body(const PanelDesc &pd, const NodeInfo &n,
+                                                T *__restrict__ pool, double *dvec,
+                                                const double *__restrict__ adiag, double *dcor,
+                                                FactorStatus *st, const PivotArgs &pa, T *bmscr,
+                                                T *pub, BdLds<T> &S)
 {
-    constexpr int NB = kPanelNB, W = kPanelOuter, LS = NB + 1;
+    constexpr int NB = kPanelNB, LS = NB + 1;
     typedef typename Mfma<T>::acc_t acc_t;
-    __shared__ T Bm[NB][NB + 1];
-    __shared__ __attribute__((aligned(16))) T Lsh[kLshSize + 2 * kPanelNB * sizeof(double) / sizeof(T)];
-    __shared__ T Ls[W * LS];  // the sub-step's two tile columns, rows relative to k0
-    __shared__ T dsh[NB];
-    const PanelDesc pd = descs[blockIdx.x];
-    const NodeInfo n = nodes[pd.node];
     T *fr = pool + n.foff + pd.k0 + pd.k0 * n.ld;  // block origin
     const i64 ld = n.ld;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -795,17 +889,18 @@ __global__ __launch_bounds__(64 * kBdWaves) void k_block_diag(const PanelDesc *_
                 for (int e = 0; e < 4; ++e) {
                     const int row = sti[q] * 16 + Mfma<T>::row(lane, e) - k0;
                     const int col = (stj[q] - 2 * j) * 16 + lr;
-                    Ls[row * LS + col] = acc[q][e];
+                    S.Ls[row * LS + col] = acc[q][e];
                 }
             }
         }
         __syncthreads();
         // 2. the diagonal 32 x 32 block
         {
-            const T dr = diag_block_factor<T>(Ls, LS, 1, nb, n.first + pd.k0 + k0, adiag, true, pa,
-                                              dvec, dcor, st, Bm, Lsh, fr + k0 + k0 * ld, 1, ld,
-                                              false, bmscr + i64(n.first + pd.k0 + k0) * NB);
-            if ((tid & 7) == 0) dsh[tid >> 3] = dr;
+            T *bm_out = PUB ? pub + j * NB * NB : bmscr + i64(n.first + pd.k0 + k0) * NB;
+            const T dr = diag_block_factor<T>(S.Ls, LS, 1, nb, n.first + pd.k0 + k0, adiag, true,
+                                              pa, dvec, dcor, st, S.Bm, S.Lsh, fr + k0 + k0 * ld,
+                                              1, ld, false, bm_out, PUB);
+            if ((tid & 7) == 0) S.dsh[tid >> 3] = dr;
         }
         __syncthreads();
         // 3. rows below it: L = A Bm, 16-row slices dealt to the wavefronts
@@ -815,9 +910,9 @@ __global__ __launch_bounds__(64 * kBdWaves) void k_block_diag(const PanelDesc *_
 #pragma unroll
             for (int ks = 0; ks < NB / 4; ++ks) {
                 const int k = ks * 4 + (lane >> 4);
-                const T a = Ls[(u * 16 + lr) * LS + k];
+                const T a = S.Ls[(u * 16 + lr) * LS + k];
 #pragma unroll
-                for (int h = 0; h < 2; ++h) o[h] = Mfma<T>::op(a, Bm[k][h * 16 + lr], o[h]);
+                for (int h = 0; h < 2; ++h) o[h] = Mfma<T>::op(a, S.Bm[k][h * 16 + lr], o[h]);
             }
 #pragma unroll
             for (int h = 0; h < 2; ++h)
@@ -825,8 +920,13 @@ __global__ __launch_bounds__(64 * kBdWaves) void k_block_diag(const PanelDesc *_
                 for (int e = 0; e < 4; ++e) {
                     const int row = u * 16 + Mfma<T>::row(lane, e), col = h * 16 + lr;
                     const T v = (row < m && col < nb) ? o[h][e] : T(0);
-                    Ls[row * LS + col] = v;
-                    if (row < m && col < nb) fr[(k0 + row) + (k0 + col) * ld] = v;
+                    S.Ls[row * LS + col] = v;
+                    if (row < m && col < nb) {
+                        fr[(k0 + row) + (k0 + col) * ld] = v;
+                        if (PUB)  // (row >= NB: a row of a later sub-step)
+                            pub_store(pub + kPubBm + (k0 + row - NB) * kPubRows + k0 + col,
+                                      S.dsh[col] * v);
+                    }
                 }
         }
         __syncthreads();
@@ -840,8 +940,8 @@ __global__ __launch_bounds__(64 * kBdWaves) void k_block_diag(const PanelDesc *_
 #pragma unroll
                 for (int ks = 0; ks < NB / 4; ++ks) {
                     const int k = ks * 4 + (lane >> 4);
-                    const T a = Ls[ra * LS + k];
-                    const T b = Ls[rb * LS + k] * dsh[k];
+                    const T a = S.Ls[ra * LS + k];
+                    const T b = S.Ls[rb * LS + k] * S.dsh[k];
                     u = Mfma<T>::op(a, b, u);
                 }
                 acc[q] -= u;
@@ -862,26 +962,21 @@ __global__ __launch_bounds__(64 * kBdWaves) void k_block_diag(const PanelDesc *_
 // operands (d_k L(jb, kb)^T and Bm, staged in LDS) follow that k assignment.
 // The last sub-step of a block may be narrower than kPanelNB: its Bm rows
 // k >= nb are staged as zeros and addresses are clamped into the block.
-constexpr int kBtWaves = kTrsmWaves;
-
-template <typename T>
-__global__ __launch_bounds__(64 * kBtWaves) void k_block_trsm(const PanelDesc *__restrict__ descs,
-                                                             int ndesc,
-                                                             const i64 *__restrict__ prefix,
-                                                             const NodeInfo *__restrict__ nodes,
-                                                             T *__restrict__ pool,
-                                                             const double *__restrict__ dvec,
-                                                             const T *__restrict__ bmscr)
+// With PUB, the operands come from the descriptor's publication area as
+// the diagonal workgroup stores them (all loads of a sub-step issued, then
+// each awaited).
+template <typename T, bool PUB>
+__device__ __forceinline__ void block_trsm_body(const PanelDesc &pd, int sb, const NodeInfo &n,
+                                                T *__restrict__ pool,
+                                                const double *__restrict__ dvec,
+                                                const T *__restrict__ bmscr, const T *pub,
+                                                FactorStatus *st, BtLds<T> &S)
 {
-    constexpr int NB = kPanelNB, NJB = kPanelOuter / kPanelNB, LR = NB + 16 / int(sizeof(T));
+    constexpr int NB = kPanelNB, NJB = kPanelOuter / kPanelNB, LR = bt_ld<T>;
+    constexpr int NT = 64 * kBtWaves;
     static_assert(NB == 32 && kTrsmRows == 16, "two 16-column halves per sub-step");
+    static_assert(NB * NB == 4 * NT, "four Bm entries per thread");
     typedef typename Mfma<T>::acc_t acc_t;
-    __shared__ __attribute__((aligned(16))) T Rs[(kPanelOuter - NB) * LR];  // d_k L(k0 + c, k)
-    __shared__ __attribute__((aligned(16))) T Bs[NB * LR];                   // Bm of the sub-step
-    const int q = find_desc(prefix, ndesc, blockIdx.x);
-    const PanelDesc pd = descs[q];
-    const int sb = int(blockIdx.x - prefix[q]) * kBtWaves + (threadIdx.x >> 6);
-    const NodeInfo n = nodes[pd.node];
     T *fr = pool + n.foff;
     const i64 ld = n.ld;
     const int tid = threadIdx.x, lane = tid & 63, c = lane & 15;
@@ -890,6 +985,7 @@ __global__ __launch_bounds__(64 * kBtWaves) void k_block_trsm(const PanelDesc *_
     const int row = bend + sb * kTrsmRows + c;  // row r = lane & 15 of the stripe
     const bool rowok = row < n.F;
     const T *frow = fr + (rowok ? row : n.F - 1);
+    bool dead = false;
     int kk[4];  // k slot (within a 16-wide half) this lane supplies at step e
 #pragma unroll
     for (int e = 0; e < 4; ++e) kk[e] = Mfma<T>::row(lane, e);
@@ -907,14 +1003,47 @@ __global__ __launch_bounds__(64 * kBtWaves) void k_block_trsm(const PanelDesc *_
             for (int e = 0; e < 4; ++e) acc[h][e] = frow[i64(k0 + min(16 * h + kk[e], nb - 1)) * ld];
         // stage d_k L(k0 + cc, k) for k in [ob, k0) and Bm (rows k >= nb zero)
         __syncthreads();  // the previous sub-step's reads of Rs/Bs are done
-        for (int x = tid; x < NB * jb * NB; x += 64 * kBtWaves) {
-            const int cc = x % NB, kx = x / NB;
-            const int kcol = pd.k0 + kx;
-            Rs[kx * LR + cc] = T(dvec[n.first + kcol]) * fr[(k0 + min(cc, nb - 1)) + i64(kcol) * ld];
-        }
-        for (int x = tid; x < NB * NB; x += 64 * kBtWaves) {
-            const int cc = x % NB, k = x / NB;
-            Bs[k * LR + cc] = (k < nb) ? bmscr[i64(n.first + k0 + k) * NB + cc] : T(0);
+        if (PUB) {
+            // x = tid + NT u: Rs entry (kx, cc) = (x / NB, x % NB), Bm entry
+            // (k, cc) likewise
+            T rv[4 * (NJB - 1)], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4 * (NJB - 1); ++u) {
+                if (u >= 4 * jb) break;
+                const int x = tid + NT * u, cc = x % NB, kx = x / NB;
+                rv[u] = pub_load(pub + kPubBm + (k0 - pd.k0 + min(cc, nb - 1) - NB) * kPubRows + kx);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {  // (entries k, cc >= nb are not made)
+                const int x = tid + NT * u, cc = x % NB, k = x / NB;
+                bv[u] = pub_load(pub + jb * NB * NB + min(k, nb - 1) * NB + min(cc, nb - 1));
+            }
+#pragma unroll
+            for (int u = 0; u < 4 * (NJB - 1); ++u) {
+                if (u >= 4 * jb) break;
+                const int x = tid + NT * u, cc = x % NB, kx = x / NB;
+                S.Rs[kx * LR + cc] = pub_take(
+                    pub + kPubBm + (k0 - pd.k0 + min(cc, nb - 1) - NB) * kPubRows + kx, rv[u], st,
+                    dead);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int x = tid + NT * u, cc = x % NB, k = x / NB;
+                const T v = pub_take(pub + jb * NB * NB + min(k, nb - 1) * NB + min(cc, nb - 1),
+                                     bv[u], st, dead);
+                S.Bs[k * LR + cc] = (k < nb && cc < nb) ? v : T(0);
+            }
+        } else {
+            for (int x = tid; x < NB * jb * NB; x += NT) {
+                const int cc = x % NB, kx = x / NB;
+                const int kcol = pd.k0 + kx;
+                S.Rs[kx * LR + cc] =
+                    T(dvec[n.first + kcol]) * fr[(k0 + min(cc, nb - 1)) + i64(kcol) * ld];
+            }
+            for (int x = tid; x < NB * NB; x += NT) {
+                const int cc = x % NB, k = x / NB;
+                S.Bs[k * LR + cc] = (k < nb) ? bmscr[i64(n.first + k0 + k) * NB + cc] : T(0);
+            }
         }
         __syncthreads();
         // acc(c, r) -= sum_k d_k L(k0 + c, kb cols) L(r, kb cols) over kb < jb,
@@ -927,7 +1056,7 @@ __global__ __launch_bounds__(64 * kBtWaves) void k_block_trsm(const PanelDesc *_
             for (int hk = 0; hk < 2; ++hk)
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const T *rk = Rs + (kb * NB + 16 * hk + kk[e]) * LR;
+                    const T *rk = S.Rs + (kb * NB + 16 * hk + kk[e]) * LR;
 #pragma unroll
                     for (int h = 0; h < 2; ++h)
                         u[h] = Mfma<T>::op(rk[16 * h + c], O[kb][hk][e], u[h]);
@@ -942,7 +1071,7 @@ __global__ __launch_bounds__(64 * kBtWaves) void k_block_trsm(const PanelDesc *_
         for (int hk = 0; hk < 2; ++hk)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const T *bk = Bs + (16 * hk + kk[e]) * LR;
+                const T *bk = S.Bs + (16 * hk + kk[e]) * LR;
 #pragma unroll
                 for (int h = 0; h < 2; ++h) O[jb][h] = Mfma<T>::op(bk[16 * h + c], acc[hk][e], O[jb][h]);
             }
@@ -955,6 +1084,68 @@ __global__ __launch_bounds__(64 * kBtWaves) void k_block_trsm(const PanelDesc *_
                     if (col < nb) fr[row + i64(k0 + col) * ld] = O[jb][h][e];
                 }
         }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(64 * kBdWaves) void k_block_diag(const PanelDesc *__restrict__ descs,
+                                                             const NodeInfo *__restrict__ nodes,
+                                                             T *__restrict__ pool, double *dvec,
+                                                             const double *__restrict__ adiag,
+                                                             double *dcor, FactorStatus *st,
+                                                             PivotArgs pa, T *bmscr)
+{
+    __shared__ BdLds<T> S;
+    const PanelDesc pd = descs[blockIdx.x];
+    const NodeInfo n = nodes[pd.node];
+    block_diag_body<T, false>(pd, n, pool, dvec, adiag, dcor, st, pa, bmscr, nullptr, S);
+}
+
+template <typename T>
+__global__ __launch_bounds__(64 * kBtWaves) void k_block_trsm(const PanelDesc *__restrict__ descs,
+                                                             int ndesc,
+                                                             const i64 *__restrict__ prefix,
+                                                             const NodeInfo *__restrict__ nodes,
+                                                             T *__restrict__ pool,
+                                                             const double *__restrict__ dvec,
+                                                             const T *__restrict__ bmscr)
+{
+    __shared__ BtLds<T> S;
+    const int q = find_desc(prefix, ndesc, blockIdx.x);
+    const PanelDesc pd = descs[q];
+    const int sb = int(blockIdx.x - prefix[q]) * kBtWaves + (threadIdx.x >> 6);
+    const NodeInfo n = nodes[pd.node];
+    block_trsm_body<T, false>(pd, sb, n, pool, dvec, bmscr, nullptr, nullptr, S);
+}
+
+// Workgroups [0, ndesc): the diagonal blocks; then the rows below them
+// (prefix as k_block_trsm's).  pub: the publication areas of the launch's
+// descriptors, kPubElems each, filled with pub_pending.
+template <typename T>
+__global__ __launch_bounds__(64 * kBdWaves) void k_block_fused(const PanelDesc *__restrict__ descs,
+                                                              int ndesc,
+                                                              const i64 *__restrict__ prefix,
+                                                              const NodeInfo *__restrict__ nodes,
+                                                              T *__restrict__ pool, double *dvec,
+                                                              const double *__restrict__ adiag,
+                                                              double *dcor, FactorStatus *st,
+                                                              PivotArgs pa, T *pub)
+{
+    constexpr size_t kBytes = sizeof(BdLds<T>) > sizeof(BtLds<T>) ? sizeof(BdLds<T>) : sizeof(BtLds<T>);
+    __shared__ __attribute__((aligned(16))) char lds[kBytes];
+    if (int(blockIdx.x) < ndesc) {
+        const PanelDesc pd = descs[blockIdx.x];
+        const NodeInfo n = nodes[pd.node];
+        block_diag_body<T, true>(pd, n, pool, dvec, adiag, dcor, st, pa, nullptr,
+                           pub + i64(blockIdx.x) * kPubElems, *reinterpret_cast<BdLds<T> *>(lds));
+    } else {
+        const i64 b = i64(blockIdx.x) - ndesc;
+        const int q = find_desc(prefix, ndesc, b);
+        const PanelDesc pd = descs[q];
+        const int sb = int(b - prefix[q]) * kBtWaves + (threadIdx.x >> 6);
+        const NodeInfo n = nodes[pd.node];
+        block_trsm_body<T, true>(pd, sb, n, pool, dvec, nullptr, pub + i64(q) * kPubElems, st,
+                           *reinterpret_cast<BtLds<T> *>(lds));
     }
 }
 
@@ -1424,6 +1615,11 @@ static int factor_impl(sprgpu_handle *h, const double *sm_dev, int mode, const d
     mark(P_OTHER, 0.0);
     k_status_reset<<<1, 64, 0, s>>>(d.status);
     mark(P_OTHER, 0.0);
+    if (d.pub_elems > 0) {  // the fused outer-block launches' publication areas
+        k_pub_mark<T><<<unsigned(std::min<i64>((d.pub_elems + 255) / 256, 2048)), 256, 0, s>>>(
+            static_cast<T *>(d.pub), d.pub_elems);
+        mark(P_OTHER, 0.0);
+    }
     // The plan's far updates go to a second stream, ordered by events
     // (L_RECORD, L_WAIT); the profiling pass runs everything on one stream.
     const bool two = (ev == nullptr);
@@ -1481,6 +1677,13 @@ static int factor_impl(sprgpu_handle *h, const double *sm_dev, int mode, const d
                                                   d.panel_prefix + L.off, d.nodes, pool, d.dvec,
                                                   static_cast<const T *>(d.dscr));
             mark(P_PROWS, 0.0);
+            break;
+        case L_BFUSED:
+            // publication area q of the launch: the descriptor's index in the plan
+            k_block_fused<T><<<blocks, 64 * kBdWaves, 0, ls>>>(
+                d.panel_desc + L.off, int(L.count), d.panel_prefix + L.off, d.nodes, pool, d.dvec,
+                d.adiag, d.dcor, d.status, pa, static_cast<T *>(d.pub) + L.off * kPubElems);
+            mark(P_PDIAG, 0.0);
             break;
         case L_GEMM:
             if (L.tile == kTileBig) {
@@ -1558,13 +1761,17 @@ int profile_factor(sprgpu_handle *h, const double *sm_dev, int mode, const doubl
 
 int read_factor_info(sprgpu_handle *h, sprgpu_factor_info *info)
 {
-    if (h->mode == 2) return cc_factor_info(h, info);
     FactorStatus st;
     double agg[4];
     SPRGPU_HIP(hipMemcpyAsync(&st, h->d.status, sizeof(st), hipMemcpyDeviceToHost, h->stream));
     SPRGPU_HIP(hipMemcpyAsync(agg, h->d.stats + 4 * kStatBlocks, sizeof(agg),
                               hipMemcpyDeviceToHost, h->stream));
     SPRGPU_HIP(hipStreamSynchronize(h->stream));
+    if (st.wait_timeout) {
+        set_error("a wait of the fused outer-block factorization timed out");
+        return -5;
+    }
+    if (h->mode == 2) return cc_factor_info(h, info);
     const i64 neq = h->plan.neq;
     info->nzero_diag = i64(st.nzero);
     info->ierr = 0;

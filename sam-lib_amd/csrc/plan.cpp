@@ -44,7 +44,7 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
     if (params.narrow_threads != 0 && params.narrow_threads != 64 &&
         params.narrow_threads != 128 && params.narrow_threads != 256)
         return "narrow_threads must be 0, 64, 128 or 256";
-    if (params.block_factor != 0 && params.block_factor != 1) return "block_factor must be 0 or 1";
+    if (params.block_factor < 0 || params.block_factor > 2) return "block_factor must be 0, 1 or 2";
     p.big_tile = params.big_tile;  // 0: chosen per launch (below)
     // Tile of a rank-kOuterRank / Schur update launch: the configured one,
     // or with big_tile 0 in FP32 the 128 tile (32x32x2 MFMA kernel) for
@@ -274,8 +274,17 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
                     nstripes += ns;
                 }
                 const i64 cnt = i64(p.panel_desc.size()) - off;
-                add_launch(L_BDIAG, l, off, cnt, cnt);
-                if (nstripes > 0) add_launch(L_BTRSM, l, off, cnt, nstripes);
+                // One launch for the diagonal blocks and the rows below
+                // them where the launch is short of work (few fronts: the
+                // rows below then overlap the diagonal chains), two where
+                // many fronts fill the device anyway (the separate kernels
+                // keep more workgroups resident).
+                if (p.params.block_factor == 1 && cnt <= kFusedMaxFronts) {
+                    add_launch(L_BFUSED, l, off, cnt, cnt + nstripes);
+                } else {
+                    add_launch(L_BDIAG, l, off, cnt, cnt);
+                    if (nstripes > 0) add_launch(L_BTRSM, l, off, cnt, nstripes);
+                }
                 // Updates by the blocks of a kOuterRank-wide group.  After an
                 // earlier block of the group: a rank-kPanelOuter update of the
                 // group's remaining columns.  After its last block: the

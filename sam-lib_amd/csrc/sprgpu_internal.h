@@ -29,6 +29,11 @@ constexpr int kPanelOuter = 128; // outer block of the two-level panel blocking 
 constexpr int kOuterRank = 256;  // rank of the trailing updates between outer blocks
 constexpr int kTrsmRows = 16;    // rows below an outer block per wavefront (k_block_trsm) ...
 constexpr int kTrsmWaves = 4;    // ... and wavefronts per workgroup
+// Elements of T per outer block that the fused outer-block launch hands from
+// the diagonal workgroup to the rows below: Bm of the four sub-steps and
+// d_k L(r, k) of the block's rows past the first sub-step.
+constexpr int kFusedMaxFronts = 64;  // outer-block launches of at most this many fronts are fused
+constexpr int kPubElems = 4 * kPanelNB * kPanelNB + (kPanelOuter - kPanelNB) * (kPanelOuter - kPanelNB);
 constexpr int kSmallF = 64;      // fronts with F <= kSmallF can be factored in LDS (one lane a row)
 constexpr int kInitCols = 8;     // front columns per workgroup in the init kernel
 constexpr int kEaCols = 8;       // child columns per workgroup in extend-add
@@ -103,6 +108,7 @@ enum LaunchKind : i32 {
     L_BTRSM = 6,   // rows below an outer block (16 per wavefront)
     L_RECORD = 7,  // no kernel: record the event of stream `stream`
     L_WAIT = 8,    // no kernel: stream `stream` waits for the other stream's event
+    L_BFUSED = 9,  // outer-block diagonal factor and the rows below it in one launch
 };
 
 // A launch of one kernel over a contiguous range of a task array.
@@ -125,7 +131,8 @@ struct PlanParams {
     i32 big_tile = 0;         // MFMA tile of rank-kOuterRank/Schur updates (0: per launch,
                               // 128 for large FP32 launches, else 64)
     i32 narrow_threads = 0;   // workgroup size of the narrow solves (0: by front size)
-    i32 block_factor = 1;     // blocked fronts: 1 = outer-block kernels, 0 = panel steps
+    i32 block_factor = 1;     // blocked fronts: 1 = outer blocks in one launch each, 2 = their
+                              // diagonal blocks and rows below as two launches, 0 = panel steps
 };
 
 struct Plan {
@@ -197,7 +204,8 @@ struct FactorStatus {
     unsigned long long nzero;        // zero diagonals before the factor
     unsigned long long nrepl;        // pivots replaced (SPRFC3 mode)
     unsigned long long nnz_seen;     // non-zeros in the packed input
-    unsigned long long pad[3];
+    unsigned long long wait_timeout; // waits of the fused outer-block launch that gave up
+    unsigned long long pad[2];
 };
 
 struct DeviceBuffers {
@@ -219,6 +227,8 @@ struct DeviceBuffers {
     double *stats = nullptr;    // reduction scratch for TOL(2)/TOL(3)/NUMNEG
     void *dscr = nullptr;       // factored diagonal blocks of the panel steps, T[NEQ*kPanelNB]
     void *dinv = nullptr;       // per wide-solve column block: inverses, M, M' (T[nflags*kBlockSolveData])
+    void *pub = nullptr;        // fused outer-block launches: T[pub_elems], kPubElems per descriptor
+    i64 pub_elems = 0, pub_cap = 0;
     double *vwork = nullptr;    // solve workspace (vwork_elems * nrhs_cap)
     i64 vwork_cap = 0;
     // residual pattern

@@ -201,13 +201,15 @@ def test_sprfc3_replacement_matches_reference():
 
 
 @pytest.mark.parametrize("mode", [1, 3])
-@pytest.mark.parametrize("block", [1, 0], ids=["outer_blocks", "panel_steps"])
+@pytest.mark.parametrize("block", [1, 2, 0], ids=["outer_blocks_fused", "outer_blocks",
+                                                   "panel_steps"])
 def test_pivot_rules_inside_blocked_fronts(block, mode):
     """TOL(1) = 0.6 puts about 50 pivots of the wide fronts (the root has
-    S = 702: three 256-column outer blocks) under the threshold d/a_jj:
+    S = 702: six 128-column outer blocks) under the threshold d/a_jj:
     SPRFC1 stops at the first one (IERR -3, MSPAR(27) = its equation),
-    SPRFC3 replaces every one of them; both schedules must match the
-    oracle."""
+    SPRFC3 replaces every one of them; every schedule (outer blocks with the
+    rows below in the same launch or in their own, panel steps) must match
+    the oracle."""
     torch = _torch()
     from sprgpu.device import SprFactor
 
@@ -250,22 +252,26 @@ def test_pivot_rules_inside_blocked_fronts(block, mode):
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_outer_blocks_match_panel_steps(precision):
-    """The two blocked-front schedules compute the same factor (the rank-32
-    updates of a row are summed in a different order)."""
+    """The blocked-front schedules compute the same factor: panel steps and
+    outer blocks to rounding (the rank-32 updates of a row are summed in a
+    different order), the outer blocks in one launch or two bit for bit (the
+    same operations; only where the rows below read their operands
+    differs)."""
     torch = _torch()
     from sprgpu.device import SprFactor
 
     model, sysm, sm = make_system("hex8", 12, e_sigma=0.5, e_seed=1)
     sm_dev = torch.from_numpy(sm).cuda()
     fs = []
-    for block in (0, 1):
+    for block in (0, 1, 2):
         h = SprFactor(sysm.mspar, sysm.mtrees, sysm.msifa, precision)
         h.set_plan_params(block_factor=block)
         info = h.factor(sm_dev)
         assert info["ierr"] == 0
         fs.append(h.get_factor())
         h.close()
-    a, b = fs
+    a, b, c = fs
+    assert np.array_equal(b, c)
     tol = 1e-11 if precision == "fp64" else 2e-4
     assert np.max(np.abs(a[:sysm.neq] / b[:sysm.neq] - 1)) < tol
     assert np.linalg.norm(a[sysm.neq:] - b[sysm.neq:]) / np.linalg.norm(b[sysm.neq:]) < tol
