@@ -312,7 +312,8 @@ int sprgpu_profile_factor(sprgpu_handle *handle, const double *sm_dev, const dou
 /* Host-only: builds the device schedule for a symbolic structure without
  * touching a GPU and reports it, for checking and sizing.  out[0..9] =
  * NEQ, NSUPER, tree heights, kernel launches per factorization, front-pool
- * elements, solve-workspace elements per RHS, fronts factored in LDS,
+ * elements (the fronts alive at once; the factored panels are kept in a
+ * separate store of sum ld*S elements), solve-workspace elements per RHS, fronts factored in LDS,
  * outer blocks (or panel steps), MFMA update tiles, factor flops (RINFO(3) formula,
  * sprtrs.f:1961-1969).  Returns 0 or -2 (structure not supported). */
 int sprgpu_plan_summary(const sprgpu_int *mspar, const sprgpu_int *mtrees,

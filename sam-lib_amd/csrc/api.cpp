@@ -38,7 +38,7 @@ static void free_all(sprgpu_handle *h)
     void *ptrs[] = {d.nodes,    d.children,   d.lindx,       d.relmap,     d.init_tasks,
                     d.copy_tasks,
                     d.ea_tasks, d.small_nodes, d.panel_desc, d.panel_prefix, d.gemm_desc,
-                    d.gemm_prefix, d.pool,     d.dvec,       d.adiag,      d.dcor,
+                    d.gemm_prefix, d.pool,     d.store,      d.dvec,       d.adiag,      d.dcor,
                     d.status,   d.stats,      d.vwork,       d.csr_rowptr, d.csr_col,
                     d.dscr,     d.dinv,       d.pub,
                     d.csr_src,  d.csr_val,    d.irwork,      d.level_nodes,
@@ -146,6 +146,10 @@ static int replan(sprgpu_handle *h, const PlanParams &params)
         set_error(err);
         return -2;
     }
+    if (np.pool_elems != h->plan.pool_elems || np.store_elems != h->plan.store_elems) {
+        set_error("re-planning changed the front pool or the factor store");  // (not expected)
+        return -2;
+    }
     SPRGPU_HIP(hipStreamSynchronize(h->stream));
     drop_graph(h);
     free_plan_buffers(h);
@@ -207,6 +211,7 @@ int sprgpu_create(const sprgpu_int *mspar, const sprgpu_int *mtrees, const sprgp
     sprgpu_int &B = h->device_bytes;
     const size_t esz = precision == SPRGPU_FP32 ? sizeof(float) : sizeof(double);
     hipError_t e = hipMalloc(&d.pool, esz * size_t(p.pool_elems));
+    if (e == hipSuccess) e = hipMalloc(&d.store, esz * size_t(p.store_elems));
     if (e == hipSuccess) e = hipMalloc(&d.dvec, sizeof(double) * p.neq);
     if (e == hipSuccess) e = hipMalloc(&d.adiag, sizeof(double) * p.neq);
     if (e == hipSuccess) e = hipMalloc(&d.dcor, sizeof(double) * 2 * p.neq);
@@ -228,7 +233,7 @@ int sprgpu_create(const sprgpu_int *mspar, const sprgpu_int *mtrees, const sprgp
         free_all(h.get());
         return hip_fail(hipGetLastError(), "device memset");
     }
-    B += esz * p.pool_elems + sizeof(double) * 4 * p.neq + sizeof(FactorStatus) + 8 * 4 * 257 +
+    B += esz * (p.pool_elems + p.store_elems) + sizeof(double) * 4 * p.neq + sizeof(FactorStatus) + 8 * 4 * 257 +
          esz * (size_t(p.neq) * kPanelNB + ninv);
     // the factorization's second stream and its two events (default priority)
     if (hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess ||

@@ -628,7 +628,7 @@ __device__ __forceinline__ T diag_block_factor(const T *src, i64 srs, i64 scs, i
 // MFMA, one 64-row slice per wavefront.  The A21 loads are issued before the diagonal factor so their
 // latency overlaps it.  Workgroup 0 records the pivots and stores the factored
 // block in a scratch array indexed by equation; the diagonal blocks are
-// written into the fronts after the whole factorization (k_diag_writeback),
+// written into the factor store after the whole factorization (k_diag_writeback),
 // so no workgroup ever reads a block another one writes.
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel(const PanelDesc *__restrict__ descs, int ndesc,
@@ -695,16 +695,17 @@ __global__ __launch_bounds__(256) void k_panel(const PanelDesc *__restrict__ des
             }
 }
 
-// Writes the factored diagonal blocks of every panel step into the fronts.
+// Writes the factored diagonal blocks of every panel step into the factored
+// panels in the store (after the whole factorization: the panels are there).
 template <typename T>
 __global__ __launch_bounds__(256) void k_diag_writeback(const PanelDesc *__restrict__ descs,
                                                         const NodeInfo *__restrict__ nodes,
-                                                        T *__restrict__ pool,
+                                                        T *__restrict__ store,
                                                         const T *__restrict__ dscr)
 {
     const PanelDesc pd = descs[blockIdx.x];
     const NodeInfo n = nodes[pd.node];
-    T *fr = pool + n.foff;
+    T *fr = store + n.loff;
     const i64 ld = n.ld;
     for (int idx = threadIdx.x; idx < pd.nb * pd.nb; idx += 256) {
         const int r = idx % pd.nb, c = idx / pd.nb;
@@ -1539,10 +1540,36 @@ __global__ void k_stats_final(double *part, int nparts)
 // ---------------------------------------------------------------------------
 // copy the factor into the packed SM layout
 // ---------------------------------------------------------------------------
+// A level's factored panels (rows c..F-1 of the columns c < S) from the
+// fronts to the factor store, where the solves read them after the fronts'
+// memory is reused.
+template <typename T>
+__global__ __launch_bounds__(256) void k_store_panel(const ColTask *__restrict__ tasks,
+                                                     const NodeInfo *__restrict__ nodes,
+                                                     const T *__restrict__ pool,
+                                                     T *__restrict__ store)
+{
+    const ColTask t = tasks[blockIdx.x];
+    const NodeInfo n = nodes[t.node];
+    for (int c = t.c0; c < t.c1; ++c) {
+        const T *src = pool + n.foff + i64(c) * n.ld;
+        T *dst = store + n.loff + i64(c) * n.ld;
+        int r = c + threadIdx.x;
+        for (; r + 3 * 256 < n.F; r += 4 * 256) {
+            T v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = src[r + 256 * u];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) dst[r + 256 * u] = v[u];
+        }
+        for (; r < n.F; r += 256) dst[r] = src[r];
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_copy_out(const ColTask *__restrict__ tasks,
                                                   const NodeInfo *__restrict__ nodes,
-                                                  const T *__restrict__ pool, double *sm,
+                                                  const T *__restrict__ store, double *sm,
                                                   i64 neq, i32 node_limit)
 {
     const ColTask t = tasks[blockIdx.x];
@@ -1551,7 +1578,7 @@ __global__ __launch_bounds__(256) void k_copy_out(const ColTask *__restrict__ ta
     double *lnz = sm + neq;
     for (int c = t.c0; c < t.c1 && c < n.S; ++c) {
         double *dst = lnz + n.xlnz + i64(c) * n.F - (i64(c) * (c - 1)) / 2;
-        const T *col = pool + n.foff + i64(c) * n.ld;
+        const T *col = store + n.loff + i64(c) * n.ld;
         for (int r = c + threadIdx.x; r < n.F; r += blockDim.x) dst[r - c] = double(col[r]);
     }
 }
@@ -1559,7 +1586,7 @@ __global__ __launch_bounds__(256) void k_copy_out(const ColTask *__restrict__ ta
 template <typename T>
 __global__ __launch_bounds__(256) void k_load_factor(const ColTask *__restrict__ tasks,
                                                      const NodeInfo *__restrict__ nodes,
-                                                     T *__restrict__ pool,
+                                                     T *__restrict__ store,
                                                      const double *__restrict__ sm, i64 neq)
 {
     const ColTask t = tasks[blockIdx.x];
@@ -1567,7 +1594,7 @@ __global__ __launch_bounds__(256) void k_load_factor(const ColTask *__restrict__
     const double *lnz = sm + neq;
     for (int c = t.c0; c < t.c1 && c < n.S; ++c) {
         const double *src = lnz + n.xlnz + i64(c) * n.F - (i64(c) * (c - 1)) / 2;
-        T *col = pool + n.foff + i64(c) * n.ld;
+        T *col = store + n.loff + i64(c) * n.ld;
         for (int r = c + threadIdx.x; r < n.F; r += blockDim.x) col[r] = T(src[r - c]);
     }
 }
@@ -1678,6 +1705,11 @@ static int factor_impl(sprgpu_handle *h, const double *sm_dev, int mode, const d
                                                   static_cast<const T *>(d.dscr));
             mark(P_PROWS, 0.0);
             break;
+        case L_STORE:
+            k_store_panel<T><<<blocks, 256, 0, ls>>>(d.copy_tasks + L.off, d.nodes, pool,
+                                                     static_cast<T *>(d.store));
+            mark(P_OTHER, 0.0);
+            break;
         case L_BFUSED:
             // publication area q of the launch: the descriptor's index in the plan
             k_block_fused<T><<<blocks, 64 * kBdWaves, 0, ls>>>(
@@ -1710,7 +1742,7 @@ static int factor_impl(sprgpu_handle *h, const double *sm_dev, int mode, const d
     }
     if (!p.params.block_factor && !p.panel_desc.empty()) {
         k_diag_writeback<T><<<unsigned(p.panel_desc.size()), 256, 0, s>>>(
-            d.panel_desc, d.nodes, pool, static_cast<const T *>(d.dscr));
+            d.panel_desc, d.nodes, static_cast<T *>(d.store), static_cast<const T *>(d.dscr));
         mark(P_PDIAG, 0.0);
     }
     if (int rc = compute_block_inverses(h)) return rc;
@@ -1838,11 +1870,11 @@ int copy_factor_to_sm(sprgpu_handle *h, double *sm_out)
     if (nt) {
         if (h->precision == SPRGPU_FP32)
             k_copy_out<float><<<nt, 256, 0, h->stream>>>(
-                h->d.copy_tasks, h->d.nodes, static_cast<const float *>(h->d.pool), sm_out,
+                h->d.copy_tasks, h->d.nodes, static_cast<const float *>(h->d.store), sm_out,
                 p.neq, node_limit);
         else
             k_copy_out<double><<<nt, 256, 0, h->stream>>>(
-                h->d.copy_tasks, h->d.nodes, static_cast<const double *>(h->d.pool), sm_out,
+                h->d.copy_tasks, h->d.nodes, static_cast<const double *>(h->d.store), sm_out,
                 p.neq, node_limit);
     }
     k_copy_diag<<<256, 256, 0, h->stream>>>(h->d.dvec, h->d.adiag, sm_out, p.neq, diag_limit);
@@ -1857,10 +1889,10 @@ int load_factor_from_sm(sprgpu_handle *h, const double *sm_dev)
     if (nt) {
         if (h->precision == SPRGPU_FP32)
             k_load_factor<float><<<nt, 256, 0, h->stream>>>(
-                h->d.copy_tasks, h->d.nodes, static_cast<float *>(h->d.pool), sm_dev, p.neq);
+                h->d.copy_tasks, h->d.nodes, static_cast<float *>(h->d.store), sm_dev, p.neq);
         else
             k_load_factor<double><<<nt, 256, 0, h->stream>>>(
-                h->d.copy_tasks, h->d.nodes, static_cast<double *>(h->d.pool), sm_dev, p.neq);
+                h->d.copy_tasks, h->d.nodes, static_cast<double *>(h->d.store), sm_dev, p.neq);
     }
     SPRGPU_HIP(hipMemcpyAsync(h->d.dvec, sm_dev, sizeof(double) * p.neq,
                               hipMemcpyDeviceToDevice, h->stream));

@@ -14,6 +14,7 @@
 // one parent are extend-added in rounds by sibling index so that no two
 // work items of one launch write the same front entry.
 #include <algorithm>
+#include <map>
 #include <numeric>
 #include <thread>
 
@@ -117,21 +118,126 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
         for (i64 j = 0; j < ns; ++j) p.level_nodes[fill[height[j]]++] = i32(j);
     }
 
-    // front pool, vector workspace and child relative maps (SPRF12 logic,
-    // sprfc1.f:716-790: monotone merge of the sorted row lists)
-    i64 pool = 0, vw = 0, rm = 0;
+    // factor store, vector workspace and child relative maps (SPRF12 logic,
+    // sprfc1.f:716-790: monotone merge of the sorted row lists).  The store
+    // holds each supernode's factored panel (its S columns, leading
+    // dimension ld) for the solves.
+    i64 store = 0, vw = 0, rm = 0;
     for (i64 j = 0; j < ns; ++j) {
         NodeInfo &n = p.nodes[j];
         n.ld = i32(round_up(n.F, kFrontLdAlign));
-        n.foff = pool;
-        pool += round_up(i64(n.ld) * n.F, 64);
+        n.loff = store;
+        store += round_up(i64(n.ld) * n.S, 64);
         n.voff = vw;
         vw += n.F;
         n.mapoff = rm;
         rm += n.F - n.S;
     }
-    p.pool_elems = pool;
+    p.store_elems = store;
     p.vwork_elems = vw;
+    // Front pool: a front (ld x F) lives from its level's initialisation to
+    // its parent's extend-add, in the parent's level; its panel goes to the
+    // store at the end of its level (on the second stream, which the next
+    // level's end waits for: a root's memory lives one level more).  Two
+    // placements are made and the smaller kept:
+    //  - level by level, the level's fronts (those living longest first,
+    //    then the largest) by first fit over the free ranges, after which the
+    //    fronts whose parents are in the level are released;
+    //  - the kPoolExactFronts largest fronts first, largest first, each at
+    //    the lowest offset that no placed front overlaps during its lifetime,
+    //    and the others above them as in the first placement.
+    // (The first reaches the most front memory alive at once on 2-D grids,
+    // the second on 3-D ones, whose large separators live long.)
+    {
+        auto fsize = [&](i64 j) { return round_up(i64(p.nodes[j].ld) * p.nodes[j].F, 64); };
+        // the last level of a front's memory: its parent's level (a root:
+        // the level after its own, as its panel is stored during that level)
+        auto last = [&](i64 j) { return parent[j] >= 0 ? height[parent[j]] : height[j] + 1; };
+        // first fit, level by level, over the fronts with skip[j] == 0;
+        // offsets from base; returns the top
+        auto by_levels = [&](const std::vector<char> &skip, i64 base, std::vector<i64> &off) {
+            std::vector<std::vector<i32>> die(p.nlevels + 1);
+            for (i64 j = 0; j < ns; ++j)
+                if (!skip[j]) die[last(j)].push_back(i32(j));
+            std::map<i64, i64> freel;  // offset -> length of the free ranges below top
+            i64 top = 0;
+            auto take = [&](i64 len) {
+                for (auto it = freel.begin(); it != freel.end(); ++it) {
+                    if (it->second < len) continue;
+                    const i64 o = it->first, rest = it->second - len;
+                    freel.erase(it);
+                    if (rest > 0) freel[o + len] = rest;
+                    return o;
+                }
+                const i64 o = top;
+                top += len;
+                return o;
+            };
+            auto give = [&](i64 o, i64 len) {
+                auto nx = freel.lower_bound(o);
+                if (nx != freel.end() && nx->first == o + len) {
+                    len += nx->second;
+                    freel.erase(nx);
+                }
+                auto pv = freel.lower_bound(o);
+                if (pv != freel.begin()) {
+                    --pv;
+                    if (pv->first + pv->second == o) {
+                        pv->second += len;
+                        return;
+                    }
+                }
+                freel[o] = len;
+            };
+            for (int l = 0; l < p.nlevels; ++l) {
+                std::vector<i32> v;
+                for (i32 q = p.level_ptr[l]; q < p.level_ptr[l + 1]; ++q)
+                    if (!skip[p.level_nodes[q]]) v.push_back(p.level_nodes[q]);
+                std::stable_sort(v.begin(), v.end(), [&](i32 a, i32 b) {
+                    return last(a) != last(b) ? last(a) > last(b) : fsize(a) > fsize(b);
+                });
+                for (i32 j : v) off[j] = base + take(fsize(j));
+                for (i32 j : die[l]) give(off[j] - base, fsize(j));
+            }
+            return top;
+        };
+        std::vector<i64> off1(ns), off2(ns);
+        const i64 top1 = by_levels(std::vector<char>(ns, 0), 0, off1);
+        std::vector<i32> order(ns);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](i32 a, i32 b) { return fsize(a) > fsize(b); });
+        const i64 nbig = std::min<i64>(ns, kPoolExactFronts);
+        std::vector<char> big(ns, 0);
+        struct Placed {
+            i64 off, end;
+            i32 a, b;  // levels of the lifetime
+        };
+        std::vector<Placed> placed;  // ascending offset
+        placed.reserve(nbig);
+        i64 topb = 0;
+        for (i64 q = 0; q < nbig; ++q) {
+            const i32 j = order[q], a = height[j], b = last(j);
+            const i64 len = fsize(j);
+            i64 cand = 0;  // scanning by offset: past every overlapping front that starts below
+            for (const Placed &u : placed) {
+                if (u.b < a || b < u.a) continue;
+                if (u.off >= cand + len) break;
+                cand = std::max(cand, u.end);
+            }
+            const Placed np{cand, cand + len, a, b};
+            placed.insert(std::upper_bound(placed.begin(), placed.end(), np,
+                                           [](const Placed &x, const Placed &y) { return x.off < y.off; }),
+                          np);
+            off2[j] = cand;
+            topb = std::max(topb, cand + len);
+            big[j] = 1;
+        }
+        const i64 top2 = topb + by_levels(big, topb, off2);
+        const std::vector<i64> &off = top1 <= top2 ? off1 : off2;
+        for (i64 j = 0; j < ns; ++j) p.nodes[j].foff = off[j];
+        p.pool_elems = std::min(top1, top2);
+    }
     p.relmap.assign(rm, -1);
     for (i64 c = 0; c < ns; ++c) {
         const NodeInfo &n = p.nodes[c];
@@ -162,9 +268,15 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
     // ----------------------------------------------------------------------
     p.init_tasks.clear();
     p.copy_tasks.clear();
-    for (i64 j = 0; j < ns; ++j)
-        for (i32 c = 0; c < p.nodes[j].S; c += kInitCols)
-            p.copy_tasks.push_back(ColTask{i32(j), c, std::min(p.nodes[j].S, c + kInitCols), 0});
+    p.copy_ptr.assign(p.nlevels + 1, 0);
+    for (int l = 0; l < p.nlevels; ++l) {
+        for (i32 q = p.level_ptr[l]; q < p.level_ptr[l + 1]; ++q) {
+            const i32 j = p.level_nodes[q];
+            for (i32 c = 0; c < p.nodes[j].S; c += kInitCols)
+                p.copy_tasks.push_back(ColTask{j, c, std::min(p.nodes[j].S, c + kInitCols), 0});
+        }
+        p.copy_ptr[l + 1] = i64(p.copy_tasks.size());
+    }
     p.ea_tasks.clear();
     p.small_nodes.clear();
     p.panel_desc.clear();
@@ -457,6 +569,16 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
             fl += gemm_flops(g);
         }
         add_launch(L_GEMM, l, off, i64(p.gemm_desc.size()) - off, ntiles, fl, stile, 1);
+        // 5. the level's factored panels to the store, on the second stream
+        //    beside the next level; the handle's stream waits for it at the
+        //    end of the next level, before any front memory of this level
+        //    is reused (two levels on at the earliest)
+        add_sync(L_WAIT, l, 0);
+        add_sync(L_RECORD, l, 0);
+        add_sync(L_WAIT, l, 1);
+        add_launch(L_STORE, l, p.copy_ptr[l], p.copy_ptr[l + 1] - p.copy_ptr[l],
+                   p.copy_ptr[l + 1] - p.copy_ptr[l], 0.0, 0, 0, 1);
+        add_sync(L_RECORD, l, 1);
     }
 
     // ----------------------------------------------------------------------

@@ -32,6 +32,7 @@ constexpr int kTrsmWaves = 4;    // ... and wavefronts per workgroup
 // Elements of T per outer block that the fused outer-block launch hands from
 // the diagonal workgroup to the rows below: Bm of the four sub-steps and
 // d_k L(r, k) of the block's rows past the first sub-step.
+constexpr int kPoolExactFronts = 16384;  // largest fronts placed in the pool by lifetime overlap
 constexpr int kFusedMaxFronts = 64;  // outer-block launches of at most this many fronts are fused
 constexpr int kPubElems = 4 * kPanelNB * kPanelNB + (kPanelOuter - kPanelNB) * (kPanelOuter - kPanelNB);
 constexpr int kSmallF = 64;      // fronts with F <= kSmallF can be factored in LDS (one lane a row)
@@ -67,7 +68,8 @@ struct WideDesc {
 // LINDX list (sorted ascending, the S own columns first); the front is a dense
 // column-major F x F block (lower triangle used) with leading dimension ld.
 struct NodeInfo {
-    i64 foff;      // offset of the front in the pool (elements)
+    i64 foff;      // offset of the front in the pool (elements), while it is alive
+    i64 loff;      // offset of the factored panel (ld x S) in the factor store
     i64 lxoff;     // offset of the row list in lindx
     i64 xlnz;      // offset of the packed trapezoid in LNZ (0-based)
     i64 voff;      // offset of the node's vector slot in the solve workspace
@@ -109,6 +111,7 @@ enum LaunchKind : i32 {
     L_RECORD = 7,  // no kernel: record the event of stream `stream`
     L_WAIT = 8,    // no kernel: stream `stream` waits for the other stream's event
     L_BFUSED = 9,  // outer-block diagonal factor and the rows below it in one launch
+    L_STORE = 10,  // a level's factored panels from the pool to the factor store
 };
 
 // A launch of one kernel over a contiguous range of a task array.
@@ -138,7 +141,7 @@ struct PlanParams {
 struct Plan {
     PlanParams params;
     i64 neq = 0, nsuper = 0, nofsub = 0, nzerol = 0, mfront = 0, maxsup = 0;
-    i64 pool_elems = 0, vwork_elems = 0;
+    i64 pool_elems = 0, store_elems = 0, vwork_elems = 0;
     i32 nlevels = 0;
     i32 big_tile = 0;               // configured tile of the rank-kOuterRank and Schur updates
                                     // (0: per launch, Launch::tile)
@@ -150,7 +153,8 @@ struct Plan {
     std::vector<i32> level_ptr, level_nodes;  // nodes grouped by height
     // factor schedule
     std::vector<ColTask> init_tasks;   // front initialisation of the blocked fronts
-    std::vector<ColTask> copy_tasks;   // the S columns of every supernode (factor <-> SM)
+    std::vector<ColTask> copy_tasks;   // the S columns of every supernode (factor <-> SM), by level
+    std::vector<i64> copy_ptr;         // nlevels+1: each level's range of copy_tasks
     std::vector<EaTask> ea_tasks;
     std::vector<i32> small_nodes;
     std::vector<PanelDesc> panel_desc;  // panel steps, or outer blocks (k0 = ob, nb = width,
@@ -219,7 +223,8 @@ struct DeviceBuffers {
     i64 *panel_prefix = nullptr;
     GemmDesc *gemm_desc = nullptr;
     i64 *gemm_prefix = nullptr;
-    void *pool = nullptr;       // T[pool_elems]
+    void *pool = nullptr;       // T[pool_elems]: the fronts being factored
+    void *store = nullptr;      // T[store_elems]: the factored panels
     double *dvec = nullptr;     // pivots d (NEQ), FP64
     double *adiag = nullptr;    // a_jj of the assembled matrix (NEQ)
     double *dcor = nullptr;     // DCOR(2,NEQ) for SPRFC3 mode

@@ -156,7 +156,7 @@ __global__ __launch_bounds__(256) void k_fwd_narrow(const i32 *__restrict__ list
                                                     const i32 *__restrict__ lindx,
                                                     const i64 *__restrict__ upd_ptr,
                                                     const i32 *__restrict__ upd_src,
-                                                    const T *__restrict__ pool,
+                                                    const T *__restrict__ store,
                                                     const double *__restrict__ dvec, double *b,
                                                     i64 ldb, double *vwork, i64 vstride, int nrhs)
 {
@@ -170,7 +170,7 @@ __global__ __launch_bounds__(256) void k_fwd_narrow(const i32 *__restrict__ list
     const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
     const int S = n.S, F = n.F;
     const i32 *rows = lindx + n.lxoff;
-    const T *L = pool + n.foff;
+    const T *L = store + n.loff;
     const i64 ld = n.ld;
     const i64 *up = upd_ptr + n.voff;
     for (int i = tid; i < F; i += nt) {
@@ -251,7 +251,7 @@ template <typename T, int R>
 __global__ __launch_bounds__(256) void k_bwd_narrow(const i32 *__restrict__ list, int maxF,
                                                     const NodeInfo *__restrict__ nodes,
                                                     const i32 *__restrict__ lindx,
-                                                    const T *__restrict__ pool, double *b,
+                                                    const T *__restrict__ store, double *b,
                                                     i64 ldb, int nrhs)
 {
     extern __shared__ double smem[];
@@ -265,7 +265,7 @@ __global__ __launch_bounds__(256) void k_bwd_narrow(const i32 *__restrict__ list
     const int rhs0 = int(blockIdx.x % nrb) * R;
     const int S = n.S, F = n.F;
     const i32 *rows = lindx + n.lxoff;
-    const T *L = pool + n.foff;
+    const T *L = store + n.loff;
     const i64 ld = n.ld;
     for (int i = tid; i < F + 16; i += nt) {
         const i32 gi = (i < F) ? rows[i] : 0;
@@ -490,7 +490,7 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
                                                   const i32 *__restrict__ lindx,
                                                   const i64 *__restrict__ upd_ptr,
                                                   const i32 *__restrict__ upd_src,
-                                                  const T *__restrict__ pool,
+                                                  const T *__restrict__ store,
                                                   const double *__restrict__ dvec, double *b,
                                                   i64 ldb, double *vwork, i64 vstride, int *err,
                                                   int nrhs, const T *__restrict__ dinv)
@@ -509,7 +509,7 @@ __global__ __launch_bounds__(256) void k_fwd_wide(const WideDesc *__restrict__ w
     const int tid = threadIdx.x;
     const int S = n.S, F = n.F;
     const i32 *rows = lindx + n.lxoff;
-    const T *L = pool + n.foff;
+    const T *L = store + n.loff;
     const i64 ld = n.ld;
     const i64 *up = upd_ptr + n.voff;
     i64 vo[R];  // this group's vectors of the supernode (the last valid one past nrhs)
@@ -693,7 +693,7 @@ __global__ __launch_bounds__(256) void k_fwd_rect(const WideDesc *__restrict__ w
                                                   const NodeInfo *__restrict__ nodes,
                                                   const i64 *__restrict__ upd_ptr,
                                                   const i32 *__restrict__ upd_src,
-                                                  const T *__restrict__ pool, double *vwork,
+                                                  const T *__restrict__ store, double *vwork,
                                                   i64 vstride, int nrhs)
 {
     __shared__ double xs[2][R][kXsLd];
@@ -706,7 +706,7 @@ __global__ __launch_bounds__(256) void k_fwd_rect(const WideDesc *__restrict__ w
     const NodeInfo n = nodes[w.node];
     const int tid = threadIdx.x;
     const int S = n.S, F = n.F;
-    const T *L = pool + n.foff;
+    const T *L = store + n.loff;
     const i64 ld = n.ld;
     const i64 *up = upd_ptr + n.voff;
     const int r0 = S + chunk * NB;
@@ -767,7 +767,7 @@ __global__ __launch_bounds__(256) void k_bwd_rect(const WideDesc *__restrict__ w
                                                   const i64 *__restrict__ prefix,
                                                   const NodeInfo *__restrict__ nodes,
                                                   const i32 *__restrict__ lindx,
-                                                  const T *__restrict__ pool,
+                                                  const T *__restrict__ store,
                                                   const double *__restrict__ b, i64 ldb,
                                                   double *partials, i64 npart, int nrhs)
 {
@@ -786,7 +786,7 @@ __global__ __launch_bounds__(256) void k_bwd_rect(const WideDesc *__restrict__ w
     const int S = n.S, F = n.F;
     const int c0 = blk * NB, nc = min(NB, S - c0);
     const int r0 = S + k * NB, nr = min(NB, F - r0);
-    const T *L = pool + n.foff;
+    const T *L = store + n.loff;
     const i64 ld = n.ld;
     if (tid < NB) {
         const i32 gi = lindx[n.lxoff + r0 + min(tid, nr - 1)];
@@ -859,26 +859,26 @@ static_assert(kMmCols == 64 && NB % kMmCols == 0, "four wavefronts of 16, whole 
 
 typedef double mm_acc_t __attribute__((ext_vector_type(4)));
 
-// The stage loop shared by both kernels: load(k0) issues the global loads of
-// the stage at k0 into registers, store(buf, k0) writes them to LDS.  The
+// The stage loop shared by both kernels: fetch(k0) issues the global loads
+// of the stage at k0 into registers, put(buf, k0) writes them to LDS.  The
 // accumulator element e of tile t of a lane is C(16t + lane/16 + 4e, 16w +
 // lane%16) (FP64 16x16x4: A[i][k] from lane i + 16k, B[k][j] from lane
 // j + 16k, element e of row lane/16 + 4e).
-template <typename LoadF, typename StoreF>
+template <typename FetchF, typename PutF>
 __device__ __forceinline__ void mm_stages(double (*As)[kMmK][kMmLd], double (*Bs)[kMmK][kMmLd],
-                                          int nst, LoadF &&load, StoreF &&store,
+                                          int nst, FetchF &&fetch, PutF &&put,
                                           mm_acc_t (&acc)[kMmTiles])
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int t = 0; t < kMmTiles; ++t) acc[t] = mm_acc_t{0.0, 0.0, 0.0, 0.0};
-    load(0);
-    store(0, 0);
+    fetch(0);
+    put(0, 0);
     __syncthreads();
     for (int s = 0; s < nst; ++s) {
         const int buf = s & 1;
         const bool more = s + 1 < nst;
-        if (more) load((s + 1) * kMmK);
+        if (more) fetch((s + 1) * kMmK);
 #pragma unroll
         for (int kk = 0; kk < kMmK / 4; ++kk) {
             const int k = 4 * kk + (lane >> 4);
@@ -888,7 +888,7 @@ __device__ __forceinline__ void mm_stages(double (*As)[kMmK][kMmLd], double (*Bs
                 acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(As[buf][k][16 * t + (lane & 15)], bv,
                                                               acc[t], 0, 0, 0);
         }
-        if (more) store(buf ^ 1, (s + 1) * kMmK);
+        if (more) put(buf ^ 1, (s + 1) * kMmK);
         __syncthreads();
     }
 }
@@ -907,7 +907,7 @@ __global__ __launch_bounds__(256) void k_fwd_rect_mm(const WideDesc *__restrict_
                                                      const NodeInfo *__restrict__ nodes,
                                                      const i64 *__restrict__ upd_ptr,
                                                      const i32 *__restrict__ upd_src,
-                                                     const T *__restrict__ pool, double *vwork,
+                                                     const T *__restrict__ store, double *vwork,
                                                      i64 vstride, int nrhs)
 {
     __shared__ double As[2][kMmK][kMmLd];
@@ -922,13 +922,13 @@ __global__ __launch_bounds__(256) void k_fwd_rect_mm(const WideDesc *__restrict_
     const int S = n.S, F = n.F;
     const int r0 = S + int(cid - prefix[q]) * NB + int(bid % (NB / kMmCols)) * kMmCols;
     if (r0 >= F) return;  // the whole workgroup: a chunk's second half past F
-    const T *L = pool + n.foff;
+    const T *L = store + n.loff;
     const i64 ld = n.ld;
     const int tid = threadIdx.x;
     const int sc = tid >> 3, sr = 8 * (tid & 7), sk = 4 * (tid & 7);
     Rows4<T> lv[2];
     double xv[2][4];
-    auto load = [&](int k0) {
+    auto fetch = [&](int k0) {
         const T *Lc = L + i64(min(k0 + sc, S - 1)) * ld;
         lv[0] = load_rows4(Lc, r0 + sr, F);
         lv[1] = load_rows4(Lc, r0 + sr + 4, F);
@@ -939,7 +939,7 @@ __global__ __launch_bounds__(256) void k_fwd_rect_mm(const WideDesc *__restrict_
             for (int i = 0; i < 4; ++i) xv[h][i] = xp[min(k0 + sk + i, S - 1)];
         }
     };
-    auto store = [&](int buf, int k0) {
+    auto put = [&](int buf, int k0) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             Bs[buf][sc][sr + i] = double(lv[0].v[i]);
@@ -953,7 +953,7 @@ __global__ __launch_bounds__(256) void k_fwd_rect_mm(const WideDesc *__restrict_
                     (k0 + sk + i < S && rhs0 + sc + 32 * h < nrhs) ? xv[h][i] : 0.0;
     };
     mm_acc_t acc[kMmTiles];
-    mm_stages(As, Bs, (S + kMmK - 1) / kMmK, load, store, acc);
+    mm_stages(As, Bs, (S + kMmK - 1) / kMmK, fetch, put, acc);
     const int lane = tid & 63;
     const int r = r0 + 16 * (tid >> 6) + (lane & 15);
     if (r < F) {
@@ -987,7 +987,7 @@ __global__ __launch_bounds__(256) void k_bwd_rect_mm(const WideDesc *__restrict_
                                                      const i64 *__restrict__ prefix,
                                                      const NodeInfo *__restrict__ nodes,
                                                      const i32 *__restrict__ lindx,
-                                                     const T *__restrict__ pool,
+                                                     const T *__restrict__ store,
                                                      const double *__restrict__ b, i64 ldb,
                                                      double *partials, i64 npart, int nrhs)
 {
@@ -1007,7 +1007,7 @@ __global__ __launch_bounds__(256) void k_bwd_rect_mm(const WideDesc *__restrict_
     const int c0 = blk * NB + half * kMmCols, nc = min(kMmCols, S - c0);
     if (nc <= 0) return;  // the whole workgroup: a block's second half past S
     const int r0 = S + kch * NB, nr = min(NB, F - r0);
-    const T *L = pool + n.foff;
+    const T *L = store + n.loff;
     const i64 ld = n.ld;
     const int tid = threadIdx.x;
     const int lc = tid >> 2, lr = 8 * (tid & 3);
@@ -1016,7 +1016,7 @@ __global__ __launch_bounds__(256) void k_bwd_rect_mm(const WideDesc *__restrict_
     const i32 *rows = lindx + n.lxoff + r0;
     Rows4<T> lv[2];
     double xv[2][4];
-    auto load = [&](int k0) {
+    auto fetch = [&](int k0) {
         lv[0] = load_rows4(Lc, k0 + lr, F - r0);
         lv[1] = load_rows4(Lc, k0 + lr + 4, F - r0);
 #pragma unroll
@@ -1027,7 +1027,7 @@ __global__ __launch_bounds__(256) void k_bwd_rect_mm(const WideDesc *__restrict_
                 xv[h][i] = b[i64(min(rhs0 + xc + 32 * h, nrhs - 1)) * ldb + gi];
         }
     };
-    auto store = [&](int buf, int k0) {
+    auto put = [&](int buf, int k0) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {  // rows past the chunk may be unwritten memory
             Bs[buf][lr + i][lc] = (k0 + lr + i < nr) ? double(lv[0].v[i]) : 0.0;
@@ -1041,7 +1041,7 @@ __global__ __launch_bounds__(256) void k_bwd_rect_mm(const WideDesc *__restrict_
                     (k0 + xk + i < nr && rhs0 + xc + 32 * h < nrhs) ? xv[h][i] : 0.0;
     };
     mm_acc_t acc[kMmTiles];
-    mm_stages(As, Bs, (nr + kMmK - 1) / kMmK, load, store, acc);
+    mm_stages(As, Bs, (nr + kMmK - 1) / kMmK, fetch, put, acc);
     const int lane = tid & 63;
     const int c = 16 * (tid >> 6) + (lane & 15);
     if (c < nc) {
@@ -1068,7 +1068,7 @@ __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ w
                                                   const i64 *__restrict__ prefix,
                                                   const NodeInfo *__restrict__ nodes,
                                                   const i32 *__restrict__ lindx,
-                                                  const T *__restrict__ pool, double *b, i64 ldb,
+                                                  const T *__restrict__ store, double *b, i64 ldb,
                                                   double *vwork, i64 vstride,
                                                   const double *__restrict__ partials, i64 npart,
                                                   int *err, int nrhs, const T *__restrict__ dinv)
@@ -1087,7 +1087,7 @@ __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ w
     const int tid = threadIdx.x;
     const int S = n.S, F = n.F;
     const i32 *rows = lindx + n.lxoff;
-    const T *L = pool + n.foff;
+    const T *L = store + n.loff;
     const i64 ld = n.ld;
     i64 vo[R];
 #pragma unroll
@@ -1247,7 +1247,7 @@ __global__ __launch_bounds__(256) void k_bwd_wide(const WideDesc *__restrict__ w
 template <typename T>
 __global__ __launch_bounds__(256) void k_block_inverse(const WideDesc *__restrict__ wide, int nwide,
                                                        const NodeInfo *__restrict__ nodes,
-                                                       const T *__restrict__ pool, T *dinv)
+                                                       const T *__restrict__ store, T *dinv)
 {
     __shared__ double X[NB][NB + 1];  // X[c][r]; odd stride: a column walk is conflict-free
     __shared__ double Lii[32][33];    // the block row's diagonal 32 x 32 block of L
@@ -1264,7 +1264,7 @@ __global__ __launch_bounds__(256) void k_block_inverse(const WideDesc *__restric
     const WideDesc w = wide[lo];
     const int blk = int(b - w.flag0);
     const NodeInfo n = nodes[w.node];
-    const T *L = pool + n.foff;
+    const T *L = store + n.loff;
     const i64 ld = n.ld;
     const int c0 = blk * NB, nc = min(NB, n.S - c0);
     const int tid = threadIdx.x;
@@ -1486,13 +1486,13 @@ static void fwd_narrow_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nr
 {
     const Plan &p = h->plan;
     DeviceBuffers &d = h->d;
-    const T *pool = static_cast<const T *>(d.pool);
+    const T *store = static_cast<const T *>(d.store);
     const unsigned groups = unsigned((nrhs + R - 1) / R);
     const unsigned ns = unsigned(p.solve_small_ptr[l + 1] - p.solve_small_ptr[l]);
     const int mf = p.small_maxF[l], ms = p.small_maxS[l];
     const unsigned nt = narrow_threads(p, l);
     k_fwd_narrow<T, R><<<ns * groups, nt, fwd_narrow_lds<T>(R, mf, ms, nt), h->stream>>>(
-        d.solve_small + p.solve_small_ptr[l], mf, d.nodes, d.lindx, d.upd_ptr, d.upd_src, pool,
+        d.solve_small + p.solve_small_ptr[l], mf, d.nodes, d.lindx, d.upd_ptr, d.upd_src, store,
         d.dvec, b, ldb, d.vwork, p.vwork_elems, int(nrhs));
 }
 
@@ -1503,13 +1503,13 @@ static void fwd_wide_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs
 {
     const Plan &p = h->plan;
     DeviceBuffers &d = h->d;
-    const T *pool = static_cast<const T *>(d.pool);
+    const T *store = static_cast<const T *>(d.store);
     const unsigned groups = unsigned((nrhs + R - 1) / R);
     const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
     const i64 wgs = fused ? p.wide_fwd_wgs[l] : p.wide_bwd_wgs[l];
     const i64 *prefix = fused ? d.wide_fwd_prefix : d.wide_bwd_prefix;  // all blocks, or the chains'
     k_fwd_wide<T, R><<<unsigned(wgs) * groups, 256, 0, h->stream>>>(
-        d.wide + w0, nw, prefix + w0, d.nodes, d.lindx, d.upd_ptr, d.upd_src, pool, d.dvec, b,
+        d.wide + w0, nw, prefix + w0, d.nodes, d.lindx, d.upd_ptr, d.upd_src, store, d.dvec, b,
         ldb, d.vwork, p.vwork_elems, d.spin_error, int(nrhs), block_inverses<T>(h));
 }
 
@@ -1523,7 +1523,7 @@ static void fwd_rect_level(sprgpu_handle *h, int l, i64 nrhs)
     const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
     k_fwd_rect<T, R><<<unsigned(p.wide_frect_wgs[l]) * groups, 256, 0, h->stream>>>(
         d.wide + w0, nw, d.wide_frect_prefix + w0, d.nodes, d.upd_ptr, d.upd_src,
-        static_cast<const T *>(d.pool), d.vwork, p.vwork_elems, int(nrhs));
+        static_cast<const T *>(d.store), d.vwork, p.vwork_elems, int(nrhs));
 }
 
 template <typename T, int R>
@@ -1531,13 +1531,13 @@ static void bwd_narrow_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nr
 {
     const Plan &p = h->plan;
     DeviceBuffers &d = h->d;
-    const T *pool = static_cast<const T *>(d.pool);
+    const T *store = static_cast<const T *>(d.store);
     const unsigned groups = unsigned((nrhs + R - 1) / R);
     const unsigned ns = unsigned(p.solve_small_ptr[l + 1] - p.solve_small_ptr[l]);
     const int mf = p.small_maxF[l], ms = p.small_maxS[l];
     const unsigned nt = narrow_threads(p, l);
     k_bwd_narrow<T, R><<<ns * groups, nt, bwd_narrow_lds<T>(R, mf, ms, nt), h->stream>>>(
-        d.solve_small + p.solve_small_ptr[l], mf, d.nodes, d.lindx, pool, b, ldb, int(nrhs));
+        d.solve_small + p.solve_small_ptr[l], mf, d.nodes, d.lindx, store, b, ldb, int(nrhs));
 }
 
 template <typename T, int R>
@@ -1545,12 +1545,12 @@ static void bwd_wide_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs
 {
     const Plan &p = h->plan;
     DeviceBuffers &d = h->d;
-    const T *pool = static_cast<const T *>(d.pool);
+    const T *store = static_cast<const T *>(d.store);
     const unsigned groups = unsigned((nrhs + R - 1) / R);
     const i64 npart = std::max<i64>(p.npart, 1);
     const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
     k_bwd_wide<T, R><<<unsigned(p.wide_bwd_wgs[l]) * groups, 256, 0, h->stream>>>(
-        d.wide + w0, nw, d.wide_bwd_prefix + w0, d.nodes, d.lindx, pool, b, ldb, d.vwork,
+        d.wide + w0, nw, d.wide_bwd_prefix + w0, d.nodes, d.lindx, store, b, ldb, d.vwork,
         p.vwork_elems, d.partials, npart, d.spin_error, int(nrhs), block_inverses<T>(h));
 }
 
@@ -1563,7 +1563,7 @@ static void bwd_rect_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs
     const unsigned groups = unsigned((nrhs + R - 1) / R);
     const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
     k_bwd_rect<T, R><<<unsigned(p.wide_rect_wgs[l]) * groups, 256, 0, h->stream>>>(
-        d.wide + w0, nw, d.wide_rect_prefix + w0, d.nodes, d.lindx, static_cast<const T *>(d.pool),
+        d.wide + w0, nw, d.wide_rect_prefix + w0, d.nodes, d.lindx, static_cast<const T *>(d.store),
         b, ldb, d.partials, std::max<i64>(p.npart, 1), int(nrhs));
 }
 
@@ -1582,7 +1582,7 @@ static void fwd_rect_mm_level(sprgpu_handle *h, int l, i64 nrhs)
     const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
     k_fwd_rect_mm<T><<<unsigned(p.wide_frect_wgs[l]) * (NB / kMmCols) * groups, 256, 0,
                        h->stream>>>(d.wide + w0, nw, d.wide_frect_prefix + w0, d.nodes, d.upd_ptr,
-                                    d.upd_src, static_cast<const T *>(d.pool), d.vwork,
+                                    d.upd_src, static_cast<const T *>(d.store), d.vwork,
                                     p.vwork_elems, int(nrhs));
 }
 
@@ -1596,7 +1596,7 @@ static void bwd_rect_mm_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 n
     const int w0 = p.wide_ptr[l], nw = p.wide_ptr[l + 1] - w0;
     k_bwd_rect_mm<T><<<unsigned(p.wide_rect_wgs[l]) * (NB / kMmCols) * groups, 256, 0,
                        h->stream>>>(d.wide + w0, nw, d.wide_rect_prefix + w0, d.nodes, d.lindx,
-                                    static_cast<const T *>(d.pool), b, ldb, d.partials,
+                                    static_cast<const T *>(d.store), b, ldb, d.partials,
                                     std::max<i64>(p.npart, 1), int(nrhs));
 }
 
@@ -1690,11 +1690,11 @@ int compute_block_inverses(sprgpu_handle *h)
     if (p.nflags == 0) return 0;
     if (h->precision == SPRGPU_FP32)
         k_block_inverse<float><<<unsigned(p.nflags), 256, 0, h->stream>>>(
-            h->d.wide, int(p.wide.size()), h->d.nodes, static_cast<const float *>(h->d.pool),
+            h->d.wide, int(p.wide.size()), h->d.nodes, static_cast<const float *>(h->d.store),
             static_cast<float *>(h->d.dinv));
     else
         k_block_inverse<double><<<unsigned(p.nflags), 256, 0, h->stream>>>(
-            h->d.wide, int(p.wide.size()), h->d.nodes, static_cast<const double *>(h->d.pool),
+            h->d.wide, int(p.wide.size()), h->d.nodes, static_cast<const double *>(h->d.store),
             static_cast<double *>(h->d.dinv));
     SPRGPU_HIP(hipGetLastError());
     return 0;
