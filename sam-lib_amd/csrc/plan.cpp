@@ -237,6 +237,16 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
         const std::vector<i64> &off = top1 <= top2 ? off1 : off2;
         for (i64 j = 0; j < ns; ++j) p.nodes[j].foff = off[j];
         p.pool_elems = std::min(top1, top2);
+        // check: the fronts alive in each level are disjoint and inside the pool
+        std::vector<std::vector<std::pair<i64, i64>>> alive(p.nlevels + 1);
+        for (i64 j = 0; j < ns; ++j)
+            for (i32 l = height[j]; l <= last(j); ++l) alive[l].push_back({off[j], off[j] + fsize(j)});
+        for (auto &v : alive) {
+            std::sort(v.begin(), v.end());
+            for (size_t k = 0; k < v.size(); ++k)
+                if (v[k].second > p.pool_elems || (k > 0 && v[k].first < v[k - 1].second))
+                    return "front pool placement overlaps (internal error)";
+        }
     }
     p.relmap.assign(rm, -1);
     for (i64 c = 0; c < ns; ++c) {

@@ -67,6 +67,41 @@ def test_plan_is_consistent(kind, n):
     assert out[7] == np.sum(np.where(F > 64, (S + 127) // 128, 0))
 
 
+@pytest.mark.parametrize("kind,n", [("hex8", 14), ("quad4", 60), ("frame", 300)])
+def test_front_pool_holds_only_live_fronts(kind, n):
+    """Fronts live in the pool from their level to their parent's assembly
+    (the factored panels are kept apart); the planner checks that the fronts
+    alive in each level are disjoint (it fails the plan otherwise), and the
+    pool is well below the sum of all fronts (the layout in which every
+    front stays allocated): at most the largest front memory alive in any
+    level plus 25%."""
+    kw = {"seed": 1} if kind == "frame" else {}
+    model, sysm, _ = make_system(kind, n, **kw)
+    out = _summary(sysm)
+    xs, xl = sysm.xsuper(), sysm.xlindx()
+    F, S = np.diff(xl), np.diff(xs)
+    ld = (F + 15) // 16 * 16
+    size = (ld * F + 63) // 64 * 64
+    # heights and parents from the row lists (parent: the supernode of the
+    # first row below the diagonal block)
+    ns = len(F)
+    col_node = np.repeat(np.arange(ns), S)
+    lindx = sysm.lindx() - 1
+    parent = np.array([col_node[lindx[xl[j] - 1 + S[j]]] if F[j] > S[j] else -1
+                       for j in range(ns)])
+    height = np.zeros(ns, dtype=int)
+    for j in range(ns):
+        if parent[j] >= 0:
+            height[parent[j]] = max(height[parent[j]], height[j] + 1)
+    last = np.where(parent >= 0, height[np.maximum(parent, 0)], height + 1)
+    alive = np.zeros(height.max() + 2)
+    for j in range(ns):
+        alive[height[j]:last[j] + 1] += size[j]
+    print(f"{kind} {n}: pool {out[4]:.3e}, live max {alive.max():.3e}, all fronts {size.sum():.3e}")
+    assert alive.max() <= out[4] <= 1.25 * alive.max()
+    assert out[4] < 0.6 * size.sum()
+
+
 def test_plan_rejects_retained_supernodes():
     from sprgpu import _lib
 
