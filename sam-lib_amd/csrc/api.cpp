@@ -36,7 +36,7 @@ static void free_all(sprgpu_handle *h)
 {
     DeviceBuffers &d = h->d;
     void *ptrs[] = {d.nodes,    d.children,   d.lindx,       d.relmap,     d.init_tasks,
-                    d.copy_tasks,
+                    d.copy_tasks, d.store_tasks,
                     d.ea_tasks, d.small_nodes, d.panel_desc, d.panel_prefix, d.gemm_desc,
                     d.gemm_prefix, d.pool,     d.store,      d.dvec,       d.adiag,      d.dcor,
                     d.status,   d.stats,      d.vwork,       d.csr_rowptr, d.csr_col,
@@ -61,6 +61,7 @@ static void free_plan_buffers(sprgpu_handle *h)
     void **ptrs[] = {(void **)&d.nodes,         (void **)&d.children,
                      (void **)&d.lindx,         (void **)&d.relmap,
                      (void **)&d.init_tasks,    (void **)&d.copy_tasks,
+                     (void **)&d.store_tasks,
                      (void **)&d.ea_tasks,      (void **)&d.small_nodes,
                      (void **)&d.panel_desc,    (void **)&d.panel_prefix,
                      (void **)&d.gemm_desc,     (void **)&d.gemm_prefix,
@@ -110,6 +111,7 @@ static int upload_plan(sprgpu_handle *h)
         (rc = upload(d.lindx, p.lindx, B)) || (rc = upload(d.relmap, p.relmap, B)) ||
         (rc = upload(d.init_tasks, p.init_tasks, B)) || (rc = upload(d.ea_tasks, p.ea_tasks, B)) ||
         (rc = upload(d.copy_tasks, p.copy_tasks, B)) ||
+        (rc = upload(d.store_tasks, p.store_tasks, B)) ||
         (rc = upload(d.small_nodes, p.small_nodes, B)) ||
         (rc = upload(d.panel_desc, p.panel_desc, B)) ||
         (rc = upload(d.panel_prefix, p.panel_prefix, B)) ||

@@ -225,8 +225,8 @@ __global__ __launch_bounds__(256) void k_extend_add(const EaTask *__restrict__ t
 // trapezoid from SM (as k_front_init), then the children's update blocks
 // added in SIBL order (as k_extend_add), then eliminates the S columns
 // (SPRF17/LMMY18's recurrence: l = w * (1/d), a_rc -= w_r l_c) and writes the
-// lower triangle back to the pool, where the parent's extend-add and the
-// solves read it.  A workgroup holds `waves` fronts, one per wavefront; the
+// factored panel to the factor store, where the solves read it, and the
+// update block back to the pool, where the parent's extend-add reads it.  A workgroup holds `waves` fronts, one per wavefront; the
 // wavefronts synchronise only among their own lanes.  Dynamic LDS per
 // wavefront: F x (F+1) elements of T (F = the launch's largest front) plus
 // the pivots' a_jj and kSmallF multipliers.
@@ -249,7 +249,8 @@ __global__ __launch_bounds__(256) void k_small(const i32 *__restrict__ list, int
                                                const i32 *__restrict__ children,
                                                const i32 *__restrict__ relmap,
                                                const double *__restrict__ lnz,
-                                               T *__restrict__ pool, double *dvec, double *adiag,
+                                               T *__restrict__ pool, T *__restrict__ store,
+                                               double *dvec, double *adiag,
                                                double *dcor, FactorStatus *st, PivotArgs pa)
 {
     extern __shared__ double smem_small[];
@@ -364,11 +365,12 @@ __global__ __launch_bounds__(256) void k_small(const i32 *__restrict__ list, int
         if (r == j) m[j] = T(d);
         wave_sync();  // lcol is rewritten by the next step
     }
-    // 4. lower triangle back to the pool
-    T *fr = pool + n.foff;
+    // 4. the factored panel (columns < S) to the factor store, the update
+    //    block (the parent's extend-add reads it) back to the pool
+    T *fr = pool + n.foff, *fl = store + n.loff;
 #pragma unroll
     for (int c = 0; c < kSmallF; ++c)
-        if (rv && c <= r) fr[r + i64(c) * n.ld] = m[c];
+        if (rv && c <= r) (c < S ? fl : fr)[r + i64(c) * n.ld] = m[c];
 }
 
 // Reciprocal from the hardware estimate refined by Newton steps (one for
@@ -408,7 +410,8 @@ __device__ __forceinline__ double recip(double d)
 // c = r) go to out[r*rs + c*cs] when out is set, with zeros right of the
 // diagonal if zero_upper; the rows of Bm go to bm_out + r*kPanelNB when
 // bm_out is set (rows r < nb), as agent-scope atomic stores if bm_shared
-// (read by other workgroups of the same launch).  lsh holds kLshSize elements of T and then
+// (read by other workgroups of the same launch); out2, when set, receives
+// the factored rows as out does (the lower triangle only).  lsh holds kLshSize elements of T and then
 // 2 kPanelNB doubles of LDS scratch (16-byte aligned).  Returns the pivot d_r of the thread's row (zero for
 // r >= nb).  Every thread of the workgroup calls it.
 constexpr int kChainThreads = 256;
@@ -420,7 +423,7 @@ __device__ __forceinline__ T diag_block_factor(const T *src, i64 srs, i64 scs, i
                                                const PivotArgs &pa, double *dvec, double *dcor,
                                                FactorStatus *st, T (*Bm)[kPanelNB + 1], T *lsh,
                                                T *out, i64 rs, i64 cs, bool zero_upper, T *bm_out,
-                                               bool bm_shared = false)
+                                               bool bm_shared = false, T *out2 = nullptr)
 {
     constexpr int NB = kPanelNB, Q = 4;
     static_assert(NB * NB == kChainThreads * Q, "four entries of the block per thread");
@@ -598,6 +601,7 @@ __device__ __forceinline__ T diag_block_factor(const T *src, i64 srs, i64 scs, i
         for (int q = 0; q < Q; ++q) {
             const int c = c0 + q;
             if (c <= r || zero_upper) out[r * rs + c * cs] = (c <= r) ? a[q] : T(0);
+            if (out2 && c <= r) out2[r * rs + c * cs] = a[q];
         }
     }
     // Bm[i][c] = X[c][i] / d_c: the threads of row c write column c (zero
@@ -849,11 +853,12 @@ __device__ __forceinline__ void block_diag_body(const PanelDesc &pd, const NodeI
                                                 T *__restrict__ pool, double *dvec,
                                                 const double *__restrict__ adiag, double *dcor,
                                                 FactorStatus *st, const PivotArgs &pa, T *bmscr,
-                                                T *pub, BdLds<T> &S)
+                                                T *pub, T *__restrict__ store, BdLds<T> &S)
 {
     constexpr int NB = kPanelNB, LS = NB + 1;
     typedef typename Mfma<T>::acc_t acc_t;
     T *fr = pool + n.foff + pd.k0 + pd.k0 * n.ld;  // block origin
+    T *fs = store + n.loff + pd.k0 + pd.k0 * n.ld;  // the same in the factor store
     const i64 ld = n.ld;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nbw = pd.nb, nt = (nbw + 15) / 16, ntiles = nt * (nt + 1) / 2;
@@ -900,7 +905,7 @@ __device__ __forceinline__ void block_diag_body(const PanelDesc &pd, const NodeI
             T *bm_out = PUB ? pub + j * NB * NB : bmscr + i64(n.first + pd.k0 + k0) * NB;
             const T dr = diag_block_factor<T>(S.Ls, LS, 1, nb, n.first + pd.k0 + k0, adiag, true,
                                               pa, dvec, dcor, st, S.Bm, S.Lsh, fr + k0 + k0 * ld,
-                                              1, ld, false, bm_out, PUB);
+                                              1, ld, false, bm_out, PUB, fs + k0 + k0 * ld);
             if ((tid & 7) == 0) S.dsh[tid >> 3] = dr;
         }
         __syncthreads();
@@ -924,6 +929,7 @@ __device__ __forceinline__ void block_diag_body(const PanelDesc &pd, const NodeI
                     S.Ls[row * LS + col] = v;
                     if (row < m && col < nb) {
                         fr[(k0 + row) + (k0 + col) * ld] = v;
+                        fs[(k0 + row) + (k0 + col) * ld] = v;
                         if (PUB)  // (row >= NB: a row of a later sub-step)
                             pub_store(pub + kPubBm + (k0 + row - NB) * kPubRows + k0 + col,
                                       S.dsh[col] * v);
@@ -971,7 +977,8 @@ __device__ __forceinline__ void block_trsm_body(const PanelDesc &pd, int sb, con
                                                 T *__restrict__ pool,
                                                 const double *__restrict__ dvec,
                                                 const T *__restrict__ bmscr, const T *pub,
-                                                FactorStatus *st, BtLds<T> &S)
+                                                FactorStatus *st, T *__restrict__ store,
+                                                BtLds<T> &S)
 {
     constexpr int NB = kPanelNB, NJB = kPanelOuter / kPanelNB, LR = bt_ld<T>;
     constexpr int NT = 64 * kBtWaves;
@@ -1082,7 +1089,10 @@ __device__ __forceinline__ void block_trsm_body(const PanelDesc &pd, int sb, con
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int col = 16 * h + kk[e];
-                    if (col < nb) fr[row + i64(k0 + col) * ld] = O[jb][h][e];
+                    if (col < nb) {
+                        fr[row + i64(k0 + col) * ld] = O[jb][h][e];
+                        store[n.loff + row + i64(k0 + col) * ld] = O[jb][h][e];
+                    }
                 }
         }
     }
@@ -1094,12 +1104,12 @@ __global__ __launch_bounds__(64 * kBdWaves) void k_block_diag(const PanelDesc *_
                                                              T *__restrict__ pool, double *dvec,
                                                              const double *__restrict__ adiag,
                                                              double *dcor, FactorStatus *st,
-                                                             PivotArgs pa, T *bmscr)
+                                                             PivotArgs pa, T *bmscr, T *store)
 {
     __shared__ BdLds<T> S;
     const PanelDesc pd = descs[blockIdx.x];
     const NodeInfo n = nodes[pd.node];
-    block_diag_body<T, false>(pd, n, pool, dvec, adiag, dcor, st, pa, bmscr, nullptr, S);
+    block_diag_body<T, false>(pd, n, pool, dvec, adiag, dcor, st, pa, bmscr, nullptr, store, S);
 }
 
 template <typename T>
@@ -1109,14 +1119,15 @@ __global__ __launch_bounds__(64 * kBtWaves) void k_block_trsm(const PanelDesc *_
                                                              const NodeInfo *__restrict__ nodes,
                                                              T *__restrict__ pool,
                                                              const double *__restrict__ dvec,
-                                                             const T *__restrict__ bmscr)
+                                                             const T *__restrict__ bmscr,
+                                                             T *__restrict__ store)
 {
     __shared__ BtLds<T> S;
     const int q = find_desc(prefix, ndesc, blockIdx.x);
     const PanelDesc pd = descs[q];
     const int sb = int(blockIdx.x - prefix[q]) * kBtWaves + (threadIdx.x >> 6);
     const NodeInfo n = nodes[pd.node];
-    block_trsm_body<T, false>(pd, sb, n, pool, dvec, bmscr, nullptr, nullptr, S);
+    block_trsm_body<T, false>(pd, sb, n, pool, dvec, bmscr, nullptr, nullptr, store, S);
 }
 
 // Workgroups [0, ndesc): the diagonal blocks; then the rows below them
@@ -1130,7 +1141,7 @@ __global__ __launch_bounds__(64 * kBdWaves) void k_block_fused(const PanelDesc *
                                                               T *__restrict__ pool, double *dvec,
                                                               const double *__restrict__ adiag,
                                                               double *dcor, FactorStatus *st,
-                                                              PivotArgs pa, T *pub)
+                                                              PivotArgs pa, T *pub, T *store)
 {
     constexpr size_t kBytes = sizeof(BdLds<T>) > sizeof(BtLds<T>) ? sizeof(BdLds<T>) : sizeof(BtLds<T>);
     __shared__ __attribute__((aligned(16))) char lds[kBytes];
@@ -1138,7 +1149,8 @@ __global__ __launch_bounds__(64 * kBdWaves) void k_block_fused(const PanelDesc *
         const PanelDesc pd = descs[blockIdx.x];
         const NodeInfo n = nodes[pd.node];
         block_diag_body<T, true>(pd, n, pool, dvec, adiag, dcor, st, pa, nullptr,
-                           pub + i64(blockIdx.x) * kPubElems, *reinterpret_cast<BdLds<T> *>(lds));
+                           pub + i64(blockIdx.x) * kPubElems, store,
+                           *reinterpret_cast<BdLds<T> *>(lds));
     } else {
         const i64 b = i64(blockIdx.x) - ndesc;
         const int q = find_desc(prefix, ndesc, b);
@@ -1146,7 +1158,7 @@ __global__ __launch_bounds__(64 * kBdWaves) void k_block_fused(const PanelDesc *
         const int sb = int(b - prefix[q]) * kBtWaves + (threadIdx.x >> 6);
         const NodeInfo n = nodes[pd.node];
         block_trsm_body<T, true>(pd, sb, n, pool, dvec, nullptr, pub + i64(q) * kPubElems, st,
-                           *reinterpret_cast<BtLds<T> *>(lds));
+                                 store, *reinterpret_cast<BtLds<T> *>(lds));
     }
 }
 
@@ -1681,7 +1693,7 @@ static int factor_impl(sprgpu_handle *h, const double *sm_dev, int mode, const d
             const unsigned nb = unsigned((L.count + waves - 1) / waves);
             k_small<T><<<nb, 64 * waves, per * waves, ls>>>(
                 d.small_nodes + L.off, int(L.count), mf, d.nodes, d.children, d.relmap, lnz, pool,
-                d.dvec, d.adiag, d.dcor, d.status, pa);
+                static_cast<T *>(d.store), d.dvec, d.adiag, d.dcor, d.status, pa);
             mark(P_SMALL, 0.0);
             break;
         }
@@ -1696,17 +1708,18 @@ static int factor_impl(sprgpu_handle *h, const double *sm_dev, int mode, const d
             k_block_diag<T><<<blocks, 64 * kBdWaves, 0, ls>>>(d.panel_desc + L.off, d.nodes, pool,
                                                              d.dvec,
                                                    d.adiag, d.dcor, d.status, pa,
-                                                   static_cast<T *>(d.dscr));
+                                                   static_cast<T *>(d.dscr), static_cast<T *>(d.store));
             mark(P_PDIAG, 0.0);
             break;
         case L_BTRSM:
             k_block_trsm<T><<<blocks, 64 * kBtWaves, 0, ls>>>(d.panel_desc + L.off, int(L.count),
                                                   d.panel_prefix + L.off, d.nodes, pool, d.dvec,
-                                                  static_cast<const T *>(d.dscr));
+                                                  static_cast<const T *>(d.dscr),
+                                                  static_cast<T *>(d.store));
             mark(P_PROWS, 0.0);
             break;
         case L_STORE:
-            k_store_panel<T><<<blocks, 256, 0, ls>>>(d.copy_tasks + L.off, d.nodes, pool,
+            k_store_panel<T><<<blocks, 256, 0, ls>>>(d.store_tasks + L.off, d.nodes, pool,
                                                      static_cast<T *>(d.store));
             mark(P_OTHER, 0.0);
             break;
@@ -1714,7 +1727,8 @@ static int factor_impl(sprgpu_handle *h, const double *sm_dev, int mode, const d
             // publication area q of the launch: the descriptor's index in the plan
             k_block_fused<T><<<blocks, 64 * kBdWaves, 0, ls>>>(
                 d.panel_desc + L.off, int(L.count), d.panel_prefix + L.off, d.nodes, pool, d.dvec,
-                d.adiag, d.dcor, d.status, pa, static_cast<T *>(d.pub) + L.off * kPubElems);
+                d.adiag, d.dcor, d.status, pa, static_cast<T *>(d.pub) + L.off * kPubElems,
+                static_cast<T *>(d.store));
             mark(P_PDIAG, 0.0);
             break;
         case L_GEMM:

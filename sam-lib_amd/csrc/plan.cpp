@@ -278,15 +278,10 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
     // ----------------------------------------------------------------------
     p.init_tasks.clear();
     p.copy_tasks.clear();
-    p.copy_ptr.assign(p.nlevels + 1, 0);
-    for (int l = 0; l < p.nlevels; ++l) {
-        for (i32 q = p.level_ptr[l]; q < p.level_ptr[l + 1]; ++q) {
-            const i32 j = p.level_nodes[q];
-            for (i32 c = 0; c < p.nodes[j].S; c += kInitCols)
-                p.copy_tasks.push_back(ColTask{j, c, std::min(p.nodes[j].S, c + kInitCols), 0});
-        }
-        p.copy_ptr[l + 1] = i64(p.copy_tasks.size());
-    }
+    for (i64 j = 0; j < ns; ++j)
+        for (i32 c = 0; c < p.nodes[j].S; c += kInitCols)
+            p.copy_tasks.push_back(ColTask{i32(j), c, std::min(p.nodes[j].S, c + kInitCols), 0});
+    p.store_tasks.clear();
     p.ea_tasks.clear();
     p.small_nodes.clear();
     p.panel_desc.clear();
@@ -332,6 +327,7 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
         return g;
     };
 
+    bool store_pending = false;  // the previous level's panels are being stored
     for (int l = 0; l < p.nlevels; ++l) {
         const i32 *lv = &p.level_nodes[p.level_ptr[l]];
         const i64 nl = p.level_ptr[l + 1] - p.level_ptr[l];
@@ -579,16 +575,26 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
             fl += gemm_flops(g);
         }
         add_launch(L_GEMM, l, off, i64(p.gemm_desc.size()) - off, ntiles, fl, stile, 1);
-        // 5. the level's factored panels to the store, on the second stream
-        //    beside the next level; the handle's stream waits for it at the
-        //    end of the next level, before any front memory of this level
-        //    is reused (two levels on at the earliest)
-        add_sync(L_WAIT, l, 0);
-        add_sync(L_RECORD, l, 0);
-        add_sync(L_WAIT, l, 1);
-        add_launch(L_STORE, l, p.copy_ptr[l], p.copy_ptr[l + 1] - p.copy_ptr[l],
-                   p.copy_ptr[l + 1] - p.copy_ptr[l], 0.0, 0, 0, 1);
-        add_sync(L_RECORD, l, 1);
+        // 5. with panel steps, the blocked fronts' factored panels to the
+        //    store (the small fronts' and the outer blocks' kernels write
+        //    theirs there as they make them), on the second stream beside
+        //    the next level; the handle's stream waits for it at the end of
+        //    the next level, before any front memory of this level is
+        //    reused (two levels on at the earliest)
+        off = i64(p.store_tasks.size());
+        if (!p.params.block_factor)
+            for (i32 j : big)
+                for (i32 c = 0; c < p.nodes[j].S; c += kInitCols)
+                    p.store_tasks.push_back(ColTask{j, c, std::min(p.nodes[j].S, c + kInitCols), 0});
+        if (store_pending) add_sync(L_WAIT, l, 0);
+        store_pending = i64(p.store_tasks.size()) > off;
+        if (store_pending) {
+            add_sync(L_RECORD, l, 0);
+            add_sync(L_WAIT, l, 1);
+            add_launch(L_STORE, l, off, i64(p.store_tasks.size()) - off,
+                       i64(p.store_tasks.size()) - off, 0.0, 0, 0, 1);
+            add_sync(L_RECORD, l, 1);
+        }
     }
 
     // ----------------------------------------------------------------------

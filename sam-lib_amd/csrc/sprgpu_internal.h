@@ -153,8 +153,8 @@ struct Plan {
     std::vector<i32> level_ptr, level_nodes;  // nodes grouped by height
     // factor schedule
     std::vector<ColTask> init_tasks;   // front initialisation of the blocked fronts
-    std::vector<ColTask> copy_tasks;   // the S columns of every supernode (factor <-> SM), by level
-    std::vector<i64> copy_ptr;         // nlevels+1: each level's range of copy_tasks
+    std::vector<ColTask> copy_tasks;   // the S columns of every supernode (factor <-> SM)
+    std::vector<ColTask> store_tasks;  // the blocked fronts' S columns (pool -> store), by level
     std::vector<EaTask> ea_tasks;
     std::vector<i32> small_nodes;
     std::vector<PanelDesc> panel_desc;  // panel steps, or outer blocks (k0 = ob, nb = width,
@@ -215,7 +215,7 @@ struct FactorStatus {
 struct DeviceBuffers {
     NodeInfo *nodes = nullptr;
     i32 *children = nullptr, *lindx = nullptr, *relmap = nullptr;
-    ColTask *init_tasks = nullptr, *copy_tasks = nullptr;
+    ColTask *init_tasks = nullptr, *copy_tasks = nullptr, *store_tasks = nullptr;
     EaTask *ea_tasks = nullptr;
     i32 *small_nodes = nullptr;
     i32 *level_nodes = nullptr;
