@@ -692,12 +692,14 @@ def bench_frames(args):
     _lib.check(L.sprgpu_autotune(h._h, C.c_void_p(sm_dev.data_ptr()), C.c_void_p(x_dev.data_ptr()),
                                  merged.neq, 1, report.ctypes.data_as(C.c_void_p)), "autotune")
     torch.cuda.synchronize()
-    params = (C.c_int * 5)()
+    params = (C.c_int * 6)()
     L.sprgpu_plan_params(h._h, params)
     ncand = int(report[0])
-    cands = report[1:1 + 5 * ncand].reshape(ncand, 5)
-    log(f"autotune: {ncand} candidates in {time.perf_counter() - t:.1f}s, chose small_f={params[0]} "
-        f"waves={params[1]} narrow={params[2]}; ms best {cands[:, 4].min():.3f} worst {cands[:, 4].max():.3f}")
+    cands = report[1:1 + 6 * ncand].reshape(ncand, 6)
+    level = cands[cands[:, 4] == 0]
+    log(f"autotune: {ncand} candidates in {time.perf_counter() - t:.1f}s, chose forest={params[5]} "
+        f"small_f={params[0]} waves={params[1]} narrow={params[2]}; ms best {cands[:, 5].min():.3f} "
+        f"worst {cands[:, 5].max():.3f}, best level schedule {level[:, 5].min():.3f}")
 
     def run():
         x_dev.copy_(b_dev)
@@ -735,9 +737,11 @@ def bench_frames(args):
                    "neq_total": merged.neq, "nzerol_total": merged.nzerol,
                    "systems": nsys},
         "graph_s": tg, "eager_s": te,
-        "autotune": {"candidates": ncand, "chosen": {"small_f": params[0], "waves": params[1],
+        "autotune": {"candidates": ncand, "chosen": {"forest": params[5], "small_f": params[0],
+                                                     "waves": params[1],
                                                      "narrow_threads": params[2]},
-                     "best_ms": float(cands[:, 4].min()), "worst_ms": float(cands[:, 4].max())},
+                     "best_ms": float(cands[:, 5].min()), "worst_ms": float(cands[:, 5].max()),
+                     "best_level_schedule_ms": float(level[:, 5].min())},
         "cpu_baseline": None,
     }
     if not args.no_cpu:

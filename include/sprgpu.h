@@ -325,22 +325,30 @@ int sprgpu_plan_summary(const sprgpu_int *mspar, const sprgpu_int *mtrees,
  * and nrhs (their contents may change; other pointers capture a new graph).
  * No counterpart in the reference; for callers that repeat factor+solve on
  * new values of one structure (batches of small systems, time stepping).
- * Errors of the factor are read afterwards with sprgpu_factor_info_get. */
+ * Errors of the factor are read afterwards with sprgpu_factor_info_get.
+ * A structure whose fronts all have F <= 64 (a forest of small systems) on
+ * the forest schedule (sprgpu_set_plan_params) with one right-hand side is
+ * factored and solved by one persistent launch. */
 int sprgpu_factor_solve_graph(sprgpu_handle *handle, const double *sm_dev, double *b_dev,
                               sprgpu_int ldb, sprgpu_int nrhs);
 
 /* Chooses the schedule parameters (small-front threshold, fronts per
- * workgroup, narrow-solve workgroup size, FP32 update tile) for this
- * structure by timing factor + solve on sm_dev / b_dev for each candidate,
- * and keeps the fastest.  b_dev is overwritten.  report (NULL or 256 doubles):
- * report[0] = candidates, then per candidate small_f, waves, narrow threads,
- * tile, milliseconds.  No counterpart in the reference. */
+ * workgroup, narrow-solve workgroup size, FP32 update tile, forest schedule)
+ * for this structure by timing factor + solve on sm_dev / b_dev for each
+ * candidate, and keeps the fastest.  b_dev is overwritten.  report (NULL or
+ * 256 doubles): report[0] = candidates, then per candidate small_f, waves,
+ * narrow threads, tile, forest, milliseconds.  No counterpart in the
+ * reference. */
 int sprgpu_autotune(sprgpu_handle *handle, const double *sm_dev, double *b_dev, sprgpu_int ldb,
                     sprgpu_int nrhs, double *report);
-/* The parameters in use: out[0..4] = small_f, fronts per workgroup, narrow
+/* The parameters in use: out[0..5] = small_f, fronts per workgroup, narrow
  * threads (0 = by size), update tile, blocked-front scheme (1 = 128-column
  * outer blocks, the diagonal block and the rows below it in one launch;
- * 2 = the same as two launches; 0 = 32-column panel steps). */
+ * 2 = the same as two launches; 0 = 32-column panel steps), forest schedule
+ * (requested by default: a structure whose fronts all have F <= 64 is
+ * factored by one persistent launch over all supernodes, and its
+ * one-right-hand-side solves too; out[5] = 1 when the schedule is in use, 0
+ * for the level schedule: one launch per tree height and kernel kind). */
 int sprgpu_plan_params(const sprgpu_handle *handle, int *out);
 /* Re-plans with the given parameters, in the order of sprgpu_plan_params
  * (update tile 0 = per launch: 128 for FP32 launches of at least 4096 64-tiles,

@@ -45,7 +45,9 @@ static void free_all(sprgpu_handle *h)
                     d.solve_small, d.wide,     d.wide_fwd_prefix, d.wide_bwd_prefix,
                     d.wide_rect_prefix, d.wide_frect_prefix, d.upd_ptr, d.upd_src, d.partials,
                     d.spin_error, d.col_node, d.prm_count,
-                    d.tmm_rptr, d.tmm_segs, d.tmm_long, d.peri2e, d.permwork};
+                    d.tmm_rptr, d.tmm_segs, d.tmm_long, d.peri2e, d.permwork,
+                    d.forest_nodes, d.forest_child, d.forest_chunks, d.forest_order, d.ublk,
+                    d.forest_sync};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     free_assembly(h);
@@ -71,7 +73,9 @@ static void free_plan_buffers(sprgpu_handle *h)
                      (void **)&d.wide_frect_prefix,
                      (void **)&d.upd_ptr,       (void **)&d.upd_src,
                      (void **)&d.col_node,      (void **)&d.tmm_rptr,
-                     (void **)&d.tmm_segs,      (void **)&d.tmm_long};
+                     (void **)&d.tmm_segs,      (void **)&d.tmm_long,
+                     (void **)&d.forest_nodes,  (void **)&d.forest_child,
+                     (void **)&d.forest_chunks, (void **)&d.forest_order};
     for (void **q : ptrs) {
         if (*q) (void)hipFree(*q);
         *q = nullptr;
@@ -101,6 +105,22 @@ static int ensure_pub(sprgpu_handle *h)
     return 0;
 }
 
+// The persistent forest schedule's update blocks and counters, sized for the
+// current plan (kept when a later plan needs no more).
+static int ensure_forest(sprgpu_handle *h)
+{
+    const Plan &p = h->plan;
+    DeviceBuffers &d = h->d;
+    if (!p.forest) return 0;
+    const i64 esz = h->precision == SPRGPU_FP32 ? i64(sizeof(float)) : i64(sizeof(double));
+    if (!d.ublk) {
+        SPRGPU_HIP(hipMalloc(&d.ublk, size_t(esz * std::max<i64>(p.ublk_elems, 1))));
+        SPRGPU_HIP(hipMalloc(&d.forest_sync, sizeof(unsigned) * size_t(2 + 3 * p.nsuper)));
+        h->device_bytes += esz * std::max<i64>(p.ublk_elems, 1) + 4 * (2 + 3 * p.nsuper);
+    }
+    return 0;
+}
+
 static int upload_plan(sprgpu_handle *h)
 {
     const Plan &p = h->plan;
@@ -124,7 +144,11 @@ static int upload_plan(sprgpu_handle *h)
         (rc = upload(d.wide_bwd_prefix, p.wide_bwd_prefix, B)) ||
         (rc = upload(d.wide_rect_prefix, p.wide_rect_prefix, B)) ||
         (rc = upload(d.wide_frect_prefix, p.wide_frect_prefix, B)) ||
-        (rc = upload(d.upd_ptr, p.upd_ptr, B)) || (rc = upload(d.upd_src, p.upd_src, B)))
+        (rc = upload(d.upd_ptr, p.upd_ptr, B)) || (rc = upload(d.upd_src, p.upd_src, B)) ||
+        (rc = upload(d.forest_nodes, p.forest_nodes, B)) ||
+        (rc = upload(d.forest_child, p.forest_child, B)) ||
+        (rc = upload(d.forest_chunks, p.forest_chunks, B)) ||
+        (rc = upload(d.forest_order, p.forest_order, B)))
         return rc;
     return 0;
 }
@@ -157,6 +181,7 @@ static int replan(sprgpu_handle *h, const PlanParams &params)
     free_plan_buffers(h);
     h->plan = std::move(np);
     if (int rc = upload_plan(h)) return rc;
+    if (int rc = ensure_forest(h)) return rc;
     return ensure_pub(h);
 }
 
@@ -230,6 +255,10 @@ int sprgpu_create(const sprgpu_int *mspar, const sprgpu_int *mtrees, const sprgp
         free_all(h.get());
         return rc;
     }
+    if (int rc = ensure_forest(h.get())) {
+        free_all(h.get());
+        return rc;
+    }
     if (hipMemset(d.dcor, 0, sizeof(double) * 2 * p.neq) != hipSuccess ||
         hipMemset(d.status, 0, sizeof(FactorStatus)) != hipSuccess) {
         free_all(h.get());
@@ -289,8 +318,10 @@ int sprgpu_factor_solve_graph(sprgpu_handle *h, const double *sm_dev, double *b_
         h->stream = h->capture_stream;
         hipGraph_t g = nullptr;
         SPRGPU_HIP(hipStreamBeginCapture(h->capture_stream, hipStreamCaptureModeThreadLocal));
-        int rc = run_factor(h, sm_dev, 1, tol);
-        if (rc == 0 && !h->d.peri2e) rc = run_solve(h, b_dev, ldb, nrhs, 3);
+        // a forest of small fronts: factor and solve in one persistent launch
+        const bool fused = h->plan.forest && nrhs == 1 && !h->d.peri2e;
+        int rc = fused ? run_factor_solve(h, sm_dev, 1, tol, b_dev) : run_factor(h, sm_dev, 1, tol);
+        if (rc == 0 && !fused && !h->d.peri2e) rc = run_solve(h, b_dev, ldb, nrhs, 3);
         if (rc == 0 && h->d.peri2e) {  // B in external order
             if (!(rc = perm_gather(h, b_dev, ldb, w0, h->plan.neq, nrhs)) &&
                 !(rc = run_solve(h, w0, h->plan.neq, nrhs, 3)))
@@ -326,8 +357,10 @@ int sprgpu_autotune(sprgpu_handle *h, const double *sm_dev, double *b_dev, sprgp
 {
     if (!h || !sm_dev || !b_dev || nrhs < 1 || ldb < h->plan.neq) return -2;
     std::vector<PlanParams> cand;
-    // FP32 tries the per-launch tile choice and the 128 tile everywhere
-    // (36 candidates: the report's 256 doubles hold 51)
+    // The level schedule's parameters (FP32 tries the per-launch tile choice
+    // and the 128 tile everywhere), then, for a structure whose fronts all
+    // have F <= kSmallF, the persistent forest schedule (at most 37
+    // candidates: the report's 256 doubles hold 42)
     const int tiles[2] = {0, kTileBig};
     for (int sf : {32, 48, 64})
         for (int w : {1, 2, 4})
@@ -338,8 +371,12 @@ int sprgpu_autotune(sprgpu_handle *h, const double *sm_dev, double *b_dev, sprgp
                     pp.small_waves = w;
                     pp.narrow_threads = nt;
                     pp.big_tile = (h->precision == SPRGPU_FP32) ? tiles[t] : 0;
+                    pp.forest = 0;
                     cand.push_back(pp);
                 }
+    bool small_fronts = h->plan.mfront <= kSmallF;
+    for (const NodeInfo &n : h->plan.nodes) small_fronts &= n.F <= kSmallF;
+    if (small_fronts) cand.push_back(PlanParams{});
     const double tol[4] = {0.0, 0.0, 0.0, 0.0};
     hipEvent_t e0, e1;
     SPRGPU_HIP(hipEventCreate(&e0));
@@ -360,12 +397,13 @@ int sprgpu_autotune(sprgpu_handle *h, const double *sm_dev, double *b_dev, sprgp
             break;
         }
         const double t = ms / 3.0;
-        if (report && 5 * i + 5 < 256) {
-            report[1 + 5 * i] = cand[i].small_f;
-            report[2 + 5 * i] = cand[i].small_waves;
-            report[3 + 5 * i] = cand[i].narrow_threads;
-            report[4 + 5 * i] = cand[i].big_tile;
-            report[5 + 5 * i] = t;
+        if (report && 6 * i + 6 < 256) {
+            report[1 + 6 * i] = cand[i].small_f;
+            report[2 + 6 * i] = cand[i].small_waves;
+            report[3 + 6 * i] = cand[i].narrow_threads;
+            report[4 + 6 * i] = cand[i].big_tile;
+            report[5 + 6 * i] = cand[i].forest;
+            report[6 + 6 * i] = t;
         }
         if (t < best) {
             best = t;
@@ -387,6 +425,7 @@ int sprgpu_plan_params(const sprgpu_handle *h, int *out)
     out[2] = h->plan.params.narrow_threads;
     out[3] = h->plan.big_tile;
     out[4] = h->plan.params.block_factor;
+    out[5] = h->plan.forest ? 1 : 0;  // the forest schedule in use
     return 0;
 }
 
@@ -399,6 +438,11 @@ int sprgpu_set_plan_params(sprgpu_handle *h, const int *params)
     pp.narrow_threads = params[2];
     pp.big_tile = params[3];
     pp.block_factor = params[4];
+    pp.forest = params[5];
+    if (pp.forest != 0 && pp.forest != 1) {
+        set_error("forest must be 0 or 1");
+        return -2;
+    }
     return replan(h, pp);
 }
 

@@ -25,6 +25,7 @@
 #include <type_traits>
 
 #include "lanes.h"
+#include "pivots.h"
 #include "sprgpu_internal.h"
 
 namespace sprgpu {
@@ -54,42 +55,6 @@ template <> struct Mfma<float> {
     }
     static __device__ __forceinline__ int row(int lane, int e) { return 4 * (lane >> 4) + e; }
 };
-
-// ---------------------------------------------------------------------------
-// pivot rule
-// ---------------------------------------------------------------------------
-struct PivotArgs {
-    double tol1, tol4;
-    int mode;  // 1: SPRFC1 (record failure), 3: SPRFC3 (replace)
-};
-
-// Value of the pivot after the SPRFC3 replacement rule (identity in mode 1).
-__device__ __forceinline__ double pivot_value(double d, double a, const PivotArgs &pa)
-{
-    if (pa.mode == 3 && fabs(d) <= pa.tol1 * fabs(a)) {
-        const double mag = pa.tol1 * fmax(1.0, fabs(pa.tol4 * a));
-        return copysign(mag, d);
-    }
-    return d;
-}
-
-// Book-keeping for pivot eq (one thread, once per pivot).
-__device__ __forceinline__ void pivot_record(double d_in, double d_out, double a, int eq,
-                                             const PivotArgs &pa, double *dvec, double *dcor,
-                                             FactorStatus *st)
-{
-    dvec[eq] = d_out;
-    if (fabs(d_in) <= pa.tol1 * fabs(a)) {
-        if (pa.mode == 3) {
-            atomicAdd(&st->nrepl, 1ull);
-            dcor[2 * i64(eq)] = d_in;
-            dcor[2 * i64(eq) + 1] = d_out;
-        } else {
-            atomicMin(&st->first_fail, (unsigned long long)eq);
-        }
-    }
-    if (pa.mode == 3 && d_out < 0.0 && dcor[2 * i64(eq)] == 0.0) dcor[2 * i64(eq)] = d_out;
-}
 
 // ---------------------------------------------------------------------------
 // status reset (a kernel rather than a memset so that the sequence can be
@@ -1631,10 +1596,12 @@ enum ProfClass {
 // Launches the whole factorization.  With ev != nullptr an event is recorded
 // after every kernel and the kernel's class appended to cls, so consecutive
 // events bracket one kernel each (the stream serialises them).
+// With b set (a plan on the forest schedule, one right-hand side) the
+// launch also solves for b.
 template <typename T>
 static int factor_impl(sprgpu_handle *h, const double *sm_dev, int mode, const double *tol,
                        std::vector<hipEvent_t> *ev = nullptr, std::vector<int> *cls = nullptr,
-                       std::vector<double> *fl = nullptr)
+                       std::vector<double> *fl = nullptr, double *b = nullptr)
 {
     const Plan &p = h->plan;
     DeviceBuffers &d = h->d;
@@ -1654,6 +1621,18 @@ static int factor_impl(sprgpu_handle *h, const double *sm_dev, int mode, const d
     mark(P_OTHER, 0.0);
     k_status_reset<<<1, 64, 0, s>>>(d.status);
     mark(P_OTHER, 0.0);
+    if (p.forest) {  // the whole factorization (and the solve) in one persistent launch
+        if (int rc = forest_factor(h, sm_dev, mode, tol, b)) return rc;
+        mark(P_SMALL, 0.0);
+        k_stats_partial<<<kStatBlocks, 256, 0, s>>>(d.dvec, d.adiag, p.neq, d.status, d.stats);
+        k_stats_final<<<1, 64, 0, s>>>(d.stats, kStatBlocks);
+        SPRGPU_HIP(hipGetLastError());
+        return 0;
+    }
+    if (b) {
+        set_error("factor and solve in one launch needs the forest schedule");
+        return -2;
+    }
     if (d.pub_elems > 0) {  // the fused outer-block launches' publication areas
         k_pub_mark<T><<<unsigned(std::min<i64>((d.pub_elems + 255) / 256, 2048)), 256, 0, s>>>(
             static_cast<T *>(d.pub), d.pub_elems);
@@ -1774,6 +1753,24 @@ int run_factor(sprgpu_handle *h, const double *sm_dev, int mode, const double *t
     h->last_sm_dev = sm_dev;
     int rc = (h->precision == SPRGPU_FP32) ? factor_impl<float>(h, sm_dev, mode, tol)
                                            : factor_impl<double>(h, sm_dev, mode, tol);
+    if (rc == 0) h->factored = true;
+    return rc;
+}
+
+int run_factor_solve(sprgpu_handle *h, const double *sm_dev, int mode, const double *tol,
+                     double *b)
+{
+    if (!h->plan.forest) {
+        set_error("factor and solve in one launch needs the forest schedule");
+        return -2;
+    }
+    if (int rc = prepare_solve(h, 1)) return rc;
+    h->mode = mode;
+    for (int k = 0; k < 4; ++k) h->tol[k] = tol[k];
+    h->last_sm_dev = sm_dev;
+    int rc = (h->precision == SPRGPU_FP32)
+                 ? factor_impl<float>(h, sm_dev, mode, tol, nullptr, nullptr, nullptr, b)
+                 : factor_impl<double>(h, sm_dev, mode, tol, nullptr, nullptr, nullptr, b);
     if (rc == 0) h->factored = true;
     return rc;
 }

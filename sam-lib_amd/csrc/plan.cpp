@@ -675,6 +675,95 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
             }
         }
     }
+    // forest schedule: every update block packed at its own offset; the
+    // ascending pass takes the supernodes by depth in the tree, deepest
+    // first (a child is deeper than its parent), so that the longest chains
+    // to a root start first; the backward pass by height, highest first (a
+    // parent is higher than its child), so that the longest chains to a leaf
+    // start first.  Items of one depth (height) are independent; each is
+    // split into workgroup chunks.
+    p.forest = params.forest != 0 && p.mfront <= kSmallF;
+    for (i64 j = 0; p.forest && j < ns; ++j) p.forest = p.nodes[j].F <= kSmallF;
+    p.forest_nodes.clear();
+    p.forest_child.clear();
+    p.forest_order.clear();
+    p.forest_chunks.clear();
+    p.forest_nasc = 0;
+    p.ublk_elems = 0;
+    if (p.forest) {
+        std::vector<i64> uoff(ns);
+        for (i64 j = 0; j < ns; ++j) {
+            const i64 U = p.nodes[j].F - p.nodes[j].S;
+            uoff[j] = p.ublk_elems;
+            p.ublk_elems += round_up(U * (U + 1) / 2, 16);
+        }
+        p.forest_nodes.resize(ns);
+        for (i64 j = 0; j < ns; ++j) {
+            const NodeInfo &n = p.nodes[j];
+            ForestNode &f = p.forest_nodes[j];
+            f.xlnz = n.xlnz;
+            f.loff = n.loff;
+            f.uoff = uoff[j];
+            f.voff = n.voff;
+            f.lxoff = n.lxoff;
+            f.first = n.first;
+            f.S = n.S;
+            f.F = n.F;
+            f.ld = n.ld;
+            f.parent = n.parent;
+            f.child0 = i32(p.forest_child.size());
+            f.nchild = n.nchild;
+            f.pad = 0;
+            for (i32 c = 0; c < n.nchild; ++c) {
+                const i32 cj = p.children[n.child0 + c];
+                const NodeInfo &cn = p.nodes[cj];
+                ForestChild fc;
+                fc.uoff = uoff[cj];
+                fc.vofs = cn.voff + cn.S;
+                fc.mapoff = cn.mapoff;
+                fc.U = cn.F - cn.S;
+                fc.pad = 0;
+                p.forest_child.push_back(fc);
+            }
+        }
+        std::vector<i32> depth(ns, 0);
+        for (i64 j = ns - 1; j >= 0; --j)  // parents come after their children
+            depth[j] = parent[j] >= 0 ? depth[parent[j]] + 1 : 0;
+        const i32 maxd = *std::max_element(depth.begin(), depth.end());
+        // chunks of a group of independent supernodes [a0, a1) of forest_order
+        auto chunk = [&](i64 a0, i64 a1) {
+            const i64 per = std::max<i64>(
+                kForestWaves,
+                round_up((a1 - a0 + kForestWgsPerLevel - 1) / kForestWgsPerLevel, kForestWaves));
+            for (i64 k = a0; k < a1; k += per) p.forest_chunks.push_back(i32(k));
+        };
+        std::vector<std::vector<i32>> bydepth(maxd + 1);
+        for (i64 j = 0; j < ns; ++j) bydepth[depth[j]].push_back(i32(j));
+        for (i32 dd = maxd; dd >= 0; --dd) {
+            const i64 a0 = i64(p.forest_order.size());
+            p.forest_order.insert(p.forest_order.end(), bydepth[dd].begin(), bydepth[dd].end());
+            chunk(a0, i64(p.forest_order.size()));
+        }
+        p.forest_nasc = i32(p.forest_chunks.size());
+        for (int l = p.nlevels - 1; l >= 0; --l) {
+            const i64 a0 = i64(p.forest_order.size());
+            p.forest_order.insert(p.forest_order.end(), p.level_nodes.begin() + p.level_ptr[l],
+                                  p.level_nodes.begin() + p.level_ptr[l + 1]);
+            chunk(a0, i64(p.forest_order.size()));
+        }
+        p.forest_chunks.push_back(i32(p.forest_order.size()));
+        // every item waits only for items of earlier chunks (its workgroup's
+        // predecessors): a child's ascending item, a parent's backward item
+        std::vector<i32> achunk(ns, -1), bchunk(ns, -1);
+        for (size_t q = 0; q + 1 < p.forest_chunks.size(); ++q)
+            for (i32 k = p.forest_chunks[q]; k < p.forest_chunks[q + 1]; ++k)
+                (i32(q) < p.forest_nasc ? achunk : bchunk)[p.forest_order[k]] = i32(q);
+        for (i64 j = 0; j < ns; ++j) {
+            if (achunk[j] < 0 || bchunk[j] < 0) return "forest order misses a supernode (internal error)";
+            if (parent[j] >= 0 && (achunk[parent[j]] <= achunk[j] || bchunk[parent[j]] >= bchunk[j]))
+                return "forest order breaks a dependency (internal error)";
+        }
+    }
     return "";
 }
 

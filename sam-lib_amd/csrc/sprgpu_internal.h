@@ -36,6 +36,8 @@ constexpr int kPoolExactFronts = 16384;  // largest fronts placed in the pool by
 constexpr int kFusedMaxFronts = 64;  // outer-block launches of at most this many fronts are fused
 constexpr int kPubElems = 4 * kPanelNB * kPanelNB + (kPanelOuter - kPanelNB) * (kPanelOuter - kPanelNB);
 constexpr int kSmallF = 64;      // fronts with F <= kSmallF can be factored in LDS (one lane a row)
+constexpr int kForestWaves = 4;  // wavefronts per workgroup of the forest schedule (one item each)
+constexpr int kForestWgsPerLevel = 1024;  // at most this many workgroups per tree height
 constexpr int kInitCols = 8;     // front columns per workgroup in the init kernel
 constexpr int kEaCols = 8;       // child columns per workgroup in extend-add
 constexpr int kSolveSmallS = 64;   // narrow supernodes: S <= this ...
@@ -78,6 +80,54 @@ struct NodeInfo {
     i32 S, F, ld;
     i32 parent;    // -1 for a root
     i32 child0, nchild;  // range in the children array (SIBL order)
+};
+
+// Pivot rule arguments of the factor kernels: TOL(1), TOL(4) and the mode
+// (1: SPRFC1, record a failing pivot; 3: SPRFC3, replace it).
+struct PivotArgs {
+    double tol1, tol4;
+    int mode;
+};
+
+// The forest schedule (every front F <= kSmallF): per supernode what its
+// work item needs in one record, and its children's records (SIBL order).
+struct ForestNode {
+    i64 xlnz;      // packed trapezoid in LNZ (0-based)
+    i64 loff;      // factored panel in the factor store
+    i64 uoff;      // packed update block in the forest's update-block array
+    i64 voff;      // vector slot in the solve workspace
+    i64 lxoff;     // row list in lindx
+    i32 first, S, F, ld;
+    i32 parent;    // -1 for a root
+    i32 child0, nchild;  // range in the child records
+    i32 pad;
+};
+struct ForestChild {
+    i64 uoff;      // the child's packed update block
+    i64 vofs;      // its update vector: vwork[vofs .. vofs + U)
+    i64 mapoff;    // its U rows' positions in the parent front (relmap)
+    i32 U, pad;
+};
+struct FactorStatus;
+// Arguments of the persistent forest kernel.
+struct ForestArgs {
+    const ForestNode *fnodes;
+    const ForestChild *fchild;
+    i32 n;          // supernodes
+    i32 phases;     // bit 0: ascending pass (factor and/or forward), bit 1: backward pass
+    const i32 *order;   // supernodes: the ascending pass's order, then the backward pass's
+    const i32 *chunks;  // workgroup chunks over order: nasc ascending, then nbwd backward, end
+    i32 nasc, nbwd;
+    const i32 *relmap, *lindx;
+    const double *lnz;
+    void *ublk, *store;
+    double *dvec, *adiag, *dcor;
+    FactorStatus *st;  // factor launches
+    int *err;          // solve launches: set when a wait gave up
+    PivotArgs pa;
+    double *b, *vwork;  // one right-hand side
+    unsigned *sync;     // [0] unused, [1] a wait gave up, then per supernode: children
+                        // done, ascending pass done, backward pass done
 };
 
 // Work descriptors consumed by the kernels (built once per plan).
@@ -136,6 +186,9 @@ struct PlanParams {
     i32 narrow_threads = 0;   // workgroup size of the narrow solves (0: by front size)
     i32 block_factor = 1;     // blocked fronts: 1 = outer blocks in one launch each, 2 = their
                               // diagonal blocks and rows below as two launches, 0 = panel steps
+    i32 forest = 1;           // 1: a plan whose fronts all have F <= kSmallF is factored (and
+                              // solved, one right-hand side at a time) by the persistent forest
+                              // kernel; 0: always the level schedule
 };
 
 struct Plan {
@@ -181,6 +234,17 @@ struct Plan {
     // [upd_ptr[i], upd_ptr[i+1]), the children's update entries in SIBL order
     std::vector<i64> upd_ptr;
     std::vector<i32> upd_src;
+    // persistent forest schedule (forest true): see ForestNode
+    bool forest = false;
+    std::vector<ForestNode> forest_nodes;
+    std::vector<ForestChild> forest_child;
+    // the ascending pass's supernodes (deepest first), then the backward
+    // pass's (highest first), and the workgroup chunks over them (starts,
+    // one depth or height each): forest_nasc ascending chunks, then the
+    // backward chunks, then the end
+    std::vector<i32> forest_order, forest_chunks;
+    i32 forest_nasc = 0;
+    i64 ublk_elems = 0;             // packed update blocks of all supernodes
 };
 
 // Builds the plan from the reference symbolic arrays (1-based, INTEGER*8)
@@ -273,6 +337,12 @@ struct DeviceBuffers {
     i32 *peri2e = nullptr;
     double *permwork = nullptr;
     i64 permwork_cap = 0;
+    // persistent forest schedule
+    ForestNode *forest_nodes = nullptr;
+    ForestChild *forest_child = nullptr;
+    i32 *forest_chunks = nullptr, *forest_order = nullptr;
+    void *ublk = nullptr;       // T[ublk_elems]
+    unsigned *forest_sync = nullptr;  // 2 + 3 NSUPER counters and flags
 };
 
 }  // namespace sprgpu
@@ -323,8 +393,14 @@ int hip_fail(hipError_t e, const char *what);
         if (e_ != hipSuccess) return ::sprgpu::hip_fail(e_, #call);   \
     } while (0)
 
+// forest.hip: the persistent forest schedule (plan.forest)
+int forest_factor(sprgpu_handle *h, const double *sm_dev, int mode, const double *tol, double *b);
+int forest_solve(sprgpu_handle *h, double *b, int which);
 // factor.hip
 int run_factor(sprgpu_handle *h, const double *sm_dev, int mode, const double *tol);
+// factor and solve of one right-hand side b in the same launch (plan.forest)
+int run_factor_solve(sprgpu_handle *h, const double *sm_dev, int mode, const double *tol,
+                     double *b);
 int read_factor_info(sprgpu_handle *h, sprgpu_factor_info *info);
 int copy_factor_to_sm(sprgpu_handle *h, double *sm_dev_out);
 int load_factor_from_sm(sprgpu_handle *h, const double *sm_dev);

@@ -1665,6 +1665,7 @@ static int solve_impl(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which)
 {
     const Plan &p = h->plan;
     if (int rc = ensure_solve_buffers(h, nrhs)) return rc;
+    if (p.forest && nrhs == 1) return forest_solve(h, b, which);
     constexpr i64 kMaxWgs = i64(1) << 23;
     i64 chunk = nrhs;
     for (int l = 0; l < p.nlevels; ++l) {

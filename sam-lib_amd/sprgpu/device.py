@@ -66,11 +66,12 @@ class SprFactor:
     def device_bytes(self) -> int:
         return int(lib().sprgpu_device_bytes(self._h))
 
-    _PARAM_NAMES = ("small_f", "small_waves", "narrow_threads", "big_tile", "block_factor")
+    _PARAM_NAMES = ("small_f", "small_waves", "narrow_threads", "big_tile", "block_factor",
+                    "forest")
 
     def plan_params(self) -> dict:
         """Schedule parameters in use (sprgpu_plan_params)."""
-        out = (C.c_int * 5)()
+        out = (C.c_int * len(self._PARAM_NAMES))()
         check(lib().sprgpu_plan_params(self._h, out), "sprgpu_plan_params")
         return dict(zip(self._PARAM_NAMES, list(out)))
 
@@ -81,7 +82,7 @@ class SprFactor:
         if unknown:
             raise ValueError(f"unknown plan parameters {sorted(unknown)}")
         cur.update(kw)
-        arr = (C.c_int * 5)(*[int(cur[k]) for k in self._PARAM_NAMES])
+        arr = (C.c_int * len(self._PARAM_NAMES))(*[int(cur[k]) for k in self._PARAM_NAMES])
         check(lib().sprgpu_set_plan_params(self._h, arr), "sprgpu_set_plan_params")
 
     def set_pattern(self, sm_host: np.ndarray):

@@ -107,8 +107,8 @@ def test_autotune_keeps_results():
                                               C.c_void_p(scratch.data_ptr()), sysm.neq, 1,
                                               report.ctypes.data_as(C.c_void_p)), "autotune")
         n = int(report[0])
-        assert n >= 2 and np.all(report[5:5 * n + 1:5] > 0)
-        params = (C.c_int * 5)()
+        assert n >= 2 and np.all(report[6:6 * n + 1:6] > 0)
+        params = (C.c_int * 6)()
         _lib.lib().sprgpu_plan_params(h._h, params)
         assert params[0] in (32, 48, 64) and params[1] in (1, 2, 4)
         x1 = b.clone()
