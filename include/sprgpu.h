@@ -50,6 +50,20 @@ void sprgpu_sprfc3(sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa, dou
                    const sprgpu_int *lpu, sprgpu_int *ierr, void (*mvmult)(void),
                    void (*mmmult)(void));
 
+/* Replaces SPRFC4 (src/SPR/sprfc4.f:5-7), the CC' factorization with
+ * SPRFC3-style replacement (pivot step sprfc4.f:1016-1048): a zero diagonal
+ * is replaced by one (DCOR(2,j) = 1); a pivot d with
+ * -TOL(1)*|a_jj| <= d <= TOL(1)*|a_jj| by TOL(1)*max(1,|TOL(4)*a_jj|),
+ * counted (IERR = -3, MSPAR(27) = -count) and recorded in DCOR(1:2,j);
+ * a pivot below -TOL(1)*|a_jj| fails (IERR = -3, MSPAR(27) = its equation,
+ * TOL(2:3) = d and a_jj, DCOR(1:2,j) = d).  MSPAR(26) = 0, MSPAR(1) = 6;
+ * SM out holds C = L sqrt(D) (c_jj on the diagonal and in SM(1:NEQ)), for
+ * SPRFS2/SPRBS2.  C agrees with the reference's to rounding. */
+void sprgpu_sprfc4(sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa, double *sm,
+                   double *tol, sprgpu_int *iwork, double *rwork, double *dcor,
+                   const sprgpu_int *lpu, sprgpu_int *ierr, void (*mvmult)(void),
+                   void (*mmmult)(void));
+
 /* Replaces SPRFS1 (src/SPR/sprfs1.f:5-7): B(LDB,NRHS) <- D^-1 L^-1 B. */
 void sprgpu_sprfs1(sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa, double *sm,
                    double *b, const sprgpu_int *ldb, const sprgpu_int *nrhs, sprgpu_int *iwork,
@@ -89,10 +103,12 @@ void sprgpu_release(const double *sm);
 
 /* LIERR of the last drop-in factorization on the calling thread: the
  * reference's internal error code behind IERR = -3 (sprfc1.f:250-262,
- * sprfc2.f:244-255, sprfc3.f:250-262): -30 a pivot at or below TOL(1)|a_jj|
- * (SPRFC1), -32 the same in SPRFC2, -31 pivots replaced (SPRFC3), -35 a zero
- * diagonal found before the factorization; 0 otherwise.  The Fortran layer
- * hands it to SPRER1 as the reference drivers do. */
+ * sprfc2.f:244-255, sprfc3.f:250-262, sprfc4.f:240-256): -30 a pivot at or
+ * below TOL(1)|a_jj| (SPRFC1), -32 the same in SPRFC2 and a pivot below
+ * -TOL(1)|a_jj| in SPRFC4, -31 pivots replaced (SPRFC3), -33 pivots
+ * replaced (SPRFC4), -35 a zero diagonal found before the factorization; 0
+ * otherwise.  The Fortran layer hands it to SPRER1 as the reference drivers
+ * do. */
 sprgpu_int sprgpu_dropin_lierr(void);
 
 /* ------------------------------------------------------------------------ */
@@ -163,14 +179,16 @@ int sprgpu_assemble(sprgpu_handle *handle, const double *em_dev, const sprgpu_in
 
 /* Numeric factorization of SM given in device memory (NEQ+NZEROL doubles, the
  * SPRADM layout; not modified).  tol points to host TOL(1..4) (TOL(4) is read
- * only when mode==3).  mode 1 = SPRFC1 semantics, 3 = SPRFC3 semantics,
- * 2 = SPRFC2 (src/SPR/sprfc2.f:5-7, the CC' form: a pivot fails when
- * d <= TOL(1)*|a_jj|, sign included, sprfc2.f:934-1037; IERR, SUPEQT, TOL(2:3)
- * and MSPAR(26) as SPRFC2 reports them).  After a mode-2 factorization,
- * sprgpu_solve with which 1 is SPRFS2 (C^-1 B), 2 is SPRBS2 (C^-T B), 3 both,
- * and sprgpu_get_factor gives C in SM layout.  The CC' factor is C = L
- * sqrt(D) of the LDL' factorization, so it agrees with the reference's to
- * rounding, not bit for bit.
+ * only when mode is 3 or 4).  mode 1 = SPRFC1 semantics, 3 = SPRFC3
+ * semantics, 2 = SPRFC2 (src/SPR/sprfc2.f:5-7, the CC' form: a pivot fails
+ * when d <= TOL(1)*|a_jj|, sign included, sprfc2.f:934-1037; IERR, SUPEQT,
+ * TOL(2:3) and MSPAR(26) as SPRFC2 reports them), 4 = SPRFC4 (sprfc4.f:5-7,
+ * the CC' form with replacement, see sprgpu_sprfc4; DCOR from
+ * sprgpu_get_dcor).  After a mode-2 or mode-4 factorization, sprgpu_solve
+ * with which 1 is SPRFS2 (C^-1 B), 2 is SPRBS2 (C^-T B), 3 both, and
+ * sprgpu_get_factor gives C in SM layout.  The CC' factor is C = L sqrt(D)
+ * of the LDL' factorization, so it agrees with the reference's to rounding,
+ * not bit for bit.
  * info may be NULL (then the call returns without synchronizing). */
 int sprgpu_factor(sprgpu_handle *handle, const double *sm_dev, const double *tol, int mode,
                   sprgpu_factor_info *info);
@@ -178,12 +196,14 @@ int sprgpu_factor(sprgpu_handle *handle, const double *sm_dev, const double *tol
 /* Reads the factor statistics of the last factorization (synchronizes). */
 int sprgpu_factor_info_get(sprgpu_handle *handle, sprgpu_factor_info *info);
 
-/* DCOR(2,NEQ) of the last mode-3 (SPRFC3) factorization, the array SPRFC3
- * returns (sprfc3.f:5-7, filled at sprfc3.f:560-575 and :1083-1099):
- * DCOR(1,j) the pivot before its replacement (or a negative pivot), DCOR(2,j)
- * the replacement (1 for a zero diagonal), zero elsewhere.  dcor_out (2*NEQ
- * doubles) may be host or device memory.  Returns -1 if the last
- * factorization was not in mode 3. */
+/* DCOR(2,NEQ) of the last mode-3 (SPRFC3) or mode-4 (SPRFC4) factorization,
+ * the array SPRFC3/SPRFC4 return (sprfc3.f:5-7, filled at sprfc3.f:560-575
+ * and :1083-1099; sprfc4.f:459-472, :1016-1037): DCOR(1,j) the pivot before
+ * its replacement (or a negative pivot: SPRFC3's kept one, SPRFC4's failing
+ * one), DCOR(2,j) the replacement (1 for a zero diagonal; SPRFC4's failing
+ * pivot again), zero elsewhere.  dcor_out (2*NEQ doubles) may be host or
+ * device memory.  Returns -1 if the last factorization was not in mode 3
+ * or 4. */
 int sprgpu_get_dcor(sprgpu_handle *handle, double *dcor_out);
 
 /* Forward (D^-1 L^-1) and/or backward (L^-T) substitution in place on device

@@ -52,3 +52,17 @@ fint spr_fc2(fint *mspar, fint *mtrees, fint *msifa, double *sm, double *tol, fi
     sprfc2_(mspar, mtrees, msifa, sm, tol, iwork, rwork, &lpu, &ierr, cmvy18_, cmmy18_);
     return ierr;
 }
+
+/* SPRFC4 (sprfc4.f:5-7), the CC' factorization with replacement, with the
+ * same CC' kernels. */
+extern void sprfc4_(fint *mspar, fint *mtrees, fint *msifa, double *sm, double *tol,
+                    fint *iwork, double *rwork, double *dcor, const fint *lpu, fint *ierr,
+                    void (*mvmult)(void), void (*mmmult)(void));
+
+fint spr_fc4(fint *mspar, fint *mtrees, fint *msifa, double *sm, double *tol, fint *iwork,
+             double *rwork, double *dcor, fint lpu)
+{
+    fint ierr = 0;
+    sprfc4_(mspar, mtrees, msifa, sm, tol, iwork, rwork, dcor, &lpu, &ierr, cmvy18_, cmmy18_);
+    return ierr;
+}

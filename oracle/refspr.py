@@ -126,6 +126,25 @@ def fc2(sysm: SprSystem, sm: np.ndarray, tol=(0.0, 0.0, 0.0), mspar: np.ndarray 
     return int(ierr), tolv
 
 
+def fc4(sysm: SprSystem, sm: np.ndarray, tol=(0.0, 0.0, 0.0, 0.0),
+        mspar: np.ndarray | None = None):
+    """Reference SPRFC4 (sprfc4.f:5-7), the CC' factorization with
+    replacement, with CMVY18 / CMMY18.  ``sm`` and ``mspar`` are modified in
+    place.  Returns (ierr, tol, dcor (neq, 2))."""
+    mspar = sysm.mspar if mspar is None else mspar
+    mfront = int(mspar[13])
+    iwork = np.zeros(mfront + 2, dtype=I8)
+    rwork = np.zeros(max(int(mspar[16]), mfront) + 1)
+    tolv = np.zeros(4)
+    tolv[:len(tol)] = tol
+    dcor = np.zeros(2 * int(mspar[7]))
+    f = lib().spr_fc4
+    f.restype = C.c_int64
+    ierr = f(_p(mspar), _p(sysm.mtrees), _p(sysm.msifa), _p(sm), _p(tolv), _p(iwork), _p(rwork),
+             _p(dcor), C.c_int64(-1))
+    return int(ierr), tolv, dcor.reshape(-1, 2)
+
+
 def fs2_bs2(which: str, sysm: SprSystem, sm: np.ndarray, b: np.ndarray,
             mspar: np.ndarray | None = None):
     """Reference SPRFS2 (C y = b, sprfs2.f:5) or SPRBS2 (C^T x = y, sprbs2.f:5)

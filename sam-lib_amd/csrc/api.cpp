@@ -479,12 +479,13 @@ int sprgpu_factor(sprgpu_handle *h, const double *sm_dev, const double *tol, int
                   sprgpu_factor_info *info)
 {
     if (!h || !sm_dev || !tol) return -2;
-    if (mode != 1 && mode != 2 && mode != 3) {
-        set_error("mode must be 1 (SPRFC1), 2 (SPRFC2) or 3 (SPRFC3)");
+    if (mode < 1 || mode > 4) {
+        set_error("mode must be 1 (SPRFC1), 2 (SPRFC2), 3 (SPRFC3) or 4 (SPRFC4)");
         return -2;
     }
-    // SPRFC2: the LDL' factorization with every pivot computed, then its test
-    double t4[4] = {mode == 2 ? 0.0 : tol[0], 0.0, 0.0, mode == 3 ? tol[3] : 0.0};
+    // SPRFC2: the LDL' factorization with every pivot computed, then its
+    // test; SPRFC4: the LDL' factorization with its pivot rule, C = L sqrt(D)
+    double t4[4] = {mode == 2 ? 0.0 : tol[0], 0.0, 0.0, mode >= 3 ? tol[3] : 0.0};
     int rc = run_factor(h, sm_dev, mode == 2 ? 1 : mode, t4);
     if (rc) return rc;
     if (mode == 2) {
@@ -516,8 +517,8 @@ int sprgpu_factor_info_get(sprgpu_handle *h, sprgpu_factor_info *info)
 int sprgpu_get_dcor(sprgpu_handle *h, double *dcor_out)
 {
     if (!h || !dcor_out) return -2;
-    if (!h->factored || h->mode != 3) {
-        set_error("DCOR exists after a mode-3 (SPRFC3) factorization only");
+    if (!h->factored || h->mode < 3) {
+        set_error("DCOR exists after a mode-3 (SPRFC3) or mode-4 (SPRFC4) factorization only");
         return -1;
     }
     SPRGPU_HIP(hipMemcpyAsync(dcor_out, h->d.dcor, sizeof(double) * 2 * size_t(h->plan.neq),
@@ -530,7 +531,7 @@ int sprgpu_get_dcor(sprgpu_handle *h, double *dcor_out)
 static int solve_internal(sprgpu_handle *h, double *b, sprgpu_int ldb, sprgpu_int nrhs,
                           int which)
 {
-    if (h->mode == 2 && which != 3) {
+    if ((h->mode == 2 || h->mode == 4) && which != 3) {
         // C^-1 b = sqrt(D) D^-1 L^-1 b; C^-T y = L^-T D^-1/2 y
         if (which == 1) {
             if (int rc = run_solve(h, b, ldb, nrhs, 1)) return rc;
@@ -674,7 +675,7 @@ int sprgpu_get_factor(sprgpu_handle *h, double *sm_out)
     const size_t bytes = sizeof(double) * size_t(p.neq + p.nzerol);
     if (on_device) {
         int rc = copy_factor_to_sm(h, sm_out);
-        if (rc == 0 && h->mode == 2) rc = cc_factor_layout(h, sm_out);
+        if (rc == 0 && (h->mode == 2 || h->mode == 4)) rc = cc_factor_layout(h, sm_out);
         return rc;
     }
     double *tmp = nullptr;
@@ -683,7 +684,7 @@ int sprgpu_get_factor(sprgpu_handle *h, double *sm_out)
     if (h->last_sm_dev)
         SPRGPU_HIP(hipMemcpyAsync(tmp, h->last_sm_dev, bytes, hipMemcpyDeviceToDevice, h->stream));
     int rc = copy_factor_to_sm(h, tmp);
-    if (rc == 0 && h->mode == 2) rc = cc_factor_layout(h, tmp);
+    if (rc == 0 && (h->mode == 2 || h->mode == 4)) rc = cc_factor_layout(h, tmp);
     if (rc == 0) {
         hipError_t e = hipMemcpyAsync(sm_out, tmp, bytes, hipMemcpyDeviceToHost, h->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);

@@ -5,6 +5,7 @@
 // / MSPAR(27) / TOL(2:3) / DCOR outputs:
 //   sprgpu_sprfc1  <- SPRFC1  sprfc1.f:5-283
 //   sprgpu_sprfc3  <- SPRFC3  sprfc3.f:5-280
+//   sprgpu_sprfc4  <- SPRFC4  sprfc4.f:5-265 (the CC' factor with SPRFC3-style replacement)
 //   sprgpu_sprfs1  <- SPRFS1  sprfs1.f:5-263 (+ SPRFS4 permutation, :396-528)
 //   sprgpu_sprbs1  <- SPRBS1  sprbs1.f:5-248 (+ SPRBS5 permutation, :371-495)
 //   sprgpu_sprsol  <- SPRSOL  sprsol.f:5-52
@@ -47,9 +48,11 @@ std::map<const double *, Entry> g_cache;
 unsigned long long g_clock = 0;
 // LIERR of the last drop-in factorization on this thread: the reference's
 // internal error code behind IERR = -3 (sprfc1.f:250-262, sprfc2.f:244-255,
-// sprfc3.f:250-262): -30 a pivot at or below TOL(1)|a_jj| (SPRFC1), -32 the
-// same in SPRFC2, -31 pivots replaced (SPRFC3), -35 a zero diagonal before
-// the factorization; 0 otherwise.  The Fortran layer passes it to SPRER1.
+// sprfc3.f:250-262, sprfc4.f:240-256): -30 a pivot at or below
+// TOL(1)|a_jj| (SPRFC1), -32 the same in SPRFC2 and a pivot below
+// -TOL(1)|a_jj| in SPRFC4, -31 pivots replaced (SPRFC3), -33 pivots
+// replaced (SPRFC4), -35 a zero diagonal before the factorization; 0
+// otherwise.  The Fortran layer passes it to SPRER1.
 thread_local i64 g_lierr = 0;
 
 // Signature of a factor in SM: FNV-1a over the bits of SM(1:NEQ) and of
@@ -170,8 +173,10 @@ void factor_dropin(int mode, sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *
                    double *sm, double *tol, double *dcor, const sprgpu_int *lpu,
                    sprgpu_int *ierr, const char *name)
 {
-    // stage protocol (sprfc1.f:140-148; SPRFC2 uses 6, sprfc2.f)
-    const sprgpu_int stage = mode == 2 ? 6 : 5;
+    // stage protocol (sprfc1.f:140-148; SPRFC2 and SPRFC4 use 6, sprfc2.f,
+    // sprfc4.f:133-141)
+    const bool cc = mode == 2 || mode == 4;
+    const sprgpu_int stage = cc ? 6 : 5;
     g_lierr = 0;
     if (mspar[0] < 0) {
         mspar[0] = -stage;
@@ -202,12 +207,15 @@ void factor_dropin(int mode, sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *
     sprgpu_handle *h = e->h;
     const i64 neq = h->plan.neq, n = neq + h->plan.nzerol;
     sprgpu_factor_info info{};
-    double t4[4] = {tol[0], 0.0, 0.0, mode == 3 ? tol[3] : 0.0};
+    double t4[4] = {tol[0], 0.0, 0.0, mode >= 3 ? tol[3] : 0.0};
     hipError_t he = hipMemcpy(e->d_sm, sm, sizeof(double) * n, hipMemcpyHostToDevice);
     if (he != hipSuccess) rc = hip_fail(he, "upload SM");
     if (!rc) rc = sprgpu_factor(h, e->d_sm, t4, mode, &info);
     if (!rc) rc = copy_factor_to_sm(h, e->d_sm);
-    if (!rc && mode == 2 && info.ierr == 0) rc = cc_factor_layout(h, e->d_sm);
+    // C = L sqrt(D) in SM once every pivot is taken (SPRFC4: also with
+    // replaced pivots, MSPAR(27) < 0)
+    if (!rc && cc && (info.ierr == 0 || (mode == 4 && info.supeqt < 0)))
+        rc = cc_factor_layout(h, e->d_sm);
     if (!rc) {
         he = hipMemcpy(sm, e->d_sm, sizeof(double) * n, hipMemcpyDeviceToHost);
         if (he != hipSuccess) rc = hip_fail(he, "download SM");
@@ -216,7 +224,7 @@ void factor_dropin(int mode, sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *
         e->sig = factor_signature(sm, neq, h->plan.nzerol);
         e->sig_valid = true;
     }
-    if (!rc && mode == 3 && dcor) {
+    if (!rc && mode >= 3 && dcor) {
         he = hipMemcpy(dcor, h->d.dcor, sizeof(double) * 2 * neq, hipMemcpyDeviceToHost);
         if (he != hipSuccess) rc = hip_fail(he, "download DCOR");
     }
@@ -231,7 +239,11 @@ void factor_dropin(int mode, sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *
     if (info.ierr == -3) {
         *ierr = -3;
         mspar[26] = info.supeqt;
-        g_lierr = (mode != 3 && info.nzero_diag > 0) ? -35 : mode == 3 ? -31 : mode == 2 ? -32 : -30;
+        g_lierr = (mode <= 2 && info.nzero_diag > 0) ? -35
+                  : mode == 3                        ? -31
+                  : mode == 4                        ? (info.supeqt < 0 ? -33 : -32)
+                  : mode == 2                        ? -32
+                                                     : -30;
     } else if (info.ierr == 3) {
         *ierr = 3;
     }
@@ -382,6 +394,18 @@ void sprgpu_sprfc2(sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa, dou
     (void)mvmult;
     (void)mmmult;
     factor_dropin(2, mspar, mtrees, msifa, sm, tol, nullptr, lpu, ierr, "SPRFC2");
+}
+
+void sprgpu_sprfc4(sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa, double *sm,
+                   double *tol, sprgpu_int *iwork, double *rwork, double *dcor,
+                   const sprgpu_int *lpu, sprgpu_int *ierr, void (*mvmult)(void),
+                   void (*mmmult)(void))
+{
+    (void)iwork;
+    (void)rwork;
+    (void)mvmult;
+    (void)mmmult;
+    factor_dropin(4, mspar, mtrees, msifa, sm, tol, dcor, lpu, ierr, "SPRFC4");
 }
 
 void sprgpu_sprfs2(sprgpu_int *mspar, sprgpu_int *mtrees, sprgpu_int *msifa, double *sm,

@@ -112,12 +112,12 @@ __global__ __launch_bounds__(256) void k_front_init(const ColTask *__restrict__ 
                 double v = src[0];
                 nnz += (v != 0.0);
                 const int eq = n.first + c;
-                if (mode == 3) {
+                if (mode >= 3) {
                     dcor[2 * i64(eq)] = 0.0;
                     dcor[2 * i64(eq) + 1] = 0.0;
                 }
                 if (v == 0.0) {
-                    if (mode == 3) {  // sprfc3.f:560-575
+                    if (mode >= 3) {  // sprfc3.f:560-575, sprfc4.f:459-472
                         v = 1.0;
                         dcor[2 * i64(eq) + 1] = 1.0;
                     } else {          // sprfc1.f:473-480
@@ -253,12 +253,12 @@ __global__ __launch_bounds__(256) void k_small(const i32 *__restrict__ list, int
                 nnz += (v != 0.0);
                 if (r == c) {
                     const int eq = n.first + c;
-                    if (pa.mode == 3) {
+                    if (pa.mode >= 3) {
                         dcor[2 * i64(eq)] = 0.0;
                         dcor[2 * i64(eq) + 1] = 0.0;
                     }
                     if (v == 0.0) {
-                        if (pa.mode == 3) {  // sprfc3.f:560-575
+                        if (pa.mode >= 3) {  // sprfc3.f:560-575, sprfc4.f:459-472
                             v = 1.0;
                             dcor[2 * i64(eq) + 1] = 1.0;
                         } else {             // sprfc1.f:473-480
@@ -415,8 +415,9 @@ __device__ __forceinline__ T diag_block_factor(const T *src, i64 srs, i64 scs, i
     // barrier.
     // The pairs are unrolled (j is a constant in each copy, so the thread's
     // entries of columns j, j+1 are named statically) and leave at nb.
-    // Outside SPRFC3 mode no pivot is replaced, and the FP64 test is skipped.
-    const bool replace = pa.mode == 3;
+    // Outside SPRFC3/SPRFC4 mode no pivot is replaced, and the FP64 test is
+    // skipped.
+    const bool replace = pa.mode >= 3;
     auto pivot = [&](T dj, double a, double &dp) -> T {
         if (replace) {
             dp = pivot_value(double(dj), a, pa);
@@ -1816,13 +1817,16 @@ int read_factor_info(sprgpu_handle *h, sprgpu_factor_info *info)
     }
     if (h->mode == 2) return cc_factor_info(h, info);
     const i64 neq = h->plan.neq;
+    // modes 3 and 4 replace zero diagonals (no failure); modes 1, 2 and 4
+    // fail on a pivot (4: only below -TOL(1)|a_jj|)
+    const bool zero_fails = h->mode <= 2, pivot_fails = h->mode != 3;
     info->nzero_diag = i64(st.nzero);
     info->ierr = 0;
     info->numneg = 0;
     info->supeqt = 0;
     info->tol2 = 0;
     info->tol3 = 0;
-    if (h->mode != 3 && st.nzero > 0) {
+    if (zero_fails && st.nzero > 0) {
         // zero diagonals before the factorization: sprfc1.f:463-488
         info->ierr = -3;
         info->numneg = i64(st.nzero);
@@ -1831,14 +1835,15 @@ int read_factor_info(sprgpu_handle *h, sprgpu_factor_info *info)
         info->tol3 = agg[2];
         return 0;
     }
-    if (h->mode != 3 && st.first_fail != ULLONG_MAX) {
-        // pivot reduced to zero: sprfc1.f:1274-1279, :604-611
+    if (pivot_fails && st.first_fail != ULLONG_MAX) {
+        // pivot reduced to zero: sprfc1.f:1274-1279, :604-611; SPRFC4's
+        // negative pivot: sprfc4.f:1016-1025, :611-614 (MSPAR(26) stays 0)
         const i64 eq = i64(st.first_fail);
         double dfail = 0, afail = 0;
         SPRGPU_HIP(hipMemcpy(&dfail, h->d.dvec + eq, sizeof(double), hipMemcpyDeviceToHost));
         SPRGPU_HIP(hipMemcpy(&afail, h->d.adiag + eq, sizeof(double), hipMemcpyDeviceToHost));
         info->ierr = -3;
-        info->numneg = i64(agg[0]);
+        info->numneg = h->mode == 4 ? 0 : i64(agg[0]);
         info->supeqt = eq + 1;
         info->tol2 = dfail;
         info->tol3 = afail;
@@ -1848,7 +1853,8 @@ int read_factor_info(sprgpu_handle *h, sprgpu_factor_info *info)
     info->tol2 = (agg[1] != 0.0) ? 1.0 / agg[1] : 0.0;
     const double mdiag = (agg[3] == DBL_MAX) ? 0.0 : agg[3];
     info->tol3 = (mdiag != 0.0) ? agg[2] / mdiag : 0.0;
-    if (h->mode == 3 && st.nrepl > 0) {
+    if (h->mode == 4) info->numneg = 0;  // SPRFC4 counts no negative pivots (sprfc4.f:438)
+    if (h->mode >= 3 && st.nrepl > 0) {
         info->ierr = -3;
         info->supeqt = -i64(st.nrepl);
     } else if (info->numneg > 0) {
@@ -1867,7 +1873,7 @@ int copy_factor_to_sm(sprgpu_handle *h, double *sm_out)
     const Plan &p = h->plan;
     i32 node_limit = i32(p.nsuper);
     i64 diag_limit = p.neq;
-    if (h->mode != 3 && st.nzero > 0) {
+    if (h->mode <= 2 && st.nzero > 0) {
         node_limit = 0;
         diag_limit = 0;
     } else if (h->mode != 3 && st.first_fail != ULLONG_MAX) {

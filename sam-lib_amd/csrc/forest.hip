@@ -186,12 +186,12 @@ __device__ __forceinline__ void ascend(const ForestArgs &a, int id, ForestLds<T>
         if (r < S) {  // the pivots' a_jj: sprfc1.f:473-480, sprfc3.f:560-575
             const int eq = nd.first + r;
             double v = a.lnz[nd.xlnz + pcol(r, F)];
-            if (pa.mode == 3) {
+            if (pa.mode >= 3) {
                 a.dcor[2 * i64(eq)] = 0.0;
                 a.dcor[2 * i64(eq) + 1] = 0.0;
             }
             if (v == 0.0) {
-                if (pa.mode == 3) {
+                if (pa.mode >= 3) {
                     v = 1.0;
                     a.dcor[2 * i64(eq) + 1] = 1.0;
                     L.front[prow(r) + r] = T(1);

@@ -83,7 +83,8 @@ struct NodeInfo {
 };
 
 // Pivot rule arguments of the factor kernels: TOL(1), TOL(4) and the mode
-// (1: SPRFC1, record a failing pivot; 3: SPRFC3, replace it).
+// (1: SPRFC1, record a failing pivot; 3: SPRFC3, replace it; 4: SPRFC4,
+// replace it or, below -TOL(1)|a_jj|, record the failure).
 struct PivotArgs {
     double tol1, tol4;
     int mode;
@@ -355,7 +356,8 @@ struct sprgpu_handle {
     int device = 0;
     bool factored = false;
     bool has_pattern = false;
-    int mode = 1;               // 1 SPRFC1, 2 SPRFC2 (CC' from the LDL' factor), 3 SPRFC3
+    int mode = 1;               // 1 SPRFC1, 2 SPRFC2 (CC' from the LDL' factor), 3 SPRFC3,
+                                // 4 SPRFC4 (CC' from the LDL' factor with SPRFC4's pivot rule)
     double cc_tol1 = 0.0;       // TOL(1) of the last SPRFC2-mode factorization
     double tol[4] = {0, 0, 0, 0};
     const double *last_sm_dev = nullptr;  // input of the last factorization

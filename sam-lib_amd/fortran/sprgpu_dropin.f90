@@ -1,8 +1,8 @@
 ! Fortran drop-in layer of sprgpu.
 !
-! SPRFC1, SPRFC2, SPRFC3, SPRFS1, SPRFS2, SPRBS1, SPRBS2 and SPRSOL with the
+! SPRFC1, SPRFC2, SPRFC3, SPRFC4, SPRFS1, SPRFS2, SPRBS1, SPRBS2 and SPRSOL with the
 ! argument lists of the reference routines (src/SPR/sprfc1.f:5-7,
-! sprfc2.f:5-7, sprfc3.f:5-7, sprfs1.f:5-7, sprfs2.f:5-7, sprbs1.f:5-6,
+! sprfc2.f:5-7, sprfc3.f:5-7, sprfc4.f:5-7, sprfs1.f:5-7, sprfs2.f:5-7, sprbs1.f:5-6,
 ! sprbs2.f:5-6, sprsol.f:5-6), forwarding through ISO_C_BINDING to the C ABI
 ! declared in include/sprgpu.h.  A program that links this library ahead of
 ! the reference SPR library keeps its call sequence unchanged: the symbolic
@@ -36,6 +36,13 @@ module sprgpu_c
     end subroutine
     subroutine c_sprfc3(mspar, mtrees, msifa, sm, tol, iwork, rwork, dcor, lpu, ierr, mv, mm) &
         bind(C, name='sprgpu_sprfc3')
+      import :: c_int64_t, c_double, c_funptr
+      integer(c_int64_t) :: mspar(*), mtrees(*), msifa(*), iwork(*), lpu, ierr
+      real(c_double) :: sm(*), tol(*), rwork(*), dcor(*)
+      type(c_funptr), value :: mv, mm
+    end subroutine
+    subroutine c_sprfc4(mspar, mtrees, msifa, sm, tol, iwork, rwork, dcor, lpu, ierr, mv, mm) &
+        bind(C, name='sprgpu_sprfc4')
       import :: c_int64_t, c_double, c_funptr
       integer(c_int64_t) :: mspar(*), mtrees(*), msifa(*), iwork(*), lpu, ierr
       real(c_double) :: sm(*), tol(*), rwork(*), dcor(*)
@@ -98,8 +105,8 @@ contains
     if (ierr == -5) call hip_message(name, lpu)
     if (ierr == -3) then
       lierr = c_lierr()
-      if (lierr == -31) then
-        ! SPRFC3: SUPEQT counts the replaced pivots, MSPAR(27) = -SUPEQT
+      if (lierr == -31 .or. lierr == -33) then
+        ! SPRFC3, SPRFC4: SUPEQT counts the replaced pivots, MSPAR(27) = -SUPEQT
         call SPRER1(40_c_int64_t, name, mspar(26), -mspar(27), lierr, lpu, ierr)
         if (lpu >= 0) then
           write(lpu, 700) tol(1), -mspar(27)
@@ -112,8 +119,8 @@ contains
         call SPRER1(40_c_int64_t, name, mspar(26), mspar(27), lierr, lpu, ierr)
         if (lpu >= 0 .and. lierr == -30) write(lpu, 600) tol(1), tol(2), tol(3), &
                                                     abs(tol(2) / tol(3))
-        if (lpu >= 0 .and. lierr == -32) write(lpu, 601) tol(1), tol(2), tol(3), &
-                                                    tol(2) / abs(tol(3))
+        if (lpu >= 0 .and. lierr == -32 .and. name == 'SPRFC2') write(lpu, 601) tol(1), &
+                                                    tol(2), tol(3), tol(2) / abs(tol(3))
       end if
     else if (ierr == 3) then
       call SPRER1(60_c_int64_t, name, 0_c_int64_t, mspar(27), mspar(26), lpu, ierr)
@@ -193,6 +200,17 @@ subroutine SPRFC3(MSPAR, MTREES, MSIFA, SM, TOL, IWORK, RWORK, DCOR, LPU, IERR, 
                 c_null_funptr, c_null_funptr)
   call factor_messages('SPRFC3', MSPAR, TOL, DCOR, MSPAR(8), LPU, IERR)
 end subroutine SPRFC3
+
+subroutine SPRFC4(MSPAR, MTREES, MSIFA, SM, TOL, IWORK, RWORK, DCOR, LPU, IERR, MVMULT, MMMULT)
+  use sprgpu_c
+  implicit none
+  integer(c_int64_t) :: MSPAR(*), MTREES(*), MSIFA(*), IWORK(*), LPU, IERR
+  real(c_double) :: SM(*), TOL(4), RWORK(*), DCOR(*)
+  external :: MVMULT, MMMULT
+  call c_sprfc4(MSPAR, MTREES, MSIFA, SM, TOL, IWORK, RWORK, DCOR, LPU, IERR, &
+                c_null_funptr, c_null_funptr)
+  call factor_messages('SPRFC4', MSPAR, TOL, DCOR, MSPAR(8), LPU, IERR)
+end subroutine SPRFC4
 
 subroutine SPRFS1(MSPAR, MTREES, MSIFA, SM, B, LDB, NRHS, IWORK, RWORK, LPU, IERR)
   use sprgpu_c

@@ -34,6 +34,7 @@ class IrInfo(C.Structure):
 SIGNATURES = {
     "sprgpu_sprfc1": (None, [P] * 9 + [P, P]),
     "sprgpu_sprfc3": (None, [P] * 10 + [P, P]),
+    "sprgpu_sprfc4": (None, [P] * 10 + [P, P]),
     "sprgpu_sprfs1": (None, [P] * 11),
     "sprgpu_sprbs1": (None, [P] * 10),
     "sprgpu_sprfc2": (None, [P] * 9 + [P, P]),

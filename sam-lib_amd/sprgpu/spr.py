@@ -7,6 +7,7 @@ same:
 
     sprfc1(mspar, mtrees, msifa, sm, tol)            # SPRFC1, sprfc1.f:5-7
     sprfc3(mspar, mtrees, msifa, sm, tol, dcor)      # SPRFC3, sprfc3.f:5-7
+    sprfc4(mspar, mtrees, msifa, sm, tol, dcor)      # SPRFC4, sprfc4.f:5-7 (CC')
     sprfs1(mspar, mtrees, msifa, sm, b)              # SPRFS1, sprfs1.f:5-7
     sprbs1(mspar, mtrees, msifa, sm, b)              # SPRBS1, sprbs1.f:5-6
     sprsol(iop, mspar, mtrees, msifa, sm, b, tol)    # SPRSOL, sprsol.f:5-6
@@ -68,6 +69,17 @@ def sprfc3(mspar, mtrees, msifa, sm, tol, dcor, lpu: int = -1) -> int:
         raise ValueError("SPRFC3 needs TOL(4)")
     ierr = C.c_int64(0)
     lib().sprgpu_sprfc3(_p(mspar), _p(mtrees), _p(msifa), _p(sm), _p(tol), None, None,
+                        _p(dcor), _ref(lpu), C.byref(ierr), None, None)
+    return int(ierr.value)
+
+
+def sprfc4(mspar, mtrees, msifa, sm, tol, dcor, lpu: int = -1) -> int:
+    _check_int(mspar, mtrees, msifa)
+    _check_real(sm, tol, dcor)
+    if tol.size < 4:
+        raise ValueError("SPRFC4 needs TOL(4)")
+    ierr = C.c_int64(0)
+    lib().sprgpu_sprfc4(_p(mspar), _p(mtrees), _p(msifa), _p(sm), _p(tol), None, None,
                         _p(dcor), _ref(lpu), C.byref(ierr), None, None)
     return int(ierr.value)
 

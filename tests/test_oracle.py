@@ -344,3 +344,93 @@ def test_reference_cct_is_ldlt_scaled():
     x = b.copy(order="F")
     assert oracle.sprsol(3, sysm.mspar.copy(), sysm.mtrees, sysm.msifa, sm.copy(), x)[0] == 0
     np.testing.assert_allclose(y, x, rtol=1e-10, atol=1e-12 * np.abs(x).max())
+
+
+def _dense_lower(sysm, lnz):
+    """The packed supernodal lower triangle (SM(NEQ+1:) layout) as a dense matrix."""
+    n = sysm.neq
+    xs, xl = sysm.xsuper() - 1, sysm.xlindx() - 1
+    rows_all = sysm.lindx() - 1
+    Ld = np.zeros((n, n))
+    pos = 0
+    for s in range(sysm.nsuper):
+        rows = rows_all[xl[s]:xl[s + 1]]
+        for k in range(xs[s + 1] - xs[s]):
+            Ld[rows[k:], xs[s] + k] = lnz[pos:pos + len(rows) - k]
+            pos += len(rows) - k
+    return Ld
+
+
+def test_reference_sprfc4_rules():
+    """The reference's SPRFC4 (sprfc4.f), the checker of the GPU's mode 4, on
+    the rules the GPU restates: with TOL(1) = 0 it is SPRFC2 bit for bit;
+    with replacements its C satisfies C C' = A + sum_j (DCOR(2,j) -
+    DCOR(1,j)) e_j e_j' (a replaced pivot is a modified a_jj), the replaced
+    values are TOL(1) max(1, |TOL(4) a_jj|) and MSPAR(27) = -count; on an
+    indefinite matrix it fails at the first LDL' pivot below -TOL(1)|a_jj|
+    (from the reference SPRFC1), with TOL(2:3) = (d, a_jj), DCOR = (d, d)."""
+    import types
+
+    import refspr
+
+    model, sysm, sm = make_system("hex8", 4, e_sigma=0.5, e_seed=1)
+    n = sysm.neq
+    c4, ms4 = sm.copy(), sysm.mspar.copy()
+    ierr, tol4, dcor = refspr.fc4(sysm, c4, (0.0, 0.0, 0.0, 0.0), mspar=ms4)
+    c2, ms2 = sm.copy(), sysm.mspar.copy()
+    assert ierr == 0 and refspr.fc2(sysm, c2, mspar=ms2)[0] == 0
+    assert np.array_equal(c4, c2) and ms4[0] == ms2[0] == 6 and not dcor.any()
+    # replacements
+    A = dense_equation_matrix(model, sysm)
+    tol = (0.6, 0.0, 0.0, 1.7)
+    c4, ms4 = sm.copy(), sysm.mspar.copy()
+    ierr, tolv, dcor = refspr.fc4(sysm, c4, tol, mspar=ms4)
+    rep = dcor[:, 1] != 0
+    assert ierr == -3 and rep.sum() > 0 and ms4[26] == -rep.sum() and ms4[25] == 0
+    a = np.diag(A)
+    np.testing.assert_array_equal(dcor[rep, 1], 0.6 * np.maximum(1.0, np.abs(1.7 * a[rep])))
+    assert np.all(np.abs(dcor[rep, 0]) <= 0.6 * np.abs(a[rep]))
+    Cd = _dense_lower(sysm, c4[n:])
+    np.testing.assert_allclose(np.diag(Cd), c4[:n], rtol=0)
+    Acorr = A + np.diag(np.where(rep, dcor[:, 1] - dcor[:, 0], 0.0))
+    err = np.abs(Cd @ Cd.T - Acorr).max() / np.abs(A).max()
+    assert err < 1e-12, err
+    # a planted mistake (one correction left out) breaks the identity
+    Aw = Acorr.copy()
+    j = int(np.flatnonzero(rep)[0])
+    Aw[j, j] -= dcor[j, 1] - dcor[j, 0]
+    assert np.abs(Cd @ Cd.T - Aw).max() / np.abs(A).max() > 1e-6
+    # the first negative pivot fails
+    g = dict(np.load(os.path.join(os.path.dirname(__file__), "golden", "hex8_3_indefinite.npz"),
+                     allow_pickle=False))
+    s = types.SimpleNamespace(mspar=g["mspar_in"], mtrees=g["mtrees"], msifa=g["msifa"])
+    c4, ms4 = g["sm_in"].copy(), g["mspar_in"].copy()
+    ierr, tolv, dcor = refspr.fc4(s, c4, (1e-12, 0.0, 0.0, 1.0), mspar=ms4)
+    ld, ms1 = g["sm_in"].copy(), g["mspar_in"].copy()
+    ierr1, _ = oracle.sprsol(1, ms1, g["mtrees"], g["msifa"], ld)
+    assert ierr1 == 3  # negative pivots, counted (NUMNEG)
+    neq = int(g["mspar_in"][7])
+    d = ld[:neq]
+    adiag = _assembled_diagonal(s, g["sm_in"])
+    first = int(np.flatnonzero(d < -1e-12 * np.abs(adiag))[0])
+    assert ierr == -3 and ms4[26] == first + 1 and ms4[25] == 0
+    np.testing.assert_allclose(tolv[1:3], [d[first], adiag[first]], rtol=1e-12)
+    np.testing.assert_allclose(dcor[first], [d[first], d[first]], rtol=1e-12)
+
+
+def _assembled_diagonal(sysm, sm):
+    """a_jj of the assembled SM: the first entry of each packed column."""
+    mspar = sysm.mspar
+    ns, neq = int(mspar[10]), int(mspar[7])
+    p45, p46, p48 = int(mspar[44]) - 1, int(mspar[45]) - 1, int(mspar[47]) - 1
+    xsuper = sysm.mtrees[p45:p45 + ns + 1] - 1
+    xlindx = sysm.mtrees[p46:p46 + ns + 1] - 1
+    xlnz = sysm.msifa[p48:p48 + ns + 1] - 1
+    a = np.empty(neq)
+    for s in range(ns):
+        f = xlindx[s + 1] - xlindx[s]
+        pos = xlnz[s]
+        for k in range(xsuper[s + 1] - xsuper[s]):
+            a[xsuper[s] + k] = sm[neq + pos]
+            pos += f - k
+    return a
