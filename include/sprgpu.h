@@ -248,6 +248,18 @@ int sprgpu_sprprm(sprgpu_handle *handle, const double *a_dev, const double *b_de
                   double *c_dev, double *wa_dev, sprgpu_int m, sprgpu_int n, sprgpu_int ksa,
                   sprgpu_int iflag);
 
+/* Replaces SPRTRA (src/SPR/sprtra.f:5-56) on device arrays: C = B' A B for
+ * B(M,N) and A as in sprgpu_sprprm (KSA > 0 the SPR layout, KSA <= 0 a
+ * diagonal); C is N x N, column-major, symmetric.  As the reference: per
+ * column J, WA = A B(:,J) by SPRPRM (IFLAG 88; wa_dev, M doubles, holds the
+ * last column's on return), then C(I,J) = C(J,I) = B(:,I) . WA for I >= J.
+ * The dot products are summed in a fixed order of this library's own (the
+ * reference calls the host BLAS DDOT, whose order is the BLAS build's), so
+ * C agrees with the reference's to rounding.  Returns 0, -2 (arguments) or
+ * sprgpu_sprprm's error. */
+int sprgpu_sprtra(sprgpu_handle *handle, const double *a_dev, const double *b_dev,
+                  double *c_dev, double *wa_dev, sprgpu_int m, sprgpu_int n, sprgpu_int ksa);
+
 /* Replaces SPRTMM (src/SPR/sprtmm.f:5-7) on device arrays.  With L the stored
  * lower triangle, diagonal included, of U (NZEROL doubles in the LNZ layout:
  * an assembled matrix or a factor), for each of the N columns of B(M,N):

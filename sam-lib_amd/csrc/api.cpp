@@ -597,6 +597,20 @@ int sprgpu_sprprm(sprgpu_handle *h, const double *a_dev, const double *b_dev, do
     return 0;
 }
 
+int sprgpu_sprtra(sprgpu_handle *h, const double *a_dev, const double *b_dev, double *c_dev,
+                  double *wa_dev, sprgpu_int m, sprgpu_int n, sprgpu_int ksa)
+{
+    // sprtra.f:38-50: per column J, WA = A B(:, J) (SPRPRM, IFLAG 88), then
+    // C(I, J) = C(J, I) = B(:, I) . WA for I >= J
+    if (!h || !a_dev || !b_dev || !c_dev || !wa_dev || n < 1) return -2;
+    for (i64 j = 0; j < n; ++j) {
+        if (int rc = sprgpu_sprprm(h, a_dev, b_dev + j * m, nullptr, wa_dev, m, 1, ksa, 88))
+            return rc;
+        if (int rc = run_tra_dots(h, b_dev, m, wa_dev, n, j, c_dev)) return rc;
+    }
+    return 0;
+}
+
 int sprgpu_sprtmm(sprgpu_handle *h, const double *u_dev, const double *b_dev, double *c_dev,
                   double *wa_dev, sprgpu_int m, sprgpu_int n, sprgpu_int ksu, sprgpu_int iflag)
 {

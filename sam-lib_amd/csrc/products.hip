@@ -262,4 +262,51 @@ int run_sprtmm(sprgpu_handle *h, const double *u, const double *b, double *c, do
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// SPRTRA's dot products (sprtra.f:40-46): C(I, J) = C(J, I) = B(:, I) . WA
+// for I = J..N-1, one workgroup per I.  Each thread sums its strided rows in
+// order (eight loads in flight), the workgroup's 256 sums are added by a
+// fixed tree: the result does not depend on scheduling.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_tra_dots(const double *__restrict__ b, i64 ldb,
+                                                  const double *__restrict__ wa, i64 m, i64 j,
+                                                  double *c, i64 ldc)
+{
+    __shared__ double part[256];
+    const i64 i = j + blockIdx.x;
+    const double *bi = b + i * ldb;
+    const int t = threadIdx.x;
+    double s = 0.0;
+    i64 r = t;
+    for (; r + 7 * 256 < m; r += 8 * 256) {
+        double x[8], y[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            x[u] = bi[r + u * 256];
+            y[u] = wa[r + u * 256];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += x[u] * y[u];
+    }
+    for (; r < m; r += 256) s += bi[r] * wa[r];
+    part[t] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) part[t] += part[t + w];
+        __syncthreads();
+    }
+    if (t == 0) {
+        c[i + j * ldc] = part[0];
+        c[j + i * ldc] = part[0];
+    }
+}
+
+int run_tra_dots(sprgpu_handle *h, const double *b, i64 m, const double *wa, i64 n, i64 j,
+                 double *c)
+{
+    k_tra_dots<<<unsigned(n - j), 256, 0, h->stream>>>(b, m, wa, m, j, c, n);
+    SPRGPU_HIP(hipGetLastError());
+    return 0;
+}
+
 }  // namespace sprgpu

@@ -438,6 +438,10 @@ int run_lanczos(sprgpu_handle *h, const double *a_sm, const double *b, int ksb, 
 // products.hip
 int run_sprtmm(sprgpu_handle *h, const double *u, const double *b, double *c, double *wa,
                i64 m, i64 n, i64 ksu, i64 iflag);
+// C(i, j) = C(j, i) = B(:, i) . wa for i = j..n-1 (B is m x n, C n x n,
+// column-major): SPRTRA's dot products
+int run_tra_dots(sprgpu_handle *h, const double *b, i64 m, const double *wa, i64 n, i64 j,
+                 double *c);
 // noalloc.hip: the static scratch of sprgpu_solve_noalloc, located once per
 // device at handle creation
 int noalloc_scratch_init();

@@ -55,6 +55,7 @@ SIGNATURES = {
     "sprgpu_solve_noalloc": (C.c_int, [P, P, I64, I64, C.c_int]),
     "sprgpu_spmv": (C.c_int, [P, P, P, I64, I64]),
     "sprgpu_sprprm": (C.c_int, [P, P, P, P, P, I64, I64, I64, I64]),
+    "sprgpu_sprtra": (C.c_int, [P, P, P, P, P, I64, I64, I64]),
     "sprgpu_sprtmm": (C.c_int, [P, P, P, P, P, I64, I64, I64, I64]),
     "sprgpu_lanczos": (C.c_int, [P, P, P, C.c_int, C.c_double, I64, I64, I64, C.c_double, P, P,
                                   P, P]),

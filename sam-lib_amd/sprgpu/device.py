@@ -157,6 +157,19 @@ class SprFactor:
               "sprgpu_sprprm")
         return c_dev if iflag != 88 else wa_dev
 
+    def sprtra(self, a_dev, b_dev, ksa: int = 1):
+        """SPRTRA on the device: C = B^T A B (n x n) for b_dev (n, m) float64 (the
+        n columns of B(M, N)), A as in sprprm.  Returns (C as an (n, n) tensor,
+        WA = A b of the last column)."""
+        torch = self._torch
+        b2 = b_dev if b_dev.dim() == 2 else b_dev.view(1, -1)
+        n, m = b2.shape
+        c = torch.empty(n, n, dtype=torch.float64, device=b2.device)
+        wa = torch.empty(m, dtype=torch.float64, device=b2.device)
+        check(lib().sprgpu_sprtra(self._h, _ptr(a_dev), _ptr(b2), _ptr(c), _ptr(wa), m, n,
+                                  int(ksa)), "sprgpu_sprtra")
+        return c.T, wa
+
     def sprtmm(self, u_dev, b_dev, c_dev=None, iflag: int = 1, ksu: int = 1, wa_dev=None):
         """SPRTMM on the device: C = L^T B (iflag 1) or L B (2) with L the stored
         lower triangle of u_dev (LNZ, or a diagonal when ksu <= 0); 11/22 with

@@ -149,3 +149,38 @@ def test_sprtmm_matches_oracle(kind, n, kw):
                                           sysm.msifa, iflag, ksu)
             assert ierr == 0 and np.array_equal(bits(wa.cpu().numpy()), bits(wa_o))
     h.close()
+
+
+@pytest.mark.parametrize("kind,n,kw", [("hex8", 10, dict(e_sigma=0.5, e_seed=1)),
+                                       ("quad4", 30, {})])
+def test_sprtra_is_bt_a_b(kind, n, kw):
+    """SPRTRA (sprtra.f:5-56): C = B' A B.  The reference cannot be built here
+    (it calls the host BLAS DDOT, absent from the image), so C is held to the
+    mathematics: B' W with W = A B from the oracle's SPRPRM (bit-exact to the
+    reference's), to 1e-13 of |B'||A B|, exactly symmetric; WA is the last
+    column's A b bit for bit (the reference's SPRPRM IFLAG 88); a diagonal A
+    (KSA <= 0) gives B' diag(a) B."""
+    from sprgpu.device import SprFactor
+
+    model, sysm, sm = make_system(kind, n, **kw)
+    h = SprFactor(sysm.mspar, sysm.mtrees, sysm.msifa, "fp64")
+    try:
+        h.set_pattern(sm)
+        rng = np.random.default_rng(4)
+        b = rng.normal(size=(sysm.neq, 6))
+        a = sm[sysm.neq:]
+        c, wa = h.sprtra(_dev(a), _dev(b.T), ksa=1)
+        c = c.cpu().numpy()
+        w = _oracle_prm(sysm, a, b, 1)
+        want = b.T @ w
+        scale = np.abs(b).T @ np.abs(w)
+        assert np.all(np.abs(c - want) <= 1e-13 * scale), np.abs(c - want).max()
+        assert np.array_equal(c, c.T)
+        assert np.array_equal(wa.cpu().numpy(), w[:, -1])
+        dvec = rng.normal(size=sysm.neq)
+        cd, _ = h.sprtra(_dev(dvec), _dev(b.T), ksa=0)
+        want = b.T @ (dvec[:, None] * b)
+        scale = np.abs(b).T @ (np.abs(dvec)[:, None] * np.abs(b))
+        assert np.all(np.abs(cd.cpu().numpy() - want) <= 1e-13 * scale)
+    finally:
+        h.close()
