@@ -160,9 +160,9 @@ int sprgpu_set_pattern(sprgpu_handle *handle, const double *sm_host);
  * order, 0 for a dof without one: SPREEQ's MEEN (spradm.f:169-221).  Builds
  * the map from element entries to LNZ slots and sets the residual pattern
  * to the slots that receive an entry (so sprgpu_set_pattern is not needed).
- * Returns -2 for a constrained dof (MEEN < 0: its TTCC couplings,
- * spradm.f:861-1002, stay with the host SPRADM), an equation beyond NEQ, or
- * an element coupling two equations outside the symbolic pattern.
+ * Returns -2 for a constrained dof (MEEN < 0: use sprgpu_set_assembly_ceq
+ * below), an equation beyond NEQ, or an element coupling two equations
+ * outside the symbolic pattern.
  *
  * sprgpu_assemble (device arrays, asynchronous on the handle's stream):
  * SM(NEQ+1:) receives sum_e EM_e(I,J) over I <= J, in SPRADM's order (by
@@ -176,6 +176,47 @@ int sprgpu_set_assembly(sprgpu_handle *handle, sprgpu_int nel, const sprgpu_int 
                         const sprgpu_int *meen);
 int sprgpu_assemble(sprgpu_handle *handle, const double *em_dev, const sprgpu_int *em_off_dev,
                     const double *escale_dev, double *sm_dev, int accumulate);
+
+/* Element assembly with constraint equations: the whole of SPRAD2
+ * (src/SPR/spradm.f:743-1002) for a host loop that calls SPRADM with
+ * NSV = nsv, TTCC, MPMCEQ, MMCEQ and MEQN (spradm.f:5-13).
+ *
+ * sprgpu_set_assembly_ceq (host arrays): as sprgpu_set_assembly, where
+ * meen may also hold -ICEQ for a constrained dof (SPREEQ's MEEN), ICEQ
+ * naming equation ICEQ of mpmceq(nceq+1)/mmceq (SAM's MPMCEQ and MMCEQ:
+ * equation I holds its constrained dof at MPMCEQ(I) and its masters after
+ * it), meqn(ndof) SAM's MEQN (the masters' equations).  nsv: SPRADM's NSV
+ * -- 0: SM only; 1: SM and one vector SV(NEQ); -1: SV only; |nsv| > 1: SV
+ * holds |nsv| vectors, one per prescribed equation (no masters) in equation
+ * order, and slaves add nothing to SV (spradm.f:866-881).  Returns -2 for
+ * the errors of sprgpu_set_assembly, MPMCEQ(1) != 1 or an equation without
+ * its constrained dof, a master with MMCEQ in 1..NDOF whose MEQN is not in
+ * 1..NEQ, -ICEQ beyond NCEQ, or (|nsv| > 1) an element prescribed dof
+ * whose vector lies beyond |nsv|.  The map depends on MPMCEQ/MMCEQ/MEQN only;
+ * TTCC is read at every assembly, so C0 and the coefficients may change.
+ *
+ * sprgpu_assemble_ceq (device arrays, asynchronous): for nsv >= 0, SM as
+ * sprgpu_assemble plus, per constrained dof J and master JP of it,
+ * TTCC(JP)*EM(I,J) at (MEEN(I), MEQN(master)) (2*TTCC(JP) on the master's
+ * own diagonal) and TTCC(IP)*TTCC(JP)*EM(I,J) for each master IP of a
+ * constrained I with MEQN(IP's dof) <= MEQN(JP's dof); for nsv != 0,
+ * SV(NEQ*|nsv|) -= C0*EM(I,J) (C0 = TTCC(MPMCEQ(JCEQ))) at the free I and,
+ * weighted by TTCC(IP), at the masters of a constrained I.  Every position
+ * sums its terms in SPRADM's order, so SM and SV equal the host loop's bit
+ * for bit when the element matrices are finite and the output is cleared
+ * (accumulate 0).  The host loop skips a term whose TTCC entry is zero; the
+ * device adds it (a signed zero), which changes no bits of a sum that starts
+ * at +0, but can turn a -0 of an accumulated input into +0.
+ * ttcc_dev: TTCC(NMMCEQ) in device memory (may be NULL if no element has a
+ * constrained dof); sm_dev may be NULL for nsv < 0, sv_dev for nsv = 0.
+ * accumulate 0 clears SM (nsv >= 0) and SV (nsv != 0) first. */
+int sprgpu_set_assembly_ceq(sprgpu_handle *handle, sprgpu_int nel, const sprgpu_int *meen_ptr,
+                            const sprgpu_int *meen, sprgpu_int nceq, const sprgpu_int *mpmceq,
+                            const sprgpu_int *mmceq, sprgpu_int ndof, const sprgpu_int *meqn,
+                            sprgpu_int nsv);
+int sprgpu_assemble_ceq(sprgpu_handle *handle, const double *em_dev, const sprgpu_int *em_off_dev,
+                        const double *escale_dev, const double *ttcc_dev, double *sm_dev,
+                        double *sv_dev, int accumulate);
 
 /* Numeric factorization of SM given in device memory (NEQ+NZEROL doubles, the
  * SPRADM layout; not modified).  tol points to host TOL(1..4) (TOL(4) is read

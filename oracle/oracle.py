@@ -95,6 +95,26 @@ def assemble_free(mspar, mtrees, msifa, meen_ptr, meen, em, em_off, escale=None,
     return int(rc), sm
 
 
+def assemble_ceq(mspar, mtrees, msifa, meen_ptr, meen, em, em_off, escale, mpmceq, mmceq,
+                 ttcc, meqn, nsv, sm=None, sv=None):
+    """Restated SPRADM element loop with constraint equations (SPRAD2 complete):
+    SM(NEQ+NZEROL) for nsv >= 0 and SV(NEQ*|nsv|) for nsv != 0.  Returns
+    (rc, sm, sv); zero arrays unless given (then updated in place)."""
+    neq, nzerol = int(mspar[7]), int(mspar[15])
+    sm = np.zeros(neq + nzerol) if sm is None else sm
+    sv = np.zeros(max(neq * max(abs(nsv), 1), 1)) if sv is None else sv
+    a = [np.ascontiguousarray(x, dtype=I8) for x in (meen_ptr, meen, em_off, mpmceq, mmceq, meqn)]
+    em = np.ascontiguousarray(em, dtype=np.float64)
+    tt = np.ascontiguousarray(ttcc, dtype=np.float64)
+    sc = None if escale is None else np.ascontiguousarray(escale, dtype=np.float64)
+    f = lib().oracle_assemble_ceq
+    f.restype = C.c_int64
+    rc = f(_p(mspar), _p(mtrees), _p(msifa), C.c_int64(a[0].size - 1), _p(a[0]), _p(a[1]),
+           _p(em), _p(a[2]), _p(sc) if sc is not None else None, _p(a[3]), _p(a[4]), _p(tt),
+           _p(a[5]), C.c_int64(nsv), _p(sm), _p(sv))
+    return int(rc), sm, sv
+
+
 def sprtmm(u, b, mspar, mtrees, msifa, iflag=1, ksu=1):
     """Restated SPRTMM: IFLAG 1 C = L^T B, 2 C = L B (L the stored lower
     triangle of U); 11/22 with one column: the result in WA.  Returns

@@ -737,6 +737,32 @@ static fint lnz_position(const spr_view *v, const fint *col_sup, fint r, fint c)
     return 0;
 }
 
+/* SPRAD2's free loop for element e (spradm.f:818-859): 0 or -(e+1). */
+static fint free_element(const spr_view *v, const fint *col_sup, const fint *meen,
+                         const fint *meen_ptr, const double *em, const fint *em_off,
+                         const double *escale, fint e, double *lnz)
+{
+    const fint *mn = meen + meen_ptr[e];
+    const fint nd = meen_ptr[e + 1] - meen_ptr[e];
+    const double *ek = em + em_off[e];
+    for (fint j = 1; j <= nd; ++j) {
+        const fint jeq = F1(mn, j);
+        if (jeq < 1) continue;
+        for (fint i = 1; i <= j; ++i) {
+            const fint ieq = F1(mn, i);
+            if (ieq < 1) continue;
+            const fint r = ieq > jeq ? ieq : jeq, c = ieq > jeq ? jeq : ieq;
+            const fint k = lnz_position(v, col_sup, r, c);
+            if (k == 0) return -(e + 1);
+            /* the host loop scales the base matrix before the call */
+            const double x = ek[(i - 1) + (j - 1) * nd];
+            const double emij = escale ? escale[e] * x : x;
+            F1(lnz, k) = F1(lnz, k) + emij;
+        }
+    }
+    return 0;
+}
+
 fint oracle_assemble_free(const fint *mspar, const fint *mtrees, const fint *msifa, fint nel,
                           const fint *meen_ptr, const fint *meen, const double *em,
                           const fint *em_off, const double *escale, double *sm)
@@ -749,29 +775,124 @@ fint oracle_assemble_free(const fint *mspar, const fint *mtrees, const fint *msi
         for (fint c = F1(v.xsuper, js); c < F1(v.xsuper, js + 1); ++c) col_sup[c - 1] = js;
     double *lnz = sm + v.neq;  /* SPRADM passes SM(NEQ+1) (spradm.f:161) */
     fint rc = 0;
-    for (fint e = 0; e < nel && rc == 0; ++e) {
+    for (fint e = 0; e < nel && rc == 0; ++e)
+        rc = free_element(&v, col_sup, meen, meen_ptr, em, em_off, escale, e, lnz);
+    free(col_sup);
+    return rc;
+}
+
+/* SPRAD2 complete (spradm.f:743-1002), element by element as a host loop
+ * over SPRADM leaves it: the free part (above) when nsv >= 0, then the
+ * constrained dofs (MEEN(J) = -JCEQ < 0).  SV(NEQ*|nsv|) receives
+ * SV(KEQ) - C0*EM(I,J) for each constrained J with C0 = TTCC(MPMCEQ(JCEQ))
+ * != 0 (spradm.f:866-908; for |nsv| > 1 only prescribed J, into vector K,
+ * the rank of JCEQ among the equations without masters), and, when nsv >= 0,
+ * SM receives TTCC(JP)*EM(I,J) per master JP of J (doubled on the master's
+ * own diagonal) and TTCC(IP)*TTCC(JP)*EM(I,J) per master pair with
+ * MEQN(master I) <= MEQN(master J) (spradm.f:910-999).  Masters with
+ * MMCEQ <= 0 or TTCC = 0 are skipped.  Returns 0, -(e+1) for an element
+ * entry outside the pattern, or -(nel+1) for a vector index beyond |nsv|. */
+fint oracle_assemble_ceq(const fint *mspar, const fint *mtrees, const fint *msifa, fint nel,
+                         const fint *meen_ptr, const fint *meen, const double *em,
+                         const fint *em_off, const double *escale, const fint *mpmceq,
+                         const fint *mmceq, const double *ttcc, const fint *meqn, fint nsv,
+                         double *sm, double *sv)
+{
+    spr_view v;
+    make_view(mspar, mtrees, msifa, &v);
+    const fint neq = v.neq;
+    fint *col_sup = (fint *)malloc(sizeof(fint) * (size_t)(neq > 0 ? neq : 1));
+    if (!col_sup) return -1;
+    for (fint js = 1; js <= v.nsuper; ++js)
+        for (fint c = F1(v.xsuper, js); c < F1(v.xsuper, js + 1); ++c) col_sup[c - 1] = js;
+    double *lnz = sm + neq;
+    const fint nvec = nsv < 0 ? -nsv : nsv;
+    fint rc = 0;
+#define SM_ADD(r_, c_, val_)                                                  \
+    do {                                                                      \
+        const fint k_ = lnz_position(&v, col_sup, (r_), (c_));                \
+        if (k_ == 0) {                                                        \
+            rc = -(e + 1);                                                    \
+            goto done;                                                        \
+        }                                                                     \
+        F1(lnz, k_) = F1(lnz, k_) + (val_);                                   \
+    } while (0)
+    for (fint e = 0; e < nel; ++e) {
         const fint *mn = meen + meen_ptr[e];
         const fint nd = meen_ptr[e + 1] - meen_ptr[e];
         const double *ek = em + em_off[e];
-        for (fint j = 1; j <= nd && rc == 0; ++j) {
-            const fint jeq = F1(mn, j);
-            if (jeq < 1) continue;
-            for (fint i = 1; i <= j; ++i) {
-                const fint ieq = F1(mn, i);
-                if (ieq < 1) continue;
-                const fint r = ieq > jeq ? ieq : jeq, c = ieq > jeq ? jeq : ieq;
-                const fint k = lnz_position(&v, col_sup, r, c);
-                if (k == 0) {
-                    rc = -(e + 1);
-                    break;
+        if (nsv >= 0 && (rc = free_element(&v, col_sup, meen, meen_ptr, em, em_off, escale, e,
+                                            lnz)) != 0)
+            goto done;
+        for (fint j = 1; j <= nd; ++j) {
+            if (F1(mn, j) >= 0) continue;
+            const fint jceq = -F1(mn, j);
+            fint jp = F1(mpmceq, jceq);
+            if (nsv != 0) {
+                const double c0 = F1(ttcc, jp);
+                fint kp = -1;
+                if (c0 != 0.0) {
+                    if (nvec == 1) {
+                        kp = 0;
+                    } else if (F1(mpmceq, jceq + 1) - F1(mpmceq, jceq) == 1) {
+                        fint k = 0;
+                        for (fint i = 1; i <= jceq; ++i)
+                            if (F1(mpmceq, i + 1) - F1(mpmceq, i) == 1) ++k;
+                        if (k > nvec) {
+                            rc = -(nel + 1);
+                            goto done;
+                        }
+                        kp = (k - 1) * neq;
+                    }
                 }
-                /* the host loop scales the base matrix before the call */
-                const double x = ek[(i - 1) + (j - 1) * nd];
-                const double emij = escale ? escale[e] * x : x;
-                F1(lnz, k) = F1(lnz, k) + emij;
+                for (fint i = 1; kp >= 0 && i <= nd; ++i) {
+                    const fint ieq = F1(mn, i);
+                    if (ieq == 0) continue;
+                    const double x = ek[(i - 1) + (j - 1) * nd];
+                    const double emij = escale ? escale[e] * x : x;
+                    if (ieq > 0) {
+                        F1(sv, kp + ieq) = F1(sv, kp + ieq) - c0 * emij;
+                        continue;
+                    }
+                    const fint iceq = -ieq;
+                    for (fint ip = F1(mpmceq, iceq) + 1; ip < F1(mpmceq, iceq + 1); ++ip) {
+                        if (F1(mmceq, ip) <= 0 || F1(ttcc, ip) == 0.0) continue;
+                        const fint keq = kp + F1(meqn, F1(mmceq, ip));
+                        F1(sv, keq) = F1(sv, keq) - c0 * F1(ttcc, ip) * emij;
+                    }
+                }
+                if (nsv < 0) continue;
+            }
+            const fint nmstj = F1(mpmceq, jceq + 1) - F1(mpmceq, jceq) - 1;
+            for (fint jj = 1; jj <= nmstj; ++jj) {
+                ++jp;
+                if (F1(mmceq, jp) <= 0 || F1(ttcc, jp) == 0.0) continue;
+                const fint jeq = F1(meqn, F1(mmceq, jp));
+                for (fint i = 1; i <= nd; ++i) {
+                    const fint ieq = F1(mn, i);
+                    if (ieq == 0) continue;
+                    const double x = ek[(i - 1) + (j - 1) * nd];
+                    const double emij = escale ? escale[e] * x : x;
+                    if (ieq > 0) {
+                        if (ieq != jeq)
+                            SM_ADD(ieq > jeq ? ieq : jeq, ieq > jeq ? jeq : ieq,
+                                   F1(ttcc, jp) * emij);
+                        else
+                            SM_ADD(ieq, ieq, (F1(ttcc, jp) + F1(ttcc, jp)) * emij);
+                        continue;
+                    }
+                    const fint iceq = -ieq;
+                    for (fint ip = F1(mpmceq, iceq) + 1; ip < F1(mpmceq, iceq + 1); ++ip) {
+                        if (F1(mmceq, ip) <= 0 || F1(ttcc, ip) == 0.0) continue;
+                        const fint meq = F1(meqn, F1(mmceq, ip));
+                        if (meq <= jeq) SM_ADD(jeq, meq, F1(ttcc, ip) * F1(ttcc, jp) * emij);
+                    }
+                }
             }
         }
     }
+#undef SM_ADD
+done:
     free(col_sup);
     return rc;
 }

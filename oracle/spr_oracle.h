@@ -17,6 +17,8 @@
  *                              stored lower triangle)
  *   oracle_assemble_free <- SPRADM called once per element (spradm.f:5-166),
  *                              its SPRAD2 free-dof loop (spradm.f:818-859)
+ *   oracle_assemble_ceq  <- the same with SPRAD2's constrained part
+ *                              (spradm.f:861-1002) and the SV vectors
  * The MVMULT/MMMULT callback arguments of SPRFC1/SPRFC3 are fixed to the
  * restated LMVY18/LMMY18 (what SPRSOL passes, sprsol.f:29-35).
  */
@@ -58,6 +60,18 @@ void oracle_sprprm(const double *a, const double *b, double *c, double *wa, fint
 fint oracle_assemble_free(const fint *mspar, const fint *mtrees, const fint *msifa, fint nel,
                           const fint *meen_ptr, const fint *meen, const double *em,
                           const fint *em_off, const double *escale, double *sm);
+/* SPRADM per element with constraint equations (SPRAD2, spradm.f:743-1002):
+ * with nsv >= 0 the free loop above, then for every constrained dof J
+ * (MEEN(J) = -JCEQ) the weighted columns: SV(NEQ*|nsv|) -= C0*EM(:,J) (for
+ * nsv != 0) and SM += TTCC*EM(:,J) through J's masters (for nsv >= 0), in
+ * SPRAD2's loop order.  meqn: MEQN(NDOF), to find the masters' equations.
+ * Returns 0, -(e+1) for element e outside the pattern, -(nel+1) if a
+ * prescribed dof's vector lies beyond |nsv|, -1 on allocation failure. */
+fint oracle_assemble_ceq(const fint *mspar, const fint *mtrees, const fint *msifa, fint nel,
+                         const fint *meen_ptr, const fint *meen, const double *em,
+                         const fint *em_off, const double *escale, const fint *mpmceq,
+                         const fint *mmceq, const double *ttcc, const fint *meqn, fint nsv,
+                         double *sm, double *sv);
 /* SPRTMM(U,B,C,WA,MSPAR,MTREES,MSIFA,M,N,KSU,IFLAG,LPU,IERR): with L the
  * stored lower triangle of U (diagonal included), IFLAG 1: C = L^T B, 2:
  * C = L B; 11/22 (N = 1): the same into WA only (SPRTMQ, sprtmm.f:242-344).

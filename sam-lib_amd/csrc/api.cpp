@@ -465,14 +465,38 @@ int sprgpu_set_assembly(sprgpu_handle *h, sprgpu_int nel, const sprgpu_int *meen
                         const sprgpu_int *meen)
 {
     if (!h || nel < 0 || (nel > 0 && (!meen_ptr || !meen))) return -2;
-    return set_assembly(h, nel, meen_ptr, meen);
+    return set_assembly(h, nel, meen_ptr, meen, nullptr);
+}
+
+int sprgpu_set_assembly_ceq(sprgpu_handle *h, sprgpu_int nel, const sprgpu_int *meen_ptr,
+                            const sprgpu_int *meen, sprgpu_int nceq, const sprgpu_int *mpmceq,
+                            const sprgpu_int *mmceq, sprgpu_int ndof, const sprgpu_int *meqn,
+                            sprgpu_int nsv)
+{
+    if (!h || nel < 0 || (nel > 0 && (!meen_ptr || !meen))) return -2;
+    CeqInput c;
+    c.nceq = nceq;
+    c.mpmceq = mpmceq;
+    c.mmceq = mmceq;
+    c.ndof = ndof;
+    c.meqn = meqn;
+    c.nsv = nsv;
+    return set_assembly(h, nel, meen_ptr, meen, &c);
 }
 
 int sprgpu_assemble(sprgpu_handle *h, const double *em_dev, const sprgpu_int *em_off_dev,
                     const double *escale_dev, double *sm_dev, int accumulate)
 {
     if (!h || !sm_dev || (!em_dev && h->d.asm_nel > 0)) return -2;
-    return run_assemble(h, em_dev, em_off_dev, escale_dev, sm_dev, accumulate);
+    return run_assemble(h, em_dev, em_off_dev, escale_dev, nullptr, sm_dev, nullptr, accumulate);
+}
+
+int sprgpu_assemble_ceq(sprgpu_handle *h, const double *em_dev, const sprgpu_int *em_off_dev,
+                        const double *escale_dev, const double *ttcc_dev, double *sm_dev,
+                        double *sv_dev, int accumulate)
+{
+    if (!h || (!em_dev && h->d.asm_nel > 0)) return -2;
+    return run_assemble(h, em_dev, em_off_dev, escale_dev, ttcc_dev, sm_dev, sv_dev, accumulate);
 }
 
 int sprgpu_factor(sprgpu_handle *h, const double *sm_dev, const double *tol, int mode,

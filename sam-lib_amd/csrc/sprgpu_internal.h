@@ -327,12 +327,17 @@ struct DeviceBuffers {
     i32 *tmm_long = nullptr;    // columns summed by a wavefront each (SPRTMM's L^T B)
     i64 tmm_nlong = 0;
     i64 irwork_cap = 0;
-    // element assembly map: LNZ positions that receive element entries
-    // (ascending), the range of each one's contributions, and the
-    // contributions in SPRADM order, each (element << 32) | (I + J*nedof)
+    // element assembly map: the positions that receive element entries
+    // (ascending; LNZ positions, then NZEROL + the SV entries), the range of
+    // each one's contributions, and the contributions in SPRADM order, each
+    // (element << 32) | (I + J*nedof); with constraint equations also each
+    // contribution's weight code (kind << 56 | IP << 28 | JP, into TTCC)
     i64 *asm_tpos = nullptr, *asm_tptr = nullptr, *asm_src = nullptr;
     i64 *asm_emoff = nullptr;   // default element-matrix offsets (consecutive)
+    i64 *asm_code = nullptr;
     i64 asm_ntgt = 0, asm_ncontrib = 0, asm_nel = 0;
+    i64 asm_nsv = 0;            // NSV of the map (0: SM only)
+    bool asm_ceq = false;       // built by set_assembly with constraint equations
     // PERI2E (MSPAR(58) >= 1): internal -> external equation, 0-based, and
     // the work space of the permuted entry points (two blocks)
     i32 *peri2e = nullptr;
@@ -453,8 +458,17 @@ int perm_work(sprgpu_handle *h, i64 nrhs, double **w0, double **w1);
 int perm_gather(sprgpu_handle *h, const double *src, i64 lds, double *dst, i64 ldd, i64 nrhs);
 int perm_scatter(sprgpu_handle *h, const double *src, i64 lds, double *dst, i64 ldd, i64 nrhs);
 // assemble.hip
-int set_assembly(sprgpu_handle *h, i64 nel, const i64 *meen_ptr, const i64 *meen);
+// constraint equations of the assembly (SAM's MPMCEQ, MMCEQ, MEQN; 1-based)
+struct CeqInput {
+    i64 nceq = 0;
+    const i64 *mpmceq = nullptr, *mmceq = nullptr;
+    i64 ndof = 0;
+    const i64 *meqn = nullptr;
+    i64 nsv = 0;
+};
+int set_assembly(sprgpu_handle *h, i64 nel, const i64 *meen_ptr, const i64 *meen,
+                 const CeqInput *ceq);
 int run_assemble(sprgpu_handle *h, const double *em, const i64 *em_off, const double *escale,
-                 double *sm, int accumulate);
+                 const double *ttcc, double *sm, double *sv, int accumulate);
 void free_assembly(sprgpu_handle *h);
 }  // namespace sprgpu

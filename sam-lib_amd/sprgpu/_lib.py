@@ -49,6 +49,8 @@ SIGNATURES = {
     "sprgpu_set_pattern": (C.c_int, [P, P]),
     "sprgpu_set_assembly": (C.c_int, [P, I64, P, P]),
     "sprgpu_assemble": (C.c_int, [P, P, P, P, P, C.c_int]),
+    "sprgpu_set_assembly_ceq": (C.c_int, [P, I64, P, P, I64, P, P, I64, P, I64]),
+    "sprgpu_assemble_ceq": (C.c_int, [P, P, P, P, P, P, P, C.c_int]),
     "sprgpu_factor": (C.c_int, [P, P, P, C.c_int, C.POINTER(FactorInfo)]),
     "sprgpu_factor_info_get": (C.c_int, [P, C.POINTER(FactorInfo)]),
     "sprgpu_solve": (C.c_int, [P, P, I64, I64, C.c_int]),

@@ -107,6 +107,27 @@ class SprFactor:
                                     _ptr(sm_dev), int(bool(accumulate))), "sprgpu_assemble")
         return sm_dev
 
+    def set_assembly_ceq(self, meen_ptr: np.ndarray, meen: np.ndarray, mpmceq: np.ndarray,
+                         mmceq: np.ndarray, meqn: np.ndarray, nsv: int = 0):
+        """``set_assembly`` with constraint equations: meen may hold -ICEQ
+        (SPREEQ's MEEN), MPMCEQ/MMCEQ/MEQN are SAM's arrays (1-based), nsv is
+        SPRADM's NSV (0 SM, 1 SM+SV, -1 SV, |nsv|>1 one SV per prescribed dof)."""
+        a = [np.ascontiguousarray(x, dtype=np.int64) for x in (meen_ptr, meen, mpmceq, mmceq, meqn)]
+        check(lib().sprgpu_set_assembly_ceq(self._h, a[0].size - 1, _np(a[0]), _np(a[1]),
+                                            a[2].size - 1, _np(a[2]), _np(a[3]), a[4].size,
+                                            _np(a[4]), int(nsv)),
+              "sprgpu_set_assembly_ceq")
+
+    def assemble_ceq(self, em_dev, ttcc_dev, sm_dev=None, sv_dev=None, em_off_dev=None,
+                     escale_dev=None, accumulate: bool = False):
+        """SM and/or SV (device float64) with the constraint equations of
+        ``set_assembly_ceq``; TTCC (device) is read at this call."""
+        opt = (lambda t: None if t is None else _ptr(t))
+        check(lib().sprgpu_assemble_ceq(self._h, _ptr(em_dev), opt(em_off_dev), opt(escale_dev),
+                                        opt(ttcc_dev), opt(sm_dev), opt(sv_dev),
+                                        int(bool(accumulate))), "sprgpu_assemble_ceq")
+        return sm_dev, sv_dev
+
     def factor(self, sm_dev, tol=(0.0, 0.0, 0.0, 0.0), mode: int = 1, info: bool = True):
         tolv = np.zeros(4)
         tolv[:len(tol)] = tol

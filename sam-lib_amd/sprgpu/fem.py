@@ -8,7 +8,11 @@ INTEGER*8):
 * MADOF(NANOD+1) -- accumulated dofs per node
 * MSC(NDOF)  -- status code per dof: 1 free, 0 fixed (sprseq.f:61-78)
 * MPMNPC(NEL+1), MMNPC -- element -> node connectivity
-* MPMCEQ(NCEQ+1), MMCEQ, TTCC -- constraint equations (none here)
+* MPMCEQ(NCEQ+1), MMCEQ, TTCC -- constraint equations: equation I has its
+  constrained dof MMCEQ(MPMCEQ(I)) (status code 0) with TTCC(MPMCEQ(I)) = c0,
+  and masters MMCEQ(MPMCEQ(I)+1 .. MPMCEQ(I+1)-1) with their coefficients, so
+  u_slave = c0 + sum(TTCC * u_master); no masters: a prescribed dof
+  (sprseq.f:108-140)
 * MINEX(NANOD) -- external node numbers (used in messages only)
 
 Each generator also returns the element matrices, column-major, in the element
@@ -43,14 +47,30 @@ class SamModel:
     em_off: np.ndarray         # (nel,) int64 offsets into em
     escale: np.ndarray | None  # optional per-element multiplier on em
     meta: dict = field(default_factory=dict)
+    mpmceq: np.ndarray | None = None  # (nceq+1,) int64, 1-based; None: no constraints
+    mmceq: np.ndarray | None = None   # (nmmceq,) int64, 1-based dofs
+    ttcc: np.ndarray | None = None    # (nmmceq,) float64
+
+    @property
+    def nceq(self) -> int:
+        return 0 if self.mpmceq is None else int(self.mpmceq.size - 1)
 
     def mpar(self) -> np.ndarray:
         mpar = np.zeros(50, dtype=I8)
         mpar[0] = self.nanod
         mpar[1] = self.nel
         mpar[2] = self.ndof
-        mpar[6] = 0  # NCEQ
+        mpar[6] = self.nceq  # NCEQ
         return mpar
+
+    def ceq_arrays(self):
+        """(MPMCEQ, MMCEQ, TTCC) as the reference routines take them; one-entry
+        placeholders when the model has no constraint equations."""
+        if self.mpmceq is None:
+            return np.ones(1, dtype=I8), np.zeros(1, dtype=I8), np.zeros(1)
+        return (np.ascontiguousarray(self.mpmceq, dtype=I8),
+                np.ascontiguousarray(self.mmceq, dtype=I8),
+                np.ascontiguousarray(self.ttcc, dtype=np.float64))
 
     def element_matrix(self, iel: int) -> np.ndarray:
         """Element matrix of element ``iel`` (0-based) as a dense array."""
@@ -257,6 +277,54 @@ def hex8_grid(n: int, E: float = 1.0, nu: float = 0.3, e_sigma: float | None = N
     return SamModel(f"hex8_{n}", nanod, nel, ndof, madof, msc, mpmnpc, mmnpc,
                     np.full(nel, 24, dtype=I8), K0.ravel(order="F").copy(),
                     np.zeros(nel, dtype=I8), escale, meta)
+
+
+def hex8_constrained(n: int, e_sigma: float | None = 0.5, e_seed: int | None = 1) -> SamModel:
+    """hex8_grid(n) with constraint equations on the face x = 1 (nodes i = n,
+    k >= 1), in node order and per node in dof order:
+
+    * ux slave: 0.75 ux(n-1, j, k) + 0.25 ux(n-1, j', k) + 0.0 uz(n-1, j, k),
+      j' = j+1 (j-1 on the edge j = n), c0 = 1e-3 k when j is a multiple of 3,
+      else 0 (a master with coefficient zero, which the assembly skips);
+    * uy slave (k < n): 0.5 uz(n, j, k), a master in the same node, c0 = 0;
+    * uz prescribed (k = n): c0 = -0.01 for even j, else 0.
+
+    The constrained dofs get status code 0 and the masters stay free, as SPRSEQ
+    requires (sprseq.f:108-140).
+    """
+    mdl = hex8_grid(n, e_sigma=e_sigma, e_seed=e_seed)
+    m = n + 1
+
+    def dof(i, j, k, c):
+        return 3 * (i + m * (j + m * k)) + c  # 0-based
+
+    mp, mm, tt = [1], [], []
+
+    def add(slave, c0, masters=()):
+        mdl.msc[slave] = 0
+        mm.append(slave + 1)
+        tt.append(c0)
+        for d, t in masters:
+            mm.append(d + 1)
+            tt.append(t)
+        mp.append(len(mm) + 1)
+
+    for k in range(1, m):
+        for j in range(m):
+            j2 = j + 1 if j < n else j - 1
+            add(dof(n, j, k, 0), 1e-3 * k if j % 3 == 0 else 0.0,
+                [(dof(n - 1, j, k, 0), 0.75), (dof(n - 1, j2, k, 0), 0.25),
+                 (dof(n - 1, j, k, 2), 0.0)])
+            if k < n:
+                add(dof(n, j, k, 1), 0.0, [(dof(n, j, k, 2), 0.5)])
+            else:
+                add(dof(n, j, k, 2), -0.01 if j % 2 == 0 else 0.0)
+    mdl.name = f"hex8c_{n}"
+    mdl.meta.update(kind="hex8c")
+    mdl.mpmceq = np.array(mp, dtype=I8)
+    mdl.mmceq = np.array(mm, dtype=I8)
+    mdl.ttcc = np.array(tt, dtype=np.float64)
+    return mdl
 
 
 def quad4_grid(n: int, E: float = 1.0, nu: float = 0.25, t: float = 1.0) -> SamModel:

@@ -119,8 +119,7 @@ def symbolic(model, nspar: int = NSPAR, cache_kb: int = 0, nperm: int = 0) -> Sp
     mpmnpc = np.ascontiguousarray(model.mpmnpc, dtype=I8)
     mmnpc = np.ascontiguousarray(model.mmnpc, dtype=I8)
     minex = np.arange(1, nanod + 1, dtype=I8)
-    mpmceq = np.ones(1, dtype=I8)
-    mmceq = np.zeros(1, dtype=I8)
+    mpmceq, mmceq, _ = _ceq(model)
     mspar = np.zeros(nspar, dtype=I8)
     meqn = np.zeros(ndof, dtype=I8)
     ierr = C.c_int64(0)
@@ -184,20 +183,40 @@ def set_permutation(sysm: SprSystem, meqn_ext: np.ndarray) -> None:
         raise SprError(f"SPRABC IERR={ierr.value}")
 
 
+def _ceq(model):
+    """(MPMCEQ, MMCEQ, TTCC) of a model, placeholders when it has none."""
+    f = getattr(model, "ceq_arrays", None)
+    if f is None:
+        return np.ones(1, dtype=I8), np.zeros(1, dtype=I8), np.zeros(1)
+    return f()
+
+
 def assemble(model, sysm: SprSystem) -> np.ndarray:
     """Assemble all element matrices with the reference SPRADM (spradm.f:5).
 
     Returns SM(NEQ+NZEROL): SM(1:NEQ) is untouched scratch (zero) and
     SM(NEQ+1:) holds the lower triangle of A in the supernodal LNZ layout.
+    Constraint equations of the model enter through SPRAD2's constrained part
+    (spradm.f:861-1002).
     """
+    return assemble_rhs(model, sysm, 0)[0]
+
+
+def assemble_rhs(model, sysm: SprSystem, nsv: int, sm: np.ndarray | None = None,
+                 sv: np.ndarray | None = None):
+    """SPRADM over all elements with NSV = nsv (spradm.f:5, SPRAD2 at
+    spradm.f:743-1002): nsv >= 0 adds to SM, nsv != 0 subtracts the
+    prescribed (and, for |nsv| = 1, dependent) dofs' columns, weighted by
+    their c0, from SV(NEQ, |nsv|).  Returns (sm, sv); zero arrays unless given
+    (then updated in place)."""
     L = lib()
     neq, nzerol = sysm.neq, sysm.nzerol
-    sm = np.zeros(neq + nzerol)
-    sv = np.zeros(max(neq, 1))
+    if sm is None:
+        sm = np.zeros(neq + nzerol)
+    if sv is None:
+        sv = np.zeros(max(neq * max(abs(nsv), 1), 1))
     iwork = np.zeros(int(sysm.mspar[38]) + 16, dtype=I8)
-    mpmceq = np.ones(1, dtype=I8)
-    mmceq = np.zeros(1, dtype=I8)
-    ttcc = np.zeros(1)
+    mpmceq, mmceq, ttcc = _ceq(model)
     nedof = np.ascontiguousarray(model.nedof, dtype=I8)
     em = np.ascontiguousarray(model.em, dtype=np.float64)
     em_off = np.ascontiguousarray(model.em_off, dtype=I8)
@@ -212,10 +231,10 @@ def assemble(model, sysm: SprSystem) -> np.ndarray:
         _p(scale) if scale is not None else None, _p(ttcc), _p(sysm.mpar), _p(sysm.mspar),
         _p(madof), _p(sysm.meqn), _p(mpmnpc), _p(mmnpc), _p(mpmceq), _p(mmceq),
         _p(sysm.msica), _p(sysm.mtrees), _p(sysm.msifa), _p(sysm.mdofnc), _p(sm), _p(sv),
-        C.c_int64(0), _p(iwork), C.c_int64(LPU), C.byref(bad))
+        C.c_int64(nsv), _p(iwork), C.c_int64(LPU), C.byref(bad))
     if ierr != 0:
         raise SprError(f"SPRADM IERR={ierr} at element {bad.value}")
-    return sm
+    return sm, sv
 
 
 def loads_to_equations(sysm: SprSystem, f_dof: np.ndarray) -> np.ndarray:

@@ -26,6 +26,8 @@ def make_system(kind: str, n: int, **kw):
     if key not in _CACHE:
         if kind == "hex8":
             model = fem.hex8_grid(n, **kw)
+        elif kind == "hex8c":
+            model = fem.hex8_constrained(n, **kw)
         elif kind == "quad4":
             model = fem.quad4_grid(n, **kw)
         elif kind == "frame":

@@ -7,6 +7,9 @@ scales of one base matrix, hex8 20^3 (the C1 mesh), quad4, beam frames with
 one matrix per element, consecutive element matrices (no offsets), adding to
 an existing SM, the residual pattern that the map sets (FP32 factor + FP64
 refinement on the device-assembled SM, without sprgpu_set_pattern), and the
+error paths.  With constraint equations (sprgpu_set_assembly_ceq): SM and SV
+against the reference SPRADM for NSV = 0, 1, -1, NPDOF, new TTCC values on
+one map, accumulation, the solution of the constrained problem, and the
 error paths.
 """
 import numpy as np
@@ -147,4 +150,145 @@ def test_assembly_error_paths():
                                     model.em_off, model.escale)
     torch.cuda.synchronize()
     assert np.array_equal(_bits(out.cpu().numpy()), _bits(sm_o))
+    h.close()
+
+
+# ---------------------------------------------------------------------------
+# constraint equations (sprgpu_set_assembly_ceq / sprgpu_assemble_ceq)
+# ---------------------------------------------------------------------------
+def _ceq_setup(n, nsv):
+    from sprgpu import host
+    from sprgpu.device import SprFactor
+
+    model, sysm, _ = make_system("hex8c", n)
+    mp, me = host.element_meen(model, sysm)
+    h = SprFactor(sysm.mspar, sysm.mtrees, sysm.msifa, "fp64")
+    mpmceq, mmceq, ttcc = model.ceq_arrays()
+    h.set_assembly_ceq(mp, me, mpmceq, mmceq, sysm.meqn, nsv)
+    return model, sysm, h, mp, me
+
+
+def _npdof(model):
+    return int(np.count_nonzero(np.diff(model.mpmceq) == 1))
+
+
+def _run_ceq(h, model, sysm, nsv, ttcc, sm0=None, sv0=None):
+    torch = _torch()
+    nvec = max(abs(nsv), 1)
+    sm = None if nsv < 0 else (_dev(sm0) if sm0 is not None else torch.full(
+        (sysm.neq + sysm.nzerol,), float("nan"), dtype=torch.float64, device="cuda"))
+    sv = None if nsv == 0 else (_dev(sv0) if sv0 is not None else torch.full(
+        (nvec * sysm.neq,), float("nan"), dtype=torch.float64, device="cuda"))
+    h.assemble_ceq(_dev(model.em), _dev(ttcc), sm, sv, _dev(model.em_off), _dev(model.escale),
+                   accumulate=sm0 is not None or sv0 is not None)
+    torch.cuda.synchronize()
+    return (None if sm is None else sm.cpu().numpy()), (None if sv is None else sv.cpu().numpy())
+
+
+@pytest.mark.parametrize("n", [4, 12])
+def test_device_constrained_assembly_equals_spradm(n):
+    """SM and SV bit for bit against the reference SPRADM with constraint
+    equations, for NSV = 0, 1, -1 and NPDOF."""
+    from sprgpu import host
+
+    model, sysm, _ = make_system("hex8c", n)
+    for nsv in (0, 1, -1, _npdof(model)):
+        print(f"hex8c {n}: nsv {nsv} neq {sysm.neq} nceq {model.nceq}", flush=True)
+        _, _, h, _, _ = _ceq_setup(n, nsv)
+        sm, sv = _run_ceq(h, model, sysm, nsv, model.ttcc)
+        sm_r, sv_r = host.assemble_rhs(model, sysm, nsv)
+        if nsv >= 0:
+            assert np.array_equal(_bits(sm), _bits(sm_r)), nsv
+        if nsv != 0:
+            assert np.array_equal(_bits(sv), _bits(sv_r)), nsv
+        h.close()
+
+
+def test_constrained_assembly_reads_ttcc_at_each_call():
+    """One map, new TTCC values (C0 changed, a zero coefficient made non-zero,
+    a coupling coefficient changed): each assembly equals SPRADM's with that
+    TTCC; and accumulation into given SM/SV."""
+    import dataclasses
+
+    from sprgpu import host
+
+    model, sysm, h, _, _ = _ceq_setup(6, 1)
+    rng = np.random.default_rng(7)
+    for step in range(3):
+        tt = model.ttcc.copy()
+        if step > 0:
+            tt = tt * rng.uniform(0.5, 1.5, tt.size)
+            tt[tt == 0] = rng.uniform(-0.1, 0.1, int(np.count_nonzero(tt == 0)))
+        m2 = dataclasses.replace(model, ttcc=tt)
+        sm, sv = _run_ceq(h, m2, sysm, 1, tt)
+        sm_r, sv_r = host.assemble_rhs(m2, sysm, 1)
+        assert np.array_equal(_bits(sm), _bits(sm_r)), step
+        assert np.array_equal(_bits(sv), _bits(sv_r)), step
+    sm0 = rng.normal(size=sysm.neq + sysm.nzerol)
+    sv0 = rng.normal(size=sysm.neq)
+    sm, sv = _run_ceq(h, model, sysm, 1, model.ttcc, sm0, sv0)
+    sm_r, sv_r = host.assemble_rhs(model, sysm, 1, sm=sm0.copy(), sv=sv0.copy())
+    assert np.array_equal(_bits(sm), _bits(sm_r))
+    assert np.array_equal(_bits(sv), _bits(sv_r))
+    h.close()
+
+
+def test_constrained_assembly_solves_the_constrained_problem():
+    """Device SM and SV, factored and solved on the device with loads F:
+    the solution q gives u = T q + u0 with T'(K u - F) = 0 (equilibrium of
+    the constrained model), to 1e-10."""
+    from test_assembly import _transformation
+
+    torch = _torch()
+    model, sysm, h, _, _ = _ceq_setup(6, 1)
+    sm, sv = _run_ceq(h, model, sysm, 1, model.ttcc)
+    rng = np.random.default_rng(3)
+    F = rng.uniform(-1, 1, model.ndof)
+    T, u0 = _transformation(model, sysm)
+    b = T.T @ F + sv
+    assert h.factor(_dev(sm))["ierr"] == 0
+    x = _dev(b.copy())
+    h.solve(x)
+    torch.cuda.synchronize()
+    q = x.cpu().numpy()
+    K = np.zeros((model.ndof, model.ndof))
+    for e in range(model.nel):
+        d = model.element_dofs(e)
+        K[np.ix_(d, d)] += model.element_matrix(e)
+    u = T @ q + u0
+    r = T.T @ (K @ u - F)
+    assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(T.T @ F)
+    h.close()
+
+
+def test_constrained_assembly_error_paths():
+    from sprgpu import host
+    from sprgpu._lib import SprGpuError
+    from sprgpu.device import SprFactor
+
+    torch = _torch()
+    model, sysm, _ = make_system("hex8c", 4)
+    mp, me = host.element_meen(model, sysm)
+    mpmceq, mmceq, ttcc = model.ceq_arrays()
+    h = SprFactor(sysm.mspar, sysm.mtrees, sysm.msifa, "fp64")
+    with pytest.raises(SprGpuError, match="sprgpu_set_assembly_ceq"):
+        h.set_assembly(mp, me)
+    bad = mpmceq.copy()
+    bad[0] = 2
+    with pytest.raises(SprGpuError, match="MPMCEQ"):
+        h.set_assembly_ceq(mp, me, bad, mmceq, sysm.meqn)
+    bad = mmceq.copy()
+    slave = mmceq[mpmceq[0] - 1]
+    bad[mpmceq[0]] = slave  # a master that is a constrained dof
+    with pytest.raises(SprGpuError, match="no equation"):
+        h.set_assembly_ceq(mp, me, mpmceq, bad, sysm.meqn)
+    with pytest.raises(SprGpuError, match="beyond"):
+        h.set_assembly_ceq(mp, me, mpmceq, mmceq, sysm.meqn, 2)
+    h.set_assembly_ceq(mp, me, mpmceq, mmceq, sysm.meqn, 1)
+    out = torch.empty(sysm.neq + sysm.nzerol, dtype=torch.float64, device="cuda")
+    with pytest.raises(SprGpuError, match="TTCC is required"):
+        h.assemble(_dev(model.em), out, _dev(model.em_off), _dev(model.escale))
+    with pytest.raises(SprGpuError, match="SV is required"):
+        h.assemble_ceq(_dev(model.em), _dev(ttcc), out, None, _dev(model.em_off),
+                       _dev(model.escale))
     h.close()
