@@ -253,6 +253,19 @@ int sprgpu_get_dcor(sprgpu_handle *handle, double *dcor_out);
 int sprgpu_solve(sprgpu_handle *handle, double *b_dev, sprgpu_int ldb, sprgpu_int nrhs,
                  int which);
 
+/* Partial solve, the reference's MSPAR(57) = 1 with LDB < NEQ
+ * (src/SPR/sprfs1.f:200-201, sprbs1.f:189, sprfs2.f:203, sprbs2.f:191): B
+ * holds only the leading LDB equations; the others enter the sweeps as zero
+ * and their results are dropped (SPRFB1/SPRFB2, sprfs1.f:636-848).  The
+ * forward result is the leading part of L^-1 b padded with zeros, the
+ * backward one solves with the trailing equations held at zero, so which 3
+ * gives A11^-1 b1 for the leading block A11.  ldb >= NEQ is sprgpu_solve.
+ * With PERI2E (MSPAR(58) >= 1) the leading equations are external ones and
+ * must be the leading internal ones (-2 otherwise).  Structures with
+ * retained supernodes (SUPSUP > 0) are rejected when the handle is created. */
+int sprgpu_solve_partial(sprgpu_handle *handle, double *b_dev, sprgpu_int ldb, sprgpu_int nrhs,
+                         int which);
+
 /* sprgpu_solve with no allocation (no counterpart in the reference): for
  * callers that cannot allocate device memory during a call -- code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.

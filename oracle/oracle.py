@@ -43,9 +43,10 @@ def sprsol(iop: int, mspar, mtrees, msifa, sm, b=None, tol=None):
         b = np.zeros((neq, 1), order="F")
     assert b.flags.f_contiguous and b.dtype == np.float64
     nrhs = b.shape[1] if b.ndim == 2 else 1
+    ldb = b.shape[0]  # < NEQ: the partial solve of MSPAR(57) = 1
     ierr = C.c_int64(0)
     lib().oracle_sprsol(_i(iop), _p(mspar), _p(mtrees), _p(msifa), _p(sm), _p(b),
-                        _i(neq), _i(nrhs), _p(tolv), None, None, _i(-1), C.byref(ierr))
+                        _i(ldb), _i(nrhs), _p(tolv), None, None, _i(-1), C.byref(ierr))
     return int(ierr.value), tolv
 
 

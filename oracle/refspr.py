@@ -69,10 +69,11 @@ def sol(iop: int, sysm: SprSystem, sm: np.ndarray, b: np.ndarray | None, tol=Non
         b = np.zeros((neq, 1), order="F")
     assert b.flags.f_contiguous and b.dtype == np.float64
     nrhs = b.shape[1] if b.ndim == 2 else 1
+    ldb = b.shape[0]  # < NEQ: the partial solve of MSPAR(57) = 1
     secs = C.c_double(0.0)
     L.spr_sol_timed.restype = C.c_int64
     ierr = L.spr_sol_timed(C.c_int64(iop), _p(mspar), _p(sysm.mtrees), _p(sysm.msifa), _p(sm),
-                           _p(b), C.c_int64(neq), C.c_int64(nrhs), _p(tolv), _p(iwork),
+                           _p(b), C.c_int64(ldb), C.c_int64(nrhs), _p(tolv), _p(iwork),
                            _p(rwork), C.c_int64(LPU), C.byref(secs))
     return int(ierr), secs.value, tolv
 

@@ -584,6 +584,39 @@ int sprgpu_solve(sprgpu_handle *h, double *b, sprgpu_int ldb, sprgpu_int nrhs, i
     return perm_scatter(h, w0, neq, b, ldb, nrhs);
 }
 
+int sprgpu_solve_partial(sprgpu_handle *h, double *b, sprgpu_int ldb, sprgpu_int nrhs,
+                         int which)
+{
+    if (!h || !b || ldb < 0 || which < 1 || which > 3) return -2;
+    const i64 neq = h->plan.neq;
+    if (ldb >= neq) return sprgpu_solve(h, b, ldb, nrhs, which);
+    if (nrhs <= 0 || ldb == 0) return 0;
+    if (!h->factored) {
+        set_error("solve called before a successful factorization");
+        return -1;
+    }
+    // The equations beyond LDB enter as zero, and results for them are
+    // dropped.  In the backward sweep the reference also reads them as zero
+    // where other equations use them; a zero right-hand side on the trailing
+    // internal equations gives exactly that, so with PERI2E the leading
+    // external equations must be the leading internal ones.
+    if (h->d.peri2e && h->d.peri2e_pmax[ldb] != ldb - 1) {
+        set_error("partial solve: with PERI2E the first LDB external equations must be the first "
+                  "LDB internal ones");
+        return -2;
+    }
+    double *w0, *w1;
+    if (int rc = perm_work(h, nrhs, &w0, &w1)) return rc;
+    if (int rc = lead_gather(h, b, ldb, w0, neq, nrhs, ldb)) return rc;
+    if (which & 1)
+        if (int rc = solve_internal(h, w0, neq, nrhs, 1)) return rc;
+    if (which == 3)
+        if (int rc = lead_mask(h, w0, neq, nrhs, ldb)) return rc;
+    if (which & 2)
+        if (int rc = solve_internal(h, w0, neq, nrhs, 2)) return rc;
+    return lead_scatter(h, w0, neq, b, ldb, nrhs, ldb);
+}
+
 int sprgpu_spmv(sprgpu_handle *h, const double *x, double *y, sprgpu_int ld, sprgpu_int nrhs)
 {
     if (!h || !x || !y || ld < h->plan.neq) return -2;

@@ -268,14 +268,10 @@ void solve_dropin(int which, bool cc, sprgpu_int *mspar, sprgpu_int *mtrees, spr
     mspar[0] = stage;
     *ierr = 0;
     const i64 neq = mspar[7];
-    if (mspar[56] == 1 && *ldb < neq) {
-        mspar[0] = -2;
-        *ierr = -2;
-        set_error("partial solve (MSPAR(57)=1, LDB<NEQ) is not supported");
-        report(lpu, name, *ierr);
-        return;
-    }
-    if (!lengths_ok_solve(mspar) || *ldb < neq) {
+    // MSPAR(57) = 1 and LDB < NEQ: only the leading LDB equations
+    // (sprfs1.f:200-201, sprbs1.f:189)
+    const bool partial = mspar[56] == 1 && *ldb < neq;
+    if (!lengths_ok_solve(mspar) || (*ldb < neq && !partial) || *ldb < 0) {
         mspar[0] = -2;
         *ierr = -2;
         set_error("MTREES, MSIFA or LDB are not great enough");
@@ -323,20 +319,25 @@ void solve_dropin(int which, bool cc, sprgpu_int *mspar, sprgpu_int *mtrees, spr
     // With MSPAR(58) >= 1 B is in the caller's external order; the handle
     // applies PERI2E on the device (SPRFS4 / SPRBS5 through SPRFB3 / SPRFB4,
     // sprfs1.f:396-528, sprbs1.f:371-495)
-    if (!rc) {
-        he = hipMemcpy2D(e->d_b, sizeof(double) * neq, b, sizeof(double) * (*ldb),
-                         sizeof(double) * neq, k, hipMemcpyHostToDevice);
+    const i64 rows = partial ? *ldb : neq;  // rows of B held on the device
+    if (!rc && rows > 0) {
+        he = hipMemcpy2D(e->d_b, sizeof(double) * rows, b, sizeof(double) * (*ldb),
+                         sizeof(double) * rows, k, hipMemcpyHostToDevice);
         if (he != hipSuccess) rc = hip_fail(he, "upload B");
     }
-    if (!rc) rc = sprgpu_solve(h, e->d_b, neq, k, which);
+    if (!rc)
+        rc = partial ? sprgpu_solve_partial(h, e->d_b, rows, k, which)
+                     : sprgpu_solve(h, e->d_b, neq, k, which);
     if (!rc) rc = check_solve_error(h);
-    if (!rc) {
-        he = hipMemcpy2D(b, sizeof(double) * (*ldb), e->d_b, sizeof(double) * neq,
-                         sizeof(double) * neq, k, hipMemcpyDeviceToHost);
+    if (!rc && rows > 0) {
+        he = hipMemcpy2D(b, sizeof(double) * (*ldb), e->d_b, sizeof(double) * rows,
+                         sizeof(double) * rows, k, hipMemcpyDeviceToHost);
         if (he != hipSuccess) rc = hip_fail(he, "download B");
     }
     if (rc) {
-        *ierr = -5;
+        // -2: a partial solve the device cannot do (PERI2E not leading)
+        *ierr = rc == -2 ? -2 : -5;
+        if (rc == -2) mspar[0] = -2;
         report(lpu, name, *ierr);
     }
 }

@@ -38,6 +38,45 @@ __global__ __launch_bounds__(256) void k_perm_scatter(const i32 *__restrict__ pe
         dst[c * ldd + perm[i]] = src[c * lds + i];
 }
 
+// The leading equations of a partial solve (MSPAR(57) = 1, LDB < NEQ): the
+// reference gathers zero for an equation beyond LDB and never writes it back
+// (SPRFB1/SPRFB3 and SPRFB2/SPRFB4, sprfs1.f:636-926).  e(i) is PERI2E(i)
+// when perm is set, else i.
+__global__ __launch_bounds__(256) void k_lead_gather(const i32 *__restrict__ perm,
+                                                     const double *__restrict__ src, i64 lds,
+                                                     double *__restrict__ dst, i64 ldd, i64 n,
+                                                     i64 nlead)
+{
+    const i64 c = blockIdx.y;
+    for (i64 i = blockIdx.x * 256 + threadIdx.x; i < n; i += i64(gridDim.x) * 256) {
+        const i64 e = perm ? perm[i] : i;
+        dst[c * ldd + i] = e < nlead ? src[c * lds + e] : 0.0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_lead_mask(const i32 *__restrict__ perm,
+                                                   double *__restrict__ x, i64 ldx, i64 n,
+                                                   i64 nlead)
+{
+    const i64 c = blockIdx.y;
+    for (i64 i = blockIdx.x * 256 + threadIdx.x; i < n; i += i64(gridDim.x) * 256) {
+        const i64 e = perm ? perm[i] : i;
+        if (e >= nlead) x[c * ldx + i] = 0.0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_lead_scatter(const i32 *__restrict__ perm,
+                                                      const double *__restrict__ src, i64 lds,
+                                                      double *__restrict__ dst, i64 ldd, i64 n,
+                                                      i64 nlead)
+{
+    const i64 c = blockIdx.y;
+    for (i64 i = blockIdx.x * 256 + threadIdx.x; i < n; i += i64(gridDim.x) * 256) {
+        const i64 e = perm ? perm[i] : i;
+        if (e < nlead) dst[c * ldd + e] = src[c * lds + i];
+    }
+}
+
 int upload_permutation(sprgpu_handle *h, const i64 *msifa)
 {
     const i64 neq = h->plan.neq;
@@ -52,6 +91,8 @@ int upload_permutation(sprgpu_handle *h, const i64 *msifa)
         seen[e - 1] = 1;
         perm[i] = i32(e - 1);
     }
+    h->d.peri2e_pmax.assign(neq + 1, -1);
+    for (i64 i = 0; i < neq; ++i) h->d.peri2e_pmax[i + 1] = std::max(h->d.peri2e_pmax[i], perm[i]);
     SPRGPU_HIP(hipMalloc(&h->d.peri2e, sizeof(i32) * neq));
     SPRGPU_HIP(hipMemcpy(h->d.peri2e, perm.data(), sizeof(i32) * neq, hipMemcpyHostToDevice));
     h->device_bytes += sizeof(i32) * neq;
@@ -92,6 +133,45 @@ int perm_gather(sprgpu_handle *h, const double *src, i64 lds, double *dst, i64 l
         const i64 nc = std::min<i64>(65535, nrhs - c0);
         k_perm_gather<<<perm_grid(n, nc), 256, 0, h->stream>>>(h->d.peri2e, src + c0 * lds, lds,
                                                                dst + c0 * ldd, ldd, n);
+    }
+    SPRGPU_HIP(hipGetLastError());
+    return 0;
+}
+
+int lead_gather(sprgpu_handle *h, const double *src, i64 lds, double *dst, i64 ldd, i64 nrhs,
+                i64 nlead)
+{
+    const i64 n = h->plan.neq;
+    for (i64 c0 = 0; c0 < nrhs; c0 += 65535) {
+        const i64 nc = std::min<i64>(65535, nrhs - c0);
+        k_lead_gather<<<perm_grid(n, nc), 256, 0, h->stream>>>(h->d.peri2e, src + c0 * lds, lds,
+                                                               dst + c0 * ldd, ldd, n, nlead);
+    }
+    SPRGPU_HIP(hipGetLastError());
+    return 0;
+}
+
+int lead_mask(sprgpu_handle *h, double *x, i64 ldx, i64 nrhs, i64 nlead)
+{
+    const i64 n = h->plan.neq;
+    for (i64 c0 = 0; c0 < nrhs; c0 += 65535) {
+        const i64 nc = std::min<i64>(65535, nrhs - c0);
+        k_lead_mask<<<perm_grid(n, nc), 256, 0, h->stream>>>(h->d.peri2e, x + c0 * ldx, ldx, n,
+                                                             nlead);
+    }
+    SPRGPU_HIP(hipGetLastError());
+    return 0;
+}
+
+int lead_scatter(sprgpu_handle *h, const double *src, i64 lds, double *dst, i64 ldd, i64 nrhs,
+                 i64 nlead)
+{
+    const i64 n = h->plan.neq;
+    for (i64 c0 = 0; c0 < nrhs; c0 += 65535) {
+        const i64 nc = std::min<i64>(65535, nrhs - c0);
+        k_lead_scatter<<<perm_grid(n, nc), 256, 0, h->stream>>>(h->d.peri2e, src + c0 * lds,
+                                                                lds, dst + c0 * ldd, ldd, n,
+                                                                nlead);
     }
     SPRGPU_HIP(hipGetLastError());
     return 0;

@@ -64,7 +64,6 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
     p.mfront = M(14);
     p.nofsub = M(15);
     p.nzerol = M(16);
-    if (M(57) > 0) return "partial factorization (MSPAR(57)>0, retained supernodes) is not supported";
     if (p.neq <= 0 || p.nsuper <= 0) return "empty system";
     if (p.neq >= (i64(1) << 31)) return "NEQ too large for 32-bit row indices";
     const i64 ns = p.nsuper;
@@ -76,7 +75,8 @@ std::string build_plan(const i64 *mspar, const i64 *mtrees, const i64 *msifa, in
     const i64 *lindx = msifa + (M(47) - 1);
     const i64 *xlnz = msifa + (M(48) - 1);
     for (i64 j = 0; j < ns; ++j)
-        if (supsup[j] > 0) return "retained supernodes (SUPSUP>0) are not supported";
+        if (supsup[j] > 0)
+            return "retained supernodes (SUPSUP>0, a superelement separator) are not supported";
 
     p.lindx.resize(p.nofsub);
     for (i64 k = 0; k < p.nofsub; ++k) p.lindx[k] = i32(lindx[k] - 1);

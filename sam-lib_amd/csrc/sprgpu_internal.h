@@ -343,6 +343,10 @@ struct DeviceBuffers {
     i32 *peri2e = nullptr;
     double *permwork = nullptr;
     i64 permwork_cap = 0;
+    // host: peri2e_pmax[k] = max(PERI2E(1..k)) - 1 (0-based), k <= NEQ; the
+    // leading k external equations are the leading k internal ones iff
+    // peri2e_pmax[k] == k - 1
+    std::vector<i32> peri2e_pmax;
     // persistent forest schedule
     ForestNode *forest_nodes = nullptr;
     ForestChild *forest_child = nullptr;
@@ -457,6 +461,15 @@ int perm_work(sprgpu_handle *h, i64 nrhs, double **w0, double **w1);
 // dst(i, c) = src(PERI2E(i), c) and back, i < NEQ, c < nrhs
 int perm_gather(sprgpu_handle *h, const double *src, i64 lds, double *dst, i64 ldd, i64 nrhs);
 int perm_scatter(sprgpu_handle *h, const double *src, i64 lds, double *dst, i64 ldd, i64 nrhs);
+// the leading nlead equations (external order when PERI2E is set):
+// dst(i, c) = src(e(i), c) if e(i) < nlead else 0, e = PERI2E or identity
+int lead_gather(sprgpu_handle *h, const double *src, i64 lds, double *dst, i64 ldd, i64 nrhs,
+                i64 nlead);
+// x(i, c) = 0 where e(i) >= nlead
+int lead_mask(sprgpu_handle *h, double *x, i64 ldx, i64 nrhs, i64 nlead);
+// dst(e(i), c) = src(i, c) where e(i) < nlead
+int lead_scatter(sprgpu_handle *h, const double *src, i64 lds, double *dst, i64 ldd, i64 nrhs,
+                 i64 nlead);
 // assemble.hip
 // constraint equations of the assembly (SAM's MPMCEQ, MMCEQ, MEQN; 1-based)
 struct CeqInput {

@@ -55,6 +55,7 @@ SIGNATURES = {
     "sprgpu_factor_info_get": (C.c_int, [P, C.POINTER(FactorInfo)]),
     "sprgpu_solve": (C.c_int, [P, P, I64, I64, C.c_int]),
     "sprgpu_solve_noalloc": (C.c_int, [P, P, I64, I64, C.c_int]),
+    "sprgpu_solve_partial": (C.c_int, [P, P, I64, I64, C.c_int]),
     "sprgpu_spmv": (C.c_int, [P, P, P, I64, I64]),
     "sprgpu_sprprm": (C.c_int, [P, P, P, P, P, I64, I64, I64, I64]),
     "sprgpu_sprtra": (C.c_int, [P, P, P, P, P, I64, I64, I64]),

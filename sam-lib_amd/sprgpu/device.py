@@ -156,6 +156,15 @@ class SprFactor:
         check(lib().sprgpu_solve(self._h, _ptr(b2), self.neq, b2.shape[0], which), "sprgpu_solve")
         return b_dev
 
+    def solve_partial(self, b_dev, which: int = 3):
+        """The partial solve of MSPAR(57) = 1: b_dev (nrhs, ldb) holds the
+        leading ldb < NEQ equations only (the others are zero in the sweeps
+        and dropped); which 3 gives A11^-1 b1."""
+        b2 = b_dev if b_dev.dim() == 2 else b_dev.view(1, -1)
+        check(lib().sprgpu_solve_partial(self._h, _ptr(b2), b2.shape[1], b2.shape[0], which),
+              "sprgpu_solve_partial")
+        return b_dev
+
     def spmv(self, x_dev, y_dev):
         x2 = x_dev if x_dev.dim() == 2 else x_dev.view(1, -1)
         y2 = y_dev if y_dev.dim() == 2 else y_dev.view(1, -1)

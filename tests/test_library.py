@@ -103,15 +103,20 @@ def test_front_pool_holds_only_live_fronts(kind, n):
 
 
 def test_plan_rejects_retained_supernodes():
+    """MSPAR(57) = 1 alone is accepted (without retained supernodes it changes
+    only the solves' LDB rule); a supernode with SUPSUP > 0 is rejected."""
     from sprgpu import _lib
 
     model, sysm, _ = make_system("hex8", 3)
     ms = sysm.mspar.copy()
-    ms[56] = 1  # MSPAR(57): partial factorization
+    ms[56] = 1  # MSPAR(57)
     out = np.zeros(10)
     p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
-    assert _lib.lib().sprgpu_plan_summary(p(ms), p(sysm.mtrees), p(sysm.msifa), p(out)) == -2
-    assert "MSPAR(57)" in _lib.last_error()
+    assert _lib.lib().sprgpu_plan_summary(p(ms), p(sysm.mtrees), p(sysm.msifa), p(out)) == 0
+    mt = sysm.mtrees.copy()
+    mt[int(ms[45]) - 1 + sysm.nsuper + 1 + sysm.nsuper - 1] = sysm.nsuper  # SUPSUP of the root
+    assert _lib.lib().sprgpu_plan_summary(p(ms), p(mt), p(sysm.msifa), p(out)) == -2
+    assert "retained" in _lib.last_error()
 
 
 def test_dropin_stage_protocol_without_gpu():

@@ -434,3 +434,43 @@ def _assembled_diagonal(sysm, sm):
             a[xsuper[s] + k] = sm[neq + pos]
             pos += f - k
     return a
+
+
+def _partial_cases(sysm):
+    """LDB values for the partial solve: inside a supernode, at a supernode
+    boundary, and one equation short."""
+    xs = sysm.xsuper() - 1
+    mid = int(xs[-3]) + 1 if xs[-2] - xs[-3] > 1 else int(xs[-3])
+    return sorted({sysm.neq - 1, int(xs[-2]), mid, sysm.neq // 2})
+
+
+@pytest.mark.parametrize("kind,n", [("hex8", 5), ("quad4", 14)])
+def test_partial_solve_oracle_equals_reference(kind, n):
+    """MSPAR(57) = 1 with LDB < NEQ (sprfs1.f:200-201, sprbs1.f:189): the
+    restated SPRFS1/SPRBS1 equal the reference's bit for bit for IOP 4 (both
+    sweeps), 5 (forward) and 6 (backward); IOP 4 solves the leading block,
+    x1 = A11^-1 b1 (mathematics, dense); with MSPAR(57) = 0 LDB < NEQ is
+    IERR = -2 in both."""
+    import refspr
+
+    model, sysm, sm = make_system(kind, n)
+    A = dense_equation_matrix(model, sysm)
+    fac = sm.copy()
+    assert oracle.sprsol(1, sysm.mspar.copy(), sysm.mtrees, sysm.msifa, fac)[0] == 0
+    rng = np.random.default_rng(11)
+    for ldb in _partial_cases(sysm):
+        mspar = sysm.mspar.copy()
+        mspar[56] = 1
+        b = np.asfortranarray(rng.uniform(-1, 1, (ldb, 2)))
+        for iop in (4, 5, 6):
+            xo, xr = b.copy(order="F"), b.copy(order="F")
+            ierr_o, _ = oracle.sprsol(iop, mspar.copy(), sysm.mtrees, sysm.msifa, fac.copy(), xo)
+            ierr_r, _, _ = refspr.sol(iop, sysm, fac.copy(), xr, mspar=mspar.copy())
+            assert ierr_o == 0 and ierr_r == 0
+            assert np.array_equal(xo.view(np.int64), xr.view(np.int64)), (ldb, iop)
+            if iop == 4:
+                x1 = np.linalg.solve(A[:ldb, :ldb], b)
+                assert np.max(np.abs(xo - x1)) <= 1e-10 * np.max(np.abs(x1)), ldb
+        xo, xr = b.copy(order="F"), b.copy(order="F")
+        assert oracle.sprsol(4, sysm.mspar.copy(), sysm.mtrees, sysm.msifa, fac.copy(), xo)[0] == -2
+        assert refspr.sol(4, sysm, fac.copy(), xr, mspar=sysm.mspar.copy())[0] == -2
