@@ -182,7 +182,9 @@ def bench_assembly(model, sysm, sm, h, reps, t_host):
     time per assembly from HIP events on the handle's stream (the clearing of
     SM included).  Algorithmic bytes: the cleared SM, per LNZ slot with element
     entries its position, range and result (24 B), per element entry its map
-    word, element offset, scale and matrix value (32 B)."""
+    word, element offset, scale and matrix value (32 B).  Also the element
+    matrices themselves made on the device (sprgpu_hex8_stiffness) and
+    assembled: their time, and the SM's difference from the host's."""
     import torch
     from sprgpu import host
 
@@ -204,7 +206,27 @@ def bench_assembly(model, sysm, sm, h, reps, t_host):
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
-    del out
+    # the element matrices themselves on the device (sprgpu_hex8_stiffness):
+    # unit cubes with the moduli as E, assembled again; the SM equals the one
+    # from base * scale to rounding (the element formula's own order)
+    from sprgpu import fem
+    xyz = torch.from_numpy(fem.hex8_coordinates(int(model.meta["n"]))).cuda()
+    conn = torch.from_numpy(model.mmnpc).cuda()
+    em_d = h.hex8_stiffness(xyz, conn, sc, float(model.meta["nu"]))
+    h.assemble(em_d, out)
+    torch.cuda.synchronize()
+    el_rel = float(torch.max(torch.abs(out - torch.from_numpy(sm).cuda())) /
+                   float(np.max(np.abs(sm))))
+    e0.record()
+    for _ in range(reps):
+        h.hex8_stiffness(xyz, conn, sc, float(model.meta["nu"]), em_d)
+    e1.record()
+    torch.cuda.synchronize()
+    el_ms = e0.elapsed_time(e1) / reps
+    el_bytes = model.nel * (576 * 8 + 8 * 8 + 8 + 8 * 24)  # EM written; nodes, E, coordinates read
+    del out, em_d
+    log(f"device element matrices {el_ms:.3f} ms ({el_bytes / (el_ms * 1e-3) / 1e9:.0f} GB/s), "
+        f"SM from them vs host: {el_rel:.1e}")
     nf = np.add.reduceat(me > 0, mp[:-1]).astype(np.int64)
     ncontrib = int(np.sum(nf * (nf + 1) // 2))
     nslots = int(np.count_nonzero(sm[sysm.neq:]))
@@ -214,7 +236,11 @@ def bench_assembly(model, sysm, sm, h, reps, t_host):
     return {"device_ms": ms, "bit_equal_to_host_spradm": same, "map_setup_s": t_map,
             "host_spradm_loop_s": t_host, "element_entries": ncontrib, "lnz_slots": nslots,
             "algorithmic_bytes": nbytes, "achieved_GBps": nbytes / (ms * 1e-3) / 1e9,
-            "frac_hbm": nbytes / (ms * 1e-3) / 8e12}
+            "frac_hbm": nbytes / (ms * 1e-3) / 8e12,
+            "element_matrices": {"device_ms": el_ms, "algorithmic_bytes": el_bytes,
+                                 "achieved_GBps": el_bytes / (el_ms * 1e-3) / 1e9,
+                                 "frac_hbm": el_bytes / (el_ms * 1e-3) / 8e12,
+                                 "sm_rel_diff_vs_host": el_rel}}
 
 
 def cpu_baseline():

@@ -177,6 +177,24 @@ int sprgpu_set_assembly(sprgpu_handle *handle, sprgpu_int nel, const sprgpu_int 
 int sprgpu_assemble(sprgpu_handle *handle, const double *em_dev, const sprgpu_int *em_off_dev,
                     const double *escale_dev, double *sm_dev, int accumulate);
 
+/* Element stiffness matrices of trilinear hexahedra on the device, the
+ * matrices a host loop computes before each SPRADM call (spradm.f:5-13 takes
+ * EM from its caller): isotropic linear elasticity, 2x2x2 Gauss quadrature,
+ * K_e = sum_g det(J_g) B_g' D B_g with engineering shear strains.  Element e
+ * has the 1-based nodes mmnpc_dev[8e .. 8e+7] (local order (-1,-1,-1)
+ * (1,-1,-1) (1,1,-1) (-1,1,-1) (-1,-1,1) (1,-1,1) (1,1,1) (-1,1,1)), node k
+ * the coordinates xyz_dev[3(k-1) .. 3(k-1)+2], modulus emod_dev[e] (NULL:
+ * 1) and Poisson's ratio nu; its dofs are its nodes in that order, each
+ * (ux, uy, uz).  em_dev receives nel column-major 24x24 matrices one after
+ * the other (sprgpu_assemble with em_off_dev NULL takes them so).  A node
+ * number outside 1..nanod gives a NaN matrix.  Asynchronous on the handle's
+ * stream; -2 for bad arguments or nu outside (-1, 0.5).  No counterpart in
+ * SPR itself (element routines are the caller's); it completes the device
+ * assembly path. */
+int sprgpu_hex8_stiffness(sprgpu_handle *handle, sprgpu_int nel, sprgpu_int nanod,
+                          const double *xyz_dev, const sprgpu_int *mmnpc_dev,
+                          const double *emod_dev, double nu, double *em_dev);
+
 /* Element assembly with constraint equations: the whole of SPRAD2
  * (src/SPR/spradm.f:743-1002) for a host loop that calls SPRADM with
  * NSV = nsv, TTCC, MPMCEQ, MMCEQ and MEQN (spradm.f:5-13).

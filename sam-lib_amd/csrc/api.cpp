@@ -491,6 +491,14 @@ int sprgpu_assemble(sprgpu_handle *h, const double *em_dev, const sprgpu_int *em
     return run_assemble(h, em_dev, em_off_dev, escale_dev, nullptr, sm_dev, nullptr, accumulate);
 }
 
+int sprgpu_hex8_stiffness(sprgpu_handle *h, sprgpu_int nel, sprgpu_int nanod,
+                          const double *xyz_dev, const sprgpu_int *mmnpc_dev,
+                          const double *emod_dev, double nu, double *em_dev)
+{
+    if (!h) return -2;
+    return hex8_stiffness(h, nel, nanod, xyz_dev, mmnpc_dev, emod_dev, nu, em_dev);
+}
+
 int sprgpu_assemble_ceq(sprgpu_handle *h, const double *em_dev, const sprgpu_int *em_off_dev,
                         const double *escale_dev, const double *ttcc_dev, double *sm_dev,
                         double *sv_dev, int accumulate)

@@ -484,4 +484,7 @@ int set_assembly(sprgpu_handle *h, i64 nel, const i64 *meen_ptr, const i64 *meen
 int run_assemble(sprgpu_handle *h, const double *em, const i64 *em_off, const double *escale,
                  const double *ttcc, double *sm, double *sv, int accumulate);
 void free_assembly(sprgpu_handle *h);
+// elements.hip: hex8 element stiffness matrices (576 doubles each, consecutive)
+int hex8_stiffness(sprgpu_handle *h, i64 nel, i64 nanod, const double *xyz, const i64 *mmnpc,
+                   const double *emod, double nu, double *em);
 }  // namespace sprgpu

@@ -51,6 +51,7 @@ SIGNATURES = {
     "sprgpu_assemble": (C.c_int, [P, P, P, P, P, C.c_int]),
     "sprgpu_set_assembly_ceq": (C.c_int, [P, I64, P, P, I64, P, P, I64, P, I64]),
     "sprgpu_assemble_ceq": (C.c_int, [P, P, P, P, P, P, P, C.c_int]),
+    "sprgpu_hex8_stiffness": (C.c_int, [P, I64, I64, P, P, P, C.c_double, P]),
     "sprgpu_factor": (C.c_int, [P, P, P, C.c_int, C.POINTER(FactorInfo)]),
     "sprgpu_factor_info_get": (C.c_int, [P, C.POINTER(FactorInfo)]),
     "sprgpu_solve": (C.c_int, [P, P, I64, I64, C.c_int]),

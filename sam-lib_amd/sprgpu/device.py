@@ -107,6 +107,21 @@ class SprFactor:
                                     _ptr(sm_dev), int(bool(accumulate))), "sprgpu_assemble")
         return sm_dev
 
+    def hex8_stiffness(self, xyz_dev, mmnpc_dev, emod_dev=None, nu: float = 0.3, em_dev=None):
+        """Element matrices of hex8 elements on the device: xyz_dev (nanod, 3)
+        float64, mmnpc_dev (nel*8,) int64 1-based nodes, emod_dev (nel,) or
+        None.  Returns em_dev (nel*576,) float64, the column-major 24x24
+        matrices one after the other."""
+        torch = self._torch
+        nel = mmnpc_dev.numel() // 8
+        if em_dev is None:
+            em_dev = torch.empty(nel * 576, dtype=torch.float64, device=xyz_dev.device)
+        opt = (lambda t: None if t is None else _ptr(t))
+        check(lib().sprgpu_hex8_stiffness(self._h, nel, xyz_dev.shape[0], _ptr(xyz_dev),
+                                          _ptr(mmnpc_dev), opt(emod_dev), float(nu),
+                                          _ptr(em_dev)), "sprgpu_hex8_stiffness")
+        return em_dev
+
     def set_assembly_ceq(self, meen_ptr: np.ndarray, meen: np.ndarray, mpmceq: np.ndarray,
                          mmceq: np.ndarray, meqn: np.ndarray, nsv: int = 0):
         """``set_assembly`` with constraint equations: meen may hold -ICEQ

@@ -133,6 +133,59 @@ def hex8_stiffness(E: float = 1.0, nu: float = 0.3, h: float = 1.0) -> np.ndarra
     return 0.5 * (K + K.T)
 
 
+_HEX8_NODES = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                        [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], dtype=float)
+
+
+def hex8_stiffness_at(X: np.ndarray, E: float = 1.0, nu: float = 0.3) -> np.ndarray:
+    """24x24 stiffness of a trilinear hexahedron with node coordinates X (8, 3)
+    in the local node order of hex8_stiffness: sum over the 2x2x2 Gauss points
+    of det(J) B' D B, J = dN/dxi X, dN/dx = J^-1 dN/dxi (isoparametric map)."""
+    X = np.asarray(X, dtype=float)
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    D = np.zeros((6, 6))
+    D[:3, :3] = lam
+    D[np.arange(3), np.arange(3)] += 2 * mu
+    D[3, 3] = D[4, 4] = D[5, 5] = mu
+    s = _HEX8_NODES
+    K = np.zeros((24, 24))
+    for xi in _GAUSS2:
+        for et in _GAUSS2:
+            for ze in _GAUSS2:
+                dN = np.empty((3, 8))
+                dN[0] = s[:, 0] * (1 + et * s[:, 1]) * (1 + ze * s[:, 2]) / 8.0
+                dN[1] = s[:, 1] * (1 + xi * s[:, 0]) * (1 + ze * s[:, 2]) / 8.0
+                dN[2] = s[:, 2] * (1 + xi * s[:, 0]) * (1 + et * s[:, 1]) / 8.0
+                J = dN @ X
+                dx = np.linalg.solve(J, dN)
+                B = np.zeros((6, 24))
+                B[0, 0::3] = dx[0]
+                B[1, 1::3] = dx[1]
+                B[2, 2::3] = dx[2]
+                B[3, 0::3] = dx[1]
+                B[3, 1::3] = dx[0]
+                B[4, 1::3] = dx[2]
+                B[4, 2::3] = dx[1]
+                B[5, 0::3] = dx[2]
+                B[5, 2::3] = dx[0]
+                K += B.T @ D @ B * np.linalg.det(J)
+    return K
+
+
+def hex8_coordinates(n: int, distort: float = 0.0, seed: int = 0) -> np.ndarray:
+    """(nanod, 3) node coordinates of hex8_grid(n): node i + (n+1)(j + (n+1)k)
+    at (i, j, k), unit elements; with ``distort`` every node is moved by
+    U[-distort, distort] per coordinate (default_rng(seed)), nodes on the
+    clamped face z = 0 included."""
+    m = n + 1
+    k, j, i = np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij")
+    xyz = np.stack([i.ravel(), j.ravel(), k.ravel()], axis=1).astype(float)
+    if distort:
+        xyz += np.random.default_rng(seed).uniform(-distort, distort, xyz.shape)
+    return xyz
+
+
 def quad4_stiffness(E: float = 1.0, nu: float = 0.25, t: float = 1.0, h: float = 1.0) -> np.ndarray:
     """8x8 plane-stress stiffness of a bilinear square (edge h, thickness t), 2x2 Gauss.
 

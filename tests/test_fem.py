@@ -95,3 +95,44 @@ def test_hex8_grid_counts():
 def test_quad4_grid_counts():
     m = fem.quad4_grid(1000)
     assert int(np.sum(m.msc)) == 2002000
+
+
+def test_hex8_stiffness_at_pins_the_element_formula():
+    """The general (isoparametric) hex8 element against what the mathematics
+    fixes: on the unit cube it is hex8_stiffness; on a distorted element it
+    is symmetric positive semi-definite with exactly the six rigid-body modes
+    as its null space; on a parallelepiped (an affine image of the cube) a
+    linear displacement field u = G x gives the exact strain energy
+    vol * eps:D:eps / 2 (patch test); scaling all coordinates by h scales K
+    by h."""
+    from sprgpu import fem
+
+    cube = fem.hex8_coordinates(1)
+    X = cube[[0, 1, 3, 2, 4, 5, 7, 6]]  # the grid's nodes in local node order
+    K0 = fem.hex8_stiffness(2.5, 0.3)
+    K1 = fem.hex8_stiffness_at(X, 2.5, 0.3)
+    assert np.max(np.abs(K1 - K0)) <= 1e-14 * np.max(np.abs(K0))
+    rng = np.random.default_rng(4)
+    Xd = X + rng.uniform(-0.2, 0.2, X.shape)
+    K = fem.hex8_stiffness_at(Xd, 1.7, 0.25)
+    assert np.max(np.abs(K - K.T)) <= 1e-14 * np.max(np.abs(K))
+    w = np.linalg.eigvalsh(0.5 * (K + K.T))
+    assert np.all(w[:6] < 1e-12 * w[-1]) and w[6] > 1e-6 * w[-1]
+    rigid = [np.tile(e, 8) for e in np.eye(3)]
+    for axis in np.eye(3):
+        rigid.append(np.cross(axis, Xd).ravel())
+    for r in rigid:
+        assert np.max(np.abs(K @ r)) <= 1e-13 * np.max(np.abs(K)) * np.max(np.abs(r))
+    # patch test on a parallelepiped (affine map of the cube): exact energy
+    A = np.array([[1.2, 0.1, 0.0], [0.2, 0.9, 0.1], [0.0, 0.3, 1.1]])
+    Xp = X @ A.T
+    E, nu = 3.0, 0.3
+    Kp = fem.hex8_stiffness_at(Xp, E, nu)
+    G = rng.normal(size=(3, 3))
+    u = (Xp @ G.T).ravel()
+    eps = 0.5 * (G + G.T)
+    lam, mu = E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+    vol = abs(np.linalg.det(A))
+    energy = 0.5 * vol * (lam * np.trace(eps) ** 2 + 2 * mu * np.sum(eps * eps))
+    assert abs(0.5 * u @ Kp @ u - energy) <= 1e-12 * energy
+    assert np.max(np.abs(fem.hex8_stiffness_at(2 * Xd, 1.7, 0.25) - 2 * K)) <= 1e-13 * np.max(np.abs(K))

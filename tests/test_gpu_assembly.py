@@ -292,3 +292,49 @@ def test_constrained_assembly_error_paths():
         h.assemble_ceq(_dev(model.em), _dev(ttcc), out, None, _dev(model.em_off),
                        _dev(model.escale))
     h.close()
+
+
+# ---------------------------------------------------------------------------
+# element matrices on the device (sprgpu_hex8_stiffness)
+# ---------------------------------------------------------------------------
+def test_device_hex8_stiffness_and_assembly():
+    """Distorted hex8 6^3 with log-normal moduli: the device element matrices
+    against the isoparametric formula (fem.hex8_stiffness_at, itself pinned
+    by tests/test_fem.py) to 1e-13 of each matrix's largest entry; assembled
+    on the device they give the host SPRADM loop's SM of those matrices to
+    1e-13; a node number out of range gives that element a NaN matrix and
+    leaves the others alone."""
+    import dataclasses
+
+    from sprgpu import fem, host
+
+    torch = _torch()
+    n, nu = 6, 0.28
+    model, sysm, _, h, mp, me = _setup("hex8", n, dict(e_sigma=0.5, e_seed=1))
+    xyz = fem.hex8_coordinates(n, distort=0.2, seed=5)
+    E = model.escale
+    conn = model.mmnpc.reshape(model.nel, 8) - 1
+    em_ref = np.concatenate([fem.hex8_stiffness_at(xyz[conn[e]], E[e], nu).ravel(order="F")
+                             for e in range(model.nel)])
+    em = h.hex8_stiffness(_dev(xyz), _dev(model.mmnpc), _dev(E), nu)
+    torch.cuda.synchronize()
+    got = em.cpu().numpy().reshape(model.nel, 576)
+    ref = em_ref.reshape(model.nel, 576)
+    err = np.max(np.abs(got - ref), axis=1) / np.max(np.abs(ref), axis=1)
+    print(f"hex8 {n}^3 distorted: max element error {err.max():.2e}", flush=True)
+    assert err.max() <= 1e-13
+    out = torch.empty(sysm.neq + sysm.nzerol, dtype=torch.float64, device="cuda")
+    h.assemble(em, out)
+    m2 = dataclasses.replace(model, em=em_ref, em_off=(576 * np.arange(model.nel)).astype(np.int64),
+                             escale=None)
+    sm_ref = host.assemble(m2, sysm)
+    torch.cuda.synchronize()
+    sm = out.cpu().numpy()
+    assert np.max(np.abs(sm - sm_ref)) <= 1e-13 * np.max(np.abs(sm_ref))
+    bad = model.mmnpc.copy()
+    bad[8 * 5 + 3] = model.nanod + 1
+    em2 = h.hex8_stiffness(_dev(xyz), _dev(bad), _dev(E), nu)
+    torch.cuda.synchronize()
+    g2 = em2.cpu().numpy().reshape(model.nel, 576)
+    assert np.all(np.isnan(g2[5])) and np.array_equal(np.delete(g2, 5, 0), np.delete(got, 5, 0))
+    h.close()
