@@ -1330,10 +1330,13 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmDesc *__restrict__ descs
 // launches; accumulator element r of block (a, b) is row
 // 32a + 8(r/4) + 4(lane/32) + r%4 and column 32b + lane%32 of the quadrant,
 // so four consecutive elements are four consecutive rows of C.
+// Launch bounds of four workgroups per CU hold it to 128 registers (the
+// accumulators in VGPRs, no spills): four waves per SIMD instead of three,
+// which the LDS (40 KB per workgroup) allows; about 3 % faster per launch.
 typedef float fv16 __attribute__((ext_vector_type(16)));
 
 template <int KIND>
-__global__ __launch_bounds__(256) void k_gemm_x32(const GemmDesc *__restrict__ descs, int ndesc,
+__global__ __launch_bounds__(256, 4) void k_gemm_x32(const GemmDesc *__restrict__ descs, int ndesc,
                                                   const i64 *__restrict__ prefix,
                                                   float *__restrict__ pool,
                                                   const double *__restrict__ dvec)
