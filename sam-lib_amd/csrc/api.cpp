@@ -47,7 +47,7 @@ static void free_all(sprgpu_handle *h)
                     d.spin_error, d.col_node, d.prm_count,
                     d.tmm_rptr, d.tmm_segs, d.tmm_long, d.peri2e, d.permwork,
                     d.forest_nodes, d.forest_child, d.forest_chunks, d.forest_order, d.ublk,
-                    d.forest_sync};
+                    d.forest_sync, d.spmm_work};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     free_assembly(h);

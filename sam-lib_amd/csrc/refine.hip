@@ -41,6 +41,95 @@ __global__ __launch_bounds__(256) void k_spmv(const i64 *__restrict__ rowptr,
     }
 }
 
+// Many right-hand sides: y = b + beta A X as a product with X stored
+// equation-major (row i holds x_i of every right-hand side, contiguous), so
+// that every matrix entry is read once per row for all right-hand sides and
+// every x row read is one coalesced run.  The caller's column-major vectors
+// are transposed in and out through LDS tiles.
+constexpr int kTr = 32;  // transpose tile edge
+// dst(i, r) = src[r * lds + i] into dst[i * nrhs + r], i < n, r < nrhs
+__global__ __launch_bounds__(256) void k_to_eq_major(const double *__restrict__ src, i64 lds,
+                                                     double *__restrict__ dst, i64 n, int nrhs)
+{
+    __shared__ double t[kTr][kTr + 1];
+    const i64 i0 = i64(blockIdx.x) * kTr;
+    const int r0 = blockIdx.y * kTr;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+#pragma unroll
+    for (int k = 0; k < kTr; k += 8) {
+        const int r = r0 + ty + k;
+        const i64 i = i0 + tx;
+        t[ty + k][tx] = (r < nrhs && i < n) ? src[r * lds + i] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kTr; k += 8) {
+        const i64 i = i0 + ty + k;
+        const int r = r0 + tx;
+        if (i < n && r < nrhs) dst[i * nrhs + r] = t[tx][ty + k];
+    }
+}
+
+// y[r * ld + i] = b[r * ld + i] + beta * src[i * nrhs + r] (b may be null)
+__global__ __launch_bounds__(256) void k_from_eq_major(const double *__restrict__ src,
+                                                       double *__restrict__ y,
+                                                       const double *__restrict__ b, i64 ld,
+                                                       double beta, i64 n, int nrhs)
+{
+    __shared__ double t[kTr][kTr + 1];
+    const i64 i0 = i64(blockIdx.x) * kTr;
+    const int r0 = blockIdx.y * kTr;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+#pragma unroll
+    for (int k = 0; k < kTr; k += 8) {
+        const i64 i = i0 + ty + k;
+        const int r = r0 + tx;
+        t[ty + k][tx] = (i < n && r < nrhs) ? src[i * nrhs + r] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kTr; k += 8) {
+        const int r = r0 + ty + k;
+        const i64 i = i0 + tx;
+        if (r < nrhs && i < n) {
+            const double b0 = b ? b[r * ld + i] : 0.0;
+            y[r * ld + i] = b0 + beta * t[tx][ty + k];
+        }
+    }
+}
+
+// yt[i * nrhs + r] = sum_k val[k] xt[col[k] * nrhs + r] over row i: one
+// wavefront per row, the lanes over the right-hand sides (64 at a time), the
+// row's entries in order with eight loads in flight
+constexpr int kSpmmBatch = 8;
+__global__ __launch_bounds__(256) void k_spmm_rows(const i64 *__restrict__ rowptr,
+                                                   const i32 *__restrict__ col,
+                                                   const double *__restrict__ val,
+                                                   const double *__restrict__ xt,
+                                                   double *__restrict__ yt, i64 n, int nrhs)
+{
+    const i64 row = i64(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const i64 k0 = rowptr[row], k1 = rowptr[row + 1];
+    for (int r0 = 0; r0 < nrhs; r0 += 64) {
+        const int r = min(r0 + lane, nrhs - 1);
+        double s = 0.0;
+        for (i64 kb = k0; kb < k1; kb += kSpmmBatch) {
+            double a[kSpmmBatch], xv[kSpmmBatch];
+#pragma unroll
+            for (int q = 0; q < kSpmmBatch; ++q) {
+                const i64 k = min(kb + q, k1 - 1);
+                a[q] = (kb + q < k1) ? val[k] : 0.0;
+                xv[q] = xt[i64(col[k]) * nrhs + r];
+            }
+#pragma unroll
+            for (int q = 0; q < kSpmmBatch; ++q) s += a[q] * xv[q];
+        }
+        if (r0 + lane < nrhs) yt[row * nrhs + r0 + lane] = s;
+    }
+}
+
 // gathers the CSR values from the packed assembled matrix and counts the
 // non-zeros found on or below the diagonal (each packed entry once); sixteen
 // lanes per row, so that a row's index and value reads are coalesced and
@@ -179,12 +268,36 @@ static int refresh_values(sprgpu_handle *h)
     return 0;
 }
 
-// y = b + beta A x for nrhs columns (b may be null), in chunks of right-hand
-// sides that keep a launch within 2^23 workgroups
-static void launch_spmv(sprgpu_handle *h, const double *x, double *y, const double *b, i64 ld,
-                        i64 nrhs, double beta)
+constexpr i64 kSpmmMinRhs = 16;  // from here on the equation-major product
+
+// y = b + beta A x for nrhs columns (b may be null): up to kSpmmMinRhs - 1
+// columns one product per column (in chunks of right-hand sides that keep a
+// launch within 2^23 workgroups), from kSpmmMinRhs on the equation-major
+// product (a work space of 2 n nrhs doubles, grown on demand)
+static int launch_spmv(sprgpu_handle *h, const double *x, double *y, const double *b, i64 ld,
+                       i64 nrhs, double beta)
 {
     const i64 n = h->plan.neq;
+    DeviceBuffers &d = h->d;
+    if (nrhs >= kSpmmMinRhs && nrhs < 65535 * i64(kTr) && !h->noalloc) {
+        const i64 need = 2 * n * nrhs;
+        if (d.spmm_cap < need) {
+            ++h->buf_gen;
+            if (d.spmm_work) SPRGPU_HIP(hipFree(d.spmm_work));
+            d.spmm_work = nullptr;
+            SPRGPU_HIP(hipMalloc(&d.spmm_work, sizeof(double) * need));
+            h->device_bytes += sizeof(double) * (need - d.spmm_cap);
+            d.spmm_cap = need;
+        }
+        double *xt = d.spmm_work, *yt = d.spmm_work + n * nrhs;
+        const dim3 gt(unsigned((n + kTr - 1) / kTr), unsigned((nrhs + kTr - 1) / kTr));
+        k_to_eq_major<<<gt, 256, 0, h->stream>>>(x, ld, xt, n, int(nrhs));
+        k_spmm_rows<<<unsigned((n + 3) / 4), 256, 0, h->stream>>>(d.csr_rowptr, d.csr_col,
+                                                                 d.csr_val, xt, yt, n, int(nrhs));
+        k_from_eq_major<<<gt, 256, 0, h->stream>>>(yt, y, b, ld, beta, n, int(nrhs));
+        SPRGPU_HIP(hipGetLastError());
+        return 0;
+    }
     const i64 nb = (n * 16 + 255) / 256;
     const i64 chunk = std::max<i64>(1, std::min<i64>(nrhs, (i64(1) << 23) / nb));
     for (i64 r0 = 0; r0 < nrhs; r0 += chunk) {
@@ -193,6 +306,8 @@ static void launch_spmv(sprgpu_handle *h, const double *x, double *y, const doub
             h->d.csr_rowptr, h->d.csr_col, h->d.csr_val, x + r0 * ld, y + r0 * ld,
             b ? b + r0 * ld : nullptr, n, ld, beta, int(nr));
     }
+    SPRGPU_HIP(hipGetLastError());
+    return 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -369,7 +484,7 @@ int run_sprprm(sprgpu_handle *h, const double *a, const double *b, double *c, do
 int run_spmv(sprgpu_handle *h, const double *x, double *y, i64 ld, i64 nrhs)
 {
     if (int rc = refresh_values(h)) return rc;
-    launch_spmv(h, x, y, nullptr, ld, nrhs, 1.0);
+    if (int rc = launch_spmv(h, x, y, nullptr, ld, nrhs, 1.0)) return rc;
     SPRGPU_HIP(hipGetLastError());
     return 0;
 }
@@ -422,7 +537,7 @@ int run_solve_ir(sprgpu_handle *h, const double *b, double *x, i64 ld, i64 nrhs,
             set_error("solve_ir requires ld == neq");
             return -1;
         }
-        launch_spmv(h, x, r, b, n, nrhs, -1.0);
+        if (int rc = launch_spmv(h, x, r, b, n, nrhs, -1.0)) return rc;
         if (int rc = col_norms(h, r, n, n, nrhs, part, nrm, rnorm)) return rc;
         if (int rc = check_solve_error(h)) return rc;
         worst = 0.0;

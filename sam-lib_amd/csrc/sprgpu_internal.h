@@ -327,6 +327,8 @@ struct DeviceBuffers {
     i32 *tmm_long = nullptr;    // columns summed by a wavefront each (SPRTMM's L^T B)
     i64 tmm_nlong = 0;
     i64 irwork_cap = 0;
+    double *spmm_work = nullptr;  // many-RHS product: X and Y equation-major
+    i64 spmm_cap = 0;
     // element assembly map: the positions that receive element entries
     // (ascending; LNZ positions, then NZEROL + the SV entries), the range of
     // each one's contributions, and the contributions in SPRADM order, each

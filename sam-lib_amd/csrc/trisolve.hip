@@ -1482,7 +1482,8 @@ static const T *block_inverses(sprgpu_handle *h)
 }
 
 template <typename T, int R>
-static void fwd_narrow_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs)
+static void fwd_narrow_level(sprgpu_handle *h, hipStream_t st, int l, double *b, i64 ldb,
+                             i64 nrhs)
 {
     const Plan &p = h->plan;
     DeviceBuffers &d = h->d;
@@ -1491,7 +1492,7 @@ static void fwd_narrow_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nr
     const unsigned ns = unsigned(p.solve_small_ptr[l + 1] - p.solve_small_ptr[l]);
     const int mf = p.small_maxF[l], ms = p.small_maxS[l];
     const unsigned nt = narrow_threads(p, l);
-    k_fwd_narrow<T, R><<<ns * groups, nt, fwd_narrow_lds<T>(R, mf, ms, nt), h->stream>>>(
+    k_fwd_narrow<T, R><<<ns * groups, nt, fwd_narrow_lds<T>(R, mf, ms, nt), st>>>(
         d.solve_small + p.solve_small_ptr[l], mf, d.nodes, d.lindx, d.upd_ptr, d.upd_src, store,
         d.dvec, b, ldb, d.vwork, p.vwork_elems, int(nrhs));
 }
@@ -1527,7 +1528,8 @@ static void fwd_rect_level(sprgpu_handle *h, int l, i64 nrhs)
 }
 
 template <typename T, int R>
-static void bwd_narrow_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nrhs)
+static void bwd_narrow_level(sprgpu_handle *h, hipStream_t st, int l, double *b, i64 ldb,
+                             i64 nrhs)
 {
     const Plan &p = h->plan;
     DeviceBuffers &d = h->d;
@@ -1536,7 +1538,7 @@ static void bwd_narrow_level(sprgpu_handle *h, int l, double *b, i64 ldb, i64 nr
     const unsigned ns = unsigned(p.solve_small_ptr[l + 1] - p.solve_small_ptr[l]);
     const int mf = p.small_maxF[l], ms = p.small_maxS[l];
     const unsigned nt = narrow_threads(p, l);
-    k_bwd_narrow<T, R><<<ns * groups, nt, bwd_narrow_lds<T>(R, mf, ms, nt), h->stream>>>(
+    k_bwd_narrow<T, R><<<ns * groups, nt, bwd_narrow_lds<T>(R, mf, ms, nt), st>>>(
         d.solve_small + p.solve_small_ptr[l], mf, d.nodes, d.lindx, store, b, ldb, int(nrhs));
 }
 
@@ -1609,21 +1611,50 @@ static void mark_pending(sprgpu_handle *h, i64 nrhs)
         h->d.wide, int(p.wide.size()), h->d.nodes, h->d.vwork, p.vwork_elems);
 }
 
+// A level with both narrow and wide supernodes runs the two kinds side by
+// side: the narrow launch on the handle's second stream, forked from and
+// joined to the handle's stream by events (they touch different supernodes,
+// and the next level waits for both).
+struct LevelFork {
+    sprgpu_handle *h;
+    bool on = false;
+    // the stream for the narrow launch of level l (forks when both kinds are there)
+    hipStream_t begin(bool both)
+    {
+        on = both && h->stream2;
+        if (!on) return h->stream;
+        (void)hipEventRecord(h->ev[0], h->stream);
+        (void)hipStreamWaitEvent(h->stream2, h->ev[0], 0);
+        return h->stream2;
+    }
+    void end()
+    {
+        if (!on) return;
+        (void)hipEventRecord(h->ev[1], h->stream2);
+        (void)hipStreamWaitEvent(h->stream, h->ev[1], 0);
+        on = false;
+    }
+};
+
 template <typename T>
 static int solve_chunk(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which)
 {
     const Plan &p = h->plan;
     const bool grouped = nrhs >= kSolveGroup;
+    LevelFork fk{h};
     if (which & 1) {
         mark_pending(h, nrhs);
         for (int l = 0; l < p.nlevels; ++l) {
-            if (p.solve_small_ptr[l + 1] > p.solve_small_ptr[l]) {
+            const bool narrow = p.solve_small_ptr[l + 1] > p.solve_small_ptr[l];
+            const bool wide = p.wide_ptr[l + 1] > p.wide_ptr[l];
+            if (narrow) {
+                const hipStream_t st = fk.begin(wide);
                 const int R = narrow_group<T>(p, l, nrhs);
-                if (R == kSolveGroup) fwd_narrow_level<T, kSolveGroup>(h, l, b, ldb, nrhs);
-                else if (R == 4) fwd_narrow_level<T, 4>(h, l, b, ldb, nrhs);
-                else fwd_narrow_level<T, 1>(h, l, b, ldb, nrhs);
+                if (R == kSolveGroup) fwd_narrow_level<T, kSolveGroup>(h, st, l, b, ldb, nrhs);
+                else if (R == 4) fwd_narrow_level<T, 4>(h, st, l, b, ldb, nrhs);
+                else fwd_narrow_level<T, 1>(h, st, l, b, ldb, nrhs);
             }
-            if (p.wide_ptr[l + 1] > p.wide_ptr[l]) {
+            if (wide) {
                 if (grouped) {  // the chains, then the rectangles in wider groups
                     fwd_wide_level<T, kSolveGroup>(h, l, b, ldb, nrhs, false);
                     if (nrhs >= kMmGroup) fwd_rect_mm_level<T>(h, l, nrhs);
@@ -1633,12 +1664,22 @@ static int solve_chunk(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which
                     fwd_wide_level<T, 1>(h, l, b, ldb, nrhs, true);
                 }
             }
+            fk.end();
         }
     }
     if (which & 2) {
         mark_pending(h, nrhs);
         for (int l = p.nlevels - 1; l >= 0; --l) {
-            if (p.wide_ptr[l + 1] > p.wide_ptr[l]) {
+            const bool narrow = p.solve_small_ptr[l + 1] > p.solve_small_ptr[l];
+            const bool wide = p.wide_ptr[l + 1] > p.wide_ptr[l];
+            if (narrow) {
+                const hipStream_t st = fk.begin(wide);
+                const int R = narrow_group<T>(p, l, nrhs);
+                if (R == kSolveGroup) bwd_narrow_level<T, kSolveGroup>(h, st, l, b, ldb, nrhs);
+                else if (R == 4) bwd_narrow_level<T, 4>(h, st, l, b, ldb, nrhs);
+                else bwd_narrow_level<T, 1>(h, st, l, b, ldb, nrhs);
+            }
+            if (wide) {
                 if (nrhs >= kMmGroup) bwd_rect_mm_level<T>(h, l, b, ldb, nrhs);
                 else if (nrhs >= kRectGroup) bwd_rect_level<T, kRectGroup>(h, l, b, ldb, nrhs);
                 else if (grouped) bwd_rect_level<T, kSolveGroup>(h, l, b, ldb, nrhs);
@@ -1646,12 +1687,7 @@ static int solve_chunk(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which
                 if (grouped) bwd_wide_level<T, kSolveGroup>(h, l, b, ldb, nrhs);
                 else bwd_wide_level<T, 1>(h, l, b, ldb, nrhs);
             }
-            if (p.solve_small_ptr[l + 1] > p.solve_small_ptr[l]) {
-                const int R = narrow_group<T>(p, l, nrhs);
-                if (R == kSolveGroup) bwd_narrow_level<T, kSolveGroup>(h, l, b, ldb, nrhs);
-                else if (R == 4) bwd_narrow_level<T, 4>(h, l, b, ldb, nrhs);
-                else bwd_narrow_level<T, 1>(h, l, b, ldb, nrhs);
-            }
+            fk.end();
         }
     }
     SPRGPU_HIP(hipGetLastError());
