@@ -1148,7 +1148,6 @@ __global__ __launch_bounds__(64 * kBdWaves) void k_block_fused(const PanelDesc *
 // blocks share an L2.
 // Tiles are anchored at r0 rounded down to kGemmAlign and front leading
 // dimensions are multiples of kFrontLdAlign, so all vectors are aligned.
-constexpr int kXcds = 8;
 constexpr int kSuperCols = 8;  // tile columns per super-column of the tile order
 
 // Tile (ti, tj) of the t-th tile of a lower trapezoid of nti x ntj tiles

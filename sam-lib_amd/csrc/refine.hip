@@ -99,8 +99,11 @@ __global__ __launch_bounds__(256) void k_from_eq_major(const double *__restrict_
 }
 
 // yt[i * nrhs + r] = sum_k val[k] xt[col[k] * nrhs + r] over row i: one
-// wavefront per row, the lanes over the right-hand sides (64 at a time), the
-// row's entries in order with eight loads in flight
+// wavefront per row, the lanes over the right-hand sides (two each, 128 at a
+// time, when nrhs is even; else one each, 64 at a time), the row's entries in
+// order with eight loads in flight.  Rows go to the XCDs in contiguous
+// ranges (block ids are dealt to the XCDs in turn), so that the rows that
+// share x rows (a mesh's neighbours) share an L2.
 constexpr int kSpmmBatch = 8;
 __global__ __launch_bounds__(256) void k_spmm_rows(const i64 *__restrict__ rowptr,
                                                    const i32 *__restrict__ col,
@@ -108,10 +111,35 @@ __global__ __launch_bounds__(256) void k_spmm_rows(const i64 *__restrict__ rowpt
                                                    const double *__restrict__ xt,
                                                    double *__restrict__ yt, i64 n, int nrhs)
 {
-    const i64 row = i64(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const i64 nb = gridDim.x, bid = blockIdx.x;
+    const i64 per = nb / kXcds, rem = nb % kXcds;
+    const i64 xcd = bid % kXcds, idx = bid / kXcds;
+    const i64 lin = (xcd < rem) ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+    const i64 row = lin * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (row >= n) return;
     const i64 k0 = rowptr[row], k1 = rowptr[row + 1];
+    if ((nrhs & 1) == 0) {
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        for (int r0 = 0; r0 < nrhs; r0 += 128) {
+            const int r = min(r0 + 2 * lane, nrhs - 2);
+            d2 s = d2{0.0, 0.0};
+            for (i64 kb = k0; kb < k1; kb += kSpmmBatch) {
+                double a[kSpmmBatch];
+                d2 xv[kSpmmBatch];
+#pragma unroll
+                for (int q = 0; q < kSpmmBatch; ++q) {
+                    const i64 k = min(kb + q, k1 - 1);
+                    a[q] = (kb + q < k1) ? val[k] : 0.0;
+                    xv[q] = *reinterpret_cast<const d2 *>(xt + i64(col[k]) * nrhs + r);
+                }
+#pragma unroll
+                for (int q = 0; q < kSpmmBatch; ++q) s += a[q] * xv[q];
+            }
+            if (r0 + 2 * lane < nrhs) *reinterpret_cast<d2 *>(yt + row * nrhs + r) = s;
+        }
+        return;
+    }
     for (int r0 = 0; r0 < nrhs; r0 += 64) {
         const int r = min(r0 + lane, nrhs - 1);
         double s = 0.0;

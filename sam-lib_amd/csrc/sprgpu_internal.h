@@ -36,6 +36,9 @@ constexpr int kPoolExactFronts = 16384;  // largest fronts placed in the pool by
 constexpr int kFusedMaxFronts = 64;  // outer-block launches of at most this many fronts are fused
 constexpr int kPubElems = 4 * kPanelNB * kPanelNB + (kPanelOuter - kPanelNB) * (kPanelOuter - kPanelNB);
 constexpr int kSmallF = 64;      // fronts with F <= kSmallF can be factored in LDS (one lane a row)
+// XCDs of the device: consecutive workgroup ids are dealt to them in turn,
+// each with its own L2
+constexpr int kXcds = 8;
 constexpr int kForestWaves = 4;  // wavefronts per workgroup of the forest schedule (one item each)
 constexpr int kForestWgsPerLevel = 1024;  // at most this many workgroups per tree height
 constexpr int kInitCols = 8;     // front columns per workgroup in the init kernel
