@@ -1461,6 +1461,16 @@ static unsigned narrow_threads(const Plan &p, int l)
 // right-hand sides.  (Larger groups were measured slower, 32 on quad4 1000^2
 // with 128 right-hand sides by 1.6x: a workgroup's gathers and triangular
 // chains for its right-hand sides run in sequence.)
+// Right-hand sides per workgroup of the wide supernodes' block chains when a
+// call has at least kSolveGroup of them: 4 with an FP32 factor, kSolveGroup
+// with FP64 (40^3 x 128: the FP32 + IR step 140.7 ms with 8, 136.7 ms with
+// 4, 138.6 ms with 2; FP64 137.2 ms with 8, 139.0 ms with 4)
+template <typename T>
+constexpr int wide_group()
+{
+    return sizeof(T) == 4 ? 4 : kSolveGroup;
+}
+
 template <typename T>
 static int narrow_group(const Plan &p, int l, i64 nrhs)
 {
@@ -1656,7 +1666,7 @@ static int solve_chunk(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which
             }
             if (wide) {
                 if (grouped) {  // the chains, then the rectangles in wider groups
-                    fwd_wide_level<T, kSolveGroup>(h, l, b, ldb, nrhs, false);
+                    fwd_wide_level<T, wide_group<T>()>(h, l, b, ldb, nrhs, false);
                     if (nrhs >= kMmGroup) fwd_rect_mm_level<T>(h, l, nrhs);
                     else if (nrhs >= kRectGroup) fwd_rect_level<T, kRectGroup>(h, l, nrhs);
                     else fwd_rect_level<T, kSolveGroup>(h, l, nrhs);
@@ -1684,7 +1694,7 @@ static int solve_chunk(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which
                 else if (nrhs >= kRectGroup) bwd_rect_level<T, kRectGroup>(h, l, b, ldb, nrhs);
                 else if (grouped) bwd_rect_level<T, kSolveGroup>(h, l, b, ldb, nrhs);
                 else bwd_rect_level<T, 1>(h, l, b, ldb, nrhs);
-                if (grouped) bwd_wide_level<T, kSolveGroup>(h, l, b, ldb, nrhs);
+                if (grouped) bwd_wide_level<T, wide_group<T>()>(h, l, b, ldb, nrhs);
                 else bwd_wide_level<T, 1>(h, l, b, ldb, nrhs);
             }
             fk.end();
