@@ -1628,21 +1628,24 @@ static void mark_pending(sprgpu_handle *h, i64 nrhs)
 struct LevelFork {
     sprgpu_handle *h;
     bool on = false;
-    // the stream for the narrow launch of level l (forks when both kinds are there)
-    hipStream_t begin(bool both)
+    // *st: the stream for the narrow launch of a level (forks when both
+    // kinds are there); 0 or a HIP error code as hip_fail reports it
+    int begin(bool both, hipStream_t *st)
     {
         on = both && h->stream2;
-        if (!on) return h->stream;
-        (void)hipEventRecord(h->ev[0], h->stream);
-        (void)hipStreamWaitEvent(h->stream2, h->ev[0], 0);
-        return h->stream2;
+        *st = on ? h->stream2 : h->stream;
+        if (!on) return 0;
+        SPRGPU_HIP(hipEventRecord(h->ev[0], h->stream));
+        SPRGPU_HIP(hipStreamWaitEvent(h->stream2, h->ev[0], 0));
+        return 0;
     }
-    void end()
+    int end()
     {
-        if (!on) return;
-        (void)hipEventRecord(h->ev[1], h->stream2);
-        (void)hipStreamWaitEvent(h->stream, h->ev[1], 0);
+        if (!on) return 0;
         on = false;
+        SPRGPU_HIP(hipEventRecord(h->ev[1], h->stream2));
+        SPRGPU_HIP(hipStreamWaitEvent(h->stream, h->ev[1], 0));
+        return 0;
     }
 };
 
@@ -1658,7 +1661,8 @@ static int solve_chunk(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which
             const bool narrow = p.solve_small_ptr[l + 1] > p.solve_small_ptr[l];
             const bool wide = p.wide_ptr[l + 1] > p.wide_ptr[l];
             if (narrow) {
-                const hipStream_t st = fk.begin(wide);
+                hipStream_t st;
+                if (int rc = fk.begin(wide, &st)) return rc;
                 const int R = narrow_group<T>(p, l, nrhs);
                 if (R == kSolveGroup) fwd_narrow_level<T, kSolveGroup>(h, st, l, b, ldb, nrhs);
                 else if (R == 4) fwd_narrow_level<T, 4>(h, st, l, b, ldb, nrhs);
@@ -1674,7 +1678,7 @@ static int solve_chunk(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which
                     fwd_wide_level<T, 1>(h, l, b, ldb, nrhs, true);
                 }
             }
-            fk.end();
+            if (int rc = fk.end()) return rc;
         }
     }
     if (which & 2) {
@@ -1683,7 +1687,8 @@ static int solve_chunk(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which
             const bool narrow = p.solve_small_ptr[l + 1] > p.solve_small_ptr[l];
             const bool wide = p.wide_ptr[l + 1] > p.wide_ptr[l];
             if (narrow) {
-                const hipStream_t st = fk.begin(wide);
+                hipStream_t st;
+                if (int rc = fk.begin(wide, &st)) return rc;
                 const int R = narrow_group<T>(p, l, nrhs);
                 if (R == kSolveGroup) bwd_narrow_level<T, kSolveGroup>(h, st, l, b, ldb, nrhs);
                 else if (R == 4) bwd_narrow_level<T, 4>(h, st, l, b, ldb, nrhs);
@@ -1697,7 +1702,7 @@ static int solve_chunk(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which
                 if (grouped) bwd_wide_level<T, wide_group<T>()>(h, l, b, ldb, nrhs);
                 else bwd_wide_level<T, 1>(h, l, b, ldb, nrhs);
             }
-            fk.end();
+            if (int rc = fk.end()) return rc;
         }
     }
     SPRGPU_HIP(hipGetLastError());
