@@ -1622,53 +1622,46 @@ static void mark_pending(sprgpu_handle *h, i64 nrhs)
 }
 
 // A level with both narrow and wide supernodes runs the two kinds side by
-// side: the narrow launch on the handle's second stream, forked from and
-// joined to the handle's stream by events (they touch different supernodes,
-// and the next level waits for both).
-struct LevelFork {
-    sprgpu_handle *h;
-    bool on = false;
-    // *st: the stream for the narrow launch of a level (forks when both
-    // kinds are there); 0 or a HIP error code as hip_fail reports it
-    int begin(bool both, hipStream_t *st)
-    {
-        on = both && h->stream2;
-        *st = on ? h->stream2 : h->stream;
-        if (!on) return 0;
-        SPRGPU_HIP(hipEventRecord(h->ev[0], h->stream));
-        SPRGPU_HIP(hipStreamWaitEvent(h->stream2, h->ev[0], 0));
+// side: the wide launch on the handle's stream, enqueued first (its block
+// chains are the sweep's critical path), and the narrow launch on the
+// handle's second stream, forked from the handle's stream before the wide
+// launch and joined to it after, by events (the two touch different
+// supernodes, and the next level waits for both).
+template <typename NarrowF, typename WideF>
+static int run_level(sprgpu_handle *h, bool narrow, bool wide, NarrowF &&launch_narrow,
+                     WideF &&launch_wide)
+{
+    if (!(narrow && wide && h->stream2)) {
+        if (narrow) launch_narrow(h->stream);
+        if (wide) launch_wide();
         return 0;
     }
-    int end()
-    {
-        if (!on) return 0;
-        on = false;
-        SPRGPU_HIP(hipEventRecord(h->ev[1], h->stream2));
-        SPRGPU_HIP(hipStreamWaitEvent(h->stream, h->ev[1], 0));
-        return 0;
-    }
-};
+    SPRGPU_HIP(hipEventRecord(h->ev[0], h->stream));
+    launch_wide();
+    SPRGPU_HIP(hipStreamWaitEvent(h->stream2, h->ev[0], 0));
+    launch_narrow(h->stream2);
+    SPRGPU_HIP(hipEventRecord(h->ev[1], h->stream2));
+    SPRGPU_HIP(hipStreamWaitEvent(h->stream, h->ev[1], 0));
+    return 0;
+}
 
 template <typename T>
 static int solve_chunk(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which)
 {
     const Plan &p = h->plan;
     const bool grouped = nrhs >= kSolveGroup;
-    LevelFork fk{h};
     if (which & 1) {
         mark_pending(h, nrhs);
         for (int l = 0; l < p.nlevels; ++l) {
             const bool narrow = p.solve_small_ptr[l + 1] > p.solve_small_ptr[l];
             const bool wide = p.wide_ptr[l + 1] > p.wide_ptr[l];
-            if (narrow) {
-                hipStream_t st;
-                if (int rc = fk.begin(wide, &st)) return rc;
+            auto nar = [&](hipStream_t st) {
                 const int R = narrow_group<T>(p, l, nrhs);
                 if (R == kSolveGroup) fwd_narrow_level<T, kSolveGroup>(h, st, l, b, ldb, nrhs);
                 else if (R == 4) fwd_narrow_level<T, 4>(h, st, l, b, ldb, nrhs);
                 else fwd_narrow_level<T, 1>(h, st, l, b, ldb, nrhs);
-            }
-            if (wide) {
+            };
+            auto wid = [&]() {
                 if (grouped) {  // the chains, then the rectangles in wider groups
                     fwd_wide_level<T, wide_group<T>()>(h, l, b, ldb, nrhs, false);
                     if (nrhs >= kMmGroup) fwd_rect_mm_level<T>(h, l, nrhs);
@@ -1677,8 +1670,8 @@ static int solve_chunk(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which
                 } else {
                     fwd_wide_level<T, 1>(h, l, b, ldb, nrhs, true);
                 }
-            }
-            if (int rc = fk.end()) return rc;
+            };
+            if (int rc = run_level(h, narrow, wide, nar, wid)) return rc;
         }
     }
     if (which & 2) {
@@ -1686,23 +1679,21 @@ static int solve_chunk(sprgpu_handle *h, double *b, i64 ldb, i64 nrhs, int which
         for (int l = p.nlevels - 1; l >= 0; --l) {
             const bool narrow = p.solve_small_ptr[l + 1] > p.solve_small_ptr[l];
             const bool wide = p.wide_ptr[l + 1] > p.wide_ptr[l];
-            if (narrow) {
-                hipStream_t st;
-                if (int rc = fk.begin(wide, &st)) return rc;
+            auto nar = [&](hipStream_t st) {
                 const int R = narrow_group<T>(p, l, nrhs);
                 if (R == kSolveGroup) bwd_narrow_level<T, kSolveGroup>(h, st, l, b, ldb, nrhs);
                 else if (R == 4) bwd_narrow_level<T, 4>(h, st, l, b, ldb, nrhs);
                 else bwd_narrow_level<T, 1>(h, st, l, b, ldb, nrhs);
-            }
-            if (wide) {
+            };
+            auto wid = [&]() {
                 if (nrhs >= kMmGroup) bwd_rect_mm_level<T>(h, l, b, ldb, nrhs);
                 else if (nrhs >= kRectGroup) bwd_rect_level<T, kRectGroup>(h, l, b, ldb, nrhs);
                 else if (grouped) bwd_rect_level<T, kSolveGroup>(h, l, b, ldb, nrhs);
                 else bwd_rect_level<T, 1>(h, l, b, ldb, nrhs);
                 if (grouped) bwd_wide_level<T, wide_group<T>()>(h, l, b, ldb, nrhs);
                 else bwd_wide_level<T, 1>(h, l, b, ldb, nrhs);
-            }
-            if (int rc = fk.end()) return rc;
+            };
+            if (int rc = run_level(h, narrow, wide, nar, wid)) return rc;
         }
     }
     SPRGPU_HIP(hipGetLastError());
