@@ -146,7 +146,6 @@ __device__ __forceinline__ void ascend(const ForestArgs &a, int id, ForestLds<T>
     const int F = nd.F, S = nd.S, r = threadIdx.x & 63;
     const bool rv = r < F;
     unsigned *cdone = a.sync + 2, *adone = cdone + a.n;
-    if (nd.nchild > 0) wait_for(cdone + id, unsigned(nd.nchild), a.sync, a.st, a.err);
     T *store = static_cast<T *>(a.store);
     T *ublk = static_cast<T *>(a.ublk);
     const i32 *rows = a.lindx + nd.lxoff;
@@ -206,6 +205,10 @@ __device__ __forceinline__ void ascend(const ForestArgs &a, int id, ForestLds<T>
     }
     if (SOLVE && r < kSmallF) L.yl[r] = (r < S) ? get(a.b + rows[r]) : 0.0;
     wave_sync();
+    // (everything above is the item's own data, which no child writes: it is
+    // loaded while the children may still be running, and only what follows
+    // waits for them)
+    if (nd.nchild > 0) wait_for(cdone + id, unsigned(nd.nchild), a.sync, a.st, a.err);
 
     // 2. the children's update blocks and update vectors, one child at a
     //    time in SIBL order (a front entry receives at most one entry per
@@ -326,10 +329,9 @@ __device__ __forceinline__ void ascend(const ForestArgs &a, int id, ForestLds<T>
     }
     stores_done();
     if (r == 0) {
+        put(adone + id, 1u);  // for this item's backward step: its panel is final
         if (nd.parent >= 0)
             __hip_atomic_fetch_add(cdone + nd.parent, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        else
-            put(adone + id, 1u);
     }
 }
 
@@ -339,19 +341,20 @@ __device__ __forceinline__ void descend(const ForestArgs &a, int id, ForestLds<T
     const ForestNode nd = a.fnodes[id];
     const int F = nd.F, S = nd.S, r = threadIdx.x & 63;
     unsigned *cdone = a.sync + 2, *adone = cdone + a.n, *bdone = adone + a.n;
-    if (nd.parent >= 0)
-        wait_for(bdone + nd.parent, 1u, a.sync, a.st, a.err);
-    else if (a.phases & 1)
-        wait_for(adone + id, 1u, a.sync, a.st, a.err);
+    // the panel's rows below the diagonal (lane = row), once this item's own
+    // ascending step has made them (when it ran in this launch), loaded
+    // while the parent's backward step may still be running
+    if (a.phases & 1) wait_for(adone + id, 1u, a.sync, a.st, a.err);
     const T *store = static_cast<const T *>(a.store);
     const i32 *rows = a.lindx + nd.lxoff;
-    // the vector entries of the front's rows (z of the own rows, x of the
-    // ancestors' rows) and the panel's rows below the diagonal, lane = row
-    const double xr = (r < F) ? get(a.b + rows[r]) : 0.0;
     T lv[kSmallF];
     const T *pr = store + nd.loff + r;  // row r of the panel, column by column
 #pragma unroll
     for (int c = 0; c < kSmallF; ++c, pr += nd.ld) lv[c] = (r < F && c < S && c < r) ? get(pr) : T(0);
+    // then the vector entries of the front's rows (z of the own rows, x of
+    // the ancestors' rows), once the parent's backward step is done
+    if (nd.parent >= 0) wait_for(bdone + nd.parent, 1u, a.sync, a.st, a.err);
+    const double xr = (r < F) ? get(a.b + rows[r]) : 0.0;
     if (r < F) L.xs[r] = xr;
 #pragma unroll
     for (int c = 0; c < kSmallF; ++c)
