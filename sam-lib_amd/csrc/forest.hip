@@ -208,27 +208,37 @@ __device__ __forceinline__ void ascend(const ForestArgs &a, int id, ForestLds<T>
     // (everything above is the item's own data, which no child writes: it is
     // loaded while the children may still be running, and only what follows
     // waits for them)
+    ForestChild chn = (nd.nchild > 0) ? a.fchild[nd.child0] : ForestChild{};
     if (nd.nchild > 0) wait_for(cdone + id, unsigned(nd.nchild), a.sync, a.st, a.err);
 
     // 2. the children's update blocks and update vectors, one child at a
     //    time in SIBL order (a front entry receives at most one entry per
-    //    child)
+    //    child); a child's map, update vector and first round of its update
+    //    block are loaded together, and the next child's record meanwhile
     for (int k = 0; k < nd.nchild; ++k) {
-        const ForestChild ch = a.fchild[nd.child0 + k];
+        const ForestChild ch = chn;
+        if (k + 1 < nd.nchild) chn = a.fchild[nd.child0 + k + 1];
         const int U = ch.U;
+        const int cnt = FACT ? U * (U + 1) / 2 : 0;
+        T uv[kRound];
+#pragma unroll
+        for (int u = 0; u < kRound; ++u) {
+            const int e = r + 64 * u;
+            uv[u] = (e < cnt) ? get(ublk + ch.uoff + e) : T(0);
+        }
         if (r < U) L.mapl[r] = a.relmap[ch.mapoff + r];
         double vv = 0.0;
         if (SOLVE && r < U) vv = get(a.vwork + ch.vofs + r);
         wave_sync();
         if (FACT) {  // the row-packed update block, kRound loads per lane at a time
-            const int cnt = U * (U + 1) / 2;
 #pragma nounroll
             for (int t0 = 0; 64 * t0 < cnt; t0 += kRound) {
-                T uv[kRound];
+                if (t0 > 0) {
 #pragma unroll
-                for (int u = 0; u < kRound; ++u) {
-                    const int e = r + 64 * (t0 + u);
-                    uv[u] = (e < cnt) ? get(ublk + ch.uoff + e) : T(0);
+                    for (int u = 0; u < kRound; ++u) {
+                        const int e = r + 64 * (t0 + u);
+                        uv[u] = (e < cnt) ? get(ublk + ch.uoff + e) : T(0);
+                    }
                 }
 #pragma unroll
                 for (int u = 0; u < kRound; ++u) {
