@@ -112,9 +112,11 @@ __device__ bool wait_for(const unsigned *p, unsigned target, unsigned *sync,
 {
     unsigned spins = 0;
     for (;;) {
-        const unsigned v = __builtin_amdgcn_readfirstlane(get(p));
+        // both looks in one memory round trip
+        const unsigned v0 = get(p), d0 = get(sync + 1);
+        const unsigned v = __builtin_amdgcn_readfirstlane(v0);
         if (v >= target) return true;
-        const unsigned dead = __builtin_amdgcn_readfirstlane(get(sync + 1));
+        const unsigned dead = __builtin_amdgcn_readfirstlane(d0);
         if (dead) return false;
         if (++spins > kForestSpin) {
             if ((threadIdx.x & 63) == 0) {
