@@ -313,9 +313,10 @@ int sprgpu_spmv(sprgpu_handle *handle, const double *x_dev, double *y_dev, sprgp
  * the last column's +-A b, as the reference's WA does.  Each C(r) is summed
  * in the reference's order (row r in ascending column order, every product
  * rounded before it is added), so the result equals SPRPRM's, up to the sign
- * of a zero sum.  Returns 0, -2 (IFLAG, N, missing C/WA, or the PERI2E
- * permutation, MSPAR(58) >= 1, not supported), -3 (KSA <= 0 with PERI2E),
- * -4 (M < NEQ) as the reference's IERR, or -1 (pattern). */
+ * of a zero sum.  With MSPAR(58) >= 1 B, C and WA are in the external
+ * (PERI2E) order, as SPRPRR takes them (sprprm.f:327-397).  Returns 0, -2
+ * (IFLAG, N, missing C/WA), -3 (KSA <= 0 with PERI2E), -4 (M < NEQ) as the
+ * reference's IERR, or -1 (pattern). */
 int sprgpu_sprprm(sprgpu_handle *handle, const double *a_dev, const double *b_dev,
                   double *c_dev, double *wa_dev, sprgpu_int m, sprgpu_int n, sprgpu_int ksa,
                   sprgpu_int iflag);
@@ -339,8 +340,9 @@ int sprgpu_sprtra(sprgpu_handle *handle, const double *a_dev, const double *b_de
  * KSU <= 0: U is a diagonal of NEQ doubles.  wa_dev (M doubles; may be NULL
  * unless IFLAG is 11 or 22) receives the last column, as the reference's WA.
  * Every stored entry is used, in the reference's order, so C equals
- * SPRTMM's bit for bit.  Returns 0, -2 (IFLAG, N, missing C/WA, or PERI2E,
- * MSPAR(58) >= 1, not supported) or -3 (M < NEQ) as the reference's IERR. */
+ * SPRTMM's bit for bit.  With MSPAR(58) >= 1 B, C and WA are in the
+ * external (PERI2E) order, as SPRTMR takes them (sprtmm.f:345-480).  Returns
+ * 0, -2 (IFLAG, N, missing C/WA) or -3 (M < NEQ) as the reference's IERR. */
 int sprgpu_sprtmm(sprgpu_handle *handle, const double *u_dev, const double *b_dev,
                   double *c_dev, double *wa_dev, sprgpu_int m, sprgpu_int n, sprgpu_int ksu,
                   sprgpu_int iflag);
